@@ -1,0 +1,196 @@
+/*
+ * uegccd.h -- C ABI of the MI355X CCD amplitude solver (libuegccd.so).
+ *
+ * The library replaces the body of the reference's CCD driver,
+ *     Subroutine DrvCCD(HEGData, UEGInfo, T2aaaa, T2abab, T2abba)
+ *         (source/CCD.f90:18-32, public at source/CCD.f90:13-14),
+ * together with what that body calls per iteration: TermsWrapper
+ * (source/CCD.f90:146-181), SolveCC/GetRes/CnvrgCC (source/CCD.f90:354-462),
+ * CCEnergy (source/CCD.f90:186-237) and the DIIS module (source/DIISCC.f90).
+ * The MP2 initial guess (DrvMBPT, source/MP2.f90:11-157) and the plane-wave
+ * basis (SetupBasis/change_rs, source/HEG.f90:150-324,494-533) are exported as
+ * well so a host can drive a whole single-point calculation through it.
+ *
+ * Conventions
+ *  - Amplitude arrays are the reference's explicit-shape FP64 arrays
+ *    T(NOcc,NOcc,NOcc+1:NAO) in Fortran (column-major) order:
+ *        element (i,j,a) at (i-1) + NOcc*(j-1) + NOcc*NOcc*(a-NOcc-1).
+ *    Host pointers; the library copies in and out.  Entries whose momentum
+ *    partner b = k_i + k_j - k_a is not a virtual orbital are zero on output.
+ *  - Every function returns UEGCCD_OK (0) or a negative status; nothing calls
+ *    exit/stop.  uegccd_strerror() names each status.
+ *  - A context owns its device buffers and one HIP stream; contexts are
+ *    independent (no globals), calls on one context must be serialised by the
+ *    caller.  Nothing falls back to the CPU: without a usable HIP device the
+ *    calls fail with UEGCCD_ERR_HIP.
+ */
+#ifndef UEGCCD_H
+#define UEGCCD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    UEGCCD_OK = 0,
+    UEGCCD_ERR_INVALID = -1,      /* bad argument or inconsistent system      */
+    UEGCCD_ERR_ALLOC = -2,        /* device or host allocation failed         */
+    UEGCCD_ERR_HIP = -3,          /* HIP runtime error / no device            */
+    UEGCCD_ERR_UNSUPPORTED = -4,  /* an Input option this build does not run  */
+    UEGCCD_ERR_NOT_CONVERGED = -5,/* solve stopped at max_iter (soft failure)  */
+    UEGCCD_ERR_NONFINITE = -6,    /* NaN/Inf in amplitudes or residual         */
+    UEGCCD_ERR_BUSY = -7,         /* server: no free slot / HBM budget now     */
+    UEGCCD_ERR_NOT_FOUND = -8     /* server: unknown request id               */
+};
+
+/* Iteration schemes. */
+enum {
+    /* The reference's own iteration: damped Jacobi with the DIIS of
+       source/DIISCC.f90 (whose B-matrix is overwritten, so it never
+       extrapolates), stopped at the first max|dT| < tol_dt exactly as the
+       loop at source/CCD.f90:80-99.  Same NIter and trajectory. */
+    UEGCCD_MODE_REFERENCE = 0,
+    /* Pulay DIIS of length ndiis on the device; stops when |dE| < tol_e and
+       max|R| < tol_r (R = D*T - G of the input amplitudes). */
+    UEGCCD_MODE_DIIS = 1
+};
+
+enum { UEGCCD_PREC_FP64 = 0, UEGCCD_PREC_MIXED = 1 };
+
+/*
+ * The data of HEGDataType/UEGInfoType the solver reads (source/Types.f90:7-82).
+ * nvec is the reference's nVec(NAO,3) array, column-major: component d of
+ * orbital p (1-based) at nvec[(d-1)*nao + (p-1)].
+ */
+typedef struct {
+    int32_t nocc;          /* UEGInfo%NOcc                                  */
+    int32_t nao;           /* UEGInfo%NAO                                   */
+    const int32_t* nvec;   /* HEGData%nVec(NAO,3)                            */
+    const double* eigen;   /* HEGData%Eigen(NAO)                             */
+    double L;              /* HEGData%L (box length at r_s)                  */
+    double madelung;       /* HEGData%Madelung (already x MadelungFactor)    */
+    double rs;             /* HEGData%rS                                     */
+    double gapii;          /* UEGInfo%GapII                                  */
+    int32_t coren;         /* UEGInfo%CoreN                                  */
+    int32_t safe_eri;      /* UEGInfo%SafeERI                                */
+    int32_t irange[15];    /* IRange* in Input-key order (source/IO.f90:88-102) */
+    int32_t do_ring, do_xring, do_ladder, do_mosaic;
+} uegccd_system;
+
+typedef struct {
+    int32_t mode;          /* UEGCCD_MODE_*                                  */
+    int32_t ndiis;         /* DIIS length (DIIS mode), 2..16                 */
+    int32_t diis_start;    /* first NIter at which DIIS extrapolates         */
+    int32_t max_iter;      /* <= 0: 1000 (source/CCD.f90:36)                 */
+    double tol_dt;         /* <= 0: FindTol's TolMax (source/HEG.f90:1073)    */
+    double tol_e;          /* DIIS mode energy tolerance (Ha)                */
+    double tol_r;          /* DIIS mode max|R| tolerance                     */
+    double damping;        /* <= 0: FindTol's DenomFactor (2, or 5 if rs>5.1) */
+    int32_t precision;     /* UEGCCD_PREC_*                                  */
+    double mixed_switch;   /* MIXED: FP32 ladders while max|R| >= this       */
+    int32_t device;        /* HIP device ordinal                             */
+    int32_t iters_per_sync;/* iterations launched between host polls (>=1)  */
+} uegccd_options;
+
+typedef struct {
+    double ecorr;          /* Ec(CCD)                                        */
+    double ehf_plus_ec;    /* unused by the library (host fills it)          */
+    double dt;             /* last max|dT|                                   */
+    double maxres;         /* max|D*T - G| at the returned amplitudes        */
+    int32_t niter;         /* NIter as the reference counts it               */
+    int32_t failed;        /* MaxRes > FailRatio*TolMax (source/CCD.f90:107) */
+    int32_t iters_fp32;    /* iterations whose ladders ran in FP32           */
+    int32_t status;        /* UEGCCD_OK / NOT_CONVERGED / NONFINITE          */
+    double solve_ms;       /* device wall time of the iteration loop         */
+} uegccd_result;
+
+typedef struct uegccd_ctx uegccd_ctx;
+
+/* Fills *opt with the reference behaviour: MODE_REFERENCE, FindTol values,
+ * FP64, max_iter 1000. */
+void uegccd_default_options(uegccd_options* opt);
+
+/* Builds the per-basis index maps (momentum partners, pair-momentum blocks,
+ * Coulomb table) and device buffers.  Replaces SetUpCCScr/SetUpDIISCC
+ * (source/CCScr.f90:9-30, source/DIISCC.f90:10-45) and the FindIndex/ERI
+ * machinery (source/HEG.f90:357-404, 689-978). */
+int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt, uegccd_ctx** out);
+void uegccd_ctx_destroy(uegccd_ctx* ctx);
+
+/* Sizes: NOcc, NAO, number of valid (i,j,a), number of pair-momentum blocks. */
+int uegccd_ctx_info(const uegccd_ctx* ctx, int32_t* nocc, int32_t* nao, int64_t* nvalid,
+                    int32_t* nblocks);
+
+/* One residual evaluation, TermsWrapper (source/CCD.f90:146-181):
+ * G = drivers + mosaics + ladders + rings + xrings, R = D*T - G.
+ * Any of g2abab/g2abba/r2abab/r2abba may be NULL. */
+int uegccd_residual(uegccd_ctx* ctx, const double* t2abab, const double* t2abba,
+                    double* g2abab, double* g2abba, double* r2abab, double* r2abba);
+
+/* CCEnergy (source/CCD.f90:186-237). */
+int uegccd_energy(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, double* ecorr);
+
+/* DrvCCD (source/CCD.f90:18-183): iterate from the given amplitudes (the MP2
+ * guess in the reference) to convergence.  t2aaaa is written as
+ * t2abab + t2abba on output (may be NULL). */
+int uegccd_solve(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, double* t2abba,
+                 uegccd_result* res);
+
+/* As uegccd_solve, but the amplitudes stay in device memory: from_mp2 != 0
+ * first writes the MP2 guess on the device (the caller-side copy of
+ * source/Main.f90:85-87 without a host round trip); otherwise the solve starts
+ * from the amplitudes the context holds (the last solve's or uegccd_mp2's).
+ * Read them with uegccd_get_amplitudes.  No counterpart in the reference; for
+ * callers that keep T2 resident across solves (serving, benchmarks). */
+int uegccd_solve_resident(uegccd_ctx* ctx, int32_t from_mp2, uegccd_result* res);
+int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, double* t2abba);
+
+/* Average device time (microseconds, HIP events on the context's stream) of
+ * one kernel of the residual pass over the context's current amplitudes:
+ * 0 mosaic intermediates, 1 ring intermediates, 2 ladder GEMMs, 3 residual
+ * assembly + update, 4 extrapolation.  No counterpart in the reference;
+ * measurement support. */
+int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
+
+/* Per-iteration (E, dT) of the last solve, index NIter-1 (the lines the
+ * reference writes with format 1040, source/CCD.f90:76,97). */
+int uegccd_trace(const uegccd_ctx* ctx, double* e, double* dt, int32_t n);
+
+/* DrvMBPT amplitudes and E(2) (source/MP2.f90:56-98).  Any array may be NULL. */
+int uegccd_mp2(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, double* t2abba, double* e2);
+
+/* Plane-wave basis of init_HEG + change_rs (source/HEG.f90:16-43,150-324,
+ * 494-533): enumeration |n|^2 <= maxkpoint, stable ordering by |n+twist|^2,
+ * fcut/corefcut truncation, HF eigenvalues with the Madelung term, E_HF.
+ * Call with nvec == NULL to query nao; then nvec holds nao*3 (column-major
+ * like nVec(NAO,3)) and eigen nao entries.  scalars (may be NULL) receives
+ * L, Omega, Madelung, EHF, ScreeningDistance, DenomFactor, TolMax, FailRatio. */
+typedef struct {
+    int32_t nel;
+    int32_t maxkpoint;
+    double rs;
+    double twist[3];
+    double fcut, corefcut;
+    double madelungfactor, gap, exchangefactor;
+} uegccd_heg_params;
+
+int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao, int32_t* nvec, double* eigen,
+                     double* scalars);
+
+const char* uegccd_strerror(int status);
+
+/* ------------------------------------------------------------------------
+ * Batched solves (twist averaging / r_s scans): several independent systems
+ * iterated together, each kernel launch covering every system's tiles.  No
+ * counterpart in the reference (which runs twists one after another,
+ * source/Main.f90:112-168); each system's result equals its uegccd_solve.
+ * ---------------------------------------------------------------------- */
+int uegccd_solve_batch(uegccd_ctx** ctxs, int32_t n, double** t2abab, double** t2abba,
+                       uegccd_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

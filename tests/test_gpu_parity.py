@@ -1,0 +1,131 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: residual terms within 1e-12 relative to the largest G entry (FP64,
+different summation order), energies within 1e-12 Ha, the reference-mode
+trajectory with identical NIter and every printed (E, dT) within 1e-12, and
+DIIS-mode energies within 1e-9 Ha of the oracle converged to max|dT| < 1e-12.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import info_for, oracle_for, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TERM_TOL = 1e-12
+
+
+def _product(info, **opt):
+    from uegccd_amd.driver import CCDContext, init_heg, options
+    heg = init_heg(info)
+    return heg, CCDContext(heg, info, options(**opt) if opt else None)
+
+
+def _flags(info, mask):
+    info.DoMosaic = bool(mask & 2)
+    info.DoLadder = bool(mask & 4)
+    info.DoRing = bool(mask & 8)
+    info.DoXRing = bool(mask & 16)
+    return info
+
+
+def _amplitudes(orc, kind, seed=0):
+    """MP2 amplitudes, or the oracle's amplitudes after a few iterations, or
+    random values on the valid entries."""
+    e2, taa, tab, tba = orc.mp2()
+    if kind == "mp2":
+        return tab, tba
+    if kind == "iter":
+        r = orc.ccd(maxiter=4)
+        return r["tab"], r["tba"]
+    rng = np.random.default_rng(seed)
+    mask = (tab != 0) | (tba != 0)
+    a = np.where(mask, rng.normal(0, 1e-2, tab.shape), 0.0)
+    b = np.where(mask, rng.normal(0, 1e-2, tab.shape), 0.0)
+    return np.asfortranarray(a), np.asfortranarray(b)
+
+
+@pytest.mark.parametrize("name", ["n14_cut4", "n14_cut5_tw", "n24_rs5", "n38_cut9"])
+@pytest.mark.parametrize("mask", [1, 3, 5, 9, 17, 31], ids=["drv", "mos", "lad", "ring", "xring", "all"])
+def test_residual_terms(name, mask):
+    print(f"[residual] {name} mask={mask}", flush=True)
+    info = _flags(info_for(name), mask)
+    orc = oracle_for(info)
+    heg, ctx = _product(info)
+    for kind in ("mp2", "rand"):
+        tab, tba = _amplitudes(orc, kind, seed=mask)
+        g_ab, g_ba, r_ab, r_ba = ctx.residual(tab, tba)
+        o_gab, o_gba, o_rab, o_rba = orc.terms(tab, tba, mask=31)
+        scale = max(np.abs(o_gab).max(), np.abs(o_gba).max())
+        assert np.abs(g_ab - o_gab).max() / scale < TERM_TOL, (kind, "G abab")
+        assert np.abs(g_ba - o_gba).max() / scale < TERM_TOL, (kind, "G abba")
+        rs = max(np.abs(o_rab).max(), np.abs(o_rba).max())
+        assert np.abs(r_ab - o_rab).max() / rs < 1e-11
+        assert np.abs(r_ba - o_rba).max() / rs < 1e-11
+
+
+@pytest.mark.parametrize("name", ["n14_cut5", "n24_rs5", "n54_cut9"])
+def test_mp2_and_energy(name):
+    print(f"[mp2] {name}", flush=True)
+    info = info_for(name)
+    orc = oracle_for(info)
+    heg, ctx = _product(info)
+    e2, taa, tab, tba = ctx.mp2()
+    o_e2, o_taa, o_tab, o_tba = orc.mp2()
+    assert abs(e2 - o_e2) < 1e-12
+    assert rel_err(tab, o_tab) < 1e-13 and rel_err(tba, o_tba) < 1e-13
+    tab2, tba2 = _amplitudes(orc, "iter")
+    assert abs(ctx.energy(tab2, tba2) - orc.energy(tab2 + tba2, tab2, tba2)) < 1e-12
+
+
+def test_reference_trajectory_n24(fixtures):
+    """The reference's own 70-iteration trajectory (source/Test/Output)."""
+    print("[traj] n24", flush=True)
+    fx = fixtures["n24"]
+    info = info_for("n24_rs5")
+    heg, ctx = _product(info)
+    e2, taa, tab, tba = ctx.mp2()
+    assert abs(e2 - fx["e2"]) < 5.1e-13
+    r = ctx.solve(tab, tba)
+    assert r.niter == 70
+    ref = np.array(fx["ccd_trace"])
+    np.testing.assert_allclose(r.trace_e, ref[:, 0], rtol=0, atol=5.1e-11)
+    np.testing.assert_allclose(r.trace_dt, ref[:, 2], rtol=0, atol=5.1e-11)
+    assert abs(r.ecorr - fx["ccd_ecorr"]) < 5.1e-13
+    assert abs(r.maxres - fx["ccd_maxres"]) < 1e-14
+    orc = oracle_for(info)
+    o = orc.ccd()
+    assert abs(r.ecorr - o["ecorr"]) < 1e-12
+    np.testing.assert_allclose(r.trace_e, o["trace_e"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r.trace_dt, o["trace_dt"], rtol=0, atol=1e-12)
+    assert rel_err(r.tab, o["tab"]) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["n14_cut5", "n14_cut5_tw", "n54_cut9"])
+def test_reference_trajectory_vs_oracle(name):
+    print(f"[traj] {name}", flush=True)
+    info = info_for(name)
+    orc = oracle_for(info)
+    heg, ctx = _product(info)
+    _, _, tab, tba = ctx.mp2()
+    r = ctx.solve(tab, tba)
+    o = orc.ccd()
+    assert r.niter == o["niter"]
+    np.testing.assert_allclose(r.trace_e, o["trace_e"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r.trace_dt, o["trace_dt"], rtol=0, atol=1e-12)
+    assert abs(r.ecorr - o["ecorr"]) < 1e-12
+    assert abs(r.maxres - o["maxres"]) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["n14_cut5", "n24_rs5", "n54_cut9"])
+def test_diis_converges_to_oracle_fixed_point(name):
+    print(f"[diis] {name}", flush=True)
+    info = info_for(name)
+    orc = oracle_for(info)
+    tight = orc.ccd(tolmax=1e-12, maxiter=1000)
+    heg, ctx = _product(info, mode="diis", ndiis=8)
+    _, _, tab, tba = ctx.mp2()
+    r = ctx.solve(tab, tba)
+    assert r.status == 0 and not r.failed
+    assert abs(r.ecorr - tight["ecorr"]) < 1e-9
+    assert r.niter < tight["niter"]

@@ -1,0 +1,113 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports what
+include/uegccd.h declares, the host basis builder agrees with the oracle, the
+Input parser follows source/IO.f90, and calls fail loudly without a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.helpers import SYSTEMS, info_for, oracle_for
+from uegccd_amd import lib as L
+from uegccd_amd.inputfile import InputError, parse_input, standard_input
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "uegccd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(uegccd_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    names = header_functions()
+    assert len(names) >= 12
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(L.EXPORTS) <= set(names)
+
+
+def test_strerror_and_defaults():
+    lib = L.load()
+    for st in range(0, -9, -1):
+        assert lib.uegccd_strerror(st)
+    o = L.Options()
+    lib.uegccd_default_options(ctypes.byref(o))
+    assert o.mode == L.MODE_REFERENCE and o.max_iter == 1000 and o.precision == L.PREC_FP64
+
+
+@pytest.mark.parametrize("name", ["n14_cut4", "n14_cut5_tw", "n24_rs5", "n54_cut16", "n14_cut25"])
+def test_heg_build_matches_oracle(name):
+    from uegccd_amd.driver import init_heg
+    info = info_for(name)
+    heg = init_heg(info)
+    orc = oracle_for(info)
+    assert heg.NAO == orc.nao and heg.NOcc == orc.nocc
+    np.testing.assert_array_equal(heg.nVec, orc.nvec)
+    np.testing.assert_allclose(heg.Eigen, orc.eigen, rtol=0, atol=1e-14)
+    assert heg.L == pytest.approx(orc.L, rel=1e-15)
+    assert heg.Madelung == pytest.approx(orc.madelung, rel=1e-13)
+    assert heg.EHF == pytest.approx(orc.ehf, rel=1e-13, abs=1e-13)
+    assert heg.DenomFactor == orc.denomfactor and heg.TolMax == orc.tolmax
+
+
+def test_basis_sizes_of_the_configs():
+    from uegccd_amd.driver import init_heg
+    sizes = {n: init_heg(info_for(n)).NAO for n in ("n14_cut4", "n14_cut5", "n14_cut25", "n54_cut16")}
+    assert sizes == {"n14_cut4": 33, "n14_cut5": 57, "n14_cut25": 515, "n54_cut16": 257}
+
+
+def test_no_device_fails_loudly():
+    """Without a HIP device the solver refuses to run (no CPU fallback)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from uegccd_amd.driver import CCDContext, init_heg
+    info = info_for("n14_cut4")
+    with pytest.raises(L.UegccdError) as e:
+        CCDContext(init_heg(info), info)
+    assert e.value.status in (L.ERR_HIP, L.ERR_INVALID)
+
+
+# --- Input parser (source/IO.f90) ------------------------------------------
+
+def test_parse_standard_input():
+    info = parse_input(standard_input(54, 16, 2.0))
+    assert (info.NElectron, info.NOcc, info.MaxKPoint, info.rSMin) == (54, 27, 16, 2.0)
+    assert info.SafeERI and info.DoRing and info.DoXRing and info.DoLadder and info.DoMosaic
+    assert info.irange == [0] * 15 and info.exchangefactor == 1.0 and info.DoSkipTA
+
+
+def test_parse_reference_test_input_layout():
+    # the layout of source/Test/Input: comments after values, NTwist, odd spacing
+    text = standard_input(14, 2, 1.0, (0.1, 0.1, 0.1), DoSingleCalc=False)
+    text = text.replace("corefcut:          1", "corefcut:         1     ! e.g. 1.414213562")
+    info = parse_input(text)
+    assert info.NTwist == (0.1, 0.1, 0.1) and info.corefcut == 1.0 and not info.DoSingleCalc
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda t: t + "Bogus Key: 1\n", "not recognized"),
+    (lambda t: t + "Safe ERI: T\n", "multiply set"),
+    (lambda t: t.replace("DoKS:", "!DoKS:"), "DoKS not set"),
+    (lambda t: t.replace("EndInput:          T", "EndInput:          F"), "EndInput must be T"),
+    (lambda t: t.replace("# Electrons:       14", "# Electrons:       13"), "closed shell"),
+    (lambda t: t.replace("# rS Points:       1", "# rS Points:       2"), "more than one rs"),
+    (lambda t: t.replace("DoTruncCoulombHF:  F", "DoTruncCoulombHF:  T"), "Not Implemented"),
+])
+def test_parse_errors(mutate, msg):
+    text = mutate(standard_input(14, 2, 1.0))
+    with pytest.raises(InputError) as e:
+        parse_input(text)
+    assert msg in str(e.value)
+
+
+def test_corefactor_and_fcut2ecut_derivations():
+    t = standard_input(14, 2, 1.0, CoreFactor=0.5, fcut2ecut=True, fcut=3.0)
+    info = parse_input(t)
+    assert info.CoreN == 3  # int(14/2 * 0.5)
+    # Nfermi = 14^(2/3)/4 < 10 -> MaxKPoint = int(10 * fcut * 2)
+    assert info.MaxKPoint == 60

@@ -1,0 +1,873 @@
+// Host side of libuegccd: per-basis index maps, device buffers, the iteration
+// driver and the C ABI declared in include/uegccd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <new>
+#include <tuple>
+#include <vector>
+
+#include "../../include/uegccd.h"
+#include "uegccd_internal.h"
+
+using namespace uegccd;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+#define HIPCHK(expr)                                                    \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) {                                             \
+      std::fprintf(stderr, "uegccd: %s failed: %s\n", #expr,            \
+                   hipGetErrorString(e_));                              \
+      return UEGCCD_ERR_HIP;                                            \
+    }                                                                   \
+  } while (0)
+
+// FindTol (source/HEG.f90:1073-1086).
+void find_tol(double rs, double* tolmax, double* failratio, double* df) {
+  *tolmax = 1.0e-8;
+  *failratio = 100;
+  *df = 2.0;
+  if (rs > 5.1) *tolmax = 1.0e-6;
+  if (rs > 5.1) *df = 5.0;
+  if (rs > 9.9) *tolmax = 1.0e-5;
+  if (rs > 80.0) *tolmax = 1.0e-3;
+  if (rs < 0.95) *failratio = 1e6;
+}
+
+// Which ERI variant SetPointers would bind (source/Main.f90:274-318).
+enum EriKind { ERI_CASED, ERI_NOCASE, ERI_CORE, ERI_GAP, ERI_BASE };
+EriKind eri_kind(const uegccd_system& s) {
+  int sum = 0;
+  for (int k = 0; k < 15; ++k) sum += std::abs(s.irange[k]);
+  if ((sum > 0 && s.gapii > 0.0 && s.coren > 0) || s.safe_eri) return ERI_CASED;
+  if (s.gapii > 0.0 && s.coren > 0) return ERI_NOCASE;
+  if (s.coren > 0) return ERI_CORE;
+  if (s.gapii > 0.0) return ERI_GAP;
+  return ERI_BASE;
+}
+
+struct DeviceArena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  template <typename T>
+  T* take(size_t count) {
+    size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+size_t aligned(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct uegccd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uegccd_options opt{};
+  int o = 0, v = 0, nao = 0, n3 = 0, nK = 0, n2max = 0, wtot = 0, maxiter = 1000;
+  int64_t nvalid = 0;
+  double tolmax = 1e-8, failratio = 100, df = 2.0;
+  // host copies of the maps
+  std::vector<int> pidx, kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
+  std::vector<int4> nv4;
+  std::vector<double> eig, vtab;
+  std::vector<int4> lad_items;  // (0, K, row0, col0)
+  // device
+  DeviceArena arena;
+  SolveDev sd{};           // host copy of the device descriptor
+  SolveDev* d_sd = nullptr;
+  BatchDev batch{};        // batch of one
+  State* d_state = nullptr;
+  double* h_trace = nullptr;
+  int last_niter = 0;
+};
+
+namespace {
+
+int build_maps(uegccd_ctx* c, const uegccd_system& s) {
+  const int o = s.nocc, nao = s.nao, v = nao - o;
+  c->o = o;
+  c->v = v;
+  c->nao = nao;
+  c->n3 = o * o * v;
+  c->nv4.resize(nao);
+  int mx = 0;
+  for (int p = 0; p < nao; ++p) {
+    int4 q;
+    q.x = s.nvec[p];
+    q.y = s.nvec[nao + p];
+    q.z = s.nvec[2 * nao + p];
+    q.w = 0;
+    c->nv4[p] = q;
+    mx = std::max({mx, std::abs(q.x), std::abs(q.y), std::abs(q.z)});
+  }
+  // momentum -> orbital lookup over a box holding every n_p + n_q - n_r
+  const int h = 3 * mx + 1, w = 2 * h + 1;
+  std::vector<int> box((size_t)w * w * w, -1);
+  auto bidx = [&](int x, int y, int z) { return (size_t)(x + h) + (size_t)w * ((y + h) + (size_t)w * (z + h)); };
+  for (int p = 0; p < nao; ++p) {
+    size_t k = bidx(c->nv4[p].x, c->nv4[p].y, c->nv4[p].z);
+    if (box[k] >= 0) return UEGCCD_ERR_INVALID;  // duplicate plane wave
+    box[k] = p;
+  }
+  auto lookup = [&](int m, int n, int l) {
+    const int4 a = c->nv4[m], b = c->nv4[n], d = c->nv4[l];
+    return box[bidx(a.x + b.x - d.x, a.y + b.y - d.y, a.z + b.z - d.z)];
+  };
+  // partner map: the FindIndex(i,j,a) > NOcc test of every loop in source/CCD.f90
+  c->pidx.assign(c->n3, -1);
+  c->nvalid = 0;
+  for (int av = 0; av < v; ++av)
+    for (int j = 0; j < o; ++j)
+      for (int i = 0; i < o; ++i) {
+        int b = lookup(i, j, o + av);
+        if (b >= o) {
+          c->pidx[i + o * j + o * o * av] = b;
+          ++c->nvalid;
+        }
+      }
+  // pair-momentum blocks, in order of first occurrence of K over x = i + o*j
+  std::map<std::tuple<int, int, int>, int> kid;
+  std::vector<std::vector<int>> pairs;
+  c->pair_kb.assign(o * o, 0);
+  c->pair_pos.assign(o * o, 0);
+  for (int x = 0; x < o * o; ++x) {
+    const int i = x % o, j = x / o;
+    auto key = std::make_tuple(c->nv4[i].x + c->nv4[j].x, c->nv4[i].y + c->nv4[j].y,
+                               c->nv4[i].z + c->nv4[j].z);
+    auto it = kid.find(key);
+    int K;
+    if (it == kid.end()) {
+      K = (int)pairs.size();
+      kid[key] = K;
+      pairs.emplace_back();
+    } else {
+      K = it->second;
+    }
+    c->pair_kb[x] = K;
+    c->pair_pos[x] = (int)pairs[K].size();
+    pairs[K].push_back(x);
+  }
+  c->nK = (int)pairs.size();
+  c->kb_poff.assign(c->nK + 1, 0);
+  c->kb_voff.assign(c->nK + 1, 0);
+  c->kb_woff.assign(c->nK + 1, 0);
+  c->kb_pairs.clear();
+  c->kb_va.clear();
+  c->kb_vb.clear();
+  for (int K = 0; K < c->nK; ++K) {
+    const int x0 = pairs[K][0];
+    for (int x : pairs[K]) c->kb_pairs.push_back(x);
+    for (int av = 0; av < v; ++av) {
+      int b = c->pidx[x0 + o * o * av];
+      if (b >= 0) {
+        c->kb_va.push_back(o + av);
+        c->kb_vb.push_back(b);
+      }
+    }
+    c->kb_poff[K + 1] = (int)c->kb_pairs.size();
+    c->kb_voff[K + 1] = (int)c->kb_va.size();
+    const int noo = (int)pairs[K].size();
+    c->kb_woff[K + 1] = c->kb_woff[K] + noo * noo;
+  }
+  c->wtot = c->kb_woff[c->nK];
+  // ladder tiles: 16 pair rows x 16 columns (a list then pair list)
+  c->lad_items.clear();
+  for (int K = 0; K < c->nK; ++K) {
+    const int noo = c->kb_poff[K + 1] - c->kb_poff[K];
+    const int nvv = c->kb_voff[K + 1] - c->kb_voff[K];
+    if (nvv == 0) continue;
+    const int ncol = nvv + noo;
+    for (int r0 = 0; r0 < noo; r0 += 16)
+      for (int c0 = 0; c0 < ncol; c0 += 16) c->lad_items.push_back(make_int4(0, K, r0, c0));
+  }
+  // Coulomb table, flag 0 of the ERI variants (source/HEG.f90:729-772)
+  int n2max = 0;
+  for (int p = 0; p < nao; ++p)
+    for (int q = 0; q < nao; ++q) {
+      const int4 a = c->nv4[p], b = c->nv4[q];
+      n2max = std::max(n2max, (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) +
+                                  (a.z - b.z) * (a.z - b.z));
+    }
+  if (n2max >= kMaxN2 || nao > kMaxNao) return UEGCCD_ERR_UNSUPPORTED;
+  c->n2max = n2max;
+  const EriKind kind = eri_kind(s);
+  const bool gap_applies = (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_GAP);
+  c->vtab.assign(n2max + 1, 0.0);
+  for (int q = 0; q <= n2max; ++q) {
+    double val = (q == 0) ? -s.madelung : 1.0 / (s.L * kPi * (double)q);
+    if (gap_applies && val > s.gapii && s.gapii > 0.0) val = 0.0;
+    c->vtab[q] = val;
+  }
+  c->eig.assign(s.eigen, s.eigen + nao);
+  return UEGCCD_OK;
+}
+
+int check_supported(const uegccd_system& s) {
+  const EriKind kind = eri_kind(s);
+  // CoreN > 0 masks integrals by orbital index (CAS), which the Coulomb table
+  // does not express; range flags 1-3 change the kernel per term.
+  if (s.coren > 0 && (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_CORE))
+    return UEGCCD_ERR_UNSUPPORTED;
+  if (kind == ERI_CASED)
+    for (int k = 0; k < 15; ++k)
+      if (s.irange[k] != 0) return UEGCCD_ERR_UNSUPPORTED;
+  // a negative v(0) makes the checked ERI variants return their -100 sentinel
+  if (kind != ERI_BASE && -s.madelung < 0.0) return UEGCCD_ERR_UNSUPPORTED;
+  return UEGCCD_OK;
+}
+
+int upload_batch_of_one(uegccd_ctx* c) {
+  BatchDev& b = c->batch;
+  b.n = 1;
+  b.solves = c->d_sd;
+  const int eb = (c->n3 + kElemThreads - 1) / kElemThreads;
+  const int mb = (c->sd.do_mosaic ? c->v + c->o : 0);
+  const int rb = ((c->sd.do_ring || c->sd.do_xring) ? (2 * c->o * c->v * c->o + kElemThreads - 1) / kElemThreads : 0);
+  int h_pref[6] = {0, eb, 0, mb, 0, rb};
+  int* d_pref = c->arena.take<int>(6);
+  HIPCHK(hipMemcpy(d_pref, h_pref, sizeof(h_pref), hipMemcpyHostToDevice));
+  b.elem_prefix = d_pref;
+  b.mos_prefix = d_pref + 2;
+  b.ring_prefix = d_pref + 4;
+  b.elem_blocks = eb;
+  b.mos_blocks = mb;
+  b.ring_blocks = rb;
+  b.nlad = c->sd.do_ladder ? (int)c->lad_items.size() : 0;
+  b.lad_items = c->arena.take<int4>(std::max<size_t>(1, c->lad_items.size()));
+  if (!c->lad_items.empty())
+    HIPCHK(hipMemcpy(b.lad_items, c->lad_items.data(), sizeof(int4) * c->lad_items.size(),
+                     hipMemcpyHostToDevice));
+  return UEGCCD_OK;
+}
+
+template <typename T>
+int upload(uegccd_ctx* c, const std::vector<T>& h, const T** dst) {
+  T* d = c->arena.take<T>(std::max<size_t>(1, h.size()));
+  if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  *dst = d;
+  return UEGCCD_OK;
+}
+
+int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
+  std::vector<double> h(2 * (size_t)c->n3, 0.0);
+  for (int x = 0; x < c->n3; ++x)
+    if (c->pidx[x] >= 0) {
+      h[x] = tab ? tab[x] : 0.0;
+      h[c->n3 + x] = tba ? tba[x] : 0.0;
+    }
+  HIPCHK(hipMemcpyAsync(c->sd.tcur, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return UEGCCD_OK;
+}
+
+int read_state(uegccd_ctx* c, State* st) {
+  HIPCHK(hipMemcpyAsync(st, c->d_state, sizeof(State), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void uegccd_default_options(uegccd_options* opt) {
+  std::memset(opt, 0, sizeof(*opt));
+  opt->mode = UEGCCD_MODE_REFERENCE;
+  opt->ndiis = 8;
+  opt->diis_start = 3;
+  opt->max_iter = 1000;
+  opt->tol_dt = 0.0;
+  opt->tol_e = 1e-10;
+  opt->tol_r = 1e-8;
+  opt->damping = 0.0;
+  opt->precision = UEGCCD_PREC_FP64;
+  opt->mixed_switch = 1e-5;
+  opt->device = 0;
+  opt->iters_per_sync = 8;
+}
+
+const char* uegccd_strerror(int st) {
+  switch (st) {
+    case UEGCCD_OK: return "ok";
+    case UEGCCD_ERR_INVALID: return "invalid argument or inconsistent system";
+    case UEGCCD_ERR_ALLOC: return "allocation failed";
+    case UEGCCD_ERR_HIP: return "HIP runtime error or no usable device";
+    case UEGCCD_ERR_UNSUPPORTED: return "Input option not supported by this build";
+    case UEGCCD_ERR_NOT_CONVERGED: return "CCD equations did not converge";
+    case UEGCCD_ERR_NONFINITE: return "non-finite amplitudes or residual";
+    case UEGCCD_ERR_BUSY: return "no free slot or HBM budget";
+    case UEGCCD_ERR_NOT_FOUND: return "unknown request";
+    default: return "unknown status";
+  }
+}
+
+int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx** out) {
+  if (!sys || !out || !sys->nvec || !sys->eigen) return UEGCCD_ERR_INVALID;
+  *out = nullptr;
+  if (sys->nocc < 1 || sys->nao <= sys->nocc || !(sys->L > 0.0)) return UEGCCD_ERR_INVALID;
+  int rc = check_supported(*sys);
+  if (rc) return rc;
+  uegccd_options opt;
+  if (opt_in) opt = *opt_in; else uegccd_default_options(&opt);
+  if (opt.mode != UEGCCD_MODE_REFERENCE && opt.mode != UEGCCD_MODE_DIIS) return UEGCCD_ERR_INVALID;
+  if (opt.mode == UEGCCD_MODE_DIIS && (opt.ndiis < 2 || opt.ndiis > kMaxDiis)) return UEGCCD_ERR_INVALID;
+  if (opt.precision != UEGCCD_PREC_FP64 && opt.precision != UEGCCD_PREC_MIXED) return UEGCCD_ERR_INVALID;
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UEGCCD_ERR_HIP;
+  if (opt.device < 0 || opt.device >= ndev) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(opt.device));
+
+  uegccd_ctx* c = new (std::nothrow) uegccd_ctx();
+  if (!c) return UEGCCD_ERR_ALLOC;
+  c->device = opt.device;
+  c->opt = opt;
+  rc = build_maps(c, *sys);
+  if (rc) { delete c; return rc; }
+  find_tol(sys->rs, &c->tolmax, &c->failratio, &c->df);
+  if (opt.tol_dt > 0.0) c->tolmax = opt.tol_dt;
+  if (opt.damping > 0.0) c->df = opt.damping;
+  c->maxiter = opt.max_iter > 0 ? opt.max_iter : 1000;
+
+  const int ndiis = opt.mode == UEGCCD_MODE_DIIS ? opt.ndiis : 1;
+  const size_t n3 = c->n3;
+  const int eb = (c->n3 + kElemThreads - 1) / kElemThreads;
+  const size_t ovo = (size_t)c->o * c->v * c->o;
+  size_t bytes = 0;
+  bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
+  bytes += aligned(sizeof(double) * (c->n2max + 1)) + aligned(sizeof(float) * (c->n2max + 1));
+  bytes += aligned(sizeof(int) * n3);
+  bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
+  bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
+  bytes += 2 * aligned(sizeof(int) * c->o * c->o);
+  bytes += aligned(sizeof(double) * 2 * n3) * (1 + 2 * ndiis + 3);
+  bytes += aligned(sizeof(double) * 2 * std::max(1, c->wtot));
+  bytes += aligned(sizeof(double) * 6 * std::max<size_t>(1, ovo));
+  bytes += aligned(sizeof(double) * (c->v + c->o));
+  bytes += aligned(sizeof(double) * (size_t)eb * kPartialStride);
+  bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
+  bytes += aligned(sizeof(double) * 2 * c->maxiter);
+  bytes += aligned(sizeof(int) * 6) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
+  bytes += 4096;
+  if (hipMalloc(&c->arena.base, bytes) != hipSuccess) { delete c; return UEGCCD_ERR_ALLOC; }
+  c->arena.size = bytes;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    uegccd_ctx_destroy(c);
+    return UEGCCD_ERR_HIP;
+  }
+  HIPCHK(hipMemsetAsync(c->arena.base, 0, bytes, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+
+  SolveDev& S = c->sd;
+  S.o = c->o; S.v = c->v; S.nao = c->nao; S.nK = c->nK; S.n3 = c->n3; S.n2max = c->n2max;
+  S.ndiis = ndiis;
+  S.mode = opt.mode;
+  S.maxiter = c->maxiter;
+  S.diis_start = std::max(1, opt.diis_start);
+  S.diis_err_is_dt = 1;
+  S.do_ring = sys->do_ring; S.do_xring = sys->do_xring;
+  S.do_ladder = sys->do_ladder; S.do_mosaic = sys->do_mosaic;
+  S.df = c->df;
+  S.tol_dt = c->tolmax;
+  S.tol_e = opt.tol_e;
+  S.tol_r = opt.tol_r;
+  S.mixed_switch = opt.precision == UEGCCD_PREC_MIXED ? opt.mixed_switch : 0.0;
+  std::vector<float> vt32(c->vtab.begin(), c->vtab.end());
+  if ((rc = upload(c, c->nv4, &S.nv4)) || (rc = upload(c, c->eig, &S.eig)) ||
+      (rc = upload(c, c->vtab, &S.vtab)) || (rc = upload(c, vt32, &S.vtab32)) ||
+      (rc = upload(c, c->pidx, &S.pidx)) || (rc = upload(c, c->kb_poff, &S.kb_poff)) ||
+      (rc = upload(c, c->kb_pairs, &S.kb_pairs)) || (rc = upload(c, c->kb_voff, &S.kb_voff)) ||
+      (rc = upload(c, c->kb_va, &S.kb_va)) || (rc = upload(c, c->kb_vb, &S.kb_vb)) ||
+      (rc = upload(c, c->kb_woff, &S.kb_woff)) || (rc = upload(c, c->pair_kb, &S.pair_kb)) ||
+      (rc = upload(c, c->pair_pos, &S.pair_pos))) {
+    uegccd_ctx_destroy(c);
+    return rc;
+  }
+  S.tcur = c->arena.take<double>(2 * n3);
+  S.histT = c->arena.take<double>(2 * n3 * ndiis);
+  S.histE = c->arena.take<double>(2 * n3 * ndiis);
+  S.gout = c->arena.take<double>(2 * n3);
+  S.rout = c->arena.take<double>(2 * n3);
+  S.lad = c->arena.take<double>(2 * n3);
+  S.wtot = c->wtot;
+  S.w = c->arena.take<double>(2 * std::max(1, c->wtot));
+  S.jr = c->arena.take<double>(6 * std::max<size_t>(1, ovo));
+  S.jvo = c->arena.take<double>(c->v + c->o);
+  S.nblk_elem = eb;
+  S.partials = c->arena.take<double>((size_t)eb * kPartialStride);
+  c->d_state = c->arena.take<State>(1);
+  S.st = c->d_state;
+  S.trace = c->arena.take<double>(2 * (size_t)c->maxiter);
+  c->d_sd = c->arena.take<SolveDev>(1);
+  if (c->arena.used > c->arena.size) { uegccd_ctx_destroy(c); return UEGCCD_ERR_ALLOC; }
+  if (hipMemcpy(c->d_sd, &S, sizeof(SolveDev), hipMemcpyHostToDevice) != hipSuccess) {
+    uegccd_ctx_destroy(c);
+    return UEGCCD_ERR_HIP;
+  }
+  rc = upload_batch_of_one(c);
+  if (rc) { uegccd_ctx_destroy(c); return rc; }
+  if (c->arena.used > c->arena.size) { uegccd_ctx_destroy(c); return UEGCCD_ERR_ALLOC; }
+  *out = c;
+  return UEGCCD_OK;
+}
+
+void uegccd_ctx_destroy(uegccd_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) {
+    hipStreamSynchronize(c->stream);
+    hipStreamDestroy(c->stream);
+  }
+  if (c->arena.base) hipFree(c->arena.base);
+  delete c;
+}
+
+int uegccd_ctx_info(const uegccd_ctx* c, int32_t* nocc, int32_t* nao, int64_t* nvalid,
+                    int32_t* nblocks) {
+  if (!c) return UEGCCD_ERR_INVALID;
+  if (nocc) *nocc = c->o;
+  if (nao) *nao = c->nao;
+  if (nvalid) *nvalid = c->nvalid;
+  if (nblocks) *nblocks = c->nK;
+  return UEGCCD_OK;
+}
+
+int uegccd_residual(uegccd_ctx* c, const double* tab, const double* tba, double* gab,
+                    double* gba, double* rab, double* rba) {
+  if (!c || !tab || !tba) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = copy_in_amplitudes(c, tab, tba);
+  if (rc) return rc;
+  launch_state_reset(c->batch, PH_ITER, 1, c->stream);
+  launch_iteration(c->batch, MF_WRITE_GR, c->stream);
+  HIPCHK(hipGetLastError());
+  const size_t n3 = c->n3;
+  std::vector<double> h(4 * n3);
+  HIPCHK(hipMemcpyAsync(h.data(), c->sd.gout, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(h.data() + 2 * n3, c->sd.rout, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t x = 0; x < n3; ++x) {
+    const bool ok = c->pidx[x] >= 0;
+    if (gab) gab[x] = ok ? h[x] : 0.0;
+    if (gba) gba[x] = ok ? h[n3 + x] : 0.0;
+    if (rab) rab[x] = ok ? h[2 * n3 + x] : 0.0;
+    if (rba) rba[x] = ok ? h[3 * n3 + x] : 0.0;
+  }
+  return UEGCCD_OK;
+}
+
+int uegccd_energy(uegccd_ctx* c, const double* tab, const double* tba, double* ecorr) {
+  if (!c || !tab || !tba || !ecorr) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = copy_in_amplitudes(c, tab, tba);
+  if (rc) return rc;
+  launch_state_reset(c->batch, PH_ITER, 1, c->stream);
+  launch_init(c->batch, c->stream);
+  launch_finalize(c->batch, FIN_SUM, c->stream);
+  HIPCHK(hipGetLastError());
+  State st;
+  rc = read_state(c, &st);
+  if (rc) return rc;
+  *ecorr = st.ecorr;
+  return st.nonfinite ? UEGCCD_ERR_NONFINITE : UEGCCD_OK;
+}
+
+int uegccd_mp2(uegccd_ctx* c, double* taa, double* tab, double* tba, double* e2) {
+  if (!c) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  launch_state_reset(c->batch, PH_ITER, 1, c->stream);
+  launch_mp2(c->batch, c->stream);
+  launch_finalize(c->batch, FIN_SUM, c->stream);
+  HIPCHK(hipGetLastError());
+  const size_t n3 = c->n3;
+  std::vector<double> h(2 * n3);
+  HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, c->stream));
+  State st;
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  for (size_t x = 0; x < n3; ++x) {
+    if (tab) tab[x] = h[x];
+    if (tba) tba[x] = h[n3 + x];
+    if (taa) taa[x] = h[x] + h[n3 + x];
+  }
+  if (e2) *e2 = st.ecorr;
+  return UEGCCD_OK;
+}
+
+int uegccd_trace(const uegccd_ctx* c, double* e, double* dt, int32_t n) {
+  if (!c) return UEGCCD_ERR_INVALID;
+  const int m = std::min<int>(n, std::min(c->last_niter, c->maxiter));
+  if (m <= 0) return UEGCCD_OK;
+  std::vector<double> h(2 * (size_t)m);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(h.data(), c->sd.trace, sizeof(double) * 2 * m, hipMemcpyDeviceToHost));
+  for (int k = 0; k < m; ++k) {
+    if (e) e[k] = h[2 * k];
+    if (dt) dt[k] = h[2 * k + 1];
+  }
+  return UEGCCD_OK;
+}
+
+int uegccd_solve(uegccd_ctx* c, double* taa, double* tab, double* tba, uegccd_result* res) {
+  uegccd_ctx* cs[1] = {c};
+  double* pab[1] = {tab};
+  double* pba[1] = {tba};
+  if (!c || !tab || !tba) return UEGCCD_ERR_INVALID;
+  int rc = uegccd_solve_batch(cs, 1, pab, pba, res);
+  if (taa && (rc == UEGCCD_OK || rc == UEGCCD_ERR_NOT_CONVERGED))
+    for (int x = 0; x < c->n3; ++x) taa[x] = tab[x] + tba[x];
+  return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Solves of one or more contexts, iterated together: every launch of a pass
+// covers all systems' tiles.  Each system keeps its own state, convergence
+// test and history; a finished system's blocks return at once.
+namespace {
+
+struct BatchScratch {
+  char* base = nullptr;
+  BatchDev b{};
+};
+
+int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
+  std::vector<SolveDev> sds(n);
+  std::vector<int> ep(n + 1, 0), mp(n + 1, 0), rp(n + 1, 0);
+  std::vector<int4> items;
+  for (int k = 0; k < n; ++k) {
+    uegccd_ctx* c = cs[k];
+    sds[k] = c->sd;
+    ep[k + 1] = ep[k] + (c->n3 + kElemThreads - 1) / kElemThreads;
+    mp[k + 1] = mp[k] + (c->sd.do_mosaic ? c->v + c->o : 0);
+    rp[k + 1] = rp[k] + ((c->sd.do_ring || c->sd.do_xring)
+                             ? (2 * c->o * c->v * c->o + kElemThreads - 1) / kElemThreads : 0);
+    if (c->sd.do_ladder)
+      for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
+  }
+  size_t bytes = aligned(sizeof(SolveDev) * n) + 3 * aligned(sizeof(int) * (n + 1)) +
+                 aligned(sizeof(int4) * std::max<size_t>(1, items.size()));
+  if (hipMalloc(&out->base, bytes) != hipSuccess) return UEGCCD_ERR_ALLOC;
+  DeviceArena a;
+  a.base = out->base;
+  a.size = bytes;
+  BatchDev& b = out->b;
+  b.n = n;
+  b.solves = a.take<SolveDev>(n);
+  b.elem_prefix = a.take<int>(n + 1);
+  b.mos_prefix = a.take<int>(n + 1);
+  b.ring_prefix = a.take<int>(n + 1);
+  b.lad_items = a.take<int4>(std::max<size_t>(1, items.size()));
+  b.nlad = (int)items.size();
+  b.elem_blocks = ep[n];
+  b.mos_blocks = mp[n];
+  b.ring_blocks = rp[n];
+  HIPCHK(hipMemcpy(b.solves, sds.data(), sizeof(SolveDev) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.elem_prefix, ep.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.mos_prefix, mp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.ring_prefix, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  if (!items.empty())
+    HIPCHK(hipMemcpy(b.lad_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+  return UEGCCD_OK;
+}
+
+// Runs the iteration of a batch already holding its starting amplitudes in
+// tcur: init (E, dT of the guess), then passes until every system is done.
+int iterate(uegccd_ctx** cs, int n, const BatchDev& b, std::vector<State>& st, float* ms_out) {
+  uegccd_ctx* c0 = cs[0];
+  hipStream_t s = c0->stream;
+  const int prec64 = c0->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
+  for (int k = 0; k < n; ++k)
+    if ((cs[k]->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1) != prec64) return UEGCCD_ERR_INVALID;
+  int maxiter = 0;
+  for (int k = 0; k < n; ++k) maxiter = std::max(maxiter, cs[k]->maxiter);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  launch_state_reset(b, PH_ITER, prec64, s);
+  launch_init(b, s);
+  launch_finalize(b, FIN_INIT, s);
+  HIPCHK(hipGetLastError());
+  const int chunk = std::max(1, c0->opt.iters_per_sync);
+  st.assign(n, State{});
+  int passes = 0;
+  bool all_done = false;
+  while (!all_done && passes <= maxiter + 2) {
+    for (int k = 0; k < chunk; ++k) launch_iteration(b, MF_ITERATE, s);
+    passes += chunk;
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < n; ++k)
+      HIPCHK(hipMemcpyAsync(&st[k], cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    all_done = true;
+    for (int k = 0; k < n; ++k) all_done = all_done && st[k].phase == PH_DONE;
+  }
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ms_out) *ms_out = ms;
+  for (int k = 0; k < n; ++k) cs[k]->last_niter = st[k].niter;
+  return UEGCCD_OK;
+}
+
+int fill_result(uegccd_ctx* c, const State& st, float ms, uegccd_result* res) {
+  int status = UEGCCD_OK;
+  if (st.nonfinite) status = UEGCCD_ERR_NONFINITE;
+  else if (st.phase != PH_DONE || !st.converged) status = UEGCCD_ERR_NOT_CONVERGED;
+  if (res) {
+    std::memset(res, 0, sizeof(*res));
+    res->ecorr = st.ecorr;
+    res->dt = st.dt;
+    res->maxres = st.maxres;
+    res->niter = st.niter;
+    res->failed = (st.maxres > c->failratio * c->tolmax) || status != UEGCCD_OK;
+    res->iters_fp32 = st.iters_fp32;
+    res->status = status;
+    res->solve_ms = ms;
+  }
+  return status;
+}
+
+int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
+  const size_t n3 = c->n3;
+  std::vector<double> h(2 * n3);
+  HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t x = 0; x < n3; ++x) {
+    const bool ok = c->pidx[x] >= 0;
+    const double a = ok ? h[x] : 0.0, b = ok ? h[n3 + x] : 0.0;
+    if (tab) tab[x] = a;
+    if (tba) tba[x] = b;
+    if (taa) taa[x] = a + b;
+  }
+  return UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" int uegccd_solve_batch(uegccd_ctx** cs, int32_t n, double** tab, double** tba,
+                                  uegccd_result* res) {
+  if (!cs || n <= 0 || !tab || !tba) return UEGCCD_ERR_INVALID;
+  for (int k = 0; k < n; ++k)
+    if (!cs[k] || !tab[k] || !tba[k] || cs[k]->device != cs[0]->device) return UEGCCD_ERR_INVALID;
+  uegccd_ctx* c0 = cs[0];
+  HIPCHK(hipSetDevice(c0->device));
+  BatchScratch scratch;
+  const BatchDev* b = &c0->batch;
+  if (n > 1) {
+    int rc = make_batch(cs, n, &scratch);
+    if (rc) { if (scratch.base) hipFree(scratch.base); return rc; }
+    b = &scratch.b;
+  }
+  int rc = UEGCCD_OK;
+  for (int k = 0; k < n && !rc; ++k) rc = copy_in_amplitudes(cs[k], tab[k], tba[k]);
+  std::vector<State> st;
+  float ms = 0.f;
+  if (!rc) rc = iterate(cs, n, *b, st, &ms);
+  if (scratch.base) {
+    hipStreamSynchronize(c0->stream);
+    hipFree(scratch.base);
+  }
+  if (rc) return rc;
+  int worst = UEGCCD_OK;
+  for (int k = 0; k < n; ++k) {
+    int r2 = copy_out_amplitudes(cs[k], nullptr, tab[k], tba[k]);
+    if (r2) return r2;
+    int status = fill_result(cs[k], st[k], ms, res ? &res[k] : nullptr);
+    if (status != UEGCCD_OK && worst == UEGCCD_OK) worst = status;
+  }
+  return worst;
+}
+
+extern "C" int uegccd_solve_resident(uegccd_ctx* c, int32_t from_mp2, uegccd_result* res) {
+  if (!c) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  if (from_mp2) {
+    launch_state_reset(c->batch, PH_ITER, 1, c->stream);
+    launch_mp2(c->batch, c->stream);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<State> st;
+  float ms = 0.f;
+  uegccd_ctx* cs[1] = {c};
+  int rc = iterate(cs, 1, c->batch, st, &ms);
+  if (rc) return rc;
+  return fill_result(c, st[0], ms, res);
+}
+
+extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
+  if (!c) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  return copy_out_amplitudes(c, taa, tab, tba);
+}
+
+extern "C" int uegccd_kernel_time(uegccd_ctx* c, int32_t which, int32_t reps, double* avg_us) {
+  if (!c || !avg_us || reps <= 0 || which < 0 || which > 4) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  launch_state_reset(c->batch, PH_ITER, 1, s);
+  launch_one(c->batch, which, s);  // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  for (int k = 0; k < reps; ++k) launch_one(c->batch, which, s);
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipGetLastError());
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *avg_us = 1000.0 * ms / reps;
+  launch_state_reset(c->batch, PH_DONE, 1, s);
+  HIPCHK(hipStreamSynchronize(s));
+  return UEGCCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plane-wave basis, init_HEG + SetupBasis + change_rs (source/HEG.f90:16-43,
+// 150-324, 494-533), CalcMadelung (source/HEG.f90:89-148) and print_HF's E_HF
+// (source/HEG.f90:581-615).
+namespace {
+
+double madelung_ewald(double omega, double L) {
+  const double kappa = 2.8 / std::pow(omega, 1.0 / 3.0);
+  const double term2 = -kPi / (std::pow(kappa, 2.0) * std::pow(L, 3.0));
+  const double term4 = -2.0 * kappa / std::sqrt(kPi);
+  // the reference accumulates cubes of half-width 1..10 and keeps the last
+  double recip = 0.0, real = 0.0;
+  const int l4 = 10;
+  for (int a = -l4; a <= l4; ++a)
+    for (int b = -l4; b <= l4; ++b)
+      for (int c = -l4; c <= l4; ++c) {
+        const int n2 = a * a + b * b + c * c;
+        if (n2 == 0) continue;
+        const double k2 = (1.0 / std::pow(L, 2.0)) * (double)n2;
+        recip += (1.0 / std::pow(L, 3.0)) * (1.0 / (kPi * k2)) *
+                 std::exp(-std::pow(kPi, 2.0) * k2 / std::pow(kappa, 2.0));
+        const double r = L * std::sqrt((double)n2);
+        real += std::erfc(kappa * r) / r;
+      }
+  return real + recip + term2 + term4;
+}
+
+int fcut_count(double f, int nel) {
+  int m = (int)(std::pow(f, 1.5) * nel / 2.0);
+  const double t = std::pow(f, 1.5) * nel / 2.0;
+  if (std::fabs(m - t) > std::fabs(m + 1 - t)) m += 1;
+  return m;
+}
+
+}  // namespace
+
+extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, int32_t* nvec,
+                                double* eigen, double* scalars) {
+  if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0))
+    return UEGCCD_ERR_INVALID;
+  const int nel = p->nel, no = nel / 2;
+  const int cell = (int)std::sqrt((double)p->maxkpoint) + 3;
+  struct PW { int n[3]; double nt2; double nt[3]; };
+  std::vector<PW> pw;
+  for (int a = -cell; a <= cell; ++a)
+    for (int b = -cell; b <= cell; ++b)
+      for (int c = -cell; c <= cell; ++c) {
+        if (a * a + b * b + c * c > p->maxkpoint) continue;
+        PW w;
+        w.n[0] = a; w.n[1] = b; w.n[2] = c;
+        w.nt[0] = a + p->twist[0];
+        w.nt[1] = b + p->twist[1];
+        w.nt[2] = c + p->twist[2];
+        w.nt2 = w.nt[0] * w.nt[0] + w.nt[1] * w.nt[1] + w.nt[2] * w.nt[2];
+        pw.push_back(w);
+      }
+  // ties keep enumeration order, as the reference's bubble sort on strict '<'
+  std::stable_sort(pw.begin(), pw.end(), [](const PW& x, const PW& y) { return x.nt2 < y.nt2; });
+  int nb = (int)pw.size();
+  const int mf = fcut_count(p->fcut, nel);
+  if (mf < nb) nb = mf;
+  const int mc = fcut_count(p->corefcut, nel);
+  if ((p->corefcut - 1.0) < 1e-6 && mc != no) return UEGCCD_ERR_INVALID;
+  nb = nb - mc + no;
+  if (nb <= no) return UEGCCD_ERR_INVALID;
+  *nao_out = nb;
+  if (!nvec && !eigen && !scalars) return UEGCCD_OK;
+  // geometry at r_s = 1, then rescaled to r_s as change_rs does
+  const double L1 = 1.0 * std::pow(4.0 / 3.0 * kPi * nel, 1.0 / 3.0);
+  const double rescale = p->rs / 1.0;
+  const double L = L1 * rescale;
+  const double omega = std::pow(L1, 3) * std::pow(rescale, 3.0);
+  const double mad = madelung_ewald(omega, L) * p->madelungfactor;
+  std::vector<double> kv((size_t)nb * 3);
+  for (int q = 0; q < nb; ++q) {
+    const PW& w = pw[q < no ? q : q + mc - no];
+    for (int d = 0; d < 3; ++d) {
+      kv[3 * q + d] = (w.nt[d] * (2.0 * kPi / L1)) * (1.0 / rescale);
+      if (nvec) nvec[(size_t)d * nb + q] = w.n[d];
+    }
+  }
+  if (eigen) {
+    for (int q = 0; q < nb; ++q) {
+      const double* k1 = &kv[3 * q];
+      double e = 0.5 * (k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+      for (int l = 0; l < no; ++l) {
+        if (l != q) {
+          const double* k2 = &kv[3 * l];
+          const double d0 = k1[0] - k2[0], d1 = k1[1] - k2[1], d2v = k1[2] - k2[2];
+          e = e - (1.0 / omega) * (4.0 * kPi / (d0 * d0 + d1 * d1 + d2v * d2v)) * p->exchangefactor;
+        } else {
+          e = e + mad - p->gap;
+        }
+      }
+      eigen[q] = e;
+    }
+  }
+  if (scalars) {
+    double ehf = 0.0;
+    for (int q = 0; q < no; ++q) {
+      const double* k1 = &kv[3 * q];
+      ehf = ehf + 0.5 * (k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+      for (int l = 0; l < no; ++l) {
+        if (l == q) continue;
+        const double* k2 = &kv[3 * l];
+        const double d0 = k1[0] - k2[0], d1 = k1[1] - k2[1], d2v = k1[2] - k2[2];
+        ehf = ehf - 0.5 * (1.0 / omega) * (4.0 * kPi / (d0 * d0 + d1 * d1 + d2v * d2v));
+      }
+    }
+    ehf = ehf * 2.0 + mad * 0.5 * nel;
+    const double dens = nel / std::pow(L, 3.0);
+    const double kf = std::pow(3.0 * std::pow(kPi, 2.0) * dens, 1.0 / 3.0);
+    double tolmax, failratio, df;
+    find_tol(p->rs, &tolmax, &failratio, &df);
+    scalars[0] = L;
+    scalars[1] = omega;
+    scalars[2] = mad;
+    scalars[3] = ehf;
+    scalars[4] = std::pow(4.0 / kPi * kf, 0.5);
+    scalars[5] = df;
+    scalars[6] = tolmax;
+    scalars[7] = failratio;
+  }
+  return UEGCCD_OK;
+}

@@ -1,0 +1,106 @@
+// Internal data model of libuegccd: device-resident per-solve descriptors and
+// the iteration state the kernels read and advance on the device.
+//
+// Layout in HBM (per solve, o = NOcc, v = NAO - NOcc, n3 = o*o*v):
+//   amplitudes, residuals, G: dense FP64 T(i,j,a) with i fastest, then j, then
+//   the virtual a (the reference's T(NOcc,NOcc,NOcc+1:NAO) column-major order),
+//   spin blocks "abab" then "abba" back to back (2*n3 doubles).
+//   pidx[n3]: momentum partner b(i,j,a) = k_i + k_j - k_a as a 0-based orbital
+//   index when it is a virtual orbital of the basis, else -1.
+//   Pair-momentum blocks K = n_i + n_j: their occupied pairs (x = i + o*j) and
+//   their virtual columns a (with partner b), as CSR lists.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace uegccd {
+
+constexpr int kMaxDiis = 16;
+constexpr int kPartialStride = 4 + kMaxDiis;  // E, dT, max|R|, non-finite count, DIIS dots
+constexpr int kElemThreads = 256;
+constexpr int kMaxNao = 1024;                 // orbitals staged in LDS
+constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the Coulomb table
+
+enum Phase : int { PH_ITER = 0, PH_FINAL = 1, PH_DONE = 2 };
+enum FinKind : int { FIN_ITER = 0, FIN_INIT = 1, FIN_SUM = 2 };
+enum MainFlags : int { MF_ITERATE = 1, MF_WRITE_GR = 2 };
+
+struct State {
+  int niter;        // NIter as counted at source/CCD.f90:62,445
+  int phase;        // Phase
+  int slot;         // history slot the next residual pass writes
+  int slot_cur;     // slot written by the last completed pass
+  int nhist;        // filled DIIS history entries
+  int use_coef;     // 1: next amplitudes = sum_k coef[k] histT[k]
+  int prec64;       // 1: FP64 ladders, 0: FP32 ladders
+  int iters_fp32;
+  int nonfinite;
+  int converged;
+  double ecorr, ecorr_prev, dt, maxres, maxres_in;
+  double coef[kMaxDiis];
+  double bmat[kMaxDiis * kMaxDiis];
+};
+
+struct SolveDev {
+  int o, v, nao, nK;
+  int n3;           // o*o*v
+  int n2max;        // largest |n_p - n_q|^2 in the basis
+  int ndiis;        // history length (1 in reference mode)
+  int mode;         // UEGCCD_MODE_*
+  int maxiter, diis_start, diis_err_is_dt;
+  int do_ring, do_xring, do_ladder, do_mosaic;
+  double df;        // DenomFactor
+  double tol_dt, tol_e, tol_r, mixed_switch;
+  // constant per basis
+  const int4* nv4;      // [nao] (n1,n2,n3,0)
+  const double* eig;    // [nao]
+  const double* vtab;   // [n2max+1] Coulomb table v(|dn|^2)
+  const float* vtab32;  // same, FP32
+  const int* pidx;      // [n3]
+  const int* kb_poff;   // [nK+1]
+  const int* kb_pairs;  // [o*o] pair ids x = i + o*j grouped by K
+  const int* kb_voff;   // [nK+1]
+  const int* kb_va;     // virtual columns a (orbital index) grouped by K
+  const int* kb_vb;     // their partners b
+  const int* kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
+  const int* pair_kb;   // [o*o] block of pair x
+  const int* pair_pos;  // [o*o] position of pair x inside its block
+  // mutable buffers
+  double* tcur;         // [2*n3] amplitudes entering the next residual pass
+  double* histT;        // [ndiis][2*n3] updated amplitudes
+  double* histE;        // [ndiis][2*n3] DIIS error vectors
+  double* gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
+  double* rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
+  double* lad;          // [2*n3] particle-particle ladder, S and P parts
+  double* w;            // [2*wtot] hole-hole W intermediates, S and P parts
+  int wtot;
+  double* jr;           // [6*o*v*o] ring / crossed-ring quadratic intermediates
+  double* jvo;          // [v+o] mosaic Jvv then Joo
+  double* partials;     // [nblk_elem * kPartialStride]
+  int nblk_elem;
+  State* st;
+  double* trace;        // [2*maxiter] (E, dT) by NIter
+};
+
+// Launch geometry of a batch of solves iterated together.
+struct BatchDev {
+  int n;
+  SolveDev* solves;     // device array [n]
+  int* elem_prefix;     // [n+1] 256-thread blocks over each solve's n3
+  int* mos_prefix;      // [n+1] v+o blocks
+  int* ring_prefix;     // [n+1] 256-thread blocks over 2*o*v*o
+  int4* lad_items;      // (solve, K, row0, col0) ladder tiles
+  int nlad;
+  int elem_blocks, mos_blocks, ring_blocks;
+};
+
+// Kernel launchers (kernels.hip).
+void launch_init(const BatchDev& b, hipStream_t s);
+void launch_mp2(const BatchDev& b, hipStream_t s);
+void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s);
+void launch_finalize(const BatchDev& b, int kind, hipStream_t s);
+void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
+// One kernel of the pass: 0 mosaic J, 1 ring J, 2 ladder, 3 main, 4 extrapolation.
+void launch_one(const BatchDev& b, int which, hipStream_t s);
+
+}  // namespace uegccd

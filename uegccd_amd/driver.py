@@ -1,0 +1,297 @@
+"""Host mirror of the reference's solver interface, running on libuegccd.
+
+Names follow the reference so a caller can switch over routine by routine:
+  init_heg      init_HEG + change_rs      (source/HEG.f90:16-43, 494-533)
+  DrvMBPT       MP2 guess and E(2)        (source/MP2.f90:11-157)
+  DrvCCD        the CCD solve             (source/CCD.f90:18-183)
+  CCDContext    a solver bound to one basis (residual / energy / solve)
+  single_point  Main's DoSingleCalc path  (source/Main.f90:44-97)
+Amplitude arrays are numpy float64 arrays of shape (NOcc, NOcc, NAO-NOcc) in
+Fortran order, the reference's T(NOcc,NOcc,NOcc+1:NAO).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lib as L
+from .inputfile import UEGInfo, parse_input
+
+
+@dataclass
+class HEGData:
+    """HEGDataType (source/Types.f90:7-27) at the calculation's r_s."""
+    Nel: int
+    NOcc: int
+    NAO: int
+    L: float
+    Omega: float
+    Madelung: float
+    EHF: float
+    ScreeningDistance: float
+    rS: float
+    nTwist: tuple
+    nVec: np.ndarray          # (NAO, 3) int32
+    Eigen: np.ndarray         # (NAO,) float64
+    DenomFactor: float = 2.0
+    TolMax: float = 1e-8
+    FailRatio: float = 100.0
+    ECorr: float = 0.0
+
+
+def init_heg(info: UEGInfo, twist=None) -> HEGData:
+    """Basis, eigenvalues, Madelung and E_HF through uegccd_heg_build."""
+    lib = L.load()
+    p = L.HegParams()
+    p.nel, p.maxkpoint, p.rs = info.NElectron, info.MaxKPoint, info.rSMin
+    tw = info.NTwist if twist is None else twist
+    for d in range(3):
+        p.twist[d] = float(tw[d])
+    p.fcut, p.corefcut = info.fcut, info.corefcut
+    p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+    nao = ctypes.c_int32()
+    L.check(lib.uegccd_heg_build(ctypes.byref(p), ctypes.byref(nao), None, None, None),
+            "uegccd_heg_build")
+    n = nao.value
+    nvec_cm = np.zeros(3 * n, dtype=np.int32)
+    eig = np.zeros(n)
+    sc = np.zeros(8)
+    L.check(lib.uegccd_heg_build(ctypes.byref(p), ctypes.byref(nao), L.iptr(nvec_cm),
+                                 L.dptr(eig), L.dptr(sc)), "uegccd_heg_build")
+    info.NAO = n
+    return HEGData(Nel=info.NElectron, NOcc=info.NElectron // 2, NAO=n, L=sc[0], Omega=sc[1],
+                   Madelung=sc[2], EHF=sc[3], ScreeningDistance=sc[4], rS=info.rSMin,
+                   nTwist=tuple(tw), nVec=nvec_cm.reshape(3, n).T.copy(), Eigen=eig,
+                   DenomFactor=sc[5], TolMax=sc[6], FailRatio=sc[7])
+
+
+def options(mode="reference", ndiis=8, precision="fp64", **kw) -> L.Options:
+    lib = L.load()
+    o = L.Options()
+    lib.uegccd_default_options(ctypes.byref(o))
+    o.mode = {"reference": L.MODE_REFERENCE, "diis": L.MODE_DIIS}[mode]
+    o.ndiis = ndiis
+    o.precision = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}[precision]
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class CCDContext:
+    """A device context for one basis: uegccd_ctx_create .. uegccd_ctx_destroy."""
+
+    def __init__(self, heg: HEGData, info: UEGInfo, opts: L.Options | None = None):
+        self._lib = L.load()
+        self.heg, self.info = heg, info
+        self.nocc, self.nao = heg.NOcc, heg.NAO
+        self.shape = (self.nocc, self.nocc, self.nao - self.nocc)
+        s = L.System()
+        self._nvec = np.ascontiguousarray(heg.nVec.T, dtype=np.int32).reshape(-1)  # nVec(NAO,3)
+        self._eig = np.ascontiguousarray(heg.Eigen, dtype=np.float64)
+        s.nocc, s.nao = self.nocc, self.nao
+        s.nvec, s.eigen = L.iptr(self._nvec), L.dptr(self._eig)
+        s.L, s.madelung, s.rs, s.gapii = heg.L, heg.Madelung, heg.rS, info.GapII
+        s.coren, s.safe_eri = info.CoreN, int(info.SafeERI)
+        for k in range(15):
+            s.irange[k] = int(info.irange[k])
+        s.do_ring, s.do_xring = int(info.DoRing), int(info.DoXRing)
+        s.do_ladder, s.do_mosaic = int(info.DoLadder), int(info.DoMosaic)
+        self.opts = opts if opts is not None else options()
+        h = ctypes.c_void_p()
+        L.check(self._lib.uegccd_ctx_create(ctypes.byref(s), ctypes.byref(self.opts),
+                                            ctypes.byref(h)), "uegccd_ctx_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.uegccd_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def zeros(self):
+        return np.zeros(self.shape, order="F")
+
+    def info_sizes(self):
+        no, na, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        nv = ctypes.c_int64()
+        L.check(self._lib.uegccd_ctx_info(self._h, ctypes.byref(no), ctypes.byref(na),
+                                          ctypes.byref(nv), ctypes.byref(nb)))
+        return dict(nocc=no.value, nao=na.value, nvalid=nv.value, nblocks=nb.value)
+
+    def _f(self, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"amplitude shape {a.shape} != {self.shape}")
+        return a
+
+    def mp2(self):
+        taa, tab, tba = self.zeros(), self.zeros(), self.zeros()
+        e2 = ctypes.c_double()
+        L.check(self._lib.uegccd_mp2(self._h, L.dptr(taa), L.dptr(tab), L.dptr(tba),
+                                     ctypes.byref(e2)), "uegccd_mp2")
+        return e2.value, taa, tab, tba
+
+    def residual(self, tab, tba):
+        tab, tba = self._f(tab), self._f(tba)
+        out = [self.zeros() for _ in range(4)]
+        L.check(self._lib.uegccd_residual(self._h, L.dptr(tab), L.dptr(tba),
+                                          *[L.dptr(x) for x in out]), "uegccd_residual")
+        return tuple(out)
+
+    def energy(self, tab, tba):
+        tab, tba = self._f(tab), self._f(tba)
+        e = ctypes.c_double()
+        L.check(self._lib.uegccd_energy(self._h, L.dptr(tab), L.dptr(tba), ctypes.byref(e)),
+                "uegccd_energy")
+        return e.value
+
+    def solve(self, tab, tba, allow_unconverged=False):
+        tab = np.array(self._f(tab), order="F", copy=True)
+        tba = np.array(self._f(tba), order="F", copy=True)
+        taa = self.zeros()
+        r = L.Result()
+        st = self._lib.uegccd_solve(self._h, L.dptr(taa), L.dptr(tab), L.dptr(tba),
+                                    ctypes.byref(r))
+        if st not in (L.OK, L.ERR_NOT_CONVERGED) or (st != L.OK and not allow_unconverged):
+            if not (allow_unconverged and st == L.ERR_NOT_CONVERGED):
+                raise L.UegccdError(st, "uegccd_solve")
+        return SolveResult.from_c(r, taa, tab, tba, self.trace(r.niter))
+
+    def trace(self, n):
+        e, dt = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        L.check(self._lib.uegccd_trace(self._h, L.dptr(e), L.dptr(dt), n), "uegccd_trace")
+        return e[:n], dt[:n]
+
+
+@dataclass
+class SolveResult:
+    ecorr: float
+    niter: int
+    dt: float
+    maxres: float
+    failed: bool
+    iters_fp32: int
+    status: int
+    solve_ms: float
+    taa: np.ndarray = field(repr=False, default=None)
+    tab: np.ndarray = field(repr=False, default=None)
+    tba: np.ndarray = field(repr=False, default=None)
+    trace_e: np.ndarray = field(repr=False, default=None)
+    trace_dt: np.ndarray = field(repr=False, default=None)
+
+    @classmethod
+    def from_c(cls, r, taa, tab, tba, trace):
+        return cls(ecorr=r.ecorr, niter=r.niter, dt=r.dt, maxres=r.maxres, failed=bool(r.failed),
+                   iters_fp32=r.iters_fp32, status=r.status, solve_ms=r.solve_ms,
+                   taa=taa, tab=tab, tba=tba, trace_e=trace[0], trace_dt=trace[1])
+
+
+def solve_batch(ctxs, tabs, tbas):
+    """uegccd_solve_batch over several contexts (one launch per kernel per pass)."""
+    lib = L.load()
+    n = len(ctxs)
+    P = ctypes.c_void_p * n
+    D = ctypes.POINTER(ctypes.c_double) * n
+    tabs = [np.array(np.asfortranarray(t, dtype=np.float64), order="F", copy=True) for t in tabs]
+    tbas = [np.array(np.asfortranarray(t, dtype=np.float64), order="F", copy=True) for t in tbas]
+    res = (L.Result * n)()
+    st = lib.uegccd_solve_batch(P(*[c.handle.value for c in ctxs]), n,
+                                D(*[L.dptr(t) for t in tabs]), D(*[L.dptr(t) for t in tbas]), res)
+    if st not in (L.OK, L.ERR_NOT_CONVERGED):
+        raise L.UegccdError(st, "uegccd_solve_batch")
+    out = []
+    for k in range(n):
+        r = res[k]
+        out.append(SolveResult.from_c(r, tabs[k] + tbas[k], tabs[k], tbas[k],
+                                      ctxs[k].trace(r.niter)))
+    return out
+
+
+def DrvMBPT(heg: HEGData, info: UEGInfo, ctx: CCDContext | None = None):
+    """MP2 amplitudes (T2aaaa, T2abab, T2abba); sets heg.ECorr = E(2)."""
+    c = ctx or CCDContext(heg, info)
+    e2, taa, tab, tba = c.mp2()
+    heg.ECorr = e2
+    return taa, tab, tba
+
+
+def DrvCCD(heg: HEGData, info: UEGInfo, T2aaaa, T2abab, T2abba, opts: L.Options | None = None,
+           ctx: CCDContext | None = None):
+    """Solve in place from the given amplitudes; sets heg.ECorr; returns SolveResult."""
+    c = ctx or CCDContext(heg, info, opts)
+    r = c.solve(T2abab, T2abba, allow_unconverged=True)
+    T2abab[...] = r.tab
+    T2abba[...] = r.tba
+    T2aaaa[...] = r.tab + r.tba
+    heg.ECorr = r.ecorr
+    return r
+
+
+# --- Output blocks, formats of source/MP2.f90:143-150 and source/CCD.f90:127-140 ---
+_STAR = " " * 14 + "*" * 50
+_DASH = " " * 14 + "*" + "-" * 48 + "*"
+
+
+def format_mp2_block(heg: HEGData, e2: float) -> str:
+    out = [_STAR, " " * 14 + f"* E(SCF) is {'':12}{heg.EHF:18.12f} a.u.  *", _DASH,
+           " " * 14 + "*                RHF Eigenvalues:                *", _DASH]
+    for e in heg.Eigen[:heg.NOcc]:
+        out.append(" " * 14 + "*" + " " * 12 + f"{e:24.16E}" + " " * 12 + "*")
+    out.append(" " * 14 + "*" + " " * 12 + "-" * 24 + " " * 12 + "*")
+    for e in heg.Eigen[heg.NOcc:]:
+        out.append(" " * 14 + "*" + " " * 12 + f"{e:24.16E}" + " " * 12 + "*")
+    out += [_STAR, " " * 14 + f"* E(2) is {'':14}{e2:18.12f} a.u.  *",
+            " " * 14 + f"* E(SCF) + E(2) is {'':5}{e2 + heg.EHF:18.12f} a.u.  *", _STAR]
+    return "\n".join(out) + "\n"
+
+
+def format_ccd_block(heg: HEGData, info: UEGInfo, r: SolveResult) -> str:
+    out = [" " * 14 + "*               CCD summary follows              *", _DASH]
+    if info.DoRing:
+        out.append(" " * 14 + "* Including ring diagrams...                     *")
+    if info.DoXRing:
+        out.append(" " * 14 + "* Including crossed ring diagrams...             *")
+    if info.DoLadder:
+        out.append(" " * 14 + "* Including ladder diagrams...                   *")
+    if info.DoMosaic:
+        out.append(" " * 14 + "* Including mosaic diagrams...                   *")
+    out += [_DASH, " " * 14 + "*  CCD Energy      Iteration    Biggest change   *"]
+    for k in range(r.niter):
+        out.append(" " * 14 + f"* {r.trace_e[k]:15.10f}  {k + 1:5d}{'':7}{r.trace_dt[k]:14.10f}    *")
+    out.append(_DASH)
+    if r.failed:
+        out.append(" " * 14 + "* Final CCD Energy is   DID NOT CONVERGE         *")
+    else:
+        out.append(" " * 14 + f"* CCD has converged in {r.niter:3d} iterations{'':12}*")
+        out.append(" " * 14 + f"* Ec(CCD) is {'':10}{r.ecorr:18.12f} a.u.   *")
+        out.append(" " * 14 + f"* Final CCD Energy is  {r.ecorr + heg.EHF:18.12f} a.u.   *")
+    out.append(" " * 14 + f"* Max CCD Residual is  {'':4}{r.maxres:18.12E}    *")
+    out.append(_STAR)
+    return "\n".join(out) + "\n"
+
+
+def single_point(input_text: str, opts: L.Options | None = None):
+    """Main's DoSingleCalc path (source/Main.f90:44-97): basis, MP2, CCD; returns
+    (Output text, HEGData, E(2), SolveResult)."""
+    info = parse_input(input_text)
+    if info.DodRPASOSEX:
+        raise L.UegccdError(L.ERR_UNSUPPORTED, "dRPA/SOSEX driver")
+    heg = init_heg(info)
+    ctx = CCDContext(heg, info, opts)
+    taa, tab, tba = DrvMBPT(heg, info, ctx)
+    e2 = heg.ECorr
+    text = format_mp2_block(heg, e2)
+    r = DrvCCD(heg, info, taa, tab, tba, ctx=ctx)
+    text += format_ccd_block(heg, info, r)
+    return text, heg, e2, r
