@@ -59,7 +59,8 @@ def test_residual_terms(name, mask):
         scale = max(np.abs(o_gab).max(), np.abs(o_gba).max())
         assert np.abs(g_ab - o_gab).max() / scale < TERM_TOL, (kind, "G abab")
         assert np.abs(g_ba - o_gba).max() / scale < TERM_TOL, (kind, "G abba")
-        rs = max(np.abs(o_rab).max(), np.abs(o_rba).max())
+        # R = D*T - G vanishes identically for drivers-only at MP2: scale by G too
+        rs = max(np.abs(o_rab).max(), np.abs(o_rba).max(), scale)
         assert np.abs(r_ab - o_rab).max() / rs < 1e-11
         assert np.abs(r_ba - o_rba).max() / rs < 1e-11
 
@@ -117,15 +118,25 @@ def test_reference_trajectory_vs_oracle(name):
     assert abs(r.maxres - o["maxres"]) < 1e-12
 
 
-@pytest.mark.parametrize("name", ["n14_cut5", "n24_rs5", "n54_cut9"])
-def test_diis_converges_to_oracle_fixed_point(name):
+@pytest.mark.parametrize("name,tol", [("n14_cut5", 1e-11), ("n24_rs5", 1e-10), ("n54_cut9", 1e-11)])
+def test_diis_converges_to_oracle_fixed_point(name, tol):
+    """DIIS mode reaches the fixed point of the reference's equations: the
+    oracle's own residual at the returned amplitudes is below 1e-7, its energy
+    of them equals the returned one, and the energy is within 1e-9 Ha of the
+    oracle's damped iteration run to max|dT| < tol.  (For the N=24 r_s=5
+    system the damped iteration turns unstable past ~100 iterations, so its
+    reference is taken at 1e-10.)"""
     print(f"[diis] {name}", flush=True)
     info = info_for(name)
     orc = oracle_for(info)
-    tight = orc.ccd(tolmax=1e-12, maxiter=1000)
+    tight = orc.ccd(tolmax=tol, maxiter=1000)
+    assert np.isfinite(tight["ecorr"]) and tight["trace_dt"][-1] < tol
     heg, ctx = _product(info, mode="diis", ndiis=8)
     _, _, tab, tba = ctx.mp2()
     r = ctx.solve(tab, tba)
     assert r.status == 0 and not r.failed
+    _, _, rab, rba = orc.terms(r.tab, r.tba)
+    assert max(np.abs(rab).max(), np.abs(rba).max()) < 1e-7
+    assert abs(orc.energy(r.tab + r.tba, r.tab, r.tba) - r.ecorr) < 1e-12
     assert abs(r.ecorr - tight["ecorr"]) < 1e-9
     assert r.niter < tight["niter"]

@@ -255,12 +255,16 @@ __global__ __launch_bounds__(kElemThreads) void k_ring_j(const SolveDev* __restr
 // f32 form while the mixed-precision schedule is in its FP32 phase).
 template <typename Real>
 struct MfmaTraits;
+// Operands of both forms: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result rows differ (measured on gfx950): the f64 form returns
+// D[(l/16) + 4r][l%16], the f32 form D[4(l/16) + r][l%16], r = 0..3.
 template <>
 struct MfmaTraits<double> {
   typedef v4d acc_t;
   __device__ static acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int kq, int r) { return kq + 4 * r; }
 };
 template <>
 struct MfmaTraits<float> {
@@ -268,6 +272,7 @@ struct MfmaTraits<float> {
   __device__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static int row(int kq, int r) { return 4 * kq + r; }
 };
 
 template <typename Real>
@@ -331,10 +336,8 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
     accp = MfmaTraits<Real>::mma(pv, bp, accp);
   }
   if (!colv) return;
-  // D layout of the 16x16 MFMA result: lane holds column lane%16, rows
-  // 4*(lane/16) + r for r = 0..3.
   for (int r = 0; r < 4; ++r) {
-    const int rowo = row0 + 4 * kq + r;
+    const int rowo = row0 + MfmaTraits<Real>::row(kq, r);
     if (rowo >= noo) continue;
     const int xpo = S.kb_pairs[poff + rowo];
     const double as = (double)accs[r], ap = (double)accp[r];

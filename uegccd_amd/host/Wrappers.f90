@@ -1,0 +1,117 @@
+! ISO_C_BINDING interfaces to libuegccd (include/uegccd.h).  The derived
+! types are bind(C) mirrors of the C structs, field for field.
+Module Wrappers
+  Use, Intrinsic :: ISO_C_Binding
+  Implicit None
+
+  Integer(c_int), Parameter :: UEGCCD_OK = 0, UEGCCD_ERR_NOT_CONVERGED = -5
+  Integer(c_int32_t), Parameter :: UEGCCD_MODE_REFERENCE = 0, UEGCCD_MODE_DIIS = 1
+  Integer(c_int32_t), Parameter :: UEGCCD_PREC_FP64 = 0, UEGCCD_PREC_MIXED = 1
+
+  Type, Bind(C) :: uegccd_system
+    Integer(c_int32_t) :: nocc, nao
+    Type(c_ptr) :: nvec, eigen
+    Real(c_double) :: L, madelung, rs, gapii
+    Integer(c_int32_t) :: coren, safe_eri
+    Integer(c_int32_t) :: irange(15)
+    Integer(c_int32_t) :: do_ring, do_xring, do_ladder, do_mosaic
+  End Type uegccd_system
+
+  Type, Bind(C) :: uegccd_options
+    Integer(c_int32_t) :: mode, ndiis, diis_start, max_iter
+    Real(c_double) :: tol_dt, tol_e, tol_r, damping
+    Integer(c_int32_t) :: precision
+    Real(c_double) :: mixed_switch
+    Integer(c_int32_t) :: device, iters_per_sync
+  End Type uegccd_options
+
+  Type, Bind(C) :: uegccd_result
+    Real(c_double) :: ecorr, ehf_plus_ec, dt, maxres
+    Integer(c_int32_t) :: niter, failed, iters_fp32, status
+    Real(c_double) :: solve_ms
+  End Type uegccd_result
+
+  Type, Bind(C) :: uegccd_heg_params
+    Integer(c_int32_t) :: nel, maxkpoint
+    Real(c_double) :: rs, twist(3), fcut, corefcut, madelungfactor, gap, exchangefactor
+  End Type uegccd_heg_params
+
+  Interface
+    Subroutine uegccd_default_options(opt) Bind(C, name="uegccd_default_options")
+      Import :: uegccd_options
+      Type(uegccd_options), Intent(Out) :: opt
+    End Subroutine
+
+    Integer(c_int) Function uegccd_heg_build(p, nao, nvec, eigen, scalars) &
+        Bind(C, name="uegccd_heg_build")
+      Import :: uegccd_heg_params, c_int, c_int32_t, c_ptr
+      Type(uegccd_heg_params), Intent(In) :: p
+      Integer(c_int32_t), Intent(Out) :: nao
+      Type(c_ptr), Value :: nvec, eigen, scalars
+    End Function
+
+    Integer(c_int) Function uegccd_ctx_create(sys, opt, ctx) Bind(C, name="uegccd_ctx_create")
+      Import :: uegccd_system, uegccd_options, c_int, c_ptr
+      Type(uegccd_system), Intent(In) :: sys
+      Type(uegccd_options), Intent(In) :: opt
+      Type(c_ptr), Intent(Out) :: ctx
+    End Function
+
+    Subroutine uegccd_ctx_destroy(ctx) Bind(C, name="uegccd_ctx_destroy")
+      Import :: c_ptr
+      Type(c_ptr), Value :: ctx
+    End Subroutine
+
+    Integer(c_int) Function uegccd_mp2(ctx, taa, tab, tba, e2) Bind(C, name="uegccd_mp2")
+      Import :: c_int, c_ptr, c_double
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(Out) :: taa(*), tab(*), tba(*)
+      Real(c_double), Intent(Out) :: e2
+    End Function
+
+    Integer(c_int) Function uegccd_solve(ctx, taa, tab, tba, res) Bind(C, name="uegccd_solve")
+      Import :: c_int, c_ptr, c_double, uegccd_result
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(InOut) :: taa(*), tab(*), tba(*)
+      Type(uegccd_result), Intent(Out) :: res
+    End Function
+
+    Integer(c_int) Function uegccd_trace(ctx, e, dt, n) Bind(C, name="uegccd_trace")
+      Import :: c_int, c_ptr, c_double, c_int32_t
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(Out) :: e(*), dt(*)
+      Integer(c_int32_t), Value :: n
+    End Function
+
+    Type(c_ptr) Function uegccd_strerror_c(st) Bind(C, name="uegccd_strerror")
+      Import :: c_ptr, c_int
+      Integer(c_int), Value :: st
+    End Function
+  End Interface
+
+Contains
+
+  ! The library's message for a status, as a Fortran string.
+  Function uegccd_message(st) Result(msg)
+    Integer(c_int), Intent(In) :: st
+    Character(Len=80) :: msg
+    Character(Kind=c_char), Pointer :: cs(:)
+    Integer :: k
+    msg = ' '
+    Call c_f_pointer(uegccd_strerror_c(st), cs, [80])
+    Do k = 1, 80
+      If (cs(k) == c_null_char) Exit
+      msg(k:k) = cs(k)
+    End Do
+  End Function uegccd_message
+
+  Subroutine Check(st, what)
+    Integer(c_int), Intent(In) :: st
+    Character(Len=*), Intent(In) :: what
+    If (st /= UEGCCD_OK) Then
+      Write(6,'(A,": ",A)') what, Trim(uegccd_message(st))
+      Stop 'uegccd library error'
+    End If
+  End Subroutine Check
+
+End Module Wrappers
