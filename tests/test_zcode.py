@@ -1,0 +1,87 @@
+"""The ZCode Fortran host (ISO_C_BINDING over libuegccd) on reference Inputs:
+its Output file carries the reference's blocks and numbers."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from uegccd_amd.inputfile import standard_input
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ZCODE = os.path.join(ROOT, "uegccd_amd", "host", "ZCode")
+FLOAT = r"[-+]?\d*\.\d+(?:[EeDd][-+]?\d+)?"
+
+
+def run_zcode(tmp_path, text, env=None):
+    if not os.path.exists(ZCODE):
+        pytest.fail("ZCode was not built (uegccd_amd/build.py zcode)")
+    (tmp_path / "Input").write_text(text)
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([ZCODE], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=e)
+    assert "Finishing calculation after single point calculation" in (r.stdout + r.stderr), r.stdout + r.stderr
+    return (tmp_path / "Output").read_text().splitlines()
+
+
+def parse(lines):
+    out = {"eig": [], "trace": []}
+    for ln in lines:
+        if "E(SCF) is" in ln:
+            out["ehf"] = float(re.findall(FLOAT, ln)[0])
+        elif "E(SCF) + E(2)" in ln:
+            pass
+        elif "E(2) is" in ln:
+            out["e2"] = float(re.findall(FLOAT, ln)[0])
+        elif re.match(r"^\s+\*\s+" + FLOAT + r"\s+\*\s*$", ln):
+            out["eig"].append(float(re.findall(FLOAT, ln)[0]))
+        elif re.match(r"^\s+\*\s+" + FLOAT + r"\s+\d+\s+" + FLOAT + r"\s+\*", ln):
+            t = ln.split()
+            out["trace"].append((float(t[1]), int(t[2]), float(t[3])))
+        elif "converged in" in ln:
+            out["niter"] = int(re.findall(r"in\s+(\d+)", ln)[0])
+        elif "Ec(CCD) is" in ln:
+            out["ecorr"] = float(re.findall(FLOAT, ln)[0])
+        elif "Max CCD Residual" in ln:
+            out["maxres"] = float(re.findall(FLOAT, ln)[0])
+    return out
+
+
+def test_zcode_n24_reference_output(tmp_path, fixtures):
+    print("[zcode] n24", flush=True)
+    fx = fixtures["n24"]
+    lines = run_zcode(tmp_path, standard_input(24, 40, 5.0, (0.1, 0.2, 0.45), fcut=2.0))
+    o = parse(lines)
+    assert o["ehf"] == pytest.approx(fx["ehf"], abs=1e-12)
+    assert o["e2"] == pytest.approx(fx["e2"], abs=1e-12)
+    np.testing.assert_allclose(o["eig"], fx["eigen"], rtol=0, atol=1e-15)
+    assert o["niter"] == 70
+    ref = np.array(fx["ccd_trace"])
+    got = np.array(o["trace"])
+    np.testing.assert_array_equal(got[:, 1], ref[:, 1])
+    np.testing.assert_allclose(got[:, 0], ref[:, 0], rtol=0, atol=1.1e-10)
+    np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=0, atol=1.1e-10)
+    assert o["ecorr"] == pytest.approx(fx["ccd_ecorr"], abs=1e-12)
+    assert o["maxres"] == pytest.approx(fx["ccd_maxres"], rel=1e-6)
+
+
+def test_zcode_matches_python_host(tmp_path):
+    """Same Input through ZCode (Fortran) and the Python host: same Output text."""
+    print("[zcode] n14 cut5 vs python host", flush=True)
+    from uegccd_amd.driver import single_point
+    text = standard_input(14, 5, 1.0)
+    zlines = run_zcode(tmp_path, text)
+    ptext, heg, e2, r = single_point(text)
+    assert zlines == ptext.splitlines()
+
+
+def test_zcode_diis_mode(tmp_path):
+    """DIIS mode through the environment knob converges to the same energy."""
+    import json
+    cfg = json.load(open(os.path.join(ROOT, "tests", "golden", "config_energies.json")))
+    lines = run_zcode(tmp_path, standard_input(14, 5, 1.0), env={"UEGCCD_MODE": "diis"})
+    o = parse(lines)
+    assert o["ecorr"] == pytest.approx(cfg["n14_cut5"]["ecorr_tight"], abs=1e-9)
+    assert o["niter"] < cfg["n14_cut5"]["niter_reference_mode"]

@@ -25,14 +25,15 @@ ZCODE = os.path.join(HOST, "ZCode")
 HIP_SOURCES = ["kernels.hip", "api.hip", "server.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
-    # packed FP32 instructions stay off (hardware notes)
+    # no packed-FP32 (v_pk_*_f32) instructions: some of their forms have given
+    # wrong results on these MI355X boxes
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
 # Fortran host: the reference's file set (Main, IO, HEG-facing types, CCD,
 # MP2, DIISCC, Wrappers) compiled in dependency order.
-FORTRAN_SOURCES = ["Precision.f90", "Types.f90", "Wrappers.f90", "IO.f90", "MP2.f90",
-                   "DIISCC.f90", "CCD.f90", "Main.f90"]
+FORTRAN_SOURCES = ["Precision.f90", "Types.f90", "Wrappers.f90", "IO.f90", "HEG.f90",
+                   "DIISCC.f90", "CCD.f90", "MP2.f90", "Main.f90"]
 
 
 def _run(cmd, cwd=None):

@@ -1,0 +1,53 @@
+! Plane-wave basis of the single-point calculation, built by libuegccd
+! (uegccd_heg_build: init_HEG + SetupBasis + change_rs of source/HEG.f90).
+Module HEG
+  Use, Intrinsic :: ISO_C_Binding
+  Use Precision
+  Use Types, Only: UEGInfoType, HEGDataType
+  Use Wrappers
+  Implicit None
+  Private
+  Public :: InitHEG
+
+Contains
+
+  Subroutine InitHEG(UEGInfo, HEGData)
+    Type (UEGInfoType), Intent(InOut) :: UEGInfo
+    Type (HEGDataType), Intent(InOut) :: HEGData
+    Type (uegccd_heg_params) :: P
+    Integer(c_int32_t) :: NAO
+    Integer(c_int32_t), Allocatable, Target :: NVecC(:)
+    Real(c_double), Allocatable, Target :: Eig(:)
+    Real(c_double), Target :: Scal(8)
+
+    P%nel = UEGInfo%NElectron
+    P%maxkpoint = UEGInfo%MaxKPoint
+    P%rs = UEGInfo%rSMin
+    P%twist = UEGInfo%NTwist
+    P%fcut = UEGInfo%fcut
+    P%corefcut = UEGInfo%corefcut
+    P%madelungfactor = UEGInfo%MadelungFactor
+    P%gap = UEGInfo%Gap
+    P%exchangefactor = UEGInfo%ExchangeFactor
+    Call Check(uegccd_heg_build(P, NAO, c_null_ptr, c_null_ptr, c_null_ptr), 'uegccd_heg_build')
+    Allocate(NVecC(3*NAO), Eig(NAO))
+    Call Check(uegccd_heg_build(P, NAO, c_loc(NVecC), c_loc(Eig), c_loc(Scal)), 'uegccd_heg_build')
+    UEGInfo%NAO = NAO
+    HEGData%Nel = UEGInfo%NElectron
+    HEGData%rS = UEGInfo%rSMin
+    HEGData%L = Scal(1)
+    HEGData%Omega = Scal(2)
+    HEGData%Madelung = Scal(3)
+    HEGData%EHF = Scal(4)
+    HEGData%ScreeningDistance = Scal(5)
+    HEGData%DenomFactor = Scal(6)
+    HEGData%TolMax = Scal(7)
+    HEGData%FailRatio = Scal(8)
+    If (Allocated(HEGData%nVec)) Deallocate(HEGData%nVec)
+    If (Allocated(HEGData%Eigen)) Deallocate(HEGData%Eigen)
+    Allocate(HEGData%nVec(NAO,3), HEGData%Eigen(NAO))
+    HEGData%nVec = Reshape(NVecC, [Int(NAO), 3])
+    HEGData%Eigen = Eig
+  End Subroutine InitHEG
+
+End Module HEG

@@ -1,0 +1,36 @@
+! ZCode: the single-point driver of the reference (Program TermedCC,
+! source/Main.f90:2-97) on the MI355X solver: read Input, build the basis,
+! MP2 guess, CCD.  Twist averaging (DoSkipTA F) and dRPA (DodRPASOSEX T) are
+! not part of this host and stop with a message.
+Program TermedCC
+  Use Precision
+  Use Types, Only: UEGInfoType, HEGDataType
+  Use IO, Only: ReadInput
+  Use HEG, Only: InitHEG
+  Use MP2, Only: DrvMBPT
+  Use CCD, Only: DrvCCD, ShutDownSolver
+  Implicit None
+  Type (UEGInfoType) :: UEGInfo
+  Type (HEGDataType) :: HEGData
+  Real (Kind=pr), Allocatable :: T2aaaa(:,:,:), T2abab(:,:,:), T2abba(:,:,:)
+  Integer :: IAlloc(3)
+
+  Call ReadInput(UEGInfo)
+  UEGInfo%NOcc = UEGInfo%NElectron/2
+  Call InitHEG(UEGInfo, HEGData)
+  Allocate(T2aaaa(UEGInfo%NOcc,UEGInfo%NOcc,UEGInfo%NOcc+1:UEGInfo%NAO), Stat=IAlloc(1))
+  Allocate(T2abab(UEGInfo%NOcc,UEGInfo%NOcc,UEGInfo%NOcc+1:UEGInfo%NAO), Stat=IAlloc(2))
+  Allocate(T2abba(UEGInfo%NOcc,UEGInfo%NOcc,UEGInfo%NOcc+1:UEGInfo%NAO), Stat=IAlloc(3))
+  If (Any(IAlloc /= 0)) Stop 'Could not allocate in main'
+  Open(7, File='Output', Status='Replace')
+  Close(7)
+
+  If (UEGInfo%DoSingleCalc) Then
+    If (UEGInfo%DodRPASOSEX) Stop 'dRPA/SOSEX is not part of this build'
+    Call DrvMBPT(HEGData, UEGInfo, T2aaaa, T2abab, T2abba)
+    Call DrvCCD(HEGData, UEGInfo, T2aaaa, T2abab, T2abba)
+  End If
+  Call ShutDownSolver()
+  If (UEGInfo%DoSkipTA) Stop 'Finishing calculation after single point calculation'
+  Stop 'Twist averaging is not part of this build (set DoSkipTA: T)'
+End Program TermedCC
