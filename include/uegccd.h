@@ -149,8 +149,8 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
 
 /* Average device time (microseconds, HIP events on the context's stream) of
  * one kernel of the residual pass over the context's current amplitudes:
- * 0 mosaic intermediates, 1 ring intermediates, 2 ladder GEMMs, 3 residual
- * assembly + update, 4 extrapolation.  No counterpart in the reference;
+ * 0 per-slice ring/mosaic products, 2 ladder GEMMs, 3 residual assembly +
+ * update, 4 extrapolation (1 is unused).  No counterpart in the reference;
  * measurement support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
 

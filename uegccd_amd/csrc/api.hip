@@ -201,7 +201,8 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       n2max = std::max(n2max, (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) +
                                   (a.z - b.z) * (a.z - b.z));
     }
-  if (n2max >= kMaxN2 || nao > kMaxNao) return UEGCCD_ERR_UNSUPPORTED;
+  if (n2max >= kMaxN2 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
+  if (joo_lds_bytes(o, v) > kMaxDynLds) return UEGCCD_ERR_UNSUPPORTED;
   c->n2max = n2max;
   const EriKind kind = eri_kind(s);
   const bool gap_applies = (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_GAP);
@@ -233,18 +234,18 @@ int upload_batch_of_one(uegccd_ctx* c) {
   BatchDev& b = c->batch;
   b.n = 1;
   b.solves = c->d_sd;
-  const int eb = (c->n3 + kElemThreads - 1) / kElemThreads;
-  const int mb = (c->sd.do_mosaic ? c->v + c->o : 0);
-  const int rb = ((c->sd.do_ring || c->sd.do_xring) ? (2 * c->o * c->v * c->o + kElemThreads - 1) / kElemThreads : 0);
-  int h_pref[6] = {0, eb, 0, mb, 0, rb};
-  int* d_pref = c->arena.take<int>(6);
+  const int eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
+  const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
+  const int sb = need_slice ? c->v : 0;
+  int h_pref[4] = {0, eb, 0, sb};
+  int* d_pref = c->arena.take<int>(4);
   HIPCHK(hipMemcpy(d_pref, h_pref, sizeof(h_pref), hipMemcpyHostToDevice));
   b.elem_prefix = d_pref;
-  b.mos_prefix = d_pref + 2;
-  b.ring_prefix = d_pref + 4;
+  b.slice_prefix = d_pref + 2;
   b.elem_blocks = eb;
-  b.mos_blocks = mb;
-  b.ring_blocks = rb;
+  b.slice_blocks = sb;
+  b.slice_lds = slice_lds_bytes(c->o);
+  b.joo_lds = joo_lds_bytes(c->o, c->v);
   b.nlad = c->sd.do_ladder ? (int)c->lad_items.size() : 0;
   b.lad_items = c->arena.take<int4>(std::max<size_t>(1, c->lad_items.size()));
   if (!c->lad_items.empty())
@@ -261,12 +262,14 @@ int upload(uegccd_ctx* c, const std::vector<T>& h, const T** dst) {
   return UEGCCD_OK;
 }
 
+// Host (i,j,a) arrays of each spin -> interleaved device amplitudes; entries
+// with no virtual momentum partner are zeroed.
 int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
   std::vector<double> h(2 * (size_t)c->n3, 0.0);
   for (int x = 0; x < c->n3; ++x)
     if (c->pidx[x] >= 0) {
-      h[x] = tab ? tab[x] : 0.0;
-      h[c->n3 + x] = tba ? tba[x] : 0.0;
+      h[2 * (size_t)x] = tab ? tab[x] : 0.0;
+      h[2 * (size_t)x + 1] = tba ? tba[x] : 0.0;
     }
   HIPCHK(hipMemcpyAsync(c->sd.tcur, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                         c->stream));
@@ -345,7 +348,7 @@ int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, ue
 
   const int ndiis = opt.mode == UEGCCD_MODE_DIIS ? opt.ndiis : 1;
   const size_t n3 = c->n3;
-  const int eb = (c->n3 + kElemThreads - 1) / kElemThreads;
+  const int eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
   const size_t ovo = (size_t)c->o * c->v * c->o;
   size_t bytes = 0;
   bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
@@ -356,9 +359,11 @@ int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, ue
   bytes += 2 * aligned(sizeof(int) * c->o * c->o);
   bytes += aligned(sizeof(double) * 2 * n3) * (1 + 2 * ndiis + 3);
   bytes += aligned(sizeof(double) * 2 * std::max(1, c->wtot));
-  bytes += aligned(sizeof(double) * 6 * std::max<size_t>(1, ovo));
-  bytes += aligned(sizeof(double) * (c->v + c->o));
-  bytes += aligned(sizeof(double) * (size_t)eb * kPartialStride);
+  bytes += 2 * aligned(sizeof(double4) * std::max<size_t>(1, ovo));
+  bytes += aligned(sizeof(double) * 4 * n3) + aligned(sizeof(double) * 2 * c->o * c->v);
+  bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
+  bytes += aligned(sizeof(double) * c->o) + aligned(sizeof(unsigned));
+  bytes += aligned(sizeof(double) * (size_t)eb * kPartialStride /* slot-major: [q][block] */);
   bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
   bytes += aligned(sizeof(int) * 6) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
@@ -397,18 +402,24 @@ int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, ue
     uegccd_ctx_destroy(c);
     return rc;
   }
-  S.tcur = c->arena.take<double>(2 * n3);
-  S.histT = c->arena.take<double>(2 * n3 * ndiis);
-  S.histE = c->arena.take<double>(2 * n3 * ndiis);
+  S.tcur = c->arena.take<double2>(n3);
+  S.histT = c->arena.take<double2>(n3 * ndiis);
+  S.histE = c->arena.take<double2>(n3 * ndiis);
   S.gout = c->arena.take<double>(2 * n3);
   S.rout = c->arena.take<double>(2 * n3);
-  S.lad = c->arena.take<double>(2 * n3);
+  S.lad = c->arena.take<double2>(n3);
   S.wtot = c->wtot;
-  S.w = c->arena.take<double>(2 * std::max(1, c->wtot));
-  S.jr = c->arena.take<double>(6 * std::max<size_t>(1, ovo));
-  S.jvo = c->arena.take<double>(c->v + c->o);
+  S.w = c->arena.take<double2>(std::max(1, c->wtot));
+  S.jring = c->arena.take<double4>(std::max<size_t>(1, ovo));
+  S.jxring = c->arena.take<double4>(std::max<size_t>(1, ovo));
+  S.tv = c->arena.take<double4>(n3);
+  S.rsum = c->arena.take<double>((size_t)c->o * c->v);
+  S.csum = c->arena.take<double>((size_t)c->o * c->v);
+  S.jvv = c->arena.take<double>(c->v);
+  S.joo_part = c->arena.take<double>((size_t)c->v * c->o);
+  S.joo = c->arena.take<double>(c->o);
   S.nblk_elem = eb;
-  S.partials = c->arena.take<double>((size_t)eb * kPartialStride);
+  S.partials = c->arena.take<double>((size_t)eb * kPartialStride /* slot-major: [q][block] */);
   c->d_state = c->arena.take<State>(1);
   S.st = c->d_state;
   S.trace = c->arena.take<double>(2 * (size_t)c->maxiter);
@@ -501,9 +512,9 @@ int uegccd_mp2(uegccd_ctx* c, double* taa, double* tab, double* tba, double* e2)
   int rc = read_state(c, &st);
   if (rc) return rc;
   for (size_t x = 0; x < n3; ++x) {
-    if (tab) tab[x] = h[x];
-    if (tba) tba[x] = h[n3 + x];
-    if (taa) taa[x] = h[x] + h[n3 + x];
+    if (tab) tab[x] = h[2 * x];
+    if (tba) tba[x] = h[2 * x + 1];
+    if (taa) taa[x] = h[2 * x] + h[2 * x + 1];
   }
   if (e2) *e2 = st.ecorr;
   return UEGCCD_OK;
@@ -549,19 +560,20 @@ struct BatchScratch {
 
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
   std::vector<SolveDev> sds(n);
-  std::vector<int> ep(n + 1, 0), mp(n + 1, 0), rp(n + 1, 0);
+  std::vector<int> ep(n + 1, 0), sp(n + 1, 0);
   std::vector<int4> items;
+  int omax = 1;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
     sds[k] = c->sd;
-    ep[k + 1] = ep[k] + (c->n3 + kElemThreads - 1) / kElemThreads;
-    mp[k + 1] = mp[k] + (c->sd.do_mosaic ? c->v + c->o : 0);
-    rp[k + 1] = rp[k] + ((c->sd.do_ring || c->sd.do_xring)
-                             ? (2 * c->o * c->v * c->o + kElemThreads - 1) / kElemThreads : 0);
+    ep[k + 1] = ep[k] + (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
+    const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
+    sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
+    omax = std::max(omax, c->o);
     if (c->sd.do_ladder)
       for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
   }
-  size_t bytes = aligned(sizeof(SolveDev) * n) + 3 * aligned(sizeof(int) * (n + 1)) +
+  size_t bytes = aligned(sizeof(SolveDev) * n) + 2 * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(int4) * std::max<size_t>(1, items.size()));
   if (hipMalloc(&out->base, bytes) != hipSuccess) return UEGCCD_ERR_ALLOC;
   DeviceArena a;
@@ -571,17 +583,18 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
   b.n = n;
   b.solves = a.take<SolveDev>(n);
   b.elem_prefix = a.take<int>(n + 1);
-  b.mos_prefix = a.take<int>(n + 1);
-  b.ring_prefix = a.take<int>(n + 1);
+  b.slice_prefix = a.take<int>(n + 1);
   b.lad_items = a.take<int4>(std::max<size_t>(1, items.size()));
   b.nlad = (int)items.size();
   b.elem_blocks = ep[n];
-  b.mos_blocks = mp[n];
-  b.ring_blocks = rp[n];
+  b.slice_blocks = sp[n];
+  b.slice_lds = slice_lds_bytes(omax);
+  int jl = 0;
+  for (int k = 0; k < n; ++k) jl = std::max(jl, joo_lds_bytes(cs[k]->o, cs[k]->v));
+  b.joo_lds = jl;
   HIPCHK(hipMemcpy(b.solves, sds.data(), sizeof(SolveDev) * n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b.elem_prefix, ep.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b.mos_prefix, mp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b.ring_prefix, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.slice_prefix, sp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   if (!items.empty())
     HIPCHK(hipMemcpy(b.lad_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
   return UEGCCD_OK;
@@ -656,7 +669,7 @@ int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
   HIPCHK(hipStreamSynchronize(c->stream));
   for (size_t x = 0; x < n3; ++x) {
     const bool ok = c->pidx[x] >= 0;
-    const double a = ok ? h[x] : 0.0, b = ok ? h[n3 + x] : 0.0;
+    const double a = ok ? h[2 * x] : 0.0, b = ok ? h[2 * x + 1] : 0.0;
     if (tab) tab[x] = a;
     if (tba) tba[x] = b;
     if (taa) taa[x] = a + b;

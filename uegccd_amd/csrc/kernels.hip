@@ -1,18 +1,25 @@
 // CDNA4 (gfx950) kernels of the CCD amplitude iteration.
 //
 // One residual pass (TermsWrapper, source/CCD.f90:146-181) plus the update
-// (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is five launches:
-//   k_mosaic_j   Jvv(c), Joo(k) of GetG2Mosaics (source/CCD.f90:548-567)
-//   k_ring_j     quadratic ring / crossed-ring intermediates
-//                (source/CCD.f90:771-797, 899-925)
+// (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is six launches:
+//   k_slice      per virtual slice: linear ring / crossed-ring products,
+//                quadratic ring intermediates, mosaic partial sums
+//                (source/CCD.f90:507-585, 689-940)
+//   k_joo        mosaic Joo(k), the reduction of the per-slice partials
 //   k_ladder     both ladders as pair-momentum-blocked GEMMs on MFMA
 //                (source/CCD.f90:611-681)
-//   k_main       drivers + mosaics + rings + crossed rings + ladder contraction,
-//                then R = D*T - G, the damped update, dT, energy, DIIS dots
-//   k_finalize   deterministic reductions, convergence, DIIS solve
-//   k_extrap     next amplitudes (DIIS extrapolation or the plain update)
+//   k_main       residual assembly, R = D*T - G, damped update, dT, energy,
+//                DIIS dot products
+//   k_finalize   deterministic reductions, convergence test, DIIS solve
+//   k_extrap     amplitudes of the next pass (DIIS extrapolation or update)
 // Every kernel resolves its solve from a block-prefix table, so one launch
 // covers any number of independent solves (twists, r_s values, bases).
+//
+// Interleaved layouts keep each gathered access one 16/32/64-byte load:
+//   tcur, lad, histT/histE entries   double2  (abab, abba) or (S, P) per (i,j,a)
+//   tv                               double4  (TV_ab, TV_ba, VT_ab, VT_ba)
+//   jring / jxring                   double4  (J1, J2, J3, 0) per (s, i, l)
+//   histT / histE                    element-major: slot k of x at x*ndiis + k
 #include "uegccd_internal.h"
 #include "../../include/uegccd.h"
 
@@ -30,12 +37,24 @@ __device__ __forceinline__ int find_solve(const int* __restrict__ prefix, int n,
   return lo;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Element kernels renumber their blocks so that XCD x owns one contiguous
+// range of amplitudes (a bijection on [0, nblocks)); the per-element streams
+// (amplitudes, histories, slice products of the element's own slice) are then
+// fetched into one L2 instead of all eight.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+  constexpr int kXcd = 8;
+  const int per = nblocks / kXcd, rem = nblocks % kXcd;
+  const int x = bid % kXcd, idx = bid / kXcd;
+  return x * per + min(x, rem) + idx;
+}
+
 __device__ __forceinline__ int d2(int4 p, int4 q) {
   int a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
   return a * a + b * b + c * c;
 }
 
-// Block-wide reductions (256 threads = 4 wave64s), fixed order.
+// Block-wide reductions (wave64 shuffles, then LDS), fixed order.
 __device__ __forceinline__ double wave_sum(double x) {
   for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
   return x;
@@ -67,7 +86,36 @@ __device__ double block_max(double x, double* red) {
   return r;
 }
 
-// Coulomb table and n-vectors staged in LDS for the element kernels.
+// Reduces NQ values per thread over the block in one pass: q < NMAX are
+// maxima, the rest sums.  Wave shuffles, then one LDS exchange; the result is
+// valid in thread 0 only.  red must hold (blockDim/64)*NQ doubles.
+template <int NQ, int NMAX>
+__device__ __forceinline__ void block_reduce_multi(double (&v)[NQ], double* red) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double o = __shfl_xor(v[q], m, 64);
+      v[q] = (q < NMAX) ? fmax(v[q], o) : v[q] + o;
+    }
+  }
+  __syncthreads();
+  if (l == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) red[w * NQ + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < nw; ++k) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        v[q] = (q < NMAX) ? fmax(v[q], red[k * NQ + q]) : v[q] + red[k * NQ + q];
+    }
+  }
+}
+
+// Coulomb table and n-vectors staged in LDS.
 struct Lds {
   int4 nv[kMaxNao];
   double vt[kMaxN2];
@@ -82,14 +130,20 @@ __device__ __forceinline__ void stage(const SolveDev& S, Lds& L) {
 
 #define VPQ(p, q) (L.vt[d2(L.nv[p], L.nv[q])])
 
+// Slot-major block partials: quantity q of element block lb.
+__device__ __forceinline__ double* partial(const SolveDev& S, int q, int lb) {
+  return S.partials + (size_t)q * S.nblk_elem + lb;
+}
+
 // --------------------------------------------------------------------------
 // Initial state: E(T0) and max|T0| (source/CCD.f90:62-64).
-__global__ __launch_bounds__(kElemThreads) void k_init(const SolveDev* __restrict__ solves,
-                                                        const int* __restrict__ prefix, int n) {
+__global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restrict__ solves,
+                                                         const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
-  const int s = find_solve(prefix, n, blockIdx.x);
+  const int gb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
-  const int lb = blockIdx.x - prefix[s];
+  const int lb = gb - prefix[s];
   stage(S, L);
   const int o = S.o, o2 = o * o;
   const int x = lb * blockDim.x + threadIdx.x;
@@ -98,26 +152,32 @@ __global__ __launch_bounds__(kElemThreads) void k_init(const SolveDev* __restric
     const int b = S.pidx[x];
     if (b >= 0) {
       const int i = x % o, j = (x / o) % o, a = o + x / o2;
-      const double tab = S.tcur[x], tba = S.tcur[S.n3 + x], taa = tab + tba;
-      e = 0.5 * (VPQ(a, i) * (taa + tab) - VPQ(a, j) * (taa + tba));
-      tmax = fmax(fabs(tab), fabs(tba));
-      if (!isfinite(tab) || !isfinite(tba)) bad = 1.0;
+      const double2 t = S.tcur[x];
+      const double taa = t.x + t.y;
+      e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
+      tmax = fmax(fabs(t.x), fabs(t.y));
+      if (!isfinite(t.x) || !isfinite(t.y)) bad = 1.0;
     }
   }
-  double* P = S.partials + (size_t)lb * kPartialStride;
   e = block_sum(e, L.red);
   tmax = block_max(tmax, L.red);
   bad = block_sum(bad, L.red);
-  if (threadIdx.x == 0) { P[0] = e; P[1] = tmax; P[2] = 0.0; P[3] = bad; }
+  if (threadIdx.x == 0) {
+    *partial(S, 0, lb) = e;
+    *partial(S, 1, lb) = tmax;
+    *partial(S, 2, lb) = 0.0;
+    *partial(S, 3, lb) = bad;
+  }
 }
 
 // DrvMBPT amplitudes and E(2) (source/MP2.f90:56-74); writes tcur.
-__global__ __launch_bounds__(kElemThreads) void k_mp2(const SolveDev* __restrict__ solves,
-                                                       const int* __restrict__ prefix, int n) {
+__global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restrict__ solves,
+                                                        const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
-  const int s = find_solve(prefix, n, blockIdx.x);
+  const int gb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
-  const int lb = blockIdx.x - prefix[s];
+  const int lb = gb - prefix[s];
   stage(S, L);
   const int o = S.o, o2 = o * o;
   const int x = lb * blockDim.x + threadIdx.x;
@@ -134,107 +194,142 @@ __global__ __launch_bounds__(kElemThreads) void k_mp2(const SolveDev* __restrict
       const double taa = tab + tba;
       e = 0.5 * taa * (vab - vba) + 0.5 * tab * vab - 0.5 * tba * vba;
     }
-    S.tcur[x] = tab;
-    S.tcur[S.n3 + x] = tba;
+    S.tcur[x] = make_double2(tab, tba);
   }
   e = block_sum(e, L.red);
   if (threadIdx.x == 0) {
-    double* P = S.partials + (size_t)lb * kPartialStride;
-    P[0] = e; P[1] = 0.0; P[2] = 0.0; P[3] = 0.0;
+    *partial(S, 0, lb) = e;
+    *partial(S, 1, lb) = 0.0;
+    *partial(S, 2, lb) = 0.0;
+    *partial(S, 3, lb) = 0.0;
   }
 }
 
 // --------------------------------------------------------------------------
-// Mosaic intermediates (source/CCD.f90:548-567): blocks [0,v) give Jvv(c),
-// blocks [v, v+o) give Joo(k).
-__global__ __launch_bounds__(kElemThreads) void k_mosaic_j(const SolveDev* __restrict__ solves,
-                                                            const int* __restrict__ prefix, int n) {
+// Per-virtual-slice work, one workgroup per slice s (LDS-staged).
+//
+// With T(:,:,s) of both spins and V(k,j) = v(n_k - n_j) in LDS, and using that
+// invalid amplitudes are exactly zero (so the reference's FindIndex skips
+// only skip zero terms), every linear ring / crossed-ring term of
+// source/CCD.f90:716-763, 845-892 is a product of a slice with V:
+//   TV[s](i,j) = sum_k T(i,k,s) V(k,j)      VT[s](j,i) = sum_k V(j,k) T(k,i,s)
+//   rsum[s](i) = sum_k (T^aa + T^ab)(i,k,s) csum[s](i) = sum_k (T^aa + T^ba)(k,i,s)
+// The quadratic intermediates (source/CCD.f90:771-797, 899-925) share one
+// Coulomb factor Y(k,l) = v(n_i + n_k - n_s - n_l) between rings (a = s) and
+// crossed rings (b = s), with q = n_i - n_s:
+//   J1 = v(q) rsum(i) - sum_k T^aa(i,k,s) Y    J1x = v(q) csum(i) - sum_k T^aa(k,i,s) Y
+//   J2 = v(q) rsum(i) - sum_k T^ab(i,k,s) Y    J2x = v(q) csum(i) - sum_k T^ba(k,i,s) Y
+//   J3 =              - sum_k T^ba(i,k,s) Y    J3x =                sum_k T^ab(k,i,s) Y
+// (the momentum test on d is applied where J is contracted).  The mosaic
+// sums of source/CCD.f90:548-567 are per-slice reductions as well:
+//   Jvv(s) = sum_k P_s(k),  Joo(k) = sum_s P_s(k)  (k_joo),
+//   P_s(k) = sum_l v(n_s-n_k)(T^aa+T^ab)(k,l,s) - v(n_s-n_l)(T^aa+T^ba)(k,l,s).
+__global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restrict__ solves,
+                                                         const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
-  const int s = find_solve(prefix, n, blockIdx.x);
-  const SolveDev& S = solves[s];
-  if (S.st->phase == PH_DONE || !S.do_mosaic) return;
-  const int lb = blockIdx.x - prefix[s];
-  stage(S, L);
-  const int o = S.o, o2 = o * o, v = S.v;
-  const double* tab = S.tcur;
-  const double* tba = S.tcur + S.n3;
-  double acc = 0.0;
-  if (lb < v) {
-    const int cv = lb, c = o + cv;
-    for (int r = threadIdx.x; r < o2; r += blockDim.x) {
-      const int x = r + o2 * cv;
-      if (S.pidx[x] < 0) continue;
-      const int k = r % o, l = r / o;
-      const double ta = tab[x], tb = tba[x], taa = ta + tb;
-      acc += VPQ(c, k) * (taa + ta) - VPQ(c, l) * (taa + tb);
-    }
-  } else {
-    const int k = lb - v;
-    for (int r = threadIdx.x; r < o * v; r += blockDim.x) {
-      const int l = r % o, cv = r / o, c = o + cv;
-      const int x = k + o * l + o2 * cv;
-      if (S.pidx[x] < 0) continue;
-      const double ta = tab[x], tb = tba[x], taa = ta + tb;
-      acc += VPQ(c, k) * (taa + ta) - VPQ(c, l) * (taa + tb);
-    }
-  }
-  acc = block_sum(acc, L.red);
-  if (threadIdx.x == 0) S.jvo[lb] = acc;
-}
-
-// Quadratic ring (J1,J2,J3 of source/CCD.f90:771-797) and crossed-ring
-// (source/CCD.f90:899-925) intermediates, one thread per (i, a|b, l).
-// Layout: jr[m*ovo + (av*o + i)*o + l], m = 0..2 rings, 3..5 crossed rings.
-__global__ __launch_bounds__(kElemThreads) void k_ring_j(const SolveDev* __restrict__ solves,
-                                                          const int* __restrict__ prefix, int n) {
-  __shared__ Lds L;
+  extern __shared__ double dyn[];
   const int s = find_solve(prefix, n, blockIdx.x);
   const SolveDev& S = solves[s];
   if (S.st->phase == PH_DONE) return;
-  const int lb = blockIdx.x - prefix[s];
+  const int sv = blockIdx.x - prefix[s];
   stage(S, L);
-  const int o = S.o, o2 = o * o, v = S.v;
-  const int ovo = o * v * o;
-  const int t = lb * blockDim.x + threadIdx.x;
-  if (t >= 2 * ovo) return;
-  const int half = t / ovo, r = t - half * ovo;
-  if (half == 0 && !S.do_ring) return;
-  if (half == 1 && !S.do_xring) return;
-  const int l = r % o, i = (r / o) % o, av = r / o2;
-  const double* tab = S.tcur;
-  const double* tba = S.tcur + S.n3;
-  double j1 = 0.0, j2 = 0.0, j3 = 0.0;
-  if (half == 0) {
-    for (int k = 0; k < o; ++k) {
-      const int y = i + o * k + o2 * av;
-      const int c = S.pidx[y];
-      if (c < 0) continue;
-      const int d = S.pidx[k + o * l + o2 * (c - o)];
-      if (d < 0) continue;
-      const double vkl = VPQ(c, k), vlk = VPQ(c, l);
-      const double ta = tab[y], tb = tba[y], taa = ta + tb;
-      j1 += taa * (vkl - vlk) + ta * vkl;
-      j2 += ta * (vkl - vlk) + taa * vkl;
-      j3 -= tb * vlk;
-    }
-  } else {
-    for (int k = 0; k < o; ++k) {
-      const int c = S.pidx[i + o * k + o2 * av];
-      if (c < 0) continue;
-      const int d = S.pidx[k + o * l + o2 * (c - o)];
-      if (d < 0) continue;
-      const double vkl = VPQ(c, k), vlk = VPQ(c, l);
-      const int y = k + o * i + o2 * av;
-      const double ta = tab[y], tb = tba[y], taa = ta + tb;
-      j1 += taa * (vkl - vlk) + tb * vkl;
-      j2 += tb * (vkl - vlk) + taa * vkl;
-      j3 += ta * vlk;
-    }
+  const int o = S.o, o2 = o * o, so = o + sv;
+  double* Tab = dyn;
+  double* Tba = dyn + o2;
+  double* Vo = dyn + 2 * o2;
+  double* rs = dyn + 3 * o2;
+  double* cs = rs + o;
+  double* jp = cs + o;
+  const double2* tslab = S.tcur + (size_t)o2 * sv;
+  for (int idx = threadIdx.x; idx < o2; idx += blockDim.x) {
+    const double2 t = tslab[idx];
+    Tab[idx] = t.x;
+    Tba[idx] = t.y;
+    Vo[idx] = VPQ(idx % o, idx / o);
   }
-  double* J = S.jr + (size_t)(3 * half) * ovo + r;
-  J[0] = j1;
-  J[ovo] = j2;
-  J[2 * ovo] = j3;
+  __syncthreads();
+  for (int i = threadIdx.x; i < o; i += blockDim.x) {
+    double r = 0.0, c = 0.0, p = 0.0;
+    const double vsi = VPQ(so, i);
+    for (int k = 0; k < o; ++k) {
+      const double ta = Tab[i + o * k], tb = Tba[i + o * k];
+      r += 2.0 * ta + tb;
+      c += Tab[k + o * i] + 2.0 * Tba[k + o * i];
+      p += vsi * (2.0 * ta + tb) - VPQ(so, k) * (ta + 2.0 * tb);
+    }
+    rs[i] = r;
+    cs[i] = c;
+    jp[i] = p;
+    S.rsum[sv * o + i] = r;
+    S.csum[sv * o + i] = c;
+    S.joo_part[sv * o + i] = p;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double jv = 0.0;
+    for (int k = 0; k < o; ++k) jv += jp[k];
+    S.jvv[sv] = jv;
+  }
+  const int4 ns = L.nv[so];
+  for (int idx = threadIdx.x; idx < o2; idx += blockDim.x) {
+    const int r = idx % o, c = idx / o;
+    double tva = 0.0, tvb = 0.0, vta = 0.0, vtb = 0.0;
+    for (int k = 0; k < o; ++k) {
+      const double vkc = Vo[k + o * c], vrk = Vo[r + o * k];
+      tva += Tab[r + o * k] * vkc;
+      tvb += Tba[r + o * k] * vkc;
+      vta += vrk * Tab[k + o * c];
+      vtb += vrk * Tba[k + o * c];
+    }
+    S.tv[(size_t)o2 * sv + idx] = make_double4(tva, tvb, vta, vtb);
+    // quadratic intermediates for i = r, l = c
+    const int i = r, l = c;
+    const int4 ni = L.nv[i], nl = L.nv[l];
+    const int bx = ni.x - ns.x - nl.x, by = ni.y - ns.y - nl.y, bz = ni.z - ns.z - nl.z;
+    double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0;
+    for (int k = 0; k < o; ++k) {
+      const int4 nk = L.nv[k];
+      const int dx = bx + nk.x, dy = by + nk.y, dz = bz + nk.z;
+      const double y = L.vt[dx * dx + dy * dy + dz * dz];
+      const double ta = Tab[i + o * k], tb = Tba[i + o * k];
+      const double ua = Tab[k + o * i], ub = Tba[k + o * i];
+      a1 += (ta + tb) * y;
+      a2 += ta * y;
+      a3 += tb * y;
+      a4 += (ua + ub) * y;
+      a5 += ub * y;
+      a6 += ua * y;
+    }
+    const double vq = VPQ(i, so);
+    const size_t jx = (size_t)(sv * o + i) * o + l;
+    S.jring[jx] = make_double4(vq * rs[i] - a1, vq * rs[i] - a2, -a3, 0.0);
+    S.jxring[jx] = make_double4(vq * cs[i] - a4, vq * cs[i] - a5, a6, 0.0);
+  }
+}
+
+// Mosaic Joo(k) = sum over slices of k_slice's partials, one block per solve:
+// the v*o partials are read coalesced into LDS, then summed per k in a fixed
+// order.
+__global__ __launch_bounds__(kElemThreads) void k_joo(const SolveDev* __restrict__ solves) {
+  extern __shared__ double part[];
+  const SolveDev& S = solves[blockIdx.x];
+  if (S.st->phase == PH_DONE || !S.do_mosaic) return;
+  const int o = S.o, nvo = S.v * o;
+  for (int t = threadIdx.x; t < nvo; t += blockDim.x) part[t] = S.joo_part[t];
+  __syncthreads();
+  const int nc = blockDim.x / o;
+  const int t = threadIdx.x, k = t % o, c = t / o;
+  double acc = 0.0;
+  if (c < nc)
+    for (int r = c; r < S.v; r += nc) acc += part[r * o + k];
+  __syncthreads();
+  if (c < nc) part[c * o + k] = acc;
+  __syncthreads();
+  if (t < o) {
+    double sum = 0.0;
+    for (int cc = 0; cc < nc; ++cc) sum += part[cc * o + t];
+    S.joo[t] = sum;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -283,12 +378,9 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int ncol = nvv + noo;
   const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const Real* vt = nullptr;
   const double* vt64 = S.vtab;
   const float* vt32 = S.vtab32;
-  (void)vt;
-  const double* tab = S.tcur;
-  const double* tba = S.tcur + S.n3;
+  const double2* T = S.tcur;
 
   const int rowA = row0 + r16;
   const int xpA = rowA < noo ? S.kb_pairs[poff + rowA] : -1;
@@ -314,10 +406,9 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
       const int c = S.kb_va[voff + ci];
       const int4 nc = S.nv4[c];
       if (xpA >= 0) {
-        const size_t y = (size_t)xpA + (size_t)o2 * (c - o);
-        const double ta = tab[y], tb = tba[y];
-        sv = (Real)(ta + tb);
-        pv = (Real)(ta - tb);
+        const double2 t = T[(size_t)xpA + (size_t)o2 * (c - o)];
+        sv = (Real)(t.x + t.y);
+        pv = (Real)(t.x - t.y);
       }
       if (colv) {
         Real vx, vy;
@@ -342,16 +433,12 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
     const int xpo = S.kb_pairs[poff + rowo];
     const double as = (double)accs[r], ap = (double)accp[r];
     if (col < nvv) {
-      const size_t y = (size_t)xpo + (size_t)o2 * (cx - o);
-      S.lad[y] = 0.5 * as;
-      S.lad[S.n3 + y] = 0.5 * ap;
+      S.lad[(size_t)xpo + (size_t)o2 * (cx - o)] = make_double2(0.5 * as, 0.5 * ap);
     } else {
       const int q = col - nvv;
       const int i = xpo % o;
       const double vik = vt64[d2(S.nv4[i], nx)], vil = vt64[d2(S.nv4[i], ny)];
-      const int wi = S.kb_woff[K] + rowo * noo + q;
-      S.w[wi] = vik - vil + 0.5 * as;
-      S.w[S.wtot + wi] = vik + vil + 0.5 * ap;
+      S.w[S.kb_woff[K] + rowo * noo + q] = make_double2(vik - vil + 0.5 * as, vik + vil + 0.5 * ap);
     }
   }
 }
@@ -368,354 +455,499 @@ __global__ __launch_bounds__(64) void k_ladder(const SolveDev* __restrict__ solv
 }
 
 // --------------------------------------------------------------------------
-// Residual assembly + update, one thread per dense (i,j,a).
+// Residual assembly + update: kSplit lanes per dense (i,j,a).  The lanes of
+// an element share its three contraction loops (hole-hole W, quadratic rings,
+// quadratic crossed rings) and combine them with two shuffles; lane 0 of the
+// group then assembles G, R = D*T - G, the damped update, dT, the energy and
+// the DIIS dot products.
 __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restrict__ solves,
                                                         const int* __restrict__ prefix, int n,
                                                         int flags) {
   __shared__ Lds L;
-  const int s = find_solve(prefix, n, blockIdx.x);
+  __shared__ double red[(kElemThreads / 64) * kPartialStride];
+  const int gb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
   if (S.st->phase == PH_DONE) return;
-  const int lb = blockIdx.x - prefix[s];
-  stage(S, L);
+  const int lb = gb - prefix[s];
   const int o = S.o, o2 = o * o, n3 = S.n3;
-  const int x = lb * blockDim.x + threadIdx.x;
+  stage(S, L);
+  const int part = threadIdx.x % kSplit;
+  const int x = lb * kElemPerBlock + threadIdx.x / kSplit;
   const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
   const int slot = S.st->slot;
-  const double* tab = S.tcur;
-  const double* tba = S.tcur + n3;
+  const double2* __restrict__ T = S.tcur;
 
-  double e_part = 0.0, dt_part = 0.0, r_part = 0.0, bad = 0.0;
-  double eab = 0.0, eba = 0.0;
-  bool valid = false;
-  if (x < n3) {
-    const int b = S.pidx[x];
-    if (b >= 0) {
-      valid = true;
-      const int i = x % o, j = (x / o) % o, av = x / o2, a = o + av, bv = b - o;
-      const double t_ab = tab[x], t_ba = tba[x];
-      // drivers (source/CCD.f90:484-501)
-      double gab = VPQ(i, a);
-      double gba = -VPQ(i, b);
-      // mosaics (source/CCD.f90:571-582)
-      if (S.do_mosaic) {
-        const double* jvv = S.jvo;
-        const double* joo = S.jvo + S.v;
-        gab = gab - 0.5 * (jvv[av] * t_ab + jvv[bv] * t_ab);
-        gba = gba - 0.5 * (jvv[av] * t_ba + jvv[bv] * t_ba);
-        gab = gab - 0.5 * (joo[i] * t_ab + joo[j] * t_ab);
-        gba = gba - 0.5 * (joo[i] * t_ba + joo[j] * t_ba);
+  // --- contraction loops, split over the kSplit lanes of the element ---
+  int b = -1;
+  if (x < n3) b = S.pidx[x];
+  const bool valid = b >= 0;
+  const int i = x % o, j = (x / o) % o, av = x / o2, a = o + av, bv = b - o;
+  double hs = 0.0, hp = 0.0, qab = 0.0, qba = 0.0;
+  if (valid) {
+    if (S.do_ladder) {
+      // hole-hole ladder, sum_q W[p,q] S|P[q,a] (source/CCD.f90:651-656)
+      const int xp = i + o * j;
+      const int K = S.pair_kb[xp], pos = S.pair_pos[xp];
+      const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
+      const double2* W = S.w + S.kb_woff[K] + pos * noo;
+      for (int q = part; q < noo; q += kSplit) {
+        const double2 t = T[S.kb_pairs[poff + q] + o2 * av];
+        const double2 w = W[q];
+        hs += w.x * (t.x + t.y);
+        hp += w.y * (t.x - t.y);
       }
-      // ladders: pp part from k_ladder, hh contraction with W (source/CCD.f90:651-656)
-      if (S.do_ladder) {
-        const int xp = i + o * j;
-        const int K = S.pair_kb[xp], pos = S.pair_pos[xp];
-        const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
-        const double* ws = S.w + S.kb_woff[K] + pos * noo;
-        const double* wp = ws + S.wtot;
-        double hs = 0.0, hp = 0.0;
-        for (int q = 0; q < noo; ++q) {
-          const int y = S.kb_pairs[poff + q] + o2 * av;
-          const double ta = tab[y], tb = tba[y];
-          hs += ws[q] * (ta + tb);
-          hp += wp[q] * (ta - tb);
-        }
-        const double ls = S.lad[x] + 0.5 * hs;
-        const double lp = S.lad[n3 + x] + 0.5 * hp;
-        gab += 0.5 * (ls + lp);
-        gba += 0.5 * (ls - lp);
+    }
+    if (S.do_ring) {
+      // quadratic rings, sum_l J(i,a,l) T(l,j,d), d = FindIndex(J,L,B)
+      // (source/CCD.f90:801-808)
+      const double4* J = S.jring + (size_t)(av * o + i) * o;
+      for (int l = part; l < o; l += kSplit) {
+        const int d = S.pidx[j + o * l + o2 * bv];
+        if (d < 0) continue;
+        const double2 t = T[l + o * j + o2 * (d - o)];
+        const double4 jj = J[l];
+        qab += jj.y * (t.x + t.y) + jj.x * t.x;
+        qba += jj.z * t.y;
       }
-      const int ovo = o * S.v * o;
-      // rings (source/CCD.f90:716-811)
-      if (S.do_ring) {
-        double v1 = VPQ(i, a);
-        for (int k = 0; k < o; ++k) {
-          const int y = i + o * k + o2 * av;
-          if (S.pidx[y] < 0) continue;
-          const double v2 = VPQ(k, j);
-          const double ta = tab[y], tb = tba[y];
-          gab += (v1 - v2) * ta + v1 * (ta + tb);
-          gba -= v2 * tb;
-        }
-        v1 = VPQ(j, b);
-        for (int k = 0; k < o; ++k) {
-          const int y = j + o * k + o2 * bv;
-          if (S.pidx[y] < 0) continue;
-          const double v2 = VPQ(k, i);
-          const double ta = tab[y], tb = tba[y];
-          gab += (v1 - v2) * ta + v1 * (ta + tb);
-          gba -= v2 * tb;
-        }
-        const double* J = S.jr + (size_t)(av * o + i) * o;
-        for (int l = 0; l < o; ++l) {
-          const int d = S.pidx[j + o * l + o2 * bv];
-          if (d < 0) continue;
-          const int y = l + o * j + o2 * (d - o);
-          const double ta = tab[y], tb = tba[y];
-          gab += J[ovo + l] * (ta + tb) + J[l] * ta;
-          gba += J[2 * ovo + l] * tb;
-        }
-      }
-      // crossed rings (source/CCD.f90:845-937)
-      if (S.do_xring) {
-        const double vja = VPQ(j, a);
-        for (int k = 0; k < o; ++k) {
-          const int y = k + o * i + o2 * bv;
-          if (S.pidx[y] < 0) continue;
-          const double vjk = VPQ(j, k);
-          const double ta = tab[y], tb = tba[y];
-          gab -= vjk * ta;
-          gba += (vja - vjk) * tb + vja * (ta + tb);
-        }
-        const double vib = VPQ(i, b);
-        for (int k = 0; k < o; ++k) {
-          const int y = k + o * j + o2 * av;
-          if (S.pidx[y] < 0) continue;
-          const double vik = VPQ(i, k);
-          const double ta = tab[y], tb = tba[y];
-          gab -= vik * ta;
-          gba += (vib - vik) * tb + vib * (ta + tb);
-        }
-        const double* J = S.jr + (size_t)3 * ovo + (size_t)(bv * o + i) * o;
-        for (int l = 0; l < o; ++l) {
-          const int y = l + o * j + o2 * av;
-          if (S.pidx[y] < 0) continue;
-          const double ta = tab[y], tb = tba[y];
-          gba = gba - J[l] * tb - J[ovo + l] * (ta + tb);
-          gab += J[2 * ovo + l] * ta;
-        }
-      }
-      // GetRes + SolveCC + CCEnergy on this element (source/CCD.f90:378-417, 225-227)
-      const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
-      const double rab = -gab + den * t_ab;
-      const double rba = -gba + den * t_ba;
-      const double df = S.df;
-      const double nab = (gab / den - (1.0 - df) * t_ab) / df;
-      const double nba = (gba / den - (1.0 - df) * t_ba) / df;
-      dt_part = fmax(fabs(nab - t_ab), fabs(nba - t_ba));
-      r_part = fmax(fabs(rab), fabs(rba));
-      const double naa = nab + nba;
-      e_part = 0.5 * (VPQ(a, i) * (naa + nab) - VPQ(a, j) * (naa + nba));
-      if (!isfinite(nab) || !isfinite(nba) || !isfinite(rab) || !isfinite(rba)) bad = 1.0;
-      if (flags & MF_WRITE_GR) {
-        if (S.gout) { S.gout[x] = gab; S.gout[n3 + x] = gba; }
-        if (S.rout) { S.rout[x] = rab; S.rout[n3 + x] = rba; }
-      }
-      if (flags & MF_ITERATE) {
-        double* HT = S.histT + (size_t)slot * 2 * n3;
-        HT[x] = nab;
-        HT[n3 + x] = nba;
-        if (S.diis_err_is_dt) { eab = nab - t_ab; eba = nba - t_ba; }
-        else { eab = rab; eba = rba; }
-        if (S.mode == UEGCCD_MODE_DIIS) {
-          double* HE = S.histE + (size_t)slot * 2 * n3;
-          HE[x] = eab;
-          HE[n3 + x] = eba;
-        }
+    }
+    if (S.do_xring) {
+      // quadratic crossed rings, sum_l J(i,b,l) T(l,j,a); T(l,j,a) is zero
+      // unless FindIndex(L,J,A) is virtual (source/CCD.f90:927-934)
+      const double4* J = S.jxring + (size_t)(bv * o + i) * o;
+      for (int l = part; l < o; l += kSplit) {
+        const double2 t = T[l + o * j + o2 * av];
+        const double4 jj = J[l];
+        qba = qba - jj.x * t.y - jj.y * (t.x + t.y);
+        qab += jj.z * t.x;
       }
     }
   }
-  double* P = S.partials + (size_t)lb * kPartialStride;
-  e_part = block_sum(e_part, L.red);
-  dt_part = block_max(dt_part, L.red);
-  r_part = block_max(r_part, L.red);
-  bad = block_sum(bad, L.red);
-  if (threadIdx.x == 0) { P[0] = e_part; P[1] = dt_part; P[2] = r_part; P[3] = bad; }
-  if (S.mode == UEGCCD_MODE_DIIS && nh > 0) {
-    for (int k = 0; k < nh; ++k) {
-      double dk = 0.0;
-      if (valid) {
-        if (k == slot) {
-          dk = eab * eab + eba * eba;
-        } else {
-          const double* HE = S.histE + (size_t)k * 2 * n3;
-          dk = eab * HE[x] + eba * HE[n3 + x];
-        }
-      }
-      dk = block_sum(dk, L.red);
-      if (threadIdx.x == 0) P[4 + k] = dk;
+  for (int m = 1; m < kSplit; m <<= 1) {
+    hs += __shfl_xor(hs, m, 64);
+    hp += __shfl_xor(hp, m, 64);
+    qab += __shfl_xor(qab, m, 64);
+    qba += __shfl_xor(qba, m, 64);
+  }
+
+  double e_part = 0.0, dt_part = 0.0, r_part = 0.0, bad = 0.0;
+  double eab = 0.0, eba = 0.0;
+  const bool lead = valid && part == 0;
+  if (lead) {
+    const double2 t0 = T[x];
+    const double t_ab = t0.x, t_ba = t0.y;
+    // drivers (source/CCD.f90:484-501)
+    double gab = VPQ(i, a);
+    double gba = -VPQ(i, b);
+    // mosaics (source/CCD.f90:571-582)
+    if (S.do_mosaic) {
+      const double* jvv = S.jvv;
+      const double* joo = S.joo;
+      gab = gab - 0.5 * (jvv[av] * t_ab + jvv[bv] * t_ab);
+      gba = gba - 0.5 * (jvv[av] * t_ba + jvv[bv] * t_ba);
+      gab = gab - 0.5 * (joo[i] * t_ab + joo[j] * t_ab);
+      gba = gba - 0.5 * (joo[i] * t_ba + joo[j] * t_ba);
     }
+    // ladders: particle-particle part from k_ladder plus the hole-hole sum
+    if (S.do_ladder) {
+      const double2 l2 = S.lad[x];
+      const double ls = l2.x + 0.5 * hs;
+      const double lp = l2.y + 0.5 * hp;
+      gab += 0.5 * (ls + lp);
+      gba += 0.5 * (ls - lp);
+    }
+    // slice products of k_slice: element (i,j) of slice a, (j,i) of slice b
+    if (S.do_ring || S.do_xring) {
+      const double4 ta = S.tv[x];
+      const double4 tb = S.tv[(size_t)j + o * i + (size_t)o2 * bv];
+      if (S.do_ring) {
+        // linear rings (source/CCD.f90:716-763): v(q) rowsum - (T V), a- and b-slice
+        gab += VPQ(i, a) * S.rsum[av * o + i] - ta.x;
+        gba -= ta.y;
+        gab += VPQ(j, b) * S.rsum[bv * o + j] - tb.x;
+        gba -= tb.y;
+      }
+      if (S.do_xring) {
+        // linear crossed rings (source/CCD.f90:845-892): (V T) of the b- and a-slices
+        gab -= tb.z;
+        gba += VPQ(j, a) * S.csum[bv * o + i] - tb.w;
+        gab -= ta.z;
+        gba += VPQ(i, b) * S.csum[av * o + j] - ta.w;
+      }
+      gab += qab;
+      gba += qba;
+    }
+    // GetRes + SolveCC + CCEnergy on this element (source/CCD.f90:378-417, 225-227)
+    const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
+    const double rab = -gab + den * t_ab;
+    const double rba = -gba + den * t_ba;
+    const double df = S.df;
+    const double nab = (gab / den - (1.0 - df) * t_ab) / df;
+    const double nba = (gba / den - (1.0 - df) * t_ba) / df;
+    dt_part = fmax(fabs(nab - t_ab), fabs(nba - t_ba));
+    r_part = fmax(fabs(rab), fabs(rba));
+    const double naa = nab + nba;
+    e_part = 0.5 * (VPQ(a, i) * (naa + nab) - VPQ(a, j) * (naa + nba));
+    if (!isfinite(nab) || !isfinite(nba) || !isfinite(rab) || !isfinite(rba)) bad = 1.0;
+    if (flags & MF_WRITE_GR) {
+      if (S.gout) { S.gout[x] = gab; S.gout[n3 + x] = gba; }
+      if (S.rout) { S.rout[x] = rab; S.rout[n3 + x] = rba; }
+    }
+    if (flags & MF_ITERATE) {
+      S.histT[(size_t)x * S.ndiis + slot] = make_double2(nab, nba);
+      if (S.diis_err_is_dt) { eab = nab - t_ab; eba = nba - t_ba; }
+      else { eab = rab; eba = rba; }
+      if (S.mode == UEGCCD_MODE_DIIS) S.histE[(size_t)x * S.ndiis + slot] = make_double2(eab, eba);
+    }
+  }
+  // block partials: max dT, max |R|, sum E, non-finite count, DIIS dots
+  const int nd = (S.mode == UEGCCD_MODE_DIIS) ? nh : 0;
+  double v[kPartialStride];
+  v[0] = dt_part;
+  v[1] = r_part;
+  v[2] = e_part;
+  v[3] = bad;
+  const double2* HE = S.histE + (size_t)(lead ? x : 0) * S.ndiis;
+#pragma unroll
+  for (int k = 0; k < kMaxDiis; ++k) {
+    double dk = 0.0;
+    if (k < nd && lead) {
+      if (k == slot) {
+        dk = eab * eab + eba * eba;
+      } else {
+        const double2 h = HE[k];
+        dk = eab * h.x + eba * h.y;
+      }
+    }
+    v[4 + k] = dk;
+  }
+  block_reduce_multi<kPartialStride, 2>(v, red);
+  if (threadIdx.x == 0) {
+    *partial(S, 0, lb) = v[2];
+    *partial(S, 1, lb) = v[0];
+    *partial(S, 2, lb) = v[1];
+    *partial(S, 3, lb) = v[3];
+#pragma unroll
+    for (int k = 0; k < kMaxDiis; ++k)
+      if (k < nd) *partial(S, 4 + k, lb) = v[4 + k];
   }
 }
 
 // --------------------------------------------------------------------------
-// Small dense solve for the DIIS coefficients (FP64, partial pivoting).
+// Small dense solve for the DIIS coefficients (FP64, partial pivoting), by one
+// wave64: lane r owns row r of the bordered system
+//   [ B/scale  -1 ] [c]   [ 0]
+//   [ -1^T      0 ] [l] = [-1]
+// over the nh newest history slots; the rows live in registers (columns
+// unrolled), pivots and row swaps go through cross-lane shuffles.
 __device__ bool diis_coefficients(const State& st, int ndiis, double* coef) {
-  const int nh = st.nhist;
-  int sl[kMaxDiis];
-  for (int r = 0; r < nh; ++r) sl[r] = ((st.slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis;
-  const int m = nh + 1;
-  double A[(kMaxDiis + 1) * (kMaxDiis + 1)];
-  double rhs[kMaxDiis + 1];
-  double scale = 0.0;
-  for (int r = 0; r < nh; ++r) scale = fmax(scale, fabs(st.bmat[sl[r] * kMaxDiis + sl[r]]));
+  constexpr int M = kMaxDiis + 1;
+  const int lane = threadIdx.x & 63;
+  const int nh = st.nhist, m = nh + 1;
+  auto slot_of = [&](int r) { return ((st.slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis; };
+  const int slr = lane < nh ? slot_of(lane) : 0;
+  double scale = lane < nh ? fabs(st.bmat[slr * kMaxDiis + slr]) : 0.0;
+  scale = wave_max(scale);
   if (!(scale > 0.0) || !isfinite(scale)) return false;
-  for (int r = 0; r < nh; ++r) {
-    for (int c = 0; c < nh; ++c) A[r * m + c] = st.bmat[sl[r] * kMaxDiis + sl[c]] / scale;
-    A[r * m + nh] = -1.0;
-    A[nh * m + r] = -1.0;
-    rhs[r] = 0.0;
-  }
-  A[nh * m + nh] = 0.0;
-  rhs[nh] = -1.0;
-  for (int c = 0; c < m; ++c) {
-    int p = c;
-    for (int r = c + 1; r < m; ++r)
-      if (fabs(A[r * m + c]) > fabs(A[p * m + c])) p = r;
-    if (fabs(A[p * m + c]) < 1e-14) return false;
-    if (p != c) {
-      for (int k = 0; k < m; ++k) { double t = A[c * m + k]; A[c * m + k] = A[p * m + k]; A[p * m + k] = t; }
-      double t = rhs[c]; rhs[c] = rhs[p]; rhs[p] = t;
+  double row[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    double val = 0.0;
+    if (lane < nh) {
+      if (c < nh) val = st.bmat[slr * kMaxDiis + slot_of(c)] / scale;
+      else if (c == nh) val = -1.0;
+    } else if (lane == nh && c < nh) {
+      val = -1.0;
     }
-    for (int r = c + 1; r < m; ++r) {
-      const double f = A[r * m + c] / A[c * m + c];
-      for (int k = c; k < m; ++k) A[r * m + k] -= f * A[c * m + k];
+    row[c] = val;
+  }
+  double rhs = lane == nh ? -1.0 : 0.0;
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    if (c >= m || !ok) continue;
+    double best = (lane >= c && lane < m) ? fabs(row[c]) : -1.0;
+    int bl = lane;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int ol = __shfl_xor(bl, off, 64);
+      if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
+    }
+    if (!(best >= 1e-14)) { ok = false; continue; }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double vc = __shfl(row[k], c, 64), vp = __shfl(row[k], bl, 64);
+      if (lane == c) row[k] = vp; else if (lane == bl) row[k] = vc;
+    }
+    {
+      const double vc = __shfl(rhs, c, 64), vp = __shfl(rhs, bl, 64);
+      if (lane == c) rhs = vp; else if (lane == bl) rhs = vc;
+    }
+    const double piv = __shfl(row[c], c, 64);
+    const double f = (lane > c && lane < m) ? row[c] / piv : 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double pk = __shfl(row[k], c, 64);
+      if (lane > c && lane < m) row[k] -= f * pk;
+    }
+    const double pr = __shfl(rhs, c, 64);
+    if (lane > c && lane < m) rhs -= f * pr;
+  }
+  if (!ok) return false;
+  double xs = 0.0;
+#pragma unroll
+  for (int r = M - 1; r >= 0; --r) {
+    if (r >= m) continue;
+    double acc = rhs;
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      if (k > r && k < m) acc -= row[k] * __shfl(xs, k, 64);
+    if (lane == r) xs = acc / row[r];
+  }
+  const bool fin = isfinite(xs) || lane >= nh;
+  if (!__all(fin)) return false;
+  if (lane < ndiis) coef[lane] = 0.0;
+  if (lane < nh) coef[slr] = xs;
+  return true;
+}
+
+// The same bordered system for histories of up to M-1 entries, by one thread
+// with the whole matrix in registers: every loop is unrolled, the pivot row is
+// swapped in with selects, and rows past the system are padded with the
+// identity so the unrolled elimination runs over a fixed M x M.
+template <int M>
+__device__ bool diis_coefficients_regs(const State& st, int ndiis, double* coef) {
+  const int nh = st.nhist;
+  int sl[M];  // history slot of row r (oldest first)
+  {
+    int sv = st.slot_cur - (nh - 1);
+    if (sv < 0) sv += ndiis;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      sl[r] = sv;
+      sv = (sv + 1 == ndiis) ? 0 : sv + 1;
+    }
+  }
+  double scale = 0.0;
+#pragma unroll
+  for (int r = 0; r < M - 1; ++r)
+    if (r < nh) scale = fmax(scale, fabs(st.bmat[sl[r] * (kMaxDiis + 1)]));
+  if (!(scale > 0.0) || !isfinite(scale)) return false;
+  const double inv_scale = 1.0 / scale;
+  double A[M][M], rhs[M];
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      double val = 0.0;
+      if (r < nh && c < nh) val = st.bmat[sl[r] * kMaxDiis + sl[c]] * inv_scale;
+      else if ((r < nh && c == nh) || (r == nh && c < nh)) val = -1.0;
+      else if (r > nh && r == c) val = 1.0;
+      A[r][c] = val;
+    }
+    rhs[r] = (r == nh) ? -1.0 : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    double best = fabs(A[c][c]);
+    int p = c;
+#pragma unroll
+    for (int r = c + 1; r < M; ++r)
+      if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); p = r; }
+    if (!(best >= 1e-14)) return false;
+#pragma unroll
+    for (int r = c + 1; r < M; ++r) {
+      const bool sw = (r == p);
+#pragma unroll
+      for (int k = c; k < M; ++k) {
+        const double t = A[c][k];
+        A[c][k] = sw ? A[r][k] : t;
+        A[r][k] = sw ? t : A[r][k];
+      }
+      const double t = rhs[c];
+      rhs[c] = sw ? rhs[r] : t;
+      rhs[r] = sw ? t : rhs[r];
+    }
+    const double inv_piv = 1.0 / A[c][c];
+#pragma unroll
+    for (int r = c + 1; r < M; ++r) {
+      const double f = A[r][c] * inv_piv;
+#pragma unroll
+      for (int k = c; k < M; ++k) A[r][k] -= f * A[c][k];
       rhs[r] -= f * rhs[c];
     }
   }
-  for (int r = m - 1; r >= 0; --r) {
+  double x[M];
+#pragma unroll
+  for (int r = M - 1; r >= 0; --r) {
     double acc = rhs[r];
-    for (int k = r + 1; k < m; ++k) acc -= A[r * m + k] * rhs[k];
-    rhs[r] = acc / A[r * m + r];
+#pragma unroll
+    for (int k = r + 1; k < M; ++k) acc -= A[r][k] * x[k];
+    x[r] = acc / A[r][r];
   }
   for (int k = 0; k < ndiis; ++k) coef[k] = 0.0;
-  for (int r = 0; r < nh; ++r) {
-    if (!isfinite(rhs[r])) return false;
-    coef[sl[r]] = rhs[r];
+#pragma unroll
+  for (int r = 0; r < M - 1; ++r) {
+    if (r >= nh) continue;
+    if (!isfinite(x[r])) return false;
+    coef[sl[r]] = x[r];
   }
   return true;
 }
 
 // Reductions of the block partials and the iteration bookkeeping of
 // source/CCD.f90:80-99 (one block per solve).
-__global__ __launch_bounds__(256) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
-  __shared__ double red[16];
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
   __shared__ double vals[4 + kMaxDiis];
+  __shared__ double redm[(kFinThreads / 64) * kPartialStride];
+  __shared__ int want_diis;
+  // The iteration state is worked on in LDS: one parallel copy in, one out,
+  // instead of a chain of dependent global accesses by a single thread.
+  __shared__ State sst;
   const SolveDev& S = solves[blockIdx.x];
-  State* st = S.st;
-  if (kind == FIN_ITER && st->phase == PH_DONE) return;
+  const int phase0 = S.st->phase;
+  if (kind == FIN_ITER && phase0 == PH_DONE) return;
+  constexpr int kStateWords = sizeof(State) / sizeof(double);
+  for (int k = threadIdx.x; k < kStateWords; k += blockDim.x)
+    reinterpret_cast<double*>(&sst)[k] = reinterpret_cast<const double*>(S.st)[k];
+  State* st = &sst;
   const int nb = S.nblk_elem;
-  const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS && st->phase == PH_ITER) ? S.ndiis : 0;
+  const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS && phase0 == PH_ITER) ? S.ndiis : 0;
   {
-    double e = 0.0, dt = 0.0, r = 0.0, bad = 0.0;
+    // one coalesced pass over the slot-major partials:
+    // v = (max dT, max |R|, sum E, sum bad, dots)
+    double v[kPartialStride];
+#pragma unroll
+    for (int q = 0; q < kPartialStride; ++q) v[q] = 0.0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-      const double* P = S.partials + (size_t)k * kPartialStride;
-      e += P[0];
-      dt = fmax(dt, P[1]);
-      r = fmax(r, P[2]);
-      bad += P[3];
+      v[0] = fmax(v[0], *partial(S, 1, k));
+      v[1] = fmax(v[1], *partial(S, 2, k));
+      v[2] += *partial(S, 0, k);
+      v[3] += *partial(S, 3, k);
+#pragma unroll
+      for (int q = 0; q < kMaxDiis; ++q)
+        if (q < nd) v[4 + q] += *partial(S, 4 + q, k);
     }
-    e = block_sum(e, red);
-    dt = block_max(dt, red);
-    r = block_max(r, red);
-    bad = block_sum(bad, red);
-    if (threadIdx.x == 0) { vals[0] = e; vals[1] = dt; vals[2] = r; vals[3] = bad; }
-    for (int q = 0; q < nd; ++q) {
-      double d = 0.0;
-      for (int k = threadIdx.x; k < nb; k += blockDim.x) d += S.partials[(size_t)k * kPartialStride + 4 + q];
-      d = block_sum(d, red);
-      if (threadIdx.x == 0) vals[4 + q] = d;
+    block_reduce_multi<kPartialStride, 2>(v, redm);
+    if (threadIdx.x == 0) {
+      vals[0] = v[2];
+      vals[1] = v[0];
+      vals[2] = v[1];
+      vals[3] = v[3];
+#pragma unroll
+      for (int q = 0; q < kMaxDiis; ++q)
+        if (q < nd) vals[4 + q] = v[4 + q];
+    }
+  }
+  if (threadIdx.x == 0) want_diis = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double e = vals[0], dt = vals[1], rmax = vals[2];
+    if (vals[3] > 0.0) st->nonfinite = 1;
+    if (kind == FIN_SUM) {
+      st->ecorr = e;
+    } else if (kind == FIN_INIT) {
+      st->niter = 1;
+      st->ecorr = e;
+      st->ecorr_prev = e;
+      st->dt = dt;
+      st->slot = 0;
+      st->slot_cur = 0;
+      st->nhist = 0;
+      st->use_coef = 0;
+      st->converged = 0;
+      if (S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
+      const bool go = S.mode == UEGCCD_MODE_REFERENCE ? (dt >= S.tol_dt) : true;
+      st->phase = (go && st->niter < S.maxiter && !st->nonfinite) ? PH_ITER : PH_FINAL;
+      if (!go) st->converged = 1;
+    } else if (phase0 == PH_FINAL) {
+      st->maxres = rmax;
+      st->phase = PH_DONE;
+    } else {
+      st->niter += 1;
+      st->ecorr_prev = st->ecorr;
+      st->ecorr = e;
+      st->dt = dt;
+      st->maxres_in = rmax;
+      if (S.trace && st->niter <= S.maxiter) {
+        S.trace[2 * (st->niter - 1)] = e;
+        S.trace[2 * (st->niter - 1) + 1] = dt;
+      }
+      const int slot = st->slot;
+      st->slot_cur = slot;
+      st->slot = (slot + 1) % S.ndiis;
+      bool switched = false;
+      if (!st->prec64) {
+        st->iters_fp32 += 1;
+        if (rmax < S.mixed_switch) { st->prec64 = 1; switched = true; }
+      }
+      bool conv;
+      if (S.mode == UEGCCD_MODE_REFERENCE)
+        conv = dt < S.tol_dt;
+      else
+        conv = st->prec64 && !switched && fabs(e - st->ecorr_prev) < S.tol_e && rmax < S.tol_r;
+      const bool stop = conv || st->niter >= S.maxiter;
+      st->use_coef = 0;
+      if (S.mode == UEGCCD_MODE_DIIS) {
+        if (switched) {
+          st->nhist = 0;  // residuals of the FP32 phase do not enter FP64 extrapolation
+        } else {
+          st->nhist = st->nhist < S.ndiis ? st->nhist + 1 : S.ndiis;
+          for (int k = 0; k < S.ndiis; ++k) {
+            st->bmat[slot * kMaxDiis + k] = vals[4 + k];
+            st->bmat[k * kMaxDiis + slot] = vals[4 + k];
+          }
+          if (!stop && st->niter >= S.diis_start && st->nhist >= 2 && !st->nonfinite)
+            want_diis = 1;
+        }
+      }
+      if (st->nonfinite) {
+        st->phase = PH_DONE;
+      } else {
+        if (conv) st->converged = 1;
+        if (stop) st->phase = PH_FINAL;
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double e = vals[0], dt = vals[1], rmax = vals[2];
-  if (vals[3] > 0.0) st->nonfinite = 1;
-  if (kind == FIN_SUM) { st->ecorr = e; return; }
-  if (kind == FIN_INIT) {
-    st->niter = 1;
-    st->ecorr = e;
-    st->ecorr_prev = e;
-    st->dt = dt;
-    st->slot = 0;
-    st->slot_cur = 0;
-    st->nhist = 0;
-    st->use_coef = 0;
-    st->converged = 0;
-    if (S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
-    const bool go = S.mode == UEGCCD_MODE_REFERENCE ? (dt >= S.tol_dt) : true;
-    st->phase = (go && st->niter < S.maxiter && !st->nonfinite) ? PH_ITER : PH_FINAL;
-    if (!go) st->converged = 1;
-    return;
-  }
-  // FIN_ITER
-  if (st->phase == PH_FINAL) {
-    st->maxres = rmax;
-    st->phase = PH_DONE;
-    return;
-  }
-  st->niter += 1;
-  st->ecorr_prev = st->ecorr;
-  st->ecorr = e;
-  st->dt = dt;
-  st->maxres_in = rmax;
-  if (S.trace && st->niter <= S.maxiter) {
-    S.trace[2 * (st->niter - 1)] = e;
-    S.trace[2 * (st->niter - 1) + 1] = dt;
-  }
-  const int slot = st->slot;
-  st->slot_cur = slot;
-  st->slot = (slot + 1) % S.ndiis;
-  bool switched = false;
-  if (!st->prec64) {
-    st->iters_fp32 += 1;
-    if (rmax < S.mixed_switch) { st->prec64 = 1; switched = true; }
-  }
-  bool conv;
-  if (S.mode == UEGCCD_MODE_REFERENCE)
-    conv = dt < S.tol_dt;
-  else
-    conv = st->prec64 && !switched && fabs(e - st->ecorr_prev) < S.tol_e && rmax < S.tol_r;
-  const bool stop = conv || st->niter >= S.maxiter;
-  st->use_coef = 0;
-  if (S.mode == UEGCCD_MODE_DIIS) {
-    if (switched) {
-      st->nhist = 0;  // residuals of the FP32 phase do not enter FP64 extrapolation
-    } else {
-      // B-matrix row/column of the new entry
-      st->nhist = st->nhist < S.ndiis ? st->nhist + 1 : S.ndiis;
-      for (int k = 0; k < S.ndiis; ++k) {
-        st->bmat[slot * kMaxDiis + k] = vals[4 + k];
-        st->bmat[k * kMaxDiis + slot] = vals[4 + k];
-      }
-      if (!stop && st->niter >= S.diis_start && st->nhist >= 2)
-        st->use_coef = diis_coefficients(*st, S.ndiis, st->coef) ? 1 : 0;
+  if (want_diis) {
+    if (S.ndiis <= 8) {
+      if (threadIdx.x == 0) st->use_coef = diis_coefficients_regs<9>(*st, S.ndiis, st->coef) ? 1 : 0;
+    } else if (threadIdx.x < 64) {
+      const bool ok = diis_coefficients(*st, S.ndiis, st->coef);
+      if (threadIdx.x == 0) st->use_coef = ok ? 1 : 0;
     }
   }
-  if (st->nonfinite) {
-    st->phase = PH_DONE;
-    return;
-  }
-  if (conv) st->converged = 1;
-  if (stop) st->phase = PH_FINAL;
+  __syncthreads();
+  for (int k = threadIdx.x; k < kStateWords; k += blockDim.x)
+    reinterpret_cast<double*>(S.st)[k] = reinterpret_cast<const double*>(&sst)[k];
 }
 
 // Amplitudes for the next pass: DIIS extrapolation or the plain update.
-__global__ __launch_bounds__(kElemThreads) void k_extrap(const SolveDev* __restrict__ solves,
-                                                          const int* __restrict__ prefix, int n) {
-  const int s = find_solve(prefix, n, blockIdx.x);
+__global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __restrict__ solves,
+                                                           const int* __restrict__ prefix, int n) {
+  const int gb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
   const State* st = S.st;
   if (st->phase == PH_DONE) return;
-  const int lb = blockIdx.x - prefix[s];
+  const int lb = gb - prefix[s];
   const int x = lb * blockDim.x + threadIdx.x;
-  const int n3 = S.n3;
-  if (x >= n3 || S.pidx[x] < 0) return;
+  if (x >= S.n3 || S.pidx[x] < 0) return;
+  const double2* HT = S.histT + (size_t)x * S.ndiis;
   if (st->use_coef) {
     double ab = 0.0, ba = 0.0;
     for (int k = 0; k < S.ndiis; ++k) {
       const double c = st->coef[k];
       if (c == 0.0) continue;
-      const double* HT = S.histT + (size_t)k * 2 * n3;
-      ab += c * HT[x];
-      ba += c * HT[n3 + x];
+      const double2 h = HT[k];
+      ab += c * h.x;
+      ba += c * h.y;
     }
-    S.tcur[x] = ab;
-    S.tcur[n3 + x] = ba;
+    S.tcur[x] = make_double2(ab, ba);
   } else {
-    const double* HT = S.histT + (size_t)st->slot_cur * 2 * n3;
-    S.tcur[x] = HT[x];
-    S.tcur[n3 + x] = HT[n3 + x];
+    S.tcur[x] = HT[st->slot_cur];
   }
 }
 
@@ -737,31 +969,44 @@ __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, in
 }
 
 // --------------------------------------------------------------------------
+// Dynamic LDS above the 64 KiB default needs an explicit per-kernel limit.
+static void allow_large_lds() {
+  static bool done = false;
+  if (done) return;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_slice),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds - (int)sizeof(Lds));
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_joo),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds);
+  done = true;
+}
+
+static void launch_slice(const BatchDev& b, hipStream_t s) {
+  if (b.slice_blocks <= 0) return;
+  allow_large_lds();
+  hipLaunchKernelGGL(k_slice, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
+                     b.solves, b.slice_prefix, b.n);
+  hipLaunchKernelGGL(k_joo, dim3(b.n), dim3(kElemThreads), b.joo_lds, s, b.solves);
+}
+
 void launch_init(const BatchDev& b, hipStream_t s) {
-  hipLaunchKernelGGL(k_init, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
+  hipLaunchKernelGGL(k_init, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
                      b.elem_prefix, b.n);
 }
 void launch_mp2(const BatchDev& b, hipStream_t s) {
-  hipLaunchKernelGGL(k_mp2, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
+  hipLaunchKernelGGL(k_mp2, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
                      b.elem_prefix, b.n);
 }
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(256), 0, s, b.solves, kind);
+  hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, kind);
 }
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s) {
   hipLaunchKernelGGL(k_state_reset, dim3(b.n), dim3(64), 0, s, b.solves, phase, prec64);
 }
+
 void launch_one(const BatchDev& b, int which, hipStream_t s) {
   switch (which) {
     case 0:
-      if (b.mos_blocks > 0)
-        hipLaunchKernelGGL(k_mosaic_j, dim3(b.mos_blocks), dim3(kElemThreads), 0, s, b.solves,
-                           b.mos_prefix, b.n);
-      break;
-    case 1:
-      if (b.ring_blocks > 0)
-        hipLaunchKernelGGL(k_ring_j, dim3(b.ring_blocks), dim3(kElemThreads), 0, s, b.solves,
-                           b.ring_prefix, b.n);
+      launch_slice(b, s);
       break;
     case 2:
       if (b.nlad > 0)
@@ -772,7 +1017,7 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
                          b.elem_prefix, b.n, (int)MF_ITERATE);
       break;
     case 4:
-      hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
+      hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
                          b.elem_prefix, b.n);
       break;
     default:
@@ -781,19 +1026,14 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
 }
 
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s) {
-  if (b.mos_blocks > 0)
-    hipLaunchKernelGGL(k_mosaic_j, dim3(b.mos_blocks), dim3(kElemThreads), 0, s, b.solves,
-                       b.mos_prefix, b.n);
-  if (b.ring_blocks > 0)
-    hipLaunchKernelGGL(k_ring_j, dim3(b.ring_blocks), dim3(kElemThreads), 0, s, b.solves,
-                       b.ring_prefix, b.n);
+  launch_slice(b, s);
   if (b.nlad > 0)
     hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(64), 0, s, b.solves, b.lad_items, b.nlad);
   hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
                      b.elem_prefix, b.n, main_flags);
   if (main_flags & MF_ITERATE) {
-    hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(256), 0, s, b.solves, (int)FIN_ITER);
-    hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
+    hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, (int)FIN_ITER);
+    hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
                        b.elem_prefix, b.n);
   }
 }

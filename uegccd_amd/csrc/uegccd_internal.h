@@ -17,9 +17,13 @@ namespace uegccd {
 
 constexpr int kMaxDiis = 16;
 constexpr int kPartialStride = 4 + kMaxDiis;  // E, dT, max|R|, non-finite count, DIIS dots
-constexpr int kElemThreads = 256;
+constexpr int kElemThreads = 256;                 // threads of the element-wise blocks
+constexpr int kSplit = 4;                         // lanes per amplitude in k_main
+constexpr int kElemPerBlock = kElemThreads / kSplit;  // amplitudes per element block
+constexpr int kFinThreads = 512;                  // threads of the per-solve reduction block
 constexpr int kMaxNao = 1024;                 // orbitals staged in LDS
 constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the Coulomb table
+constexpr int kMaxOcc = 64;                   // NOcc staged per slice in LDS
 
 enum Phase : int { PH_ITER = 0, PH_FINAL = 1, PH_DONE = 2 };
 enum FinKind : int { FIN_ITER = 0, FIN_INIT = 1, FIN_SUM = 2 };
@@ -40,6 +44,7 @@ struct State {
   double coef[kMaxDiis];
   double bmat[kMaxDiis * kMaxDiis];
 };
+static_assert(sizeof(State) % sizeof(double) == 0, "State is copied as doubles");
 
 struct SolveDev {
   int o, v, nao, nK;
@@ -66,16 +71,22 @@ struct SolveDev {
   const int* pair_kb;   // [o*o] block of pair x
   const int* pair_pos;  // [o*o] position of pair x inside its block
   // mutable buffers
-  double* tcur;         // [2*n3] amplitudes entering the next residual pass
-  double* histT;        // [ndiis][2*n3] updated amplitudes
-  double* histE;        // [ndiis][2*n3] DIIS error vectors
+  double2* tcur;        // [n3] (abab, abba) amplitudes entering the next pass
+  double2* histT;       // [n3][ndiis] updated amplitudes, element-major
+  double2* histE;       // [n3][ndiis] DIIS error vectors, element-major
   double* gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
   double* rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
-  double* lad;          // [2*n3] particle-particle ladder, S and P parts
-  double* w;            // [2*wtot] hole-hole W intermediates, S and P parts
+  double2* lad;         // [n3] particle-particle ladder, (S, P) parts
+  double2* w;           // [wtot] hole-hole W intermediates, (S, P) parts
   int wtot;
-  double* jr;           // [6*o*v*o] ring / crossed-ring quadratic intermediates
-  double* jvo;          // [v+o] mosaic Jvv then Joo
+  double4* jring;       // [o*v*o] quadratic ring intermediates (J1, J2, J3, 0)
+  double4* jxring;      // [o*v*o] quadratic crossed-ring intermediates
+  double4* tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
+  double* rsum;         // [v*o] slice row sums
+  double* csum;         // [v*o] slice column sums
+  double* jvv;          // [v] mosaic Jvv
+  double* joo_part;     // [v*o] per-slice parts of the mosaic Joo
+  double* joo;          // [o] mosaic Joo (k_joo)
   double* partials;     // [nblk_elem * kPartialStride]
   int nblk_elem;
   State* st;
@@ -87,12 +98,17 @@ struct BatchDev {
   int n;
   SolveDev* solves;     // device array [n]
   int* elem_prefix;     // [n+1] 256-thread blocks over each solve's n3
-  int* mos_prefix;      // [n+1] v+o blocks
-  int* ring_prefix;     // [n+1] 256-thread blocks over 2*o*v*o
+  int* slice_prefix;    // [n+1] one block per virtual slice
   int4* lad_items;      // (solve, K, row0, col0) ladder tiles
   int nlad;
-  int elem_blocks, mos_blocks, ring_blocks;
+  int elem_blocks, slice_blocks;
+  int slice_lds;        // dynamic LDS bytes of k_slice (largest NOcc in the batch)
+  int joo_lds;          // dynamic LDS bytes of k_joo (largest v*NOcc in the batch)
 };
+
+inline int slice_lds_bytes(int o) { return (3 * o * o + 3 * o) * 8; }
+inline int joo_lds_bytes(int o, int v) { return (v * o > kElemThreads ? v * o : kElemThreads) * 8; }
+constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 
 // Kernel launchers (kernels.hip).
 void launch_init(const BatchDev& b, hipStream_t s);
