@@ -190,6 +190,59 @@ const char* uegccd_strerror(int status);
 int uegccd_solve_batch(uegccd_ctx** ctxs, int32_t n, double** t2abab, double** t2abba,
                        uegccd_result* res);
 
+/* Device bytes a context for this system and these options would hold (the
+ * admission measure of the server).  No counterpart in the reference. */
+int uegccd_footprint(const uegccd_system* sys, const uegccd_options* opt, int64_t* bytes);
+
+/* ------------------------------------------------------------------------
+ * Serving: a request loop that admits solves (twist averaging, r_s scans) as
+ * device memory frees up, keeps each one's amplitudes and DIIS history
+ * resident, and iterates every admitted solve together (continuous batching:
+ * solves join and leave between passes).  Each request's result equals its
+ * solve alone.  No counterpart in the reference, whose twist loop is serial
+ * (source/Main.f90:112-168).  Single-threaded: the caller drives it with
+ * uegccd_server_step / uegccd_server_drain.
+ * ---------------------------------------------------------------------- */
+typedef struct uegccd_server uegccd_server;
+
+typedef struct {
+    int32_t device;
+    int64_t hbm_budget_bytes;  /* admission limit on resident solves (<= 0: 64 GiB) */
+    int32_t max_active;        /* most solves iterated together (<= 0: 64)        */
+    int32_t iters_per_step;    /* passes launched per scheduling step (<= 0: 4)   */
+} uegccd_server_options;
+
+typedef struct {
+    uegccd_heg_params heg;     /* the system; the server builds its basis       */
+    double gapii;              /* the UEGInfo fields of uegccd_system            */
+    int32_t coren, safe_eri;
+    int32_t irange[15];
+    int32_t do_ring, do_xring, do_ladder, do_mosaic;
+    uegccd_options opt;        /* this request's solver options                  */
+} uegccd_request;
+
+enum { UEGCCD_REQ_QUEUED = 0, UEGCCD_REQ_RUNNING = 1, UEGCCD_REQ_DONE = 2 };
+
+typedef struct {
+    int32_t state;             /* UEGCCD_REQ_*                                   */
+    int32_t status;            /* final status of a done request                 */
+    int32_t nao, nocc;
+    int32_t niter, iters_fp32;
+    double ehf, e2, ecorr, maxres;
+    double t_submit_ms, t_start_ms, t_done_ms;   /* server clock                 */
+    int64_t footprint_bytes;
+} uegccd_request_result;
+
+int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out);
+void uegccd_server_destroy(uegccd_server* srv);
+int uegccd_server_submit(uegccd_server* srv, const uegccd_request* req, int64_t* id);
+/* One scheduling step: admit what fits, run iters_per_step passes of every
+ * admitted solve, retire the finished ones. */
+int uegccd_server_step(uegccd_server* srv, int32_t* n_active, int32_t* n_queued);
+int uegccd_server_drain(uegccd_server* srv);
+int uegccd_server_poll(const uegccd_server* srv, int64_t id, uegccd_request_result* res);
+double uegccd_server_clock_ms(const uegccd_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""The request server (continuous batching) against solves run alone.
+
+Bars: every served request's E_c equals the same system solved alone from
+the MP2 guess to 1e-12 Ha with the same iteration count, its E(2) and E_HF
+equal the standalone DrvMBPT / init_heg values; admission never holds more
+solves than max_active nor more bytes than the budget at once; a request
+larger than the budget is refused with UEGCCD_ERR_BUSY.
+"""
+import pytest
+
+from tests.helpers import info_for
+from uegccd_amd import lib as L
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    ("n14_cut5", {}, "fp64"),
+    ("n14_cut5_tw", {}, "fp64"),
+    ("n38_cut9", {}, "fp64"),
+    ("n24_rs5", {}, "fp64"),
+    ("n14_cut4", {"rs": 5.0}, "fp64"),
+    ("n54_cut9", {}, "mixed"),
+    ("n14_cut5", {"twist": (0.25, 0.0, -0.125)}, "fp64"),
+]
+
+
+def _alone(info, prec):
+    from uegccd_amd.driver import CCDContext, init_heg, options
+    heg = init_heg(info)
+    c = CCDContext(heg, info, options(mode="diis", ndiis=8, precision=prec))
+    e2, _, tab, tba = c.mp2()
+    r = c.solve(tab, tba)
+    c.close()
+    return heg, e2, r
+
+
+def _info(name, over):
+    from uegccd_amd.inputfile import parse_input, standard_input
+    from tests.helpers import SYSTEMS
+    nel, cut, rs, tw = SYSTEMS[name]
+    rs = over.get("rs", rs)
+    tw = over.get("twist", tw)
+    extra = {"fcut": 2.0} if name == "n24_rs5" else {}
+    return parse_input(standard_input(nel, cut, rs, tw, **extra))
+
+
+def _overlaps(results):
+    """Largest number of solves, and of bytes, resident at one time."""
+    ev = []
+    for r in results:
+        ev.append((r.t_start_ms, 1, r.footprint_bytes))
+        ev.append((r.t_done_ms, -1, -r.footprint_bytes))
+    ev.sort(key=lambda e: (e[0], e[1]))
+    n = b = nmax = bmax = 0
+    for _, dn, db in ev:
+        n += dn
+        b += db
+        nmax, bmax = max(nmax, n), max(bmax, b)
+    return nmax, bmax
+
+
+def test_served_equals_alone():
+    from uegccd_amd.driver import Server, options
+    print("[server] alone solves", flush=True)
+    ref = []
+    for name, over, prec in MIX:
+        ref.append(_alone(_info(name, over), prec))
+    print("[server] served mix", flush=True)
+    srv = Server(max_active=4, iters_per_step=3)
+    ids = [srv.submit(_info(name, over), options(mode="diis", ndiis=8, precision=prec))
+           for name, over, prec in MIX]
+    srv.drain()
+    out = [srv.poll(i) for i in ids]
+    srv.close()
+    for (name, over, prec), (heg, e2, r), s in zip(MIX, ref, out):
+        assert s.state == L.REQ_DONE and s.status == 0, (name, s)
+        assert s.niter == r.niter, (name, s.niter, r.niter)
+        assert abs(s.ecorr - r.ecorr) < 1e-12, (name, s.ecorr, r.ecorr)
+        assert abs(s.e2 - e2) < 1e-12
+        assert abs(s.ehf - heg.EHF) < 1e-12
+        assert s.nao == heg.NAO and s.nocc == heg.NOcc
+        assert s.iters_fp32 == r.iters_fp32
+        assert s.t_submit_ms <= s.t_start_ms <= s.t_done_ms
+    nmax, _ = _overlaps(out)
+    assert nmax <= 4
+
+
+def test_admission_respects_budget_and_refuses_oversize():
+    from uegccd_amd.driver import Server, footprint, init_heg, options
+    opts = options(mode="diis", ndiis=8)
+    small = [_info("n14_cut5", {"twist": (0.05 * k, 0.0, 0.0)}) for k in range(6)]
+    big = _info("n38_cut9", {})
+    fs = footprint(init_heg(small[0]), small[0], opts)
+    fb = footprint(init_heg(big), big, opts)
+    assert fb > 2 * fs
+    budget = 2 * fs + fs // 2      # two small solves at a time, never the big one
+    print(f"[server] budget {budget} small {fs} big {fb}", flush=True)
+    srv = Server(hbm_budget_bytes=budget, max_active=16, iters_per_step=2)
+    ids = [srv.submit(i, opts) for i in small]
+    bid = srv.submit(big, opts)
+    srv.drain()
+    out = [srv.poll(i) for i in ids]
+    rb = srv.poll(bid)
+    srv.close()
+    assert rb.state == L.REQ_DONE and rb.status == L.ERR_BUSY
+    for s in out:
+        assert s.state == L.REQ_DONE and s.status == 0
+    nmax, bmax = _overlaps(out)
+    assert bmax <= budget and nmax <= 2
+    with pytest.raises(L.UegccdError) as e:
+        Server().poll(12345)
+    assert e.value.status == L.ERR_NOT_FOUND

@@ -72,6 +72,36 @@ def test_no_device_fails_loudly():
     assert e.value.status in (L.ERR_HIP, L.ERR_INVALID)
 
 
+def test_footprint_is_host_only_and_grows_with_history():
+    """uegccd_footprint plans a context without a device: it covers at least
+    the amplitudes, the DIIS history (T and error vectors), the residual
+    outputs and the ring intermediates, and grows by exactly two amplitude
+    vectors (rounded to 256 B) per history entry."""
+    from uegccd_amd.driver import footprint, init_heg, options
+    info = info_for("n54_cut9")
+    heg = init_heg(info)
+    o, v = heg.NOcc, heg.NAO - heg.NOcc
+    n3 = o * o * v
+    b4 = footprint(heg, info, options(mode="diis", ndiis=4))
+    b8 = footprint(heg, info, options(mode="diis", ndiis=8))
+    vec = (16 * n3 + 255) // 256 * 256
+    assert b8 - b4 == 2 * 4 * vec
+    assert b8 >= vec * (1 + 2 * 8 + 2) + 2 * 32 * o * v * o
+    assert b8 < 4 * b4
+    with pytest.raises(L.UegccdError):
+        footprint(heg, info, options(mode="diis", ndiis=17))
+
+
+def test_server_without_device_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from uegccd_amd.driver import Server
+    with pytest.raises(L.UegccdError) as e:
+        Server()
+    assert e.value.status in (L.ERR_HIP, L.ERR_INVALID)
+
+
 # --- Input parser (source/IO.f90) ------------------------------------------
 
 def test_parse_standard_input():

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <new>
 #include <tuple>
 #include <vector>
@@ -216,6 +217,34 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   return UEGCCD_OK;
 }
 
+// Device bytes of a context: the arena ctx_create carves its buffers from.
+size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
+  const size_t n3 = c->n3;
+  const size_t eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
+  const size_t ovo = (size_t)c->o * c->v * c->o;
+  size_t bytes = 0;
+  bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
+  bytes += aligned(sizeof(double) * (c->n2max + 1)) + aligned(sizeof(float) * (c->n2max + 1));
+  bytes += aligned(sizeof(int) * n3);
+  bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
+  bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
+  bytes += 2 * aligned(sizeof(int) * c->o * c->o);
+  bytes += aligned(sizeof(double2) * n3) * (1 + 2 * ndiis + 2);      // tcur, histT, histE, gout, rout
+  bytes += aligned(sizeof(double2) * n3);                              // lad
+  bytes += aligned(sizeof(double2) * std::max(1, c->wtot));            // w
+  bytes += 2 * aligned(sizeof(double4) * std::max<size_t>(1, ovo));    // jring, jxring
+  bytes += aligned(sizeof(double4) * n3);                              // tv
+  bytes += 2 * aligned(sizeof(double) * c->o * c->v);                  // rsum, csum
+  bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
+  bytes += aligned(sizeof(double) * c->o);
+  bytes += aligned(sizeof(double) * eb * kPartialStride);
+  bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
+  bytes += aligned(sizeof(double) * 2 * c->maxiter);
+  bytes += aligned(sizeof(int) * 6) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
+  bytes += 4096;
+  return bytes;
+}
+
 int check_supported(const uegccd_system& s) {
   const EriKind kind = eri_kind(s);
   // CoreN > 0 masks integrals by orbital index (CAS), which the Coulomb table
@@ -318,7 +347,13 @@ const char* uegccd_strerror(int st) {
   }
 }
 
-int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx** out) {
+}  // extern "C"
+
+namespace {
+
+// Checks a system and options and builds the host maps of a context that is
+// not yet on a device.
+int plan_ctx(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx** out) {
   if (!sys || !out || !sys->nvec || !sys->eigen) return UEGCCD_ERR_INVALID;
   *out = nullptr;
   if (sys->nocc < 1 || sys->nao <= sys->nocc || !(sys->L > 0.0)) return UEGCCD_ERR_INVALID;
@@ -329,12 +364,6 @@ int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, ue
   if (opt.mode != UEGCCD_MODE_REFERENCE && opt.mode != UEGCCD_MODE_DIIS) return UEGCCD_ERR_INVALID;
   if (opt.mode == UEGCCD_MODE_DIIS && (opt.ndiis < 2 || opt.ndiis > kMaxDiis)) return UEGCCD_ERR_INVALID;
   if (opt.precision != UEGCCD_PREC_FP64 && opt.precision != UEGCCD_PREC_MIXED) return UEGCCD_ERR_INVALID;
-
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UEGCCD_ERR_HIP;
-  if (opt.device < 0 || opt.device >= ndev) return UEGCCD_ERR_INVALID;
-  HIPCHK(hipSetDevice(opt.device));
-
   uegccd_ctx* c = new (std::nothrow) uegccd_ctx();
   if (!c) return UEGCCD_ERR_ALLOC;
   c->device = opt.device;
@@ -345,37 +374,38 @@ int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, ue
   if (opt.tol_dt > 0.0) c->tolmax = opt.tol_dt;
   if (opt.damping > 0.0) c->df = opt.damping;
   c->maxiter = opt.max_iter > 0 ? opt.max_iter : 1000;
+  *out = c;
+  return UEGCCD_OK;
+}
 
-  const int ndiis = opt.mode == UEGCCD_MODE_DIIS ? opt.ndiis : 1;
+int ndiis_of(const uegccd_options& opt) { return opt.mode == UEGCCD_MODE_DIIS ? opt.ndiis : 1; }
+
+// Puts a planned context on its device: one arena holding every buffer.
+// Deletes c on failure.
+int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
+  *out = nullptr;
+  const uegccd_options opt = c->opt;
+  int rc = UEGCCD_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { delete c; return UEGCCD_ERR_HIP; }
+  if (opt.device < 0 || opt.device >= ndev) { delete c; return UEGCCD_ERR_INVALID; }
+  if (hipSetDevice(opt.device) != hipSuccess) { delete c; return UEGCCD_ERR_HIP; }
+  const int ndiis = ndiis_of(opt);
   const size_t n3 = c->n3;
   const int eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
   const size_t ovo = (size_t)c->o * c->v * c->o;
-  size_t bytes = 0;
-  bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
-  bytes += aligned(sizeof(double) * (c->n2max + 1)) + aligned(sizeof(float) * (c->n2max + 1));
-  bytes += aligned(sizeof(int) * n3);
-  bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
-  bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
-  bytes += 2 * aligned(sizeof(int) * c->o * c->o);
-  bytes += aligned(sizeof(double) * 2 * n3) * (1 + 2 * ndiis + 3);
-  bytes += aligned(sizeof(double) * 2 * std::max(1, c->wtot));
-  bytes += 2 * aligned(sizeof(double4) * std::max<size_t>(1, ovo));
-  bytes += aligned(sizeof(double) * 4 * n3) + aligned(sizeof(double) * 2 * c->o * c->v);
-  bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
-  bytes += aligned(sizeof(double) * c->o) + aligned(sizeof(unsigned));
-  bytes += aligned(sizeof(double) * (size_t)eb * kPartialStride /* slot-major: [q][block] */);
-  bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
-  bytes += aligned(sizeof(double) * 2 * c->maxiter);
-  bytes += aligned(sizeof(int) * 6) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
-  bytes += 4096;
+  const size_t bytes = arena_bytes(c, ndiis);
   if (hipMalloc(&c->arena.base, bytes) != hipSuccess) { delete c; return UEGCCD_ERR_ALLOC; }
   c->arena.size = bytes;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     uegccd_ctx_destroy(c);
     return UEGCCD_ERR_HIP;
   }
-  HIPCHK(hipMemsetAsync(c->arena.base, 0, bytes, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (hipMemsetAsync(c->arena.base, 0, bytes, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    uegccd_ctx_destroy(c);
+    return UEGCCD_ERR_HIP;
+  }
 
   SolveDev& S = c->sd;
   S.o = c->o; S.v = c->v; S.nao = c->nao; S.nK = c->nK; S.n3 = c->n3; S.n2max = c->n2max;
@@ -433,6 +463,29 @@ int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, ue
   if (rc) { uegccd_ctx_destroy(c); return rc; }
   if (c->arena.used > c->arena.size) { uegccd_ctx_destroy(c); return UEGCCD_ERR_ALLOC; }
   *out = c;
+  return UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx** out) {
+  if (!out) return UEGCCD_ERR_INVALID;
+  *out = nullptr;
+  uegccd_ctx* c = nullptr;
+  int rc = plan_ctx(sys, opt_in, &c);
+  if (rc) return rc;
+  return place_ctx(c, sys, out);
+}
+
+int uegccd_footprint(const uegccd_system* sys, const uegccd_options* opt, int64_t* bytes) {
+  if (!bytes) return UEGCCD_ERR_INVALID;
+  uegccd_ctx* c = nullptr;
+  int rc = plan_ctx(sys, opt, &c);
+  if (rc) return rc;
+  *bytes = (int64_t)arena_bytes(c, ndiis_of(c->opt));
+  delete c;
   return UEGCCD_OK;
 }
 
@@ -555,6 +608,7 @@ namespace {
 
 struct BatchScratch {
   char* base = nullptr;
+  size_t cap = 0;
   BatchDev b{};
 };
 
@@ -575,7 +629,13 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
   }
   size_t bytes = aligned(sizeof(SolveDev) * n) + 2 * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(int4) * std::max<size_t>(1, items.size()));
-  if (hipMalloc(&out->base, bytes) != hipSuccess) return UEGCCD_ERR_ALLOC;
+  if (out->cap < bytes) {
+    if (out->base) hipFree(out->base);
+    out->base = nullptr;
+    out->cap = 0;
+    if (hipMalloc(&out->base, bytes) != hipSuccess) return UEGCCD_ERR_ALLOC;
+    out->cap = bytes;
+  }
   DeviceArena a;
   a.base = out->base;
   a.size = bytes;
@@ -884,3 +944,266 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
   }
   return UEGCCD_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Request server: continuous batching of independent solves.  Each step runs
+// a few passes of every admitted solve in shared launches; between steps,
+// finished solves leave (their memory is freed) and queued ones join.  While
+// the passes run, the host builds the next requests' bases and index maps.
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uegccd_request req{};
+  uegccd_request_result res{};
+  uegccd_ctx* ctx = nullptr;       // planned (host maps) or placed on the device
+  bool placed = false;
+  std::vector<int32_t> nvec;
+  std::vector<double> eigen;
+  uegccd_system sys{};
+};
+
+}  // namespace
+
+struct uegccd_server {
+  uegccd_server_options opt{};
+  hipStream_t stream = nullptr;
+  Clock::time_point t0;
+  std::vector<Request> reqs;       // indexed by request id
+  std::vector<int64_t> queue;      // waiting, in arrival order (head at qhead)
+  size_t qhead = 0;
+  std::vector<int64_t> active;     // iterated together
+  size_t active_bytes = 0;
+  BatchScratch batch;
+  bool batch_stale = true;
+  State* h_states = nullptr;       // pinned, one per active solve
+};
+
+namespace {
+
+double server_now(const uegccd_server* s) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - s->t0).count();
+}
+
+void finish_request(uegccd_server* s, Request& r, int status) {
+  r.res.state = UEGCCD_REQ_DONE;
+  r.res.status = status;
+  r.res.t_done_ms = server_now(s);
+  if (r.ctx) {
+    if (r.placed) uegccd_ctx_destroy(r.ctx); else delete r.ctx;
+  }
+  r.ctx = nullptr;
+  r.placed = false;
+  r.nvec.clear();
+  r.nvec.shrink_to_fit();
+  r.eigen.clear();
+  r.eigen.shrink_to_fit();
+}
+
+// Basis, eigenvalues and host maps of a request (init_HEG + SetUpCCScr work).
+int plan_request(Request& r) {
+  const uegccd_request& q = r.req;
+  int32_t nao = 0;
+  int rc = uegccd_heg_build(&q.heg, &nao, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  r.nvec.assign((size_t)nao * 3, 0);
+  r.eigen.assign(nao, 0.0);
+  double sc[8];
+  rc = uegccd_heg_build(&q.heg, &nao, r.nvec.data(), r.eigen.data(), sc);
+  if (rc) return rc;
+  uegccd_system& s = r.sys;
+  s.nocc = q.heg.nel / 2;
+  s.nao = nao;
+  s.nvec = r.nvec.data();
+  s.eigen = r.eigen.data();
+  s.L = sc[0];
+  s.madelung = sc[2];
+  s.rs = q.heg.rs;
+  s.gapii = q.gapii;
+  s.coren = q.coren;
+  s.safe_eri = q.safe_eri;
+  for (int k = 0; k < 15; ++k) s.irange[k] = q.irange[k];
+  s.do_ring = q.do_ring;
+  s.do_xring = q.do_xring;
+  s.do_ladder = q.do_ladder;
+  s.do_mosaic = q.do_mosaic;
+  r.res.nao = nao;
+  r.res.nocc = s.nocc;
+  r.res.ehf = sc[3];
+  rc = plan_ctx(&s, &q.opt, &r.ctx);
+  if (rc) return rc;
+  r.res.footprint_bytes = (int64_t)arena_bytes(r.ctx, ndiis_of(r.ctx->opt));
+  return UEGCCD_OK;
+}
+
+// Admits queued requests in arrival order while slots and memory allow; each
+// admitted solve gets its MP2 guess, E(2) and first dT on the server stream.
+int admit(uegccd_server* s) {
+  const size_t budget = (size_t)s->opt.hbm_budget_bytes;
+  while (s->qhead < s->queue.size() && (int)s->active.size() < s->opt.max_active) {
+    Request& r = s->reqs[s->queue[s->qhead]];
+    if (!r.ctx) {
+      int rc = plan_request(r);
+      if (rc) { finish_request(s, r, rc); ++s->qhead; continue; }
+    }
+    const size_t fp = (size_t)r.res.footprint_bytes;
+    if (fp > budget) { finish_request(s, r, UEGCCD_ERR_BUSY); ++s->qhead; continue; }
+    if (s->active_bytes + fp > budget) break;
+    r.ctx->opt.device = s->opt.device;
+    uegccd_ctx* placed = nullptr;
+    int rc = place_ctx(r.ctx, &r.sys, &placed);
+    r.ctx = nullptr;  // place_ctx owns (and on failure deletes) the plan
+    if (rc == UEGCCD_ERR_ALLOC && !s->active.empty()) break;  // re-planned after frees
+    if (rc) { finish_request(s, r, rc); ++s->qhead; continue; }
+    r.ctx = placed;
+    r.placed = true;
+    const int prec64 = r.ctx->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
+    launch_state_reset(r.ctx->batch, PH_ITER, prec64, s->stream);
+    launch_mp2(r.ctx->batch, s->stream);
+    launch_init(r.ctx->batch, s->stream);
+    launch_finalize(r.ctx->batch, FIN_INIT, s->stream);
+    HIPCHK(hipGetLastError());
+    r.res.state = UEGCCD_REQ_RUNNING;
+    r.res.t_start_ms = server_now(s);
+    s->active.push_back(s->queue[s->qhead]);
+    s->active_bytes += fp;
+    s->batch_stale = true;
+    ++s->qhead;
+  }
+  if (s->qhead > 1024 && s->qhead * 2 > s->queue.size()) {
+    s->queue.erase(s->queue.begin(), s->queue.begin() + s->qhead);
+    s->qhead = 0;
+  }
+  return UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out) {
+  if (!out) return UEGCCD_ERR_INVALID;
+  *out = nullptr;
+  uegccd_server_options o{};
+  if (opt) o = *opt;
+  if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = int64_t(64) << 30;
+  if (o.max_active <= 0) o.max_active = 64;
+  if (o.iters_per_step <= 0) o.iters_per_step = 4;
+  if (o.max_active > 4096) return UEGCCD_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UEGCCD_ERR_HIP;
+  if (o.device < 0 || o.device >= ndev) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(o.device));
+  uegccd_server* s = new (std::nothrow) uegccd_server();
+  if (!s) return UEGCCD_ERR_ALLOC;
+  s->opt = o;
+  s->t0 = Clock::now();
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&s->h_states), sizeof(State) * o.max_active,
+                    hipHostMallocDefault) != hipSuccess) {
+    uegccd_server_destroy(s);
+    return UEGCCD_ERR_HIP;
+  }
+  *out = s;
+  return UEGCCD_OK;
+}
+
+void uegccd_server_destroy(uegccd_server* s) {
+  if (!s) return;
+  hipSetDevice(s->opt.device);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (Request& r : s->reqs)
+    if (r.ctx) {
+      if (r.placed) uegccd_ctx_destroy(r.ctx); else delete r.ctx;
+      r.ctx = nullptr;
+    }
+  if (s->batch.base) hipFree(s->batch.base);
+  if (s->h_states) hipHostFree(s->h_states);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int uegccd_server_submit(uegccd_server* s, const uegccd_request* req, int64_t* id) {
+  if (!s || !req) return UEGCCD_ERR_INVALID;
+  Request r;
+  r.req = *req;
+  r.res.state = UEGCCD_REQ_QUEUED;
+  r.res.t_submit_ms = server_now(s);
+  const int64_t k = (int64_t)s->reqs.size();
+  // a planned request's system points into its own vectors, whose buffers
+  // stay put when the request is moved by this growth
+  s->reqs.push_back(std::move(r));
+  s->queue.push_back(k);
+  if (id) *id = k;
+  return UEGCCD_OK;
+}
+
+int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
+  if (!s) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(s->opt.device));
+  // 1. passes of every admitted solve (solves already done skip their tiles)
+  if (!s->active.empty()) {
+    if (s->batch_stale) {
+      std::vector<uegccd_ctx*> cs;
+      for (int64_t k : s->active) cs.push_back(s->reqs[k].ctx);
+      int rc = make_batch(cs.data(), (int)cs.size(), &s->batch);
+      if (rc) return rc;
+      s->batch_stale = false;
+    }
+    for (int k = 0; k < s->opt.iters_per_step; ++k) launch_iteration(s->batch.b, MF_ITERATE, s->stream);
+    HIPCHK(hipGetLastError());
+  }
+  // 2. admissions, prepared on the host while the passes run
+  int rc = admit(s);
+  if (rc) return rc;
+  // 3. states of every active solve
+  const size_t na = s->active.size();
+  for (size_t k = 0; k < na; ++k)
+    HIPCHK(hipMemcpyAsync(&s->h_states[k], s->reqs[s->active[k]].ctx->d_state, sizeof(State),
+                          hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  // 4. retire finished solves
+  std::vector<int64_t> keep;
+  keep.reserve(na);
+  for (size_t k = 0; k < na; ++k) {
+    Request& r = s->reqs[s->active[k]];
+    const State& st = s->h_states[k];
+    if (st.phase != PH_DONE) { keep.push_back(s->active[k]); continue; }
+    uegccd_result res;
+    const int status = fill_result(r.ctx, st, 0.f, &res);
+    r.res.niter = st.niter;
+    r.res.iters_fp32 = st.iters_fp32;
+    r.res.ecorr = st.ecorr;
+    r.res.e2 = st.e_guess;
+    r.res.maxres = st.maxres;
+    s->active_bytes -= (size_t)r.res.footprint_bytes;
+    finish_request(s, r, status);
+    s->batch_stale = true;
+  }
+  s->active.swap(keep);
+  if (n_active) *n_active = (int32_t)s->active.size();
+  if (n_queued) *n_queued = (int32_t)(s->queue.size() - s->qhead);
+  return UEGCCD_OK;
+}
+
+int uegccd_server_drain(uegccd_server* s) {
+  if (!s) return UEGCCD_ERR_INVALID;
+  int32_t na = 0, nq = 0;
+  do {
+    int rc = uegccd_server_step(s, &na, &nq);
+    if (rc) return rc;
+  } while (na > 0 || nq > 0);
+  return UEGCCD_OK;
+}
+
+int uegccd_server_poll(const uegccd_server* s, int64_t id, uegccd_request_result* res) {
+  if (!s || !res) return UEGCCD_ERR_INVALID;
+  if (id < 0 || id >= (int64_t)s->reqs.size()) return UEGCCD_ERR_NOT_FOUND;
+  *res = s->reqs[id].res;
+  return UEGCCD_OK;
+}
+
+double uegccd_server_clock_ms(const uegccd_server* s) { return s ? server_now(s) : 0.0; }
+
+}  // extern "C"

@@ -851,6 +851,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
       st->niter = 1;
       st->ecorr = e;
       st->ecorr_prev = e;
+      st->e_guess = e;
       st->dt = dt;
       st->slot = 0;
       st->slot_cur = 0;
@@ -965,7 +966,7 @@ __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, in
   st->iters_fp32 = 0;
   st->nonfinite = 0;
   st->converged = 0;
-  st->ecorr = st->ecorr_prev = st->dt = st->maxres = st->maxres_in = 0.0;
+  st->ecorr = st->ecorr_prev = st->dt = st->maxres = st->maxres_in = st->e_guess = 0.0;
 }
 
 // --------------------------------------------------------------------------

@@ -41,6 +41,7 @@ struct State {
   int nonfinite;
   int converged;
   double ecorr, ecorr_prev, dt, maxres, maxres_in;
+  double e_guess;   // energy of the starting amplitudes (E(2) for the MP2 guess)
   double coef[kMaxDiis];
   double bmat[kMaxDiis * kMaxDiis];
 };

@@ -208,7 +208,8 @@ def solve_batch(ctxs, tabs, tbas):
     res = (L.Result * n)()
     st = lib.uegccd_solve_batch(P(*[c.handle.value for c in ctxs]), n,
                                 D(*[L.dptr(t) for t in tabs]), D(*[L.dptr(t) for t in tbas]), res)
-    if st not in (L.OK, L.ERR_NOT_CONVERGED):
+    # per-system failures (not converged, non-finite) are reported in each result
+    if st not in (L.OK, L.ERR_NOT_CONVERGED, L.ERR_NONFINITE):
         raise L.UegccdError(st, "uegccd_solve_batch")
     out = []
     for k in range(n):
@@ -216,6 +217,120 @@ def solve_batch(ctxs, tabs, tbas):
         out.append(SolveResult.from_c(r, tabs[k] + tbas[k], tabs[k], tbas[k],
                                       ctxs[k].trace(r.niter)))
     return out
+
+
+def _system(heg: HEGData, info: UEGInfo):
+    """uegccd_system for a basis; the arrays it points into are returned too."""
+    nvec = np.ascontiguousarray(heg.nVec.T, dtype=np.int32).reshape(-1)
+    eig = np.ascontiguousarray(heg.Eigen, dtype=np.float64)
+    s = L.System()
+    s.nocc, s.nao = heg.NOcc, heg.NAO
+    s.nvec, s.eigen = L.iptr(nvec), L.dptr(eig)
+    s.L, s.madelung, s.rs, s.gapii = heg.L, heg.Madelung, heg.rS, info.GapII
+    s.coren, s.safe_eri = info.CoreN, int(info.SafeERI)
+    for k in range(15):
+        s.irange[k] = int(info.irange[k])
+    s.do_ring, s.do_xring = int(info.DoRing), int(info.DoXRing)
+    s.do_ladder, s.do_mosaic = int(info.DoLadder), int(info.DoMosaic)
+    return s, (nvec, eig)
+
+
+def footprint(heg: HEGData, info: UEGInfo, opts: L.Options | None = None) -> int:
+    """Device bytes a solver context for this basis holds (uegccd_footprint)."""
+    lib = L.load()
+    s, keep = _system(heg, info)
+    o = opts if opts is not None else options()
+    b = ctypes.c_int64()
+    L.check(lib.uegccd_footprint(ctypes.byref(s), ctypes.byref(o), ctypes.byref(b)),
+            "uegccd_footprint")
+    del keep
+    return b.value
+
+
+@dataclass
+class RequestResult:
+    state: int
+    status: int
+    nao: int
+    nocc: int
+    niter: int
+    iters_fp32: int
+    ehf: float
+    e2: float
+    ecorr: float
+    maxres: float
+    t_submit_ms: float
+    t_start_ms: float
+    t_done_ms: float
+    footprint_bytes: int
+
+    @property
+    def latency_ms(self):
+        return self.t_done_ms - self.t_submit_ms
+
+
+class Server:
+    """Continuous-batching request loop (uegccd_server_*): twist averaging and
+    r_s scans submitted as independent solves, iterated together on one GPU."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_active=64, iters_per_step=4):
+        self._lib = L.load()
+        o = L.ServerOptions()
+        o.device, o.hbm_budget_bytes = device, int(hbm_budget_bytes)
+        o.max_active, o.iters_per_step = max_active, iters_per_step
+        h = ctypes.c_void_p()
+        L.check(self._lib.uegccd_server_create(ctypes.byref(o), ctypes.byref(h)),
+                "uegccd_server_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.uegccd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, info: UEGInfo, opts: L.Options | None = None, twist=None) -> int:
+        q = L.Request()
+        p = q.heg
+        p.nel, p.maxkpoint, p.rs = info.NElectron, info.MaxKPoint, info.rSMin
+        tw = info.NTwist if twist is None else twist
+        for d in range(3):
+            p.twist[d] = float(tw[d])
+        p.fcut, p.corefcut = info.fcut, info.corefcut
+        p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+        q.gapii, q.coren, q.safe_eri = info.GapII, info.CoreN, int(info.SafeERI)
+        for k in range(15):
+            q.irange[k] = int(info.irange[k])
+        q.do_ring, q.do_xring = int(info.DoRing), int(info.DoXRing)
+        q.do_ladder, q.do_mosaic = int(info.DoLadder), int(info.DoMosaic)
+        q.opt = opts if opts is not None else options()
+        rid = ctypes.c_int64()
+        L.check(self._lib.uegccd_server_submit(self._h, ctypes.byref(q), ctypes.byref(rid)),
+                "uegccd_server_submit")
+        return rid.value
+
+    def step(self):
+        na, nq = ctypes.c_int32(), ctypes.c_int32()
+        L.check(self._lib.uegccd_server_step(self._h, ctypes.byref(na), ctypes.byref(nq)),
+                "uegccd_server_step")
+        return na.value, nq.value
+
+    def drain(self):
+        L.check(self._lib.uegccd_server_drain(self._h), "uegccd_server_drain")
+
+    def poll(self, rid: int) -> RequestResult:
+        r = L.RequestResult()
+        L.check(self._lib.uegccd_server_poll(self._h, int(rid), ctypes.byref(r)),
+                "uegccd_server_poll")
+        return RequestResult(**{k: getattr(r, k) for k, _ in L.RequestResult._fields_})
+
+    def clock_ms(self) -> float:
+        return self._lib.uegccd_server_clock_ms(self._h)
 
 
 def DrvMBPT(heg: HEGData, info: UEGInfo, ctx: CCDContext | None = None):

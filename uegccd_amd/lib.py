@@ -24,8 +24,12 @@ EXPORTS = (
     "uegccd_default_options", "uegccd_ctx_create", "uegccd_ctx_destroy", "uegccd_ctx_info",
     "uegccd_residual", "uegccd_energy", "uegccd_solve", "uegccd_trace", "uegccd_mp2",
     "uegccd_heg_build", "uegccd_strerror", "uegccd_solve_batch", "uegccd_solve_resident",
-    "uegccd_get_amplitudes", "uegccd_kernel_time",
+    "uegccd_get_amplitudes", "uegccd_kernel_time", "uegccd_footprint",
+    "uegccd_server_create", "uegccd_server_destroy", "uegccd_server_submit",
+    "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
+    "uegccd_server_clock_ms",
 )
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
 
 class System(ctypes.Structure):
@@ -59,6 +63,29 @@ class HegParams(ctypes.Structure):
                 ("twist", ctypes.c_double * 3), ("fcut", ctypes.c_double),
                 ("corefcut", ctypes.c_double), ("madelungfactor", ctypes.c_double),
                 ("gap", ctypes.c_double), ("exchangefactor", ctypes.c_double)]
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
+                ("max_active", ctypes.c_int32), ("iters_per_step", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("heg", HegParams), ("gapii", ctypes.c_double), ("coren", ctypes.c_int32),
+                ("safe_eri", ctypes.c_int32), ("irange", ctypes.c_int32 * 15),
+                ("do_ring", ctypes.c_int32), ("do_xring", ctypes.c_int32),
+                ("do_ladder", ctypes.c_int32), ("do_mosaic", ctypes.c_int32),
+                ("opt", Options)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("nao", ctypes.c_int32), ("nocc", ctypes.c_int32),
+                ("niter", ctypes.c_int32), ("iters_fp32", ctypes.c_int32),
+                ("ehf", ctypes.c_double), ("e2", ctypes.c_double), ("ecorr", ctypes.c_double),
+                ("maxres", ctypes.c_double), ("t_submit_ms", ctypes.c_double),
+                ("t_start_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double),
+                ("footprint_bytes", ctypes.c_int64)]
 
 
 class UegccdError(RuntimeError):
@@ -101,10 +128,23 @@ def load():
     lib.uegccd_solve_resident.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Result)]
     lib.uegccd_get_amplitudes.argtypes = [P, D, D, D]
     lib.uegccd_kernel_time.argtypes = [P, ctypes.c_int32, ctypes.c_int32, D]
+    I64 = ctypes.POINTER(ctypes.c_int64)
+    lib.uegccd_footprint.argtypes = [ctypes.POINTER(System), ctypes.POINTER(Options), I64]
+    lib.uegccd_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]
+    lib.uegccd_server_destroy.argtypes = [P]
+    lib.uegccd_server_destroy.restype = None
+    lib.uegccd_server_submit.argtypes = [P, ctypes.POINTER(Request), I64]
+    lib.uegccd_server_step.argtypes = [P, I32, I32]
+    lib.uegccd_server_drain.argtypes = [P]
+    lib.uegccd_server_poll.argtypes = [P, ctypes.c_int64, ctypes.POINTER(RequestResult)]
+    lib.uegccd_server_clock_ms.argtypes = [P]
+    lib.uegccd_server_clock_ms.restype = ctypes.c_double
     for name in ("uegccd_ctx_create", "uegccd_ctx_info", "uegccd_residual", "uegccd_energy",
                  "uegccd_solve", "uegccd_trace", "uegccd_mp2", "uegccd_heg_build",
                  "uegccd_solve_batch", "uegccd_solve_resident", "uegccd_get_amplitudes",
-                 "uegccd_kernel_time"):
+                 "uegccd_kernel_time", "uegccd_footprint", "uegccd_server_create",
+                 "uegccd_server_submit", "uegccd_server_step", "uegccd_server_drain",
+                 "uegccd_server_poll"):
         getattr(lib, name).restype = ctypes.c_int
     _LIB = lib
     return lib
