@@ -27,6 +27,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from uegccd_amd.twist_stream import request_info, stream_requests  # noqa: E402
+
 METRIC = "CCD solve time & twist-solves/sec at |ΔE|<1e-6 Ha vs FP64; p50/p99 latency"
 WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
@@ -75,6 +77,91 @@ def cpu_baseline(info, niter_ref, budget_s=20.0):
             "solve_s": solve_s}
 
 
+def sequential_fp64(reqs, lib, L, CCDContext, init_heg, options):
+    """Each request alone, one after another: basis, context, MP2 guess and
+    FP64 DIIS(8) solve on the GPU, context freed."""
+    res = L.Result()
+    out = []
+    t0 = time.perf_counter()
+    for q in reqs:
+        info = request_info(q)
+        heg = init_heg(info)
+        ctx = CCDContext(heg, info, options(mode="diis", ndiis=8))
+        st = lib.uegccd_solve_resident(ctx.handle, 1, ctypes.byref(res))
+        out.append((st, res.ecorr, res.niter))
+        ctx.close()
+    return time.perf_counter() - t0, out
+
+
+def served(reqs, arrivals, precision, Server, options, max_active=64, iters_per_step=4):
+    """Requests submitted at their arrival times to the continuous-batching
+    server; returns wall seconds and the per-request results."""
+    srv = Server(max_active=max_active, iters_per_step=iters_per_step)
+    opts = options(mode="diis", ndiis=8, precision=precision)
+    infos = [request_info(q) for q in reqs]
+    ids = []
+    t0 = time.perf_counter()
+    i = 0
+    n = len(reqs)
+    while True:
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i] <= now:
+            ids.append(srv.submit(infos[i], opts))
+            i += 1
+        na, nq = srv.step()
+        if i == n and na == 0 and nq == 0:
+            break
+        if na == 0 and nq == 0 and i < n:
+            time.sleep(max(0.0, min(arrivals[i] - (time.perf_counter() - t0), 0.001)))
+    wall = time.perf_counter() - t0
+    out = [srv.poll(k) for k in ids]
+    srv.close()
+    return wall, out
+
+
+def twist_stream(lib, L, n, rate):
+    from uegccd_amd.driver import CCDContext, Server, init_heg, options
+    reqs, arrivals = stream_requests(n, 1234, rate)
+    seq_s, seq = sequential_fp64(reqs, lib, L, CCDContext, init_heg, options)
+    e64 = np.array([e for _, e, _ in seq])
+    ok64 = np.array([st == L.OK for st, _, _ in seq])
+
+    def summary(wall, out):
+        lat = np.array([r.latency_ms for r in out])
+        de = np.abs(np.array([r.ecorr for r in out]) - e64)[ok64]
+        good = sum(1 for r in out if r.status == L.OK)
+        return {"wall_s": wall, "solves_per_s": len(out) / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "max_abs_dE_vs_fp64": float(de.max()) if de.size else None,
+                "p99_abs_dE_vs_fp64": float(np.percentile(de, 99)) if de.size else None,
+                "converged": good, "fp32_iters": int(sum(r.iters_fp32 for r in out)),
+                "iters": int(sum(r.niter for r in out))}
+
+    batch_wall, batch_out = served(reqs, np.zeros(n), "mixed", Server, options, max_active=n)
+    srv_wall, srv_out = served(reqs, arrivals, "mixed", Server, options)
+    return {
+        "requests": n, "seed": 1234, "poisson_rate_per_s": rate,
+        "sequential_fp64": {"wall_s": seq_s, "solves_per_s": n / seq_s,
+                            "converged": int(ok64.sum())},
+        "batched_mixed": summary(batch_wall, batch_out),
+        "served_mixed": summary(srv_wall, srv_out),
+    }
+
+
+def mixed_c2(lib, L, ctx64_ecorr, heg, info, CCDContext, options, steps):
+    ctx = CCDContext(heg, info, options(mode="diis", ndiis=8, precision="mixed"))
+    res = L.Result()
+    lat = []
+    for k in range(steps + 1):
+        s0 = time.perf_counter()
+        L.check(lib.uegccd_solve_resident(ctx.handle, 1, ctypes.byref(res)), "mixed solve")
+        if k:
+            lat.append(1000.0 * (time.perf_counter() - s0))
+    ctx.close()
+    return {"ms_per_solve": float(np.mean(lat)), "niter": res.niter, "iters_fp32": res.iters_fp32,
+            "ecorr": res.ecorr, "abs_dE_vs_fp64": abs(res.ecorr - ctx64_ecorr)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +169,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--stream", type=int, default=256, help="twist-stream requests (0: skip)")
+    ap.add_argument("--rate", type=float, default=400.0, help="Poisson arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the CCD solve "
@@ -147,6 +236,9 @@ def main():
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dominant}
 
+    mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
+    stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(info, niter_ref)
@@ -167,6 +259,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "mixed_precision": mixed,
+        "twist_stream": stream,
     }
     print(json.dumps(out))
 

@@ -85,6 +85,35 @@ def test_served_equals_alone():
     assert nmax <= 4
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_twist_stream_batch_equals_alone(precision):
+    """Sixteen requests of the seeded twist stream (N 14/38/54, r_s 0.5-5,
+    cutoffs 9/16/25) admitted together: each equals its solve alone.  Uneven
+    sizes in one launch are what exposes any dependence of a solve's
+    arithmetic on what else shares the kernels (LDS, tables, tiles)."""
+    from uegccd_amd.driver import CCDContext, Server, init_heg, options
+    from uegccd_amd.twist_stream import request_info, stream_requests
+    reqs, _ = stream_requests(16, 1234)
+    opts = options(mode="diis", ndiis=8, precision=precision)
+    alone = []
+    for q in reqs:
+        info = request_info(q)
+        c = CCDContext(init_heg(info), info, opts)
+        _, _, tab, tba = c.mp2()
+        alone.append(c.solve(tab, tba))
+        c.close()
+    print(f"[server] stream batch {precision}", flush=True)
+    srv = Server(max_active=16, iters_per_step=4)
+    ids = [srv.submit(request_info(q), opts) for q in reqs]
+    srv.drain()
+    out = [srv.poll(i) for i in ids]
+    srv.close()
+    for q, a, s in zip(reqs, alone, out):
+        assert a.status == 0, q
+        assert s.status == 0 and s.niter == a.niter, (q, s, a.niter)
+        assert abs(s.ecorr - a.ecorr) < 1e-12, (q, s.ecorr, a.ecorr)
+
+
 def test_admission_respects_budget_and_refuses_oversize():
     from uegccd_amd.driver import Server, footprint, init_heg, options
     opts = options(mode="diis", ndiis=8)

@@ -202,7 +202,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       n2max = std::max(n2max, (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) +
                                   (a.z - b.z) * (a.z - b.z));
     }
-  if (n2max >= kMaxN2 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
+  if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
   if (joo_lds_bytes(o, v) > kMaxDynLds) return UEGCCD_ERR_UNSUPPORTED;
   c->n2max = n2max;
   const EriKind kind = eri_kind(s);
@@ -610,6 +610,7 @@ struct BatchScratch {
   char* base = nullptr;
   size_t cap = 0;
   BatchDev b{};
+  State* states = nullptr;  // [n] gathered states, one host copy per poll
 };
 
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
@@ -628,7 +629,8 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
       for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
   }
   size_t bytes = aligned(sizeof(SolveDev) * n) + 2 * aligned(sizeof(int) * (n + 1)) +
-                 aligned(sizeof(int4) * std::max<size_t>(1, items.size()));
+                 aligned(sizeof(int4) * std::max<size_t>(1, items.size())) +
+                 aligned(sizeof(State) * n);
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
     out->base = nullptr;
@@ -645,6 +647,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
   b.elem_prefix = a.take<int>(n + 1);
   b.slice_prefix = a.take<int>(n + 1);
   b.lad_items = a.take<int4>(std::max<size_t>(1, items.size()));
+  out->states = a.take<State>(n);
   b.nlad = (int)items.size();
   b.elem_blocks = ep[n];
   b.slice_blocks = sp[n];
@@ -662,7 +665,8 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
 
 // Runs the iteration of a batch already holding its starting amplitudes in
 // tcur: init (E, dT of the guess), then passes until every system is done.
-int iterate(uegccd_ctx** cs, int n, const BatchDev& b, std::vector<State>& st, float* ms_out) {
+int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vector<State>& st,
+            float* ms_out) {
   uegccd_ctx* c0 = cs[0];
   hipStream_t s = c0->stream;
   const int prec64 = c0->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
@@ -686,8 +690,13 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, std::vector<State>& st, f
     for (int k = 0; k < chunk; ++k) launch_iteration(b, MF_ITERATE, s);
     passes += chunk;
     HIPCHK(hipGetLastError());
-    for (int k = 0; k < n; ++k)
-      HIPCHK(hipMemcpyAsync(&st[k], cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
+    if (gather) {
+      launch_gather_states(b, gather, s);
+      HIPCHK(hipMemcpyAsync(st.data(), gather, sizeof(State) * n, hipMemcpyDeviceToHost, s));
+    } else {
+      for (int k = 0; k < n; ++k)
+        HIPCHK(hipMemcpyAsync(&st[k], cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
+    }
     HIPCHK(hipStreamSynchronize(s));
     all_done = true;
     for (int k = 0; k < n; ++k) all_done = all_done && st[k].phase == PH_DONE;
@@ -757,7 +766,7 @@ extern "C" int uegccd_solve_batch(uegccd_ctx** cs, int32_t n, double** tab, doub
   for (int k = 0; k < n && !rc; ++k) rc = copy_in_amplitudes(cs[k], tab[k], tba[k]);
   std::vector<State> st;
   float ms = 0.f;
-  if (!rc) rc = iterate(cs, n, *b, st, &ms);
+  if (!rc) rc = iterate(cs, n, *b, scratch.states, st, &ms);
   if (scratch.base) {
     hipStreamSynchronize(c0->stream);
     hipFree(scratch.base);
@@ -784,7 +793,7 @@ extern "C" int uegccd_solve_resident(uegccd_ctx* c, int32_t from_mp2, uegccd_res
   std::vector<State> st;
   float ms = 0.f;
   uegccd_ctx* cs[1] = {c};
-  int rc = iterate(cs, 1, c->batch, st, &ms);
+  int rc = iterate(cs, 1, c->batch, nullptr, st, &ms);
   if (rc) return rc;
   return fill_result(c, st[0], ms, res);
 }
@@ -977,7 +986,8 @@ struct uegccd_server {
   size_t active_bytes = 0;
   BatchScratch batch;
   bool batch_stale = true;
-  State* h_states = nullptr;       // pinned, one per active solve
+  size_t nbatch = 0;               // the first nbatch active solves form the batch
+  State* h_states = nullptr;       // pinned, one per batch member
 };
 
 namespace {
@@ -1150,26 +1160,30 @@ int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
       int rc = make_batch(cs.data(), (int)cs.size(), &s->batch);
       if (rc) return rc;
       s->batch_stale = false;
+      s->nbatch = cs.size();
     }
     for (int k = 0; k < s->opt.iters_per_step; ++k) launch_iteration(s->batch.b, MF_ITERATE, s->stream);
+    launch_gather_states(s->batch.b, s->batch.states, s->stream);
+    HIPCHK(hipMemcpyAsync(s->h_states, s->batch.states, sizeof(State) * s->nbatch,
+                          hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipGetLastError());
+  } else {
+    s->nbatch = 0;
   }
-  // 2. admissions, prepared on the host while the passes run
+  // 2. admissions, prepared on the host while the passes run (a new solve is
+  //    never done after its first dT, so it needs no state read this step)
+  const size_t nb = s->nbatch;
   int rc = admit(s);
   if (rc) return rc;
-  // 3. states of every active solve
-  const size_t na = s->active.size();
-  for (size_t k = 0; k < na; ++k)
-    HIPCHK(hipMemcpyAsync(&s->h_states[k], s->reqs[s->active[k]].ctx->d_state, sizeof(State),
-                          hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  // 4. retire finished solves
+  // 3. retire finished solves
+  const size_t na = s->active.size();
   std::vector<int64_t> keep;
   keep.reserve(na);
   for (size_t k = 0; k < na; ++k) {
     Request& r = s->reqs[s->active[k]];
+    if (k >= nb || s->h_states[k].phase != PH_DONE) { keep.push_back(s->active[k]); continue; }
     const State& st = s->h_states[k];
-    if (st.phase != PH_DONE) { keep.push_back(s->active[k]); continue; }
     uegccd_result res;
     const int status = fill_result(r.ctx, st, 0.f, &res);
     r.res.niter = st.niter;

@@ -241,6 +241,10 @@ __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restri
   double* cs = rs + o;
   double* jp = cs + o;
   const double2* tslab = S.tcur + (size_t)o2 * sv;
+  // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
+  // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
+  // so the table reads as zero there (the last entry is always past n2max).
+  for (int k = S.n2max + 1 + threadIdx.x; k < kMaxN2; k += blockDim.x) L.vt[k] = 0.0;
   for (int idx = threadIdx.x; idx < o2; idx += blockDim.x) {
     const double2 t = tslab[idx];
     Tab[idx] = t.x;
@@ -290,7 +294,7 @@ __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restri
     for (int k = 0; k < o; ++k) {
       const int4 nk = L.nv[k];
       const int dx = bx + nk.x, dy = by + nk.y, dz = bz + nk.z;
-      const double y = L.vt[dx * dx + dy * dy + dz * dz];
+      const double y = L.vt[min(dx * dx + dy * dy + dz * dz, kMaxN2 - 1)];
       const double ta = Tab[i + o * k], tb = Tba[i + o * k];
       const double ua = Tab[k + o * i], ub = Tba[k + o * i];
       a1 += (ta + tb) * y;
@@ -969,6 +973,14 @@ __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, in
   st->ecorr = st->ecorr_prev = st->dt = st->maxres = st->maxres_in = st->e_guess = 0.0;
 }
 
+// Copies every solve's state into one contiguous buffer (one host copy per poll).
+__global__ void k_gather_states(const SolveDev* __restrict__ solves, State* __restrict__ out) {
+  constexpr int kStateWords = sizeof(State) / sizeof(double);
+  const double* src = reinterpret_cast<const double*>(solves[blockIdx.x].st);
+  double* dst = reinterpret_cast<double*>(out + blockIdx.x);
+  for (int k = threadIdx.x; k < kStateWords; k += blockDim.x) dst[k] = src[k];
+}
+
 // --------------------------------------------------------------------------
 // Dynamic LDS above the 64 KiB default needs an explicit per-kernel limit.
 static void allow_large_lds() {
@@ -1002,6 +1014,10 @@ void launch_finalize(const BatchDev& b, int kind, hipStream_t s) {
 }
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s) {
   hipLaunchKernelGGL(k_state_reset, dim3(b.n), dim3(64), 0, s, b.solves, phase, prec64);
+}
+
+void launch_gather_states(const BatchDev& b, State* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_states, dim3(b.n), dim3(64), 0, s, b.solves, out);
 }
 
 void launch_one(const BatchDev& b, int which, hipStream_t s) {

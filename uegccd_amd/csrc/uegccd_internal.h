@@ -117,6 +117,7 @@ void launch_mp2(const BatchDev& b, hipStream_t s);
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s);
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
+void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
 // One kernel of the pass: 0 mosaic J, 1 ring J, 2 ladder, 3 main, 4 extrapolation.
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
