@@ -93,6 +93,8 @@ struct uegccd_ctx {
   State* d_state = nullptr;
   double* h_trace = nullptr;
   int last_niter = 0;
+  bool owns_memory = false;  // arena from hipMalloc (else carved from a server pool)
+  bool owns_stream = false;
 };
 
 namespace {
@@ -259,37 +261,30 @@ int check_supported(const uegccd_system& s) {
   return UEGCCD_OK;
 }
 
-int upload_batch_of_one(uegccd_ctx* c) {
-  BatchDev& b = c->batch;
-  b.n = 1;
-  b.solves = c->d_sd;
-  const int eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
-  const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
-  const int sb = need_slice ? c->v : 0;
-  int h_pref[4] = {0, eb, 0, sb};
-  int* d_pref = c->arena.take<int>(4);
-  HIPCHK(hipMemcpy(d_pref, h_pref, sizeof(h_pref), hipMemcpyHostToDevice));
-  b.elem_prefix = d_pref;
-  b.slice_prefix = d_pref + 2;
-  b.elem_blocks = eb;
-  b.slice_blocks = sb;
-  b.slice_lds = slice_lds_bytes(c->o);
-  b.joo_lds = joo_lds_bytes(c->o, c->v);
-  b.nlad = c->sd.do_ladder ? (int)c->lad_items.size() : 0;
-  b.lad_items = c->arena.take<int4>(std::max<size_t>(1, c->lad_items.size()));
-  if (!c->lad_items.empty())
-    HIPCHK(hipMemcpy(b.lad_items, c->lad_items.data(), sizeof(int4) * c->lad_items.size(),
-                     hipMemcpyHostToDevice));
-  return UEGCCD_OK;
-}
-
-template <typename T>
-int upload(uegccd_ctx* c, const std::vector<T>& h, const T** dst) {
-  T* d = c->arena.take<T>(std::max<size_t>(1, h.size()));
-  if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-  *dst = d;
-  return UEGCCD_OK;
-}
+// Host image of an arena's constant section: each host array is written at
+// the offset its device copy will have, so one copy uploads them all.
+struct ArenaImage {
+  DeviceArena* a;
+  std::vector<char>* img;
+  template <typename T>
+  T* reserve(size_t count) {
+    T* d = a->take<T>(std::max<size_t>(1, count));
+    const size_t end = (size_t)(reinterpret_cast<char*>(d) - a->base) +
+                       sizeof(T) * std::max<size_t>(1, count);
+    if (img->size() < end) img->resize(end, 0);
+    return d;
+  }
+  template <typename T>
+  void write(T* d, const T* h, size_t count) {
+    if (count) std::memcpy(img->data() + (reinterpret_cast<char*>(d) - a->base), h, sizeof(T) * count);
+  }
+  template <typename T>
+  const T* put(const std::vector<T>& h) {
+    T* d = reserve<T>(h.size());
+    write(d, h.data(), h.size());
+    return d;
+  }
+};
 
 // Host (i,j,a) arrays of each spin -> interleaved device amplitudes; entries
 // with no virtual momentum partner are zeroed.
@@ -380,34 +375,25 @@ int plan_ctx(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx*
 
 int ndiis_of(const uegccd_options& opt) { return opt.mode == UEGCCD_MODE_DIIS ? opt.ndiis : 1; }
 
-// Puts a planned context on its device: one arena holding every buffer.
-// Deletes c on failure.
-int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
-  *out = nullptr;
-  const uegccd_options opt = c->opt;
-  int rc = UEGCCD_OK;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { delete c; return UEGCCD_ERR_HIP; }
-  if (opt.device < 0 || opt.device >= ndev) { delete c; return UEGCCD_ERR_INVALID; }
-  if (hipSetDevice(opt.device) != hipSuccess) { delete c; return UEGCCD_ERR_HIP; }
+// Lays a planned context out in `bytes` (= arena_bytes) of device memory at
+// `base`: the maps, the device descriptor and the batch-of-one tables form a
+// constant section written into `image` (uploaded by the caller in one copy),
+// the work buffers follow it from byte *work_off on (cleared by the caller).
+void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t bytes,
+                std::vector<char>& image, size_t* work_off) {
+  const uegccd_options& opt = c->opt;
   const int ndiis = ndiis_of(opt);
   const size_t n3 = c->n3;
   const int eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
   const size_t ovo = (size_t)c->o * c->v * c->o;
-  const size_t bytes = arena_bytes(c, ndiis);
-  if (hipMalloc(&c->arena.base, bytes) != hipSuccess) { delete c; return UEGCCD_ERR_ALLOC; }
+  c->arena.base = base;
   c->arena.size = bytes;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    uegccd_ctx_destroy(c);
-    return UEGCCD_ERR_HIP;
-  }
-  if (hipMemsetAsync(c->arena.base, 0, bytes, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess) {
-    uegccd_ctx_destroy(c);
-    return UEGCCD_ERR_HIP;
-  }
+  c->arena.used = 0;
+  image.clear();
+  ArenaImage im{&c->arena, &image};
 
   SolveDev& S = c->sd;
+  S = SolveDev{};
   S.o = c->o; S.v = c->v; S.nao = c->nao; S.nK = c->nK; S.n3 = c->n3; S.n2max = c->n2max;
   S.ndiis = ndiis;
   S.mode = opt.mode;
@@ -421,17 +407,41 @@ int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
   S.tol_e = opt.tol_e;
   S.tol_r = opt.tol_r;
   S.mixed_switch = opt.precision == UEGCCD_PREC_MIXED ? opt.mixed_switch : 0.0;
-  std::vector<float> vt32(c->vtab.begin(), c->vtab.end());
-  if ((rc = upload(c, c->nv4, &S.nv4)) || (rc = upload(c, c->eig, &S.eig)) ||
-      (rc = upload(c, c->vtab, &S.vtab)) || (rc = upload(c, vt32, &S.vtab32)) ||
-      (rc = upload(c, c->pidx, &S.pidx)) || (rc = upload(c, c->kb_poff, &S.kb_poff)) ||
-      (rc = upload(c, c->kb_pairs, &S.kb_pairs)) || (rc = upload(c, c->kb_voff, &S.kb_voff)) ||
-      (rc = upload(c, c->kb_va, &S.kb_va)) || (rc = upload(c, c->kb_vb, &S.kb_vb)) ||
-      (rc = upload(c, c->kb_woff, &S.kb_woff)) || (rc = upload(c, c->pair_kb, &S.pair_kb)) ||
-      (rc = upload(c, c->pair_pos, &S.pair_pos))) {
-    uegccd_ctx_destroy(c);
-    return rc;
-  }
+  const std::vector<float> vt32(c->vtab.begin(), c->vtab.end());
+  S.nv4 = im.put(c->nv4);
+  S.eig = im.put(c->eig);
+  S.vtab = im.put(c->vtab);
+  S.vtab32 = im.put(vt32);
+  S.pidx = im.put(c->pidx);
+  S.kb_poff = im.put(c->kb_poff);
+  S.kb_pairs = im.put(c->kb_pairs);
+  S.kb_voff = im.put(c->kb_voff);
+  S.kb_va = im.put(c->kb_va);
+  S.kb_vb = im.put(c->kb_vb);
+  S.kb_woff = im.put(c->kb_woff);
+  S.pair_kb = im.put(c->pair_kb);
+  S.pair_pos = im.put(c->pair_pos);
+  c->d_sd = im.reserve<SolveDev>(1);
+  // batch of one: element/slice block prefixes and the ladder tiles
+  BatchDev& b = c->batch;
+  const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
+  const int sb = need_slice ? c->v : 0;
+  const int h_pref[4] = {0, eb, 0, sb};
+  int* d_pref = im.reserve<int>(4);
+  im.write(d_pref, h_pref, 4);
+  b.n = 1;
+  b.solves = c->d_sd;
+  b.elem_prefix = d_pref;
+  b.slice_prefix = d_pref + 2;
+  b.elem_blocks = eb;
+  b.slice_blocks = sb;
+  b.slice_lds = slice_lds_bytes(c->o);
+  b.joo_lds = joo_lds_bytes(c->o, c->v);
+  b.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
+  b.lad_items = const_cast<int4*>(im.put(c->lad_items));
+  // work buffers
+  c->arena.used = aligned(c->arena.used);
+  *work_off = c->arena.used;
   S.tcur = c->arena.take<double2>(n3);
   S.histT = c->arena.take<double2>(n3 * ndiis);
   S.histE = c->arena.take<double2>(n3 * ndiis);
@@ -453,15 +463,38 @@ int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
   c->d_state = c->arena.take<State>(1);
   S.st = c->d_state;
   S.trace = c->arena.take<double>(2 * (size_t)c->maxiter);
-  c->d_sd = c->arena.take<SolveDev>(1);
+  im.write(c->d_sd, &S, 1);
+}
+
+// Puts a planned context on its device in memory of its own: one allocation,
+// one copy of the constant section, one clear of the work buffers.  Deletes c
+// on failure.
+int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
+  *out = nullptr;
+  const uegccd_options opt = c->opt;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { delete c; return UEGCCD_ERR_HIP; }
+  if (opt.device < 0 || opt.device >= ndev) { delete c; return UEGCCD_ERR_INVALID; }
+  if (hipSetDevice(opt.device) != hipSuccess) { delete c; return UEGCCD_ERR_HIP; }
+  const size_t bytes = arena_bytes(c, ndiis_of(opt));
+  char* base = nullptr;
+  if (hipMalloc(&base, bytes) != hipSuccess) { delete c; return UEGCCD_ERR_ALLOC; }
+  c->owns_memory = true;
+  std::vector<char> image;
+  size_t work_off = 0;
+  layout_ctx(c, sys, base, bytes, image, &work_off);
   if (c->arena.used > c->arena.size) { uegccd_ctx_destroy(c); return UEGCCD_ERR_ALLOC; }
-  if (hipMemcpy(c->d_sd, &S, sizeof(SolveDev), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     uegccd_ctx_destroy(c);
     return UEGCCD_ERR_HIP;
   }
-  rc = upload_batch_of_one(c);
-  if (rc) { uegccd_ctx_destroy(c); return rc; }
-  if (c->arena.used > c->arena.size) { uegccd_ctx_destroy(c); return UEGCCD_ERR_ALLOC; }
+  c->owns_stream = true;
+  if (hipMemcpyAsync(base, image.data(), image.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemsetAsync(base + work_off, 0, bytes - work_off, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    uegccd_ctx_destroy(c);
+    return UEGCCD_ERR_HIP;
+  }
   *out = c;
   return UEGCCD_OK;
 }
@@ -492,11 +525,11 @@ int uegccd_footprint(const uegccd_system* sys, const uegccd_options* opt, int64_
 void uegccd_ctx_destroy(uegccd_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
-  if (c->stream) {
+  if (c->stream && c->owns_stream) {
     hipStreamSynchronize(c->stream);
     hipStreamDestroy(c->stream);
   }
-  if (c->arena.base) hipFree(c->arena.base);
+  if (c->arena.base && c->owns_memory) hipFree(c->arena.base);
   delete c;
 }
 
@@ -613,14 +646,65 @@ struct BatchScratch {
   State* states = nullptr;  // [n] gathered states, one host copy per poll
 };
 
-int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
-  std::vector<SolveDev> sds(n);
+// Pinned host staging for uploads that are queued on a stream while earlier
+// work still runs: chunks are kept and handed out again after each reset
+// (callers reset only once every copy from them has completed).
+struct PinnedStage {
+  std::vector<std::pair<char*, size_t>> chunks;
+  size_t cur = 0, off = 0, used = 0;
+  void reset() { cur = 0; off = 0; used = 0; }
+  char* get(size_t bytes) {
+    bytes = aligned(bytes);
+    used += bytes;
+    while (cur < chunks.size()) {
+      if (off + bytes <= chunks[cur].second) {
+        char* p = chunks[cur].first + off;
+        off += bytes;
+        return p;
+      }
+      ++cur;
+      off = 0;
+    }
+    const size_t sz = std::max(bytes, size_t(32) << 20);
+    char* p = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), sz, hipHostMallocDefault) != hipSuccess)
+      return nullptr;
+    chunks.push_back({p, sz});
+    cur = chunks.size() - 1;
+    off = bytes;
+    return p;
+  }
+  void release() {
+    for (auto& c : chunks) hipHostFree(c.first);
+    chunks.clear();
+    reset();
+  }
+};
+
+// Copies a host image to the device: queued on `stream` through pinned
+// staging when `stage` is given, else a blocking copy.
+int upload_image(char* dst, const std::vector<char>& image, hipStream_t stream, PinnedStage* stage) {
+  if (image.empty()) return UEGCCD_OK;
+  if (stage) {
+    char* p = stage->get(image.size());
+    if (!p) return UEGCCD_ERR_ALLOC;
+    std::memcpy(p, image.data(), image.size());
+    HIPCHK(hipMemcpyAsync(dst, p, image.size(), hipMemcpyHostToDevice, stream));
+  } else {
+    HIPCHK(hipMemcpy(dst, image.data(), image.size(), hipMemcpyHostToDevice));
+  }
+  return UEGCCD_OK;
+}
+
+// Device tables of a batch: descriptors, block prefixes, ladder tiles, and a
+// state gather buffer, in one (reused) allocation uploaded in one copy.
+int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = nullptr,
+               PinnedStage* stage = nullptr) {
   std::vector<int> ep(n + 1, 0), sp(n + 1, 0);
   std::vector<int4> items;
   int omax = 1;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
-    sds[k] = c->sd;
     ep[k + 1] = ep[k] + (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
     sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
@@ -641,12 +725,15 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
   DeviceArena a;
   a.base = out->base;
   a.size = bytes;
+  std::vector<char> image;
+  ArenaImage im{&a, &image};
   BatchDev& b = out->b;
   b.n = n;
-  b.solves = a.take<SolveDev>(n);
-  b.elem_prefix = a.take<int>(n + 1);
-  b.slice_prefix = a.take<int>(n + 1);
-  b.lad_items = a.take<int4>(std::max<size_t>(1, items.size()));
+  b.solves = im.reserve<SolveDev>(n);
+  for (int k = 0; k < n; ++k) im.write(b.solves + k, &cs[k]->sd, 1);
+  b.elem_prefix = const_cast<int*>(im.put(ep));
+  b.slice_prefix = const_cast<int*>(im.put(sp));
+  b.lad_items = const_cast<int4*>(im.put(items));
   out->states = a.take<State>(n);
   b.nlad = (int)items.size();
   b.elem_blocks = ep[n];
@@ -655,12 +742,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out) {
   int jl = 0;
   for (int k = 0; k < n; ++k) jl = std::max(jl, joo_lds_bytes(cs[k]->o, cs[k]->v));
   b.joo_lds = jl;
-  HIPCHK(hipMemcpy(b.solves, sds.data(), sizeof(SolveDev) * n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b.elem_prefix, ep.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b.slice_prefix, sp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-  if (!items.empty())
-    HIPCHK(hipMemcpy(b.lad_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
-  return UEGCCD_OK;
+  return upload_image(out->base, image, stream, stage);
 }
 
 // Runs the iteration of a batch already holding its starting amplitudes in
@@ -975,9 +1057,74 @@ struct Request {
 
 }  // namespace
 
+namespace {
+
+// Device memory of the server's contexts: 1 GiB slabs (or one slab of the
+// budget when that is smaller), first-fit with coalescing free lists.  Slabs
+// are kept until the server is destroyed.
+struct DevicePool {
+  struct Slab {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free;  // offset -> length
+  };
+  std::vector<Slab> slabs;
+  size_t slab_bytes = size_t(1) << 30;
+
+  char* alloc(size_t bytes) {
+    bytes = aligned(bytes);
+    for (Slab& sl : slabs)
+      for (auto it = sl.free.begin(); it != sl.free.end(); ++it)
+        if (it->second >= bytes) {
+          const size_t off = it->first, len = it->second;
+          sl.free.erase(it);
+          if (len > bytes) sl.free[off + bytes] = len - bytes;
+          return sl.base + off;
+        }
+    Slab sl;
+    sl.size = std::max(slab_bytes, bytes);
+    if (hipMalloc(&sl.base, sl.size) != hipSuccess) return nullptr;
+    if (sl.size > bytes) sl.free[bytes] = sl.size - bytes;
+    slabs.push_back(std::move(sl));
+    return slabs.back().base;
+  }
+  void release(char* p, size_t bytes) {
+    bytes = aligned(bytes);
+    for (Slab& sl : slabs) {
+      if (p < sl.base || p >= sl.base + sl.size) continue;
+      auto it = sl.free.emplace((size_t)(p - sl.base), bytes).first;
+      auto nx = std::next(it);
+      if (nx != sl.free.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        sl.free.erase(nx);
+      }
+      if (it != sl.free.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+          pv->second += it->second;
+          sl.free.erase(it);
+        }
+      }
+      return;
+    }
+  }
+  void clear() {
+    for (Slab& sl : slabs) hipFree(sl.base);
+    slabs.clear();
+  }
+};
+
+// Host bytes staged for uploads per scheduling step before admission waits
+// for the next step.
+constexpr size_t kStagePerStep = size_t(64) << 20;
+
+}  // namespace
+
 struct uegccd_server {
   uegccd_server_options opt{};
   hipStream_t stream = nullptr;
+  DevicePool pool;
+  PinnedStage stage;
   Clock::time_point t0;
   std::vector<Request> reqs;       // indexed by request id
   std::vector<int64_t> queue;      // waiting, in arrival order (head at qhead)
@@ -996,13 +1143,17 @@ double server_now(const uegccd_server* s) {
   return std::chrono::duration<double, std::milli>(Clock::now() - s->t0).count();
 }
 
+void drop_ctx(uegccd_server* s, Request& r) {
+  if (!r.ctx) return;
+  if (r.placed) s->pool.release(r.ctx->arena.base, r.ctx->arena.size);
+  uegccd_ctx_destroy(r.ctx);  // owns neither memory nor stream: host teardown only
+}
+
 void finish_request(uegccd_server* s, Request& r, int status) {
   r.res.state = UEGCCD_REQ_DONE;
   r.res.status = status;
   r.res.t_done_ms = server_now(s);
-  if (r.ctx) {
-    if (r.placed) uegccd_ctx_destroy(r.ctx); else delete r.ctx;
-  }
+  drop_ctx(s, r);
   r.ctx = nullptr;
   r.placed = false;
   r.nvec.clear();
@@ -1060,14 +1211,25 @@ int admit(uegccd_server* s) {
     const size_t fp = (size_t)r.res.footprint_bytes;
     if (fp > budget) { finish_request(s, r, UEGCCD_ERR_BUSY); ++s->qhead; continue; }
     if (s->active_bytes + fp > budget) break;
-    r.ctx->opt.device = s->opt.device;
-    uegccd_ctx* placed = nullptr;
-    int rc = place_ctx(r.ctx, &r.sys, &placed);
-    r.ctx = nullptr;  // place_ctx owns (and on failure deletes) the plan
-    if (rc == UEGCCD_ERR_ALLOC && !s->active.empty()) break;  // re-planned after frees
-    if (rc) { finish_request(s, r, rc); ++s->qhead; continue; }
-    r.ctx = placed;
+    if (!s->active.empty() && s->stage.used >= kStagePerStep) break;
+    char* mem = s->pool.alloc(fp);
+    if (!mem) {
+      if (!s->active.empty()) break;  // retried once solves have left
+      finish_request(s, r, UEGCCD_ERR_ALLOC);
+      ++s->qhead;
+      continue;
+    }
+    uegccd_ctx* c = r.ctx;
+    c->device = s->opt.device;
+    c->opt.device = s->opt.device;
+    c->stream = s->stream;
+    std::vector<char> image;
+    size_t work_off = 0;
+    layout_ctx(c, &r.sys, mem, fp, image, &work_off);
     r.placed = true;
+    int rc = upload_image(mem, image, s->stream, &s->stage);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(mem + work_off, 0, fp - work_off, s->stream));
     const int prec64 = r.ctx->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
     launch_state_reset(r.ctx->batch, PH_ITER, prec64, s->stream);
     launch_mp2(r.ctx->batch, s->stream);
@@ -1109,6 +1271,7 @@ int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out) 
   if (!s) return UEGCCD_ERR_ALLOC;
   s->opt = o;
   s->t0 = Clock::now();
+  s->pool.slab_bytes = std::min(size_t(1) << 30, std::max(size_t(64) << 20, (size_t)o.hbm_budget_bytes));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&s->h_states), sizeof(State) * o.max_active,
                     hipHostMallocDefault) != hipSuccess) {
@@ -1123,11 +1286,12 @@ void uegccd_server_destroy(uegccd_server* s) {
   if (!s) return;
   hipSetDevice(s->opt.device);
   if (s->stream) hipStreamSynchronize(s->stream);
-  for (Request& r : s->reqs)
-    if (r.ctx) {
-      if (r.placed) uegccd_ctx_destroy(r.ctx); else delete r.ctx;
-      r.ctx = nullptr;
-    }
+  for (Request& r : s->reqs) {
+    drop_ctx(s, r);
+    r.ctx = nullptr;
+  }
+  s->pool.clear();
+  s->stage.release();
   if (s->batch.base) hipFree(s->batch.base);
   if (s->h_states) hipHostFree(s->h_states);
   if (s->stream) hipStreamDestroy(s->stream);
@@ -1152,12 +1316,13 @@ int uegccd_server_submit(uegccd_server* s, const uegccd_request* req, int64_t* i
 int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
   if (!s) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(s->opt.device));
+  s->stage.reset();  // every copy staged by the last step has completed
   // 1. passes of every admitted solve (solves already done skip their tiles)
   if (!s->active.empty()) {
     if (s->batch_stale) {
       std::vector<uegccd_ctx*> cs;
       for (int64_t k : s->active) cs.push_back(s->reqs[k].ctx);
-      int rc = make_batch(cs.data(), (int)cs.size(), &s->batch);
+      int rc = make_batch(cs.data(), (int)cs.size(), &s->batch, s->stream, &s->stage);
       if (rc) return rc;
       s->batch_stale = false;
       s->nbatch = cs.size();
