@@ -17,3 +17,10 @@ def fixtures():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "reference_fixtures.json")) as f:
         return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def config_energies():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "config_energies.json")) as f:
+        return json.load(f)

@@ -140,3 +140,36 @@ def test_diis_converges_to_oracle_fixed_point(name, tol):
     assert abs(orc.energy(r.tab + r.tba, r.tab, r.tba) - r.ecorr) < 1e-12
     assert abs(r.ecorr - tight["ecorr"]) < 1e-9
     assert r.niter < tight["niter"]
+
+
+CONFIG_NAMES = ["n14_cut4", "n14_cut5", "n14_cut25", "n38_cut9", "n54_cut9", "n54_cut16"]
+
+
+@pytest.mark.parametrize("name", CONFIG_NAMES)
+def test_config_energy_reference_mode(name, config_energies):
+    """The reference's own iteration on the configured systems (N=14 cutoffs
+    4/5/25, N=38, N=54 cutoffs 9/16): NIter and E_c of the oracle's run of
+    the same loop, committed as fixtures; E_c within 1e-9 Ha."""
+    print(f"[config ref] {name}", flush=True)
+    ref = config_energies[name]
+    heg, ctx = _product(info_for(name))
+    assert heg.NAO == ref["nao"]
+    _, _, tab, tba = ctx.mp2()
+    r = ctx.solve(tab, tba)
+    assert r.status == 0 and not r.failed
+    assert r.niter == ref["niter_reference_mode"]
+    assert abs(r.ecorr - ref["ecorr_reference_mode"]) < 1e-9
+
+
+@pytest.mark.parametrize("name", CONFIG_NAMES)
+@pytest.mark.parametrize("ndiis", [6, 8])
+def test_config_energy_diis(name, ndiis, config_energies):
+    """DIIS(6/8) on the same systems reaches the fixed point of the oracle's
+    damped iteration run to max|dT| < 1e-11: E_c within 1e-9 Ha."""
+    print(f"[config diis] {name} {ndiis}", flush=True)
+    ref = config_energies[name]
+    heg, ctx = _product(info_for(name), mode="diis", ndiis=ndiis)
+    _, _, tab, tba = ctx.mp2()
+    r = ctx.solve(tab, tba)
+    assert r.status == 0 and not r.failed
+    assert abs(r.ecorr - ref["ecorr_tight"]) < 1e-9

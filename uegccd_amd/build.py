@@ -22,7 +22,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libuegccd.so")
 ZCODE = os.path.join(HOST, "ZCode")
 
-HIP_SOURCES = ["kernels.hip", "api.hip", "server.hip"]
+HIP_SOURCES = ["kernels.hip", "api.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     # no packed-FP32 (v_pk_*_f32) instructions: some of their forms have given
@@ -56,8 +56,6 @@ def build_lib():
     objs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([hipcc(), *HIPCC_FLAGS, "-c", path, "-o", obj])
         objs.append(obj)
