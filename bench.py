@@ -33,7 +33,7 @@ METRIC = "CCD solve time & twist-solves/sec at |ΔE|<1e-6 Ha vs FP64; p50/p99 la
 WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
-KERNELS = {0: "k_slice", 2: "k_ladder", 3: "k_main", 4: "k_extrap"}
+KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_extrap"}
 
 
 def block_sums(nvec, nocc):

@@ -149,9 +149,10 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
 
 /* Average device time (microseconds, HIP events on the context's stream) of
  * one kernel of the residual pass over the context's current amplitudes:
- * 0 per-slice ring/mosaic products, 2 ladder GEMMs, 3 residual assembly +
- * update, 4 extrapolation (1 is unused).  No counterpart in the reference;
- * measurement support. */
+ * 0 per-slice ring/mosaic products, 1 contractions (quadratic rings per
+ * momentum transfer, hole-hole ladder tiles, Joo), 2 particle-particle ladder
+ * and W tiles, 3 residual assembly + update, 4 extrapolation.  No counterpart
+ * in the reference; measurement support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
 
 /* Per-iteration (E, dT) of the last solve, index NIter-1 (the lines the

@@ -85,6 +85,8 @@ struct uegccd_ctx {
   std::vector<int4> nv4;
   std::vector<double> eig, vtab;
   std::vector<int4> lad_items;  // (0, K, row0, col0)
+  std::vector<int> qb_roff, qb_rows, qb_neg, qb_d;  // hole-particle sets P(q)
+  std::vector<int4> con_items;  // (0, kind, arg, arg) of k_contract
   // device
   DeviceArena arena;
   SolveDev sd{};           // host copy of the device descriptor
@@ -205,7 +207,62 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
                                   (a.z - b.z) * (a.z - b.z));
     }
   if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
-  if (joo_lds_bytes(o, v) > kMaxDynLds) return UEGCCD_ERR_UNSUPPORTED;
+  if (con_lds_bytes(o) > kMaxDynLds || slice_lds_bytes(o) + (int)sizeof(int4) * kMaxNao +
+      (int)sizeof(double) * (kMaxN2 + 16) > kMaxDynLds)
+    return UEGCCD_ERR_UNSUPPORTED;
+  // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
+  // occurrence over (p, i), for the quadratic ring products
+  {
+    std::map<std::tuple<int, int, int>, int> qid;
+    std::vector<std::vector<int>> qrows;
+    std::vector<int4> qvec;
+    for (int pv = 0; pv < v; ++pv)
+      for (int i = 0; i < o; ++i) {
+        const int4 np = c->nv4[o + pv], ni = c->nv4[i];
+        const auto key = std::make_tuple(np.x - ni.x, np.y - ni.y, np.z - ni.z);
+        auto it = qid.find(key);
+        int q;
+        if (it == qid.end()) {
+          q = (int)qrows.size();
+          qid[key] = q;
+          qrows.emplace_back();
+          qvec.push_back(make_int4(np.x - ni.x, np.y - ni.y, np.z - ni.z, 0));
+        } else {
+          q = it->second;
+        }
+        qrows[q].push_back(i + o * pv);
+      }
+    const int nq = (int)qrows.size();
+    c->qb_roff.assign(nq + 1, 0);
+    c->qb_rows.clear();
+    c->qb_neg.assign(nq, -1);
+    c->qb_d.assign((size_t)nq * o, -1);
+    for (int q = 0; q < nq; ++q) {
+      for (int r : qrows[q]) c->qb_rows.push_back(r);
+      c->qb_roff[q + 1] = (int)c->qb_rows.size();
+      const int4 qq = qvec[q];
+      auto it = qid.find(std::make_tuple(-qq.x, -qq.y, -qq.z));
+      if (it != qid.end()) c->qb_neg[q] = it->second;
+      for (int l = 0; l < o; ++l) {
+        const int4 nl = c->nv4[l];
+        const int p = box[bidx(nl.x + qq.x, nl.y + qq.y, nl.z + qq.z)];
+        c->qb_d[(size_t)q * o + l] = p >= o ? p - o : -1;
+      }
+    }
+    // work items of the contraction pass
+    c->con_items.clear();
+    if (s.do_ring || s.do_xring)
+      for (int q = 0; q < nq; ++q)
+        if (c->qb_neg[q] >= 0) c->con_items.push_back(make_int4(0, CK_QUAD, q, 0));
+    if (s.do_ladder)
+      for (int K = 0; K < c->nK; ++K) {
+        const int noo = c->kb_poff[K + 1] - c->kb_poff[K];
+        const int nvv = c->kb_voff[K + 1] - c->kb_voff[K];
+        const int nt = ((noo + 15) / 16) * ((nvv + 15) / 16);
+        for (int t = 0; t < nt; t += kHHTilesPerBlock) c->con_items.push_back(make_int4(0, CK_HH, K, t));
+      }
+    if (s.do_mosaic) c->con_items.push_back(make_int4(0, CK_JOO, 0, 0));
+  }
   c->n2max = n2max;
   const EriKind kind = eri_kind(s);
   const bool gap_applies = (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_GAP);
@@ -243,6 +300,11 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
   bytes += aligned(sizeof(int) * 6) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
+  bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
+  bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
+           aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
+  bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->con_items.size()));
+  bytes += aligned(sizeof(double4) * n3);                              // quad
   bytes += 4096;
   return bytes;
 }
@@ -421,6 +483,11 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.kb_woff = im.put(c->kb_woff);
   S.pair_kb = im.put(c->pair_kb);
   S.pair_pos = im.put(c->pair_pos);
+  S.nqb = (int)c->qb_neg.size();
+  S.qb_roff = im.put(c->qb_roff);
+  S.qb_rows = im.put(c->qb_rows);
+  S.qb_neg = im.put(c->qb_neg);
+  S.qb_d = im.put(c->qb_d);
   c->d_sd = im.reserve<SolveDev>(1);
   // batch of one: element/slice block prefixes and the ladder tiles
   BatchDev& b = c->batch;
@@ -436,9 +503,11 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.elem_blocks = eb;
   b.slice_blocks = sb;
   b.slice_lds = slice_lds_bytes(c->o);
-  b.joo_lds = joo_lds_bytes(c->o, c->v);
   b.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
   b.lad_items = const_cast<int4*>(im.put(c->lad_items));
+  b.ncon = (int)c->con_items.size();
+  b.con_items = const_cast<int4*>(im.put(c->con_items));
+  b.con_lds = con_lds_bytes(c->o);
   // work buffers
   c->arena.used = aligned(c->arena.used);
   *work_off = c->arena.used;
@@ -453,6 +522,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.jring = c->arena.take<double4>(std::max<size_t>(1, ovo));
   S.jxring = c->arena.take<double4>(std::max<size_t>(1, ovo));
   S.tv = c->arena.take<double4>(n3);
+  S.quad = c->arena.take<double4>(n3);
   S.rsum = c->arena.take<double>((size_t)c->o * c->v);
   S.csum = c->arena.take<double>((size_t)c->o * c->v);
   S.jvv = c->arena.take<double>(c->v);
@@ -701,7 +771,7 @@ int upload_image(char* dst, const std::vector<char>& image, hipStream_t stream, 
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = nullptr,
                PinnedStage* stage = nullptr) {
   std::vector<int> ep(n + 1, 0), sp(n + 1, 0);
-  std::vector<int4> items;
+  std::vector<int4> items, con;
   int omax = 1;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
@@ -711,9 +781,11 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     omax = std::max(omax, c->o);
     if (c->sd.do_ladder)
       for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
+    for (int4 it : c->con_items) con.push_back(make_int4(k, it.y, it.z, it.w));
   }
   size_t bytes = aligned(sizeof(SolveDev) * n) + 2 * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(int4) * std::max<size_t>(1, items.size())) +
+                 aligned(sizeof(int4) * std::max<size_t>(1, con.size())) +
                  aligned(sizeof(State) * n);
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
@@ -734,14 +806,14 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.elem_prefix = const_cast<int*>(im.put(ep));
   b.slice_prefix = const_cast<int*>(im.put(sp));
   b.lad_items = const_cast<int4*>(im.put(items));
+  b.con_items = const_cast<int4*>(im.put(con));
+  b.ncon = (int)con.size();
   out->states = a.take<State>(n);
   b.nlad = (int)items.size();
   b.elem_blocks = ep[n];
   b.slice_blocks = sp[n];
   b.slice_lds = slice_lds_bytes(omax);
-  int jl = 0;
-  for (int k = 0; k < n; ++k) jl = std::max(jl, joo_lds_bytes(cs[k]->o, cs[k]->v));
-  b.joo_lds = jl;
+  b.con_lds = con_lds_bytes(omax);
   return upload_image(out->base, image, stream, stage);
 }
 

@@ -311,31 +311,6 @@ __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restri
   }
 }
 
-// Mosaic Joo(k) = sum over slices of k_slice's partials, one block per solve:
-// the v*o partials are read coalesced into LDS, then summed per k in a fixed
-// order.
-__global__ __launch_bounds__(kElemThreads) void k_joo(const SolveDev* __restrict__ solves) {
-  extern __shared__ double part[];
-  const SolveDev& S = solves[blockIdx.x];
-  if (S.st->phase == PH_DONE || !S.do_mosaic) return;
-  const int o = S.o, nvo = S.v * o;
-  for (int t = threadIdx.x; t < nvo; t += blockDim.x) part[t] = S.joo_part[t];
-  __syncthreads();
-  const int nc = blockDim.x / o;
-  const int t = threadIdx.x, k = t % o, c = t / o;
-  double acc = 0.0;
-  if (c < nc)
-    for (int r = c; r < S.v; r += nc) acc += part[r * o + k];
-  __syncthreads();
-  if (c < nc) part[c * o + k] = acc;
-  __syncthreads();
-  if (t < o) {
-    double sum = 0.0;
-    for (int cc = 0; cc < nc; ++cc) sum += part[cc * o + t];
-    S.joo[t] = sum;
-  }
-}
-
 // --------------------------------------------------------------------------
 // Ladders as GEMMs per pair-momentum block K (source/CCD.f90:611-681).
 //
@@ -459,11 +434,170 @@ __global__ __launch_bounds__(64) void k_ladder(const SolveDev* __restrict__ solv
 }
 
 // --------------------------------------------------------------------------
-// Residual assembly + update: kSplit lanes per dense (i,j,a).  The lanes of
-// an element share its three contraction loops (hole-hole W, quadratic rings,
-// quadratic crossed rings) and combine them with two shuffles; lane 0 of the
-// group then assembles G, R = D*T - G, the damped update, dT, the energy and
-// the DIIS dot products.
+// Contractions that follow the slice and ladder passes (k_contract).
+//
+// Quadratic rings and crossed rings per momentum transfer q.  With
+// P(q) = {(i,p): n_p - n_i = q}, a valid element (i,j,a) with partner b is
+//   ring:         row (i,a) of P(q),  q  = n_a - n_i, column (j,b) of P(-q)
+//   crossed ring: row (i,b) of P(q'), q' = n_b - n_i, column (j,a) of P(-q')
+// and every (row, column) pair of P(q) x P(-q) is a valid element.  So
+//   ring(i,j,a)  = sum_l J(a,i,l) . T(l,j,d_q(l)),  n_d = n_l + q  (source/CCD.f90:801-808)
+//   xring(i,j,a) = sum_l Jx(b,i,l) . T(l,j,a)                      (source/CCD.f90:927-934)
+// are, per q, one product [rows x o] . [o x columns] each: rows and columns
+// are staged in LDS, each output written once (ring to quad.xy, crossed ring
+// to quad.zw of its element).
+__device__ void quad_block(const SolveDev& S, int qb, double* lds) {
+  const int qn = S.qb_neg[qb];
+  if (qn < 0) return;
+  const int o = S.o, o2 = o * o;
+  const int r0 = S.qb_roff[qb], n = S.qb_roff[qb + 1] - r0;
+  const int c0 = S.qb_roff[qn], m = S.qb_roff[qn + 1] - c0;
+  const int* rows = S.qb_rows + r0;
+  const int* cols = S.qb_rows + c0;
+  const int* dq = S.qb_d + (size_t)qb * o;
+  double* J0 = lds;
+  double* J1 = J0 + n * o;
+  double* J2 = J1 + n * o;
+  double* Ta = J2 + n * o;
+  double* Tb = Ta + o * m;
+  const double2* T = S.tcur;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool ring = pass == 0;
+    if (ring ? !S.do_ring : !S.do_xring) continue;
+    const double4* J = ring ? S.jring : S.jxring;
+    for (int idx = threadIdx.x; idx < n * o; idx += blockDim.x) {
+      const int r = idx / o, l = idx - r * o;
+      const int pr = rows[r], i = pr % o, pv = pr / o;
+      const double4 jj = J[(size_t)(pv * o + i) * o + l];
+      J0[idx] = jj.x;
+      J1[idx] = jj.y;
+      J2[idx] = jj.z;
+    }
+    for (int idx = threadIdx.x; idx < o * m; idx += blockDim.x) {
+      const int l = idx / m, c = idx - l * m;
+      const int pc = cols[c], j = pc % o, pv = pc / o;
+      const int tv = ring ? dq[l] : pv;
+      double2 t = make_double2(0.0, 0.0);
+      if (tv >= 0) t = T[l + o * j + (size_t)o2 * tv];
+      Ta[idx] = t.x;
+      Tb[idx] = t.y;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < n * m; idx += blockDim.x) {
+      const int r = idx / m, c = idx - r * m;
+      double qab = 0.0, qba = 0.0;
+      const double* j0 = J0 + r * o;
+      const double* j1 = J1 + r * o;
+      const double* j2 = J2 + r * o;
+      if (ring) {
+        for (int l = 0; l < o; ++l) {
+          const double ta = Ta[l * m + c], tb = Tb[l * m + c];
+          qab += j1[l] * (ta + tb) + j0[l] * ta;
+          qba += j2[l] * tb;
+        }
+      } else {
+        for (int l = 0; l < o; ++l) {
+          const double ta = Ta[l * m + c], tb = Tb[l * m + c];
+          qba = qba - j0[l] * tb - j1[l] * (ta + tb);
+          qab += j2[l] * ta;
+        }
+      }
+      const int pr = rows[r], pc = cols[c];
+      const int i = pr % o, j = pc % o;
+      // ring: a is the row's particle; crossed ring: a is the column's
+      const int av = ring ? pr / o : pc / o;
+      double2* q2 = reinterpret_cast<double2*>(S.quad + (size_t)i + o * j + (size_t)o2 * av);
+      q2[ring ? 0 : 1] = make_double2(qab, qba);
+    }
+    __syncthreads();
+  }
+}
+
+// Hole-hole ladder contraction of one K block (source/CCD.f90:651-656):
+//   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a]
+// one wave64 per 16x16 (pair, a) tile on v_mfma_f64_16x16x4f64, added into
+// the particle-particle ladder of the same elements.
+__device__ void hh_tile(const SolveDev& S, int K, int tile) {
+  typedef MfmaTraits<double>::acc_t acc_t;
+  const int o = S.o, o2 = o * o;
+  const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
+  const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
+  const int nct = (nvv + 15) >> 4;
+  if (tile >= ((noo + 15) >> 4) * nct) return;
+  const int row0 = (tile / nct) * 16, col0 = (tile % nct) * 16;
+  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const double2* W = S.w + S.kb_woff[K];
+  const double2* T = S.tcur;
+  const int rowA = row0 + r16, colB = col0 + r16;
+  const bool rv = rowA < noo, cv = colB < nvv;
+  const int av = cv ? S.kb_va[voff + colB] - o : 0;
+  acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < noo; k0 += 4) {
+    const int q = k0 + kq;
+    double ws = 0.0, wp = 0.0, bs = 0.0, bp = 0.0;
+    if (q < noo) {
+      if (rv) {
+        const double2 w = W[rowA * noo + q];
+        ws = w.x;
+        wp = w.y;
+      }
+      if (cv) {
+        const double2 t = T[S.kb_pairs[poff + q] + (size_t)o2 * av];
+        bs = t.x + t.y;
+        bp = t.x - t.y;
+      }
+    }
+    accs = MfmaTraits<double>::mma(ws, bs, accs);
+    accp = MfmaTraits<double>::mma(wp, bp, accp);
+  }
+  if (!cv) return;
+  for (int r = 0; r < 4; ++r) {
+    const int p = row0 + MfmaTraits<double>::row(kq, r);
+    if (p >= noo) continue;
+    double2* l2 = S.lad + S.kb_pairs[poff + p] + (size_t)o2 * av;
+    const double2 cur = *l2;
+    *l2 = make_double2(cur.x + 0.5 * accs[r], cur.y + 0.5 * accp[r]);
+  }
+}
+
+// Mosaic Joo(k) = sum over slices of k_slice's partials (source/CCD.f90:563-567):
+// column sums read coalesced, then combined in a fixed order.
+__device__ void joo_sum(const SolveDev& S, double* part) {
+  if (!S.do_mosaic) return;
+  const int o = S.o;
+  const int nc = blockDim.x / o;
+  const int t = threadIdx.x, k = t % o, c = t / o;
+  double acc = 0.0;
+  if (c < nc)
+    for (int r = c; r < S.v; r += nc) acc += S.joo_part[r * o + k];
+  if (c < nc) part[c * o + k] = acc;
+  __syncthreads();
+  if (t < o) {
+    double sum = 0.0;
+    for (int cc = 0; cc < nc; ++cc) sum += part[cc * o + t];
+    S.joo[t] = sum;
+  }
+}
+
+__global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
+                                                            const int4* __restrict__ items) {
+  extern __shared__ double lds_con[];
+  const int4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+  const SolveDev& S = solves[it.x];
+  if (S.st->phase == PH_DONE) return;
+  if (it.y == CK_QUAD)
+    quad_block(S, it.z, lds_con);
+  else if (it.y == CK_HH)
+    hh_tile(S, it.z, it.w + (threadIdx.x >> 6));
+  else
+    joo_sum(S, lds_con);
+}
+
+// --------------------------------------------------------------------------
+// Residual assembly + update, one thread per dense (i,j,a): G from drivers,
+// mosaics, ladders (pp + hh from the ladder and contraction passes), rings and
+// crossed rings (slice products and the quadratic sums), then R = D*T - G,
+// the damped update, dT, the energy and the DIIS dot products.
 __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restrict__ solves,
                                                         const int* __restrict__ prefix, int n,
                                                         int flags) {
@@ -476,67 +610,18 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   const int lb = gb - prefix[s];
   const int o = S.o, o2 = o * o, n3 = S.n3;
   stage(S, L);
-  const int part = threadIdx.x % kSplit;
-  const int x = lb * kElemPerBlock + threadIdx.x / kSplit;
+  const int x = lb * kElemPerBlock + threadIdx.x;
   const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
   const int slot = S.st->slot;
   const double2* __restrict__ T = S.tcur;
-
-  // --- contraction loops, split over the kSplit lanes of the element ---
   int b = -1;
   if (x < n3) b = S.pidx[x];
   const bool valid = b >= 0;
   const int i = x % o, j = (x / o) % o, av = x / o2, a = o + av, bv = b - o;
-  double hs = 0.0, hp = 0.0, qab = 0.0, qba = 0.0;
-  if (valid) {
-    if (S.do_ladder) {
-      // hole-hole ladder, sum_q W[p,q] S|P[q,a] (source/CCD.f90:651-656)
-      const int xp = i + o * j;
-      const int K = S.pair_kb[xp], pos = S.pair_pos[xp];
-      const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
-      const double2* W = S.w + S.kb_woff[K] + pos * noo;
-      for (int q = part; q < noo; q += kSplit) {
-        const double2 t = T[S.kb_pairs[poff + q] + o2 * av];
-        const double2 w = W[q];
-        hs += w.x * (t.x + t.y);
-        hp += w.y * (t.x - t.y);
-      }
-    }
-    if (S.do_ring) {
-      // quadratic rings, sum_l J(i,a,l) T(l,j,d), d = FindIndex(J,L,B)
-      // (source/CCD.f90:801-808)
-      const double4* J = S.jring + (size_t)(av * o + i) * o;
-      for (int l = part; l < o; l += kSplit) {
-        const int d = S.pidx[j + o * l + o2 * bv];
-        if (d < 0) continue;
-        const double2 t = T[l + o * j + o2 * (d - o)];
-        const double4 jj = J[l];
-        qab += jj.y * (t.x + t.y) + jj.x * t.x;
-        qba += jj.z * t.y;
-      }
-    }
-    if (S.do_xring) {
-      // quadratic crossed rings, sum_l J(i,b,l) T(l,j,a); T(l,j,a) is zero
-      // unless FindIndex(L,J,A) is virtual (source/CCD.f90:927-934)
-      const double4* J = S.jxring + (size_t)(bv * o + i) * o;
-      for (int l = part; l < o; l += kSplit) {
-        const double2 t = T[l + o * j + o2 * av];
-        const double4 jj = J[l];
-        qba = qba - jj.x * t.y - jj.y * (t.x + t.y);
-        qab += jj.z * t.x;
-      }
-    }
-  }
-  for (int m = 1; m < kSplit; m <<= 1) {
-    hs += __shfl_xor(hs, m, 64);
-    hp += __shfl_xor(hp, m, 64);
-    qab += __shfl_xor(qab, m, 64);
-    qba += __shfl_xor(qba, m, 64);
-  }
 
   double e_part = 0.0, dt_part = 0.0, r_part = 0.0, bad = 0.0;
   double eab = 0.0, eba = 0.0;
-  const bool lead = valid && part == 0;
+  const bool lead = valid;
   if (lead) {
     const double2 t0 = T[x];
     const double t_ab = t0.x, t_ba = t0.y;
@@ -552,13 +637,11 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       gab = gab - 0.5 * (joo[i] * t_ab + joo[j] * t_ab);
       gba = gba - 0.5 * (joo[i] * t_ba + joo[j] * t_ba);
     }
-    // ladders: particle-particle part from k_ladder plus the hole-hole sum
+    // ladders: particle-particle part (k_ladder) with the hole-hole sum (k_contract)
     if (S.do_ladder) {
       const double2 l2 = S.lad[x];
-      const double ls = l2.x + 0.5 * hs;
-      const double lp = l2.y + 0.5 * hp;
-      gab += 0.5 * (ls + lp);
-      gba += 0.5 * (ls - lp);
+      gab += 0.5 * (l2.x + l2.y);
+      gba += 0.5 * (l2.x - l2.y);
     }
     // slice products of k_slice: element (i,j) of slice a, (j,i) of slice b
     if (S.do_ring || S.do_xring) {
@@ -578,8 +661,9 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
         gab -= ta.z;
         gba += VPQ(i, b) * S.csum[av * o + j] - ta.w;
       }
-      gab += qab;
-      gba += qba;
+      const double4 qd = S.quad[x];
+      if (S.do_ring) { gab += qd.x; gba += qd.y; }
+      if (S.do_xring) { gab += qd.z; gba += qd.w; }
     }
     // GetRes + SolveCC + CCEnergy on this element (source/CCD.f90:378-417, 225-227)
     const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
@@ -988,7 +1072,7 @@ static void allow_large_lds() {
   if (done) return;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_slice),
                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds - (int)sizeof(Lds));
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_joo),
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract),
                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds);
   done = true;
 }
@@ -998,7 +1082,13 @@ static void launch_slice(const BatchDev& b, hipStream_t s) {
   allow_large_lds();
   hipLaunchKernelGGL(k_slice, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
                      b.solves, b.slice_prefix, b.n);
-  hipLaunchKernelGGL(k_joo, dim3(b.n), dim3(kElemThreads), b.joo_lds, s, b.solves);
+}
+
+static void launch_contract(const BatchDev& b, hipStream_t s) {
+  if (b.ncon <= 0) return;
+  allow_large_lds();
+  hipLaunchKernelGGL(k_contract, dim3(b.ncon), dim3(kElemThreads), b.con_lds, s, b.solves,
+                     b.con_items);
 }
 
 void launch_init(const BatchDev& b, hipStream_t s) {
@@ -1025,6 +1115,9 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
     case 0:
       launch_slice(b, s);
       break;
+    case 1:
+      launch_contract(b, s);
+      break;
     case 2:
       if (b.nlad > 0)
         hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(64), 0, s, b.solves, b.lad_items, b.nlad);
@@ -1046,6 +1139,7 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s) {
   launch_slice(b, s);
   if (b.nlad > 0)
     hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(64), 0, s, b.solves, b.lad_items, b.nlad);
+  launch_contract(b, s);
   hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
                      b.elem_prefix, b.n, main_flags);
   if (main_flags & MF_ITERATE) {

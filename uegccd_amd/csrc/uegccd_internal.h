@@ -18,8 +18,7 @@ namespace uegccd {
 constexpr int kMaxDiis = 16;
 constexpr int kPartialStride = 4 + kMaxDiis;  // E, dT, max|R|, non-finite count, DIIS dots
 constexpr int kElemThreads = 256;                 // threads of the element-wise blocks
-constexpr int kSplit = 4;                         // lanes per amplitude in k_main
-constexpr int kElemPerBlock = kElemThreads / kSplit;  // amplitudes per element block
+constexpr int kElemPerBlock = kElemThreads;       // amplitudes per element block (one per thread)
 constexpr int kFinThreads = 512;                  // threads of the per-solve reduction block
 constexpr int kMaxNao = 1024;                 // orbitals staged in LDS
 constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the Coulomb table
@@ -71,6 +70,12 @@ struct SolveDev {
   const int* kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
   const int* pair_kb;   // [o*o] block of pair x
   const int* pair_pos;  // [o*o] position of pair x inside its block
+  // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, one per transfer q
+  int nqb;
+  const int* qb_roff;   // [nqb+1] row offsets
+  const int* qb_rows;   // [o*v] pairs i + o*(p - o), grouped by q
+  const int* qb_neg;    // [nqb] block of -q, or -1 when P(-q) is empty
+  const int* qb_d;      // [nqb*o] virtual index of n_l + q for each l, or -1
   // mutable buffers
   double2* tcur;        // [n3] (abab, abba) amplitudes entering the next pass
   double2* histT;       // [n3][ndiis] updated amplitudes, element-major
@@ -83,11 +88,12 @@ struct SolveDev {
   double4* jring;       // [o*v*o] quadratic ring intermediates (J1, J2, J3, 0)
   double4* jxring;      // [o*v*o] quadratic crossed-ring intermediates
   double4* tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
+  double4* quad;        // [n3] quadratic ring (ab, ba) and crossed-ring (ab, ba) sums
   double* rsum;         // [v*o] slice row sums
   double* csum;         // [v*o] slice column sums
   double* jvv;          // [v] mosaic Jvv
   double* joo_part;     // [v*o] per-slice parts of the mosaic Joo
-  double* joo;          // [o] mosaic Joo (k_joo)
+  double* joo;          // [o] mosaic Joo
   double* partials;     // [nblk_elem * kPartialStride]
   int nblk_elem;
   State* st;
@@ -104,11 +110,20 @@ struct BatchDev {
   int nlad;
   int elem_blocks, slice_blocks;
   int slice_lds;        // dynamic LDS bytes of k_slice (largest NOcc in the batch)
-  int joo_lds;          // dynamic LDS bytes of k_joo (largest v*NOcc in the batch)
+  int4* con_items;      // (solve, kind, arg, arg) work items of k_contract
+  int ncon;
+  int con_lds;          // dynamic LDS bytes of k_contract (largest NOcc in the batch)
 };
 
+// k_contract work items: quadratic ring products of one transfer q, four
+// hole-hole ladder tiles of one K block, the mosaic Joo sums of one solve.
+enum ContractKind : int { CK_QUAD = 0, CK_HH = 1, CK_JOO = 2 };
+constexpr int kHHTilesPerBlock = kElemThreads / 64;
+
 inline int slice_lds_bytes(int o) { return (3 * o * o + 3 * o) * 8; }
-inline int joo_lds_bytes(int o, int v) { return (v * o > kElemThreads ? v * o : kElemThreads) * 8; }
+// q-block staging: rows n <= o (J, 3 doubles) and columns m <= o (T, 2 doubles)
+// per l, or the Joo partial sums (one double per thread)
+inline int con_lds_bytes(int o) { return (5 * o * o > kElemThreads ? 5 * o * o : kElemThreads) * 8; }
 constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 
 // Kernel launchers (kernels.hip).
@@ -118,7 +133,7 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s);
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
-// One kernel of the pass: 0 mosaic J, 1 ring J, 2 ladder, 3 main, 4 extrapolation.
+// One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main, 4 extrapolation.
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
 }  // namespace uegccd
