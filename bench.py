@@ -34,6 +34,25 @@ WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_extrap"}
+# Algorithmic HBM bytes per valid amplitude of the element-wise kernels:
+# k_main reads T, the ladder, two slice products, the quadratic sums and the
+# DIIS history (8 entries) and writes one history entry of T and of the error;
+# k_extrap reads 8 history entries and writes T.
+ALGO_BYTES = {"k_main": 16 + 16 + 64 + 32 + 8 * 16 + 32, "k_extrap": 8 * 16 + 16}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_n54_cut16.md")
+
+
+def measured_traffic():
+    """HBM bytes per dispatch by kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return {}
+    block = text.split("```json", 1)
+    if len(block) < 2:
+        return {}
+    return json.loads(block[1].split("```", 1)[0])
 
 
 def block_sums(nvec, nocc):
@@ -54,6 +73,26 @@ def block_sums(nvec, nocc):
         shh += noo * noo * nvv
         nvalid += noo * nvv
     return spp, shh, nvalid
+
+
+def workload_info():
+    """Input of the timed workload: N=54, cutoff 16, r_s=2, Gamma point."""
+    from uegccd_amd.inputfile import parse_input, standard_input
+    return parse_input(standard_input(54, 16, 2.0, (0.0, 0.0, 0.0)))
+
+
+def calibrate_copy():
+    """Three device-to-device copies of 256 MiB: what FETCH_SIZE / WRITE_SIZE
+    read for a known number of bytes."""
+    import torch
+    n = 256 * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copied_bytes": n * 8, "copies": 3}))
 
 
 def cpu_baseline(info, niter_ref, budget_s=20.0):
@@ -171,19 +210,29 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--stream", type=int, default=256, help="twist-stream requests (0: skip)")
     ap.add_argument("--rate", type=float, default=400.0, help="Poisson arrivals per second")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="run only the twist stream (for profiling it)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="three 256 MiB device copies (scale of the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the CCD solve "
               "and the batched twist workload are single-device here.", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+
     import torch
-    from tests.helpers import info_for
     from uegccd_amd import lib as L
     from uegccd_amd.driver import CCDContext, init_heg, options
 
     lib = L.load()
-    info = info_for(WORKLOAD)
+    if args.stream_only:
+        print(json.dumps({"twist_stream": twist_stream(lib, L, args.stream, args.rate)}))
+        return
+    info = workload_info()
     heg = init_heg(info)
     ctx = CCDContext(heg, info, options(mode="diis", ndiis=8, precision=args.precision))
     res = L.Result()
@@ -194,9 +243,29 @@ def main():
             raise L.UegccdError(st, "uegccd_solve_resident")
         return res.ecorr, res.niter, res.solve_ms
 
+    def kernel_stats():
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        L.check(lib.uegccd_kernel_stats(ctx.handle, ctypes.byref(tot), ctypes.byref(cnt)),
+                "uegccd_kernel_stats")
+        return tot.value, cnt.value
+
+    # Per-kernel device time inside whole solves: one solve per kernel with that
+    # kernel bracketed by HIP events on the library's stream in every pass.
+    kt = {}
+    for k, name in KERNELS.items():
+        L.check(lib.uegccd_time_kernel(ctx.handle, k), "uegccd_time_kernel")
+        solve()
+        tot, cnt = kernel_stats()
+        kt[name] = tot / max(cnt, 1)
+    dominant = max(kt, key=kt.get)      # every kernel runs once per pass
+    dom_id = {v: k for k, v in KERNELS.items()}[dominant]
+    L.check(lib.uegccd_time_kernel(ctx.handle, -1), "uegccd_time_kernel")
+
     for _ in range(args.warmup):
         solve()
     torch.cuda.synchronize()
+    # timed region; the dominant kernel stays bracketed by events in every pass
+    L.check(lib.uegccd_time_kernel(ctx.handle, dom_id), "uegccd_time_kernel")
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -206,6 +275,9 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = args.steps / wall
+    dom_total_us, dom_launches = kernel_stats()
+    dom_us = dom_total_us / max(dom_launches, 1)
+    L.check(lib.uegccd_time_kernel(ctx.handle, -1), "uegccd_time_kernel")
 
     # accuracy against the committed FP64 oracle fixed point
     ref = json.load(open(os.path.join(ROOT, "tests", "golden", "config_energies.json"))).get(WORKLOAD)
@@ -217,24 +289,21 @@ def main():
     niter_ref = res.niter
     ref_mode_ms = res.solve_ms
 
-    # kernel roofline (after the timed region, same context, same stream)
-    kt = {}
-    for k in KERNELS:
-        us = ctypes.c_double()
-        L.check(lib.uegccd_kernel_time(ctx.handle, k, 50, ctypes.byref(us)), "uegccd_kernel_time")
-        kt[KERNELS[k]] = us.value
+    # roofline of the dominant kernel, from its live time in the timed region
     spp, shh, nvalid = block_sums(heg.nVec, heg.NOcc)
-    lad_flops = 8.0 * spp + 8.0 * shh      # pp ladder + hh W build per launch
-    dominant = max(kt, key=kt.get)
+    traffic = measured_traffic().get(dominant)
     if dominant == "k_ladder":
-        ach = lad_flops / (kt[dominant] * 1e-6) / 1e12
+        work = 8.0 * spp + 8.0 * shh      # pp ladder + hh W build per launch
+        ach = work / (dom_us * 1e-6) / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": None, "kernel": dominant}
+                "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": traffic}
     else:
-        nbytes = 80.0 * nvalid
-        ach = nbytes / (kt[dominant] * 1e-6) / 1e9
+        work = ALGO_BYTES.get(dominant, 80.0) * nvalid
+        ach = work / (dom_us * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dominant}
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
+    roof.update({"kernel": dominant, "avg_launch_us": dom_us, "launches": dom_launches,
+                 "work_per_launch": work})
 
     mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
     stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
@@ -255,7 +324,7 @@ def main():
                    "abs_dE_vs_fp64_oracle": de, "niter_reference_mode": niter_ref,
                    "reference_mode_solve_ms": ref_mode_ms},
         "p50_ms": float(np.percentile(lat_sorted, 50)), "p99_ms": float(np.percentile(lat_sorted, 99)),
-        "kernel_us": kt,
+        "kernel_us_in_solve": kt,
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,

@@ -155,6 +155,14 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
  * in the reference; measurement support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
 
+/* In-solve timing of one kernel (numbering as uegccd_kernel_time; -1 turns it
+ * off): every pass of the context's later solves brackets that kernel with HIP
+ * events on the context's stream; uegccd_kernel_stats returns the summed
+ * device time and the launch count since the last call, and resets them.  No
+ * counterpart in the reference; measurement support. */
+int uegccd_time_kernel(uegccd_ctx* ctx, int32_t kernel);
+int uegccd_kernel_stats(uegccd_ctx* ctx, double* total_us, int64_t* launches);
+
 /* Per-iteration (E, dT) of the last solve, index NIter-1 (the lines the
  * reference writes with format 1040, source/CCD.f90:76,97). */
 int uegccd_trace(const uegccd_ctx* ctx, double* e, double* dt, int32_t n);

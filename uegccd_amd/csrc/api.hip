@@ -82,7 +82,7 @@ struct uegccd_ctx {
   double tolmax = 1e-8, failratio = 100, df = 2.0;
   // host copies of the maps
   std::vector<int> pidx, kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
-  std::vector<int4> nv4;
+  std::vector<int4> nv4, kb_nva;
   std::vector<double> eig, vtab;
   std::vector<int4> lad_items;  // (0, K, row0, col0)
   std::vector<int> qb_roff, qb_rows, qb_neg, qb_d;  // hole-particle sets P(q)
@@ -95,6 +95,11 @@ struct uegccd_ctx {
   State* d_state = nullptr;
   double* h_trace = nullptr;
   int last_niter = 0;
+  // in-solve timing of one kernel (uegccd_time_kernel): -1 off
+  int timed_kernel = -1;
+  std::vector<hipEvent_t> tev;   // before/after pairs, one per pass of a chunk
+  double timed_us = 0.0;
+  int64_t timed_launches = 0;
   bool owns_memory = false;  // arena from hipMalloc (else carved from a server pool)
   bool owns_stream = false;
 };
@@ -172,6 +177,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   c->kb_pairs.clear();
   c->kb_va.clear();
   c->kb_vb.clear();
+  c->kb_nva.clear();
   for (int K = 0; K < c->nK; ++K) {
     const int x0 = pairs[K][0];
     for (int x : pairs[K]) c->kb_pairs.push_back(x);
@@ -180,6 +186,8 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       if (b >= 0) {
         c->kb_va.push_back(o + av);
         c->kb_vb.push_back(b);
+        const int4 na = c->nv4[o + av];
+        c->kb_nva.push_back(make_int4(na.x, na.y, na.z, o + av));
       }
     }
     c->kb_poff[K + 1] = (int)c->kb_pairs.size();
@@ -287,6 +295,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(int) * n3);
   bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
   bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
+  bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->kb_nva.size()));
   bytes += 2 * aligned(sizeof(int) * c->o * c->o);
   bytes += aligned(sizeof(double2) * n3) * (1 + 2 * ndiis + 2);      // tcur, histT, histE, gout, rout
   bytes += aligned(sizeof(double2) * n3);                              // lad
@@ -480,6 +489,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.kb_voff = im.put(c->kb_voff);
   S.kb_va = im.put(c->kb_va);
   S.kb_vb = im.put(c->kb_vb);
+  S.kb_nva = im.put(c->kb_nva);
   S.kb_woff = im.put(c->kb_woff);
   S.pair_kb = im.put(c->pair_kb);
   S.pair_pos = im.put(c->pair_pos);
@@ -595,6 +605,7 @@ int uegccd_footprint(const uegccd_system* sys, const uegccd_options* opt, int64_
 void uegccd_ctx_destroy(uegccd_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
+  for (hipEvent_t e : c->tev) hipEventDestroy(e);
   if (c->stream && c->owns_stream) {
     hipStreamSynchronize(c->stream);
     hipStreamDestroy(c->stream);
@@ -840,8 +851,23 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   st.assign(n, State{});
   int passes = 0;
   bool all_done = false;
+  const bool timing = c0->timed_kernel >= 0;
+  if (timing && (int)c0->tev.size() < 2 * chunk) {
+    for (size_t k = c0->tev.size(); k < (size_t)(2 * chunk); ++k) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c0->tev.push_back(e);
+    }
+  }
   while (!all_done && passes <= maxiter + 2) {
-    for (int k = 0; k < chunk; ++k) launch_iteration(b, MF_ITERATE, s);
+    for (int k = 0; k < chunk; ++k) {
+      if (timing) {
+        const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
+        launch_iteration(b, MF_ITERATE, s, &t);
+      } else {
+        launch_iteration(b, MF_ITERATE, s);
+      }
+    }
     passes += chunk;
     HIPCHK(hipGetLastError());
     if (gather) {
@@ -852,6 +878,13 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
         HIPCHK(hipMemcpyAsync(&st[k], cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (timing)
+      for (int k = 0; k < chunk; ++k) {
+        float us = 0.f;
+        HIPCHK(hipEventElapsedTime(&us, c0->tev[2 * k], c0->tev[2 * k + 1]));
+        c0->timed_us += 1000.0 * us;
+        c0->timed_launches += 1;
+      }
     all_done = true;
     for (int k = 0; k < n; ++k) all_done = all_done && st[k].phase == PH_DONE;
   }
@@ -956,6 +989,23 @@ extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, do
   if (!c) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(c->device));
   return copy_out_amplitudes(c, taa, tab, tba);
+}
+
+extern "C" int uegccd_time_kernel(uegccd_ctx* c, int32_t which) {
+  if (!c || which < -1 || which > 4) return UEGCCD_ERR_INVALID;
+  c->timed_kernel = which;
+  c->timed_us = 0.0;
+  c->timed_launches = 0;
+  return UEGCCD_OK;
+}
+
+extern "C" int uegccd_kernel_stats(uegccd_ctx* c, double* total_us, int64_t* launches) {
+  if (!c || !total_us || !launches) return UEGCCD_ERR_INVALID;
+  *total_us = c->timed_us;
+  *launches = c->timed_launches;
+  c->timed_us = 0.0;
+  c->timed_launches = 0;
+  return UEGCCD_OK;
 }
 
 extern "C" int uegccd_kernel_time(uegccd_ctx* c, int32_t which, int32_t reps, double* avg_us) {

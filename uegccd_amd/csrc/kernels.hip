@@ -349,17 +349,20 @@ struct MfmaTraits<float> {
   __device__ static int row(int kq, int r) { return 4 * kq + r; }
 };
 
+// One 16x16 tile, the reduction over the K block's virtual list split over
+// the four waves of the block (wave w takes steps w, w+4, ...); the waves'
+// partial sums are combined in a fixed order.  vt is the Coulomb table in LDS.
 template <typename Real>
-__device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
+__device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0, const double* vt,
+                            double* red) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int ncol = nvv + noo;
-  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const double* vt64 = S.vtab;
-  const float* vt32 = S.vtab32;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const double2* T = S.tcur;
+  const int4* cl = S.kb_nva + voff;  // (n_c, c) of the block's virtual list
 
   const int rowA = row0 + r16;
   const int xpA = rowA < noo ? S.kb_pairs[poff + rowA] : -1;
@@ -378,26 +381,19 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
   }
   const int4 nx = S.nv4[cx], ny = S.nv4[cy];
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-  for (int c0 = 0; c0 < nvv; c0 += 4) {
+  for (int c0 = 4 * wave; c0 < nvv; c0 += 16) {
     const int ci = c0 + kq;
     Real sv = 0, pv = 0, bs = 0, bp = 0;
     if (ci < nvv) {
-      const int c = S.kb_va[voff + ci];
-      const int4 nc = S.nv4[c];
+      const int4 nc = cl[ci];
       if (xpA >= 0) {
-        const double2 t = T[(size_t)xpA + (size_t)o2 * (c - o)];
+        const double2 t = T[(size_t)xpA + (size_t)o2 * (nc.w - o)];
         sv = (Real)(t.x + t.y);
         pv = (Real)(t.x - t.y);
       }
       if (colv) {
-        Real vx, vy;
-        if (sizeof(Real) == 8) {
-          vx = (Real)vt64[d2(nc, nx)];
-          vy = (Real)vt64[d2(nc, ny)];
-        } else {
-          vx = (Real)vt32[d2(nc, nx)];
-          vy = (Real)vt32[d2(nc, ny)];
-        }
+        const Real vx = (Real)vt[d2(nc, nx)];
+        const Real vy = (Real)vt[d2(nc, ny)];
         bs = vx - vy;
         bp = vx + vy;
       }
@@ -405,7 +401,22 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
     accs = MfmaTraits<Real>::mma(sv, bs, accs);
     accp = MfmaTraits<Real>::mma(pv, bp, accp);
   }
-  if (!colv) return;
+  if (wave > 0) {
+    double* mine = red + ((wave - 1) * 64 + lane) * 8;
+    for (int r = 0; r < 4; ++r) {
+      mine[r] = (double)accs[r];
+      mine[4 + r] = (double)accp[r];
+    }
+  }
+  __syncthreads();
+  if (wave != 0 || !colv) return;
+  for (int w = 1; w < 4; ++w) {
+    const double* theirs = red + ((w - 1) * 64 + lane) * 8;
+    for (int r = 0; r < 4; ++r) {
+      accs[r] = accs[r] + (Real)theirs[r];
+      accp[r] = accp[r] + (Real)theirs[4 + r];
+    }
+  }
   for (int r = 0; r < 4; ++r) {
     const int rowo = row0 + MfmaTraits<Real>::row(kq, r);
     if (rowo >= noo) continue;
@@ -416,21 +427,25 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0) {
     } else {
       const int q = col - nvv;
       const int i = xpo % o;
-      const double vik = vt64[d2(S.nv4[i], nx)], vil = vt64[d2(S.nv4[i], ny)];
+      const double vik = vt[d2(S.nv4[i], nx)], vil = vt[d2(S.nv4[i], ny)];
       S.w[S.kb_woff[K] + rowo * noo + q] = make_double2(vik - vil + 0.5 * as, vik + vil + 0.5 * ap);
     }
   }
 }
 
-__global__ __launch_bounds__(64) void k_ladder(const SolveDev* __restrict__ solves,
-                                               const int4* __restrict__ items, int nitems) {
-  const int4 it = items[blockIdx.x];
+__global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
+                                                const int4* __restrict__ items, int nitems) {
+  __shared__ double vt[kMaxN2];
+  __shared__ double red[3 * 64 * 8];
+  const int4 it = items[xcd_block(blockIdx.x, gridDim.x)];
   const SolveDev& S = solves[it.x];
   if (S.st->phase == PH_DONE || !S.do_ladder) return;
+  for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
+  __syncthreads();
   if (S.st->prec64)
-    ladder_tile<double>(S, it.y, it.z, it.w);
+    ladder_tile<double>(S, it.y, it.z, it.w, vt, red);
   else
-    ladder_tile<float>(S, it.y, it.z, it.w);
+    ladder_tile<float>(S, it.y, it.z, it.w, vt, red);
 }
 
 // --------------------------------------------------------------------------
@@ -1120,7 +1135,7 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
       break;
     case 2:
       if (b.nlad > 0)
-        hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(64), 0, s, b.solves, b.lad_items, b.nlad);
+        hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(256), 0, s, b.solves, b.lad_items, b.nlad);
       break;
     case 3:
       hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
@@ -1135,17 +1150,30 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
   }
 }
 
-void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s) {
+void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const KernelTimer* timer) {
+  auto mark = [&](int which, bool after) {
+    if (timer && timer->which == which) hipEventRecord(after ? timer->after : timer->before, s);
+  };
+  mark(0, false);
   launch_slice(b, s);
+  mark(0, true);
+  mark(2, false);
   if (b.nlad > 0)
-    hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(64), 0, s, b.solves, b.lad_items, b.nlad);
+    hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(256), 0, s, b.solves, b.lad_items, b.nlad);
+  mark(2, true);
+  mark(1, false);
   launch_contract(b, s);
+  mark(1, true);
+  mark(3, false);
   hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
                      b.elem_prefix, b.n, main_flags);
+  mark(3, true);
   if (main_flags & MF_ITERATE) {
     hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, (int)FIN_ITER);
+    mark(4, false);
     hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
                        b.elem_prefix, b.n);
+    mark(4, true);
   }
 }
 

@@ -67,6 +67,7 @@ struct SolveDev {
   const int* kb_voff;   // [nK+1]
   const int* kb_va;     // virtual columns a (orbital index) grouped by K
   const int* kb_vb;     // their partners b
+  const int4* kb_nva;   // (n_a, a) of the same list
   const int* kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
   const int* pair_kb;   // [o*o] block of pair x
   const int* pair_pos;  // [o*o] position of pair x inside its block
@@ -129,7 +130,13 @@ constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 // Kernel launchers (kernels.hip).
 void launch_init(const BatchDev& b, hipStream_t s);
 void launch_mp2(const BatchDev& b, hipStream_t s);
-void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s);
+// Events bracketing one kernel of a pass (which: as launch_one), or none.
+struct KernelTimer {
+  int which;
+  hipEvent_t before, after;
+};
+void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s,
+                      const KernelTimer* timer = nullptr);
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);

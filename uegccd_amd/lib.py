@@ -27,7 +27,7 @@ EXPORTS = (
     "uegccd_get_amplitudes", "uegccd_kernel_time", "uegccd_footprint",
     "uegccd_server_create", "uegccd_server_destroy", "uegccd_server_submit",
     "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
-    "uegccd_server_clock_ms",
+    "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -137,6 +137,8 @@ def load():
     lib.uegccd_server_step.argtypes = [P, I32, I32]
     lib.uegccd_server_drain.argtypes = [P]
     lib.uegccd_server_poll.argtypes = [P, ctypes.c_int64, ctypes.POINTER(RequestResult)]
+    lib.uegccd_time_kernel.argtypes = [P, ctypes.c_int32]
+    lib.uegccd_kernel_stats.argtypes = [P, D, I64]
     lib.uegccd_server_clock_ms.argtypes = [P]
     lib.uegccd_server_clock_ms.restype = ctypes.c_double
     for name in ("uegccd_ctx_create", "uegccd_ctx_info", "uegccd_residual", "uegccd_energy",
@@ -144,7 +146,7 @@ def load():
                  "uegccd_solve_batch", "uegccd_solve_resident", "uegccd_get_amplitudes",
                  "uegccd_kernel_time", "uegccd_footprint", "uegccd_server_create",
                  "uegccd_server_submit", "uegccd_server_step", "uegccd_server_drain",
-                 "uegccd_server_poll"):
+                 "uegccd_server_poll", "uegccd_time_kernel", "uegccd_kernel_stats"):
         getattr(lib, name).restype = ctypes.c_int
     _LIB = lib
     return lib
