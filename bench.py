@@ -209,7 +209,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--stream", type=int, default=256, help="twist-stream requests (0: skip)")
-    ap.add_argument("--rate", type=float, default=400.0, help="Poisson arrivals per second")
+    ap.add_argument("--rate", type=float, default=700.0, help="Poisson arrivals per second")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the twist stream (for profiling it)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -209,8 +209,9 @@ int uegccd_footprint(const uegccd_system* sys, const uegccd_options* opt, int64_
  * resident, and iterates every admitted solve together (continuous batching:
  * solves join and leave between passes).  Each request's result equals its
  * solve alone.  No counterpart in the reference, whose twist loop is serial
- * (source/Main.f90:112-168).  Single-threaded: the caller drives it with
- * uegccd_server_step / uegccd_server_drain.
+ * (source/Main.f90:112-168).  One caller thread drives it with
+ * uegccd_server_step / uegccd_server_drain; the server's own planner threads
+ * prepare queued requests (basis, maps) meanwhile.
  * ---------------------------------------------------------------------- */
 typedef struct uegccd_server uegccd_server;
 
@@ -219,6 +220,8 @@ typedef struct {
     int64_t hbm_budget_bytes;  /* admission limit on resident solves (<= 0: 64 GiB) */
     int32_t max_active;        /* most solves iterated together (<= 0: 64)        */
     int32_t iters_per_step;    /* passes launched per scheduling step (<= 0: 4)   */
+    int32_t planner_threads;   /* host threads building bases and maps of queued
+                                  requests (0: min(8, cores/2); < 0: none)       */
 } uegccd_server_options;
 
 typedef struct {

@@ -59,14 +59,15 @@ def _overlaps(results):
     return nmax, bmax
 
 
-def test_served_equals_alone():
+@pytest.mark.parametrize("planners", [0, -1], ids=["planner_threads", "inline_planning"])
+def test_served_equals_alone(planners):
     from uegccd_amd.driver import Server, options
-    print("[server] alone solves", flush=True)
+    print(f"[server] alone solves (planners {planners})", flush=True)
     ref = []
     for name, over, prec in MIX:
         ref.append(_alone(_info(name, over), prec))
     print("[server] served mix", flush=True)
-    srv = Server(max_active=4, iters_per_step=3)
+    srv = Server(max_active=4, iters_per_step=3, planner_threads=planners)
     ids = [srv.submit(_info(name, over), options(mode="diis", ndiis=8, precision=prec))
            for name, over, prec in MIX]
     srv.drain()

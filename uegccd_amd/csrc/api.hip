@@ -3,7 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -123,48 +129,47 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->nv4[p] = q;
     mx = std::max({mx, std::abs(q.x), std::abs(q.y), std::abs(q.z)});
   }
-  // momentum -> orbital lookup over a box holding every n_p + n_q - n_r
+  // momentum -> orbital lookup over a box holding every n_p + n_q - n_r.  The
+  // box index is affine in the momentum, so index(n_p + n_q - n_r) =
+  // center + lin[p] + lin[q] - lin[r]: FindIndex is one add and one load.
   const int h = 3 * mx + 1, w = 2 * h + 1;
+  const int center = h + w * h + w * w * h;
   std::vector<int> box((size_t)w * w * w, -1);
-  auto bidx = [&](int x, int y, int z) { return (size_t)(x + h) + (size_t)w * ((y + h) + (size_t)w * (z + h)); };
+  std::vector<int> lin(nao);
   for (int p = 0; p < nao; ++p) {
-    size_t k = bidx(c->nv4[p].x, c->nv4[p].y, c->nv4[p].z);
-    if (box[k] >= 0) return UEGCCD_ERR_INVALID;  // duplicate plane wave
-    box[k] = p;
+    const int4 n = c->nv4[p];
+    lin[p] = n.x + w * n.y + w * w * n.z;
+    int& slot = box[center + lin[p]];
+    if (slot >= 0) return UEGCCD_ERR_INVALID;  // duplicate plane wave
+    slot = p;
   }
-  auto lookup = [&](int m, int n, int l) {
-    const int4 a = c->nv4[m], b = c->nv4[n], d = c->nv4[l];
-    return box[bidx(a.x + b.x - d.x, a.y + b.y - d.y, a.z + b.z - d.z)];
-  };
   // partner map: the FindIndex(i,j,a) > NOcc test of every loop in source/CCD.f90
   c->pidx.assign(c->n3, -1);
   c->nvalid = 0;
   for (int av = 0; av < v; ++av)
-    for (int j = 0; j < o; ++j)
+    for (int j = 0; j < o; ++j) {
+      const int base = center + lin[j] - lin[o + av];
+      int* row = c->pidx.data() + (size_t)o * j + (size_t)o * o * av;
       for (int i = 0; i < o; ++i) {
-        int b = lookup(i, j, o + av);
+        const int b = box[base + lin[i]];
         if (b >= o) {
-          c->pidx[i + o * j + o * o * av] = b;
+          row[i] = b;
           ++c->nvalid;
         }
       }
+    }
   // pair-momentum blocks, in order of first occurrence of K over x = i + o*j
-  std::map<std::tuple<int, int, int>, int> kid;
+  // (block ids looked up by the box index of K)
+  std::vector<int> kid((size_t)w * w * w, -1);
   std::vector<std::vector<int>> pairs;
   c->pair_kb.assign(o * o, 0);
   c->pair_pos.assign(o * o, 0);
   for (int x = 0; x < o * o; ++x) {
     const int i = x % o, j = x / o;
-    auto key = std::make_tuple(c->nv4[i].x + c->nv4[j].x, c->nv4[i].y + c->nv4[j].y,
-                               c->nv4[i].z + c->nv4[j].z);
-    auto it = kid.find(key);
-    int K;
-    if (it == kid.end()) {
+    int& K = kid[center + lin[i] + lin[j]];
+    if (K < 0) {
       K = (int)pairs.size();
-      kid[key] = K;
       pairs.emplace_back();
-    } else {
-      K = it->second;
     }
     c->pair_kb[x] = K;
     c->pair_pos[x] = (int)pairs[K].size();
@@ -221,39 +226,32 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
   // occurrence over (p, i), for the quadratic ring products
   {
-    std::map<std::tuple<int, int, int>, int> qid;
+    std::vector<int> qid((size_t)w * w * w, -1);   // block of transfer q by box index
     std::vector<std::vector<int>> qrows;
-    std::vector<int4> qvec;
+    std::vector<int> qlin;
     for (int pv = 0; pv < v; ++pv)
       for (int i = 0; i < o; ++i) {
-        const int4 np = c->nv4[o + pv], ni = c->nv4[i];
-        const auto key = std::make_tuple(np.x - ni.x, np.y - ni.y, np.z - ni.z);
-        auto it = qid.find(key);
-        int q;
-        if (it == qid.end()) {
+        const int ql = lin[o + pv] - lin[i];
+        int& q = qid[center + ql];
+        if (q < 0) {
           q = (int)qrows.size();
-          qid[key] = q;
           qrows.emplace_back();
-          qvec.push_back(make_int4(np.x - ni.x, np.y - ni.y, np.z - ni.z, 0));
-        } else {
-          q = it->second;
+          qlin.push_back(ql);
         }
         qrows[q].push_back(i + o * pv);
       }
     const int nq = (int)qrows.size();
     c->qb_roff.assign(nq + 1, 0);
     c->qb_rows.clear();
+    c->qb_rows.reserve((size_t)o * v);
     c->qb_neg.assign(nq, -1);
     c->qb_d.assign((size_t)nq * o, -1);
     for (int q = 0; q < nq; ++q) {
       for (int r : qrows[q]) c->qb_rows.push_back(r);
       c->qb_roff[q + 1] = (int)c->qb_rows.size();
-      const int4 qq = qvec[q];
-      auto it = qid.find(std::make_tuple(-qq.x, -qq.y, -qq.z));
-      if (it != qid.end()) c->qb_neg[q] = it->second;
+      c->qb_neg[q] = qid[center - qlin[q]];
       for (int l = 0; l < o; ++l) {
-        const int4 nl = c->nv4[l];
-        const int p = box[bidx(nl.x + qq.x, nl.y + qq.y, nl.z + qq.z)];
+        const int p = box[center + lin[l] + qlin[q]];
         c->qb_d[(size_t)q * o + l] = p >= o ? p - o : -1;
       }
     }
@@ -1167,14 +1165,24 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Plan states: a planner thread (or the scheduling thread itself) claims a
+// request 0 -> 1 and publishes its plan with 1 -> 2 (release); the scheduling
+// thread reads the plan only after seeing 2 (acquire).
+enum PlanState : int { PLAN_NONE = 0, PLAN_BUSY = 1, PLAN_DONE = 2 };
+
 struct Request {
   uegccd_request req{};
-  uegccd_request_result res{};
+  uegccd_request_result res{};     // read by poll (scheduling thread only)
   uegccd_ctx* ctx = nullptr;       // planned (host maps) or placed on the device
   bool placed = false;
   std::vector<int32_t> nvec;
   std::vector<double> eigen;
   uegccd_system sys{};
+  std::atomic<int> plan_state{PLAN_NONE};
+  int plan_rc = UEGCCD_OK;
+  int32_t nao = 0;                 // results of planning, copied into res on admission
+  double ehf = 0.0;
+  int64_t footprint = 0;
 };
 
 }  // namespace
@@ -1248,7 +1256,15 @@ struct uegccd_server {
   DevicePool pool;
   PinnedStage stage;
   Clock::time_point t0;
-  std::vector<Request> reqs;       // indexed by request id
+  std::vector<std::unique_ptr<Request>> reqs;  // indexed by request id
+  // planner threads: build bases and maps of queued requests ahead of admission
+  std::vector<std::thread> planners;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> to_plan;    // guarded by mu
+  int planned_waiting = 0;         // claimed for planning, not yet admitted (mu)
+  int lookahead = 128;
+  bool stopping = false;           // mu
   std::vector<int64_t> queue;      // waiting, in arrival order (head at qhead)
   size_t qhead = 0;
   std::vector<int64_t> active;     // iterated together
@@ -1311,13 +1327,73 @@ int plan_request(Request& r) {
   s.do_xring = q.do_xring;
   s.do_ladder = q.do_ladder;
   s.do_mosaic = q.do_mosaic;
-  r.res.nao = nao;
-  r.res.nocc = s.nocc;
-  r.res.ehf = sc[3];
+  r.nao = nao;
+  r.ehf = sc[3];
   rc = plan_ctx(&s, &q.opt, &r.ctx);
   if (rc) return rc;
-  r.res.footprint_bytes = (int64_t)arena_bytes(r.ctx, ndiis_of(r.ctx->opt));
+  r.footprint = (int64_t)arena_bytes(r.ctx, ndiis_of(r.ctx->opt));
   return UEGCCD_OK;
+}
+
+void planner_loop(uegccd_server* s) {
+  for (;;) {
+    Request* r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] {
+        return s->stopping || (!s->to_plan.empty() && s->planned_waiting < s->lookahead);
+      });
+      if (s->stopping) return;
+      r = s->to_plan.front();
+      s->to_plan.pop_front();
+      int expect = PLAN_NONE;
+      if (!r->plan_state.compare_exchange_strong(expect, PLAN_BUSY)) continue;
+      ++s->planned_waiting;
+    }
+    r->plan_rc = plan_request(*r);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      r->plan_state.store(PLAN_DONE, std::memory_order_release);
+    }
+    s->cv_done.notify_all();
+  }
+}
+
+// The plan of the head request: planned inline when nothing runs (or there
+// are no planner threads), awaited when a planner holds it and nothing runs,
+// else left for a later step.  Returns false when admission should stop.
+bool head_planned(uegccd_server* s, Request& r) {
+  int st = r.plan_state.load(std::memory_order_acquire);
+  const bool idle = s->active.empty();
+  if (st == PLAN_NONE && (idle || s->planners.empty())) {
+    int expect = PLAN_NONE;
+    if (r.plan_state.compare_exchange_strong(expect, PLAN_BUSY)) {
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ++s->planned_waiting;
+      }
+      r.plan_rc = plan_request(r);
+      r.plan_state.store(PLAN_DONE, std::memory_order_release);
+      return true;
+    }
+    st = r.plan_state.load(std::memory_order_acquire);
+  }
+  if (st == PLAN_DONE) return true;
+  if (st == PLAN_BUSY && idle) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return r.plan_state.load(std::memory_order_acquire) == PLAN_DONE; });
+    return true;
+  }
+  return false;
+}
+
+// A planned request leaves the look-ahead window (admitted or finished).
+void release_plan_slot(uegccd_server* s) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    --s->planned_waiting;
+  }
+  s->cv_work.notify_one();
 }
 
 // Admits queued requests in arrival order while slots and memory allow; each
@@ -1325,19 +1401,32 @@ int plan_request(Request& r) {
 int admit(uegccd_server* s) {
   const size_t budget = (size_t)s->opt.hbm_budget_bytes;
   while (s->qhead < s->queue.size() && (int)s->active.size() < s->opt.max_active) {
-    Request& r = s->reqs[s->queue[s->qhead]];
-    if (!r.ctx) {
-      int rc = plan_request(r);
-      if (rc) { finish_request(s, r, rc); ++s->qhead; continue; }
+    Request& r = *s->reqs[s->queue[s->qhead]];
+    if (!head_planned(s, r)) break;
+    r.res.nao = r.nao;
+    r.res.nocc = r.req.heg.nel / 2;
+    r.res.ehf = r.ehf;
+    r.res.footprint_bytes = r.footprint;
+    if (r.plan_rc) {
+      finish_request(s, r, r.plan_rc);
+      release_plan_slot(s);
+      ++s->qhead;
+      continue;
     }
     const size_t fp = (size_t)r.res.footprint_bytes;
-    if (fp > budget) { finish_request(s, r, UEGCCD_ERR_BUSY); ++s->qhead; continue; }
+    if (fp > budget) {
+      finish_request(s, r, UEGCCD_ERR_BUSY);
+      release_plan_slot(s);
+      ++s->qhead;
+      continue;
+    }
     if (s->active_bytes + fp > budget) break;
     if (!s->active.empty() && s->stage.used >= kStagePerStep) break;
     char* mem = s->pool.alloc(fp);
     if (!mem) {
       if (!s->active.empty()) break;  // retried once solves have left
       finish_request(s, r, UEGCCD_ERR_ALLOC);
+      release_plan_slot(s);
       ++s->qhead;
       continue;
     }
@@ -1362,6 +1451,7 @@ int admit(uegccd_server* s) {
     r.res.t_start_ms = server_now(s);
     s->active.push_back(s->queue[s->qhead]);
     s->active_bytes += fp;
+    release_plan_slot(s);
     s->batch_stale = true;
     ++s->qhead;
   }
@@ -1384,7 +1474,7 @@ int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out) 
   if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = int64_t(64) << 30;
   if (o.max_active <= 0) o.max_active = 64;
   if (o.iters_per_step <= 0) o.iters_per_step = 4;
-  if (o.max_active > 4096) return UEGCCD_ERR_INVALID;
+  if (o.max_active > 4096 || o.planner_threads > 64) return UEGCCD_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UEGCCD_ERR_HIP;
   if (o.device < 0 || o.device >= ndev) return UEGCCD_ERR_INVALID;
@@ -1394,6 +1484,10 @@ int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out) 
   s->opt = o;
   s->t0 = Clock::now();
   s->pool.slab_bytes = std::min(size_t(1) << 30, std::max(size_t(64) << 20, (size_t)o.hbm_budget_bytes));
+  s->lookahead = std::min(1024, 2 * o.max_active);
+  int nplan = o.planner_threads;
+  if (nplan == 0) nplan = std::max(1, std::min(8, (int)std::thread::hardware_concurrency() / 2));
+  for (int k = 0; k < nplan; ++k) s->planners.emplace_back(planner_loop, s);
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&s->h_states), sizeof(State) * o.max_active,
                     hipHostMallocDefault) != hipSuccess) {
@@ -1407,10 +1501,16 @@ int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out) 
 void uegccd_server_destroy(uegccd_server* s) {
   if (!s) return;
   hipSetDevice(s->opt.device);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  for (std::thread& t : s->planners) t.join();
   if (s->stream) hipStreamSynchronize(s->stream);
-  for (Request& r : s->reqs) {
-    drop_ctx(s, r);
-    r.ctx = nullptr;
+  for (auto& r : s->reqs) {
+    drop_ctx(s, *r);
+    r->ctx = nullptr;
   }
   s->pool.clear();
   s->stage.release();
@@ -1422,15 +1522,22 @@ void uegccd_server_destroy(uegccd_server* s) {
 
 int uegccd_server_submit(uegccd_server* s, const uegccd_request* req, int64_t* id) {
   if (!s || !req) return UEGCCD_ERR_INVALID;
-  Request r;
-  r.req = *req;
-  r.res.state = UEGCCD_REQ_QUEUED;
-  r.res.t_submit_ms = server_now(s);
+  std::unique_ptr<Request> r(new (std::nothrow) Request());
+  if (!r) return UEGCCD_ERR_ALLOC;
+  r->req = *req;
+  r->res.state = UEGCCD_REQ_QUEUED;
+  r->res.t_submit_ms = server_now(s);
   const int64_t k = (int64_t)s->reqs.size();
-  // a planned request's system points into its own vectors, whose buffers
-  // stay put when the request is moved by this growth
+  Request* rp = r.get();
   s->reqs.push_back(std::move(r));
   s->queue.push_back(k);
+  if (!s->planners.empty()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->to_plan.push_back(rp);
+    }
+    s->cv_work.notify_one();
+  }
   if (id) *id = k;
   return UEGCCD_OK;
 }
@@ -1443,7 +1550,7 @@ int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
   if (!s->active.empty()) {
     if (s->batch_stale) {
       std::vector<uegccd_ctx*> cs;
-      for (int64_t k : s->active) cs.push_back(s->reqs[k].ctx);
+      for (int64_t k : s->active) cs.push_back(s->reqs[k]->ctx);
       int rc = make_batch(cs.data(), (int)cs.size(), &s->batch, s->stream, &s->stage);
       if (rc) return rc;
       s->batch_stale = false;
@@ -1468,7 +1575,7 @@ int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
   std::vector<int64_t> keep;
   keep.reserve(na);
   for (size_t k = 0; k < na; ++k) {
-    Request& r = s->reqs[s->active[k]];
+    Request& r = *s->reqs[s->active[k]];
     if (k >= nb || s->h_states[k].phase != PH_DONE) { keep.push_back(s->active[k]); continue; }
     const State& st = s->h_states[k];
     uegccd_result res;
@@ -1501,7 +1608,7 @@ int uegccd_server_drain(uegccd_server* s) {
 int uegccd_server_poll(const uegccd_server* s, int64_t id, uegccd_request_result* res) {
   if (!s || !res) return UEGCCD_ERR_INVALID;
   if (id < 0 || id >= (int64_t)s->reqs.size()) return UEGCCD_ERR_NOT_FOUND;
-  *res = s->reqs[id].res;
+  *res = s->reqs[id]->res;
   return UEGCCD_OK;
 }
 

@@ -273,11 +273,13 @@ class Server:
     """Continuous-batching request loop (uegccd_server_*): twist averaging and
     r_s scans submitted as independent solves, iterated together on one GPU."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, max_active=64, iters_per_step=4):
+    def __init__(self, device=0, hbm_budget_bytes=0, max_active=64, iters_per_step=4,
+                 planner_threads=0):
         self._lib = L.load()
         o = L.ServerOptions()
         o.device, o.hbm_budget_bytes = device, int(hbm_budget_bytes)
         o.max_active, o.iters_per_step = max_active, iters_per_step
+        o.planner_threads = planner_threads
         h = ctypes.c_void_p()
         L.check(self._lib.uegccd_server_create(ctypes.byref(o), ctypes.byref(h)),
                 "uegccd_server_create")

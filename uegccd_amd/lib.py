@@ -67,7 +67,8 @@ class HegParams(ctypes.Structure):
 
 class ServerOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
-                ("max_active", ctypes.c_int32), ("iters_per_step", ctypes.c_int32)]
+                ("max_active", ctypes.c_int32), ("iters_per_step", ctypes.c_int32),
+                ("planner_threads", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
