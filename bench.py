@@ -34,11 +34,22 @@ WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_extrap"}
-# Algorithmic HBM bytes per valid amplitude of the element-wise kernels:
-# k_main reads T, the ladder, two slice products, the quadratic sums and the
-# DIIS history (8 entries) and writes one history entry of T and of the error;
-# k_extrap reads 8 history entries and writes T.
-ALGO_BYTES = {"k_main": 16 + 16 + 64 + 32 + 8 * 16 + 32, "k_extrap": 8 * 16 + 16}
+def algo_bytes(kernel, o, v, nvalid, sum_noo2):
+    """Algorithmic bytes one launch must move (each datum once), C2 sizes:
+    k_slice reads the T slabs and writes TV/VT and the two J intermediates;
+    k_contract reads the J rows (3 doubles, rings and crossed rings) and the T
+    columns of every transfer q, writes the quadratic sums, reads the hole-hole
+    W blocks and T and updates the ladder; k_main reads T, the ladder, two slice
+    products, the quadratic sums and 8 DIIS error entries and writes one T and
+    one error entry; k_extrap reads 8 history entries and writes T."""
+    n3 = o * o * v
+    return {
+        "k_slice": 16 * n3 + 32 * n3 + 2 * 32 * n3,
+        "k_contract": 2 * 24 * n3 + 2 * 16 * n3 + 32 * nvalid + 16 * sum_noo2 + 16 * nvalid
+                      + 32 * nvalid + 8 * o * v,
+        "k_main": (16 + 16 + 64 + 32 + 8 * 16 + 32) * nvalid,
+        "k_extrap": (8 * 16 + 16) * nvalid,
+    }.get(kernel)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_n54_cut16.md")
 
 
@@ -57,7 +68,7 @@ def measured_traffic():
 
 def block_sums(nvec, nocc):
     """S_pp = sum_K n_oo n_vv^2 and S_hh = sum_K n_oo^2 n_vv over pair momenta K,
-    and the number of valid (i,j,a)."""
+    the number of valid (i,j,a) and sum_K n_oo^2."""
     n = len(nvec)
     key = {tuple(x): p for p, x in enumerate(nvec)}
     blocks = {}
@@ -65,14 +76,15 @@ def block_sums(nvec, nocc):
         for j in range(nocc):
             K = tuple(nvec[i] + nvec[j])
             blocks.setdefault(K, []).append((i, j))
-    spp = shh = nvalid = 0
+    spp = shh = nvalid = noo2 = 0
     for K, pairs in blocks.items():
         nvv = sum(1 for a in range(nocc, n) if key.get(tuple(np.array(K) - nvec[a]), -1) >= nocc)
         noo = len(pairs)
         spp += noo * nvv * nvv
         shh += noo * noo * nvv
         nvalid += noo * nvv
-    return spp, shh, nvalid
+        noo2 += noo * noo
+    return spp, shh, nvalid, noo2
 
 
 def workload_info():
@@ -290,7 +302,7 @@ def main():
     ref_mode_ms = res.solve_ms
 
     # roofline of the dominant kernel, from its live time in the timed region
-    spp, shh, nvalid = block_sums(heg.nVec, heg.NOcc)
+    spp, shh, nvalid, noo2 = block_sums(heg.nVec, heg.NOcc)
     traffic = measured_traffic().get(dominant)
     if dominant == "k_ladder":
         work = 8.0 * spp + 8.0 * shh      # pp ladder + hh W build per launch
@@ -298,7 +310,7 @@ def main():
         roof = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                 "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": traffic}
     else:
-        work = ALGO_BYTES.get(dominant, 80.0) * nvalid
+        work = algo_bytes(dominant, heg.NOcc, heg.NAO - heg.NOcc, nvalid, noo2)
         ach = work / (dom_us * 1e-6) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": traffic}

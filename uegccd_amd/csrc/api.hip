@@ -857,6 +857,7 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
       c0->tev.push_back(e);
     }
   }
+  std::vector<int> prev_niter(n, 1), prev_phase(n, PH_ITER);  // as left by the init
   while (!all_done && passes <= maxiter + 2) {
     for (int k = 0; k < chunk; ++k) {
       if (timing) {
@@ -876,13 +877,24 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
         HIPCHK(hipMemcpyAsync(&st[k], cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
-    if (timing)
-      for (int k = 0; k < chunk; ++k) {
+    if (timing) {
+      // passes of this chunk in which some solve was still iterating: its
+      // NIter advance plus its final residual pass (later passes return at once)
+      int real = 0;
+      for (int k = 0; k < n; ++k) {
+        int r = st[k].niter - prev_niter[k];
+        if (st[k].phase == PH_DONE && prev_phase[k] != PH_DONE && !st[k].nonfinite) r += 1;
+        real = std::max(real, r);
+        prev_niter[k] = st[k].niter;
+        prev_phase[k] = st[k].phase;
+      }
+      for (int k = 0; k < std::min(real, chunk); ++k) {
         float us = 0.f;
         HIPCHK(hipEventElapsedTime(&us, c0->tev[2 * k], c0->tev[2 * k + 1]));
         c0->timed_us += 1000.0 * us;
         c0->timed_launches += 1;
       }
+    }
     all_done = true;
     for (int k = 0; k < n; ++k) all_done = all_done && st[k].phase == PH_DONE;
   }
