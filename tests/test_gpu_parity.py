@@ -173,3 +173,15 @@ def test_config_energy_diis(name, ndiis, config_energies):
     r = ctx.solve(tab, tba)
     assert r.status == 0 and not r.failed
     assert abs(r.ecorr - ref["ecorr_tight"]) < 1e-9
+
+
+@pytest.mark.parametrize("ndiis", [2, 12, 16])
+def test_diis_history_lengths(ndiis, config_energies):
+    """Histories outside the default: 2 (the shortest), 12 and 16 (past the
+    8-entry register solver): the same fixed point within 1e-9 Ha."""
+    print(f"[diis length] {ndiis}", flush=True)
+    heg, ctx = _product(info_for("n54_cut9"), mode="diis", ndiis=ndiis)
+    _, _, tab, tba = ctx.mp2()
+    r = ctx.solve(tab, tba)
+    assert r.status == 0 and not r.failed
+    assert abs(r.ecorr - config_energies["n54_cut9"]["ecorr_tight"]) < 1e-9

@@ -500,7 +500,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   // batch of one: element/slice block prefixes and the ladder tiles
   BatchDev& b = c->batch;
   const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
-  const int sb = need_slice ? c->v : 0;
+  const int sb = need_slice ? c->v * slice_split(c->o) : 0;
   const int h_pref[4] = {0, eb, 0, sb};
   int* d_pref = im.reserve<int>(4);
   im.write(d_pref, h_pref, 4);
@@ -718,6 +718,8 @@ int uegccd_solve(uegccd_ctx* c, double* taa, double* tab, double* tba, uegccd_re
 // test and history; a finished system's blocks return at once.
 namespace {
 
+constexpr int kMaxChunk = 32;  // most passes launched between two host polls
+
 struct BatchScratch {
   char* base = nullptr;
   size_t cap = 0;
@@ -786,7 +788,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     uegccd_ctx* c = cs[k];
     ep[k + 1] = ep[k] + (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
-    sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
+    sp[k + 1] = sp[k] + (need_slice ? c->v * slice_split(c->o) : 0);
     omax = std::max(omax, c->o);
     if (c->sd.do_ladder)
       for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
@@ -826,6 +828,34 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   return upload_image(out->base, image, stream, stage);
 }
 
+// Passes to launch before the next host poll.  Each unfinished solve's
+// convergence measure (max|R| against tol_r in DIIS mode, max|dT| against
+// TolMax in the reference iteration) decays about geometrically; its rate
+// over the last chunk predicts the passes still needed (+1 for the stopping
+// pass, +1 for the final residual pass).  The batch waits for its slowest
+// member.  Passes launched after a solve is done return at once, so a wrong
+// guess costs either one more poll or a few empty passes, never a result.
+int next_chunk(uegccd_ctx** cs, int n, const std::vector<State>& st, std::vector<double>& prev,
+               int last, int first, int cap) {
+  int want = 1;
+  for (int k = 0; k < n; ++k) {
+    const State& x = st[k];
+    if (x.phase == PH_DONE) continue;
+    if (x.phase == PH_FINAL) { want = std::max(want, 1); continue; }
+    const bool diis = cs[k]->sd.mode == UEGCCD_MODE_DIIS;
+    const double r = diis ? x.maxres_in : x.dt;
+    const double tol = diis ? cs[k]->sd.tol_r : cs[k]->sd.tol_dt;
+    const double r0 = prev[k];
+    prev[k] = r;
+    if (!(r > 0.0) || !(r0 > r) || !std::isfinite(r)) return first;  // no decay seen yet
+    const double rate = std::pow(r / r0, 1.0 / std::max(1, last));
+    int need = 2;
+    if (r >= tol && rate < 1.0) need = (int)std::ceil(std::log(tol / r) / std::log(rate)) + 2;
+    want = std::max(want, need);
+  }
+  return std::min(std::max(want, 1), cap);
+}
+
 // Runs the iteration of a batch already holding its starting amplitudes in
 // tcur: init (E, dT of the guess), then passes until every system is done.
 int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vector<State>& st,
@@ -845,13 +875,16 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   launch_init(b, s);
   launch_finalize(b, FIN_INIT, s);
   HIPCHK(hipGetLastError());
-  const int chunk = std::max(1, c0->opt.iters_per_sync);
+  const int first = std::max(1, c0->opt.iters_per_sync);
+  const int cap = std::max(first, kMaxChunk);
+  int chunk = first;
   st.assign(n, State{});
+  std::vector<double> prev_r(n, 0.0);
   int passes = 0;
   bool all_done = false;
   const bool timing = c0->timed_kernel >= 0;
-  if (timing && (int)c0->tev.size() < 2 * chunk) {
-    for (size_t k = c0->tev.size(); k < (size_t)(2 * chunk); ++k) {
+  if (timing && (int)c0->tev.size() < 2 * cap) {
+    for (size_t k = c0->tev.size(); k < (size_t)(2 * cap); ++k) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
       c0->tev.push_back(e);
@@ -897,6 +930,7 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     }
     all_done = true;
     for (int k = 0; k < n; ++k) all_done = all_done && st[k].phase == PH_DONE;
+    if (!all_done) chunk = next_chunk(cs, n, st, prev_r, chunk, first, cap);
   }
   HIPCHK(hipEventRecord(e1, s));
   HIPCHK(hipEventSynchronize(e1));

@@ -2,16 +2,18 @@
 //
 // One residual pass (TermsWrapper, source/CCD.f90:146-181) plus the update
 // (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is six launches:
-//   k_slice      per virtual slice: linear ring / crossed-ring products,
-//                quadratic ring intermediates, mosaic partial sums
-//                (source/CCD.f90:507-585, 689-940)
-//   k_joo        mosaic Joo(k), the reduction of the per-slice partials
-//   k_ladder     both ladders as pair-momentum-blocked GEMMs on MFMA
-//                (source/CCD.f90:611-681)
-//   k_main       residual assembly, R = D*T - G, damped update, dT, energy,
-//                DIIS dot products
-//   k_finalize   deterministic reductions, convergence test, DIIS solve
-//   k_extrap     amplitudes of the next pass (DIIS extrapolation or update)
+//   k_slice         per virtual slice: linear ring / crossed-ring products,
+//                   quadratic ring intermediates, mosaic partial sums
+//                   (source/CCD.f90:507-585, 689-940)
+//   k_ladder        the particle-particle ladder and the hole-hole W
+//                   intermediates as pair-momentum-blocked MFMA tiles
+//                   (source/CCD.f90:611-681)
+//   k_contract      quadratic ring / crossed-ring products per momentum
+//                   transfer, hole-hole ladder tiles (MFMA), mosaic Joo
+//   k_main          residual assembly, R = D*T - G, damped update, dT, energy,
+//                   DIIS dot products
+//   k_finalize      deterministic reductions, convergence test, DIIS solve
+//   k_extrap        amplitudes of the next pass (DIIS extrapolation or update)
 // Every kernel resolves its solve from a block-prefix table, so one launch
 // covers any number of independent solves (twists, r_s values, bases).
 //
@@ -224,16 +226,17 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
 // sums of source/CCD.f90:548-567 are per-slice reductions as well:
 //   Jvv(s) = sum_k P_s(k),  Joo(k) = sum_s P_s(k)  (k_joo),
 //   P_s(k) = sum_l v(n_s-n_k)(T^aa+T^ab)(k,l,s) - v(n_s-n_l)(T^aa+T^ba)(k,l,s).
-__global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restrict__ solves,
-                                                         const int* __restrict__ prefix, int n) {
-  __shared__ Lds L;
-  extern __shared__ double dyn[];
-  const int s = find_solve(prefix, n, blockIdx.x);
+__device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+                            int n, int bid, Lds& L, double* dyn) {
+  const int s = find_solve(prefix, n, bid);
   const SolveDev& S = solves[s];
   if (S.st->phase == PH_DONE) return;
-  const int sv = blockIdx.x - prefix[s];
+  // slice_split blocks per slice, each computing a share of the o*o outputs
+  const int split = slice_split(S.o);
+  const int lb = bid - prefix[s], sv = lb / split, part = lb - sv * split;
   stage(S, L);
   const int o = S.o, o2 = o * o, so = o + sv;
+  const int out0 = (o2 * part) / split, out1 = (o2 * (part + 1)) / split;
   double* Tab = dyn;
   double* Tba = dyn + o2;
   double* Vo = dyn + 2 * o2;
@@ -252,6 +255,8 @@ __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restri
     Vo[idx] = VPQ(idx % o, idx / o);
   }
   __syncthreads();
+  // row/column sums (every block needs them for J); the first block of the
+  // slice stores them with the mosaic partials
   for (int i = threadIdx.x; i < o; i += blockDim.x) {
     double r = 0.0, c = 0.0, p = 0.0;
     const double vsi = VPQ(so, i);
@@ -264,18 +269,20 @@ __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restri
     rs[i] = r;
     cs[i] = c;
     jp[i] = p;
-    S.rsum[sv * o + i] = r;
-    S.csum[sv * o + i] = c;
-    S.joo_part[sv * o + i] = p;
+    if (part == 0) {
+      S.rsum[sv * o + i] = r;
+      S.csum[sv * o + i] = c;
+      S.joo_part[sv * o + i] = p;
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && part == 0) {
     double jv = 0.0;
     for (int k = 0; k < o; ++k) jv += jp[k];
     S.jvv[sv] = jv;
   }
   const int4 ns = L.nv[so];
-  for (int idx = threadIdx.x; idx < o2; idx += blockDim.x) {
+  for (int idx = out0 + threadIdx.x; idx < out1; idx += blockDim.x) {
     const int r = idx % o, c = idx / o;
     double tva = 0.0, tvb = 0.0, vta = 0.0, vtb = 0.0;
     for (int k = 0; k < o; ++k) {
@@ -433,11 +440,9 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0, const 
   }
 }
 
-__global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
-                                                const int4* __restrict__ items, int nitems) {
-  __shared__ double vt[kMaxN2];
-  __shared__ double red[3 * 64 * 8];
-  const int4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+__device__ void ladder_block(const SolveDev* __restrict__ solves, const int4* __restrict__ items,
+                             int item, double* vt, double* red) {
+  const int4 it = items[item];
   const SolveDev& S = solves[it.x];
   if (S.st->phase == PH_DONE || !S.do_ladder) return;
   for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
@@ -446,6 +451,20 @@ __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ sol
     ladder_tile<double>(S, it.y, it.z, it.w, vt, red);
   else
     ladder_tile<float>(S, it.y, it.z, it.w, vt, red);
+}
+
+__global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restrict__ solves,
+                                                         const int* __restrict__ prefix, int n) {
+  __shared__ Lds L;
+  extern __shared__ double dyn[];
+  slice_block(solves, prefix, n, blockIdx.x, L, dyn);
+}
+
+__global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
+                                                const int4* __restrict__ items) {
+  __shared__ double vt[kMaxN2];
+  __shared__ double red[3 * 64 * 8];
+  ladder_block(solves, items, xcd_block(blockIdx.x, gridDim.x), vt, red);
 }
 
 // --------------------------------------------------------------------------
@@ -575,17 +594,26 @@ __device__ void hh_tile(const SolveDev& S, int K, int tile) {
   }
 }
 
-// Mosaic Joo(k) = sum over slices of k_slice's partials (source/CCD.f90:563-567):
+// Mosaic Joo(k) = sum over slices of the slice blocks' partials (source/CCD.f90:563-567):
 // column sums read coalesced, then combined in a fixed order.
 __device__ void joo_sum(const SolveDev& S, double* part) {
   if (!S.do_mosaic) return;
   const int o = S.o;
   const int nc = blockDim.x / o;
   const int t = threadIdx.x, k = t % o, c = t / o;
-  double acc = 0.0;
-  if (c < nc)
-    for (int r = c; r < S.v; r += nc) acc += S.joo_part[r * o + k];
-  if (c < nc) part[c * o + k] = acc;
+  if (c < nc) {
+    // four independent partial sums keep several loads in flight
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int r = c;
+    for (; r + 3 * nc < S.v; r += 4 * nc) {
+      a0 += S.joo_part[r * o + k];
+      a1 += S.joo_part[(r + nc) * o + k];
+      a2 += S.joo_part[(r + 2 * nc) * o + k];
+      a3 += S.joo_part[(r + 3 * nc) * o + k];
+    }
+    for (; r < S.v; r += nc) a0 += S.joo_part[r * o + k];
+    part[c * o + k] = (a0 + a1) + (a2 + a3);
+  }
   __syncthreads();
   if (t < o) {
     double sum = 0.0;
@@ -658,7 +686,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       gab += 0.5 * (l2.x + l2.y);
       gba += 0.5 * (l2.x - l2.y);
     }
-    // slice products of k_slice: element (i,j) of slice a, (j,i) of slice b
+    // slice products: element (i,j) of slice a, (j,i) of slice b
     if (S.do_ring || S.do_xring) {
       const double4 ta = S.tv[x];
       const double4 tb = S.tv[(size_t)j + o * i + (size_t)o2 * bv];
@@ -743,8 +771,9 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
 //   [ -1^T      0 ] [l] = [-1]
 // over the nh newest history slots; the rows live in registers (columns
 // unrolled), pivots and row swaps go through cross-lane shuffles.
+// M is the unrolled system size (at least nh + 1).
+template <int M>
 __device__ bool diis_coefficients(const State& st, int ndiis, double* coef) {
-  constexpr int M = kMaxDiis + 1;
   const int lane = threadIdx.x & 63;
   const int nh = st.nhist, m = nh + 1;
   auto slot_of = [&](int r) { return ((st.slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis; };
@@ -815,14 +844,18 @@ __device__ bool diis_coefficients(const State& st, int ndiis, double* coef) {
   return true;
 }
 
-// The same bordered system for histories of up to M-1 entries, by one thread
-// with the whole matrix in registers: every loop is unrolled, the pivot row is
-// swapped in with selects, and rows past the system are padded with the
-// identity so the unrolled elimination runs over a fixed M x M.
-template <int M>
-__device__ bool diis_coefficients_regs(const State& st, int ndiis, double* coef) {
+
+// Reductions of the block partials and the iteration bookkeeping of
+// source/CCD.f90:80-99 (one block per solve).
+// The same coefficients for histories of up to 8 entries, by one thread:
+// c = B^-1 1 / (1^T B^-1 1) (the solution of the bordered system) with B
+// scaled by its largest diagonal and padded to 8 x 8 with the identity; a
+// Cholesky factorisation and two triangular solves, unrolled in registers.
+// A pivot below 1e-14 fails the extrapolation (the plain update is used).
+__device__ bool diis_cholesky8(const State& st, int ndiis, double* coef) {
+  constexpr int M = 8;
   const int nh = st.nhist;
-  int sl[M];  // history slot of row r (oldest first)
+  int sl[M];  // history slot of row r, oldest first
   {
     int sv = st.slot_cur - (nh - 1);
     if (sv < 0) sv += ndiis;
@@ -834,76 +867,66 @@ __device__ bool diis_coefficients_regs(const State& st, int ndiis, double* coef)
   }
   double scale = 0.0;
 #pragma unroll
-  for (int r = 0; r < M - 1; ++r)
+  for (int r = 0; r < M; ++r)
     if (r < nh) scale = fmax(scale, fabs(st.bmat[sl[r] * (kMaxDiis + 1)]));
   if (!(scale > 0.0) || !isfinite(scale)) return false;
   const double inv_scale = 1.0 / scale;
-  double A[M][M], rhs[M];
+  double L[M][M];
 #pragma unroll
-  for (int r = 0; r < M; ++r) {
+  for (int r = 0; r < M; ++r)
 #pragma unroll
-    for (int c = 0; c < M; ++c) {
-      double val = 0.0;
-      if (r < nh && c < nh) val = st.bmat[sl[r] * kMaxDiis + sl[c]] * inv_scale;
-      else if ((r < nh && c == nh) || (r == nh && c < nh)) val = -1.0;
-      else if (r > nh && r == c) val = 1.0;
-      A[r][c] = val;
-    }
-    rhs[r] = (r == nh) ? -1.0 : 0.0;
-  }
+    for (int c = 0; c < M; ++c)
+      L[r][c] = (r < nh && c < nh) ? st.bmat[sl[r] * kMaxDiis + sl[c]] * inv_scale
+                                   : (r == c ? 1.0 : 0.0);
 #pragma unroll
-  for (int c = 0; c < M; ++c) {
-    double best = fabs(A[c][c]);
-    int p = c;
+  for (int k = 0; k < M; ++k) {
+    double d = L[k][k];
 #pragma unroll
-    for (int r = c + 1; r < M; ++r)
-      if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); p = r; }
-    if (!(best >= 1e-14)) return false;
+    for (int j = 0; j < k; ++j) d -= L[k][j] * L[k][j];
+    if (!(d > 1e-14)) return false;
+    const double lkk = sqrt(d), inv_l = 1.0 / lkk;
+    L[k][k] = lkk;
 #pragma unroll
-    for (int r = c + 1; r < M; ++r) {
-      const bool sw = (r == p);
+    for (int i = k + 1; i < M; ++i) {
+      double v = L[i][k];
 #pragma unroll
-      for (int k = c; k < M; ++k) {
-        const double t = A[c][k];
-        A[c][k] = sw ? A[r][k] : t;
-        A[r][k] = sw ? t : A[r][k];
-      }
-      const double t = rhs[c];
-      rhs[c] = sw ? rhs[r] : t;
-      rhs[r] = sw ? t : rhs[r];
-    }
-    const double inv_piv = 1.0 / A[c][c];
-#pragma unroll
-    for (int r = c + 1; r < M; ++r) {
-      const double f = A[r][c] * inv_piv;
-#pragma unroll
-      for (int k = c; k < M; ++k) A[r][k] -= f * A[c][k];
-      rhs[r] -= f * rhs[c];
+      for (int j = 0; j < k; ++j) v -= L[i][j] * L[k][j];
+      L[i][k] = v * inv_l;
     }
   }
-  double x[M];
+  double y[M], z[M];
 #pragma unroll
-  for (int r = M - 1; r >= 0; --r) {
-    double acc = rhs[r];
+  for (int i = 0; i < M; ++i) {
+    double v = 1.0;
 #pragma unroll
-    for (int k = r + 1; k < M; ++k) acc -= A[r][k] * x[k];
-    x[r] = acc / A[r][r];
+    for (int j = 0; j < i; ++j) v -= L[i][j] * y[j];
+    y[i] = v / L[i][i];
   }
+#pragma unroll
+  for (int i = M - 1; i >= 0; --i) {
+    double v = y[i];
+#pragma unroll
+    for (int j = i + 1; j < M; ++j) v -= L[j][i] * z[j];
+    z[i] = v / L[i][i];
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int r = 0; r < M; ++r)
+    if (r < nh) sum += z[r];
+  if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
   for (int k = 0; k < ndiis; ++k) coef[k] = 0.0;
 #pragma unroll
-  for (int r = 0; r < M - 1; ++r) {
+  for (int r = 0; r < M; ++r) {
     if (r >= nh) continue;
-    if (!isfinite(x[r])) return false;
-    coef[sl[r]] = x[r];
+    const double c = z[r] / sum;
+    if (!isfinite(c)) return false;
+    coef[sl[r]] = c;
   }
   return true;
 }
 
-// Reductions of the block partials and the iteration bookkeeping of
-// source/CCD.f90:80-99 (one block per solve).
 __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
   __shared__ double vals[4 + kMaxDiis];
-  __shared__ double redm[(kFinThreads / 64) * kPartialStride];
   __shared__ int want_diis;
   // The iteration state is worked on in LDS: one parallel copy in, one out,
   // instead of a chain of dependent global accesses by a single thread.
@@ -918,31 +941,22 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
   const int nb = S.nblk_elem;
   const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS && phase0 == PH_ITER) ? S.ndiis : 0;
   {
-    // one coalesced pass over the slot-major partials:
-    // v = (max dT, max |R|, sum E, sum bad, dots)
-    double v[kPartialStride];
-#pragma unroll
-    for (int q = 0; q < kPartialStride; ++q) v[q] = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-      v[0] = fmax(v[0], *partial(S, 1, k));
-      v[1] = fmax(v[1], *partial(S, 2, k));
-      v[2] += *partial(S, 0, k);
-      v[3] += *partial(S, 3, k);
-#pragma unroll
-      for (int q = 0; q < kMaxDiis; ++q)
-        if (q < nd) v[4 + q] += *partial(S, 4 + q, k);
-    }
-    block_reduce_multi<kPartialStride, 2>(v, redm);
-    if (threadIdx.x == 0) {
-      vals[0] = v[2];
-      vals[1] = v[0];
-      vals[2] = v[1];
-      vals[3] = v[3];
-#pragma unroll
-      for (int q = 0; q < kMaxDiis; ++q)
-        if (q < nd) vals[4 + q] = v[4 + q];
+    // Slot-major partials: quantity q of every element block is contiguous.
+    // Wave w reduces quantities w, w + nw, ...: q = 1 (max dT) and q = 2
+    // (max |R|) are maxima; E (0), the non-finite count (3) and the DIIS dots
+    // (4..) are sums.  Lane-strided, then a wave tree: a fixed order.
+    const int nq = 4 + nd;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int q = wave; q < nq; q += nw) {
+      const bool is_max = q == 1 || q == 2;
+      const double* P = partial(S, q, 0);
+      double acc = 0.0;
+      for (int k = lane; k < nb; k += 64) acc = is_max ? fmax(acc, P[k]) : acc + P[k];
+      acc = is_max ? wave_max(acc) : wave_sum(acc);
+      if (lane == 0) vals[q] = acc;
     }
   }
+  __syncthreads();
   if (threadIdx.x == 0) want_diis = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1015,11 +1029,11 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
     }
   }
   __syncthreads();
-  if (want_diis) {
+  if (want_diis && threadIdx.x < 64) {
     if (S.ndiis <= 8) {
-      if (threadIdx.x == 0) st->use_coef = diis_coefficients_regs<9>(*st, S.ndiis, st->coef) ? 1 : 0;
-    } else if (threadIdx.x < 64) {
-      const bool ok = diis_coefficients(*st, S.ndiis, st->coef);
+      if (threadIdx.x == 0) st->use_coef = diis_cholesky8(*st, S.ndiis, st->coef) ? 1 : 0;
+    } else {
+      const bool ok = diis_coefficients<kMaxDiis + 1>(*st, S.ndiis, st->coef);
       if (threadIdx.x == 0) st->use_coef = ok ? 1 : 0;
     }
   }
@@ -1099,6 +1113,11 @@ static void launch_slice(const BatchDev& b, hipStream_t s) {
                      b.solves, b.slice_prefix, b.n);
 }
 
+static void launch_ladder(const BatchDev& b, hipStream_t s) {
+  if (b.nlad <= 0) return;
+  hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(256), 0, s, b.solves, b.lad_items);
+}
+
 static void launch_contract(const BatchDev& b, hipStream_t s) {
   if (b.ncon <= 0) return;
   allow_large_lds();
@@ -1134,8 +1153,7 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
       launch_contract(b, s);
       break;
     case 2:
-      if (b.nlad > 0)
-        hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(256), 0, s, b.solves, b.lad_items, b.nlad);
+      launch_ladder(b, s);
       break;
     case 3:
       hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
@@ -1158,8 +1176,7 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
   launch_slice(b, s);
   mark(0, true);
   mark(2, false);
-  if (b.nlad > 0)
-    hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(256), 0, s, b.solves, b.lad_items, b.nlad);
+  launch_ladder(b, s);
   mark(2, true);
   mark(1, false);
   launch_contract(b, s);

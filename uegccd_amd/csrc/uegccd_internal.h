@@ -19,7 +19,7 @@ constexpr int kMaxDiis = 16;
 constexpr int kPartialStride = 4 + kMaxDiis;  // E, dT, max|R|, non-finite count, DIIS dots
 constexpr int kElemThreads = 256;                 // threads of the element-wise blocks
 constexpr int kElemPerBlock = kElemThreads;       // amplitudes per element block (one per thread)
-constexpr int kFinThreads = 512;                  // threads of the per-solve reduction block
+constexpr int kFinThreads = 256;                  // threads of the per-solve reduction block
 constexpr int kMaxNao = 1024;                 // orbitals staged in LDS
 constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the Coulomb table
 constexpr int kMaxOcc = 64;                   // NOcc staged per slice in LDS
@@ -122,6 +122,11 @@ enum ContractKind : int { CK_QUAD = 0, CK_HH = 1, CK_JOO = 2 };
 constexpr int kHHTilesPerBlock = kElemThreads / 64;
 
 inline int slice_lds_bytes(int o) { return (3 * o * o + 3 * o) * 8; }
+// Blocks per virtual slice in k_slice: one per 256 of the o*o outputs, at most 4.
+__host__ __device__ inline int slice_split(int o) {
+  const int g = (o * o + 255) / 256;
+  return g < 1 ? 1 : (g > 4 ? 4 : g);
+}
 // q-block staging: rows n <= o (J, 3 doubles) and columns m <= o (T, 2 doubles)
 // per l, or the Joo partial sums (one double per thread)
 inline int con_lds_bytes(int o) { return (5 * o * o > kElemThreads ? 5 * o * o : kElemThreads) * 8; }
