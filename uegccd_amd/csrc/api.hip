@@ -91,7 +91,8 @@ struct uegccd_ctx {
   std::vector<int4> nv4, kb_nva;
   std::vector<double> eig, vtab;
   std::vector<int4> lad_items;  // (0, K, row0, col0)
-  std::vector<int> qb_roff, qb_rows, qb_neg, qb_d;  // hole-particle sets P(q)
+  std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow;  // hole-particle sets P(q)
+  std::vector<int4> qb_info;
   std::vector<int4> con_items;  // (0, kind, arg, arg) of k_contract
   // device
   DeviceArena arena;
@@ -220,7 +221,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
                                   (a.z - b.z) * (a.z - b.z));
     }
   if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
-  if (con_lds_bytes(o) > kMaxDynLds || slice_lds_bytes(o) + (int)sizeof(int4) * kMaxNao +
+  if (slice_lds_bytes(o) + (int)sizeof(int4) * kMaxNao +
       (int)sizeof(double) * (kMaxN2 + 16) > kMaxDynLds)
     return UEGCCD_ERR_UNSUPPORTED;
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
@@ -246,8 +247,13 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->qb_rows.reserve((size_t)o * v);
     c->qb_neg.assign(nq, -1);
     c->qb_d.assign((size_t)nq * o, -1);
+    c->qrow.assign((size_t)o * v, -1);
     for (int q = 0; q < nq; ++q) {
-      for (int r : qrows[q]) c->qb_rows.push_back(r);
+      for (int r : qrows[q]) {
+        const int i = r % o, pv = r / o;
+        c->qrow[(size_t)pv * o + i] = (int)c->qb_rows.size();
+        c->qb_rows.push_back(r);
+      }
       c->qb_roff[q + 1] = (int)c->qb_rows.size();
       c->qb_neg[q] = qid[center - qlin[q]];
       for (int l = 0; l < o; ++l) {
@@ -255,17 +261,29 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
         c->qb_d[(size_t)q * o + l] = p >= o ? p - o : -1;
       }
     }
+    c->qb_info.assign(nq, make_int4(0, 0, 0, 0));
+    for (int q = 0; q < nq; ++q) {
+      const int qn = c->qb_neg[q];
+      c->qb_info[q] = make_int4(c->qb_roff[q], c->qb_roff[q + 1] - c->qb_roff[q],
+                                qn >= 0 ? c->qb_roff[qn] : 0,
+                                qn >= 0 ? c->qb_roff[qn + 1] - c->qb_roff[qn] : 0);
+    }
     // work items of the contraction pass
     c->con_items.clear();
+    // one wave per item: 16x16 tiles of the per-transfer ring products, 16x16
+    // hole-hole ladder tiles, the Joo sums
     if (s.do_ring || s.do_xring)
-      for (int q = 0; q < nq; ++q)
-        if (c->qb_neg[q] >= 0) c->con_items.push_back(make_int4(0, CK_QUAD, q, 0));
+      for (int q = 0; q < nq; ++q) {
+        const int4 in = c->qb_info[q];
+        const int nt = ((in.y + 15) / 16) * ((in.w + 15) / 16);
+        for (int t = 0; t < nt; ++t) c->con_items.push_back(make_int4(0, CK_QUAD, q, t));
+      }
     if (s.do_ladder)
       for (int K = 0; K < c->nK; ++K) {
         const int noo = c->kb_poff[K + 1] - c->kb_poff[K];
         const int nvv = c->kb_voff[K + 1] - c->kb_voff[K];
         const int nt = ((noo + 15) / 16) * ((nvv + 15) / 16);
-        for (int t = 0; t < nt; t += kHHTilesPerBlock) c->con_items.push_back(make_int4(0, CK_HH, K, t));
+        for (int t = 0; t < nt; ++t) c->con_items.push_back(make_int4(0, CK_HH, K, t));
       }
     if (s.do_mosaic) c->con_items.push_back(make_int4(0, CK_JOO, 0, 0));
   }
@@ -311,6 +329,8 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
            aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->con_items.size()));
+  bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->qb_info.size()));
+  bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qrow.size()));
   bytes += aligned(sizeof(double4) * n3);                              // quad
   bytes += 4096;
   return bytes;
@@ -496,6 +516,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.qb_rows = im.put(c->qb_rows);
   S.qb_neg = im.put(c->qb_neg);
   S.qb_d = im.put(c->qb_d);
+  S.qb_info = im.put(c->qb_info);
+  S.qrow = im.put(c->qrow);
   c->d_sd = im.reserve<SolveDev>(1);
   // batch of one: element/slice block prefixes and the ladder tiles
   BatchDev& b = c->batch;
@@ -515,7 +537,6 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.lad_items = const_cast<int4*>(im.put(c->lad_items));
   b.ncon = (int)c->con_items.size();
   b.con_items = const_cast<int4*>(im.put(c->con_items));
-  b.con_lds = con_lds_bytes(c->o);
   // work buffers
   c->arena.used = aligned(c->arena.used);
   *work_off = c->arena.used;
@@ -824,7 +845,6 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.elem_blocks = ep[n];
   b.slice_blocks = sp[n];
   b.slice_lds = slice_lds_bytes(omax);
-  b.con_lds = con_lds_bytes(omax);
   return upload_image(out->base, image, stream, stage);
 }
 

@@ -228,9 +228,10 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
 //   P_s(k) = sum_l v(n_s-n_k)(T^aa+T^ab)(k,l,s) - v(n_s-n_l)(T^aa+T^ba)(k,l,s).
 __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
                             int n, int bid, Lds& L, double* dyn) {
+  // No test of the solve's phase here: a finished solve's slice products are
+  // never read again, and the test would put a dependent load before any work.
   const int s = find_solve(prefix, n, bid);
   const SolveDev& S = solves[s];
-  if (S.st->phase == PH_DONE) return;
   // slice_split blocks per slice, each computing a share of the o*o outputs
   const int split = slice_split(S.o);
   const int lb = bid - prefix[s], sv = lb / split, part = lb - sv * split;
@@ -312,7 +313,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       a6 += ua * y;
     }
     const double vq = VPQ(i, so);
-    const size_t jx = (size_t)(sv * o + i) * o + l;
+    const size_t jx = (size_t)S.qrow[sv * o + i] * o + l;
     S.jring[jx] = make_double4(vq * rs[i] - a1, vq * rs[i] - a2, -a3, 0.0);
     S.jxring[jx] = make_double4(vq * cs[i] - a4, vq * cs[i] - a5, a6, 0.0);
   }
@@ -388,25 +389,34 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0, const 
   }
   const int4 nx = S.nv4[cx], ny = S.nv4[cy];
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-  for (int c0 = 4 * wave; c0 < nvv; c0 += 16) {
-    const int ci = c0 + kq;
-    Real sv = 0, pv = 0, bs = 0, bp = 0;
-    if (ci < nvv) {
-      const int4 nc = cl[ci];
-      if (xpA >= 0) {
-        const double2 t = T[(size_t)xpA + (size_t)o2 * (nc.w - o)];
-        sv = (Real)(t.x + t.y);
-        pv = (Real)(t.x - t.y);
-      }
-      if (colv) {
-        const Real vx = (Real)vt[d2(nc, nx)];
-        const Real vy = (Real)vt[d2(nc, ny)];
+  // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
+  // their loads (column list, then amplitudes) before the MFMAs
+  constexpr int G = 4;
+  for (int g0 = 4 * wave; g0 < nvv; g0 += 16 * G) {
+    int4 nc[G];
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      const int ci = g0 + 16 * st + kq;
+      nc[st] = ci < nvv ? cl[ci] : make_int4(0, 0, 0, -1);
+    }
+    double2 t[G];
+#pragma unroll
+    for (int st = 0; st < G; ++st)
+      t[st] = (nc[st].w >= 0 && xpA >= 0) ? T[(size_t)xpA + (size_t)o2 * (nc[st].w - o)]
+                                          : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      if (g0 + 16 * st >= nvv) break;
+      Real bs = 0, bp = 0;
+      if (nc[st].w >= 0 && colv) {
+        const Real vx = (Real)vt[d2(nc[st], nx)];
+        const Real vy = (Real)vt[d2(nc[st], ny)];
         bs = vx - vy;
         bp = vx + vy;
       }
+      accs = MfmaTraits<Real>::mma((Real)(t[st].x + t[st].y), bs, accs);
+      accp = MfmaTraits<Real>::mma((Real)(t[st].x - t[st].y), bp, accp);
     }
-    accs = MfmaTraits<Real>::mma(sv, bs, accs);
-    accp = MfmaTraits<Real>::mma(pv, bp, accp);
   }
   if (wave > 0) {
     double* mine = red + ((wave - 1) * 64 + lane) * 8;
@@ -480,70 +490,75 @@ __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ sol
 // are, per q, one product [rows x o] . [o x columns] each: rows and columns
 // are staged in LDS, each output written once (ring to quad.xy, crossed ring
 // to quad.zw of its element).
-__device__ void quad_block(const SolveDev& S, int qb, double* lds) {
-  const int qn = S.qb_neg[qb];
-  if (qn < 0) return;
+__device__ void quad_tile(const SolveDev& S, int qb, int tile) {
+  typedef MfmaTraits<double>::acc_t acc_t;
+  const int4 info = S.qb_info[qb];  // rows of q: [r0, r0+n); columns (rows of -q): [c0, c0+m)
+  const int r0 = info.x, n = info.y, c0 = info.z, m = info.w;
+  const int ntc = (m + 15) >> 4;
+  if (m == 0 || tile >= ((n + 15) >> 4) * ntc) return;
   const int o = S.o, o2 = o * o;
-  const int r0 = S.qb_roff[qb], n = S.qb_roff[qb + 1] - r0;
-  const int c0 = S.qb_roff[qn], m = S.qb_roff[qn + 1] - c0;
-  const int* rows = S.qb_rows + r0;
-  const int* cols = S.qb_rows + c0;
+  const int row0 = (tile / ntc) * 16, col0 = (tile % ntc) * 16;
+  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const bool ring = S.do_ring, xring = S.do_xring;
+  // lane supplies A[row r16][l] (J rows, contiguous per q) and B[l][column r16]
+  const int rowA = row0 + r16, colB = col0 + r16;
+  const bool rv = rowA < n, cv = colB < m;
+  int j = 0, pcv = 0;
+  if (cv) {
+    const int pc = S.qb_rows[c0 + colB];
+    j = pc % o;
+    pcv = pc / o;
+  }
+  const double4* jr = S.jring + (size_t)(r0 + (rv ? rowA : 0)) * o;
+  const double4* jx = S.jxring + (size_t)(r0 + (rv ? rowA : 0)) * o;
   const int* dq = S.qb_d + (size_t)qb * o;
-  double* J0 = lds;
-  double* J1 = J0 + n * o;
-  double* J2 = J1 + n * o;
-  double* Ta = J2 + n * o;
-  double* Tb = Ta + o * m;
   const double2* T = S.tcur;
-  for (int pass = 0; pass < 2; ++pass) {
-    const bool ring = pass == 0;
-    if (ring ? !S.do_ring : !S.do_xring) continue;
-    const double4* J = ring ? S.jring : S.jxring;
-    for (int idx = threadIdx.x; idx < n * o; idx += blockDim.x) {
-      const int r = idx / o, l = idx - r * o;
-      const int pr = rows[r], i = pr % o, pv = pr / o;
-      const double4 jj = J[(size_t)(pv * o + i) * o + l];
-      J0[idx] = jj.x;
-      J1[idx] = jj.y;
-      J2[idx] = jj.z;
-    }
-    for (int idx = threadIdx.x; idx < o * m; idx += blockDim.x) {
-      const int l = idx / m, c = idx - l * m;
-      const int pc = cols[c], j = pc % o, pv = pc / o;
-      const int tv = ring ? dq[l] : pv;
-      double2 t = make_double2(0.0, 0.0);
-      if (tv >= 0) t = T[l + o * j + (size_t)o2 * tv];
-      Ta[idx] = t.x;
-      Tb[idx] = t.y;
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < n * m; idx += blockDim.x) {
-      const int r = idx / m, c = idx - r * m;
-      double qab = 0.0, qba = 0.0;
-      const double* j0 = J0 + r * o;
-      const double* j1 = J1 + r * o;
-      const double* j2 = J2 + r * o;
-      if (ring) {
-        for (int l = 0; l < o; ++l) {
-          const double ta = Ta[l * m + c], tb = Tb[l * m + c];
-          qab += j1[l] * (ta + tb) + j0[l] * ta;
-          qba += j2[l] * tb;
-        }
-      } else {
-        for (int l = 0; l < o; ++l) {
-          const double ta = Ta[l * m + c], tb = Tb[l * m + c];
-          qba = qba - j0[l] * tb - j1[l] * (ta + tb);
-          qab += j2[l] * ta;
-        }
+  acc_t rab = {0, 0, 0, 0}, rba = {0, 0, 0, 0}, xab = {0, 0, 0, 0}, xba = {0, 0, 0, 0};
+  constexpr int G = 4;  // k-steps whose loads are issued together
+  for (int g0 = 0; g0 < o; g0 += 4 * G) {
+    double4 ar[G], ax[G];
+    double2 br[G], bx[G];
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      const int l = g0 + 4 * st + kq;
+      const bool lv = l < o;
+      ar[st] = ax[st] = make_double4(0.0, 0.0, 0.0, 0.0);
+      br[st] = bx[st] = make_double2(0.0, 0.0);
+      if (lv && rv) {
+        if (ring) ar[st] = jr[l];
+        if (xring) ax[st] = jx[l];
       }
-      const int pr = rows[r], pc = cols[c];
-      const int i = pr % o, j = pc % o;
-      // ring: a is the row's particle; crossed ring: a is the column's
-      const int av = ring ? pr / o : pc / o;
-      double2* q2 = reinterpret_cast<double2*>(S.quad + (size_t)i + o * j + (size_t)o2 * av);
-      q2[ring ? 0 : 1] = make_double2(qab, qba);
+      if (lv && cv) {
+        const int dv = dq[l];
+        if (ring && dv >= 0) br[st] = T[l + o * j + (size_t)o2 * dv];
+        if (xring) bx[st] = T[l + o * j + (size_t)o2 * pcv];
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      if (g0 + 4 * st >= o) break;
+      // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab,  ba += J3 T^ba
+      rab = MfmaTraits<double>::mma(ar[st].y, br[st].x + br[st].y, rab);
+      rab = MfmaTraits<double>::mma(ar[st].x, br[st].x, rab);
+      rba = MfmaTraits<double>::mma(ar[st].z, br[st].y, rba);
+      // crossed ring:  ba -= J1 T^ba + J2 (T^ab + T^ba),  ab += J3 T^ab
+      xba = MfmaTraits<double>::mma(-ax[st].x, bx[st].y, xba);
+      xba = MfmaTraits<double>::mma(-ax[st].y, bx[st].x + bx[st].y, xba);
+      xab = MfmaTraits<double>::mma(ax[st].z, bx[st].x, xab);
+    }
+  }
+  if (!cv) return;
+  for (int r = 0; r < 4; ++r) {
+    const int row = row0 + MfmaTraits<double>::row(kq, r);
+    if (row >= n) continue;
+    const int pr = S.qb_rows[r0 + row];
+    const int i = pr % o;
+    if (ring)
+      reinterpret_cast<double2*>(S.quad + (size_t)i + o * j + (size_t)o2 * (pr / o))[0] =
+          make_double2(rab[r], rba[r]);
+    if (xring)
+      reinterpret_cast<double2*>(S.quad + (size_t)i + o * j + (size_t)o2 * pcv)[1] =
+          make_double2(xab[r], xba[r]);
   }
 }
 
@@ -566,23 +581,38 @@ __device__ void hh_tile(const SolveDev& S, int K, int tile) {
   const bool rv = rowA < noo, cv = colB < nvv;
   const int av = cv ? S.kb_va[voff + colB] - o : 0;
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < noo; k0 += 4) {
-    const int q = k0 + kq;
-    double ws = 0.0, wp = 0.0, bs = 0.0, bp = 0.0;
-    if (q < noo) {
-      if (rv) {
+  // groups of 8 k-steps (32 pairs): all loads of a group are issued before
+  // its MFMAs, so the pair-index -> amplitude chains overlap
+  constexpr int G = 8;
+  for (int g0 = 0; g0 < noo; g0 += 4 * G) {
+    int xq[G];
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      const int q = g0 + 4 * st + kq;
+      xq[st] = (q < noo && cv) ? S.kb_pairs[poff + q] : -1;
+    }
+    double ws[G], wp[G], bs[G], bp[G];
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      const int q = g0 + 4 * st + kq;
+      ws[st] = wp[st] = bs[st] = bp[st] = 0.0;
+      if (q < noo && rv) {
         const double2 w = W[rowA * noo + q];
-        ws = w.x;
-        wp = w.y;
+        ws[st] = w.x;
+        wp[st] = w.y;
       }
-      if (cv) {
-        const double2 t = T[S.kb_pairs[poff + q] + (size_t)o2 * av];
-        bs = t.x + t.y;
-        bp = t.x - t.y;
+      if (xq[st] >= 0) {
+        const double2 t = T[xq[st] + (size_t)o2 * av];
+        bs[st] = t.x + t.y;
+        bp[st] = t.x - t.y;
       }
     }
-    accs = MfmaTraits<double>::mma(ws, bs, accs);
-    accp = MfmaTraits<double>::mma(wp, bp, accp);
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      if (g0 + 4 * st >= noo) break;
+      accs = MfmaTraits<double>::mma(ws[st], bs[st], accs);
+      accp = MfmaTraits<double>::mma(wp[st], bp[st], accp);
+    }
   }
   if (!cv) return;
   for (int r = 0; r < 4; ++r) {
@@ -594,46 +624,42 @@ __device__ void hh_tile(const SolveDev& S, int K, int tile) {
   }
 }
 
-// Mosaic Joo(k) = sum over slices of the slice blocks' partials (source/CCD.f90:563-567):
-// column sums read coalesced, then combined in a fixed order.
-__device__ void joo_sum(const SolveDev& S, double* part) {
+// Mosaic Joo(k) = sum over slices of the slice blocks' partials
+// (source/CCD.f90:563-567), by one wave: lane k sums column k over the v
+// slices with four independent partial sums, combined in a fixed order.
+__device__ void joo_wave(const SolveDev& S) {
   if (!S.do_mosaic) return;
-  const int o = S.o;
-  const int nc = blockDim.x / o;
-  const int t = threadIdx.x, k = t % o, c = t / o;
-  if (c < nc) {
-    // four independent partial sums keep several loads in flight
+  const int o = S.o, lane = threadIdx.x & 63;
+  for (int k = lane; k < o; k += 64) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int r = c;
-    for (; r + 3 * nc < S.v; r += 4 * nc) {
+    int r = 0;
+    for (; r + 3 < S.v; r += 4) {
       a0 += S.joo_part[r * o + k];
-      a1 += S.joo_part[(r + nc) * o + k];
-      a2 += S.joo_part[(r + 2 * nc) * o + k];
-      a3 += S.joo_part[(r + 3 * nc) * o + k];
+      a1 += S.joo_part[(r + 1) * o + k];
+      a2 += S.joo_part[(r + 2) * o + k];
+      a3 += S.joo_part[(r + 3) * o + k];
     }
-    for (; r < S.v; r += nc) a0 += S.joo_part[r * o + k];
-    part[c * o + k] = (a0 + a1) + (a2 + a3);
-  }
-  __syncthreads();
-  if (t < o) {
-    double sum = 0.0;
-    for (int cc = 0; cc < nc; ++cc) sum += part[cc * o + t];
-    S.joo[t] = sum;
+    for (; r < S.v; ++r) a0 += S.joo_part[r * o + k];
+    S.joo[k] = (a0 + a1) + (a2 + a3);
   }
 }
 
+// One work item per wave; the four waves of a block take four consecutive
+// items of the XCD-ordered list.
 __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
-                                                            const int4* __restrict__ items) {
-  extern __shared__ double lds_con[];
-  const int4 it = items[xcd_block(blockIdx.x, gridDim.x)];
+                                                            const int4* __restrict__ items,
+                                                            int nitems) {
+  const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + (threadIdx.x >> 6);
+  if (item >= nitems) return;
+  const int4 it = items[item];
   const SolveDev& S = solves[it.x];
-  if (S.st->phase == PH_DONE) return;
+  // (no phase test: the contraction outputs of a finished solve are never read)
   if (it.y == CK_QUAD)
-    quad_block(S, it.z, lds_con);
+    quad_tile(S, it.z, it.w);
   else if (it.y == CK_HH)
-    hh_tile(S, it.z, it.w + (threadIdx.x >> 6));
+    hh_tile(S, it.z, it.w);
   else
-    joo_sum(S, lds_con);
+    joo_wave(S);
 }
 
 // --------------------------------------------------------------------------
@@ -1101,8 +1127,6 @@ static void allow_large_lds() {
   if (done) return;
   hipFuncSetAttribute(reinterpret_cast<const void*>(&k_slice),
                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds - (int)sizeof(Lds));
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_contract),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds);
   done = true;
 }
 
@@ -1120,9 +1144,9 @@ static void launch_ladder(const BatchDev& b, hipStream_t s) {
 
 static void launch_contract(const BatchDev& b, hipStream_t s) {
   if (b.ncon <= 0) return;
-  allow_large_lds();
-  hipLaunchKernelGGL(k_contract, dim3(b.ncon), dim3(kElemThreads), b.con_lds, s, b.solves,
-                     b.con_items);
+  const int grid = (b.ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock;
+  hipLaunchKernelGGL(k_contract, dim3(grid), dim3(kElemThreads), 0, s, b.solves, b.con_items,
+                     b.ncon);
 }
 
 void launch_init(const BatchDev& b, hipStream_t s) {

@@ -77,6 +77,8 @@ struct SolveDev {
   const int* qb_rows;   // [o*v] pairs i + o*(p - o), grouped by q
   const int* qb_neg;    // [nqb] block of -q, or -1 when P(-q) is empty
   const int* qb_d;      // [nqb*o] virtual index of n_l + q for each l, or -1
+  const int4* qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
+  const int* qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
   // mutable buffers
   double2* tcur;        // [n3] (abab, abba) amplitudes entering the next pass
   double2* histT;       // [n3][ndiis] updated amplitudes, element-major
@@ -86,8 +88,9 @@ struct SolveDev {
   double2* lad;         // [n3] particle-particle ladder, (S, P) parts
   double2* w;           // [wtot] hole-hole W intermediates, (S, P) parts
   int wtot;
-  double4* jring;       // [o*v*o] quadratic ring intermediates (J1, J2, J3, 0)
-  double4* jxring;      // [o*v*o] quadratic crossed-ring intermediates
+  double4* jring;       // [o*v rows][o] quadratic ring intermediates (J1, J2, J3, 0),
+                        // rows in the grouped P(q) order (qrow)
+  double4* jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
   double4* tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
   double4* quad;        // [n3] quadratic ring (ab, ba) and crossed-ring (ab, ba) sums
   double* rsum;         // [v*o] slice row sums
@@ -111,15 +114,15 @@ struct BatchDev {
   int nlad;
   int elem_blocks, slice_blocks;
   int slice_lds;        // dynamic LDS bytes of k_slice (largest NOcc in the batch)
-  int4* con_items;      // (solve, kind, arg, arg) work items of k_contract
+  int4* con_items;      // (solve, kind, arg, arg) work items of k_contract, one per wave
   int ncon;
-  int con_lds;          // dynamic LDS bytes of k_contract (largest NOcc in the batch)
 };
 
-// k_contract work items: quadratic ring products of one transfer q, four
-// hole-hole ladder tiles of one K block, the mosaic Joo sums of one solve.
+// k_contract work items, one per wave: a 16x16 tile of the quadratic ring
+// products of one transfer q, a 16x16 hole-hole ladder tile of one K block,
+// the mosaic Joo sums of one solve.
 enum ContractKind : int { CK_QUAD = 0, CK_HH = 1, CK_JOO = 2 };
-constexpr int kHHTilesPerBlock = kElemThreads / 64;
+constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
 inline int slice_lds_bytes(int o) { return (3 * o * o + 3 * o) * 8; }
 // Blocks per virtual slice in k_slice: one per 256 of the o*o outputs, at most 4.
@@ -127,9 +130,6 @@ __host__ __device__ inline int slice_split(int o) {
   const int g = (o * o + 255) / 256;
   return g < 1 ? 1 : (g > 4 ? 4 : g);
 }
-// q-block staging: rows n <= o (J, 3 doubles) and columns m <= o (T, 2 doubles)
-// per l, or the Joo partial sums (one double per thread)
-inline int con_lds_bytes(int o) { return (5 * o * o > kElemThreads ? 5 * o * o : kElemThreads) * 8; }
 constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 
 // Kernel launchers (kernels.hip).
