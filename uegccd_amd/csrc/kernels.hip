@@ -22,6 +22,8 @@
 //   tv                               double4  (TV_ab, TV_ba, VT_ab, VT_ba)
 //   jring / jxring                   double4  (J1, J2, J3, 0) per (s, i, l)
 //   histT / histE                    element-major: slot k of x at x*ndiis + k
+#include <hip/hip_ext.h>
+
 #include "uegccd_internal.h"
 #include "../../include/uegccd.h"
 
@@ -1130,23 +1132,35 @@ static void allow_large_lds() {
   done = true;
 }
 
-static void launch_slice(const BatchDev& b, hipStream_t s) {
+// Launch through the extended API: when the timer names this kernel, its
+// start/stop events are written by the dispatch itself, so they bracket
+// exactly the kernel's execution (as a kernel trace measures it).
+template <typename F, typename... Args>
+static void launch_k(F kernel, int which, const KernelTimer* t, dim3 grid, dim3 block,
+                     uint32_t lds, hipStream_t s, Args... args) {
+  const bool timed = t && t->which == which;
+  hipExtLaunchKernelGGL(kernel, grid, block, lds, s, timed ? t->before : nullptr,
+                        timed ? t->after : nullptr, 0, args...);
+}
+
+static void launch_slice(const BatchDev& b, hipStream_t s, const KernelTimer* t = nullptr) {
   if (b.slice_blocks <= 0) return;
   allow_large_lds();
-  hipLaunchKernelGGL(k_slice, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
-                     b.solves, b.slice_prefix, b.n);
+  launch_k(k_slice, 0, t, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
+           (const SolveDev*)b.solves, (const int*)b.slice_prefix, b.n);
 }
 
-static void launch_ladder(const BatchDev& b, hipStream_t s) {
+static void launch_ladder(const BatchDev& b, hipStream_t s, const KernelTimer* t = nullptr) {
   if (b.nlad <= 0) return;
-  hipLaunchKernelGGL(k_ladder, dim3(b.nlad), dim3(256), 0, s, b.solves, b.lad_items);
+  launch_k(k_ladder, 2, t, dim3(b.nlad), dim3(256), 0, s, (const SolveDev*)b.solves,
+           (const int4*)b.lad_items);
 }
 
-static void launch_contract(const BatchDev& b, hipStream_t s) {
+static void launch_contract(const BatchDev& b, hipStream_t s, const KernelTimer* t = nullptr) {
   if (b.ncon <= 0) return;
   const int grid = (b.ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock;
-  hipLaunchKernelGGL(k_contract, dim3(grid), dim3(kElemThreads), 0, s, b.solves, b.con_items,
-                     b.ncon);
+  launch_k(k_contract, 1, t, dim3(grid), dim3(kElemThreads), 0, s, (const SolveDev*)b.solves,
+           (const int4*)b.con_items, b.ncon);
 }
 
 void launch_init(const BatchDev& b, hipStream_t s) {
@@ -1193,28 +1207,15 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
 }
 
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const KernelTimer* timer) {
-  auto mark = [&](int which, bool after) {
-    if (timer && timer->which == which) hipEventRecord(after ? timer->after : timer->before, s);
-  };
-  mark(0, false);
-  launch_slice(b, s);
-  mark(0, true);
-  mark(2, false);
-  launch_ladder(b, s);
-  mark(2, true);
-  mark(1, false);
-  launch_contract(b, s);
-  mark(1, true);
-  mark(3, false);
-  hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
-                     b.elem_prefix, b.n, main_flags);
-  mark(3, true);
+  launch_slice(b, s, timer);
+  launch_ladder(b, s, timer);
+  launch_contract(b, s, timer);
+  launch_k(k_main, 3, timer, dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
+           (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
   if (main_flags & MF_ITERATE) {
     hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, (int)FIN_ITER);
-    mark(4, false);
-    hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
-                       b.elem_prefix, b.n);
-    mark(4, true);
+    launch_k(k_extrap, 4, timer, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s,
+             (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n);
   }
 }
 
