@@ -256,7 +256,7 @@ def main():
         print(json.dumps({"twist_stream": twist_stream(lib, L, args.stream, args.rate)}))
         return
     info = workload_info()
-    heg = init_heg(info)
+    heg = init_heg(info, device=0)
     ctx = CCDContext(heg, info, options(mode="diis", ndiis=8, precision=args.precision))
     res = L.Result()
 

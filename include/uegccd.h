@@ -190,6 +190,16 @@ typedef struct {
 int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao, int32_t* nvec, double* eigen,
                      double* scalars);
 
+/* The same basis built on HIP device `device` (the same results to the last
+ * bit): the |n|^2 <= maxkpoint cube is keyed by |n+twist|^2 in a coalesced
+ * kernel, sorted with its enumeration index in LDS by one block (the stable
+ * order of SetupBasis, source/HEG.f90:180-217), and the orbitals, HF
+ * eigenvalues and E_HF (source/HEG.f90:150-324, 581-615) come from one
+ * kernel; the Madelung sum and the FindTol scalars stay on the host.  Cutoffs
+ * with (2*floor(sqrt(maxkpoint)) + 7)^3 > 8192 return UEGCCD_ERR_UNSUPPORTED. */
+int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t device, int32_t* nao,
+                            int32_t* nvec, double* eigen, double* scalars);
+
 const char* uegccd_strerror(int status);
 
 /* ------------------------------------------------------------------------

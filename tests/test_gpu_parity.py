@@ -185,3 +185,39 @@ def test_diis_history_lengths(ndiis, config_energies):
     r = ctx.solve(tab, tba)
     assert r.status == 0 and not r.failed
     assert abs(r.ecorr - config_energies["n54_cut9"]["ecorr_tight"]) < 1e-9
+
+
+def _basis_cases():
+    from uegccd_amd.twist_stream import stream_requests
+    from uegccd_amd.inputfile import parse_input, standard_input
+    from tests.helpers import SYSTEMS
+    cases = [info_for(name) for name in SYSTEMS]
+    reqs, _ = stream_requests(24, 99)
+    cases += [parse_input(standard_input(nel, ec, rs, tw)) for nel, ec, rs, tw in reqs]
+    cases.append(parse_input(standard_input(14, 40, 1.0, (0.5, 0.5, 0.5))))   # maximal ties
+    cases.append(parse_input(standard_input(38, 16, 3.0, (0.25, -0.125, 0.0), fcut=1.5)))
+    return cases
+
+
+def test_device_basis_equals_host():
+    """uegccd_heg_build_device against the host builder on the configured
+    systems, 24 random stream requests, a half-integer twist (every key tied
+    with its mirror) and an fcut truncation: identical n-vectors, eigenvalues
+    and scalars to the last bit."""
+    from uegccd_amd.driver import init_heg
+    for info in _basis_cases():
+        h = init_heg(info)
+        d = init_heg(info, device=0)
+        assert d.NAO == h.NAO
+        np.testing.assert_array_equal(d.nVec, h.nVec)
+        np.testing.assert_array_equal(d.Eigen, h.Eigen)
+        assert (d.EHF, d.Madelung, d.L, d.TolMax) == (h.EHF, h.Madelung, h.L, h.TolMax)
+
+
+def test_device_basis_rejects_oversized_cube():
+    from uegccd_amd import lib as L
+    from uegccd_amd.driver import init_heg
+    from uegccd_amd.inputfile import parse_input, standard_input
+    with pytest.raises(L.UegccdError) as e:
+        init_heg(parse_input(standard_input(14, 81, 1.0, (0, 0, 0))), device=0)
+    assert e.value.status == L.ERR_UNSUPPORTED

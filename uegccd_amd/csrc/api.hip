@@ -1123,6 +1123,38 @@ double madelung_ewald(double omega, double L) {
   return real + recip + term2 + term4;
 }
 
+// Geometry at r_s = 1, then rescaled to r_s as change_rs does
+// (source/HEG.f90:494-533), and the Madelung constant.
+struct HegGeom {
+  double L1, rescale, L, omega, mad;
+};
+
+HegGeom heg_geometry(const uegccd_heg_params* p) {
+  HegGeom g;
+  g.L1 = 1.0 * std::pow(4.0 / 3.0 * kPi * p->nel, 1.0 / 3.0);
+  g.rescale = p->rs / 1.0;
+  g.L = g.L1 * g.rescale;
+  g.omega = std::pow(g.L1, 3) * std::pow(g.rescale, 3.0);
+  g.mad = madelung_ewald(g.omega, g.L) * p->madelungfactor;
+  return g;
+}
+
+// L, Omega, Madelung, E_HF, ScreeningDistance, DenomFactor, TolMax, FailRatio.
+void heg_scalars(const uegccd_heg_params* p, const HegGeom& g, double ehf, double* scalars) {
+  const double dens = p->nel / std::pow(g.L, 3.0);
+  const double kf = std::pow(3.0 * std::pow(kPi, 2.0) * dens, 1.0 / 3.0);
+  double tolmax, failratio, df;
+  find_tol(p->rs, &tolmax, &failratio, &df);
+  scalars[0] = g.L;
+  scalars[1] = g.omega;
+  scalars[2] = g.mad;
+  scalars[3] = ehf;
+  scalars[4] = std::pow(4.0 / kPi * kf, 0.5);
+  scalars[5] = df;
+  scalars[6] = tolmax;
+  scalars[7] = failratio;
+}
+
 int fcut_count(double f, int nel) {
   int m = (int)(std::pow(f, 1.5) * nel / 2.0);
   const double t = std::pow(f, 1.5) * nel / 2.0;
@@ -1163,12 +1195,8 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
   if (nb <= no) return UEGCCD_ERR_INVALID;
   *nao_out = nb;
   if (!nvec && !eigen && !scalars) return UEGCCD_OK;
-  // geometry at r_s = 1, then rescaled to r_s as change_rs does
-  const double L1 = 1.0 * std::pow(4.0 / 3.0 * kPi * nel, 1.0 / 3.0);
-  const double rescale = p->rs / 1.0;
-  const double L = L1 * rescale;
-  const double omega = std::pow(L1, 3) * std::pow(rescale, 3.0);
-  const double mad = madelung_ewald(omega, L) * p->madelungfactor;
+  const HegGeom g = heg_geometry(p);
+  const double L1 = g.L1, rescale = g.rescale, L = g.L, omega = g.omega, mad = g.mad;
   std::vector<double> kv((size_t)nb * 3);
   for (int q = 0; q < nb; ++q) {
     const PW& w = pw[q < no ? q : q + mc - no];
@@ -1206,18 +1234,7 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
       }
     }
     ehf = ehf * 2.0 + mad * 0.5 * nel;
-    const double dens = nel / std::pow(L, 3.0);
-    const double kf = std::pow(3.0 * std::pow(kPi, 2.0) * dens, 1.0 / 3.0);
-    double tolmax, failratio, df;
-    find_tol(p->rs, &tolmax, &failratio, &df);
-    scalars[0] = L;
-    scalars[1] = omega;
-    scalars[2] = mad;
-    scalars[3] = ehf;
-    scalars[4] = std::pow(4.0 / kPi * kf, 0.5);
-    scalars[5] = df;
-    scalars[6] = tolmax;
-    scalars[7] = failratio;
+    heg_scalars(p, g, ehf, scalars);
   }
   return UEGCCD_OK;
 }
@@ -1681,3 +1698,76 @@ int uegccd_server_poll(const uegccd_server* s, int64_t id, uegccd_request_result
 double uegccd_server_clock_ms(const uegccd_server* s) { return s ? server_now(s) : 0.0; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The same basis built on the device (enumeration, ordering, orbitals, E_HF);
+// geometry, Madelung and the FindTol scalars stay on the host.
+extern "C" int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t device,
+                                       int32_t* nao_out, int32_t* nvec, double* eigen,
+                                       double* scalars) {
+  if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0))
+    return UEGCCD_ERR_INVALID;
+  const int nel = p->nel, no = nel / 2;
+  if (no > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
+  const int cell = (int)std::sqrt((double)p->maxkpoint) + 3;
+  const int w = 2 * cell + 1, total = w * w * w;
+  int npad = 1;
+  while (npad < total) npad <<= 1;
+  if (npad > kMaxBasisPad) return UEGCCD_ERR_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UEGCCD_ERR_HIP;
+  if (device < 0 || device >= ndev) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(device));
+  // key[npad] | idx[npad] | count | nvec[3*npad] | eigen[npad] | ehf
+  const size_t bytes = aligned(sizeof(double) * npad) + aligned(sizeof(int) * npad) + aligned(sizeof(int)) +
+                       aligned(sizeof(int) * 3 * npad) + aligned(sizeof(double) * npad) +
+                       aligned(sizeof(double));
+  char* base = nullptr;
+  if (hipMalloc(&base, bytes) != hipSuccess) return UEGCCD_ERR_ALLOC;
+  DeviceArena a;
+  a.base = base;
+  a.size = bytes;
+  double* d_key = a.take<double>(npad);
+  int* d_idx = a.take<int>(npad);
+  int* d_count = a.take<int>(1);
+  int* d_nvec = a.take<int>(3 * (size_t)npad);
+  double* d_eig = a.take<double>(npad);
+  double* d_ehf = a.take<double>(1);
+  hipStream_t s = nullptr;
+  int rc = UEGCCD_OK;
+  auto done = [&](int r) {
+    if (s) {
+      hipStreamSynchronize(s);
+      hipStreamDestroy(s);
+    }
+    hipFree(base);
+    return r;
+  };
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return done(UEGCCD_ERR_HIP);
+  int count = 0;
+  if (basis_on_device(cell, p->maxkpoint, p->twist, npad, d_key, d_idx, d_count, s) ||
+      hipMemcpyAsync(&count, d_count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(UEGCCD_ERR_HIP);
+  int nb = count;
+  const int mf = fcut_count(p->fcut, nel);
+  if (mf < nb) nb = mf;
+  const int mc = fcut_count(p->corefcut, nel);
+  if ((p->corefcut - 1.0) < 1e-6 && mc != no) return done(UEGCCD_ERR_INVALID);
+  nb = nb - mc + no;
+  if (nb <= no) return done(UEGCCD_ERR_INVALID);
+  *nao_out = nb;
+  if (!nvec && !eigen && !scalars) return done(UEGCCD_OK);
+  const HegGeom g = heg_geometry(p);
+  double ehf = 0.0;
+  if (basis_orbitals_on_device(cell, nb, no, mc, nel, p->twist, g.L1, g.rescale, g.omega, g.mad,
+                               p->gap, p->exchangefactor, d_idx, d_nvec, d_eig, d_ehf, s))
+    return done(UEGCCD_ERR_HIP);
+  if ((nvec && hipMemcpyAsync(nvec, d_nvec, sizeof(int) * 3 * (size_t)nb, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (eigen && hipMemcpyAsync(eigen, d_eig, sizeof(double) * nb, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipMemcpyAsync(&ehf, d_ehf, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(UEGCCD_ERR_HIP);
+  if (scalars) heg_scalars(p, g, ehf, scalars);
+  return done(rc);
+}

@@ -1123,6 +1123,166 @@ __global__ void k_gather_states(const SolveDev* __restrict__ solves, State* __re
 }
 
 // --------------------------------------------------------------------------
+// Plane-wave basis on the device (init_HEG + SetupBasis + change_rs,
+// source/HEG.f90:150-324, 494-533, 581-615).  Every floating-point step is an
+// explicitly rounded operation in the host builder's order (no contraction
+// into fused multiply-adds), so the ordering keys, k-vectors, eigenvalues and
+// E_HF are the host's to the last bit.
+
+// Keys |n + twist|^2 of the cube [-cell, cell]^3 in enumeration order (x
+// slowest), +inf outside |n|^2 <= maxk and in the padding.
+__global__ void k_basis_keys(int cell, int maxk, double tx, double ty, double tz, int npad,
+                             double* __restrict__ key, int* __restrict__ idx) {
+#pragma clang fp contract(off)
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= npad) return;
+  const int w = 2 * cell + 1;
+  double kv = __longlong_as_double(0x7ff0000000000000LL);  // +inf
+  if (e < w * w * w) {
+    const int a = e / (w * w) - cell, b = (e / w) % w - cell, c = e % w - cell;
+    if (a * a + b * b + c * c <= maxk) {
+      const double x = a + tx, y = b + ty, z = c + tz;
+      kv = x * x + y * y + z * z;
+    }
+  }
+  key[e] = kv;
+  idx[e] = e;
+}
+
+// Bitonic sort of (key, enumeration index) pairs in LDS by one block: the
+// index breaks ties, which is the stable order of the reference's sort.  Also
+// counts the finite keys (the plane waves inside the cutoff).
+__global__ __launch_bounds__(1024) void k_basis_sort(int npad, double* __restrict__ key,
+                                                     int* __restrict__ idx, int* __restrict__ count) {
+  extern __shared__ double sk[];
+  int* si = reinterpret_cast<int*>(sk + npad);
+  __shared__ int nfin;
+  if (threadIdx.x == 0) nfin = 0;
+  for (int t = threadIdx.x; t < npad; t += blockDim.x) {
+    sk[t] = key[t];
+    si[t] = idx[t];
+  }
+  __syncthreads();
+  for (int k = 2; k <= npad; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < npad; t += blockDim.x) {
+        const int u = t ^ j;
+        if (u > t) {
+          const bool up = (t & k) == 0;
+          const double ka = sk[t], kb = sk[u];
+          const int ia = si[t], ib = si[u];
+          const bool a_gt_b = ka > kb || (ka == kb && ia > ib);
+          if (a_gt_b == up) {
+            sk[t] = kb; sk[u] = ka;
+            si[t] = ib; si[u] = ia;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  int mine = 0;
+  for (int t = threadIdx.x; t < npad; t += blockDim.x) {
+    key[t] = sk[t];
+    idx[t] = si[t];
+    mine += isfinite(sk[t]) ? 1 : 0;
+  }
+  atomicAdd(&nfin, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) *count = nfin;
+}
+
+struct BasisArgs {
+  int cell, nb, no, mc, nel;
+  double tx, ty, tz;
+  double L1, rescale, omega, mad, gap, xf;
+};
+
+__device__ __forceinline__ void basis_kvec(const BasisArgs& A, const int* idx, int q, double* k,
+                                           int* n) {
+#pragma clang fp contract(off)
+  const int src = q < A.no ? q : q + A.mc - A.no;
+  const int e = idx[src], w = 2 * A.cell + 1;
+  n[0] = e / (w * w) - A.cell;
+  n[1] = (e / w) % w - A.cell;
+  n[2] = e % w - A.cell;
+  const double tw[3] = {A.tx, A.ty, A.tz};
+  const double g = 2.0 * 3.14159265358979323846264338327950288 / A.L1;
+  const double inv = 1.0 / A.rescale;
+  for (int d = 0; d < 3; ++d) k[d] = ((n[d] + tw[d]) * g) * inv;
+}
+
+// Orbitals of the truncated basis: n-vectors (nVec(NAO,3) column-major), HF
+// eigenvalues with the exchange over occupied orbitals and the Madelung term
+// on the occupied diagonal; E_HF by thread 0 of block 0 in the host's order.
+__global__ __launch_bounds__(256) void k_basis_orbitals(BasisArgs A, const int* __restrict__ idx,
+                                                        int* __restrict__ nvec,
+                                                        double* __restrict__ eigen,
+                                                        double* __restrict__ ehf_out) {
+#pragma clang fp contract(off)
+  __shared__ double ko[kMaxOcc * 3];
+  for (int l = threadIdx.x; l < A.no; l += blockDim.x) {
+    int n[3];
+    basis_kvec(A, idx, l, ko + 3 * l, n);
+  }
+  __syncthreads();
+  const double pi4 = 4.0 * 3.14159265358979323846264338327950288;
+  const double inv_omega = 1.0 / A.omega;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < A.nb) {
+    double k1[3];
+    int n[3];
+    basis_kvec(A, idx, q, k1, n);
+    for (int d = 0; d < 3; ++d) nvec[(size_t)d * A.nb + q] = n[d];
+    double e = 0.5 * (k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+    for (int l = 0; l < A.no; ++l) {
+      if (l != q) {
+        const double d0 = k1[0] - ko[3 * l], d1 = k1[1] - ko[3 * l + 1], d2v = k1[2] - ko[3 * l + 2];
+        e = e - inv_omega * (pi4 / (d0 * d0 + d1 * d1 + d2v * d2v)) * A.xf;
+      } else {
+        e = e + A.mad - A.gap;
+      }
+    }
+    eigen[q] = e;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double ehf = 0.0;
+    for (int a = 0; a < A.no; ++a) {
+      const double* k1 = ko + 3 * a;
+      ehf = ehf + 0.5 * (k1[0] * k1[0] + k1[1] * k1[1] + k1[2] * k1[2]);
+      for (int l = 0; l < A.no; ++l) {
+        if (l == a) continue;
+        const double* k2 = ko + 3 * l;
+        const double d0 = k1[0] - k2[0], d1 = k1[1] - k2[1], d2v = k1[2] - k2[2];
+        ehf = ehf - 0.5 * inv_omega * (pi4 / (d0 * d0 + d1 * d1 + d2v * d2v));
+      }
+    }
+    *ehf_out = ehf * 2.0 + A.mad * 0.5 * A.nel;
+  }
+}
+
+int basis_on_device(int cell, int maxk, const double* twist, int npad, double* key, int* idx,
+                    int* count, hipStream_t s) {
+  hipLaunchKernelGGL(k_basis_keys, dim3((npad + 255) / 256), dim3(256), 0, s, cell, maxk, twist[0],
+                     twist[1], twist[2], npad, key, idx);
+  const int lds = npad * (int)(sizeof(double) + sizeof(int));
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_basis_sort),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipLaunchKernelGGL(k_basis_sort, dim3(1), dim3(1024), lds, s, npad, key, idx, count);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const double* twist,
+                             double L1, double rescale, double omega, double mad, double gap,
+                             double xf, const int* idx, int* nvec, double* eigen, double* ehf,
+                             hipStream_t s) {
+  BasisArgs A{cell, nb, no, mc, nel, twist[0], twist[1], twist[2], L1, rescale, omega, mad, gap, xf};
+  hipLaunchKernelGGL(k_basis_orbitals, dim3((nb + 255) / 256), dim3(256), 0, s, A, idx, nvec, eigen,
+                     ehf);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// --------------------------------------------------------------------------
 // Dynamic LDS above the 64 KiB default needs an explicit per-kernel limit.
 static void allow_large_lds() {
   static bool done = false;

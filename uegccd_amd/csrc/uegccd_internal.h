@@ -148,4 +148,16 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
 // One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main, 4 extrapolation.
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
+// Plane-wave basis on the device: keys of the cube [-cell, cell]^3 (npad a
+// power of two >= (2 cell + 1)^3, at most kMaxBasisPad) sorted with their
+// enumeration index, the count of plane waves inside the cutoff; then the nb
+// orbitals (n-vectors column-major, eigenvalues) and E_HF.
+constexpr int kMaxBasisPad = 8192;
+int basis_on_device(int cell, int maxk, const double* twist, int npad, double* key, int* idx,
+                    int* count, hipStream_t s);
+int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const double* twist,
+                             double L1, double rescale, double omega, double mad, double gap,
+                             double xf, const int* idx, int* nvec, double* eigen, double* ehf,
+                             hipStream_t s);
+
 }  // namespace uegccd

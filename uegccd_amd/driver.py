@@ -41,8 +41,10 @@ class HEGData:
     ECorr: float = 0.0
 
 
-def init_heg(info: UEGInfo, twist=None) -> HEGData:
-    """Basis, eigenvalues, Madelung and E_HF through uegccd_heg_build."""
+def init_heg(info: UEGInfo, twist=None, device: int | None = None) -> HEGData:
+    """Basis, eigenvalues, Madelung and E_HF: built on HIP device `device`
+    (uegccd_heg_build_device), or by the host builder (uegccd_heg_build) when
+    device is None; both give the same bits."""
     lib = L.load()
     p = L.HegParams()
     p.nel, p.maxkpoint, p.rs = info.NElectron, info.MaxKPoint, info.rSMin
@@ -51,15 +53,19 @@ def init_heg(info: UEGInfo, twist=None) -> HEGData:
         p.twist[d] = float(tw[d])
     p.fcut, p.corefcut = info.fcut, info.corefcut
     p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+    if device is None:
+        build = lambda *a: lib.uegccd_heg_build(ctypes.byref(p), *a)  # noqa: E731
+        name = "uegccd_heg_build"
+    else:
+        build = lambda *a: lib.uegccd_heg_build_device(ctypes.byref(p), int(device), *a)  # noqa: E731
+        name = "uegccd_heg_build_device"
     nao = ctypes.c_int32()
-    L.check(lib.uegccd_heg_build(ctypes.byref(p), ctypes.byref(nao), None, None, None),
-            "uegccd_heg_build")
+    L.check(build(ctypes.byref(nao), None, None, None), name)
     n = nao.value
     nvec_cm = np.zeros(3 * n, dtype=np.int32)
     eig = np.zeros(n)
     sc = np.zeros(8)
-    L.check(lib.uegccd_heg_build(ctypes.byref(p), ctypes.byref(nao), L.iptr(nvec_cm),
-                                 L.dptr(eig), L.dptr(sc)), "uegccd_heg_build")
+    L.check(build(ctypes.byref(nao), L.iptr(nvec_cm), L.dptr(eig), L.dptr(sc)), name)
     info.NAO = n
     return HEGData(Nel=info.NElectron, NOcc=info.NElectron // 2, NAO=n, L=sc[0], Omega=sc[1],
                    Madelung=sc[2], EHF=sc[3], ScreeningDistance=sc[4], rS=info.rSMin,
@@ -404,7 +410,7 @@ def single_point(input_text: str, opts: L.Options | None = None):
     info = parse_input(input_text)
     if info.DodRPASOSEX:
         raise L.UegccdError(L.ERR_UNSUPPORTED, "dRPA/SOSEX driver")
-    heg = init_heg(info)
+    heg = init_heg(info, device=0)
     ctx = CCDContext(heg, info, opts)
     taa, tab, tba = DrvMBPT(heg, info, ctx)
     e2 = heg.ECorr

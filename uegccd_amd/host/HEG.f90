@@ -1,5 +1,6 @@
-! Plane-wave basis of the single-point calculation, built by libuegccd
-! (uegccd_heg_build: init_HEG + SetupBasis + change_rs of source/HEG.f90).
+! Plane-wave basis of the single-point calculation, built on the GPU by
+! libuegccd (uegccd_heg_build_device: init_HEG + SetupBasis + change_rs of
+! source/HEG.f90).
 Module HEG
   Use, Intrinsic :: ISO_C_Binding
   Use Precision
@@ -29,9 +30,11 @@ Contains
     P%madelungfactor = UEGInfo%MadelungFactor
     P%gap = UEGInfo%Gap
     P%exchangefactor = UEGInfo%ExchangeFactor
-    Call Check(uegccd_heg_build(P, NAO, c_null_ptr, c_null_ptr, c_null_ptr), 'uegccd_heg_build')
+    Call Check(uegccd_heg_build_device(P, 0_c_int32_t, NAO, c_null_ptr, c_null_ptr, c_null_ptr), &
+               'uegccd_heg_build_device')
     Allocate(NVecC(3*NAO), Eig(NAO))
-    Call Check(uegccd_heg_build(P, NAO, c_loc(NVecC), c_loc(Eig), c_loc(Scal)), 'uegccd_heg_build')
+    Call Check(uegccd_heg_build_device(P, 0_c_int32_t, NAO, c_loc(NVecC), c_loc(Eig), c_loc(Scal)), &
+               'uegccd_heg_build_device')
     UEGInfo%NAO = NAO
     HEGData%Nel = UEGInfo%NElectron
     HEGData%rS = UEGInfo%rSMin

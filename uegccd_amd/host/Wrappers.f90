@@ -50,6 +50,15 @@ Module Wrappers
       Type(c_ptr), Value :: nvec, eigen, scalars
     End Function
 
+    Integer(c_int) Function uegccd_heg_build_device(p, device, nao, nvec, eigen, scalars) &
+        Bind(C, name="uegccd_heg_build_device")
+      Import :: uegccd_heg_params, c_int, c_int32_t, c_ptr
+      Type(uegccd_heg_params), Intent(In) :: p
+      Integer(c_int32_t), Value :: device
+      Integer(c_int32_t), Intent(Out) :: nao
+      Type(c_ptr), Value :: nvec, eigen, scalars
+    End Function
+
     Integer(c_int) Function uegccd_ctx_create(sys, opt, ctx) Bind(C, name="uegccd_ctx_create")
       Import :: uegccd_system, uegccd_options, c_int, c_ptr
       Type(uegccd_system), Intent(In) :: sys

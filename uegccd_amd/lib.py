@@ -28,6 +28,7 @@ EXPORTS = (
     "uegccd_server_create", "uegccd_server_destroy", "uegccd_server_submit",
     "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
     "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
+    "uegccd_heg_build_device",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -122,6 +123,7 @@ def load():
     lib.uegccd_trace.argtypes = [P, D, D, ctypes.c_int32]
     lib.uegccd_mp2.argtypes = [P, D, D, D, D]
     lib.uegccd_heg_build.argtypes = [ctypes.POINTER(HegParams), I32, I32, D, D]
+    lib.uegccd_heg_build_device.argtypes = [ctypes.POINTER(HegParams), ctypes.c_int32, I32, I32, D, D]
     lib.uegccd_strerror.argtypes = [ctypes.c_int]
     lib.uegccd_strerror.restype = ctypes.c_char_p
     lib.uegccd_solve_batch.argtypes = [ctypes.POINTER(P), ctypes.c_int32, ctypes.POINTER(D),
@@ -147,7 +149,8 @@ def load():
                  "uegccd_solve_batch", "uegccd_solve_resident", "uegccd_get_amplitudes",
                  "uegccd_kernel_time", "uegccd_footprint", "uegccd_server_create",
                  "uegccd_server_submit", "uegccd_server_step", "uegccd_server_drain",
-                 "uegccd_server_poll", "uegccd_time_kernel", "uegccd_kernel_stats"):
+                 "uegccd_server_poll", "uegccd_time_kernel", "uegccd_kernel_stats",
+                 "uegccd_heg_build_device"):
         getattr(lib, name).restype = ctypes.c_int
     _LIB = lib
     return lib
