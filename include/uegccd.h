@@ -172,6 +172,14 @@ int uegccd_trace(const uegccd_ctx* ctx, double* e, double* dt, int32_t n);
 /* DrvMBPT amplitudes and E(2) (source/MP2.f90:56-98).  Any array may be NULL. */
 int uegccd_mp2(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, double* t2abba, double* e2);
 
+/* DrvMBPT's connectivity (source/MP2.f90:75-85): counts[k] is the number of
+ * (i,j,a,b) excitations with |n_i - n_a|^2 = k+1 plus those with
+ * |n_i - n_b|^2 = k+1, over the valid (i,j,a); counted on the device.
+ * nbins <= 4096 (the reference uses 400); a transfer beyond nbins returns
+ * UEGCCD_ERR_INVALID, where the reference stops ("Connectivity greater than
+ * current ConnectivityLimit"). */
+int uegccd_connectivity(uegccd_ctx* ctx, int32_t* counts, int32_t nbins);
+
 /* Plane-wave basis of init_HEG + change_rs (source/HEG.f90:16-43,150-324,
  * 494-533): enumeration |n|^2 <= maxkpoint, stable ordering by |n+twist|^2,
  * fcut/corefcut truncation, HF eigenvalues with the Madelung term, E_HF.

@@ -112,6 +112,10 @@ def main(root="/root/reference"):
         "ta_final_ehf": mp2s[11]["ehf"],
         "ta_final_trace": ccds[1]["trace"], "ta_final_niter": ccds[1]["niter"],
         "ta_final_ecorr": ccds[1]["ecorr"], "ta_final_maxres": ccds[1]["maxres"],
+        # the whole fort.59 of the twist-averaging run, as printed (per-twist
+        # E_HF/E(2)/E_c, averages, squared averages, errors, eigenvalue and
+        # connectivity averages, connectivity distances, the special twist)
+        "ta_fort59_lines": read(root, "source/Test/fort.59"),
     }
     # --- N=14, r_s=1, Gamma, cutoff 5 (benchmarks/ZRun-dRPA) ---
     blk = output_blocks(read(root, "benchmarks/ZRun-dRPA/Output"))

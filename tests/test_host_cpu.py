@@ -141,3 +141,36 @@ def test_corefactor_and_fcut2ecut_derivations():
     assert info.CoreN == 3  # int(14/2 * 0.5)
     # Nfermi = 14^(2/3)/4 < 10 -> MaxKPoint = int(10 * fcut * 2)
     assert info.MaxKPoint == 60
+
+
+# --- fort.59 records (Fortran list-directed output) --------------------------
+
+def _f59_items(line):
+    import re
+    toks = line.split()
+    items, text = [], []
+    for t in toks:
+        if re.match(r"^[-+]?\d+$", t):
+            items.append(int(t))
+        elif re.match(r"^[-+]?\d*\.\d+(E[-+]\d+)?$", t):
+            items.append(float(t))
+        else:
+            text.append(t)
+    return items, " ".join(text)
+
+
+def test_fort59_records_reproduce_reference_bytes(fixtures):
+    """Every line of the reference's twist-averaging fort.59, re-printed from
+    its own numbers by uegccd_amd.fortran_io, is byte-identical."""
+    from uegccd_amd.fortran_io import record
+    lines = fixtures["n24"]["ta_fort59_lines"]
+    assert len(lines) == 27
+    for line in lines:
+        items, text = _f59_items(line)
+        if line.strip().startswith(("special", "lowest")):
+            out = record(text, items)
+        elif text:
+            out = record(items, text)
+        else:
+            out = record(*items)
+        assert out == line, (out, line)

@@ -1049,6 +1049,25 @@ extern "C" int uegccd_solve_resident(uegccd_ctx* c, int32_t from_mp2, uegccd_res
   return fill_result(c, st[0], ms, res);
 }
 
+extern "C" int uegccd_connectivity(uegccd_ctx* c, int32_t* counts, int32_t nbins) {
+  if (!c || !counts || nbins <= 0 || nbins > 4096) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  int* d = nullptr;
+  if (hipMalloc(&d, sizeof(int) * (nbins + 1)) != hipSuccess) return UEGCCD_ERR_ALLOC;
+  std::vector<int> h(nbins + 1, 0);
+  int rc = UEGCCD_OK;
+  if (hipMemsetAsync(d, 0, sizeof(int) * (nbins + 1), c->stream) != hipSuccess ||
+      connectivity_on_device(c->batch, c->n3, nbins, d, d + nbins, c->stream) ||
+      hipMemcpyAsync(h.data(), d, sizeof(int) * (nbins + 1), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = UEGCCD_ERR_HIP;
+  hipFree(d);
+  if (rc) return rc;
+  if (h[nbins]) return UEGCCD_ERR_INVALID;  // a transfer beyond the histogram
+  for (int k = 0; k < nbins; ++k) counts[k] = h[k];
+  return UEGCCD_OK;
+}
+
 extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
   if (!c) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(c->device));

@@ -1123,6 +1123,46 @@ __global__ void k_gather_states(const SolveDev* __restrict__ solves, State* __re
 }
 
 // --------------------------------------------------------------------------
+// MP2 connectivity (DrvMBPT, source/MP2.f90:75-85): for every valid (i,j,a)
+// with partner b, one count at |n_i - n_a|^2 and one at |n_i - n_b|^2.
+// Per-block histograms in LDS, flushed with integer atomics (exact, any order).
+constexpr int kMaxConnBins = 4096;
+__global__ __launch_bounds__(256) void k_connectivity(const SolveDev* __restrict__ solves,
+                                                      int nbins, int* __restrict__ hist,
+                                                      int* __restrict__ overflow) {
+  __shared__ int h[kMaxConnBins];
+  const SolveDev& S = solves[0];
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < S.n3) {
+    const int b = S.pidx[x];
+    if (b >= 0) {
+      const int o = S.o, i = x % o, a = o + x / (o * o);
+      const int4 ni = S.nv4[i];
+      const int d1 = d2(ni, S.nv4[a]), d2b = d2(ni, S.nv4[b]);
+      if (d1 > nbins || d2b > nbins) {
+        atomicOr(overflow, 1);
+      } else {
+        atomicAdd(&h[d1 - 1], 1);
+        atomicAdd(&h[d2b - 1], 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+int connectivity_on_device(const BatchDev& b, int n3, int nbins, int* hist, int* overflow,
+                           hipStream_t s) {
+  if (nbins > kMaxConnBins) return -1;
+  hipLaunchKernelGGL(k_connectivity, dim3((n3 + 255) / 256), dim3(256), 0, s, b.solves, nbins, hist,
+                     overflow);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// --------------------------------------------------------------------------
 // Plane-wave basis on the device (init_HEG + SetupBasis + change_rs,
 // source/HEG.f90:150-324, 494-533, 581-615).  Every floating-point step is an
 // explicitly rounded operation in the host builder's order (no contraction

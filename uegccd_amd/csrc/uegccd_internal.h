@@ -148,6 +148,11 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
 // One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main, 4 extrapolation.
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
+// MP2 connectivity histogram of a batch-of-one solve (nbins <= 4096 bins,
+// bin k counts |dn|^2 = k+1); hist and overflow zeroed by the caller.
+int connectivity_on_device(const BatchDev& b, int n3, int nbins, int* hist, int* overflow,
+                           hipStream_t s);
+
 // Plane-wave basis on the device: keys of the cube [-cell, cell]^3 (npad a
 // power of two >= (2 cell + 1)^3, at most kMaxBasisPad) sorted with their
 // enumeration index, the count of plane waves inside the cutoff; then the nb

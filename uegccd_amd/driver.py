@@ -174,6 +174,13 @@ class CCDContext:
                 raise L.UegccdError(st, "uegccd_solve")
         return SolveResult.from_c(r, taa, tab, tba, self.trace(r.niter))
 
+    def connectivity(self, nbins=400):
+        """DrvMBPT's connectivity histogram (uegccd_connectivity): bin k-1
+        counts the excitations with |n_i - n_a|^2 = k and |n_i - n_b|^2 = k."""
+        h = np.zeros(nbins, dtype=np.int32)
+        L.check(self._lib.uegccd_connectivity(self._h, L.iptr(h), nbins), "uegccd_connectivity")
+        return h
+
     def trace(self, n):
         e, dt = np.zeros(max(n, 1)), np.zeros(max(n, 1))
         L.check(self._lib.uegccd_trace(self._h, L.dptr(e), L.dptr(dt), n), "uegccd_trace")

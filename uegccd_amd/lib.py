@@ -28,7 +28,7 @@ EXPORTS = (
     "uegccd_server_create", "uegccd_server_destroy", "uegccd_server_submit",
     "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
     "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
-    "uegccd_heg_build_device",
+    "uegccd_heg_build_device", "uegccd_connectivity",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -141,6 +141,7 @@ def load():
     lib.uegccd_server_drain.argtypes = [P]
     lib.uegccd_server_poll.argtypes = [P, ctypes.c_int64, ctypes.POINTER(RequestResult)]
     lib.uegccd_time_kernel.argtypes = [P, ctypes.c_int32]
+    lib.uegccd_connectivity.argtypes = [P, I32, ctypes.c_int32]
     lib.uegccd_kernel_stats.argtypes = [P, D, I64]
     lib.uegccd_server_clock_ms.argtypes = [P]
     lib.uegccd_server_clock_ms.restype = ctypes.c_double
@@ -150,7 +151,7 @@ def load():
                  "uegccd_kernel_time", "uegccd_footprint", "uegccd_server_create",
                  "uegccd_server_submit", "uegccd_server_step", "uegccd_server_drain",
                  "uegccd_server_poll", "uegccd_time_kernel", "uegccd_kernel_stats",
-                 "uegccd_heg_build_device"):
+                 "uegccd_heg_build_device", "uegccd_connectivity"):
         getattr(lib, name).restype = ctypes.c_int
     _LIB = lib
     return lib
