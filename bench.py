@@ -33,7 +33,7 @@ METRIC = "CCD solve time & twist-solves/sec at |ΔE|<1e-6 Ha vs FP64; p50/p99 la
 WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
-KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_extrap"}
+KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_extrap", 5: "k_finalize"}
 def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
     """Algorithmic (flops, bytes) of one launch, each datum moved once.
     k_slice: per slice, TV/VT (8 o^3 flops) and the quadratic J intermediates
@@ -44,7 +44,7 @@ def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
     and updates the ladder (4 flops per element and pair).  k_main reads T, the
     ladder, two slice products, the quadratic sums and 8 DIIS error entries and
     writes one T and one error entry; k_extrap reads 8 history entries and
-    writes T."""
+    writes T; k_finalize reads the 4 + 8 partials of every element block."""
     n3 = o * o * v
     if kernel == "k_slice":
         return (20.0 * o * n3, 16 * n3 + 32 * n3 + 2 * 32 * n3)
@@ -58,6 +58,9 @@ def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
         return (60.0 * nvalid, (16 + 16 + 64 + 32 + 8 * 16 + 32) * nvalid)
     if kernel == "k_extrap":
         return (32.0 * nvalid, (8 * 16 + 16) * nvalid)
+    if kernel == "k_finalize":
+        nblk = -(-n3 // 256)
+        return (12.0 * nblk, 8 * 12 * nblk)
     raise KeyError(kernel)
 
 
@@ -345,7 +348,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (deterministic plane-wave basis, MP2 guess on device)",
         "config": {"workload": "N=54 r_s=2.0 Gamma |n|^2<=16 (257 orbitals, 27 occ) FP64 CCD, "
-                               "DIIS 8, |dE|<1e-10 & max|R|<1e-8",
+                               "DIIS 8 (undamped steps), |dE|<1e-10 & max|R|<1e-8",
                    "precision": args.precision, "niter": niter, "ecorr": ecorr,
                    "abs_dE_vs_fp64_oracle": de, "niter_reference_mode": niter_ref,
                    "reference_mode_solve_ms": ref_mode_ms},

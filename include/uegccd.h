@@ -87,7 +87,8 @@ typedef struct {
     double tol_dt;         /* <= 0: FindTol's TolMax (source/HEG.f90:1073)    */
     double tol_e;          /* DIIS mode energy tolerance (Ha)                */
     double tol_r;          /* DIIS mode max|R| tolerance                     */
-    double damping;        /* <= 0: FindTol's DenomFactor (2, or 5 if rs>5.1) */
+    double damping;        /* <= 0: FindTol's DenomFactor (2, or 5 if rs>5.1)
+                              in reference mode, 1 (undamped) in DIIS mode   */
     int32_t precision;     /* UEGCCD_PREC_*                                  */
     double mixed_switch;   /* MIXED: FP32 ladders while max|R| >= this       */
     int32_t device;        /* HIP device ordinal                             */
@@ -155,9 +156,10 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
  * in the reference; measurement support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
 
-/* In-solve timing of one kernel (numbering as uegccd_kernel_time; -1 turns it
- * off): every pass of the context's later solves brackets that kernel with HIP
- * events on the context's stream; uegccd_kernel_stats returns the summed
+/* In-solve timing of one kernel (numbering as uegccd_kernel_time, plus 5 for
+ * the per-solve reductions, convergence test and DIIS solve; -1 turns it
+ * off): every pass of the context's later solves
+ * brackets that kernel with HIP events on the context's stream; uegccd_kernel_stats returns the summed
  * device time and the number of such launches since the last call (passes in
  * which some solve was still iterating; the passes launched after the last
  * solve finished return at once and are not counted), and resets them.  No

@@ -456,7 +456,11 @@ int plan_ctx(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx*
   if (rc) { delete c; return rc; }
   find_tol(sys->rs, &c->tolmax, &c->failratio, &c->df);
   if (opt.tol_dt > 0.0) c->tolmax = opt.tol_dt;
+  // DIIS mode steps undamped unless told otherwise: the extrapolation supplies
+  // the stability the damping gives the plain iteration (measured over r_s
+  // 0.5-50, N 14-54: fewer iterations in nearly every case, none diverging)
   if (opt.damping > 0.0) c->df = opt.damping;
+  else if (opt.mode == UEGCCD_MODE_DIIS) c->df = 1.0;
   c->maxiter = opt.max_iter > 0 ? opt.max_iter : 1000;
   *out = c;
   return UEGCCD_OK;
@@ -1075,7 +1079,7 @@ extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, do
 }
 
 extern "C" int uegccd_time_kernel(uegccd_ctx* c, int32_t which) {
-  if (!c || which < -1 || which > 4) return UEGCCD_ERR_INVALID;
+  if (!c || which < -1 || which > 5) return UEGCCD_ERR_INVALID;
   c->timed_kernel = which;
   c->timed_us = 0.0;
   c->timed_launches = 0;

@@ -21,7 +21,7 @@
 //   tcur, lad, histT/histE entries   double2  (abab, abba) or (S, P) per (i,j,a)
 //   tv                               double4  (TV_ab, TV_ba, VT_ab, VT_ba)
 //   jring / jxring                   double4  (J1, J2, J3, 0) per (s, i, l)
-//   histT / histE                    element-major: slot k of x at x*ndiis + k
+//   histT / histE                    slot-major: slot k of x at k*n3 + x
 #include <hip/hip_ext.h>
 
 #include "uegccd_internal.h"
@@ -456,9 +456,11 @@ __device__ void ladder_block(const SolveDev* __restrict__ solves, const int4* __
                              int item, double* vt, double* red) {
   const int4 it = items[item];
   const SolveDev& S = solves[it.x];
-  if (S.st->phase == PH_DONE || !S.do_ladder) return;
+  if (!S.do_ladder) return;
+  const int phase = S.st->phase;  // requested with the table, tested after it
   for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
   __syncthreads();
+  if (phase == PH_DONE) return;
   if (S.st->prec64)
     ladder_tile<double>(S, it.y, it.z, it.w, vt, red);
   else
@@ -677,16 +679,18 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
-  if (S.st->phase == PH_DONE) return;
+  // the state, the element's partner and the LDS tables are requested
+  // together; a finished solve leaves after they arrive
+  const int phase = S.st->phase, slot = S.st->slot;
   const int lb = gb - prefix[s];
   const int o = S.o, o2 = o * o, n3 = S.n3;
-  stage(S, L);
   const int x = lb * kElemPerBlock + threadIdx.x;
-  const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
-  const int slot = S.st->slot;
-  const double2* __restrict__ T = S.tcur;
   int b = -1;
   if (x < n3) b = S.pidx[x];
+  stage(S, L);
+  if (phase == PH_DONE) return;
+  const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
+  const double2* __restrict__ T = S.tcur;
   const bool valid = b >= 0;
   const int i = x % o, j = (x / o) % o, av = x / o2, a = o + av, bv = b - o;
 
@@ -753,10 +757,10 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       if (S.rout) { S.rout[x] = rab; S.rout[n3 + x] = rba; }
     }
     if (flags & MF_ITERATE) {
-      S.histT[(size_t)x * S.ndiis + slot] = make_double2(nab, nba);
+      S.histT[(size_t)slot * n3 + x] = make_double2(nab, nba);
       if (S.diis_err_is_dt) { eab = nab - t_ab; eba = nba - t_ba; }
       else { eab = rab; eba = rba; }
-      if (S.mode == UEGCCD_MODE_DIIS) S.histE[(size_t)x * S.ndiis + slot] = make_double2(eab, eba);
+      if (S.mode == UEGCCD_MODE_DIIS) S.histE[(size_t)slot * n3 + x] = make_double2(eab, eba);
     }
   }
   // block partials: max dT, max |R|, sum E, non-finite count, DIIS dots
@@ -766,7 +770,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   v[1] = r_part;
   v[2] = e_part;
   v[3] = bad;
-  const double2* HE = S.histE + (size_t)(lead ? x : 0) * S.ndiis;
+  const double2* HE = S.histE + (lead ? x : 0);
 #pragma unroll
   for (int k = 0; k < kMaxDiis; ++k) {
     double dk = 0.0;
@@ -774,7 +778,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       if (k == slot) {
         dk = eab * eab + eba * eba;
       } else {
-        const double2 h = HE[k];
+        const double2 h = HE[(size_t)k * n3];
         dk = eab * h.x + eba * h.y;
       }
     }
@@ -801,12 +805,12 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
 // unrolled), pivots and row swaps go through cross-lane shuffles.
 // M is the unrolled system size (at least nh + 1).
 template <int M>
-__device__ bool diis_coefficients(const State& st, int ndiis, double* coef) {
+__device__ bool diis_coefficients(const double* bmat, int nh, int slot_cur, int ndiis, double* coef) {
   const int lane = threadIdx.x & 63;
-  const int nh = st.nhist, m = nh + 1;
-  auto slot_of = [&](int r) { return ((st.slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis; };
+  const int m = nh + 1;
+  auto slot_of = [&](int r) { return ((slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis; };
   const int slr = lane < nh ? slot_of(lane) : 0;
-  double scale = lane < nh ? fabs(st.bmat[slr * kMaxDiis + slr]) : 0.0;
+  double scale = lane < nh ? fabs(bmat[slr * kMaxDiis + slr]) : 0.0;
   scale = wave_max(scale);
   if (!(scale > 0.0) || !isfinite(scale)) return false;
   double row[M];
@@ -814,7 +818,7 @@ __device__ bool diis_coefficients(const State& st, int ndiis, double* coef) {
   for (int c = 0; c < M; ++c) {
     double val = 0.0;
     if (lane < nh) {
-      if (c < nh) val = st.bmat[slr * kMaxDiis + slot_of(c)] / scale;
+      if (c < nh) val = bmat[slr * kMaxDiis + slot_of(c)] / scale;
       else if (c == nh) val = -1.0;
     } else if (lane == nh && c < nh) {
       val = -1.0;
@@ -880,12 +884,11 @@ __device__ bool diis_coefficients(const State& st, int ndiis, double* coef) {
 // scaled by its largest diagonal and padded to 8 x 8 with the identity; a
 // Cholesky factorisation and two triangular solves, unrolled in registers.
 // A pivot below 1e-14 fails the extrapolation (the plain update is used).
-__device__ bool diis_cholesky8(const State& st, int ndiis, double* coef) {
+__device__ bool diis_cholesky8(const double* bmat, int nh, int slot_cur, int ndiis, double* coef) {
   constexpr int M = 8;
-  const int nh = st.nhist;
   int sl[M];  // history slot of row r, oldest first
   {
-    int sv = st.slot_cur - (nh - 1);
+    int sv = slot_cur - (nh - 1);
     if (sv < 0) sv += ndiis;
 #pragma unroll
     for (int r = 0; r < M; ++r) {
@@ -896,15 +899,15 @@ __device__ bool diis_cholesky8(const State& st, int ndiis, double* coef) {
   double scale = 0.0;
 #pragma unroll
   for (int r = 0; r < M; ++r)
-    if (r < nh) scale = fmax(scale, fabs(st.bmat[sl[r] * (kMaxDiis + 1)]));
+    if (r < nh) scale = fmax(scale, fabs(bmat[sl[r] * (kMaxDiis + 1)]));
   if (!(scale > 0.0) || !isfinite(scale)) return false;
   const double inv_scale = 1.0 / scale;
-  double L[M][M];
+  double L[M][M], inv_d[M];
 #pragma unroll
   for (int r = 0; r < M; ++r)
 #pragma unroll
     for (int c = 0; c < M; ++c)
-      L[r][c] = (r < nh && c < nh) ? st.bmat[sl[r] * kMaxDiis + sl[c]] * inv_scale
+      L[r][c] = (r < nh && c < nh) ? bmat[sl[r] * kMaxDiis + sl[c]] * inv_scale
                                    : (r == c ? 1.0 : 0.0);
 #pragma unroll
   for (int k = 0; k < M; ++k) {
@@ -914,6 +917,7 @@ __device__ bool diis_cholesky8(const State& st, int ndiis, double* coef) {
     if (!(d > 1e-14)) return false;
     const double lkk = sqrt(d), inv_l = 1.0 / lkk;
     L[k][k] = lkk;
+    inv_d[k] = inv_l;
 #pragma unroll
     for (int i = k + 1; i < M; ++i) {
       double v = L[i][k];
@@ -928,173 +932,250 @@ __device__ bool diis_cholesky8(const State& st, int ndiis, double* coef) {
     double v = 1.0;
 #pragma unroll
     for (int j = 0; j < i; ++j) v -= L[i][j] * y[j];
-    y[i] = v / L[i][i];
+    y[i] = v * inv_d[i];
   }
 #pragma unroll
   for (int i = M - 1; i >= 0; --i) {
     double v = y[i];
 #pragma unroll
     for (int j = i + 1; j < M; ++j) v -= L[j][i] * z[j];
-    z[i] = v / L[i][i];
+    z[i] = v * inv_d[i];
   }
   double sum = 0.0;
 #pragma unroll
   for (int r = 0; r < M; ++r)
     if (r < nh) sum += z[r];
   if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
+  const double inv_sum = 1.0 / sum;
   for (int k = 0; k < ndiis; ++k) coef[k] = 0.0;
 #pragma unroll
   for (int r = 0; r < M; ++r) {
     if (r >= nh) continue;
-    const double c = z[r] / sum;
+    const double c = z[r] * inv_sum;
     if (!isfinite(c)) return false;
     coef[sl[r]] = c;
   }
   return true;
 }
 
-__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
-  __shared__ double vals[4 + kMaxDiis];
-  __shared__ int want_diis;
-  // The iteration state is worked on in LDS: one parallel copy in, one out,
-  // instead of a chain of dependent global accesses by a single thread.
-  __shared__ State sst;
-  const SolveDev& S = solves[blockIdx.x];
-  const int phase0 = S.st->phase;
-  if (kind == FIN_ITER && phase0 == PH_DONE) return;
-  constexpr int kStateWords = sizeof(State) / sizeof(double);
-  for (int k = threadIdx.x; k < kStateWords; k += blockDim.x)
-    reinterpret_cast<double*>(&sst)[k] = reinterpret_cast<const double*>(S.st)[k];
-  State* st = &sst;
-  const int nb = S.nblk_elem;
-  const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS && phase0 == PH_ITER) ? S.ndiis : 0;
-  {
-    // Slot-major partials: quantity q of every element block is contiguous.
-    // Wave w reduces quantities w, w + nw, ...: q = 1 (max dT) and q = 2
-    // (max |R|) are maxima; E (0), the non-finite count (3) and the DIIS dots
-    // (4..) are sums.  Lane-strided, then a wave tree: a fixed order.
-    const int nq = 4 + nd;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    for (int q = wave; q < nq; q += nw) {
-      const bool is_max = q == 1 || q == 2;
-      const double* P = partial(S, q, 0);
-      double acc = 0.0;
-      for (int k = lane; k < nb; k += 64) acc = is_max ? fmax(acc, P[k]) : acc + P[k];
-      acc = is_max ? wave_max(acc) : wave_sum(acc);
-      if (lane == 0) vals[q] = acc;
+// Sums and maxima of one solve's element-block partials by one block, in a
+// fixed order.  Slot-major partials: quantity q of every element block is
+// contiguous; q = 1 (max dT) and q = 2 (max |R|) are maxima, E (0), the
+// non-finite count (3) and the nd DIIS dots (4..) sums.  Each thread takes
+// blocks t, t + blockDim, ... and loads every quantity of a block at once, so
+// the loads of a round are in flight together (the partials were written on
+// other XCDs and miss this L2).  Thread 0 receives vals[0..3+nd] (E, max dT,
+// max |R|, non-finite count, dots).  red holds (blockDim/64)*kPartialStride.
+__device__ void reduce_partials(const SolveDev& S, int nd, double* vals, double* red) {
+  const int nq = 4 + nd, nb = S.nblk_elem;
+  double acc[kPartialStride];  // dT, max|R|, E, non-finite, dots
+#pragma unroll
+  for (int q = 0; q < kPartialStride; ++q) acc[q] = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    double p[kPartialStride];
+#pragma unroll
+    for (int q = 0; q < kPartialStride; ++q) {
+      const int src = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q;
+      p[q] = src < nq ? *partial(S, src, k) : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < kPartialStride; ++q) acc[q] = q < 2 ? fmax(acc[q], p[q]) : acc[q] + p[q];
   }
-  __syncthreads();
-  if (threadIdx.x == 0) want_diis = 0;
-  __syncthreads();
+  block_reduce_multi<kPartialStride, 2>(acc, red);
   if (threadIdx.x == 0) {
-    const double e = vals[0], dt = vals[1], rmax = vals[2];
-    if (vals[3] > 0.0) st->nonfinite = 1;
-    if (kind == FIN_SUM) {
-      st->ecorr = e;
-    } else if (kind == FIN_INIT) {
-      st->niter = 1;
-      st->ecorr = e;
-      st->ecorr_prev = e;
-      st->e_guess = e;
-      st->dt = dt;
-      st->slot = 0;
-      st->slot_cur = 0;
-      st->nhist = 0;
-      st->use_coef = 0;
-      st->converged = 0;
-      if (S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
-      const bool go = S.mode == UEGCCD_MODE_REFERENCE ? (dt >= S.tol_dt) : true;
-      st->phase = (go && st->niter < S.maxiter && !st->nonfinite) ? PH_ITER : PH_FINAL;
-      if (!go) st->converged = 1;
-    } else if (phase0 == PH_FINAL) {
-      st->maxres = rmax;
-      st->phase = PH_DONE;
-    } else {
-      st->niter += 1;
-      st->ecorr_prev = st->ecorr;
-      st->ecorr = e;
-      st->dt = dt;
-      st->maxres_in = rmax;
-      if (S.trace && st->niter <= S.maxiter) {
-        S.trace[2 * (st->niter - 1)] = e;
-        S.trace[2 * (st->niter - 1) + 1] = dt;
-      }
-      const int slot = st->slot;
-      st->slot_cur = slot;
-      st->slot = (slot + 1) % S.ndiis;
-      bool switched = false;
-      if (!st->prec64) {
-        st->iters_fp32 += 1;
-        if (rmax < S.mixed_switch) { st->prec64 = 1; switched = true; }
-      }
-      bool conv;
-      if (S.mode == UEGCCD_MODE_REFERENCE)
-        conv = dt < S.tol_dt;
-      else
-        conv = st->prec64 && !switched && fabs(e - st->ecorr_prev) < S.tol_e && rmax < S.tol_r;
-      const bool stop = conv || st->niter >= S.maxiter;
-      st->use_coef = 0;
-      if (S.mode == UEGCCD_MODE_DIIS) {
-        if (switched) {
-          st->nhist = 0;  // residuals of the FP32 phase do not enter FP64 extrapolation
-        } else {
-          st->nhist = st->nhist < S.ndiis ? st->nhist + 1 : S.ndiis;
-          for (int k = 0; k < S.ndiis; ++k) {
-            st->bmat[slot * kMaxDiis + k] = vals[4 + k];
-            st->bmat[k * kMaxDiis + slot] = vals[4 + k];
-          }
-          if (!stop && st->niter >= S.diis_start && st->nhist >= 2 && !st->nonfinite)
-            want_diis = 1;
-        }
-      }
-      if (st->nonfinite) {
-        st->phase = PH_DONE;
-      } else {
-        if (conv) st->converged = 1;
-        if (stop) st->phase = PH_FINAL;
-      }
-    }
+    vals[0] = acc[2];
+    vals[1] = acc[0];
+    vals[2] = acc[1];
+    vals[3] = acc[3];
+#pragma unroll
+    for (int q = 4; q < kPartialStride; ++q) vals[q] = acc[q];
   }
-  __syncthreads();
-  if (want_diis && threadIdx.x < 64) {
-    if (S.ndiis <= 8) {
-      if (threadIdx.x == 0) st->use_coef = diis_cholesky8(*st, S.ndiis, st->coef) ? 1 : 0;
-    } else {
-      const bool ok = diis_coefficients<kMaxDiis + 1>(*st, S.ndiis, st->coef);
-      if (threadIdx.x == 0) st->use_coef = ok ? 1 : 0;
-    }
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < kStateWords; k += blockDim.x)
-    reinterpret_cast<double*>(S.st)[k] = reinterpret_cast<const double*>(&sst)[k];
 }
 
-// Amplitudes for the next pass: DIIS extrapolation or the plain update.
+// The iteration bookkeeping of source/CCD.f90:80-99 after a residual pass, on
+// the state copy in LDS, by one thread: NIter, E, dT, convergence, the mixed
+// precision switch, the DIIS history and its B matrix row, the trace line.
+// Returns whether the next amplitudes are to be extrapolated.
+__device__ bool advance_state(const SolveDev& S, State* st, const double* vals, int phase0) {
+  const double e = vals[0], dt = vals[1], rmax = vals[2];
+  if (vals[3] > 0.0) st->nonfinite = 1;
+  if (phase0 == PH_FINAL) {
+    st->maxres = rmax;
+    st->phase = PH_DONE;
+    return false;
+  }
+  st->niter += 1;
+  st->ecorr_prev = st->ecorr;
+  st->ecorr = e;
+  st->dt = dt;
+  st->maxres_in = rmax;
+  if (S.trace && st->niter <= S.maxiter) {
+    S.trace[2 * (st->niter - 1)] = e;
+    S.trace[2 * (st->niter - 1) + 1] = dt;
+  }
+  const int slot = st->slot;
+  st->slot_cur = slot;
+  st->slot = (slot + 1) % S.ndiis;
+  bool switched = false;
+  if (!st->prec64) {
+    st->iters_fp32 += 1;
+    if (rmax < S.mixed_switch) { st->prec64 = 1; switched = true; }
+  }
+  bool conv;
+  if (S.mode == UEGCCD_MODE_REFERENCE)
+    conv = dt < S.tol_dt;
+  else
+    conv = st->prec64 && !switched && fabs(e - st->ecorr_prev) < S.tol_e && rmax < S.tol_r;
+  const bool stop = conv || st->niter >= S.maxiter;
+  st->use_coef = 0;
+  bool want = false;
+  if (S.mode == UEGCCD_MODE_DIIS) {
+    if (switched) {
+      st->nhist = 0;  // residuals of the FP32 phase do not enter FP64 extrapolation
+    } else {
+      st->nhist = st->nhist < S.ndiis ? st->nhist + 1 : S.ndiis;
+      for (int k = 0; k < S.ndiis; ++k) {
+        st->bmat[slot * kMaxDiis + k] = vals[4 + k];
+        st->bmat[k * kMaxDiis + slot] = vals[4 + k];
+      }
+      want = !stop && st->niter >= S.diis_start && st->nhist >= 2 && !st->nonfinite;
+    }
+  }
+  if (st->nonfinite) {
+    st->phase = PH_DONE;
+  } else {
+    if (conv) st->converged = 1;
+    if (stop) st->phase = PH_FINAL;
+  }
+  return want;
+}
+
+// Reductions of a pass and the state that follows, one block per solve:
+// FIN_ITER after a residual pass (bookkeeping and DIIS solve), FIN_INIT after
+// the first pass (E and dT of the starting amplitudes, source/CCD.f90:62-64),
+// FIN_SUM after a plain energy evaluation.  The iteration state is worked on
+// in LDS: one parallel copy in, one out, instead of a chain of dependent
+// global accesses by a single thread.
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
+  __shared__ double vals[kPartialStride];
+  __shared__ double red[(kFinThreads / 64) * kPartialStride];
+  __shared__ State sst;
+  constexpr int kStateWords = sizeof(State) / sizeof(double);
+  static_assert(kStateWords <= 2 * kFinThreads, "two state words per thread");
+  const SolveDev& S = solves[blockIdx.x];
+  // One round of loads: each thread's state words go out with its partials
+  // (the DIIS dots are read whatever the phase; a finishing pass ignores them).
+  const double* src = reinterpret_cast<const double*>(S.st);
+  const int w0 = threadIdx.x, w1 = threadIdx.x + kFinThreads;
+  const double sw0 = w0 < kStateWords ? src[w0] : 0.0;
+  const double sw1 = w1 < kStateWords ? src[w1] : 0.0;
+  const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS) ? S.ndiis : 0;
+  reduce_partials(S, nd, vals, red);
+  double* dst = reinterpret_cast<double*>(&sst);
+  if (w0 < kStateWords) dst[w0] = sw0;
+  if (w1 < kStateWords) dst[w1] = sw1;
+  __syncthreads();
+  const int phase0 = sst.phase;
+  if (kind == FIN_ITER && phase0 == PH_DONE) return;  // the whole block
+  if (threadIdx.x == 0) {
+    State* st = &sst;
+    const double e = vals[0], dt = vals[1];
+    if (kind == FIN_ITER) {
+      st->use_coef = advance_state(S, st, vals, phase0) ? 1 : 0;
+    } else {
+      if (vals[3] > 0.0) st->nonfinite = 1;
+      st->ecorr = e;
+      if (kind == FIN_INIT) {
+        st->niter = 1;
+        st->ecorr_prev = e;
+        st->e_guess = e;
+        st->dt = dt;
+        st->slot = 0;
+        st->slot_cur = 0;
+        st->nhist = 0;
+        st->use_coef = 0;
+        st->converged = 0;
+        if (S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
+        const bool go = S.mode == UEGCCD_MODE_REFERENCE ? (dt >= S.tol_dt) : true;
+        st->phase = (go && st->niter < S.maxiter && !st->nonfinite) ? PH_ITER : PH_FINAL;
+        if (!go) st->converged = 1;
+      }
+    }
+  }
+  __syncthreads();
+  double* out = reinterpret_cast<double*>(S.st);
+  if (w0 < kStateWords) out[w0] = dst[w0];
+  if (w1 < kStateWords) out[w1] = dst[w1];
+}
+
+// Amplitudes for the next pass: DIIS extrapolation sum_k c_k histT[k] or the
+// plain update histT[slot].  When the state asks for the extrapolation, every
+// block solves the small DIIS system itself (the same inputs and instructions
+// in every block, so the same coefficients) while its elements' history
+// entries, requested first, are on their way.
 __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __restrict__ solves,
                                                            const int* __restrict__ prefix, int n) {
+  __shared__ double coef[kMaxDiis];
+  __shared__ double bm[kMaxDiis * kMaxDiis];
+  __shared__ int use;
+  static_assert(kMaxDiis * kMaxDiis == kElemPerBlock, "one B-matrix entry per thread");
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
-  const State* st = S.st;
-  if (st->phase == PH_DONE) return;
   const int lb = gb - prefix[s];
   const int x = lb * blockDim.x + threadIdx.x;
-  if (x >= S.n3 || S.pidx[x] < 0) return;
-  const double2* HT = S.histT + (size_t)x * S.ndiis;
-  if (st->use_coef) {
-    double ab = 0.0, ba = 0.0;
-    for (int k = 0; k < S.ndiis; ++k) {
-      const double c = st->coef[k];
-      if (c == 0.0) continue;
-      const double2 h = HT[k];
-      ab += c * h.x;
-      ba += c * h.y;
+  const bool mine = x < S.n3 && S.pidx[x] >= 0;
+  const size_t n3 = S.n3;
+  const double2* HT = S.histT + (mine ? x : 0);
+  constexpr int kPre = 8;  // history entries loaded ahead
+  double2 h[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) h[k] = (mine && k < S.ndiis) ? HT[k * n3] : make_double2(0.0, 0.0);
+  // the B matrix goes to LDS in the same round of loads as the history
+  const State& st = *S.st;
+  const double bw = st.bmat[threadIdx.x];
+  const int phase = st.phase, want = st.use_coef, nh = st.nhist, slot_cur = st.slot_cur;
+  bm[threadIdx.x] = bw;
+  __syncthreads();
+  if (phase == PH_DONE) return;  // the whole block
+  if (want && threadIdx.x < 64) {
+    bool ok;
+    if (S.ndiis <= 8) {
+      ok = threadIdx.x == 0 && diis_cholesky8(bm, nh, slot_cur, S.ndiis, coef);
+    } else {
+      ok = diis_coefficients<kMaxDiis + 1>(bm, nh, slot_cur, S.ndiis, coef);
     }
-    S.tcur[x] = make_double2(ab, ba);
-  } else {
-    S.tcur[x] = HT[st->slot_cur];
+    if (threadIdx.x == 0) use = ok ? 1 : 0;
   }
+  __syncthreads();
+  if (!mine) return;
+  double2 t;
+  if (want && use) {
+    double ab = 0.0, ba = 0.0;
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      const double c = k < S.ndiis ? coef[k] : 0.0;
+      if (c == 0.0) continue;
+      ab += c * h[k].x;
+      ba += c * h[k].y;
+    }
+    for (int k = kPre; k < S.ndiis; ++k) {
+      const double c = coef[k];
+      if (c == 0.0) continue;
+      const double2 hk = HT[k * n3];
+      ab += c * hk.x;
+      ba += c * hk.y;
+    }
+    t = make_double2(ab, ba);
+  } else {
+    const int sc = slot_cur;
+    t = h[0];
+#pragma unroll
+    for (int k = 1; k < kPre; ++k)
+      if (k == sc) t = h[k];
+    if (sc >= kPre) t = HT[sc * n3];
+  }
+  S.tcur[x] = t;
 }
 
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {
@@ -1413,7 +1494,8 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
   launch_k(k_main, 3, timer, dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
            (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
   if (main_flags & MF_ITERATE) {
-    hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, (int)FIN_ITER);
+    launch_k(k_finalize, 5, timer, dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,
+             (int)FIN_ITER);
     launch_k(k_extrap, 4, timer, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s,
              (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n);
   }

@@ -34,14 +34,13 @@ struct State {
   int slot;         // history slot the next residual pass writes
   int slot_cur;     // slot written by the last completed pass
   int nhist;        // filled DIIS history entries
-  int use_coef;     // 1: next amplitudes = sum_k coef[k] histT[k]
+  int use_coef;     // 1: next amplitudes extrapolated (k_extrap solves for the coefficients)
   int prec64;       // 1: FP64 ladders, 0: FP32 ladders
   int iters_fp32;
   int nonfinite;
   int converged;
   double ecorr, ecorr_prev, dt, maxres, maxres_in;
   double e_guess;   // energy of the starting amplitudes (E(2) for the MP2 guess)
-  double coef[kMaxDiis];
   double bmat[kMaxDiis * kMaxDiis];
 };
 static_assert(sizeof(State) % sizeof(double) == 0, "State is copied as doubles");
@@ -81,8 +80,8 @@ struct SolveDev {
   const int* qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
   // mutable buffers
   double2* tcur;        // [n3] (abab, abba) amplitudes entering the next pass
-  double2* histT;       // [n3][ndiis] updated amplitudes, element-major
-  double2* histE;       // [n3][ndiis] DIIS error vectors, element-major
+  double2* histT;       // [ndiis][n3] updated amplitudes, slot-major (coalesced per slot)
+  double2* histE;       // [ndiis][n3] DIIS error vectors, slot-major
   double* gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
   double* rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
   double2* lad;         // [n3] particle-particle ladder, (S, P) parts
@@ -135,7 +134,8 @@ constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 // Kernel launchers (kernels.hip).
 void launch_init(const BatchDev& b, hipStream_t s);
 void launch_mp2(const BatchDev& b, hipStream_t s);
-// Events bracketing one kernel of a pass (which: as launch_one), or none.
+// Events bracketing one kernel of a pass (which: as launch_one, or 5 for the
+// reductions and DIIS solve), or none.
 struct KernelTimer {
   int which;
   hipEvent_t before, after;
