@@ -90,33 +90,50 @@ __device__ double block_max(double x, double* red) {
   return r;
 }
 
-// Reduces NQ values per thread over the block in one pass: q < NMAX are
-// maxima, the rest sums.  Wave shuffles, then one LDS exchange; the result is
-// valid in thread 0 only.  red must hold (blockDim/64)*NQ doubles.
+// Reduces NQ values per thread (NQ a power of two, 2..64) over the block by
+// a reduce-scatter butterfly inside each wave: at the step with partner
+// distance m a lane keeps half of the values it holds and hands the other
+// half to its partner, so a wave exchanges NQ - 1 + log2(64/NQ) values instead
+// of 6 NQ; then one LDS exchange combines the waves in wave order (a fixed
+// order).  Quantities q < NMAX are maxima, the rest sums.  Threads q < NQ
+// return quantity q of the block; the others return 0.  red must hold
+// (blockDim/64)*NQ doubles.
 template <int NQ, int NMAX>
-__device__ __forceinline__ void block_reduce_multi(double (&v)[NQ], double* red) {
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+__device__ __forceinline__ double block_reduce_scatter(double (&v)[NQ], double* red) {
+  static_assert(NQ >= 2 && NQ <= 64 && (NQ & (NQ - 1)) == 0, "NQ a power of two");
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int base = 0;  // first quantity of the block of values this lane holds
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
+  for (int W = NQ, m = 32; W > 1; W >>= 1, m >>= 1) {
+    const bool upper = (lane & m) != 0;
+    const int h = W / 2;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const double o = __shfl_xor(v[q], m, 64);
-      v[q] = (q < NMAX) ? fmax(v[q], o) : v[q] + o;
+    for (int i = 0; i < h; ++i) {
+      const double send = upper ? v[i] : v[i + h];
+      const double keep = upper ? v[i + h] : v[i];
+      const double recv = __shfl_xor(send, m, 64);
+      const int q = base + (upper ? h : 0) + i;
+      v[i] = q < NMAX ? fmax(keep, recv) : keep + recv;
     }
+    if (upper) base += h;
+  }
+  // lanes that share the quantity differ in their low bits: finish over them
+  double x = v[0];
+#pragma unroll
+  for (int m = 32 / NQ; m >= 1; m >>= 1) {
+    const double o = __shfl_xor(x, m, 64);
+    x = base < NMAX ? fmax(x, o) : x + o;
   }
   __syncthreads();
-  if (l == 0) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) red[w * NQ + q] = v[q];
-  }
+  if ((lane & (64 / NQ - 1)) == 0) red[w * NQ + base] = x;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < nw; ++k) {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        v[q] = (q < NMAX) ? fmax(v[q], red[k * NQ + q]) : v[q] + red[k * NQ + q];
-    }
+  double r = 0.0;
+  if (threadIdx.x < NQ) {
+    const int q = threadIdx.x;
+    r = red[q];
+    for (int k = 1; k < nw; ++k) r = q < NMAX ? fmax(r, red[k * NQ + q]) : r + red[k * NQ + q];
   }
+  return r;
 }
 
 // Coulomb table and n-vectors staged in LDS.
@@ -666,6 +683,37 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
     joo_wave(S);
 }
 
+// Block partials of the residual pass: max dT, max |R|, sum E, the non-finite
+// count and the nd DIIS dots e_new . e_k (the new error against every history
+// slot), padded with zeros to NQ, written slot-major for k_finalize.
+template <int NQ>
+__device__ __forceinline__ void main_partials(const SolveDev& S, int lb, int nd, int slot, bool lead,
+                                              double dt_part, double r_part, double e_part, double bad,
+                                              double eab, double eba, const double2* HE, int n3,
+                                              double* red) {
+  double v[NQ];
+  v[0] = dt_part;
+  v[1] = r_part;
+  v[2] = e_part;
+  v[3] = bad;
+#pragma unroll
+  for (int k = 0; k < NQ - 4; ++k) {
+    double dk = 0.0;
+    if (k < nd && lead) {
+      if (k == slot) {
+        dk = eab * eab + eba * eba;
+      } else {
+        const double2 h = HE[(size_t)k * n3];
+        dk = eab * h.x + eba * h.y;
+      }
+    }
+    v[4 + k] = dk;
+  }
+  const double r = block_reduce_scatter<NQ, 2>(v, red);
+  const int q = threadIdx.x;
+  if (q < 4 + nd) *partial(S, q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q, lb) = r;
+}
+
 // --------------------------------------------------------------------------
 // Residual assembly + update, one thread per dense (i,j,a): G from drivers,
 // mosaics, ladders (pp + hh from the ladder and contraction passes), rings and
@@ -675,7 +723,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
                                                         const int* __restrict__ prefix, int n,
                                                         int flags) {
   __shared__ Lds L;
-  __shared__ double red[(kElemThreads / 64) * kPartialStride];
+  __shared__ double red[(kElemThreads / 64) * 32];
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveDev& S = solves[s];
@@ -765,35 +813,11 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   }
   // block partials: max dT, max |R|, sum E, non-finite count, DIIS dots
   const int nd = (S.mode == UEGCCD_MODE_DIIS) ? nh : 0;
-  double v[kPartialStride];
-  v[0] = dt_part;
-  v[1] = r_part;
-  v[2] = e_part;
-  v[3] = bad;
   const double2* HE = S.histE + (lead ? x : 0);
-#pragma unroll
-  for (int k = 0; k < kMaxDiis; ++k) {
-    double dk = 0.0;
-    if (k < nd && lead) {
-      if (k == slot) {
-        dk = eab * eab + eba * eba;
-      } else {
-        const double2 h = HE[(size_t)k * n3];
-        dk = eab * h.x + eba * h.y;
-      }
-    }
-    v[4 + k] = dk;
-  }
-  block_reduce_multi<kPartialStride, 2>(v, red);
-  if (threadIdx.x == 0) {
-    *partial(S, 0, lb) = v[2];
-    *partial(S, 1, lb) = v[0];
-    *partial(S, 2, lb) = v[1];
-    *partial(S, 3, lb) = v[3];
-#pragma unroll
-    for (int k = 0; k < kMaxDiis; ++k)
-      if (k < nd) *partial(S, 4 + k, lb) = v[4 + k];
-  }
+  if (nd <= 12)
+    main_partials<16>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, n3, red);
+  else
+    main_partials<32>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, n3, red);
 }
 
 // --------------------------------------------------------------------------
@@ -964,32 +988,34 @@ __device__ bool diis_cholesky8(const double* bmat, int nh, int slot_cur, int ndi
 // non-finite count (3) and the nd DIIS dots (4..) sums.  Each thread takes
 // blocks t, t + blockDim, ... and loads every quantity of a block at once, so
 // the loads of a round are in flight together (the partials were written on
-// other XCDs and miss this L2).  Thread 0 receives vals[0..3+nd] (E, max dT,
-// max |R|, non-finite count, dots).  red holds (blockDim/64)*kPartialStride.
-__device__ void reduce_partials(const SolveDev& S, int nd, double* vals, double* red) {
+// other XCDs and miss this L2).  vals[0..3+nd] (E, max dT, max |R|,
+// non-finite count, dots) are written by threads 0..3+nd: valid after a
+// barrier.  red holds (blockDim/64)*32 doubles.
+template <int NQ>
+__device__ void reduce_partials_n(const SolveDev& S, int nd, double* vals, double* red) {
   const int nq = 4 + nd, nb = S.nblk_elem;
-  double acc[kPartialStride];  // dT, max|R|, E, non-finite, dots
+  double acc[NQ];  // dT, max|R|, E, non-finite, dots
 #pragma unroll
-  for (int q = 0; q < kPartialStride; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
   for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-    double p[kPartialStride];
+    double p[NQ];
 #pragma unroll
-    for (int q = 0; q < kPartialStride; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int src = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q;
       p[q] = src < nq ? *partial(S, src, k) : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < kPartialStride; ++q) acc[q] = q < 2 ? fmax(acc[q], p[q]) : acc[q] + p[q];
+    for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], p[q]) : acc[q] + p[q];
   }
-  block_reduce_multi<kPartialStride, 2>(acc, red);
-  if (threadIdx.x == 0) {
-    vals[0] = acc[2];
-    vals[1] = acc[0];
-    vals[2] = acc[1];
-    vals[3] = acc[3];
-#pragma unroll
-    for (int q = 4; q < kPartialStride; ++q) vals[q] = acc[q];
-  }
+  const double r = block_reduce_scatter<NQ, 2>(acc, red);
+  const int q = threadIdx.x;
+  if (q < nq) vals[q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q] = r;
+}
+__device__ void reduce_partials(const SolveDev& S, int nd, double* vals, double* red) {
+  if (nd <= 12)
+    reduce_partials_n<16>(S, nd, vals, red);
+  else
+    reduce_partials_n<32>(S, nd, vals, red);
 }
 
 // The iteration bookkeeping of source/CCD.f90:80-99 after a residual pass, on
@@ -1058,7 +1084,7 @@ __device__ bool advance_state(const SolveDev& S, State* st, const double* vals, 
 // global accesses by a single thread.
 __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
   __shared__ double vals[kPartialStride];
-  __shared__ double red[(kFinThreads / 64) * kPartialStride];
+  __shared__ double red[(kFinThreads / 64) * 32];
   __shared__ State sst;
   constexpr int kStateWords = sizeof(State) / sizeof(double);
   static_assert(kStateWords <= 2 * kFinThreads, "two state words per thread");
