@@ -29,6 +29,22 @@
 
 namespace uegccd {
 
+// The descriptor as the kernels read it: its pointers are global memory, so
+// loads and stores through them compile to global (not flat) instructions.
+// (The host pass only checks device bodies; there the pointers stay plain.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using GlobalPtr = T __attribute__((address_space(1)))*;
+#else
+template <typename T>
+using GlobalPtr = T*;
+#endif
+using SolveG = SolveDevT<GlobalPtr>;
+static_assert(sizeof(SolveG) == sizeof(SolveDev), "one layout on host and device");
+__device__ __forceinline__ const SolveG* as_global(const SolveDev* p) {
+  return reinterpret_cast<const SolveG*>(p);
+}
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -98,15 +114,13 @@ __device__ double block_max(double x, double* red) {
 // order).  Quantities q < NMAX are maxima, the rest sums.  Threads q < NQ
 // return quantity q of the block; the others return 0.  red must hold
 // (blockDim/64)*NQ doubles.
-template <int NQ, int NMAX>
-__device__ __forceinline__ double block_reduce_scatter(double (&v)[NQ], double* red) {
-  static_assert(NQ >= 2 && NQ <= 64 && (NQ & (NQ - 1)) == 0, "NQ a power of two");
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  int base = 0;  // first quantity of the block of values this lane holds
-#pragma unroll
-  for (int W = NQ, m = 32; W > 1; W >>= 1, m >>= 1) {
+// One halving step of the butterfly (W values held, partner distance
+// 32 W / NQ), then the next; unrolled at compile time so v stays in registers.
+template <int NQ, int NMAX, int W>
+__device__ __forceinline__ void reduce_scatter_step(double (&v)[NQ], int lane, int& base) {
+  if constexpr (W > 1) {
+    constexpr int h = W / 2, m = 32 * W / NQ;
     const bool upper = (lane & m) != 0;
-    const int h = W / 2;
 #pragma unroll
     for (int i = 0; i < h; ++i) {
       const double send = upper ? v[i] : v[i + h];
@@ -116,7 +130,16 @@ __device__ __forceinline__ double block_reduce_scatter(double (&v)[NQ], double* 
       v[i] = q < NMAX ? fmax(keep, recv) : keep + recv;
     }
     if (upper) base += h;
+    reduce_scatter_step<NQ, NMAX, W / 2>(v, lane, base);
   }
+}
+
+template <int NQ, int NMAX>
+__device__ __forceinline__ double block_reduce_scatter(double (&v)[NQ], double* red) {
+  static_assert(NQ >= 2 && NQ <= 64 && (NQ & (NQ - 1)) == 0, "NQ a power of two");
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int base = 0;  // first quantity of the block of values this lane holds
+  reduce_scatter_step<NQ, NMAX, NQ>(v, lane, base);
   // lanes that share the quantity differ in their low bits: finish over them
   double x = v[0];
 #pragma unroll
@@ -143,7 +166,7 @@ struct Lds {
   double red[16];
 };
 
-__device__ __forceinline__ void stage(const SolveDev& S, Lds& L) {
+__device__ __forceinline__ void stage(const SolveG& S, Lds& L) {
   for (int k = threadIdx.x; k < S.nao; k += blockDim.x) L.nv[k] = S.nv4[k];
   for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) L.vt[k] = S.vtab[k];
   __syncthreads();
@@ -152,7 +175,7 @@ __device__ __forceinline__ void stage(const SolveDev& S, Lds& L) {
 #define VPQ(p, q) (L.vt[d2(L.nv[p], L.nv[q])])
 
 // Slot-major block partials: quantity q of element block lb.
-__device__ __forceinline__ double* partial(const SolveDev& S, int q, int lb) {
+__device__ __forceinline__ double* partial(const SolveG& S, int q, int lb) {
   return S.partials + (size_t)q * S.nblk_elem + lb;
 }
 
@@ -163,7 +186,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
   __shared__ Lds L;
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
-  const SolveDev& S = solves[s];
+  const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
   stage(S, L);
   const int o = S.o, o2 = o * o;
@@ -197,7 +220,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
   __shared__ Lds L;
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
-  const SolveDev& S = solves[s];
+  const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
   stage(S, L);
   const int o = S.o, o2 = o * o;
@@ -250,7 +273,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   // No test of the solve's phase here: a finished solve's slice products are
   // never read again, and the test would put a dependent load before any work.
   const int s = find_solve(prefix, n, bid);
-  const SolveDev& S = solves[s];
+  const SolveG& S = as_global(solves)[s];
   // slice_split blocks per slice, each computing a share of the o*o outputs
   const int split = slice_split(S.o);
   const int lb = bid - prefix[s], sv = lb / split, part = lb - sv * split;
@@ -380,7 +403,7 @@ struct MfmaTraits<float> {
 // the four waves of the block (wave w takes steps w, w+4, ...); the waves'
 // partial sums are combined in a fixed order.  vt is the Coulomb table in LDS.
 template <typename Real>
-__device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0, const double* vt,
+__device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
                             double* red) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
@@ -472,7 +495,7 @@ __device__ void ladder_tile(const SolveDev& S, int K, int row0, int col0, const 
 __device__ void ladder_block(const SolveDev* __restrict__ solves, const int4* __restrict__ items,
                              int item, double* vt, double* red) {
   const int4 it = items[item];
-  const SolveDev& S = solves[it.x];
+  const SolveG& S = as_global(solves)[it.x];
   if (!S.do_ladder) return;
   const int phase = S.st->phase;  // requested with the table, tested after it
   for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
@@ -511,7 +534,7 @@ __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ sol
 // are, per q, one product [rows x o] . [o x columns] each: rows and columns
 // are staged in LDS, each output written once (ring to quad.xy, crossed ring
 // to quad.zw of its element).
-__device__ void quad_tile(const SolveDev& S, int qb, int tile) {
+__device__ void quad_tile(const SolveG& S, int qb, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int4 info = S.qb_info[qb];  // rows of q: [r0, r0+n); columns (rows of -q): [c0, c0+m)
   const int r0 = info.x, n = info.y, c0 = info.z, m = info.w;
@@ -587,7 +610,7 @@ __device__ void quad_tile(const SolveDev& S, int qb, int tile) {
 //   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a]
 // one wave64 per 16x16 (pair, a) tile on v_mfma_f64_16x16x4f64, added into
 // the particle-particle ladder of the same elements.
-__device__ void hh_tile(const SolveDev& S, int K, int tile) {
+__device__ void hh_tile(const SolveG& S, int K, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
@@ -648,7 +671,7 @@ __device__ void hh_tile(const SolveDev& S, int K, int tile) {
 // Mosaic Joo(k) = sum over slices of the slice blocks' partials
 // (source/CCD.f90:563-567), by one wave: lane k sums column k over the v
 // slices with four independent partial sums, combined in a fixed order.
-__device__ void joo_wave(const SolveDev& S) {
+__device__ void joo_wave(const SolveG& S) {
   if (!S.do_mosaic) return;
   const int o = S.o, lane = threadIdx.x & 63;
   for (int k = lane; k < o; k += 64) {
@@ -673,7 +696,7 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
   const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + (threadIdx.x >> 6);
   if (item >= nitems) return;
   const int4 it = items[item];
-  const SolveDev& S = solves[it.x];
+  const SolveG& S = as_global(solves)[it.x];
   // (no phase test: the contraction outputs of a finished solve are never read)
   if (it.y == CK_QUAD)
     quad_tile(S, it.z, it.w);
@@ -687,7 +710,7 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
 // count and the nd DIIS dots e_new . e_k (the new error against every history
 // slot), padded with zeros to NQ, written slot-major for k_finalize.
 template <int NQ>
-__device__ __forceinline__ void main_partials(const SolveDev& S, int lb, int nd, int slot, bool lead,
+__device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, int slot, bool lead,
                                               double dt_part, double r_part, double e_part, double bad,
                                               double eab, double eba, const double2* HE, int n3,
                                               double* red) {
@@ -726,7 +749,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   __shared__ double red[(kElemThreads / 64) * 32];
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
-  const SolveDev& S = solves[s];
+  const SolveG& S = as_global(solves)[s];
   // the state, the element's partner and the LDS tables are requested
   // together; a finished solve leaves after they arrive
   const int phase = S.st->phase, slot = S.st->slot;
@@ -992,7 +1015,7 @@ __device__ bool diis_cholesky8(const double* bmat, int nh, int slot_cur, int ndi
 // non-finite count, dots) are written by threads 0..3+nd: valid after a
 // barrier.  red holds (blockDim/64)*32 doubles.
 template <int NQ>
-__device__ void reduce_partials_n(const SolveDev& S, int nd, double* vals, double* red) {
+__device__ void reduce_partials_n(const SolveG& S, int nd, double* vals, double* red) {
   const int nq = 4 + nd, nb = S.nblk_elem;
   double acc[NQ];  // dT, max|R|, E, non-finite, dots
 #pragma unroll
@@ -1011,7 +1034,7 @@ __device__ void reduce_partials_n(const SolveDev& S, int nd, double* vals, doubl
   const int q = threadIdx.x;
   if (q < nq) vals[q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q] = r;
 }
-__device__ void reduce_partials(const SolveDev& S, int nd, double* vals, double* red) {
+__device__ void reduce_partials(const SolveG& S, int nd, double* vals, double* red) {
   if (nd <= 12)
     reduce_partials_n<16>(S, nd, vals, red);
   else
@@ -1022,7 +1045,7 @@ __device__ void reduce_partials(const SolveDev& S, int nd, double* vals, double*
 // the state copy in LDS, by one thread: NIter, E, dT, convergence, the mixed
 // precision switch, the DIIS history and its B matrix row, the trace line.
 // Returns whether the next amplitudes are to be extrapolated.
-__device__ bool advance_state(const SolveDev& S, State* st, const double* vals, int phase0) {
+__device__ bool advance_state(const SolveG& S, State* st, const double* vals, int phase0) {
   const double e = vals[0], dt = vals[1], rmax = vals[2];
   if (vals[3] > 0.0) st->nonfinite = 1;
   if (phase0 == PH_FINAL) {
@@ -1088,7 +1111,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
   __shared__ State sst;
   constexpr int kStateWords = sizeof(State) / sizeof(double);
   static_assert(kStateWords <= 2 * kFinThreads, "two state words per thread");
-  const SolveDev& S = solves[blockIdx.x];
+  const SolveG& S = as_global(solves)[blockIdx.x];
   // One round of loads: each thread's state words go out with its partials
   // (the DIIS dots are read whatever the phase; a finishing pass ignores them).
   const double* src = reinterpret_cast<const double*>(S.st);
@@ -1147,7 +1170,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __rest
   static_assert(kMaxDiis * kMaxDiis == kElemPerBlock, "one B-matrix entry per thread");
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
-  const SolveDev& S = solves[s];
+  const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
   const int x = lb * blockDim.x + threadIdx.x;
   const bool mine = x < S.n3 && S.pidx[x] >= 0;
@@ -1205,7 +1228,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __rest
 }
 
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {
-  const SolveDev& S = solves[blockIdx.x];
+  const SolveG& S = as_global(solves)[blockIdx.x];
   if (threadIdx.x != 0) return;
   State* st = S.st;
   st->niter = 0;
@@ -1238,7 +1261,7 @@ __global__ __launch_bounds__(256) void k_connectivity(const SolveDev* __restrict
                                                       int nbins, int* __restrict__ hist,
                                                       int* __restrict__ overflow) {
   __shared__ int h[kMaxConnBins];
-  const SolveDev& S = solves[0];
+  const SolveG& S = as_global(solves)[0];
   for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
   __syncthreads();
   const int x = blockIdx.x * blockDim.x + threadIdx.x;

@@ -45,7 +45,11 @@ struct State {
 };
 static_assert(sizeof(State) % sizeof(double) == 0, "State is copied as doubles");
 
-struct SolveDev {
+// Per-solve descriptor.  Ptr chooses how its pointers are typed: plain on
+// the host (SolveDev), global-address-space in the kernels (SolveG), so that
+// loads through them compile to global rather than flat memory instructions.
+template <template <typename> class Ptr>
+struct SolveDevT {
   int o, v, nao, nK;
   int n3;           // o*o*v
   int n2max;        // largest |n_p - n_q|^2 in the basis
@@ -56,52 +60,54 @@ struct SolveDev {
   double df;        // DenomFactor
   double tol_dt, tol_e, tol_r, mixed_switch;
   // constant per basis
-  const int4* nv4;      // [nao] (n1,n2,n3,0)
-  const double* eig;    // [nao]
-  const double* vtab;   // [n2max+1] Coulomb table v(|dn|^2)
-  const float* vtab32;  // same, FP32
-  const int* pidx;      // [n3]
-  const int* kb_poff;   // [nK+1]
-  const int* kb_pairs;  // [o*o] pair ids x = i + o*j grouped by K
-  const int* kb_voff;   // [nK+1]
-  const int* kb_va;     // virtual columns a (orbital index) grouped by K
-  const int* kb_vb;     // their partners b
-  const int4* kb_nva;   // (n_a, a) of the same list
-  const int* kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
-  const int* pair_kb;   // [o*o] block of pair x
-  const int* pair_pos;  // [o*o] position of pair x inside its block
+  Ptr<const int4> nv4;      // [nao] (n1,n2,n3,0)
+  Ptr<const double> eig;    // [nao]
+  Ptr<const double> vtab;   // [n2max+1] Coulomb table v(|dn|^2)
+  Ptr<const float> vtab32;  // same, FP32
+  Ptr<const int> pidx;      // [n3]
+  Ptr<const int> kb_poff;   // [nK+1]
+  Ptr<const int> kb_pairs;  // [o*o] pair ids x = i + o*j grouped by K
+  Ptr<const int> kb_voff;   // [nK+1]
+  Ptr<const int> kb_va;     // virtual columns a (orbital index) grouped by K
+  Ptr<const int> kb_vb;     // their partners b
+  Ptr<const int4> kb_nva;   // (n_a, a) of the same list
+  Ptr<const int> kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
+  Ptr<const int> pair_kb;   // [o*o] block of pair x
+  Ptr<const int> pair_pos;  // [o*o] position of pair x inside its block
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, one per transfer q
   int nqb;
-  const int* qb_roff;   // [nqb+1] row offsets
-  const int* qb_rows;   // [o*v] pairs i + o*(p - o), grouped by q
-  const int* qb_neg;    // [nqb] block of -q, or -1 when P(-q) is empty
-  const int* qb_d;      // [nqb*o] virtual index of n_l + q for each l, or -1
-  const int4* qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
-  const int* qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
+  Ptr<const int> qb_roff;   // [nqb+1] row offsets
+  Ptr<const int> qb_rows;   // [o*v] pairs i + o*(p - o), grouped by q
+  Ptr<const int> qb_neg;    // [nqb] block of -q, or -1 when P(-q) is empty
+  Ptr<const int> qb_d;      // [nqb*o] virtual index of n_l + q for each l, or -1
+  Ptr<const int4> qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
+  Ptr<const int> qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
   // mutable buffers
-  double2* tcur;        // [n3] (abab, abba) amplitudes entering the next pass
-  double2* histT;       // [ndiis][n3] updated amplitudes, slot-major (coalesced per slot)
-  double2* histE;       // [ndiis][n3] DIIS error vectors, slot-major
-  double* gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
-  double* rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
-  double2* lad;         // [n3] particle-particle ladder, (S, P) parts
-  double2* w;           // [wtot] hole-hole W intermediates, (S, P) parts
+  Ptr<double2> tcur;        // [n3] (abab, abba) amplitudes entering the next pass
+  Ptr<double2> histT;       // [ndiis][n3] updated amplitudes, slot-major (coalesced per slot)
+  Ptr<double2> histE;       // [ndiis][n3] DIIS error vectors, slot-major
+  Ptr<double> gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
+  Ptr<double> rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
+  Ptr<double2> lad;         // [n3] particle-particle ladder, (S, P) parts
+  Ptr<double2> w;           // [wtot] hole-hole W intermediates, (S, P) parts
   int wtot;
-  double4* jring;       // [o*v rows][o] quadratic ring intermediates (J1, J2, J3, 0),
+  Ptr<double4> jring;       // [o*v rows][o] quadratic ring intermediates (J1, J2, J3, 0),
                         // rows in the grouped P(q) order (qrow)
-  double4* jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
-  double4* tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
-  double4* quad;        // [n3] quadratic ring (ab, ba) and crossed-ring (ab, ba) sums
-  double* rsum;         // [v*o] slice row sums
-  double* csum;         // [v*o] slice column sums
-  double* jvv;          // [v] mosaic Jvv
-  double* joo_part;     // [v*o] per-slice parts of the mosaic Joo
-  double* joo;          // [o] mosaic Joo
-  double* partials;     // [nblk_elem * kPartialStride]
+  Ptr<double4> jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
+  Ptr<double4> tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
+  Ptr<double4> quad;        // [n3] quadratic ring (ab, ba) and crossed-ring (ab, ba) sums
+  Ptr<double> rsum;         // [v*o] slice row sums
+  Ptr<double> csum;         // [v*o] slice column sums
+  Ptr<double> jvv;          // [v] mosaic Jvv
+  Ptr<double> joo_part;     // [v*o] per-slice parts of the mosaic Joo
+  Ptr<double> joo;          // [o] mosaic Joo
+  Ptr<double> partials;     // [nblk_elem * kPartialStride]
   int nblk_elem;
-  State* st;
-  double* trace;        // [2*maxiter] (E, dT) by NIter
+  Ptr<State> st;
+  Ptr<double> trace;        // [2*maxiter] (E, dT) by NIter
 };
+template <typename T> using HostPtr = T*;
+using SolveDev = SolveDevT<HostPtr>;
 
 // Launch geometry of a batch of solves iterated together.
 struct BatchDev {
