@@ -44,6 +44,9 @@ static_assert(sizeof(SolveG) == sizeof(SolveDev), "one layout on host and device
 __device__ __forceinline__ const SolveG* as_global(const SolveDev* p) {
   return reinterpret_cast<const SolveG*>(p);
 }
+__device__ __forceinline__ double2 ld(GlobalPtr<const double2> p) { return *p; }
+__device__ __forceinline__ double4 ld(GlobalPtr<const double4> p) { return *p; }
+__device__ __forceinline__ void st2(GlobalPtr<double2> p, double2 v) { *p = v; }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -534,6 +537,13 @@ __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ sol
 // are, per q, one product [rows x o] . [o x columns] each: rows and columns
 // are staged in LDS, each output written once (ring to quad.xy, crossed ring
 // to quad.zw of its element).
+//
+// The tile's loads go out in three rounds with no branch between them (masked
+// values, clamped in-range addresses): every index (the column's pair, the
+// four output rows' pairs, the partner slices d_q(l)), then the J rows and
+// the crossed-ring T columns, then the ring T columns that need d_q(l).
+constexpr int kQuadSteps = 8;  // k-steps of 4 l values per round of loads (l < 32)
+
 __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int4 info = S.qb_info[qb];  // rows of q: [r0, r0+n); columns (rows of -q): [c0, c0+m)
@@ -547,69 +557,83 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   // lane supplies A[row r16][l] (J rows, contiguous per q) and B[l][column r16]
   const int rowA = row0 + r16, colB = col0 + r16;
   const bool rv = rowA < n, cv = colB < m;
-  int j = 0, pcv = 0;
-  if (cv) {
-    const int pc = S.qb_rows[c0 + colB];
-    j = pc % o;
-    pcv = pc / o;
-  }
-  const double4* jr = S.jring + (size_t)(r0 + (rv ? rowA : 0)) * o;
-  const double4* jx = S.jxring + (size_t)(r0 + (rv ? rowA : 0)) * o;
-  const int* dq = S.qb_d + (size_t)qb * o;
-  const double2* T = S.tcur;
+  const int pc = S.qb_rows[c0 + (cv ? colB : 0)];
+  int prow[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) prow[r] = S.qb_rows[r0 + min(row0 + MfmaTraits<double>::row(kq, r), n - 1)];
+  const int j = pc % o, pcv = pc / o;
+  const auto jr = S.jring + (size_t)(r0 + (rv ? rowA : 0)) * o;
+  const auto jx = S.jxring + (size_t)(r0 + (rv ? rowA : 0)) * o;
+  const auto dq = S.qb_d + (size_t)qb * o;
+  const auto T = S.tcur;
+  const size_t tcol = (size_t)o * j;
   acc_t rab = {0, 0, 0, 0}, rba = {0, 0, 0, 0}, xab = {0, 0, 0, 0}, xba = {0, 0, 0, 0};
-  constexpr int G = 4;  // k-steps whose loads are issued together
-  for (int g0 = 0; g0 < o; g0 += 4 * G) {
-    double4 ar[G], ax[G];
-    double2 br[G], bx[G];
+  for (int g0 = 0; g0 < o; g0 += 4 * kQuadSteps) {
+    int dv[kQuadSteps];
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
+    for (int st = 0; st < kQuadSteps; ++st) dv[st] = dq[min(g0 + 4 * st + kq, o - 1)];
+    double a1[kQuadSteps], a2[kQuadSteps], a3[kQuadSteps];
+    double x1[kQuadSteps], x2[kQuadSteps], x3[kQuadSteps];
+    double2 bx[kQuadSteps];
+#pragma unroll
+    for (int st = 0; st < kQuadSteps; ++st) {
       const int l = g0 + 4 * st + kq;
-      const bool lv = l < o;
-      ar[st] = ax[st] = make_double4(0.0, 0.0, 0.0, 0.0);
-      br[st] = bx[st] = make_double2(0.0, 0.0);
-      if (lv && rv) {
-        if (ring) ar[st] = jr[l];
-        if (xring) ax[st] = jx[l];
-      }
-      if (lv && cv) {
-        const int dv = dq[l];
-        if (ring && dv >= 0) br[st] = T[l + o * j + (size_t)o2 * dv];
-        if (xring) bx[st] = T[l + o * j + (size_t)o2 * pcv];
-      }
+      const bool av = l < o && rv, bv = l < o && cv;
+      const int lc = min(l, o - 1);
+      const double4 ar = ld(jr + lc), ax = ld(jx + lc);
+      a1[st] = (av && ring) ? ar.x : 0.0;
+      a2[st] = (av && ring) ? ar.y : 0.0;
+      a3[st] = (av && ring) ? ar.z : 0.0;
+      x1[st] = (av && xring) ? ax.x : 0.0;
+      x2[st] = (av && xring) ? ax.y : 0.0;
+      x3[st] = (av && xring) ? ax.z : 0.0;
+      const double2 t = ld(T + lc + tcol + (size_t)o2 * pcv);
+      bx[st] = (bv && xring) ? t : make_double2(0.0, 0.0);
     }
+    double2 br[kQuadSteps];
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
-      if (g0 + 4 * st >= o) break;
+    for (int st = 0; st < kQuadSteps; ++st) {
+      const int l = g0 + 4 * st + kq;
+      const bool ok = l < o && cv && ring && dv[st] >= 0;
+      const double2 t = ld(T + min(l, o - 1) + tcol + (size_t)o2 * max(dv[st], 0));
+      br[st] = ok ? t : make_double2(0.0, 0.0);
+    }
+    // (all steps, the masked ones add zeros: a branch here would let the
+    // compiler sink each step's loads behind the previous step's MFMAs)
+#pragma unroll
+    for (int st = 0; st < kQuadSteps; ++st) {
       // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab,  ba += J3 T^ba
-      rab = MfmaTraits<double>::mma(ar[st].y, br[st].x + br[st].y, rab);
-      rab = MfmaTraits<double>::mma(ar[st].x, br[st].x, rab);
-      rba = MfmaTraits<double>::mma(ar[st].z, br[st].y, rba);
+      rab = MfmaTraits<double>::mma(a2[st], br[st].x + br[st].y, rab);
+      rab = MfmaTraits<double>::mma(a1[st], br[st].x, rab);
+      rba = MfmaTraits<double>::mma(a3[st], br[st].y, rba);
       // crossed ring:  ba -= J1 T^ba + J2 (T^ab + T^ba),  ab += J3 T^ab
-      xba = MfmaTraits<double>::mma(-ax[st].x, bx[st].y, xba);
-      xba = MfmaTraits<double>::mma(-ax[st].y, bx[st].x + bx[st].y, xba);
-      xab = MfmaTraits<double>::mma(ax[st].z, bx[st].x, xab);
+      xba = MfmaTraits<double>::mma(-x1[st], bx[st].y, xba);
+      xba = MfmaTraits<double>::mma(-x2[st], bx[st].x + bx[st].y, xba);
+      xab = MfmaTraits<double>::mma(x3[st], bx[st].x, xab);
     }
   }
   if (!cv) return;
+#pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = row0 + MfmaTraits<double>::row(kq, r);
     if (row >= n) continue;
-    const int pr = S.qb_rows[r0 + row];
-    const int i = pr % o;
-    if (ring)
-      reinterpret_cast<double2*>(S.quad + (size_t)i + o * j + (size_t)o2 * (pr / o))[0] =
-          make_double2(rab[r], rba[r]);
-    if (xring)
-      reinterpret_cast<double2*>(S.quad + (size_t)i + o * j + (size_t)o2 * pcv)[1] =
-          make_double2(xab[r], xba[r]);
+    // ring sums to element (i, j, a) with a the row's particle, crossed-ring
+    // sums to (i, j, a') with a' the column's particle
+    const int pr = prow[r];
+    const auto q = S.quad + (size_t)(pr % o) + tcol;
+    if (ring) st2(reinterpret_cast<GlobalPtr<double2>>(q + (size_t)o2 * (pr / o)), make_double2(rab[r], rba[r]));
+    if (xring) st2(reinterpret_cast<GlobalPtr<double2>>(q + (size_t)o2 * pcv) + 1, make_double2(xab[r], xba[r]));
   }
 }
 
 // Hole-hole ladder contraction of one K block (source/CCD.f90:651-656):
 //   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a]
 // one wave64 per 16x16 (pair, a) tile on v_mfma_f64_16x16x4f64, added into
-// the particle-particle ladder of the same elements.
+// the particle-particle ladder of the same elements.  Loads as in quad_tile:
+// the pair lists and W first, then the amplitudes the pairs select; the four
+// output elements are read together before they are written.
+constexpr int kHHSteps = 8;  // k-steps of 4 pairs per round of loads
+
 __device__ void hh_tile(const SolveG& S, int K, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
@@ -619,52 +643,54 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
   if (tile >= ((noo + 15) >> 4) * nct) return;
   const int row0 = (tile / nct) * 16, col0 = (tile % nct) * 16;
   const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const double2* W = S.w + S.kb_woff[K];
-  const double2* T = S.tcur;
+  const auto W = S.w + S.kb_woff[K];
+  const auto T = S.tcur;
   const int rowA = row0 + r16, colB = col0 + r16;
   const bool rv = rowA < noo, cv = colB < nvv;
-  const int av = cv ? S.kb_va[voff + colB] - o : 0;
+  const int av = S.kb_va[voff + (cv ? colB : 0)] - o;
+  int prow[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) prow[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<double>::row(kq, r), noo - 1)];
+  const size_t tslab = (size_t)o2 * av;
+  const auto Wrow = W + (size_t)(rv ? rowA : 0) * noo;
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-  // groups of 8 k-steps (32 pairs): all loads of a group are issued before
-  // its MFMAs, so the pair-index -> amplitude chains overlap
-  constexpr int G = 8;
-  for (int g0 = 0; g0 < noo; g0 += 4 * G) {
-    int xq[G];
+  for (int g0 = 0; g0 < noo; g0 += 4 * kHHSteps) {
+    int xq[kHHSteps];
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
+    for (int st = 0; st < kHHSteps; ++st) xq[st] = S.kb_pairs[poff + min(g0 + 4 * st + kq, noo - 1)];
+    double ws[kHHSteps], wp[kHHSteps];
+#pragma unroll
+    for (int st = 0; st < kHHSteps; ++st) {
       const int q = g0 + 4 * st + kq;
-      xq[st] = (q < noo && cv) ? S.kb_pairs[poff + q] : -1;
+      const double2 w = ld(Wrow + min(q, noo - 1));
+      const bool ok = q < noo && rv;
+      ws[st] = ok ? w.x : 0.0;
+      wp[st] = ok ? w.y : 0.0;
     }
-    double ws[G], wp[G], bs[G], bp[G];
+    double bs[kHHSteps], bp[kHHSteps];
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
+    for (int st = 0; st < kHHSteps; ++st) {
       const int q = g0 + 4 * st + kq;
-      ws[st] = wp[st] = bs[st] = bp[st] = 0.0;
-      if (q < noo && rv) {
-        const double2 w = W[rowA * noo + q];
-        ws[st] = w.x;
-        wp[st] = w.y;
-      }
-      if (xq[st] >= 0) {
-        const double2 t = T[xq[st] + (size_t)o2 * av];
-        bs[st] = t.x + t.y;
-        bp[st] = t.x - t.y;
-      }
+      const double2 t = ld(T + xq[st] + tslab);
+      const bool ok = q < noo && cv;
+      bs[st] = ok ? t.x + t.y : 0.0;
+      bp[st] = ok ? t.x - t.y : 0.0;
     }
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
-      if (g0 + 4 * st >= noo) break;
+    for (int st = 0; st < kHHSteps; ++st) {  // all steps, as in quad_tile
       accs = MfmaTraits<double>::mma(ws[st], bs[st], accs);
       accp = MfmaTraits<double>::mma(wp[st], bp[st], accp);
     }
   }
   if (!cv) return;
+  double2 cur[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) cur[r] = ld(S.lad + prow[r] + tslab);
+#pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int p = row0 + MfmaTraits<double>::row(kq, r);
     if (p >= noo) continue;
-    double2* l2 = S.lad + S.kb_pairs[poff + p] + (size_t)o2 * av;
-    const double2 cur = *l2;
-    *l2 = make_double2(cur.x + 0.5 * accs[r], cur.y + 0.5 * accp[r]);
+    st2(S.lad + prow[r] + tslab, make_double2(cur[r].x + 0.5 * accs[r], cur[r].y + 0.5 * accp[r]));
   }
 }
 
@@ -693,7 +719,10 @@ __device__ void joo_wave(const SolveG& S) {
 __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
                                                             const int4* __restrict__ items,
                                                             int nitems) {
-  const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + (threadIdx.x >> 6);
+  // the wave index is uniform over the wave: saying so lets the item, the
+  // descriptor and its pointers live in scalar registers (scalar loads)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + wave;
   if (item >= nitems) return;
   const int4 it = items[item];
   const SolveG& S = as_global(solves)[it.x];
