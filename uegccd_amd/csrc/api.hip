@@ -270,8 +270,11 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     }
     // work items of the contraction pass
     c->con_items.clear();
-    // one wave per item: 16x16 tiles of the per-transfer ring products, 16x16
-    // hole-hole ladder tiles, the Joo sums
+    // one wave per item: the Joo sums over slice ranges (first: they read the
+    // most data per wave), 16x16 tiles of the per-transfer ring products,
+    // 16x16 hole-hole ladder tiles
+    if (s.do_mosaic)
+      for (int p = 0; p < kJooParts; ++p) c->con_items.push_back(make_int4(0, CK_JOO, p, 0));
     if (s.do_ring || s.do_xring)
       for (int q = 0; q < nq; ++q) {
         const int4 in = c->qb_info[q];
@@ -285,7 +288,6 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
         const int nt = ((noo + 15) / 16) * ((nvv + 15) / 16);
         for (int t = 0; t < nt; ++t) c->con_items.push_back(make_int4(0, CK_HH, K, t));
       }
-    if (s.do_mosaic) c->con_items.push_back(make_int4(0, CK_JOO, 0, 0));
   }
   c->n2max = n2max;
   const EriKind kind = eri_kind(s);
@@ -320,7 +322,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double4) * n3);                              // tv
   bytes += 2 * aligned(sizeof(double) * c->o * c->v);                  // rsum, csum
   bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
-  bytes += aligned(sizeof(double) * c->o);
+  bytes += aligned(sizeof(double) * c->o * kJooParts);
   bytes += aligned(sizeof(double) * eb * kPartialStride);
   bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
@@ -560,7 +562,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.csum = c->arena.take<double>((size_t)c->o * c->v);
   S.jvv = c->arena.take<double>(c->v);
   S.joo_part = c->arena.take<double>((size_t)c->v * c->o);
-  S.joo = c->arena.take<double>(c->o);
+  S.joo = c->arena.take<double>((size_t)c->o * kJooParts);
   S.nblk_elem = eb;
   S.partials = c->arena.take<double>((size_t)eb * kPartialStride /* slot-major: [q][block] */);
   c->d_state = c->arena.take<State>(1);

@@ -46,6 +46,7 @@ __device__ __forceinline__ const SolveG* as_global(const SolveDev* p) {
 }
 __device__ __forceinline__ double2 ld(GlobalPtr<const double2> p) { return *p; }
 __device__ __forceinline__ double4 ld(GlobalPtr<const double4> p) { return *p; }
+__device__ __forceinline__ int4 ld(GlobalPtr<const int4> p) { return *p; }
 __device__ __forceinline__ void st2(GlobalPtr<double2> p, double2 v) { *p = v; }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -178,7 +179,7 @@ __device__ __forceinline__ void stage(const SolveG& S, Lds& L) {
 #define VPQ(p, q) (L.vt[d2(L.nv[p], L.nv[q])])
 
 // Slot-major block partials: quantity q of element block lb.
-__device__ __forceinline__ double* partial(const SolveG& S, int q, int lb) {
+__device__ __forceinline__ GlobalPtr<double> partial(const SolveG& S, int q, int lb) {
   return S.partials + (size_t)q * S.nblk_elem + lb;
 }
 
@@ -289,7 +290,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   double* rs = dyn + 3 * o2;
   double* cs = rs + o;
   double* jp = cs + o;
-  const double2* tslab = S.tcur + (size_t)o2 * sv;
+  const auto tslab = S.tcur + (size_t)o2 * sv;
   // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
   // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
   // so the table reads as zero there (the last entry is always past n2max).
@@ -414,8 +415,8 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int ncol = nvv + noo;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const double2* T = S.tcur;
-  const int4* cl = S.kb_nva + voff;  // (n_c, c) of the block's virtual list
+  const auto T = S.tcur;
+  const auto cl = S.kb_nva + voff;  // (n_c, c) of the block's virtual list
 
   const int rowA = row0 + r16;
   const int xpA = rowA < noo ? S.kb_pairs[poff + rowA] : -1;
@@ -442,7 +443,7 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
 #pragma unroll
     for (int st = 0; st < G; ++st) {
       const int ci = g0 + 16 * st + kq;
-      nc[st] = ci < nvv ? cl[ci] : make_int4(0, 0, 0, -1);
+      nc[st] = ci < nvv ? ld(cl + ci) : make_int4(0, 0, 0, -1);
     }
     double2 t[G];
 #pragma unroll
@@ -695,22 +696,17 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
 }
 
 // Mosaic Joo(k) = sum over slices of the slice blocks' partials
-// (source/CCD.f90:563-567), by one wave: lane k sums column k over the v
-// slices with four independent partial sums, combined in a fixed order.
-__device__ void joo_wave(const SolveG& S) {
+// (source/CCD.f90:563-567), split over kJooParts waves: wave `part` sums the
+// slices of its range in order (lane k, column k) into joo[part][k]; k_main
+// adds the parts in order.
+__device__ void joo_wave(const SolveG& S, int part) {
   if (!S.do_mosaic) return;
   const int o = S.o, lane = threadIdx.x & 63;
+  const int s0 = (S.v * part) / kJooParts, s1 = (S.v * (part + 1)) / kJooParts;
   for (int k = lane; k < o; k += 64) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int r = 0;
-    for (; r + 3 < S.v; r += 4) {
-      a0 += S.joo_part[r * o + k];
-      a1 += S.joo_part[(r + 1) * o + k];
-      a2 += S.joo_part[(r + 2) * o + k];
-      a3 += S.joo_part[(r + 3) * o + k];
-    }
-    for (; r < S.v; ++r) a0 += S.joo_part[r * o + k];
-    S.joo[k] = (a0 + a1) + (a2 + a3);
+    double a = 0.0;
+    for (int r = s0; r < s1; ++r) a += S.joo_part[r * o + k];
+    S.joo[part * o + k] = a;
   }
 }
 
@@ -732,7 +728,7 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
   else if (it.y == CK_HH)
     hh_tile(S, it.z, it.w);
   else
-    joo_wave(S);
+    joo_wave(S, it.z);
 }
 
 // Block partials of the residual pass: max dT, max |R|, sum E, the non-finite
@@ -787,10 +783,18 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   const int x = lb * kElemPerBlock + threadIdx.x;
   int b = -1;
   if (x < n3) b = S.pidx[x];
-  stage(S, L);
+  __shared__ double joo_l[kMaxOcc];
+  if (S.do_mosaic)  // Joo from its slice-range parts, in part order
+    for (int k = threadIdx.x; k < o; k += blockDim.x) {
+      double a = 0.0;
+#pragma unroll
+      for (int p = 0; p < kJooParts; ++p) a += S.joo[p * o + k];
+      joo_l[k] = a;
+    }
+  stage(S, L);  // (its barrier also publishes joo_l)
   if (phase == PH_DONE) return;
   const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
-  const double2* __restrict__ T = S.tcur;
+  const auto T = S.tcur;
   const bool valid = b >= 0;
   const int i = x % o, j = (x / o) % o, av = x / o2, a = o + av, bv = b - o;
 
@@ -805,8 +809,8 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
     double gba = -VPQ(i, b);
     // mosaics (source/CCD.f90:571-582)
     if (S.do_mosaic) {
-      const double* jvv = S.jvv;
-      const double* joo = S.joo;
+      const auto jvv = S.jvv;
+      const double* joo = joo_l;
       gab = gab - 0.5 * (jvv[av] * t_ab + jvv[bv] * t_ab);
       gba = gba - 0.5 * (jvv[av] * t_ba + jvv[bv] * t_ba);
       gab = gab - 0.5 * (joo[i] * t_ab + joo[j] * t_ab);
@@ -1259,7 +1263,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __rest
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {
   const SolveG& S = as_global(solves)[blockIdx.x];
   if (threadIdx.x != 0) return;
-  State* st = S.st;
+  const auto st = S.st;
   st->niter = 0;
   st->phase = phase;
   st->slot = 0;
@@ -1465,7 +1469,7 @@ int basis_on_device(int cell, int maxk, const double* twist, int npad, double* k
   hipLaunchKernelGGL(k_basis_keys, dim3((npad + 255) / 256), dim3(256), 0, s, cell, maxk, twist[0],
                      twist[1], twist[2], npad, key, idx);
   const int lds = npad * (int)(sizeof(double) + sizeof(int));
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_basis_sort),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_basis_sort),
                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   hipLaunchKernelGGL(k_basis_sort, dim3(1), dim3(1024), lds, s, npad, key, idx, count);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1486,7 +1490,7 @@ int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const do
 static void allow_large_lds() {
   static bool done = false;
   if (done) return;
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_slice),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_slice),
                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds - (int)sizeof(Lds));
   done = true;
 }

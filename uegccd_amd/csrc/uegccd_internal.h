@@ -100,7 +100,7 @@ struct SolveDevT {
   Ptr<double> csum;         // [v*o] slice column sums
   Ptr<double> jvv;          // [v] mosaic Jvv
   Ptr<double> joo_part;     // [v*o] per-slice parts of the mosaic Joo
-  Ptr<double> joo;          // [o] mosaic Joo
+  Ptr<double> joo;          // [kJooParts][o] mosaic Joo summed over each slice range
   Ptr<double> partials;     // [nblk_elem * kPartialStride]
   int nblk_elem;
   Ptr<State> st;
@@ -127,6 +127,7 @@ struct BatchDev {
 // products of one transfer q, a 16x16 hole-hole ladder tile of one K block,
 // the mosaic Joo sums of one solve.
 enum ContractKind : int { CK_QUAD = 0, CK_HH = 1, CK_JOO = 2 };
+constexpr int kJooParts = 32;  // slice ranges the mosaic Joo sum is split over (one wave each)
 constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
 inline int slice_lds_bytes(int o) { return (3 * o * o + 3 * o) * 8; }
