@@ -279,7 +279,10 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       for (int q = 0; q < nq; ++q) {
         const int4 in = c->qb_info[q];
         const int nt = ((in.y + 15) / 16) * ((in.w + 15) / 16);
-        for (int t = 0; t < nt; ++t) c->con_items.push_back(make_int4(0, CK_QUAD, q, t));
+        for (int t = 0; t < nt; ++t) {
+          if (s.do_ring) c->con_items.push_back(make_int4(0, CK_RING, q, t));
+          if (s.do_xring) c->con_items.push_back(make_int4(0, CK_XRING, q, t));
+        }
       }
     if (s.do_ladder)
       for (int K = 0; K < c->nK; ++K) {

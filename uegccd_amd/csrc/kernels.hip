@@ -535,16 +535,18 @@ __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ sol
 // and every (row, column) pair of P(q) x P(-q) is a valid element.  So
 //   ring(i,j,a)  = sum_l J(a,i,l) . T(l,j,d_q(l)),  n_d = n_l + q  (source/CCD.f90:801-808)
 //   xring(i,j,a) = sum_l Jx(b,i,l) . T(l,j,a)                      (source/CCD.f90:927-934)
-// are, per q, one product [rows x o] . [o x columns] each: rows and columns
-// are staged in LDS, each output written once (ring to quad.xy, crossed ring
+// are, per q, one product [rows x o] . [o x columns] each (A from the J rows,
+// B gathered from T), each output written once (ring to quad.xy, crossed ring
 // to quad.zw of its element).
 //
-// The tile's loads go out in three rounds with no branch between them (masked
-// values, clamped in-range addresses): every index (the column's pair, the
-// four output rows' pairs, the partner slices d_q(l)), then the J rows and
-// the crossed-ring T columns, then the ring T columns that need d_q(l).
+// Rings and crossed rings are separate work items (one wave each): half the
+// registers per wave, so more waves are resident.  A tile's loads go out in
+// two rounds with no branch between them (masked values, clamped in-range
+// addresses): the J rows with every index (the column's pair, the four output
+// rows' pairs, the partner slices d_q(l)), then the T columns they select.
 constexpr int kQuadSteps = 8;  // k-steps of 4 l values per round of loads (l < 32)
 
+template <bool XRING>
 __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int4 info = S.qb_info[qb];  // rows of q: [r0, r0+n); columns (rows of -q): [c0, c0+m)
@@ -554,76 +556,73 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   const int o = S.o, o2 = o * o;
   const int row0 = (tile / ntc) * 16, col0 = (tile % ntc) * 16;
   const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const bool ring = S.do_ring, xring = S.do_xring;
   // lane supplies A[row r16][l] (J rows, contiguous per q) and B[l][column r16]
   const int rowA = row0 + r16, colB = col0 + r16;
   const bool rv = rowA < n, cv = colB < m;
-  const int pc = S.qb_rows[c0 + (cv ? colB : 0)];
-  int prow[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) prow[r] = S.qb_rows[r0 + min(row0 + MfmaTraits<double>::row(kq, r), n - 1)];
-  const int j = pc % o, pcv = pc / o;
-  const auto jr = S.jring + (size_t)(r0 + (rv ? rowA : 0)) * o;
-  const auto jx = S.jxring + (size_t)(r0 + (rv ? rowA : 0)) * o;
+  const auto J = (XRING ? S.jxring : S.jring) + (size_t)(r0 + (rv ? rowA : 0)) * o;
   const auto dq = S.qb_d + (size_t)qb * o;
   const auto T = S.tcur;
-  const size_t tcol = (size_t)o * j;
-  acc_t rab = {0, 0, 0, 0}, rba = {0, 0, 0, 0}, xab = {0, 0, 0, 0}, xba = {0, 0, 0, 0};
+  acc_t cab = {0, 0, 0, 0}, cba = {0, 0, 0, 0};
+  int prow[4], j = 0, pcv = 0;
   for (int g0 = 0; g0 < o; g0 += 4 * kQuadSteps) {
+    double a1[kQuadSteps], a2[kQuadSteps], a3[kQuadSteps];
     int dv[kQuadSteps];
 #pragma unroll
-    for (int st = 0; st < kQuadSteps; ++st) dv[st] = dq[min(g0 + 4 * st + kq, o - 1)];
-    double a1[kQuadSteps], a2[kQuadSteps], a3[kQuadSteps];
-    double x1[kQuadSteps], x2[kQuadSteps], x3[kQuadSteps];
-    double2 bx[kQuadSteps];
-#pragma unroll
     for (int st = 0; st < kQuadSteps; ++st) {
       const int l = g0 + 4 * st + kq;
-      const bool av = l < o && rv, bv = l < o && cv;
-      const int lc = min(l, o - 1);
-      const double4 ar = ld(jr + lc), ax = ld(jx + lc);
-      a1[st] = (av && ring) ? ar.x : 0.0;
-      a2[st] = (av && ring) ? ar.y : 0.0;
-      a3[st] = (av && ring) ? ar.z : 0.0;
-      x1[st] = (av && xring) ? ax.x : 0.0;
-      x2[st] = (av && xring) ? ax.y : 0.0;
-      x3[st] = (av && xring) ? ax.z : 0.0;
-      const double2 t = ld(T + lc + tcol + (size_t)o2 * pcv);
-      bx[st] = (bv && xring) ? t : make_double2(0.0, 0.0);
+      const double4 a = ld(J + min(l, o - 1));
+      const bool ok = l < o && rv;
+      a1[st] = ok ? a.x : 0.0;
+      a2[st] = ok ? a.y : 0.0;
+      a3[st] = ok ? a.z : 0.0;
+      if (!XRING) dv[st] = dq[min(l, o - 1)];
     }
-    double2 br[kQuadSteps];
+    if (g0 == 0) {
+      const int pc = S.qb_rows[c0 + (cv ? colB : 0)];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        prow[r] = S.qb_rows[r0 + min(row0 + MfmaTraits<double>::row(kq, r), n - 1)];
+      j = pc % o;
+      pcv = pc / o;
+    }
+    const size_t tcol = (size_t)o * j;
+    double2 b[kQuadSteps];
 #pragma unroll
     for (int st = 0; st < kQuadSteps; ++st) {
       const int l = g0 + 4 * st + kq;
-      const bool ok = l < o && cv && ring && dv[st] >= 0;
-      const double2 t = ld(T + min(l, o - 1) + tcol + (size_t)o2 * max(dv[st], 0));
-      br[st] = ok ? t : make_double2(0.0, 0.0);
+      const int slice = XRING ? pcv : max(dv[st], 0);
+      const bool ok = l < o && cv && (XRING || dv[st] >= 0);
+      const double2 t = ld(T + min(l, o - 1) + tcol + (size_t)o2 * slice);
+      b[st] = ok ? t : make_double2(0.0, 0.0);
     }
     // (all steps, the masked ones add zeros: a branch here would let the
     // compiler sink each step's loads behind the previous step's MFMAs)
 #pragma unroll
     for (int st = 0; st < kQuadSteps; ++st) {
-      // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab,  ba += J3 T^ba
-      rab = MfmaTraits<double>::mma(a2[st], br[st].x + br[st].y, rab);
-      rab = MfmaTraits<double>::mma(a1[st], br[st].x, rab);
-      rba = MfmaTraits<double>::mma(a3[st], br[st].y, rba);
-      // crossed ring:  ba -= J1 T^ba + J2 (T^ab + T^ba),  ab += J3 T^ab
-      xba = MfmaTraits<double>::mma(-x1[st], bx[st].y, xba);
-      xba = MfmaTraits<double>::mma(-x2[st], bx[st].x + bx[st].y, xba);
-      xab = MfmaTraits<double>::mma(x3[st], bx[st].x, xab);
+      if (!XRING) {
+        // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab,  ba += J3 T^ba
+        cab = MfmaTraits<double>::mma(a2[st], b[st].x + b[st].y, cab);
+        cab = MfmaTraits<double>::mma(a1[st], b[st].x, cab);
+        cba = MfmaTraits<double>::mma(a3[st], b[st].y, cba);
+      } else {
+        // crossed ring:  ba -= J1 T^ba + J2 (T^ab + T^ba),  ab += J3 T^ab
+        cba = MfmaTraits<double>::mma(-a1[st], b[st].y, cba);
+        cba = MfmaTraits<double>::mma(-a2[st], b[st].x + b[st].y, cba);
+        cab = MfmaTraits<double>::mma(a3[st], b[st].x, cab);
+      }
     }
   }
   if (!cv) return;
+  const size_t tcol = (size_t)o * j;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = row0 + MfmaTraits<double>::row(kq, r);
     if (row >= n) continue;
-    // ring sums to element (i, j, a) with a the row's particle, crossed-ring
-    // sums to (i, j, a') with a' the column's particle
+    // ring sums go to element (i, j, a) with a the row's particle (quad.xy),
+    // crossed-ring sums to (i, j, a') with a' the column's particle (quad.zw)
     const int pr = prow[r];
-    const auto q = S.quad + (size_t)(pr % o) + tcol;
-    if (ring) st2(reinterpret_cast<GlobalPtr<double2>>(q + (size_t)o2 * (pr / o)), make_double2(rab[r], rba[r]));
-    if (xring) st2(reinterpret_cast<GlobalPtr<double2>>(q + (size_t)o2 * pcv) + 1, make_double2(xab[r], xba[r]));
+    const auto q = S.quad + (size_t)(pr % o) + tcol + (size_t)o2 * (XRING ? pcv : pr / o);
+    st2(reinterpret_cast<GlobalPtr<double2>>(q) + (XRING ? 1 : 0), make_double2(cab[r], cba[r]));
   }
 }
 
@@ -723,8 +722,10 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
   const int4 it = items[item];
   const SolveG& S = as_global(solves)[it.x];
   // (no phase test: the contraction outputs of a finished solve are never read)
-  if (it.y == CK_QUAD)
-    quad_tile(S, it.z, it.w);
+  if (it.y == CK_RING)
+    quad_tile<false>(S, it.z, it.w);
+  else if (it.y == CK_XRING)
+    quad_tile<true>(S, it.z, it.w);
   else if (it.y == CK_HH)
     hh_tile(S, it.z, it.w);
   else

@@ -126,7 +126,7 @@ struct BatchDev {
 // k_contract work items, one per wave: a 16x16 tile of the quadratic ring
 // products of one transfer q, a 16x16 hole-hole ladder tile of one K block,
 // the mosaic Joo sums of one solve.
-enum ContractKind : int { CK_QUAD = 0, CK_HH = 1, CK_JOO = 2 };
+enum ContractKind : int { CK_RING = 0, CK_HH = 1, CK_JOO = 2, CK_XRING = 3 };
 constexpr int kJooParts = 32;  // slice ranges the mosaic Joo sum is split over (one wave each)
 constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
