@@ -321,7 +321,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double2) * n3) * (1 + 2 * ndiis + 2);      // tcur, histT, histE, gout, rout
   bytes += aligned(sizeof(double2) * n3);                              // lad
   bytes += aligned(sizeof(double2) * std::max(1, c->wtot));            // w
-  bytes += 2 * aligned(sizeof(double4) * std::max<size_t>(1, ovo));    // jring, jxring
+  bytes += 2 * aligned(sizeof(double) * 3 * std::max<size_t>(1, ovo));  // jring, jxring
   bytes += aligned(sizeof(double4) * n3);                              // tv
   bytes += 2 * aligned(sizeof(double) * c->o * c->v);                  // rsum, csum
   bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
@@ -557,8 +557,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.lad = c->arena.take<double2>(n3);
   S.wtot = c->wtot;
   S.w = c->arena.take<double2>(std::max(1, c->wtot));
-  S.jring = c->arena.take<double4>(std::max<size_t>(1, ovo));
-  S.jxring = c->arena.take<double4>(std::max<size_t>(1, ovo));
+  S.jring = c->arena.take<double>(3 * std::max<size_t>(1, ovo));
+  S.jxring = c->arena.take<double>(3 * std::max<size_t>(1, ovo));
   S.tv = c->arena.take<double4>(n3);
   S.quad = c->arena.take<double4>(n3);
   S.rsum = c->arena.take<double>((size_t)c->o * c->v);

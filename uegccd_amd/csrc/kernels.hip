@@ -20,7 +20,7 @@
 // Interleaved layouts keep each gathered access one 16/32/64-byte load:
 //   tcur, lad, histT/histE entries   double2  (abab, abba) or (S, P) per (i,j,a)
 //   tv                               double4  (TV_ab, TV_ba, VT_ab, VT_ba)
-//   jring / jxring                   double4  (J1, J2, J3, 0) per (s, i, l)
+//   jring / jxring                   3 doubles (J1, J2, J3) per (row of P(q), l)
 //   histT / histE                    slot-major: slot k of x at k*n3 + x
 #include <hip/hip_ext.h>
 
@@ -281,7 +281,6 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   // slice_split blocks per slice, each computing a share of the o*o outputs
   const int split = slice_split(S.o);
   const int lb = bid - prefix[s], sv = lb / split, part = lb - sv * split;
-  stage(S, L);
   const int o = S.o, o2 = o * o, so = o + sv;
   const int out0 = (o2 * part) / split, out1 = (o2 * (part + 1)) / split;
   double* Tab = dyn;
@@ -290,43 +289,79 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   double* rs = dyn + 3 * o2;
   double* cs = rs + o;
   double* jp = cs + o;
+  double* vs = jp + o;  // v(n_s - n_k)
+  // The slab goes to registers in the same round of loads as the tables
+  // (the first kSlabPerThread * blockDim entries; the rest, for o > 32, after
+  // them), then to LDS once the tables are in.
+  constexpr int kSlabPerThread = 4;
   const auto tslab = S.tcur + (size_t)o2 * sv;
+  double2 tr[kSlabPerThread];
+#pragma unroll
+  for (int m = 0; m < kSlabPerThread; ++m) {
+    const int idx = threadIdx.x + m * blockDim.x;
+    tr[m] = idx < o2 ? ld(tslab + idx) : make_double2(0.0, 0.0);
+  }
+  stage(S, L);
   // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
   // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
   // so the table reads as zero there (the last entry is always past n2max).
   for (int k = S.n2max + 1 + threadIdx.x; k < kMaxN2; k += blockDim.x) L.vt[k] = 0.0;
-  for (int idx = threadIdx.x; idx < o2; idx += blockDim.x) {
-    const double2 t = tslab[idx];
+#pragma unroll
+  for (int m = 0; m < kSlabPerThread; ++m) {
+    const int idx = threadIdx.x + m * blockDim.x;
+    if (idx < o2) {
+      Tab[idx] = tr[m].x;
+      Tba[idx] = tr[m].y;
+      Vo[idx] = VPQ(idx % o, idx / o);
+    }
+  }
+  for (int idx = threadIdx.x + kSlabPerThread * blockDim.x; idx < o2; idx += blockDim.x) {
+    const double2 t = ld(tslab + idx);
     Tab[idx] = t.x;
     Tba[idx] = t.y;
     Vo[idx] = VPQ(idx % o, idx / o);
   }
+  for (int k = threadIdx.x; k < o; k += blockDim.x) vs[k] = VPQ(so, k);
   __syncthreads();
-  // row/column sums (every block needs them for J); the first block of the
-  // slice stores them with the mosaic partials
-  for (int i = threadIdx.x; i < o; i += blockDim.x) {
+  // Row/column sums (every block needs them for J) by groups of 8 lanes per
+  // row i, each lane a stride-8 share of k, then a 3-step shuffle tree; the
+  // first block of the slice stores them with the mosaic partials
+  // P_s(i) = sum_k v(s-i)(2T^ab + T^ba)(i,k) - v(s-k)(T^ab + 2T^ba)(i,k).
+  for (int i0 = 0; i0 < o; i0 += (int)blockDim.x / 8) {
+    const int i = i0 + (int)threadIdx.x / 8, sub = threadIdx.x & 7;
     double r = 0.0, c = 0.0, p = 0.0;
-    const double vsi = VPQ(so, i);
-    for (int k = 0; k < o; ++k) {
-      const double ta = Tab[i + o * k], tb = Tba[i + o * k];
-      r += 2.0 * ta + tb;
-      c += Tab[k + o * i] + 2.0 * Tba[k + o * i];
-      p += vsi * (2.0 * ta + tb) - VPQ(so, k) * (ta + 2.0 * tb);
+    if (i < o) {
+      const double vsi = vs[i];
+      for (int k = sub; k < o; k += 8) {
+        const double ta = Tab[i + o * k], tb = Tba[i + o * k];
+        r += 2.0 * ta + tb;
+        c += Tab[k + o * i] + 2.0 * Tba[k + o * i];
+        p += vsi * (2.0 * ta + tb) - vs[k] * (ta + 2.0 * tb);
+      }
     }
-    rs[i] = r;
-    cs[i] = c;
-    jp[i] = p;
-    if (part == 0) {
-      S.rsum[sv * o + i] = r;
-      S.csum[sv * o + i] = c;
-      S.joo_part[sv * o + i] = p;
+#pragma unroll
+    for (int m = 4; m >= 1; m >>= 1) {
+      r += __shfl_xor(r, m, 64);
+      c += __shfl_xor(c, m, 64);
+      p += __shfl_xor(p, m, 64);
+    }
+    if (i < o && sub == 0) {
+      rs[i] = r;
+      cs[i] = c;
+      jp[i] = p;
+      if (part == 0) {
+        S.rsum[sv * o + i] = r;
+        S.csum[sv * o + i] = c;
+        S.joo_part[sv * o + i] = p;
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && part == 0) {
-    double jv = 0.0;
-    for (int k = 0; k < o; ++k) jv += jp[k];
-    S.jvv[sv] = jv;
+  // Jvv(s) = sum_i P_s(i) by one wave (o <= 64), a fixed tree
+  if (part == 0 && threadIdx.x < 64) {
+    double jv = (int)threadIdx.x < o ? jp[threadIdx.x] : 0.0;
+    jv = wave_sum(jv);
+    if (threadIdx.x == 0) S.jvv[sv] = jv;
   }
   const int4 ns = L.nv[so];
   for (int idx = out0 + threadIdx.x; idx < out1; idx += blockDim.x) {
@@ -345,10 +380,13 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
     const int4 ni = L.nv[i], nl = L.nv[l];
     const int bx = ni.x - ns.x - nl.x, by = ni.y - ns.y - nl.y, bz = ni.z - ns.z - nl.z;
     double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0;
+    // (unrolled so that several table lookups, each waiting on its n_k, are
+    // in flight at once; |d| < 2^11, so 24-bit multiplies are exact)
+#pragma unroll 4
     for (int k = 0; k < o; ++k) {
       const int4 nk = L.nv[k];
       const int dx = bx + nk.x, dy = by + nk.y, dz = bz + nk.z;
-      const double y = L.vt[min(dx * dx + dy * dy + dz * dz, kMaxN2 - 1)];
+      const double y = L.vt[min(__mul24(dx, dx) + __mul24(dy, dy) + __mul24(dz, dz), kMaxN2 - 1)];
       const double ta = Tab[i + o * k], tb = Tba[i + o * k];
       const double ua = Tab[k + o * i], ub = Tba[k + o * i];
       a1 += (ta + tb) * y;
@@ -359,9 +397,13 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       a6 += ua * y;
     }
     const double vq = VPQ(i, so);
-    const size_t jx = (size_t)S.qrow[sv * o + i] * o + l;
-    S.jring[jx] = make_double4(vq * rs[i] - a1, vq * rs[i] - a2, -a3, 0.0);
-    S.jxring[jx] = make_double4(vq * cs[i] - a4, vq * cs[i] - a5, a6, 0.0);
+    const size_t jx = 3 * ((size_t)S.qrow[sv * o + i] * o + l);
+    S.jring[jx] = vq * rs[i] - a1;
+    S.jring[jx + 1] = vq * rs[i] - a2;
+    S.jring[jx + 2] = -a3;
+    S.jxring[jx] = vq * cs[i] - a4;
+    S.jxring[jx + 1] = vq * cs[i] - a5;
+    S.jxring[jx + 2] = a6;
   }
 }
 
@@ -559,7 +601,7 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   // lane supplies A[row r16][l] (J rows, contiguous per q) and B[l][column r16]
   const int rowA = row0 + r16, colB = col0 + r16;
   const bool rv = rowA < n, cv = colB < m;
-  const auto J = (XRING ? S.jxring : S.jring) + (size_t)(r0 + (rv ? rowA : 0)) * o;
+  const auto J = (XRING ? S.jxring : S.jring) + 3 * (size_t)(r0 + (rv ? rowA : 0)) * o;
   const auto dq = S.qb_d + (size_t)qb * o;
   const auto T = S.tcur;
   acc_t cab = {0, 0, 0, 0}, cba = {0, 0, 0, 0};
@@ -570,11 +612,11 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
 #pragma unroll
     for (int st = 0; st < kQuadSteps; ++st) {
       const int l = g0 + 4 * st + kq;
-      const double4 a = ld(J + min(l, o - 1));
+      const auto a = J + 3 * min(l, o - 1);
       const bool ok = l < o && rv;
-      a1[st] = ok ? a.x : 0.0;
-      a2[st] = ok ? a.y : 0.0;
-      a3[st] = ok ? a.z : 0.0;
+      a1[st] = ok ? a[0] : 0.0;
+      a2[st] = ok ? a[1] : 0.0;
+      a3[st] = ok ? a[2] : 0.0;
       if (!XRING) dv[st] = dq[min(l, o - 1)];
     }
     if (g0 == 0) {

@@ -91,9 +91,9 @@ struct SolveDevT {
   Ptr<double2> lad;         // [n3] particle-particle ladder, (S, P) parts
   Ptr<double2> w;           // [wtot] hole-hole W intermediates, (S, P) parts
   int wtot;
-  Ptr<double4> jring;       // [o*v rows][o] quadratic ring intermediates (J1, J2, J3, 0),
+  Ptr<double> jring;        // [o*v rows][o][3] quadratic ring intermediates (J1, J2, J3),
                         // rows in the grouped P(q) order (qrow)
-  Ptr<double4> jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
+  Ptr<double> jxring;       // [o*v rows][o][3] quadratic crossed-ring intermediates, same rows
   Ptr<double4> tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
   Ptr<double4> quad;        // [n3] quadratic ring (ab, ba) and crossed-ring (ab, ba) sums
   Ptr<double> rsum;         // [v*o] slice row sums
@@ -130,7 +130,7 @@ enum ContractKind : int { CK_RING = 0, CK_HH = 1, CK_JOO = 2, CK_XRING = 3 };
 constexpr int kJooParts = 32;  // slice ranges the mosaic Joo sum is split over (one wave each)
 constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
-inline int slice_lds_bytes(int o) { return (3 * o * o + 3 * o) * 8; }
+inline int slice_lds_bytes(int o) { return (3 * o * o + 4 * o) * 8; }
 // Blocks per virtual slice in k_slice: one per 256 of the o*o outputs, at most 4.
 __host__ __device__ inline int slice_split(int o) {
   const int g = (o * o + 255) / 256;
