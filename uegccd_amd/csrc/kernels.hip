@@ -447,10 +447,12 @@ struct MfmaTraits<float> {
 
 // One 16x16 tile, the reduction over the K block's virtual list split over
 // the four waves of the block (wave w takes steps w, w+4, ...); the waves'
-// partial sums are combined in a fixed order.  vt is the Coulomb table in LDS.
+// partial sums are combined in a fixed order.  vt is the Coulomb table and
+// cls the block's virtual list (n_c, c), both in LDS, so each group of steps
+// costs one round of global loads (the amplitudes its list entries select).
 template <typename Real>
 __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
-                            double* red) {
+                            const int4* cls, double* red) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
@@ -458,7 +460,6 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
   const int ncol = nvv + noo;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const auto T = S.tcur;
-  const auto cl = S.kb_nva + voff;  // (n_c, c) of the block's virtual list
 
   const int rowA = row0 + r16;
   const int xpA = rowA < noo ? S.kb_pairs[poff + rowA] : -1;
@@ -475,30 +476,36 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
       cy = xq / o;
     }
   }
+  // the output rows' pairs, requested now (used after the reduction)
+  int xpo[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) xpo[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<Real>::row(kq, r), noo - 1)];
   const int4 nx = S.nv4[cx], ny = S.nv4[cy];
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
   // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
-  // their loads (column list, then amplitudes) before the MFMAs
-  constexpr int G = 4;
+  // their amplitude loads (unconditional, in-range addresses, masked values)
+  // before the MFMAs, which run for every step (masked steps add zeros).
+  // G = 2 measured fastest (G = 8: 17.9 us, 4: 16.2, 2: 15.2 on the N=54,
+  // |n|^2 <= 16 pass): the pass is bound by memory requests, not latency.
+  constexpr int G = 2;
   for (int g0 = 4 * wave; g0 < nvv; g0 += 16 * G) {
-    int4 nc[G];
+    double2 t[G];
 #pragma unroll
     for (int st = 0; st < G; ++st) {
       const int ci = g0 + 16 * st + kq;
-      nc[st] = ci < nvv ? ld(cl + ci) : make_int4(0, 0, 0, -1);
+      const int c = ci < nvv ? cls[ci].w : -1;
+      const bool ok = c >= 0 && xpA >= 0;
+      const double2 v = ld(T + (ok ? (size_t)xpA + (size_t)o2 * (c - o) : 0));
+      t[st] = ok ? v : make_double2(0.0, 0.0);
     }
-    double2 t[G];
-#pragma unroll
-    for (int st = 0; st < G; ++st)
-      t[st] = (nc[st].w >= 0 && xpA >= 0) ? T[(size_t)xpA + (size_t)o2 * (nc[st].w - o)]
-                                          : make_double2(0.0, 0.0);
 #pragma unroll
     for (int st = 0; st < G; ++st) {
-      if (g0 + 16 * st >= nvv) break;
+      const int ci = g0 + 16 * st + kq;
+      const int4 nc = cls[min(ci, nvv - 1)];
       Real bs = 0, bp = 0;
-      if (nc[st].w >= 0 && colv) {
-        const Real vx = (Real)vt[d2(nc[st], nx)];
-        const Real vy = (Real)vt[d2(nc[st], ny)];
+      if (ci < nvv && colv) {
+        const Real vx = (Real)vt[d2(nc, nx)];
+        const Real vy = (Real)vt[d2(nc, ny)];
         bs = vx - vy;
         bp = vx + vy;
       }
@@ -522,35 +529,39 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
       accp[r] = accp[r] + (Real)theirs[4 + r];
     }
   }
+#pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int rowo = row0 + MfmaTraits<Real>::row(kq, r);
     if (rowo >= noo) continue;
-    const int xpo = S.kb_pairs[poff + rowo];
+    const int xpo_r = xpo[r];
     const double as = (double)accs[r], ap = (double)accp[r];
     if (col < nvv) {
-      S.lad[(size_t)xpo + (size_t)o2 * (cx - o)] = make_double2(0.5 * as, 0.5 * ap);
+      st2(S.lad + (size_t)xpo_r + (size_t)o2 * (cx - o), make_double2(0.5 * as, 0.5 * ap));
     } else {
       const int q = col - nvv;
-      const int i = xpo % o;
-      const double vik = vt[d2(S.nv4[i], nx)], vil = vt[d2(S.nv4[i], ny)];
-      S.w[S.kb_woff[K] + rowo * noo + q] = make_double2(vik - vil + 0.5 * as, vik + vil + 0.5 * ap);
+      const int i = xpo_r % o;
+      const int4 ni = S.nv4[i];
+      const double vik = vt[d2(ni, nx)], vil = vt[d2(ni, ny)];
+      st2(S.w + S.kb_woff[K] + rowo * noo + q, make_double2(vik - vil + 0.5 * as, vik + vil + 0.5 * ap));
     }
   }
 }
 
 __device__ void ladder_block(const SolveDev* __restrict__ solves, const int4* __restrict__ items,
-                             int item, double* vt, double* red) {
+                             int item, double* vt, int4* cls, double* red) {
   const int4 it = items[item];
   const SolveG& S = as_global(solves)[it.x];
   if (!S.do_ladder) return;
-  const int phase = S.st->phase;  // requested with the table, tested after it
+  const int phase = S.st->phase;  // requested with the tables, tested after them
+  const int K = it.y, voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
+  for (int k = threadIdx.x; k < nvv; k += blockDim.x) cls[k] = ld(S.kb_nva + voff + k);
   __syncthreads();
   if (phase == PH_DONE) return;
   if (S.st->prec64)
-    ladder_tile<double>(S, it.y, it.z, it.w, vt, red);
+    ladder_tile<double>(S, K, it.z, it.w, vt, cls, red);
   else
-    ladder_tile<float>(S, it.y, it.z, it.w, vt, red);
+    ladder_tile<float>(S, K, it.z, it.w, vt, cls, red);
 }
 
 __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restrict__ solves,
@@ -563,8 +574,9 @@ __global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restri
 __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
                                                 const int4* __restrict__ items) {
   __shared__ double vt[kMaxN2];
+  __shared__ int4 cls[kMaxNao];  // a K block's virtual list has at most v < kMaxNao entries
   __shared__ double red[3 * 64 * 8];
-  ladder_block(solves, items, xcd_block(blockIdx.x, gridDim.x), vt, red);
+  ladder_block(solves, items, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
 }
 
 // --------------------------------------------------------------------------
