@@ -1012,84 +1012,61 @@ __device__ bool diis_coefficients(const double* bmat, int nh, int slot_cur, int 
 }
 
 
-// Reductions of the block partials and the iteration bookkeeping of
-// source/CCD.f90:80-99 (one block per solve).
-// The same coefficients for histories of up to 8 entries, by one thread:
+// The same coefficients for histories of up to 8 entries, by one wave:
 // c = B^-1 1 / (1^T B^-1 1) (the solution of the bordered system) with B
-// scaled by its largest diagonal and padded to 8 x 8 with the identity; a
-// Cholesky factorisation and two triangular solves, unrolled in registers.
-// A pivot below 1e-14 fails the extrapolation (the plain update is used).
-__device__ bool diis_cholesky8(const double* bmat, int nh, int slot_cur, int ndiis, double* coef) {
+// scaled by its largest diagonal and padded to 8 x 8 with the identity.
+// Lane 8r + c holds B[r][c]; Gauss-Jordan elimination without pivoting (B is
+// symmetric positive definite, so its pivots are those of a Cholesky
+// factorisation) brings B to the identity and 1 to y = B^-1 1, one pivot per
+// step, its row and column broadcast by shuffles.  A pivot below 1e-14 fails
+// the extrapolation (the plain update is used).  Every lane of the wave must
+// call it; the result is the same on all of them.
+__device__ bool diis_gauss_jordan8(const double* bmat, int nh, int slot_cur, int ndiis,
+                                   double* coef) {
   constexpr int M = 8;
-  int sl[M];  // history slot of row r, oldest first
-  {
-    int sv = slot_cur - (nh - 1);
+  const int lane = threadIdx.x & 63, r = lane >> 3, c = lane & 7;
+  // history slot of row q, oldest first
+  auto slot_of = [&](int q) {
+    int sv = slot_cur - (nh - 1) + q;
     if (sv < 0) sv += ndiis;
-#pragma unroll
-    for (int r = 0; r < M; ++r) {
-      sl[r] = sv;
-      sv = (sv + 1 == ndiis) ? 0 : sv + 1;
-    }
-  }
-  double scale = 0.0;
-#pragma unroll
-  for (int r = 0; r < M; ++r)
-    if (r < nh) scale = fmax(scale, fabs(bmat[sl[r] * (kMaxDiis + 1)]));
+    return sv >= ndiis ? sv - ndiis : sv;
+  };
+  const int sr = slot_of(r), sc = slot_of(c);
+  double scale = (lane < nh) ? fabs(bmat[slot_of(lane) * (kMaxDiis + 1)]) : 0.0;
+  scale = wave_max(scale);
   if (!(scale > 0.0) || !isfinite(scale)) return false;
   const double inv_scale = 1.0 / scale;
-  double L[M][M], inv_d[M];
-#pragma unroll
-  for (int r = 0; r < M; ++r)
-#pragma unroll
-    for (int c = 0; c < M; ++c)
-      L[r][c] = (r < nh && c < nh) ? bmat[sl[r] * kMaxDiis + sl[c]] * inv_scale
-                                   : (r == c ? 1.0 : 0.0);
+  double a = (r < nh && c < nh) ? bmat[sr * kMaxDiis + sc] * inv_scale : (r == c ? 1.0 : 0.0);
+  double b = 1.0;
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    double d = L[k][k];
-#pragma unroll
-    for (int j = 0; j < k; ++j) d -= L[k][j] * L[k][j];
-    if (!(d > 1e-14)) return false;
-    const double lkk = sqrt(d), inv_l = 1.0 / lkk;
-    L[k][k] = lkk;
-    inv_d[k] = inv_l;
-#pragma unroll
-    for (int i = k + 1; i < M; ++i) {
-      double v = L[i][k];
-#pragma unroll
-      for (int j = 0; j < k; ++j) v -= L[i][j] * L[k][j];
-      L[i][k] = v * inv_l;
+    const double p = __shfl(a, k * M + k, 64);
+    if (!(p > 1e-14)) return false;
+    const double inv_p = 1.0 / p;
+    const double rk = __shfl(a, k * M + c, 64);  // pivot row, this lane's column
+    const double ck = __shfl(a, r * M + k, 64);  // this lane's row, pivot column
+    const double bk = __shfl(b, k * M, 64);
+    if (r == k) {
+      a *= inv_p;
+      b *= inv_p;
+    } else {
+      const double f = ck * inv_p;
+      a -= f * rk;
+      b -= f * bk;
     }
   }
-  double y[M], z[M];
-#pragma unroll
-  for (int i = 0; i < M; ++i) {
-    double v = 1.0;
-#pragma unroll
-    for (int j = 0; j < i; ++j) v -= L[i][j] * y[j];
-    y[i] = v * inv_d[i];
-  }
-#pragma unroll
-  for (int i = M - 1; i >= 0; --i) {
-    double v = y[i];
-#pragma unroll
-    for (int j = i + 1; j < M; ++j) v -= L[j][i] * z[j];
-    z[i] = v * inv_d[i];
-  }
+  // y_r on lane 8r; the sum over the filled rows in row order
   double sum = 0.0;
 #pragma unroll
-  for (int r = 0; r < M; ++r)
-    if (r < nh) sum += z[r];
-  if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
-  const double inv_sum = 1.0 / sum;
-  for (int k = 0; k < ndiis; ++k) coef[k] = 0.0;
-#pragma unroll
-  for (int r = 0; r < M; ++r) {
-    if (r >= nh) continue;
-    const double c = z[r] * inv_sum;
-    if (!isfinite(c)) return false;
-    coef[sl[r]] = c;
+  for (int q = 0; q < M; ++q) {
+    const double yq = __shfl(b, q * M, 64);
+    if (q < nh) sum += yq;
   }
+  if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
+  const double cr = b * (1.0 / sum);
+  if (!__all(isfinite(cr) || r >= nh)) return false;
+  if (lane < ndiis) coef[lane] = 0.0;
+  if (c == 0 && r < nh) coef[sr] = cr;
   return true;
 }
 
@@ -1278,7 +1255,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __rest
   if (want && threadIdx.x < 64) {
     bool ok;
     if (S.ndiis <= 8) {
-      ok = threadIdx.x == 0 && diis_cholesky8(bm, nh, slot_cur, S.ndiis, coef);
+      ok = diis_gauss_jordan8(bm, nh, slot_cur, S.ndiis, coef);
     } else {
       ok = diis_coefficients<kMaxDiis + 1>(bm, nh, slot_cur, S.ndiis, coef);
     }
