@@ -158,11 +158,15 @@ int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* av
 
 /* In-solve timing of one kernel (numbering as uegccd_kernel_time, plus 5 for
  * the per-solve reductions, convergence test and DIIS solve; -1 turns it
- * off): every pass of the context's later solves
- * brackets that kernel with HIP events on the context's stream; uegccd_kernel_stats returns the summed
- * device time and the number of such launches since the last call (passes in
- * which some solve was still iterating; the passes launched after the last
- * solve finished return at once and are not counted), and resets them.  No
+ * off): in every pass of the context's later solves, HIP stop events on the
+ * context's stream are written by that kernel's dispatch and by the dispatch
+ * before it (kernels of a stream run back to back, so the difference is the
+ * kernel's duration as a kernel trace shows it; when it is the first kernel
+ * of a pass the first pass of each host poll is not timed).
+ * uegccd_kernel_stats returns the summed device time and the number of such
+ * launches since the last call (passes in which some solve was still
+ * iterating; the passes launched after the last solve finished return at
+ * once and are not counted), and resets them.  No
  * counterpart in the reference; measurement support. */
 int uegccd_time_kernel(uegccd_ctx* ctx, int32_t kernel);
 int uegccd_kernel_stats(uegccd_ctx* ctx, double* total_us, int64_t* launches);

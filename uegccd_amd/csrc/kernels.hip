@@ -1527,34 +1527,32 @@ static void allow_large_lds() {
   done = true;
 }
 
-// Launch through the extended API: when the timer names this kernel, its
-// start/stop events are written by the dispatch itself, so they bracket
-// exactly the kernel's execution (as a kernel trace measures it).
+// Launch through the extended API with an optional stop event, which the
+// dispatch itself writes when it completes.  A kernel is timed by the stop
+// events of the kernel before it and of itself: kernels of one stream run
+// back to back (a kernel trace shows no gap between them), so the difference
+// is the kernel's trace duration.  (A start event would be a separate marker
+// packet ahead of the dispatch and add its own time.)
 template <typename F, typename... Args>
-static void launch_k(F kernel, int which, const KernelTimer* t, dim3 grid, dim3 block,
-                     uint32_t lds, hipStream_t s, Args... args) {
-  const bool timed = t && t->which == which;
-  hipExtLaunchKernelGGL(kernel, grid, block, lds, s, timed ? t->before : nullptr,
-                        timed ? t->after : nullptr, 0, args...);
+static void launch_k(F kernel, hipEvent_t stop, dim3 grid, dim3 block, uint32_t lds, hipStream_t s,
+                     Args... args) {
+  hipExtLaunchKernelGGL(kernel, grid, block, lds, s, nullptr, stop, 0, args...);
 }
 
-static void launch_slice(const BatchDev& b, hipStream_t s, const KernelTimer* t = nullptr) {
-  if (b.slice_blocks <= 0) return;
+static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   allow_large_lds();
-  launch_k(k_slice, 0, t, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
+  launch_k(k_slice, stop, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
            (const SolveDev*)b.solves, (const int*)b.slice_prefix, b.n);
 }
 
-static void launch_ladder(const BatchDev& b, hipStream_t s, const KernelTimer* t = nullptr) {
-  if (b.nlad <= 0) return;
-  launch_k(k_ladder, 2, t, dim3(b.nlad), dim3(256), 0, s, (const SolveDev*)b.solves,
+static void launch_ladder(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
+  launch_k(k_ladder, stop, dim3(b.nlad), dim3(256), 0, s, (const SolveDev*)b.solves,
            (const int4*)b.lad_items);
 }
 
-static void launch_contract(const BatchDev& b, hipStream_t s, const KernelTimer* t = nullptr) {
-  if (b.ncon <= 0) return;
+static void launch_contract(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   const int grid = (b.ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock;
-  launch_k(k_contract, 1, t, dim3(grid), dim3(kElemThreads), 0, s, (const SolveDev*)b.solves,
+  launch_k(k_contract, stop, dim3(grid), dim3(kElemThreads), 0, s, (const SolveDev*)b.solves,
            (const int4*)b.con_items, b.ncon);
 }
 
@@ -1577,16 +1575,36 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_states, dim3(b.n), dim3(64), 0, s, b.solves, out);
 }
 
+// The kernels of one pass in launch order (numbering as KernelTimer::which).
+static int pass_kernels(const BatchDev& b, int main_flags, int* order) {
+  int n = 0;
+  if (b.slice_blocks > 0) order[n++] = 0;
+  if (b.nlad > 0) order[n++] = 2;
+  if (b.ncon > 0) order[n++] = 1;
+  order[n++] = 3;
+  if (main_flags & MF_ITERATE) {
+    order[n++] = 5;
+    order[n++] = 4;
+  }
+  return n;
+}
+
+int first_pass_kernel(const BatchDev& b, int main_flags) {
+  int order[6];
+  pass_kernels(b, main_flags, order);
+  return order[0];
+}
+
 void launch_one(const BatchDev& b, int which, hipStream_t s) {
   switch (which) {
     case 0:
-      launch_slice(b, s);
+      if (b.slice_blocks > 0) launch_slice(b, s);
       break;
     case 1:
-      launch_contract(b, s);
+      if (b.ncon > 0) launch_contract(b, s);
       break;
     case 2:
-      launch_ladder(b, s);
+      if (b.nlad > 0) launch_ladder(b, s);
       break;
     case 3:
       hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
@@ -1601,17 +1619,35 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
   }
 }
 
-void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const KernelTimer* timer) {
-  launch_slice(b, s, timer);
-  launch_ladder(b, s, timer);
-  launch_contract(b, s, timer);
-  launch_k(k_main, 3, timer, dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
-           (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
-  if (main_flags & MF_ITERATE) {
-    launch_k(k_finalize, 5, timer, dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,
-             (int)FIN_ITER);
-    launch_k(k_extrap, 4, timer, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s,
-             (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n);
+void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const KernelTimer* timer,
+                      hipEvent_t tail_stop) {
+  int order[6];
+  const int n = pass_kernels(b, main_flags, order);
+  hipEvent_t stop[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < n && timer; ++i)
+    if (order[i] == timer->which) {
+      stop[i] = timer->after;
+      if (i > 0) stop[i - 1] = timer->before;
+    }
+  if (tail_stop) stop[n - 1] = tail_stop;
+  for (int i = 0; i < n; ++i) {
+    switch (order[i]) {
+      case 0: launch_slice(b, s, stop[i]); break;
+      case 2: launch_ladder(b, s, stop[i]); break;
+      case 1: launch_contract(b, s, stop[i]); break;
+      case 3:
+        launch_k(k_main, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
+                 (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
+        break;
+      case 5:
+        launch_k(k_finalize, stop[i], dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,
+                 (int)FIN_ITER);
+        break;
+      case 4:
+        launch_k(k_extrap, stop[i], dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s,
+                 (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n);
+        break;
+    }
   }
 }
 

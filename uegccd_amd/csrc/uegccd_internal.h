@@ -141,14 +141,18 @@ constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 // Kernel launchers (kernels.hip).
 void launch_init(const BatchDev& b, hipStream_t s);
 void launch_mp2(const BatchDev& b, hipStream_t s);
-// Events bracketing one kernel of a pass (which: as launch_one, or 5 for the
-// reductions and DIIS solve), or none.
+// Events timing one kernel of a pass (which: as launch_one, or 5 for the
+// reductions and DIIS solve): `after` is written when it completes, `before`
+// when the kernel launched just before it completes.  When `which` is the
+// first kernel of the pass, `before` belongs to the previous pass: pass that
+// pass `tail_stop` (the event is written when its last kernel completes).
 struct KernelTimer {
   int which;
   hipEvent_t before, after;
 };
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s,
-                      const KernelTimer* timer = nullptr);
+                      const KernelTimer* timer = nullptr, hipEvent_t tail_stop = nullptr);
+int first_pass_kernel(const BatchDev& b, int main_flags);
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
