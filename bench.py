@@ -47,7 +47,7 @@ def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
     writes T; k_finalize reads the 4 + 8 partials of every element block."""
     n3 = o * o * v
     if kernel == "k_slice":
-        return (20.0 * o * n3, 16 * n3 + 32 * n3 + 2 * 32 * n3)
+        return (20.0 * o * n3, 16 * n3 + 32 * n3 + 2 * 24 * n3)
     if kernel == "k_ladder":
         return (8.0 * spp + 8.0 * shh, 16 * nvalid + 16 * nvalid + 16 * sum_noo2)
     if kernel == "k_contract":
