@@ -88,6 +88,7 @@ struct uegccd_ctx {
   double tolmax = 1e-8, failratio = 100, df = 2.0;
   // host copies of the maps
   std::vector<int> pidx, kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
+  std::vector<int> kb_toff, kpos;  // K-blocked amplitude layout: block offsets, element -> position
   std::vector<int4> nv4, kb_nva;
   std::vector<double> eig, vtab;
   std::vector<int4> lad_items;  // (0, K, row0, col0)
@@ -202,6 +203,19 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->kb_woff[K + 1] = c->kb_woff[K] + noo * noo;
   }
   c->wtot = c->kb_woff[c->nK];
+  // K-blocked layout of the valid amplitudes: block K holds, for each entry ci
+  // of its virtual list, its noo pairs contiguously (position toff + ci*noo + p)
+  c->kb_toff.assign(c->nK + 1, 0);
+  c->kpos.assign((size_t)c->n3, -1);
+  for (int K = 0; K < c->nK; ++K) {
+    const int p0 = c->kb_poff[K], noo = c->kb_poff[K + 1] - p0;
+    const int v0 = c->kb_voff[K], nvv = c->kb_voff[K + 1] - v0;
+    for (int ci = 0; ci < nvv; ++ci)
+      for (int p = 0; p < noo; ++p)
+        c->kpos[(size_t)c->kb_pairs[p0 + p] + (size_t)o * o * (c->kb_va[v0 + ci] - o)] =
+            c->kb_toff[K] + ci * noo + p;
+    c->kb_toff[K + 1] = c->kb_toff[K] + nvv * noo;
+  }
   // ladder tiles: 16 pair rows x 16 columns (a list then pair list)
   c->lad_items.clear();
   for (int K = 0; K < c->nK; ++K) {
@@ -319,6 +333,8 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->kb_nva.size()));
   bytes += 2 * aligned(sizeof(int) * c->o * c->o);
   bytes += aligned(sizeof(double2) * n3) * (1 + 2 * ndiis + 2);      // tcur, histT, histE, gout, rout
+  bytes += aligned(sizeof(double2) * std::max<size_t>(1, c->kb_toff.back()));  // tk
+  bytes += aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * n3);  // kb_toff, kpos
   bytes += aligned(sizeof(double2) * n3);                              // lad
   bytes += aligned(sizeof(double2) * std::max(1, c->wtot));            // w
   bytes += 2 * aligned(sizeof(double) * 3 * std::max<size_t>(1, ovo));  // jring, jxring
@@ -518,6 +534,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.kb_vb = im.put(c->kb_vb);
   S.kb_nva = im.put(c->kb_nva);
   S.kb_woff = im.put(c->kb_woff);
+  S.kb_toff = im.put(c->kb_toff);
+  S.kpos = im.put(c->kpos);
   S.pair_kb = im.put(c->pair_kb);
   S.pair_pos = im.put(c->pair_pos);
   S.nqb = (int)c->qb_neg.size();
@@ -550,6 +568,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   c->arena.used = aligned(c->arena.used);
   *work_off = c->arena.used;
   S.tcur = c->arena.take<double2>(n3);
+  S.tk = c->arena.take<double2>(std::max<int64_t>(1, c->kb_toff[c->nK]));
   S.histT = c->arena.take<double2>(n3 * ndiis);
   S.histE = c->arena.take<double2>(n3 * ndiis);
   S.gout = c->arena.take<double>(2 * n3);
@@ -659,6 +678,7 @@ int uegccd_residual(uegccd_ctx* c, const double* tab, const double* tba, double*
   int rc = copy_in_amplitudes(c, tab, tba);
   if (rc) return rc;
   launch_state_reset(c->batch, PH_ITER, 1, c->stream);
+  launch_init(c->batch, c->stream);  // (also lays the amplitudes out by K block)
   launch_iteration(c->batch, MF_WRITE_GR, c->stream);
   HIPCHK(hipGetLastError());
   const size_t n3 = c->n3;

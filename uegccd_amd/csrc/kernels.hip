@@ -184,7 +184,8 @@ __device__ __forceinline__ GlobalPtr<double> partial(const SolveG& S, int q, int
 }
 
 // --------------------------------------------------------------------------
-// Initial state: E(T0) and max|T0| (source/CCD.f90:62-64).
+// Initial state: E(T0) and max|T0| (source/CCD.f90:62-64); also lays the
+// starting amplitudes out by K block (tk), as k_extrap does for later passes.
 __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restrict__ solves,
                                                          const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
@@ -201,6 +202,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
     if (b >= 0) {
       const int i = x % o, j = (x / o) % o, a = o + x / o2;
       const double2 t = S.tcur[x];
+      st2(S.tk + S.kpos[x], t);
       const double taa = t.x + t.y;
       e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
       tmax = fmax(fabs(t.x), fabs(t.y));
@@ -459,10 +461,11 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int ncol = nvv + noo;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const auto T = S.tcur;
+  // A operand from the K-blocked amplitudes: column ci of the block holds its
+  // noo pairs contiguously, so a step's 16 rows are one contiguous run
+  const auto TK = S.tk + S.kb_toff[K];
 
   const int rowA = row0 + r16;
-  const int xpA = rowA < noo ? S.kb_pairs[poff + rowA] : -1;
   const int col = col0 + r16;
   const bool colv = col < ncol;
   int cx = 0, cy = 0;
@@ -493,9 +496,8 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
 #pragma unroll
     for (int st = 0; st < G; ++st) {
       const int ci = g0 + 16 * st + kq;
-      const int c = ci < nvv ? cls[ci].w : -1;
-      const bool ok = c >= 0 && xpA >= 0;
-      const double2 v = ld(T + (ok ? (size_t)xpA + (size_t)o2 * (c - o) : 0));
+      const bool ok = ci < nvv && rowA < noo;
+      const double2 v = ld(TK + (ok ? ci * noo + rowA : 0));
       t[st] = ok ? v : make_double2(0.0, 0.0);
     }
 #pragma unroll
@@ -683,8 +685,8 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
 // Hole-hole ladder contraction of one K block (source/CCD.f90:651-656):
 //   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a]
 // one wave64 per 16x16 (pair, a) tile on v_mfma_f64_16x16x4f64, added into
-// the particle-particle ladder of the same elements.  Loads as in quad_tile:
-// the pair lists and W first, then the amplitudes the pairs select; the four
+// the particle-particle ladder of the same elements.  W and the K-blocked
+// amplitudes (a column's pairs contiguous) load in one round; the four
 // output elements are read together before they are written.
 constexpr int kHHSteps = 8;  // k-steps of 4 pairs per round of loads
 
@@ -698,10 +700,11 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
   const int row0 = (tile / nct) * 16, col0 = (tile % nct) * 16;
   const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const auto W = S.w + S.kb_woff[K];
-  const auto T = S.tcur;
   const int rowA = row0 + r16, colB = col0 + r16;
   const bool rv = rowA < noo, cv = colB < nvv;
   const int av = S.kb_va[voff + (cv ? colB : 0)] - o;
+  // B operand from the K-blocked amplitudes: column colB holds its pairs q
+  const auto TK = S.tk + S.kb_toff[K] + (size_t)(cv ? colB : 0) * noo;
   int prow[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) prow[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<double>::row(kq, r), noo - 1)];
@@ -709,9 +712,6 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
   const auto Wrow = W + (size_t)(rv ? rowA : 0) * noo;
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
   for (int g0 = 0; g0 < noo; g0 += 4 * kHHSteps) {
-    int xq[kHHSteps];
-#pragma unroll
-    for (int st = 0; st < kHHSteps; ++st) xq[st] = S.kb_pairs[poff + min(g0 + 4 * st + kq, noo - 1)];
     double ws[kHHSteps], wp[kHHSteps];
 #pragma unroll
     for (int st = 0; st < kHHSteps; ++st) {
@@ -725,7 +725,7 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
 #pragma unroll
     for (int st = 0; st < kHHSteps; ++st) {
       const int q = g0 + 4 * st + kq;
-      const double2 t = ld(T + xq[st] + tslab);
+      const double2 t = ld(TK + min(q, noo - 1));
       const bool ok = q < noo && cv;
       bs[st] = ok ? t.x + t.y : 0.0;
       bp[st] = ok ? t.x - t.y : 0.0;
@@ -1238,9 +1238,10 @@ __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __rest
   const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
   const int x = lb * blockDim.x + threadIdx.x;
-  const bool mine = x < S.n3 && S.pidx[x] >= 0;
+  const int kp = x < S.n3 ? S.kpos[x] : -1;  // position in tk; -1 for an invalid element
+  const bool mine = kp >= 0;
   const size_t n3 = S.n3;
-  const double2* HT = S.histT + (mine ? x : 0);
+  const auto HT = S.histT + (mine ? x : 0);
   constexpr int kPre = 8;  // history entries loaded ahead
   double2 h[kPre];
 #pragma unroll
@@ -1290,6 +1291,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __rest
     if (sc >= kPre) t = HT[sc * n3];
   }
   S.tcur[x] = t;
+  st2(S.tk + kp, t);
 }
 
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {

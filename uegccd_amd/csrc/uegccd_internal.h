@@ -72,6 +72,8 @@ struct SolveDevT {
   Ptr<const int> kb_vb;     // their partners b
   Ptr<const int4> kb_nva;   // (n_a, a) of the same list
   Ptr<const int> kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
+  Ptr<const int> kb_toff;   // [nK+1] offsets of the K blocks in tk
+  Ptr<const int> kpos;      // [n3] position of element x in tk, or -1 (invalid)
   Ptr<const int> pair_kb;   // [o*o] block of pair x
   Ptr<const int> pair_pos;  // [o*o] position of pair x inside its block
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, one per transfer q
@@ -84,6 +86,8 @@ struct SolveDevT {
   Ptr<const int> qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
   // mutable buffers
   Ptr<double2> tcur;        // [n3] (abab, abba) amplitudes entering the next pass
+  Ptr<double2> tk;          // [nvalid] the same amplitudes by K block: entry ci of block K's
+                            // virtual list, pair p at kb_toff[K] + ci*noo + p
   Ptr<double2> histT;       // [ndiis][n3] updated amplitudes, slot-major (coalesced per slot)
   Ptr<double2> histE;       // [ndiis][n3] DIIS error vectors, slot-major
   Ptr<double> gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
