@@ -90,7 +90,7 @@ struct uegccd_ctx {
   std::vector<int> pidx, kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
   std::vector<int> kb_toff, kpos;  // K-blocked amplitude layout: block offsets, element -> position
   std::vector<int4> nv4, kb_nva;
-  std::vector<double> eig, vtab;
+  std::vector<double> eig, vtab, voo, vso;  // Coulomb table; v(n_k - n_j) over occupied k, j; v(n_s - n_k)
   std::vector<int4> lad_items;  // (0, K, row0, col0)
   std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow;  // hole-particle sets P(q)
   std::vector<int4> qb_info;
@@ -315,6 +315,19 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     if (gap_applies && val > s.gapii && s.gapii > 0.0) val = 0.0;
     c->vtab[q] = val;
   }
+  // the Coulomb elements every slice block needs, from the same table:
+  // V(k, j) = v(n_k - n_j) over occupied k, j (k fastest) and v(n_s - n_k)
+  auto d2 = [&](int p, int q) {
+    const int4 a = c->nv4[p], b = c->nv4[q];
+    const int x = a.x - b.x, y = a.y - b.y, z = a.z - b.z;
+    return x * x + y * y + z * z;
+  };
+  c->voo.assign((size_t)o * o, 0.0);
+  for (int j = 0; j < o; ++j)
+    for (int k = 0; k < o; ++k) c->voo[k + (size_t)o * j] = c->vtab[d2(k, j)];
+  c->vso.assign((size_t)v * o, 0.0);
+  for (int sv = 0; sv < v; ++sv)
+    for (int k = 0; k < o; ++k) c->vso[(size_t)sv * o + k] = c->vtab[d2(o + sv, k)];
   c->eig.assign(s.eigen, s.eigen + nao);
   return UEGCCD_OK;
 }
@@ -327,6 +340,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   size_t bytes = 0;
   bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
   bytes += aligned(sizeof(double) * (c->n2max + 1)) + aligned(sizeof(float) * (c->n2max + 1));
+  bytes += aligned(sizeof(double) * c->o * c->o) + aligned(sizeof(double) * c->v * c->o);  // voo, vso
   bytes += aligned(sizeof(int) * n3);
   bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
   bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
@@ -525,6 +539,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.nv4 = im.put(c->nv4);
   S.eig = im.put(c->eig);
   S.vtab = im.put(c->vtab);
+  S.voo = im.put(c->voo);
+  S.vso = im.put(c->vso);
   S.vtab32 = im.put(vt32);
   S.pidx = im.put(c->pidx);
   S.kb_poff = im.put(c->kb_poff);

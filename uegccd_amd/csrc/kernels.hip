@@ -255,6 +255,29 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
   }
 }
 
+// FP64/FP32 16x16x4 MFMA (the slice products, ladders and contractions).
+template <typename Real>
+struct MfmaTraits;
+// Operands of both forms: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result rows differ (measured on gfx950): the f64 form returns
+// D[(l/16) + 4r][l%16], the f32 form D[4(l/16) + r][l%16], r = 0..3.
+template <>
+struct MfmaTraits<double> {
+  typedef v4d acc_t;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int kq, int r) { return kq + 4 * r; }
+};
+template <>
+struct MfmaTraits<float> {
+  typedef v4f acc_t;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int kq, int r) { return 4 * kq + r; }
+};
+
 // --------------------------------------------------------------------------
 // Per-virtual-slice work, one workgroup per slice s (LDS-staged).
 //
@@ -292,17 +315,20 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   double* cs = rs + o;
   double* jp = cs + o;
   double* vs = jp + o;  // v(n_s - n_k)
-  // The slab goes to registers in the same round of loads as the tables
-  // (the first kSlabPerThread * blockDim entries; the rest, for o > 32, after
-  // them), then to LDS once the tables are in.
+  // The slab and the occupied Coulomb block V(k, j) go to registers in the
+  // same round of loads as the tables (the first kSlabPerThread * blockDim
+  // entries; the rest, for o > 32, after them), then to LDS.
   constexpr int kSlabPerThread = 4;
   const auto tslab = S.tcur + (size_t)o2 * sv;
   double2 tr[kSlabPerThread];
+  double vr[kSlabPerThread];
 #pragma unroll
   for (int m = 0; m < kSlabPerThread; ++m) {
     const int idx = threadIdx.x + m * blockDim.x;
     tr[m] = idx < o2 ? ld(tslab + idx) : make_double2(0.0, 0.0);
+    vr[m] = idx < o2 ? S.voo[idx] : 0.0;
   }
+  const double vsr = (int)threadIdx.x < o ? S.vso[sv * o + threadIdx.x] : 0.0;
   stage(S, L);
   // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
   // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
@@ -314,16 +340,16 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
     if (idx < o2) {
       Tab[idx] = tr[m].x;
       Tba[idx] = tr[m].y;
-      Vo[idx] = VPQ(idx % o, idx / o);
+      Vo[idx] = vr[m];
     }
   }
   for (int idx = threadIdx.x + kSlabPerThread * blockDim.x; idx < o2; idx += blockDim.x) {
     const double2 t = ld(tslab + idx);
     Tab[idx] = t.x;
     Tba[idx] = t.y;
-    Vo[idx] = VPQ(idx % o, idx / o);
+    Vo[idx] = S.voo[idx];
   }
-  for (int k = threadIdx.x; k < o; k += blockDim.x) vs[k] = VPQ(so, k);
+  if ((int)threadIdx.x < o) vs[threadIdx.x] = vsr;
   __syncthreads();
   // Row/column sums (every block needs them for J) by groups of 8 lanes per
   // row i, each lane a stride-8 share of k, then a 3-step shuffle tree; the
@@ -358,6 +384,37 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       }
     }
   }
+  // The slice products TV = T V and VT = V T of both spins as 16x16 FP64
+  // MFMA tiles over (r, c); the slice's tiles are dealt over its blocks' waves.
+  {
+    typedef MfmaTraits<double>::acc_t acc_t;
+    const int nt = (o + 15) >> 4, ntiles = nt * nt;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+    const int nw = blockDim.x >> 6;
+    for (int t = part * nw + wave; t < ntiles; t += split * nw) {
+      const int rt = t / nt, ct = t % nt;
+      const int ra = rt * 16 + r16, cb = ct * 16 + r16;
+      acc_t tva = {0, 0, 0, 0}, tvb = {0, 0, 0, 0}, vta = {0, 0, 0, 0}, vtb = {0, 0, 0, 0};
+      for (int k0 = 0; k0 < o; k0 += 4) {
+        const int k = k0 + kq;
+        const bool ak = ra < o && k < o, bk = cb < o && k < o;
+        const double ta_rk = ak ? Tab[ra + o * k] : 0.0, tb_rk = ak ? Tba[ra + o * k] : 0.0;
+        const double v_rk = ak ? Vo[ra + o * k] : 0.0;
+        const double v_kc = bk ? Vo[k + o * cb] : 0.0;
+        const double ta_kc = bk ? Tab[k + o * cb] : 0.0, tb_kc = bk ? Tba[k + o * cb] : 0.0;
+        tva = MfmaTraits<double>::mma(ta_rk, v_kc, tva);
+        tvb = MfmaTraits<double>::mma(tb_rk, v_kc, tvb);
+        vta = MfmaTraits<double>::mma(v_rk, ta_kc, vta);
+        vtb = MfmaTraits<double>::mma(v_rk, tb_kc, vtb);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + MfmaTraits<double>::row(kq, r);
+        if (row < o && cb < o)
+          S.tv[(size_t)o2 * sv + row + o * cb] = make_double4(tva[r], tvb[r], vta[r], vtb[r]);
+      }
+    }
+  }
   __syncthreads();
   // Jvv(s) = sum_i P_s(i) by one wave (o <= 64), a fixed tree
   if (part == 0 && threadIdx.x < 64) {
@@ -367,18 +424,8 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   }
   const int4 ns = L.nv[so];
   for (int idx = out0 + threadIdx.x; idx < out1; idx += blockDim.x) {
-    const int r = idx % o, c = idx / o;
-    double tva = 0.0, tvb = 0.0, vta = 0.0, vtb = 0.0;
-    for (int k = 0; k < o; ++k) {
-      const double vkc = Vo[k + o * c], vrk = Vo[r + o * k];
-      tva += Tab[r + o * k] * vkc;
-      tvb += Tba[r + o * k] * vkc;
-      vta += vrk * Tab[k + o * c];
-      vtb += vrk * Tba[k + o * c];
-    }
-    S.tv[(size_t)o2 * sv + idx] = make_double4(tva, tvb, vta, vtb);
     // quadratic intermediates for i = r, l = c
-    const int i = r, l = c;
+    const int i = idx % o, l = idx / o;
     const int4 ni = L.nv[i], nl = L.nv[l];
     const int bx = ni.x - ns.x - nl.x, by = ni.y - ns.y - nl.y, bz = ni.z - ns.z - nl.z;
     double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0;
@@ -398,7 +445,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       a5 += ub * y;
       a6 += ua * y;
     }
-    const double vq = VPQ(i, so);
+    const double vq = vs[i];
     const size_t jx = 3 * ((size_t)S.qrow[sv * o + i] * o + l);
     S.jring[jx] = vq * rs[i] - a1;
     S.jring[jx + 1] = vq * rs[i] - a2;
@@ -425,27 +472,6 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
 // stored: each lane synthesises its element from two n-vectors and the
 // Coulomb table.  One wave64 per 16x16 tile, v_mfma_f64_16x16x4f64 (or the
 // f32 form while the mixed-precision schedule is in its FP32 phase).
-template <typename Real>
-struct MfmaTraits;
-// Operands of both forms: lane l supplies A[l%16][l/16] and B[l/16][l%16].
-// Result rows differ (measured on gfx950): the f64 form returns
-// D[(l/16) + 4r][l%16], the f32 form D[4(l/16) + r][l%16], r = 0..3.
-template <>
-struct MfmaTraits<double> {
-  typedef v4d acc_t;
-  __device__ static acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  __device__ static int row(int kq, int r) { return kq + 4 * r; }
-};
-template <>
-struct MfmaTraits<float> {
-  typedef v4f acc_t;
-  __device__ static acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static int row(int kq, int r) { return 4 * kq + r; }
-};
 
 // One 16x16 tile, the reduction over the K block's virtual list split over
 // the four waves of the block (wave w takes steps w, w+4, ...); the waves'
