@@ -391,21 +391,23 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
     const int nt = (o + 15) >> 4, ntiles = nt * nt;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
     const int nw = blockDim.x >> 6;
-    for (int t = part * nw + wave; t < ntiles; t += split * nw) {
+    // tile t goes to wave t / split of block part t % split, so every block
+    // of the slice takes a share
+    for (int t = wave * split + part; t < ntiles; t += split * nw) {
       const int rt = t / nt, ct = t % nt;
       const int ra = rt * 16 + r16, cb = ct * 16 + r16;
+      const int rac = min(ra, o - 1), cbc = min(cb, o - 1);
       acc_t tva = {0, 0, 0, 0}, tvb = {0, 0, 0, 0}, vta = {0, 0, 0, 0}, vtb = {0, 0, 0, 0};
       for (int k0 = 0; k0 < o; k0 += 4) {
-        const int k = k0 + kq;
+        // in-range LDS addresses, masked values: no branch between the loads
+        const int k = k0 + kq, kc = min(k, o - 1);
         const bool ak = ra < o && k < o, bk = cb < o && k < o;
-        const double ta_rk = ak ? Tab[ra + o * k] : 0.0, tb_rk = ak ? Tba[ra + o * k] : 0.0;
-        const double v_rk = ak ? Vo[ra + o * k] : 0.0;
-        const double v_kc = bk ? Vo[k + o * cb] : 0.0;
-        const double ta_kc = bk ? Tab[k + o * cb] : 0.0, tb_kc = bk ? Tba[k + o * cb] : 0.0;
-        tva = MfmaTraits<double>::mma(ta_rk, v_kc, tva);
-        tvb = MfmaTraits<double>::mma(tb_rk, v_kc, tvb);
-        vta = MfmaTraits<double>::mma(v_rk, ta_kc, vta);
-        vtb = MfmaTraits<double>::mma(v_rk, tb_kc, vtb);
+        const double ta_rk = Tab[rac + o * kc], tb_rk = Tba[rac + o * kc], v_rk = Vo[rac + o * kc];
+        const double v_kc = Vo[kc + o * cbc], ta_kc = Tab[kc + o * cbc], tb_kc = Tba[kc + o * cbc];
+        tva = MfmaTraits<double>::mma(ak ? ta_rk : 0.0, bk ? v_kc : 0.0, tva);
+        tvb = MfmaTraits<double>::mma(ak ? tb_rk : 0.0, bk ? v_kc : 0.0, tvb);
+        vta = MfmaTraits<double>::mma(ak ? v_rk : 0.0, bk ? ta_kc : 0.0, vta);
+        vtb = MfmaTraits<double>::mma(ak ? v_rk : 0.0, bk ? tb_kc : 0.0, vtb);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
