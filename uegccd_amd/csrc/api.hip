@@ -106,6 +106,9 @@ struct uegccd_ctx {
   // in-solve timing of one kernel (uegccd_time_kernel): -1 off
   int timed_kernel = -1;
   std::vector<hipEvent_t> tev;   // before/after pairs, one per pass of a chunk
+  hipEvent_t span[2] = {nullptr, nullptr};  // brackets of a solve's iteration (created once)
+  State* h_states = nullptr;      // pinned host copies of the polled states
+  int h_states_cap = 0;
   double timed_us = 0.0;
   int64_t timed_launches = 0;
   bool owns_memory = false;  // arena from hipMalloc (else carved from a server pool)
@@ -669,6 +672,9 @@ void uegccd_ctx_destroy(uegccd_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
+  for (hipEvent_t e : c->span)
+    if (e) hipEventDestroy(e);
+  if (c->h_states) hipHostFree(c->h_states);
   if (c->stream && c->owns_stream) {
     hipStreamSynchronize(c->stream);
     hipStreamDestroy(c->stream);
@@ -932,9 +938,18 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     if ((cs[k]->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1) != prec64) return UEGCCD_ERR_INVALID;
   int maxiter = 0;
   for (int k = 0; k < n; ++k) maxiter = std::max(maxiter, cs[k]->maxiter);
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  // events and the pinned state buffer are made once per context and reused
+  if (!c0->span[0]) HIPCHK(hipEventCreate(&c0->span[0]));
+  if (!c0->span[1]) HIPCHK(hipEventCreate(&c0->span[1]));
+  if (c0->h_states_cap < n) {
+    if (c0->h_states) hipHostFree(c0->h_states);
+    c0->h_states = nullptr;
+    c0->h_states_cap = 0;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c0->h_states), sizeof(State) * n, hipHostMallocDefault));
+    c0->h_states_cap = n;
+  }
+  State* hst = c0->h_states;
+  const hipEvent_t e0 = c0->span[0], e1 = c0->span[1];
   HIPCHK(hipEventRecord(e0, s));
   launch_state_reset(b, PH_ITER, prec64, s);
   launch_init(b, s);
@@ -973,12 +988,13 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     HIPCHK(hipGetLastError());
     if (gather) {
       launch_gather_states(b, gather, s);
-      HIPCHK(hipMemcpyAsync(st.data(), gather, sizeof(State) * n, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(hst, gather, sizeof(State) * n, hipMemcpyDeviceToHost, s));
     } else {
       for (int k = 0; k < n; ++k)
-        HIPCHK(hipMemcpyAsync(&st[k], cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hst + k, cs[k]->d_state, sizeof(State), hipMemcpyDeviceToHost, s));
     }
     HIPCHK(hipStreamSynchronize(s));
+    std::copy(hst, hst + n, st.begin());
     if (timing) {
       // passes of this chunk in which some solve was still iterating: its
       // NIter advance plus its final residual pass (later passes return at once)
@@ -1005,8 +1021,6 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0.f;
   hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if (ms_out) *ms_out = ms;
   for (int k = 0; k < n; ++k) cs[k]->last_niter = st[k].niter;
   return UEGCCD_OK;
