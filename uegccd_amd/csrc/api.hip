@@ -951,9 +951,8 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   State* hst = c0->h_states;
   const hipEvent_t e0 = c0->span[0], e1 = c0->span[1];
   HIPCHK(hipEventRecord(e0, s));
-  launch_state_reset(b, PH_ITER, prec64, s);
   launch_init(b, s);
-  launch_finalize(b, FIN_INIT, s);
+  launch_finalize(b, FIN_INIT, s, prec64);
   HIPCHK(hipGetLastError());
   const int first = std::max(1, c0->opt.iters_per_sync);
   const int cap = std::max(first, kMaxChunk);
@@ -1607,10 +1606,9 @@ int admit(uegccd_server* s) {
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(mem + work_off, 0, fp - work_off, s->stream));
     const int prec64 = r.ctx->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
-    launch_state_reset(r.ctx->batch, PH_ITER, prec64, s->stream);
     launch_mp2(r.ctx->batch, s->stream);
     launch_init(r.ctx->batch, s->stream);
-    launch_finalize(r.ctx->batch, FIN_INIT, s->stream);
+    launch_finalize(r.ctx->batch, FIN_INIT, s->stream, prec64);
     HIPCHK(hipGetLastError());
     r.res.state = UEGCCD_REQ_RUNNING;
     r.res.t_start_ms = server_now(s);

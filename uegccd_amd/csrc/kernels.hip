@@ -1198,7 +1198,8 @@ __device__ bool advance_state(const SolveG& S, State* st, const double* vals, in
 // FIN_SUM after a plain energy evaluation.  The iteration state is worked on
 // in LDS: one parallel copy in, one out, instead of a chain of dependent
 // global accesses by a single thread.
-__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind) {
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind,
+                                                          int prec64) {
   __shared__ double vals[kPartialStride];
   __shared__ double red[(kFinThreads / 64) * 32];
   __shared__ State sst;
@@ -1225,6 +1226,12 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
     if (kind == FIN_ITER) {
       st->use_coef = advance_state(S, st, vals, phase0) ? 1 : 0;
     } else {
+      if (kind == FIN_INIT) {  // a fresh state: nothing carries over from an earlier solve
+        st->nonfinite = 0;
+        st->prec64 = prec64;
+        st->iters_fp32 = 0;
+        st->maxres = st->maxres_in = 0.0;
+      }
       if (vals[3] > 0.0) st->nonfinite = 1;
       st->ecorr = e;
       if (kind == FIN_INIT) {
@@ -1594,8 +1601,8 @@ void launch_mp2(const BatchDev& b, hipStream_t s) {
   hipLaunchKernelGGL(k_mp2, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
                      b.elem_prefix, b.n);
 }
-void launch_finalize(const BatchDev& b, int kind, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, kind);
+void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64) {
+  hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, kind, prec64);
 }
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s) {
   hipLaunchKernelGGL(k_state_reset, dim3(b.n), dim3(64), 0, s, b.solves, phase, prec64);
@@ -1671,7 +1678,7 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
         break;
       case 5:
         launch_k(k_finalize, stop[i], dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,
-                 (int)FIN_ITER);
+                 (int)FIN_ITER, 1);
         break;
       case 4:
         launch_k(k_extrap, stop[i], dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s,

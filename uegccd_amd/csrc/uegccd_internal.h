@@ -159,7 +159,8 @@ struct KernelTimer {
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s,
                       const KernelTimer* timer = nullptr, hipEvent_t tail_stop = nullptr);
 int first_pass_kernel(const BatchDev& b, int main_flags);
-void launch_finalize(const BatchDev& b, int kind, hipStream_t s);
+// FIN_INIT starts a fresh state (every field; prec64 the starting precision).
+void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64 = 1);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
 // One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main, 4 extrapolation.
