@@ -853,12 +853,11 @@ __device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, i
 __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restrict__ solves,
                                                         const int* __restrict__ prefix, int n,
                                                         int flags) {
-  __shared__ Lds L;
   __shared__ double red[(kElemThreads / 64) * 32];
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
-  // the state, the element's partner and the LDS tables are requested
+  // the state, the element's partner and the Joo parts are requested
   // together; a finished solve leaves after they arrive
   const int phase = S.st->phase, slot = S.st->slot;
   const int lb = gb - prefix[s];
@@ -874,7 +873,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       for (int p = 0; p < kJooParts; ++p) a += S.joo[p * o + k];
       joo_l[k] = a;
     }
-  stage(S, L);  // (its barrier also publishes joo_l)
+  __syncthreads();  // publishes joo_l
   if (phase == PH_DONE) return;
   const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
   const auto T = S.tcur;
@@ -887,9 +886,13 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   if (lead) {
     const double2 t0 = T[x];
     const double t_ab = t0.x, t_ba = t0.y;
+    // the four Coulomb elements of the element, v(n_p - n_q) for p occupied
+    // and q virtual, from the per-solve virtual-occupied block
+    const double v_ia = S.vso[av * o + i], v_ib = S.vso[bv * o + i];
+    const double v_ja = S.vso[av * o + j], v_jb = S.vso[bv * o + j];
     // drivers (source/CCD.f90:484-501)
-    double gab = VPQ(i, a);
-    double gba = -VPQ(i, b);
+    double gab = v_ia;
+    double gba = -v_ib;
     // mosaics (source/CCD.f90:571-582)
     if (S.do_mosaic) {
       const auto jvv = S.jvv;
@@ -911,17 +914,17 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       const double4 tb = S.tv[(size_t)j + o * i + (size_t)o2 * bv];
       if (S.do_ring) {
         // linear rings (source/CCD.f90:716-763): v(q) rowsum - (T V), a- and b-slice
-        gab += VPQ(i, a) * S.rsum[av * o + i] - ta.x;
+        gab += v_ia * S.rsum[av * o + i] - ta.x;
         gba -= ta.y;
-        gab += VPQ(j, b) * S.rsum[bv * o + j] - tb.x;
+        gab += v_jb * S.rsum[bv * o + j] - tb.x;
         gba -= tb.y;
       }
       if (S.do_xring) {
         // linear crossed rings (source/CCD.f90:845-892): (V T) of the b- and a-slices
         gab -= tb.z;
-        gba += VPQ(j, a) * S.csum[bv * o + i] - tb.w;
+        gba += v_ja * S.csum[bv * o + i] - tb.w;
         gab -= ta.z;
-        gba += VPQ(i, b) * S.csum[av * o + j] - ta.w;
+        gba += v_ib * S.csum[av * o + j] - ta.w;
       }
       const double4 qd = S.quad[x];
       if (S.do_ring) { gab += qd.x; gba += qd.y; }
@@ -937,7 +940,7 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
     dt_part = fmax(fabs(nab - t_ab), fabs(nba - t_ba));
     r_part = fmax(fabs(rab), fabs(rba));
     const double naa = nab + nba;
-    e_part = 0.5 * (VPQ(a, i) * (naa + nab) - VPQ(a, j) * (naa + nba));
+    e_part = 0.5 * (v_ia * (naa + nab) - v_ja * (naa + nba));
     if (!isfinite(nab) || !isfinite(nba) || !isfinite(rab) || !isfinite(rba)) bad = 1.0;
     if (flags & MF_WRITE_GR) {
       if (S.gout) { S.gout[x] = gab; S.gout[n3 + x] = gba; }

@@ -64,7 +64,7 @@ struct SolveDevT {
   Ptr<const double> eig;    // [nao]
   Ptr<const double> vtab;   // [n2max+1] Coulomb table v(|dn|^2)
   Ptr<const double> voo;    // [o*o] v(n_k - n_j) over occupied k, j (k fastest)
-  Ptr<const double> vso;    // [v*o] v(n_s - n_k), slice s, occupied k
+  Ptr<const double> vso;    // [v*o] v(n_s - n_k), slice (virtual) s, occupied k
   Ptr<const float> vtab32;  // same, FP32
   Ptr<const int> pidx;      // [n3]
   Ptr<const int> kb_poff;   // [nK+1]
