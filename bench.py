@@ -33,21 +33,24 @@ METRIC = "CCD solve time & twist-solves/sec at |ΔE|<1e-6 Ha vs FP64; p50/p99 la
 WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
-KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_extrap", 5: "k_finalize"}
+KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_finalize"}
 def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
     """Algorithmic (flops, bytes) of one launch, each datum moved once.
-    k_slice: per slice, TV/VT (8 o^3 flops) and the quadratic J intermediates
-    (12 o^3); reads the T slabs, writes TV/VT and the two J arrays.  k_ladder:
+    k_slice: the pass's amplitudes from 8 history entries of each valid element
+    (4 flops each), written to tcur and to the K-blocked copy; per slice, TV/VT
+    (8 o^3 flops) and the quadratic J intermediates (12 o^3); writes TV/VT and
+    the two J arrays.  k_ladder:
     8 S_pp + 8 S_hh flops (pp ladder, W); reads T, writes the ladder and W.  k_contract reads the J rows (3 doubles, rings
     and crossed rings) and the T columns of every transfer q, writes the
     quadratic sums (2 x 4 flops per element and l), reads the W blocks and T
     and updates the ladder (4 flops per element and pair).  k_main reads T, the
     ladder, two slice products, the quadratic sums and 8 DIIS error entries and
-    writes one T and one error entry; k_extrap reads 8 history entries and
-    writes T; k_finalize reads the 4 + 8 partials of every element block."""
+    writes one T and one error entry; k_finalize reads the 4 + 8 partials of
+    every element block."""
     n3 = o * o * v
     if kernel == "k_slice":
-        return (20.0 * o * n3, 16 * n3 + 32 * n3 + 2 * 24 * n3)
+        return (20.0 * o * n3 + 32.0 * nvalid,
+                (8 * 16 + 16 + 16) * nvalid + 32 * n3 + 2 * 24 * n3)
     if kernel == "k_ladder":
         return (8.0 * spp + 8.0 * shh, 16 * nvalid + 16 * nvalid + 16 * sum_noo2)
     if kernel == "k_contract":
@@ -56,8 +59,6 @@ def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
                 + 32 * nvalid + 8 * o * v)
     if kernel == "k_main":
         return (60.0 * nvalid, (16 + 16 + 64 + 32 + 8 * 16 + 32) * nvalid)
-    if kernel == "k_extrap":
-        return (32.0 * nvalid, (8 * 16 + 16) * nvalid)
     if kernel == "k_finalize":
         nblk = -(-n3 // 256)
         return (12.0 * nblk, 8 * 12 * nblk)

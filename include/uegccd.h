@@ -150,13 +150,13 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
 
 /* Average device time (microseconds, HIP events on the context's stream) of
  * one kernel of the residual pass over the context's current amplitudes:
- * 0 per-slice ring/mosaic products, 1 contractions (quadratic rings per
- * momentum transfer, hole-hole ladder tiles, Joo), 2 particle-particle ladder
- * and W tiles, 3 residual assembly + update, 4 extrapolation.  No counterpart
- * in the reference; measurement support. */
+ * 0 per-slice amplitudes and ring/mosaic products, 1 contractions (quadratic
+ * rings per momentum transfer, hole-hole ladder tiles, Joo), 2
+ * particle-particle ladder and W tiles, 3 residual assembly + update.  No
+ * counterpart in the reference; measurement support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
 
-/* In-solve timing of one kernel (numbering as uegccd_kernel_time, plus 5 for
+/* In-solve timing of one kernel (numbering as uegccd_kernel_time, plus 4 for
  * the per-solve reductions, convergence test and DIIS solve; -1 turns it
  * off): in every pass of the context's later solves, HIP stop events on the
  * context's stream are written by that kernel's dispatch and by the dispatch

@@ -239,7 +239,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     }
   if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
   if (slice_lds_bytes(o) + (int)sizeof(int4) * kMaxNao +
-      (int)sizeof(double) * (kMaxN2 + 16) > kMaxDynLds)
+      (int)sizeof(double) * (kMaxN2 + 16 + kMaxDiis * kMaxDiis + kMaxDiis) > kMaxDynLds)
     return UEGCCD_ERR_UNSUPPORTED;
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
   // occurrence over (p, i), for the quadratic ring products
@@ -568,7 +568,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   // batch of one: element/slice block prefixes and the ladder tiles
   BatchDev& b = c->batch;
   const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
-  const int sb = need_slice ? c->v * slice_split(c->o) : 0;
+  const int sb = need_slice ? c->v : 0;
   const int h_pref[4] = {0, eb, 0, sb};
   int* d_pref = im.reserve<int>(4);
   im.write(d_pref, h_pref, 4);
@@ -579,6 +579,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.elem_blocks = eb;
   b.slice_blocks = sb;
   b.slice_lds = slice_lds_bytes(c->o);
+  b.slice_threads = slice_threads(c->o);
   b.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
   b.lad_items = const_cast<int4*>(im.put(c->lad_items));
   b.ncon = (int)c->con_items.size();
@@ -860,7 +861,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     uegccd_ctx* c = cs[k];
     ep[k + 1] = ep[k] + (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
-    sp[k + 1] = sp[k] + (need_slice ? c->v * slice_split(c->o) : 0);
+    sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
     omax = std::max(omax, c->o);
     if (c->sd.do_ladder)
       for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
@@ -896,6 +897,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.elem_blocks = ep[n];
   b.slice_blocks = sp[n];
   b.slice_lds = slice_lds_bytes(omax);
+  b.slice_threads = slice_threads(omax);
   return upload_image(out->base, image, stream, stage);
 }
 
@@ -1137,7 +1139,7 @@ extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, do
 }
 
 extern "C" int uegccd_time_kernel(uegccd_ctx* c, int32_t which) {
-  if (!c || which < -1 || which > 5) return UEGCCD_ERR_INVALID;
+  if (!c || which < -1 || which > 4) return UEGCCD_ERR_INVALID;
   c->timed_kernel = which;
   c->timed_us = 0.0;
   c->timed_launches = 0;
@@ -1154,7 +1156,7 @@ extern "C" int uegccd_kernel_stats(uegccd_ctx* c, double* total_us, int64_t* lau
 }
 
 extern "C" int uegccd_kernel_time(uegccd_ctx* c, int32_t which, int32_t reps, double* avg_us) {
-  if (!c || !avg_us || reps <= 0 || which < 0 || which > 4) return UEGCCD_ERR_INVALID;
+  if (!c || !avg_us || reps <= 0 || which < 0 || which > 3) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   launch_state_reset(c->batch, PH_ITER, 1, s);

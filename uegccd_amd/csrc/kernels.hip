@@ -1,9 +1,11 @@
 // CDNA4 (gfx950) kernels of the CCD amplitude iteration.
 //
 // One residual pass (TermsWrapper, source/CCD.f90:146-181) plus the update
-// (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is six launches:
-//   k_slice         per virtual slice: linear ring / crossed-ring products,
-//                   quadratic ring intermediates, mosaic partial sums
+// (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is five launches:
+//   k_slice         per virtual slice: the pass's amplitudes (the last update
+//                   or its DIIS extrapolation, source/DIISCC.f90:52-131),
+//                   linear ring / crossed-ring products, quadratic ring
+//                   intermediates, mosaic partial sums
 //                   (source/CCD.f90:507-585, 689-940)
 //   k_ladder        the particle-particle ladder and the hole-hole W
 //                   intermediates as pair-momentum-blocked MFMA tiles
@@ -13,7 +15,6 @@
 //   k_main          residual assembly, R = D*T - G, damped update, dT, energy,
 //                   DIIS dot products
 //   k_finalize      deterministic reductions, convergence test, DIIS solve
-//   k_extrap        amplitudes of the next pass (DIIS extrapolation or update)
 // Every kernel resolves its solve from a block-prefix table, so one launch
 // covers any number of independent solves (twists, r_s values, bases).
 //
@@ -185,7 +186,7 @@ __device__ __forceinline__ GlobalPtr<double> partial(const SolveG& S, int q, int
 
 // --------------------------------------------------------------------------
 // Initial state: E(T0) and max|T0| (source/CCD.f90:62-64); also lays the
-// starting amplitudes out by K block (tk), as k_extrap does for later passes.
+// starting amplitudes out by K block (tk), as k_slice does for later passes.
 __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restrict__ solves,
                                                          const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
@@ -297,17 +298,216 @@ struct MfmaTraits<float> {
 // sums of source/CCD.f90:548-567 are per-slice reductions as well:
 //   Jvv(s) = sum_k P_s(k),  Joo(k) = sum_s P_s(k)  (k_joo),
 //   P_s(k) = sum_l v(n_s-n_k)(T^aa+T^ab)(k,l,s) - v(n_s-n_l)(T^aa+T^ba)(k,l,s).
+// --------------------------------------------------------------------------
+// Small dense solve for the DIIS coefficients (FP64, partial pivoting), by one
+// wave64: lane r owns row r of the bordered system
+//   [ B/scale  -1 ] [c]   [ 0]
+//   [ -1^T      0 ] [l] = [-1]
+// over the nh newest history slots; the rows live in registers (columns
+// unrolled), pivots and row swaps go through cross-lane shuffles.
+// M is the unrolled system size (at least nh + 1).
+template <int M>
+__device__ bool diis_coefficients(const double* bmat, int nh, int slot_cur, int ndiis, double* coef) {
+  const int lane = threadIdx.x & 63;
+  const int m = nh + 1;
+  auto slot_of = [&](int r) { return ((slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis; };
+  const int slr = lane < nh ? slot_of(lane) : 0;
+  double scale = lane < nh ? fabs(bmat[slr * kMaxDiis + slr]) : 0.0;
+  scale = wave_max(scale);
+  if (!(scale > 0.0) || !isfinite(scale)) return false;
+  double row[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    double val = 0.0;
+    if (lane < nh) {
+      if (c < nh) val = bmat[slr * kMaxDiis + slot_of(c)] / scale;
+      else if (c == nh) val = -1.0;
+    } else if (lane == nh && c < nh) {
+      val = -1.0;
+    }
+    row[c] = val;
+  }
+  double rhs = lane == nh ? -1.0 : 0.0;
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < M; ++c) {
+    if (c >= m || !ok) continue;
+    double best = (lane >= c && lane < m) ? fabs(row[c]) : -1.0;
+    int bl = lane;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int ol = __shfl_xor(bl, off, 64);
+      if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
+    }
+    if (!(best >= 1e-14)) { ok = false; continue; }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double vc = __shfl(row[k], c, 64), vp = __shfl(row[k], bl, 64);
+      if (lane == c) row[k] = vp; else if (lane == bl) row[k] = vc;
+    }
+    {
+      const double vc = __shfl(rhs, c, 64), vp = __shfl(rhs, bl, 64);
+      if (lane == c) rhs = vp; else if (lane == bl) rhs = vc;
+    }
+    const double piv = __shfl(row[c], c, 64);
+    const double f = (lane > c && lane < m) ? row[c] / piv : 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double pk = __shfl(row[k], c, 64);
+      if (lane > c && lane < m) row[k] -= f * pk;
+    }
+    const double pr = __shfl(rhs, c, 64);
+    if (lane > c && lane < m) rhs -= f * pr;
+  }
+  if (!ok) return false;
+  double xs = 0.0;
+#pragma unroll
+  for (int r = M - 1; r >= 0; --r) {
+    if (r >= m) continue;
+    double acc = rhs;
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      if (k > r && k < m) acc -= row[k] * __shfl(xs, k, 64);
+    if (lane == r) xs = acc / row[r];
+  }
+  const bool fin = isfinite(xs) || lane >= nh;
+  if (!__all(fin)) return false;
+  if (lane < ndiis) coef[lane] = 0.0;
+  if (lane < nh) coef[slr] = xs;
+  return true;
+}
+
+
+// The same coefficients for histories of up to 8 entries, by one wave:
+// c = B^-1 1 / (1^T B^-1 1) (the solution of the bordered system) with B
+// scaled by its largest diagonal and padded to 8 x 8 with the identity.
+// Lane 8r + c holds B[r][c]; Gauss-Jordan elimination without pivoting (B is
+// symmetric positive definite, so its pivots are those of a Cholesky
+// factorisation) brings B to the identity and 1 to y = B^-1 1, one pivot per
+// step, its row and column broadcast by shuffles.  A pivot below 1e-14 fails
+// the extrapolation (the plain update is used).  Every lane of the wave must
+// call it; the result is the same on all of them.
+__device__ bool diis_gauss_jordan8(const double* bmat, int nh, int slot_cur, int ndiis,
+                                   double* coef) {
+  constexpr int M = 8;
+  const int lane = threadIdx.x & 63, r = lane >> 3, c = lane & 7;
+  // history slot of row q, oldest first
+  auto slot_of = [&](int q) {
+    int sv = slot_cur - (nh - 1) + q;
+    if (sv < 0) sv += ndiis;
+    return sv >= ndiis ? sv - ndiis : sv;
+  };
+  const int sr = slot_of(r), sc = slot_of(c);
+  double scale = (lane < nh) ? fabs(bmat[slot_of(lane) * (kMaxDiis + 1)]) : 0.0;
+  scale = wave_max(scale);
+  if (!(scale > 0.0) || !isfinite(scale)) return false;
+  const double inv_scale = 1.0 / scale;
+  double a = (r < nh && c < nh) ? bmat[sr * kMaxDiis + sc] * inv_scale : (r == c ? 1.0 : 0.0);
+  double b = 1.0;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const double p = __shfl(a, k * M + k, 64);
+    if (!(p > 1e-14)) return false;
+    const double inv_p = 1.0 / p;
+    const double rk = __shfl(a, k * M + c, 64);  // pivot row, this lane's column
+    const double ck = __shfl(a, r * M + k, 64);  // this lane's row, pivot column
+    const double bk = __shfl(b, k * M, 64);
+    if (r == k) {
+      a *= inv_p;
+      b *= inv_p;
+    } else {
+      const double f = ck * inv_p;
+      a -= f * rk;
+      b -= f * bk;
+    }
+  }
+  // y_r on lane 8r; the sum over the filled rows in row order
+  double sum = 0.0;
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const double yq = __shfl(b, q * M, 64);
+    if (q < nh) sum += yq;
+  }
+  if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
+  const double cr = b * (1.0 / sum);
+  if (!__all(isfinite(cr) || r >= nh)) return false;
+  if (lane < ndiis) coef[lane] = 0.0;
+  if (c == 0 && r < nh) coef[sr] = cr;
+  return true;
+}
+
+// A value the same in every lane, as a scalar (so that branches on it are
+// scalar branches).
+__device__ __forceinline__ double uniform(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Per-block LDS of k_slice for the DIIS solve: the B matrix and the
+// coefficients of the slab's amplitudes by history slot.
+struct SliceDiis {
+  double bm[kMaxDiis * kMaxDiis];
+  double coef[kMaxDiis];
+};
+
+// Amplitude of slab entry idx (element xs + idx; valid: it has a partner)
+// for the pass: tcur as it stands (SRC_TCUR), else sum_k coef[k] histT[k] in
+// slot order over the nonzero coefficients (the DIIS extrapolation, or one
+// coefficient 1 for the plain update, which the sum reproduces exactly), from
+// the preloaded history entries hm and coefficients cf of slots below P
+// (later slots are loaded here).
+template <int P>
+__device__ __forceinline__ double2 slab_amplitude(const SolveG& S, int src, int ndiis,
+                                                  const double (&cf)[P], const double2 (&hm)[P],
+                                                  const double* coef, int xs, size_t n3, int idx,
+                                                  bool valid) {
+  if (src == SRC_TCUR) return ld(S.tcur + xs + idx);
+  if (!valid) return make_double2(0.0, 0.0);
+  double ab = 0.0, ba = 0.0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const double c = cf[k];
+    if (c == 0.0) continue;
+    ab += c * hm[k].x;
+    ba += c * hm[k].y;
+  }
+  for (int k = P; k < ndiis; ++k) {
+    const double c = uniform(coef[k]);
+    if (c == 0.0) continue;
+    const double2 hk = ld(S.histT + (size_t)k * n3 + xs + idx);
+    ab += c * hk.x;
+    ba += c * hk.y;
+  }
+  return make_double2(ab, ba);
+}
+
+// Slab entry idx to LDS, and back to tcur and tk when the amplitudes were
+// built (not read from tcur).
+__device__ __forceinline__ void slab_place(const SolveG& S, int src, double* Tab, double* Tba,
+                                           double* Vo, int xs, int idx, int kp, double2 t,
+                                           double v) {
+  Tab[idx] = t.x;
+  Tba[idx] = t.y;
+  Vo[idx] = v;
+  if (src != SRC_TCUR && kp >= 0) {
+    st2(S.tcur + xs + idx, t);
+    st2(S.tk + kp, t);
+  }
+}
+
+// One block of BT threads per virtual slice s (BT = slice_threads of the
+// largest NOcc in the launch): one slab entry and one J output per thread
+// (a larger slab loops over the rest).
+template <int BT>
 __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                            int n, int bid, Lds& L, double* dyn) {
-  // No test of the solve's phase here: a finished solve's slice products are
-  // never read again, and the test would put a dependent load before any work.
+                            int n, int bid, Lds& L, SliceDiis& D, double* dyn) {
   const int s = find_solve(prefix, n, bid);
   const SolveG& S = as_global(solves)[s];
-  // slice_split blocks per slice, each computing a share of the o*o outputs
-  const int split = slice_split(S.o);
-  const int lb = bid - prefix[s], sv = lb / split, part = lb - sv * split;
+  const int sv = bid - prefix[s];
   const int o = S.o, o2 = o * o, so = o + sv;
-  const int out0 = (o2 * part) / split, out1 = (o2 * (part + 1)) / split;
   double* Tab = dyn;
   double* Tba = dyn + o2;
   double* Vo = dyn + 2 * o2;
@@ -315,47 +515,69 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   double* cs = rs + o;
   double* jp = cs + o;
   double* vs = jp + o;  // v(n_s - n_k)
-  // The slab and the occupied Coulomb block V(k, j) go to registers in the
-  // same round of loads as the tables (the first kSlabPerThread * blockDim
-  // entries; the rest, for o > 32, after them), then to LDS.
-  constexpr int kSlabPerThread = 4;
-  const auto tslab = S.tcur + (size_t)o2 * sv;
-  double2 tr[kSlabPerThread];
-  double vr[kSlabPerThread];
+  // The pass's amplitudes on the slab (source/CCD.f90:80-99 with
+  // source/DIISCC.f90:52-131): tcur as it stands after a start, else
+  // sum_k c_k histT[k]: the DIIS extrapolation, whose coefficients wave 0 of
+  // every block solves for (the same inputs and instructions in every block,
+  // so the same coefficients) while the history is on its way, or the last
+  // update histT[slot_cur] alone.  The block writes the slab back to tcur
+  // and to the K-blocked copy tk, which the later kernels of the pass read.
+  // The history entries of the thread's slab entry (up to kPre slots), the
+  // occupied Coulomb block V(k, j), the state and its B matrix go out in the
+  // same round of loads as the tables.
+  constexpr int kPre = 8;
+  const State& st = *S.st;
+  const int phase = st.phase, src = st.next_src, nh = st.nhist, slot_cur = st.slot_cur;
+  const double bw = threadIdx.x < kMaxDiis * kMaxDiis ? st.bmat[threadIdx.x] : 0.0;
+  const int ndiis = S.ndiis;
+  const size_t n3 = S.n3;
+  const int xs = o2 * sv;  // the slab's first element
+  const auto HT = S.histT + xs;
+  const int idx0 = threadIdx.x;
+  double2 h[kPre];
+  const int kp = idx0 < o2 ? S.kpos[xs + idx0] : -1;
+  const double vr = idx0 < o2 ? S.voo[idx0] : 0.0;
 #pragma unroll
-  for (int m = 0; m < kSlabPerThread; ++m) {
-    const int idx = threadIdx.x + m * blockDim.x;
-    tr[m] = idx < o2 ? ld(tslab + idx) : make_double2(0.0, 0.0);
-    vr[m] = idx < o2 ? S.voo[idx] : 0.0;
-  }
+  for (int k = 0; k < kPre; ++k)
+    h[k] = (idx0 < o2 && k < ndiis) ? ld(HT + (size_t)k * n3 + idx0) : make_double2(0.0, 0.0);
   const double vsr = (int)threadIdx.x < o ? S.vso[sv * o + threadIdx.x] : 0.0;
+  if (threadIdx.x < kMaxDiis * kMaxDiis) D.bm[threadIdx.x] = bw;
+  if (threadIdx.x < kMaxDiis) D.coef[threadIdx.x] = (int)threadIdx.x == slot_cur ? 1.0 : 0.0;
   stage(S, L);
+  if (phase == PH_DONE) return;  // the whole block: the solve has finished
+  if (src == SRC_DIIS) {  // a failed solve leaves the plain update
+    if (threadIdx.x < 64) {
+      if (ndiis <= 8)
+        diis_gauss_jordan8(D.bm, nh, slot_cur, ndiis, D.coef);
+      else
+        diis_coefficients<kMaxDiis + 1>(D.bm, nh, slot_cur, ndiis, D.coef);
+    }
+    __syncthreads();
+  }
+  double cf[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) cf[k] = k < ndiis ? uniform(D.coef[k]) : 0.0;
   // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
   // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
   // so the table reads as zero there (the last entry is always past n2max).
-  for (int k = S.n2max + 1 + threadIdx.x; k < kMaxN2; k += blockDim.x) L.vt[k] = 0.0;
+  for (int k = S.n2max + 1 + threadIdx.x; k < kMaxN2; k += BT) L.vt[k] = 0.0;
+  if (idx0 < o2) slab_place(S, src, Tab, Tba, Vo, xs, idx0, kp,
+                            slab_amplitude<kPre>(S, src, ndiis, cf, h, D.coef, xs, n3, idx0, kp >= 0), vr);
+  for (int idx = idx0 + BT; idx < o2; idx += BT) {
+    const int kpm = S.kpos[xs + idx];
+    double2 hm[kPre];
 #pragma unroll
-  for (int m = 0; m < kSlabPerThread; ++m) {
-    const int idx = threadIdx.x + m * blockDim.x;
-    if (idx < o2) {
-      Tab[idx] = tr[m].x;
-      Tba[idx] = tr[m].y;
-      Vo[idx] = vr[m];
-    }
-  }
-  for (int idx = threadIdx.x + kSlabPerThread * blockDim.x; idx < o2; idx += blockDim.x) {
-    const double2 t = ld(tslab + idx);
-    Tab[idx] = t.x;
-    Tba[idx] = t.y;
-    Vo[idx] = S.voo[idx];
+    for (int k = 0; k < kPre; ++k)
+      hm[k] = k < ndiis ? ld(HT + (size_t)k * n3 + idx) : make_double2(0.0, 0.0);
+    const double2 t = slab_amplitude<kPre>(S, src, ndiis, cf, hm, D.coef, xs, n3, idx, kpm >= 0);
+    slab_place(S, src, Tab, Tba, Vo, xs, idx, kpm, t, S.voo[idx]);
   }
   if ((int)threadIdx.x < o) vs[threadIdx.x] = vsr;
   __syncthreads();
-  // Row/column sums (every block needs them for J) by groups of 8 lanes per
-  // row i, each lane a stride-8 share of k, then a 3-step shuffle tree; the
-  // first block of the slice stores them with the mosaic partials
-  // P_s(i) = sum_k v(s-i)(2T^ab + T^ba)(i,k) - v(s-k)(T^ab + 2T^ba)(i,k).
-  for (int i0 = 0; i0 < o; i0 += (int)blockDim.x / 8) {
+  // Row/column sums (for J) by groups of 8 lanes per row i, each lane a
+  // stride-8 share of k, then a 3-step shuffle tree; stored with the mosaic
+  // partials P_s(i) = sum_k v(s-i)(2T^ab + T^ba)(i,k) - v(s-k)(T^ab + 2T^ba)(i,k).
+  for (int i0 = 0; i0 < o; i0 += BT / 8) {
     const int i = i0 + (int)threadIdx.x / 8, sub = threadIdx.x & 7;
     double r = 0.0, c = 0.0, p = 0.0;
     if (i < o) {
@@ -377,23 +599,19 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       rs[i] = r;
       cs[i] = c;
       jp[i] = p;
-      if (part == 0) {
-        S.rsum[sv * o + i] = r;
-        S.csum[sv * o + i] = c;
-        S.joo_part[sv * o + i] = p;
-      }
+      S.rsum[sv * o + i] = r;
+      S.csum[sv * o + i] = c;
+      S.joo_part[sv * o + i] = p;
     }
   }
   // The slice products TV = T V and VT = V T of both spins as 16x16 FP64
-  // MFMA tiles over (r, c); the slice's tiles are dealt over its blocks' waves.
+  // MFMA tiles over (r, c), dealt over the block's waves.
   {
     typedef MfmaTraits<double>::acc_t acc_t;
     const int nt = (o + 15) >> 4, ntiles = nt * nt;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-    const int nw = blockDim.x >> 6;
-    // tile t goes to wave t / split of block part t % split, so every block
-    // of the slice takes a share
-    for (int t = wave * split + part; t < ntiles; t += split * nw) {
+    constexpr int nw = BT / 64;
+    for (int t = wave; t < ntiles; t += nw) {
       const int rt = t / nt, ct = t % nt;
       const int ra = rt * 16 + r16, cb = ct * 16 + r16;
       const int rac = min(ra, o - 1), cbc = min(cb, o - 1);
@@ -419,13 +637,13 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   }
   __syncthreads();
   // Jvv(s) = sum_i P_s(i) by one wave (o <= 64), a fixed tree
-  if (part == 0 && threadIdx.x < 64) {
+  if (threadIdx.x < 64) {
     double jv = (int)threadIdx.x < o ? jp[threadIdx.x] : 0.0;
     jv = wave_sum(jv);
     if (threadIdx.x == 0) S.jvv[sv] = jv;
   }
   const int4 ns = L.nv[so];
-  for (int idx = out0 + threadIdx.x; idx < out1; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < o2; idx += BT) {
     // quadratic intermediates for i = r, l = c
     const int i = idx % o, l = idx / o;
     const int4 ni = L.nv[i], nl = L.nv[l];
@@ -594,11 +812,13 @@ __device__ void ladder_block(const SolveDev* __restrict__ solves, const int4* __
     ladder_tile<float>(S, K, it.z, it.w, vt, cls, red);
 }
 
-__global__ __launch_bounds__(kElemThreads) void k_slice(const SolveDev* __restrict__ solves,
-                                                         const int* __restrict__ prefix, int n) {
+template <int BT>
+__global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solves,
+                                              const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
+  __shared__ SliceDiis D;
   extern __shared__ double dyn[];
-  slice_block(solves, prefix, n, blockIdx.x, L, dyn);
+  slice_block<BT>(solves, prefix, n, blockIdx.x, L, D, dyn);
 }
 
 __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
@@ -962,145 +1182,6 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
     main_partials<32>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, n3, red);
 }
 
-// --------------------------------------------------------------------------
-// Small dense solve for the DIIS coefficients (FP64, partial pivoting), by one
-// wave64: lane r owns row r of the bordered system
-//   [ B/scale  -1 ] [c]   [ 0]
-//   [ -1^T      0 ] [l] = [-1]
-// over the nh newest history slots; the rows live in registers (columns
-// unrolled), pivots and row swaps go through cross-lane shuffles.
-// M is the unrolled system size (at least nh + 1).
-template <int M>
-__device__ bool diis_coefficients(const double* bmat, int nh, int slot_cur, int ndiis, double* coef) {
-  const int lane = threadIdx.x & 63;
-  const int m = nh + 1;
-  auto slot_of = [&](int r) { return ((slot_cur - (nh - 1 - r)) % ndiis + ndiis) % ndiis; };
-  const int slr = lane < nh ? slot_of(lane) : 0;
-  double scale = lane < nh ? fabs(bmat[slr * kMaxDiis + slr]) : 0.0;
-  scale = wave_max(scale);
-  if (!(scale > 0.0) || !isfinite(scale)) return false;
-  double row[M];
-#pragma unroll
-  for (int c = 0; c < M; ++c) {
-    double val = 0.0;
-    if (lane < nh) {
-      if (c < nh) val = bmat[slr * kMaxDiis + slot_of(c)] / scale;
-      else if (c == nh) val = -1.0;
-    } else if (lane == nh && c < nh) {
-      val = -1.0;
-    }
-    row[c] = val;
-  }
-  double rhs = lane == nh ? -1.0 : 0.0;
-  bool ok = true;
-#pragma unroll
-  for (int c = 0; c < M; ++c) {
-    if (c >= m || !ok) continue;
-    double best = (lane >= c && lane < m) ? fabs(row[c]) : -1.0;
-    int bl = lane;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const double ob = __shfl_xor(best, off, 64);
-      const int ol = __shfl_xor(bl, off, 64);
-      if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
-    }
-    if (!(best >= 1e-14)) { ok = false; continue; }
-#pragma unroll
-    for (int k = 0; k < M; ++k) {
-      const double vc = __shfl(row[k], c, 64), vp = __shfl(row[k], bl, 64);
-      if (lane == c) row[k] = vp; else if (lane == bl) row[k] = vc;
-    }
-    {
-      const double vc = __shfl(rhs, c, 64), vp = __shfl(rhs, bl, 64);
-      if (lane == c) rhs = vp; else if (lane == bl) rhs = vc;
-    }
-    const double piv = __shfl(row[c], c, 64);
-    const double f = (lane > c && lane < m) ? row[c] / piv : 0.0;
-#pragma unroll
-    for (int k = 0; k < M; ++k) {
-      const double pk = __shfl(row[k], c, 64);
-      if (lane > c && lane < m) row[k] -= f * pk;
-    }
-    const double pr = __shfl(rhs, c, 64);
-    if (lane > c && lane < m) rhs -= f * pr;
-  }
-  if (!ok) return false;
-  double xs = 0.0;
-#pragma unroll
-  for (int r = M - 1; r >= 0; --r) {
-    if (r >= m) continue;
-    double acc = rhs;
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-      if (k > r && k < m) acc -= row[k] * __shfl(xs, k, 64);
-    if (lane == r) xs = acc / row[r];
-  }
-  const bool fin = isfinite(xs) || lane >= nh;
-  if (!__all(fin)) return false;
-  if (lane < ndiis) coef[lane] = 0.0;
-  if (lane < nh) coef[slr] = xs;
-  return true;
-}
-
-
-// The same coefficients for histories of up to 8 entries, by one wave:
-// c = B^-1 1 / (1^T B^-1 1) (the solution of the bordered system) with B
-// scaled by its largest diagonal and padded to 8 x 8 with the identity.
-// Lane 8r + c holds B[r][c]; Gauss-Jordan elimination without pivoting (B is
-// symmetric positive definite, so its pivots are those of a Cholesky
-// factorisation) brings B to the identity and 1 to y = B^-1 1, one pivot per
-// step, its row and column broadcast by shuffles.  A pivot below 1e-14 fails
-// the extrapolation (the plain update is used).  Every lane of the wave must
-// call it; the result is the same on all of them.
-__device__ bool diis_gauss_jordan8(const double* bmat, int nh, int slot_cur, int ndiis,
-                                   double* coef) {
-  constexpr int M = 8;
-  const int lane = threadIdx.x & 63, r = lane >> 3, c = lane & 7;
-  // history slot of row q, oldest first
-  auto slot_of = [&](int q) {
-    int sv = slot_cur - (nh - 1) + q;
-    if (sv < 0) sv += ndiis;
-    return sv >= ndiis ? sv - ndiis : sv;
-  };
-  const int sr = slot_of(r), sc = slot_of(c);
-  double scale = (lane < nh) ? fabs(bmat[slot_of(lane) * (kMaxDiis + 1)]) : 0.0;
-  scale = wave_max(scale);
-  if (!(scale > 0.0) || !isfinite(scale)) return false;
-  const double inv_scale = 1.0 / scale;
-  double a = (r < nh && c < nh) ? bmat[sr * kMaxDiis + sc] * inv_scale : (r == c ? 1.0 : 0.0);
-  double b = 1.0;
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const double p = __shfl(a, k * M + k, 64);
-    if (!(p > 1e-14)) return false;
-    const double inv_p = 1.0 / p;
-    const double rk = __shfl(a, k * M + c, 64);  // pivot row, this lane's column
-    const double ck = __shfl(a, r * M + k, 64);  // this lane's row, pivot column
-    const double bk = __shfl(b, k * M, 64);
-    if (r == k) {
-      a *= inv_p;
-      b *= inv_p;
-    } else {
-      const double f = ck * inv_p;
-      a -= f * rk;
-      b -= f * bk;
-    }
-  }
-  // y_r on lane 8r; the sum over the filled rows in row order
-  double sum = 0.0;
-#pragma unroll
-  for (int q = 0; q < M; ++q) {
-    const double yq = __shfl(b, q * M, 64);
-    if (q < nh) sum += yq;
-  }
-  if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
-  const double cr = b * (1.0 / sum);
-  if (!__all(isfinite(cr) || r >= nh)) return false;
-  if (lane < ndiis) coef[lane] = 0.0;
-  if (c == 0 && r < nh) coef[sr] = cr;
-  return true;
-}
-
 // Sums and maxima of one solve's element-block partials by one block, in a
 // fixed order.  Slot-major partials: quantity q of every element block is
 // contiguous; q = 1 (max dT) and q = 2 (max |R|) are maxima, E (0), the
@@ -1172,7 +1253,6 @@ __device__ bool advance_state(const SolveG& S, State* st, const double* vals, in
   else
     conv = st->prec64 && !switched && fabs(e - st->ecorr_prev) < S.tol_e && rmax < S.tol_r;
   const bool stop = conv || st->niter >= S.maxiter;
-  st->use_coef = 0;
   bool want = false;
   if (S.mode == UEGCCD_MODE_DIIS) {
     if (switched) {
@@ -1227,7 +1307,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
     State* st = &sst;
     const double e = vals[0], dt = vals[1];
     if (kind == FIN_ITER) {
-      st->use_coef = advance_state(S, st, vals, phase0) ? 1 : 0;
+      st->next_src = advance_state(S, st, vals, phase0) ? SRC_DIIS : SRC_PLAIN;
     } else {
       if (kind == FIN_INIT) {  // a fresh state: nothing carries over from an earlier solve
         st->nonfinite = 0;
@@ -1245,7 +1325,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
         st->slot = 0;
         st->slot_cur = 0;
         st->nhist = 0;
-        st->use_coef = 0;
+        st->next_src = SRC_TCUR;
         st->converged = 0;
         if (S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
         const bool go = S.mode == UEGCCD_MODE_REFERENCE ? (dt >= S.tol_dt) : true;
@@ -1260,78 +1340,6 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
   if (w1 < kStateWords) out[w1] = dst[w1];
 }
 
-// Amplitudes for the next pass: DIIS extrapolation sum_k c_k histT[k] or the
-// plain update histT[slot].  When the state asks for the extrapolation, every
-// block solves the small DIIS system itself (the same inputs and instructions
-// in every block, so the same coefficients) while its elements' history
-// entries, requested first, are on their way.
-__global__ __launch_bounds__(kElemPerBlock) void k_extrap(const SolveDev* __restrict__ solves,
-                                                           const int* __restrict__ prefix, int n) {
-  __shared__ double coef[kMaxDiis];
-  __shared__ double bm[kMaxDiis * kMaxDiis];
-  __shared__ int use;
-  static_assert(kMaxDiis * kMaxDiis == kElemPerBlock, "one B-matrix entry per thread");
-  const int gb = xcd_block(blockIdx.x, gridDim.x);
-  const int s = find_solve(prefix, n, gb);
-  const SolveG& S = as_global(solves)[s];
-  const int lb = gb - prefix[s];
-  const int x = lb * blockDim.x + threadIdx.x;
-  const int kp = x < S.n3 ? S.kpos[x] : -1;  // position in tk; -1 for an invalid element
-  const bool mine = kp >= 0;
-  const size_t n3 = S.n3;
-  const auto HT = S.histT + (mine ? x : 0);
-  constexpr int kPre = 8;  // history entries loaded ahead
-  double2 h[kPre];
-#pragma unroll
-  for (int k = 0; k < kPre; ++k) h[k] = (mine && k < S.ndiis) ? HT[k * n3] : make_double2(0.0, 0.0);
-  // the B matrix goes to LDS in the same round of loads as the history
-  const State& st = *S.st;
-  const double bw = st.bmat[threadIdx.x];
-  const int phase = st.phase, want = st.use_coef, nh = st.nhist, slot_cur = st.slot_cur;
-  bm[threadIdx.x] = bw;
-  __syncthreads();
-  if (phase == PH_DONE) return;  // the whole block
-  if (want && threadIdx.x < 64) {
-    bool ok;
-    if (S.ndiis <= 8) {
-      ok = diis_gauss_jordan8(bm, nh, slot_cur, S.ndiis, coef);
-    } else {
-      ok = diis_coefficients<kMaxDiis + 1>(bm, nh, slot_cur, S.ndiis, coef);
-    }
-    if (threadIdx.x == 0) use = ok ? 1 : 0;
-  }
-  __syncthreads();
-  if (!mine) return;
-  double2 t;
-  if (want && use) {
-    double ab = 0.0, ba = 0.0;
-#pragma unroll
-    for (int k = 0; k < kPre; ++k) {
-      const double c = k < S.ndiis ? coef[k] : 0.0;
-      if (c == 0.0) continue;
-      ab += c * h[k].x;
-      ba += c * h[k].y;
-    }
-    for (int k = kPre; k < S.ndiis; ++k) {
-      const double c = coef[k];
-      if (c == 0.0) continue;
-      const double2 hk = HT[k * n3];
-      ab += c * hk.x;
-      ba += c * hk.y;
-    }
-    t = make_double2(ab, ba);
-  } else {
-    const int sc = slot_cur;
-    t = h[0];
-#pragma unroll
-    for (int k = 1; k < kPre; ++k)
-      if (k == sc) t = h[k];
-    if (sc >= kPre) t = HT[sc * n3];
-  }
-  S.tcur[x] = t;
-  st2(S.tk + kp, t);
-}
-
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {
   const SolveG& S = as_global(solves)[blockIdx.x];
   if (threadIdx.x != 0) return;
@@ -1341,7 +1349,7 @@ __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, in
   st->slot = 0;
   st->slot_cur = 0;
   st->nhist = 0;
-  st->use_coef = 0;
+  st->next_src = SRC_TCUR;
   st->prec64 = prec64;
   st->iters_fp32 = 0;
   st->nonfinite = 0;
@@ -1562,8 +1570,11 @@ int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const do
 static void allow_large_lds() {
   static bool done = false;
   if (done) return;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_slice),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds - (int)sizeof(Lds));
+  const void* fs[] = {reinterpret_cast<const void*>(&k_slice<256>), reinterpret_cast<const void*>(&k_slice<512>),
+                      reinterpret_cast<const void*>(&k_slice<768>)};
+  for (const void* f : fs)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kMaxDynLds - (int)sizeof(Lds) - (int)sizeof(SliceDiis));
   done = true;
 }
 
@@ -1581,8 +1592,14 @@ static void launch_k(F kernel, hipEvent_t stop, dim3 grid, dim3 block, uint32_t 
 
 static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   allow_large_lds();
-  launch_k(k_slice, stop, dim3(b.slice_blocks), dim3(kElemThreads), b.slice_lds, s,
-           (const SolveDev*)b.solves, (const int*)b.slice_prefix, b.n);
+  const dim3 g(b.slice_blocks), t(b.slice_threads);
+  const auto* sd = (const SolveDev*)b.solves;
+  const auto* pf = (const int*)b.slice_prefix;
+  switch (b.slice_threads) {
+    case 256: launch_k(k_slice<256>, stop, g, t, b.slice_lds, s, sd, pf, b.n); break;
+    case 512: launch_k(k_slice<512>, stop, g, t, b.slice_lds, s, sd, pf, b.n); break;
+    default: launch_k(k_slice<768>, stop, g, t, b.slice_lds, s, sd, pf, b.n); break;
+  }
 }
 
 static void launch_ladder(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
@@ -1622,15 +1639,12 @@ static int pass_kernels(const BatchDev& b, int main_flags, int* order) {
   if (b.nlad > 0) order[n++] = 2;
   if (b.ncon > 0) order[n++] = 1;
   order[n++] = 3;
-  if (main_flags & MF_ITERATE) {
-    order[n++] = 5;
-    order[n++] = 4;
-  }
+  if (main_flags & MF_ITERATE) order[n++] = 4;
   return n;
 }
 
 int first_pass_kernel(const BatchDev& b, int main_flags) {
-  int order[6];
+  int order[5];
   pass_kernels(b, main_flags, order);
   return order[0];
 }
@@ -1650,10 +1664,6 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
       hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
                          b.elem_prefix, b.n, (int)MF_ITERATE);
       break;
-    case 4:
-      hipLaunchKernelGGL(k_extrap, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
-                         b.elem_prefix, b.n);
-      break;
     default:
       break;
   }
@@ -1661,9 +1671,9 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
 
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const KernelTimer* timer,
                       hipEvent_t tail_stop) {
-  int order[6];
+  int order[5];
   const int n = pass_kernels(b, main_flags, order);
-  hipEvent_t stop[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t stop[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < n && timer; ++i)
     if (order[i] == timer->which) {
       stop[i] = timer->after;
@@ -1679,13 +1689,9 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
         launch_k(k_main, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
                  (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
         break;
-      case 5:
+      case 4:
         launch_k(k_finalize, stop[i], dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,
                  (int)FIN_ITER, 1);
-        break;
-      case 4:
-        launch_k(k_extrap, stop[i], dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s,
-                 (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n);
         break;
     }
   }

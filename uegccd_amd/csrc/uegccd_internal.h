@@ -27,6 +27,13 @@ constexpr int kMaxOcc = 64;                   // NOcc staged per slice in LDS
 enum Phase : int { PH_ITER = 0, PH_FINAL = 1, PH_DONE = 2 };
 enum FinKind : int { FIN_ITER = 0, FIN_INIT = 1, FIN_SUM = 2 };
 enum MainFlags : int { MF_ITERATE = 1, MF_WRITE_GR = 2 };
+// Where the amplitudes of the next residual pass come from (read by k_slice,
+// which builds each slab of them and writes it back to tcur and tk).
+enum AmpSource : int {
+  SRC_TCUR = 0,   // tcur as it stands (the starting amplitudes)
+  SRC_PLAIN = 1,  // the last update histT[slot_cur]
+  SRC_DIIS = 2    // the DIIS extrapolation sum_k c_k histT[k] (k_slice solves for c)
+};
 
 struct State {
   int niter;        // NIter as counted at source/CCD.f90:62,445
@@ -34,7 +41,7 @@ struct State {
   int slot;         // history slot the next residual pass writes
   int slot_cur;     // slot written by the last completed pass
   int nhist;        // filled DIIS history entries
-  int use_coef;     // 1: next amplitudes extrapolated (k_extrap solves for the coefficients)
+  int next_src;     // AmpSource of the next pass's amplitudes
   int prec64;       // 1: FP64 ladders, 0: FP32 ladders
   int iters_fp32;
   int nonfinite;
@@ -87,7 +94,8 @@ struct SolveDevT {
   Ptr<const int4> qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
   Ptr<const int> qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
   // mutable buffers
-  Ptr<double2> tcur;        // [n3] (abab, abba) amplitudes entering the next pass
+  Ptr<double2> tcur;        // [n3] (abab, abba) amplitudes of the current pass (zero where
+                            // invalid); k_slice writes them from the history
   Ptr<double2> tk;          // [nvalid] the same amplitudes by K block: entry ci of block K's
                             // virtual list, pair p at kb_toff[K] + ci*noo + p
   Ptr<double2> histT;       // [ndiis][n3] updated amplitudes, slot-major (coalesced per slot)
@@ -120,11 +128,12 @@ struct BatchDev {
   int n;
   SolveDev* solves;     // device array [n]
   int* elem_prefix;     // [n+1] 256-thread blocks over each solve's n3
-  int* slice_prefix;    // [n+1] one block per virtual slice
+  int* slice_prefix;    // [n+1] one k_slice block per virtual slice
   int4* lad_items;      // (solve, K, row0, col0) ladder tiles
   int nlad;
   int elem_blocks, slice_blocks;
   int slice_lds;        // dynamic LDS bytes of k_slice (largest NOcc in the batch)
+  int slice_threads;    // threads of the k_slice blocks (slice_threads of the largest NOcc)
   int4* con_items;      // (solve, kind, arg, arg) work items of k_contract, one per wave
   int ncon;
 };
@@ -137,17 +146,19 @@ constexpr int kJooParts = 32;  // slice ranges the mosaic Joo sum is split over 
 constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
 inline int slice_lds_bytes(int o) { return (3 * o * o + 4 * o) * 8; }
-// Blocks per virtual slice in k_slice: one per 256 of the o*o outputs, at most 4.
-__host__ __device__ inline int slice_split(int o) {
+// Threads of a k_slice block (one block per virtual slice, one slab entry
+// and J output per thread): 256 per 256 of the o*o entries, at most 768
+// (a larger slab is looped over).
+inline int slice_threads(int o) {
   const int g = (o * o + 255) / 256;
-  return g < 1 ? 1 : (g > 4 ? 4 : g);
+  return 256 * (g < 1 ? 1 : (g > 3 ? 3 : g));
 }
 constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 
 // Kernel launchers (kernels.hip).
 void launch_init(const BatchDev& b, hipStream_t s);
 void launch_mp2(const BatchDev& b, hipStream_t s);
-// Events timing one kernel of a pass (which: as launch_one, or 5 for the
+// Events timing one kernel of a pass (which: as launch_one, or 4 for the
 // reductions and DIIS solve): `after` is written when it completes, `before`
 // when the kernel launched just before it completes.  When `which` is the
 // first kernel of the pass, `before` belongs to the previous pass: pass that
@@ -163,7 +174,7 @@ int first_pass_kernel(const BatchDev& b, int main_flags);
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64 = 1);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
-// One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main, 4 extrapolation.
+// One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main.
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
 // MP2 connectivity histogram of a batch-of-one solve (nbins <= 4096 bins,
