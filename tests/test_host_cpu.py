@@ -74,19 +74,30 @@ def test_no_device_fails_loudly():
 
 def test_footprint_is_host_only_and_grows_with_history():
     """uegccd_footprint plans a context without a device: it covers at least
-    the amplitudes, the DIIS history (T and error vectors), the residual
-    outputs and the ring intermediates, and grows by exactly two amplitude
-    vectors (rounded to 256 B) per history entry."""
+    the amplitudes, the DIIS history (T and error vectors of the valid
+    elements only), the residual outputs and the ring intermediates; the
+    history grows by two vectors of the valid amplitudes per entry (each
+    history array rounded to 256 B).  The valid elements (those with a
+    virtual momentum partner) are counted with the oracle's FindIndex."""
     from uegccd_amd.driver import footprint, init_heg, options
     info = info_for("n54_cut9")
     heg = init_heg(info)
     o, v = heg.NOcc, heg.NAO - heg.NOcc
     n3 = o * o * v
+    orc = oracle_for(info)
+    assert (orc.nocc, orc.nvir) == (o, v)
+    nvalid = sum(1 for a in range(o + 1, o + v + 1) for j in range(1, o + 1)
+                 for i in range(1, o + 1) if orc.findindex(i, j, a) > o)
+    assert 0 < nvalid < n3
     b4 = footprint(heg, info, options(mode="diis", ndiis=4))
     b8 = footprint(heg, info, options(mode="diis", ndiis=8))
     vec = (16 * n3 + 255) // 256 * 256
-    assert b8 - b4 == 2 * 4 * vec
-    assert b8 >= vec * (1 + 2 * 8 + 2) + 2 * 32 * o * v * o
+
+    def hist(nd):
+        return 2 * ((16 * nvalid * nd + 255) // 256 * 256)
+
+    assert b8 - b4 == hist(8) - hist(4)
+    assert b8 >= vec * 3 + hist(8) + 2 * 32 * o * v * o
     assert b8 < 4 * b4
     with pytest.raises(L.UegccdError):
         footprint(heg, info, options(mode="diis", ndiis=17))

@@ -87,8 +87,10 @@ struct uegccd_ctx {
   int64_t nvalid = 0;
   double tolmax = 1e-8, failratio = 100, df = 2.0;
   // host copies of the maps
-  std::vector<int> pidx, kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
-  std::vector<int> kb_toff, kpos;  // K-blocked amplitude layout: block offsets, element -> position
+  std::vector<int> kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
+  std::vector<int> kb_toff;        // K-blocked amplitude layout: block offsets
+  std::vector<int> slab_coff;        // history layout: first compact index of each virtual slice
+  std::vector<uint2> cmap;           // compact index -> elem_map(element, K-blocked position, partner)
   std::vector<int4> nv4, kb_nva;
   std::vector<double> eig, vtab, voo, vso;  // Coulomb table; v(n_k - n_j) over occupied k, j; v(n_s - n_k)
   std::vector<int4> lad_items;  // (0, K, row0, col0)
@@ -148,21 +150,6 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     if (slot >= 0) return UEGCCD_ERR_INVALID;  // duplicate plane wave
     slot = p;
   }
-  // partner map: the FindIndex(i,j,a) > NOcc test of every loop in source/CCD.f90
-  c->pidx.assign(c->n3, -1);
-  c->nvalid = 0;
-  for (int av = 0; av < v; ++av)
-    for (int j = 0; j < o; ++j) {
-      const int base = center + lin[j] - lin[o + av];
-      int* row = c->pidx.data() + (size_t)o * j + (size_t)o * o * av;
-      for (int i = 0; i < o; ++i) {
-        const int b = box[base + lin[i]];
-        if (b >= o) {
-          row[i] = b;
-          ++c->nvalid;
-        }
-      }
-    }
   // pair-momentum blocks, in order of first occurrence of K over x = i + o*j
   // (block ids looked up by the box index of K)
   std::vector<int> kid((size_t)w * w * w, -1);
@@ -188,12 +175,20 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   c->kb_va.clear();
   c->kb_vb.clear();
   c->kb_nva.clear();
+  // The partner test of every loop in source/CCD.f90, FindIndex(i,j,a) > NOcc:
+  // b = n_i + n_j - n_a depends on (i,j) only through K = n_i + n_j, so a
+  // block's virtual list holds the a whose partner K - n_a is a virtual
+  // orbital, and element (i,j,a) is valid exactly when a is on the list of
+  // the block of (i,j).  civ[K*v + av]: entry of a in block K's list, or -1.
+  std::vector<int> civ((size_t)c->nK * v, -1);
   for (int K = 0; K < c->nK; ++K) {
     const int x0 = pairs[K][0];
+    const int kbase = center + lin[x0 % o] + lin[x0 / o];
     for (int x : pairs[K]) c->kb_pairs.push_back(x);
     for (int av = 0; av < v; ++av) {
-      int b = c->pidx[x0 + o * o * av];
-      if (b >= 0) {
+      const int b = box[kbase - lin[o + av]];
+      if (b >= o) {
+        civ[(size_t)K * v + av] = (int)c->kb_va.size() - c->kb_voff[K];
         c->kb_va.push_back(o + av);
         c->kb_vb.push_back(b);
         const int4 na = c->nv4[o + av];
@@ -207,17 +202,36 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   }
   c->wtot = c->kb_woff[c->nK];
   // K-blocked layout of the valid amplitudes: block K holds, for each entry ci
-  // of its virtual list, its noo pairs contiguously (position toff + ci*noo + p)
+  // of its virtual list, its noo pairs contiguously (position toff + ci*noo + p);
+  // cmap gives each compact index its element, this position and its partner
+  // The valid elements in element order (their compact index there is the
+  // layout of the DIIS history), with the count before each virtual slice.
   c->kb_toff.assign(c->nK + 1, 0);
-  c->kpos.assign((size_t)c->n3, -1);
-  for (int K = 0; K < c->nK; ++K) {
-    const int p0 = c->kb_poff[K], noo = c->kb_poff[K + 1] - p0;
-    const int v0 = c->kb_voff[K], nvv = c->kb_voff[K + 1] - v0;
-    for (int ci = 0; ci < nvv; ++ci)
-      for (int p = 0; p < noo; ++p)
-        c->kpos[(size_t)c->kb_pairs[p0 + p] + (size_t)o * o * (c->kb_va[v0 + ci] - o)] =
-            c->kb_toff[K] + ci * noo + p;
-    c->kb_toff[K + 1] = c->kb_toff[K] + nvv * noo;
+  for (int K = 0; K < c->nK; ++K)
+    c->kb_toff[K + 1] = c->kb_toff[K] + (c->kb_voff[K + 1] - c->kb_voff[K]) * (c->kb_poff[K + 1] - c->kb_poff[K]);
+  c->nvalid = c->kb_toff[c->nK];
+  c->cmap.resize((size_t)c->nvalid);
+  c->slab_coff.assign((size_t)v + 1, 0);
+  {
+    // per pair: its block, the block's pair count and the pair's first tk position
+    std::vector<int> pk(o * o), pn(o * o), pbase(o * o);
+    for (int pr = 0; pr < o * o; ++pr) {
+      const int K = c->pair_kb[pr];
+      pk[pr] = K;
+      pn[pr] = c->kb_poff[K + 1] - c->kb_poff[K];
+      pbase[pr] = c->kb_toff[K] + c->pair_pos[pr];
+    }
+    uint2* cm = c->cmap.data();
+    int nc = 0;
+    for (int av = 0, x = 0; av < v; ++av) {
+      c->slab_coff[av] = nc;
+      for (int pr = 0; pr < o * o; ++pr, ++x) {
+        const int K = pk[pr], ci = civ[(size_t)K * v + av];
+        if (ci < 0) continue;
+        cm[nc++] = elem_map(x, pbase[pr] + ci * pn[pr], c->kb_vb[c->kb_voff[K] + ci]);
+      }
+    }
+    c->slab_coff[v] = nc;
   }
   // ladder tiles: 16 pair rows x 16 columns (a list then pair list)
   c->lad_items.clear();
@@ -335,23 +349,31 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   return UEGCCD_OK;
 }
 
+// Element-wise blocks of a context: one thread per valid element (at least
+// one block, which writes the partials of an empty sum).
+int elem_blocks_of(const uegccd_ctx* c) {
+  return (int)std::max<int64_t>(1, (c->nvalid + kElemPerBlock - 1) / kElemPerBlock);
+}
+
 // Device bytes of a context: the arena ctx_create carves its buffers from.
 size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   const size_t n3 = c->n3;
-  const size_t eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
+  const size_t eb = elem_blocks_of(c);
   const size_t ovo = (size_t)c->o * c->v * c->o;
   size_t bytes = 0;
   bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
   bytes += aligned(sizeof(double) * (c->n2max + 1)) + aligned(sizeof(float) * (c->n2max + 1));
   bytes += aligned(sizeof(double) * c->o * c->o) + aligned(sizeof(double) * c->v * c->o);  // voo, vso
-  bytes += aligned(sizeof(int) * n3);
   bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
   bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->kb_nva.size()));
   bytes += 2 * aligned(sizeof(int) * c->o * c->o);
-  bytes += aligned(sizeof(double2) * n3) * (1 + 2 * ndiis + 2);      // tcur, histT, histE, gout, rout
+  bytes += aligned(sizeof(double2) * n3) * 3;                         // tcur, gout, rout
+  bytes += 2 * aligned(sizeof(double2) * std::max<size_t>(1, (size_t)c->nvalid * ndiis));  // histT, histE
+  bytes += aligned(sizeof(int) * (c->v + 1));                          // slab_coff
+  bytes += aligned(sizeof(uint2) * std::max<size_t>(1, c->cmap.size()));   // cmap
   bytes += aligned(sizeof(double2) * std::max<size_t>(1, c->kb_toff.back()));  // tk
-  bytes += aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * n3);  // kb_toff, kpos
+  bytes += aligned(sizeof(int) * (c->nK + 1));                         // kb_toff
   bytes += aligned(sizeof(double2) * n3);                              // lad
   bytes += aligned(sizeof(double2) * std::max(1, c->wtot));            // w
   bytes += 2 * aligned(sizeof(double) * 3 * std::max<size_t>(1, ovo));  // jring, jxring
@@ -417,11 +439,11 @@ struct ArenaImage {
 // with no virtual momentum partner are zeroed.
 int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
   std::vector<double> h(2 * (size_t)c->n3, 0.0);
-  for (int x = 0; x < c->n3; ++x)
-    if (c->pidx[x] >= 0) {
-      h[2 * (size_t)x] = tab ? tab[x] : 0.0;
-      h[2 * (size_t)x + 1] = tba ? tba[x] : 0.0;
-    }
+  for (const uint2& m : c->cmap) {
+    const size_t x = (size_t)m.x;
+    h[2 * x] = tab ? tab[x] : 0.0;
+    h[2 * x + 1] = tba ? tba[x] : 0.0;
+  }
   HIPCHK(hipMemcpyAsync(c->sd.tcur, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -515,12 +537,13 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   const uegccd_options& opt = c->opt;
   const int ndiis = ndiis_of(opt);
   const size_t n3 = c->n3;
-  const int eb = (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
+  const int eb = elem_blocks_of(c);
   const size_t ovo = (size_t)c->o * c->v * c->o;
   c->arena.base = base;
   c->arena.size = bytes;
   c->arena.used = 0;
   image.clear();
+  image.reserve(bytes);  // one allocation: the constant section is a prefix of the arena
   ArenaImage im{&c->arena, &image};
 
   SolveDev& S = c->sd;
@@ -545,7 +568,6 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.voo = im.put(c->voo);
   S.vso = im.put(c->vso);
   S.vtab32 = im.put(vt32);
-  S.pidx = im.put(c->pidx);
   S.kb_poff = im.put(c->kb_poff);
   S.kb_pairs = im.put(c->kb_pairs);
   S.kb_voff = im.put(c->kb_voff);
@@ -554,7 +576,9 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.kb_nva = im.put(c->kb_nva);
   S.kb_woff = im.put(c->kb_woff);
   S.kb_toff = im.put(c->kb_toff);
-  S.kpos = im.put(c->kpos);
+  S.slab_coff = im.put(c->slab_coff);
+  S.cmap = im.put(c->cmap);
+  S.nvalid = (int)c->cmap.size();
   S.pair_kb = im.put(c->pair_kb);
   S.pair_pos = im.put(c->pair_pos);
   S.nqb = (int)c->qb_neg.size();
@@ -589,8 +613,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   *work_off = c->arena.used;
   S.tcur = c->arena.take<double2>(n3);
   S.tk = c->arena.take<double2>(std::max<int64_t>(1, c->kb_toff[c->nK]));
-  S.histT = c->arena.take<double2>(n3 * ndiis);
-  S.histE = c->arena.take<double2>(n3 * ndiis);
+  S.histT = c->arena.take<double2>(std::max<size_t>(1, (size_t)c->nvalid * ndiis));
+  S.histE = c->arena.take<double2>(std::max<size_t>(1, (size_t)c->nvalid * ndiis));
   S.gout = c->arena.take<double>(2 * n3);
   S.rout = c->arena.take<double>(2 * n3);
   S.lad = c->arena.take<double2>(n3);
@@ -710,12 +734,15 @@ int uegccd_residual(uegccd_ctx* c, const double* tab, const double* tba, double*
   HIPCHK(hipMemcpyAsync(h.data() + 2 * n3, c->sd.rout, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t x = 0; x < n3; ++x) {
-    const bool ok = c->pidx[x] >= 0;
-    if (gab) gab[x] = ok ? h[x] : 0.0;
-    if (gba) gba[x] = ok ? h[n3 + x] : 0.0;
-    if (rab) rab[x] = ok ? h[2 * n3 + x] : 0.0;
-    if (rba) rba[x] = ok ? h[3 * n3 + x] : 0.0;
+  // the valid elements' values; zeros elsewhere
+  for (double* out : {gab, gba, rab, rba})
+    if (out) std::fill(out, out + n3, 0.0);
+  for (const uint2& m : c->cmap) {
+    const size_t x = (size_t)m.x;
+    if (gab) gab[x] = h[x];
+    if (gba) gba[x] = h[n3 + x];
+    if (rab) rab[x] = h[2 * n3 + x];
+    if (rba) rba[x] = h[3 * n3 + x];
   }
   return UEGCCD_OK;
 }
@@ -859,7 +886,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   int omax = 1;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
-    ep[k + 1] = ep[k] + (c->n3 + kElemPerBlock - 1) / kElemPerBlock;
+    ep[k + 1] = ep[k] + elem_blocks_of(c);
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
     sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
     omax = std::max(omax, c->o);
@@ -1051,9 +1078,12 @@ int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
   HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t x = 0; x < n3; ++x) {
-    const bool ok = c->pidx[x] >= 0;
-    const double a = ok ? h[2 * x] : 0.0, b = ok ? h[2 * x + 1] : 0.0;
+  // the valid elements' amplitudes; zeros elsewhere
+  for (double* out : {taa, tab, tba})
+    if (out) std::fill(out, out + n3, 0.0);
+  for (const uint2& m : c->cmap) {
+    const size_t x = (size_t)m.x;
+    const double a = h[2 * x], b = h[2 * x + 1];
     if (tab) tab[x] = a;
     if (tba) tba[x] = b;
     if (taa) taa[x] = a + b;
@@ -1121,7 +1151,7 @@ extern "C" int uegccd_connectivity(uegccd_ctx* c, int32_t* counts, int32_t nbins
   std::vector<int> h(nbins + 1, 0);
   int rc = UEGCCD_OK;
   if (hipMemsetAsync(d, 0, sizeof(int) * (nbins + 1), c->stream) != hipSuccess ||
-      connectivity_on_device(c->batch, c->n3, nbins, d, d + nbins, c->stream) ||
+      connectivity_on_device(c->batch, (int)c->nvalid, nbins, d, d + nbins, c->stream) ||
       hipMemcpyAsync(h.data(), d, sizeof(int) * (nbins + 1), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     rc = UEGCCD_ERR_HIP;

@@ -22,7 +22,8 @@
 //   tcur, lad, histT/histE entries   double2  (abab, abba) or (S, P) per (i,j,a)
 //   tv                               double4  (TV_ab, TV_ba, VT_ab, VT_ba)
 //   jring / jxring                   3 doubles (J1, J2, J3) per (row of P(q), l)
-//   histT / histE                    slot-major: slot k of x at k*n3 + x
+//   histT / histE                    valid elements only, in element order (compact
+//                                    index cpos[x]), slot-major: k*nvalid + cpos[x]
 #include <hip/hip_ext.h>
 
 #include "uegccd_internal.h"
@@ -48,6 +49,7 @@ __device__ __forceinline__ const SolveG* as_global(const SolveDev* p) {
 __device__ __forceinline__ double2 ld(GlobalPtr<const double2> p) { return *p; }
 __device__ __forceinline__ double4 ld(GlobalPtr<const double4> p) { return *p; }
 __device__ __forceinline__ int4 ld(GlobalPtr<const int4> p) { return *p; }
+__device__ __forceinline__ uint2 ld(GlobalPtr<const uint2> p) { return *p; }
 __device__ __forceinline__ void st2(GlobalPtr<double2> p, double2 v) { *p = v; }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -196,19 +198,18 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
   const int lb = gb - prefix[s];
   stage(S, L);
   const int o = S.o, o2 = o * o;
-  const int x = lb * blockDim.x + threadIdx.x;
+  const int c = lb * blockDim.x + threadIdx.x;  // compact index of a valid element
   double e = 0.0, tmax = 0.0, bad = 0.0;
-  if (x < S.n3) {
-    const int b = S.pidx[x];
-    if (b >= 0) {
-      const int i = x % o, j = (x / o) % o, a = o + x / o2;
-      const double2 t = S.tcur[x];
-      st2(S.tk + S.kpos[x], t);
-      const double taa = t.x + t.y;
-      e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
-      tmax = fmax(fabs(t.x), fabs(t.y));
-      if (!isfinite(t.x) || !isfinite(t.y)) bad = 1.0;
-    }
+  if (c < S.nvalid) {
+    const uint2 cm = ld(S.cmap + c);
+    const int x = (int)cm.x;
+    const int i = x % o, j = (x / o) % o, a = o + x / o2;
+    const double2 t = S.tcur[x];
+    st2(S.tk + elem_pos(cm), t);
+    const double taa = t.x + t.y;
+    e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
+    tmax = fmax(fabs(t.x), fabs(t.y));
+    if (!isfinite(t.x) || !isfinite(t.y)) bad = 1.0;
   }
   e = block_sum(e, L.red);
   tmax = block_max(tmax, L.red);
@@ -231,20 +232,17 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
   const int lb = gb - prefix[s];
   stage(S, L);
   const int o = S.o, o2 = o * o;
-  const int x = lb * blockDim.x + threadIdx.x;
+  const int c = lb * blockDim.x + threadIdx.x;  // compact index of a valid element
   double e = 0.0;
-  if (x < S.n3) {
-    const int b = S.pidx[x];
-    double tab = 0.0, tba = 0.0;
-    if (b >= 0) {
-      const int i = x % o, j = (x / o) % o, a = o + x / o2;
-      const double vab = VPQ(i, a), vba = VPQ(i, b);
-      const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
-      tab = vab / den;
-      tba = -vba / den;
-      const double taa = tab + tba;
-      e = 0.5 * taa * (vab - vba) + 0.5 * tab * vab - 0.5 * tba * vba;
-    }
+  if (c < S.nvalid) {
+    const uint2 cm = ld(S.cmap + c);
+    const int x = (int)cm.x, b = elem_partner(cm);
+    const int i = x % o, j = (x / o) % o, a = o + x / o2;
+    const double vab = VPQ(i, a), vba = VPQ(i, b);
+    const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
+    const double tab = vab / den, tba = -vba / den;
+    const double taa = tab + tba;
+    e = 0.5 * taa * (vab - vba) + 0.5 * tab * vab - 0.5 * tba * vba;
     S.tcur[x] = make_double2(tab, tba);
   }
   e = block_sum(e, L.red);
@@ -453,19 +451,15 @@ struct SliceDiis {
   double coef[kMaxDiis];
 };
 
-// Amplitude of slab entry idx (element xs + idx; valid: it has a partner)
-// for the pass: tcur as it stands (SRC_TCUR), else sum_k coef[k] histT[k] in
+// Amplitude of one valid element for the pass, sum_k coef[k] histT[k] in
 // slot order over the nonzero coefficients (the DIIS extrapolation, or one
 // coefficient 1 for the plain update, which the sum reproduces exactly), from
-// the preloaded history entries hm and coefficients cf of slots below P
-// (later slots are loaded here).
+// its preloaded history entries hm and the coefficients cf of slots below P;
+// later slots are loaded here from HC (the element's entry of slot 0).
 template <int P>
-__device__ __forceinline__ double2 slab_amplitude(const SolveG& S, int src, int ndiis,
-                                                  const double (&cf)[P], const double2 (&hm)[P],
-                                                  const double* coef, int xs, size_t n3, int idx,
-                                                  bool valid) {
-  if (src == SRC_TCUR) return ld(S.tcur + xs + idx);
-  if (!valid) return make_double2(0.0, 0.0);
+__device__ __forceinline__ double2 combine_history(const double (&cf)[P], const double2 (&hm)[P],
+                                                   const double* coef, int ndiis,
+                                                   GlobalPtr<const double2> HC, size_t nval) {
   double ab = 0.0, ba = 0.0;
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -477,25 +471,22 @@ __device__ __forceinline__ double2 slab_amplitude(const SolveG& S, int src, int 
   for (int k = P; k < ndiis; ++k) {
     const double c = uniform(coef[k]);
     if (c == 0.0) continue;
-    const double2 hk = ld(S.histT + (size_t)k * n3 + xs + idx);
+    const double2 hk = ld(HC + (size_t)k * nval);
     ab += c * hk.x;
     ba += c * hk.y;
   }
   return make_double2(ab, ba);
 }
 
-// Slab entry idx to LDS, and back to tcur and tk when the amplitudes were
-// built (not read from tcur).
-__device__ __forceinline__ void slab_place(const SolveG& S, int src, double* Tab, double* Tba,
-                                           double* Vo, int xs, int idx, int kp, double2 t,
-                                           double v) {
-  Tab[idx] = t.x;
-  Tba[idx] = t.y;
-  Vo[idx] = v;
-  if (src != SRC_TCUR && kp >= 0) {
-    st2(S.tcur + xs + idx, t);
-    st2(S.tk + kp, t);
-  }
+// A valid element's amplitude to its slab entry in LDS, to tcur and to tk;
+// cm = elem_map(element, position in tk, partner), xs the slab's first element.
+__device__ __forceinline__ void slab_place(const SolveG& S, double* Tab, double* Tba, int xs, uint2 cm,
+                                           double2 t) {
+  const int x = (int)cm.x;
+  Tab[x - xs] = t.x;
+  Tba[x - xs] = t.y;
+  st2(S.tcur + x, t);
+  st2(S.tk + elem_pos(cm), t);
 }
 
 // One block of BT threads per virtual slice s (BT = slice_threads of the
@@ -520,33 +511,40 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   // sum_k c_k histT[k]: the DIIS extrapolation, whose coefficients wave 0 of
   // every block solves for (the same inputs and instructions in every block,
   // so the same coefficients) while the history is on its way, or the last
-  // update histT[slot_cur] alone.  The block writes the slab back to tcur
-  // and to the K-blocked copy tk, which the later kernels of the pass read.
-  // The history entries of the thread's slab entry (up to kPre slots), the
-  // occupied Coulomb block V(k, j), the state and its B matrix go out in the
-  // same round of loads as the tables.
+  // update histT[slot_cur] alone.  The slab's valid elements are one
+  // contiguous run of the history (thread t takes the t-th); the block writes
+  // them back to tcur and to the K-blocked copy tk, which the later kernels
+  // of the pass read, and zeros stand at the slab's invalid entries.  The
+  // history (up to kPre slots), the occupied Coulomb block V(k, j), the state
+  // and its B matrix go out in the same round of loads as the tables.
   constexpr int kPre = 8;
   const State& st = *S.st;
   const int phase = st.phase, src = st.next_src, nh = st.nhist, slot_cur = st.slot_cur;
   const double bw = threadIdx.x < kMaxDiis * kMaxDiis ? st.bmat[threadIdx.x] : 0.0;
   const int ndiis = S.ndiis;
-  const size_t n3 = S.n3;
+  const size_t nval = S.nvalid;
   const int xs = o2 * sv;  // the slab's first element
-  const auto HT = S.histT + xs;
-  const int idx0 = threadIdx.x;
+  const int c0 = S.slab_coff[sv], nvs = S.slab_coff[sv + 1] - c0;
+  const int t0 = threadIdx.x;
+  const auto HC = S.histT + c0 + t0;
+  const uint2 cm = t0 < nvs ? ld(S.cmap + c0 + t0) : make_uint2(0u, 0u);
+  const double vr = t0 < o2 ? S.voo[t0] : 0.0;
   double2 h[kPre];
-  const int kp = idx0 < o2 ? S.kpos[xs + idx0] : -1;
-  const double vr = idx0 < o2 ? S.voo[idx0] : 0.0;
 #pragma unroll
   for (int k = 0; k < kPre; ++k)
-    h[k] = (idx0 < o2 && k < ndiis) ? ld(HT + (size_t)k * n3 + idx0) : make_double2(0.0, 0.0);
-  const double vsr = (int)threadIdx.x < o ? S.vso[sv * o + threadIdx.x] : 0.0;
-  if (threadIdx.x < kMaxDiis * kMaxDiis) D.bm[threadIdx.x] = bw;
-  if (threadIdx.x < kMaxDiis) D.coef[threadIdx.x] = (int)threadIdx.x == slot_cur ? 1.0 : 0.0;
+    h[k] = (t0 < nvs && k < ndiis) ? ld(HC + (size_t)k * nval) : make_double2(0.0, 0.0);
+  const double vsr = t0 < o ? S.vso[sv * o + t0] : 0.0;
+  if (t0 < kMaxDiis * kMaxDiis) D.bm[t0] = bw;
+  if (t0 < kMaxDiis) D.coef[t0] = t0 == slot_cur ? 1.0 : 0.0;
+  for (int idx = t0; idx < o2; idx += BT) {
+    Tab[idx] = 0.0;
+    Tba[idx] = 0.0;
+    Vo[idx] = idx == t0 ? vr : S.voo[idx];
+  }
   stage(S, L);
   if (phase == PH_DONE) return;  // the whole block: the solve has finished
   if (src == SRC_DIIS) {  // a failed solve leaves the plain update
-    if (threadIdx.x < 64) {
+    if (t0 < 64) {
       if (ndiis <= 8)
         diis_gauss_jordan8(D.bm, nh, slot_cur, ndiis, D.coef);
       else
@@ -554,23 +552,28 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
     }
     __syncthreads();
   }
-  double cf[kPre];
-#pragma unroll
-  for (int k = 0; k < kPre; ++k) cf[k] = k < ndiis ? uniform(D.coef[k]) : 0.0;
   // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
   // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
   // so the table reads as zero there (the last entry is always past n2max).
-  for (int k = S.n2max + 1 + threadIdx.x; k < kMaxN2; k += BT) L.vt[k] = 0.0;
-  if (idx0 < o2) slab_place(S, src, Tab, Tba, Vo, xs, idx0, kp,
-                            slab_amplitude<kPre>(S, src, ndiis, cf, h, D.coef, xs, n3, idx0, kp >= 0), vr);
-  for (int idx = idx0 + BT; idx < o2; idx += BT) {
-    const int kpm = S.kpos[xs + idx];
-    double2 hm[kPre];
+  for (int k = S.n2max + 1 + t0; k < kMaxN2; k += BT) L.vt[k] = 0.0;
+  if (src == SRC_TCUR) {
+    for (int idx = t0; idx < o2; idx += BT) {
+      const double2 t = ld(S.tcur + xs + idx);
+      Tab[idx] = t.x;
+      Tba[idx] = t.y;
+    }
+  } else {
+    double cf[kPre];
 #pragma unroll
-    for (int k = 0; k < kPre; ++k)
-      hm[k] = k < ndiis ? ld(HT + (size_t)k * n3 + idx) : make_double2(0.0, 0.0);
-    const double2 t = slab_amplitude<kPre>(S, src, ndiis, cf, hm, D.coef, xs, n3, idx, kpm >= 0);
-    slab_place(S, src, Tab, Tba, Vo, xs, idx, kpm, t, S.voo[idx]);
+    for (int k = 0; k < kPre; ++k) cf[k] = k < ndiis ? uniform(D.coef[k]) : 0.0;
+    if (t0 < nvs) slab_place(S, Tab, Tba, xs, cm, combine_history<kPre>(cf, h, D.coef, ndiis, HC, nval));
+    for (int c = t0 + BT; c < nvs; c += BT) {  // slabs of more than BT valid elements
+      const auto HCc = S.histT + c0 + c;
+      double2 hm[kPre];
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) hm[k] = k < ndiis ? ld(HCc + (size_t)k * nval) : make_double2(0.0, 0.0);
+      slab_place(S, Tab, Tba, xs, ld(S.cmap + c0 + c), combine_history<kPre>(cf, hm, D.coef, ndiis, HCc, nval));
+    }
   }
   if ((int)threadIdx.x < o) vs[threadIdx.x] = vsr;
   __syncthreads();
@@ -1036,11 +1039,12 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
 
 // Block partials of the residual pass: max dT, max |R|, sum E, the non-finite
 // count and the nd DIIS dots e_new . e_k (the new error against every history
-// slot), padded with zeros to NQ, written slot-major for k_finalize.
+// slot; HE is the element's entry of slot 0, slots stride apart), padded with
+// zeros to NQ, written slot-major for k_finalize.
 template <int NQ>
 __device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, int slot, bool lead,
                                               double dt_part, double r_part, double e_part, double bad,
-                                              double eab, double eba, const double2* HE, int n3,
+                                              double eab, double eba, const double2* HE, int stride,
                                               double* red) {
   double v[NQ];
   v[0] = dt_part;
@@ -1054,7 +1058,7 @@ __device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, i
       if (k == slot) {
         dk = eab * eab + eba * eba;
       } else {
-        const double2 h = HE[(size_t)k * n3];
+        const double2 h = HE[(size_t)k * stride];
         dk = eab * h.x + eba * h.y;
       }
     }
@@ -1077,14 +1081,19 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
-  // the state, the element's partner and the Joo parts are requested
+  // the state, the element (one valid element per thread, by compact index
+  // cp, which is also its history position) and the Joo parts are requested
   // together; a finished solve leaves after they arrive
   const int phase = S.st->phase, slot = S.st->slot;
   const int lb = gb - prefix[s];
   const int o = S.o, o2 = o * o, n3 = S.n3;
-  const int x = lb * kElemPerBlock + threadIdx.x;
-  int b = -1;
-  if (x < n3) b = S.pidx[x];
+  const int cp = lb * kElemPerBlock + threadIdx.x;
+  int x = 0, b = -1;
+  if (cp < S.nvalid) {
+    const uint2 cm = ld(S.cmap + cp);
+    x = (int)cm.x;
+    b = elem_partner(cm);
+  }
   __shared__ double joo_l[kMaxOcc];
   if (S.do_mosaic)  // Joo from its slice-range parts, in part order
     for (int k = threadIdx.x; k < o; k += blockDim.x) {
@@ -1167,19 +1176,19 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       if (S.rout) { S.rout[x] = rab; S.rout[n3 + x] = rba; }
     }
     if (flags & MF_ITERATE) {
-      S.histT[(size_t)slot * n3 + x] = make_double2(nab, nba);
+      S.histT[(size_t)slot * S.nvalid + cp] = make_double2(nab, nba);
       if (S.diis_err_is_dt) { eab = nab - t_ab; eba = nba - t_ba; }
       else { eab = rab; eba = rba; }
-      if (S.mode == UEGCCD_MODE_DIIS) S.histE[(size_t)slot * n3 + x] = make_double2(eab, eba);
+      if (S.mode == UEGCCD_MODE_DIIS) S.histE[(size_t)slot * S.nvalid + cp] = make_double2(eab, eba);
     }
   }
   // block partials: max dT, max |R|, sum E, non-finite count, DIIS dots
   const int nd = (S.mode == UEGCCD_MODE_DIIS) ? nh : 0;
-  const double2* HE = S.histE + (lead ? x : 0);
+  const double2* HE = S.histE + (lead ? cp : 0);
   if (nd <= 12)
-    main_partials<16>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, n3, red);
+    main_partials<16>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, S.nvalid, red);
   else
-    main_partials<32>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, n3, red);
+    main_partials<32>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, S.nvalid, red);
 }
 
 // Sums and maxima of one solve's element-block partials by one block, in a
@@ -1377,19 +1386,18 @@ __global__ __launch_bounds__(256) void k_connectivity(const SolveDev* __restrict
   const SolveG& S = as_global(solves)[0];
   for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
   __syncthreads();
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < S.n3) {
-    const int b = S.pidx[x];
-    if (b >= 0) {
-      const int o = S.o, i = x % o, a = o + x / (o * o);
-      const int4 ni = S.nv4[i];
-      const int d1 = d2(ni, S.nv4[a]), d2b = d2(ni, S.nv4[b]);
-      if (d1 > nbins || d2b > nbins) {
-        atomicOr(overflow, 1);
-      } else {
-        atomicAdd(&h[d1 - 1], 1);
-        atomicAdd(&h[d2b - 1], 1);
-      }
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;  // compact index of a valid element
+  if (c < S.nvalid) {
+    const uint2 cm = ld(S.cmap + c);
+    const int x = (int)cm.x, b = elem_partner(cm);
+    const int o = S.o, i = x % o, a = o + x / (o * o);
+    const int4 ni = S.nv4[i];
+    const int d1 = d2(ni, S.nv4[a]), d2b = d2(ni, S.nv4[b]);
+    if (d1 > nbins || d2b > nbins) {
+      atomicOr(overflow, 1);
+    } else {
+      atomicAdd(&h[d1 - 1], 1);
+      atomicAdd(&h[d2b - 1], 1);
     }
   }
   __syncthreads();
@@ -1397,11 +1405,11 @@ __global__ __launch_bounds__(256) void k_connectivity(const SolveDev* __restrict
     if (h[k]) atomicAdd(&hist[k], h[k]);
 }
 
-int connectivity_on_device(const BatchDev& b, int n3, int nbins, int* hist, int* overflow,
+int connectivity_on_device(const BatchDev& b, int nvalid, int nbins, int* hist, int* overflow,
                            hipStream_t s) {
   if (nbins > kMaxConnBins) return -1;
-  hipLaunchKernelGGL(k_connectivity, dim3((n3 + 255) / 256), dim3(256), 0, s, b.solves, nbins, hist,
-                     overflow);
+  hipLaunchKernelGGL(k_connectivity, dim3(std::max(1, (nvalid + 255) / 256)), dim3(256), 0, s, b.solves,
+                     nbins, hist, overflow);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

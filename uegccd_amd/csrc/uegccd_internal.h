@@ -5,8 +5,9 @@
 //   amplitudes, residuals, G: dense FP64 T(i,j,a) with i fastest, then j, then
 //   the virtual a (the reference's T(NOcc,NOcc,NOcc+1:NAO) column-major order),
 //   spin blocks "abab" then "abba" back to back (2*n3 doubles).
-//   pidx[n3]: momentum partner b(i,j,a) = k_i + k_j - k_a as a 0-based orbital
-//   index when it is a virtual orbital of the basis, else -1.
+//   The valid elements (momentum partner b(i,j,a) = k_i + k_j - k_a a virtual
+//   orbital of the basis) in element order: cmap[c] = elem_map(x, position in tk, b).
+//   Element-wise kernels run over them (invalid amplitudes stay zero).
 //   Pair-momentum blocks K = n_i + n_j: their occupied pairs (x = i + o*j) and
 //   their virtual columns a (with partner b), as CSR lists.
 #pragma once
@@ -23,6 +24,16 @@ constexpr int kFinThreads = 256;                  // threads of the per-solve re
 constexpr int kMaxNao = 1024;                 // orbitals staged in LDS
 constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the Coulomb table
 constexpr int kMaxOcc = 64;                   // NOcc staged per slice in LDS
+
+// Map entry of a valid element: (element x, position in tk | partner b << 22).
+constexpr int kPosBits = 22;
+static_assert((long long)kMaxOcc * kMaxOcc * kMaxNao <= (1LL << kPosBits), "tk positions fit 22 bits");
+static_assert(kMaxNao <= (1 << (32 - kPosBits)), "orbital indices fit 10 bits");
+__host__ __device__ inline uint2 elem_map(int x, int pos, int b) {
+  return make_uint2((unsigned)x, (unsigned)pos | ((unsigned)b << kPosBits));
+}
+__host__ __device__ inline int elem_pos(uint2 m) { return (int)(m.y & ((1u << kPosBits) - 1u)); }
+__host__ __device__ inline int elem_partner(uint2 m) { return (int)(m.y >> kPosBits); }
 
 enum Phase : int { PH_ITER = 0, PH_FINAL = 1, PH_DONE = 2 };
 enum FinKind : int { FIN_ITER = 0, FIN_INIT = 1, FIN_SUM = 2 };
@@ -73,7 +84,6 @@ struct SolveDevT {
   Ptr<const double> voo;    // [o*o] v(n_k - n_j) over occupied k, j (k fastest)
   Ptr<const double> vso;    // [v*o] v(n_s - n_k), slice (virtual) s, occupied k
   Ptr<const float> vtab32;  // same, FP32
-  Ptr<const int> pidx;      // [n3]
   Ptr<const int> kb_poff;   // [nK+1]
   Ptr<const int> kb_pairs;  // [o*o] pair ids x = i + o*j grouped by K
   Ptr<const int> kb_voff;   // [nK+1]
@@ -82,7 +92,10 @@ struct SolveDevT {
   Ptr<const int4> kb_nva;   // (n_a, a) of the same list
   Ptr<const int> kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
   Ptr<const int> kb_toff;   // [nK+1] offsets of the K blocks in tk
-  Ptr<const int> kpos;      // [n3] position of element x in tk, or -1 (invalid)
+  // the valid elements in element order (compact index c; the history layout)
+  int nvalid;
+  Ptr<const int> slab_coff; // [v+1] first compact index of each virtual slice
+  Ptr<const uint2> cmap;    // [nvalid] elem_map(element x, position in tk, partner b)
   Ptr<const int> pair_kb;   // [o*o] block of pair x
   Ptr<const int> pair_pos;  // [o*o] position of pair x inside its block
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, one per transfer q
@@ -98,8 +111,8 @@ struct SolveDevT {
                             // invalid); k_slice writes them from the history
   Ptr<double2> tk;          // [nvalid] the same amplitudes by K block: entry ci of block K's
                             // virtual list, pair p at kb_toff[K] + ci*noo + p
-  Ptr<double2> histT;       // [ndiis][n3] updated amplitudes, slot-major (coalesced per slot)
-  Ptr<double2> histE;       // [ndiis][n3] DIIS error vectors, slot-major
+  Ptr<double2> histT;       // [ndiis][nvalid] updated amplitudes by compact index, slot-major
+  Ptr<double2> histE;       // [ndiis][nvalid] DIIS error vectors, the same layout
   Ptr<double> gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
   Ptr<double> rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
   Ptr<double2> lad;         // [n3] particle-particle ladder, (S, P) parts
@@ -127,7 +140,7 @@ using SolveDev = SolveDevT<HostPtr>;
 struct BatchDev {
   int n;
   SolveDev* solves;     // device array [n]
-  int* elem_prefix;     // [n+1] 256-thread blocks over each solve's n3
+  int* elem_prefix;     // [n+1] 256-thread blocks over each solve's valid elements
   int* slice_prefix;    // [n+1] one k_slice block per virtual slice
   int4* lad_items;      // (solve, K, row0, col0) ladder tiles
   int nlad;
@@ -179,7 +192,7 @@ void launch_one(const BatchDev& b, int which, hipStream_t s);
 
 // MP2 connectivity histogram of a batch-of-one solve (nbins <= 4096 bins,
 // bin k counts |dn|^2 = k+1); hist and overflow zeroed by the caller.
-int connectivity_on_device(const BatchDev& b, int n3, int nbins, int* hist, int* overflow,
+int connectivity_on_device(const BatchDev& b, int nvalid, int nbins, int* hist, int* overflow,
                            hipStream_t s);
 
 // Plane-wave basis on the device: keys of the cube [-cell, cell]^3 (npad a
