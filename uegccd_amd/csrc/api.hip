@@ -415,6 +415,7 @@ int check_supported(const uegccd_system& s) {
 struct ArenaImage {
   DeviceArena* a;
   std::vector<char>* img;
+  bool copy = true;  // false: the image already holds the arrays; put() only places them
   template <typename T>
   T* reserve(size_t count) {
     T* d = a->take<T>(std::max<size_t>(1, count));
@@ -430,7 +431,7 @@ struct ArenaImage {
   template <typename T>
   const T* put(const std::vector<T>& h) {
     T* d = reserve<T>(h.size());
-    write(d, h.data(), h.size());
+    if (copy) write(d, h.data(), h.size());
     return d;
   }
 };
@@ -532,8 +533,11 @@ int ndiis_of(const uegccd_options& opt) { return opt.mode == UEGCCD_MODE_DIIS ? 
 // `base`: the maps, the device descriptor and the batch-of-one tables form a
 // constant section written into `image` (uploaded by the caller in one copy),
 // the work buffers follow it from byte *work_off on (cleared by the caller).
+// Only the descriptor depends on `base`: with fill false, `image` already
+// holds the section from an earlier layout (at any base) and only the
+// descriptor and the prefixes are written again.
 void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t bytes,
-                std::vector<char>& image, size_t* work_off) {
+                std::vector<char>& image, size_t* work_off, bool fill) {
   const uegccd_options& opt = c->opt;
   const int ndiis = ndiis_of(opt);
   const size_t n3 = c->n3;
@@ -542,9 +546,11 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   c->arena.base = base;
   c->arena.size = bytes;
   c->arena.used = 0;
-  image.clear();
-  image.reserve(bytes);  // one allocation: the constant section is a prefix of the arena
-  ArenaImage im{&c->arena, &image};
+  if (fill) {
+    image.clear();
+    image.reserve(bytes);  // one allocation: the constant section is a prefix of the arena
+  }
+  ArenaImage im{&c->arena, &image, fill};
 
   SolveDev& S = c->sd;
   S = SolveDev{};
@@ -653,7 +659,7 @@ int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
   c->owns_memory = true;
   std::vector<char> image;
   size_t work_off = 0;
-  layout_ctx(c, sys, base, bytes, image, &work_off);
+  layout_ctx(c, sys, base, bytes, image, &work_off, true);
   if (c->arena.used > c->arena.size) { uegccd_ctx_destroy(c); return UEGCCD_ERR_ALLOC; }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     uegccd_ctx_destroy(c);
@@ -1379,6 +1385,7 @@ struct Request {
   int32_t nao = 0;                 // results of planning, copied into res on admission
   double ehf = 0.0;
   int64_t footprint = 0;
+  std::vector<char> image;         // constant section laid out by the planner (descriptor at admission)
 };
 
 }  // namespace
@@ -1528,6 +1535,10 @@ int plan_request(Request& r) {
   rc = plan_ctx(&s, &q.opt, &r.ctx);
   if (rc) return rc;
   r.footprint = (int64_t)arena_bytes(r.ctx, ndiis_of(r.ctx->opt));
+  // the constant section at a placeholder base: admission writes the descriptor
+  size_t work_off = 0;
+  layout_ctx(r.ctx, &r.sys, nullptr, (size_t)r.footprint, r.image, &work_off, true);
+  r.image.shrink_to_fit();
   return UEGCCD_OK;
 }
 
@@ -1630,11 +1641,11 @@ int admit(uegccd_server* s) {
     c->device = s->opt.device;
     c->opt.device = s->opt.device;
     c->stream = s->stream;
-    std::vector<char> image;
     size_t work_off = 0;
-    layout_ctx(c, &r.sys, mem, fp, image, &work_off);
+    layout_ctx(c, &r.sys, mem, fp, r.image, &work_off, false);  // the planner built the image
     r.placed = true;
-    int rc = upload_image(mem, image, s->stream, &s->stage);
+    int rc = upload_image(mem, r.image, s->stream, &s->stage);
+    std::vector<char>().swap(r.image);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(mem + work_off, 0, fp - work_off, s->stream));
     const int prec64 = r.ctx->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
