@@ -384,7 +384,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double) * eb * kPartialStride);
   bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
-  bytes += aligned(sizeof(int) * 6) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
+  bytes += aligned(sizeof(int) * 8) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
   bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
            aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
@@ -595,25 +595,30 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.qb_info = im.put(c->qb_info);
   S.qrow = im.put(c->qrow);
   c->d_sd = im.reserve<SolveDev>(1);
-  // batch of one: element/slice block prefixes and the ladder tiles
+  // the solve's ladder tiles and contraction items (any batch reads them here)
+  S.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
+  S.lad_items = im.put(c->lad_items);
+  S.ncon = (int)c->con_items.size();
+  S.con_items = im.put(c->con_items);
+  // batch of one: block prefixes of the element, slice, ladder and contraction launches
   BatchDev& b = c->batch;
   const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
   const int sb = need_slice ? c->v : 0;
-  const int h_pref[4] = {0, eb, 0, sb};
-  int* d_pref = im.reserve<int>(4);
-  im.write(d_pref, h_pref, 4);
+  const int h_pref[8] = {0, eb, 0, sb, 0, S.nlad, 0, S.ncon};
+  int* d_pref = im.reserve<int>(8);
+  im.write(d_pref, h_pref, 8);
   b.n = 1;
   b.solves = c->d_sd;
   b.elem_prefix = d_pref;
   b.slice_prefix = d_pref + 2;
+  b.lad_prefix = d_pref + 4;
+  b.con_prefix = d_pref + 6;
   b.elem_blocks = eb;
   b.slice_blocks = sb;
   b.slice_lds = slice_lds_bytes(c->o);
   b.slice_threads = slice_threads(c->o);
-  b.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
-  b.lad_items = const_cast<int4*>(im.put(c->lad_items));
-  b.ncon = (int)c->con_items.size();
-  b.con_items = const_cast<int4*>(im.put(c->con_items));
+  b.nlad = S.nlad;
+  b.ncon = S.ncon;
   // work buffers
   c->arena.used = aligned(c->arena.used);
   *work_off = c->arena.used;
@@ -883,26 +888,23 @@ int upload_image(char* dst, const std::vector<char>& image, hipStream_t stream, 
   return UEGCCD_OK;
 }
 
-// Device tables of a batch: descriptors, block prefixes, ladder tiles, and a
-// state gather buffer, in one (reused) allocation uploaded in one copy.
+// Device tables of a batch: descriptors, block and item prefixes (each solve's
+// ladder tiles and contraction items stay in its own memory) and a state
+// gather buffer, in one (reused) allocation uploaded in one copy.
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = nullptr,
                PinnedStage* stage = nullptr) {
-  std::vector<int> ep(n + 1, 0), sp(n + 1, 0);
-  std::vector<int4> items, con;
+  std::vector<int> ep(n + 1, 0), sp(n + 1, 0), lp(n + 1, 0), cp(n + 1, 0);
   int omax = 1;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
     ep[k + 1] = ep[k] + elem_blocks_of(c);
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
     sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
+    lp[k + 1] = lp[k] + c->sd.nlad;
+    cp[k + 1] = cp[k] + c->sd.ncon;
     omax = std::max(omax, c->o);
-    if (c->sd.do_ladder)
-      for (int4 it : c->lad_items) items.push_back(make_int4(k, it.y, it.z, it.w));
-    for (int4 it : c->con_items) con.push_back(make_int4(k, it.y, it.z, it.w));
   }
-  size_t bytes = aligned(sizeof(SolveDev) * n) + 2 * aligned(sizeof(int) * (n + 1)) +
-                 aligned(sizeof(int4) * std::max<size_t>(1, items.size())) +
-                 aligned(sizeof(int4) * std::max<size_t>(1, con.size())) +
+  size_t bytes = aligned(sizeof(SolveDev) * n) + 4 * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(State) * n);
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
@@ -922,11 +924,11 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   for (int k = 0; k < n; ++k) im.write(b.solves + k, &cs[k]->sd, 1);
   b.elem_prefix = const_cast<int*>(im.put(ep));
   b.slice_prefix = const_cast<int*>(im.put(sp));
-  b.lad_items = const_cast<int4*>(im.put(items));
-  b.con_items = const_cast<int4*>(im.put(con));
-  b.ncon = (int)con.size();
+  b.lad_prefix = const_cast<int*>(im.put(lp));
+  b.con_prefix = const_cast<int*>(im.put(cp));
+  b.ncon = cp[n];
   out->states = a.take<State>(n);
-  b.nlad = (int)items.size();
+  b.nlad = lp[n];
   b.elem_blocks = ep[n];
   b.slice_blocks = sp[n];
   b.slice_lds = slice_lds_bytes(omax);

@@ -798,11 +798,11 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
   }
 }
 
-__device__ void ladder_block(const SolveDev* __restrict__ solves, const int4* __restrict__ items,
-                             int item, double* vt, int4* cls, double* red) {
-  const int4 it = items[item];
-  const SolveG& S = as_global(solves)[it.x];
-  if (!S.do_ladder) return;
+__device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+                             int n, int blk, double* vt, int4* cls, double* red) {
+  const int s = find_solve(prefix, n, blk);
+  const SolveG& S = as_global(solves)[s];
+  const int4 it = ld(S.lad_items + (blk - prefix[s]));
   const int phase = S.st->phase;  // requested with the tables, tested after them
   const int K = it.y, voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
@@ -825,11 +825,11 @@ __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solve
 }
 
 __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
-                                                const int4* __restrict__ items) {
+                                                const int* __restrict__ prefix, int n) {
   __shared__ double vt[kMaxN2];
   __shared__ int4 cls[kMaxNao];  // a K block's virtual list has at most v < kMaxNao entries
   __shared__ double red[3 * 64 * 8];
-  ladder_block(solves, items, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
+  ladder_block(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
 }
 
 // --------------------------------------------------------------------------
@@ -1017,15 +1017,16 @@ __device__ void joo_wave(const SolveG& S, int part) {
 // One work item per wave; the four waves of a block take four consecutive
 // items of the XCD-ordered list.
 __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
-                                                            const int4* __restrict__ items,
+                                                            const int* __restrict__ prefix, int n,
                                                             int nitems) {
   // the wave index is uniform over the wave: saying so lets the item, the
   // descriptor and its pointers live in scalar registers (scalar loads)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + wave;
   if (item >= nitems) return;
-  const int4 it = items[item];
-  const SolveG& S = as_global(solves)[it.x];
+  const int s = find_solve(prefix, n, item);
+  const SolveG& S = as_global(solves)[s];
+  const int4 it = ld(S.con_items + (item - prefix[s]));
   // (no phase test: the contraction outputs of a finished solve are never read)
   if (it.y == CK_RING)
     quad_tile<false>(S, it.z, it.w);
@@ -1612,13 +1613,13 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
 
 static void launch_ladder(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   launch_k(k_ladder, stop, dim3(b.nlad), dim3(256), 0, s, (const SolveDev*)b.solves,
-           (const int4*)b.lad_items);
+           (const int*)b.lad_prefix, b.n);
 }
 
 static void launch_contract(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   const int grid = (b.ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock;
   launch_k(k_contract, stop, dim3(grid), dim3(kElemThreads), 0, s, (const SolveDev*)b.solves,
-           (const int4*)b.con_items, b.ncon);
+           (const int*)b.con_prefix, b.n, b.ncon);
 }
 
 void launch_init(const BatchDev& b, hipStream_t s) {

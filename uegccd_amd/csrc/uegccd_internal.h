@@ -106,6 +106,12 @@ struct SolveDevT {
   Ptr<const int> qb_d;      // [nqb*o] virtual index of n_l + q for each l, or -1
   Ptr<const int4> qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
   Ptr<const int> qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
+  // work items of the tile launches: (0, K, row0, col0) ladder tiles and
+  // (0, kind, arg, arg) contraction items
+  Ptr<const int4> lad_items;
+  int nlad;
+  Ptr<const int4> con_items;
+  int ncon;
   // mutable buffers
   Ptr<double2> tcur;        // [n3] (abab, abba) amplitudes of the current pass (zero where
                             // invalid); k_slice writes them from the history
@@ -142,12 +148,12 @@ struct BatchDev {
   SolveDev* solves;     // device array [n]
   int* elem_prefix;     // [n+1] 256-thread blocks over each solve's valid elements
   int* slice_prefix;    // [n+1] one k_slice block per virtual slice
-  int4* lad_items;      // (solve, K, row0, col0) ladder tiles
+  int* lad_prefix;      // [n+1] ladder tiles of each solve (one block each)
   int nlad;
   int elem_blocks, slice_blocks;
   int slice_lds;        // dynamic LDS bytes of k_slice (largest NOcc in the batch)
   int slice_threads;    // threads of the k_slice blocks (slice_threads of the largest NOcc)
-  int4* con_items;      // (solve, kind, arg, arg) work items of k_contract, one per wave
+  int* con_prefix;      // [n+1] k_contract work items of each solve (one wave each)
   int ncon;
 };
 
