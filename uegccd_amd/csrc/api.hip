@@ -384,7 +384,8 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double) * eb * kPartialStride);
   bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
-  bytes += aligned(sizeof(int) * 8) + aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
+  bytes += aligned(sizeof(int) * (6 + 2 * kSliceClasses)) +
+           aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
   bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
            aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
@@ -604,19 +605,23 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   BatchDev& b = c->batch;
   const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
   const int sb = need_slice ? c->v : 0;
-  const int h_pref[8] = {0, eb, 0, sb, 0, S.nlad, 0, S.ncon};
-  int* d_pref = im.reserve<int>(8);
-  im.write(d_pref, h_pref, 8);
+  const int cls = slice_threads(c->o) / 256 - 1;
+  int h_pref[6 + 2 * kSliceClasses] = {0, eb, 0, S.nlad, 0, S.ncon};
+  h_pref[6 + 2 * cls + 1] = sb;
+  int* d_pref = im.reserve<int>(6 + 2 * kSliceClasses);
+  im.write(d_pref, h_pref, 6 + 2 * kSliceClasses);
   b.n = 1;
   b.solves = c->d_sd;
   b.elem_prefix = d_pref;
-  b.slice_prefix = d_pref + 2;
-  b.lad_prefix = d_pref + 4;
-  b.con_prefix = d_pref + 6;
+  b.lad_prefix = d_pref + 2;
+  b.con_prefix = d_pref + 4;
   b.elem_blocks = eb;
   b.slice_blocks = sb;
-  b.slice_lds = slice_lds_bytes(c->o);
-  b.slice_threads = slice_threads(c->o);
+  for (int k = 0; k < kSliceClasses; ++k) {
+    b.slice_prefix[k] = d_pref + 6 + 2 * k;
+    b.slice_cls_blocks[k] = k == cls ? sb : 0;
+    b.slice_lds[k] = k == cls ? slice_lds_bytes(c->o) : 0;
+  }
   b.nlad = S.nlad;
   b.ncon = S.ncon;
   // work buffers
@@ -893,18 +898,21 @@ int upload_image(char* dst, const std::vector<char>& image, hipStream_t stream, 
 // gather buffer, in one (reused) allocation uploaded in one copy.
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = nullptr,
                PinnedStage* stage = nullptr) {
-  std::vector<int> ep(n + 1, 0), sp(n + 1, 0), lp(n + 1, 0), cp(n + 1, 0);
-  int omax = 1;
+  std::vector<int> ep(n + 1, 0), lp(n + 1, 0), cp(n + 1, 0);
+  std::vector<int> sp[kSliceClasses];
+  int slds[kSliceClasses] = {};
+  for (auto& v : sp) v.assign(n + 1, 0);
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
     ep[k + 1] = ep[k] + elem_blocks_of(c);
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
-    sp[k + 1] = sp[k] + (need_slice ? c->v : 0);
+    const int cls = slice_threads(c->o) / 256 - 1;
+    for (int q = 0; q < kSliceClasses; ++q) sp[q][k + 1] = sp[q][k] + (need_slice && q == cls ? c->v : 0);
+    if (need_slice) slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o));
     lp[k + 1] = lp[k] + c->sd.nlad;
     cp[k + 1] = cp[k] + c->sd.ncon;
-    omax = std::max(omax, c->o);
   }
-  size_t bytes = aligned(sizeof(SolveDev) * n) + 4 * aligned(sizeof(int) * (n + 1)) +
+  size_t bytes = aligned(sizeof(SolveDev) * n) + (3 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(State) * n);
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
@@ -923,16 +931,19 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.solves = im.reserve<SolveDev>(n);
   for (int k = 0; k < n; ++k) im.write(b.solves + k, &cs[k]->sd, 1);
   b.elem_prefix = const_cast<int*>(im.put(ep));
-  b.slice_prefix = const_cast<int*>(im.put(sp));
+  b.slice_blocks = 0;
+  for (int q = 0; q < kSliceClasses; ++q) {
+    b.slice_prefix[q] = const_cast<int*>(im.put(sp[q]));
+    b.slice_cls_blocks[q] = sp[q][n];
+    b.slice_lds[q] = slds[q];
+    b.slice_blocks += sp[q][n];
+  }
   b.lad_prefix = const_cast<int*>(im.put(lp));
   b.con_prefix = const_cast<int*>(im.put(cp));
   b.ncon = cp[n];
   out->states = a.take<State>(n);
   b.nlad = lp[n];
   b.elem_blocks = ep[n];
-  b.slice_blocks = sp[n];
-  b.slice_lds = slice_lds_bytes(omax);
-  b.slice_threads = slice_threads(omax);
   return upload_image(out->base, image, stream, stage);
 }
 

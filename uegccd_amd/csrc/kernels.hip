@@ -490,7 +490,7 @@ __device__ __forceinline__ void slab_place(const SolveG& S, double* Tab, double*
 }
 
 // One block of BT threads per virtual slice s (BT = slice_threads of the
-// largest NOcc in the launch): one slab entry and one J output per thread
+// largest NOcc among the launch's solves): one slab entry and one J output per thread
 // (a larger slab loops over the rest).
 template <int BT>
 __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
@@ -1601,13 +1601,18 @@ static void launch_k(F kernel, hipEvent_t stop, dim3 grid, dim3 block, uint32_t 
 
 static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   allow_large_lds();
-  const dim3 g(b.slice_blocks), t(b.slice_threads);
   const auto* sd = (const SolveDev*)b.solves;
-  const auto* pf = (const int*)b.slice_prefix;
-  switch (b.slice_threads) {
-    case 256: launch_k(k_slice<256>, stop, g, t, b.slice_lds, s, sd, pf, b.n); break;
-    case 512: launch_k(k_slice<512>, stop, g, t, b.slice_lds, s, sd, pf, b.n); break;
-    default: launch_k(k_slice<768>, stop, g, t, b.slice_lds, s, sd, pf, b.n); break;
+  int last = -1;  // the stop event goes with the last launch
+  for (int q = 0; q < kSliceClasses; ++q)
+    if (b.slice_cls_blocks[q] > 0) last = q;
+  for (int q = 0; q < kSliceClasses; ++q) {
+    if (b.slice_cls_blocks[q] == 0) continue;
+    const dim3 g(b.slice_cls_blocks[q]), t(256 * (q + 1));
+    const auto* pf = (const int*)b.slice_prefix[q];
+    const hipEvent_t ev = q == last ? stop : nullptr;
+    if (q == 0) launch_k(k_slice<256>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+    else if (q == 1) launch_k(k_slice<512>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+    else launch_k(k_slice<768>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
   }
 }
 

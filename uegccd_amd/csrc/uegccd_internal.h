@@ -142,17 +142,23 @@ struct SolveDevT {
 template <typename T> using HostPtr = T*;
 using SolveDev = SolveDevT<HostPtr>;
 
+constexpr int kSliceClasses = 3;
+
 // Launch geometry of a batch of solves iterated together.
 struct BatchDev {
   int n;
   SolveDev* solves;     // device array [n]
   int* elem_prefix;     // [n+1] 256-thread blocks over each solve's valid elements
-  int* slice_prefix;    // [n+1] one k_slice block per virtual slice
+  // k_slice: one launch per block size (256, 512, 768 threads, class c =
+  // slice_threads / 256 - 1) over the solves of that size; one block per
+  // virtual slice, the other solves have no blocks in the class's prefix
+  int* slice_prefix[kSliceClasses];  // [n+1] each
   int* lad_prefix;      // [n+1] ladder tiles of each solve (one block each)
   int nlad;
-  int elem_blocks, slice_blocks;
-  int slice_lds;        // dynamic LDS bytes of k_slice (largest NOcc in the batch)
-  int slice_threads;    // threads of the k_slice blocks (slice_threads of the largest NOcc)
+  int elem_blocks;
+  int slice_blocks;                  // over all classes
+  int slice_cls_blocks[kSliceClasses];
+  int slice_lds[kSliceClasses];      // dynamic LDS bytes (largest NOcc of the class)
   int* con_prefix;      // [n+1] k_contract work items of each solve (one wave each)
   int ncon;
 };
