@@ -95,6 +95,11 @@ def _lib():
         lib.orc_ccd.restype = ctypes.c_int
         lib.orc_ccd.argtypes = [P, ctypes.c_double, ctypes.c_int, D, D, D, D, D, I, D, D]
         lib.orc_solvecc.argtypes = [P, D, D]
+        lib.orc_drpa_energy.restype = ctypes.c_double
+        lib.orc_drpa_energy.argtypes = [P, D, ctypes.c_double]
+        lib.orc_drpa_terms.argtypes = [P, ctypes.c_int, D, D, D]
+        lib.orc_drpa.restype = ctypes.c_int
+        lib.orc_drpa.argtypes = [P, ctypes.c_double, ctypes.c_int, D, D, D, D, I, D, D]
         _LIB = lib
     return _LIB
 
@@ -200,3 +205,31 @@ class OracleSystem:
         return dict(ecorr=ec.value, niter=niter, maxres=mr.value, failed=bool(fl.value),
                     trace_e=te[:niter].copy(), trace_dt=tdt[:niter].copy(),
                     tab=tab, tba=tba, taa=taa)
+
+    def drpa_energy(self, t2, csosex=0.0):
+        """dRPA (csosex 0) or SOSEX (csosex 1) energy of the spin-summed T2."""
+        t2 = np.asfortranarray(t2, dtype=np.float64)
+        return self._lib.orc_drpa_energy(self._h, _dp(t2), float(csosex))
+
+    def drpa_terms(self, t2, mask=11):
+        """G and R of the dRPA TermsWrapper for the spin-summed T2."""
+        t2 = np.asfortranarray(t2, dtype=np.float64)
+        g, r = self.zeros(), self.zeros()
+        self._lib.orc_drpa_terms(self._h, mask, _dp(t2), _dp(g), _dp(r))
+        return g, r
+
+    def drpa(self, t2=None, tolmax=0.0, maxiter=1000):
+        """DrvdRPA from the MP2 T2aaaa (or the given spin-summed T2)."""
+        if t2 is None:
+            _, t2, _, _ = self.mp2()
+        t2 = np.array(t2, dtype=np.float64, order="F")
+        te = np.zeros(max(maxiter, 1) + 1)
+        tdt = np.zeros(max(maxiter, 1) + 1)
+        ec, es, mr = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        fl = ctypes.c_int()
+        niter = self._lib.orc_drpa(self._h, tolmax, maxiter, _dp(t2), ctypes.byref(ec),
+                                   ctypes.byref(es), ctypes.byref(mr), ctypes.byref(fl),
+                                   _dp(te), _dp(tdt))
+        return dict(ecorr=ec.value, esosex=es.value, niter=niter, maxres=mr.value,
+                    failed=bool(fl.value), trace_e=te[:niter].copy(),
+                    trace_dt=tdt[:niter].copy(), t2=t2)

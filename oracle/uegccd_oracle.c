@@ -802,3 +802,164 @@ int orc_ccd(const orc_system* s, double tolmax, int maxiter, double* taa, double
     free(gab); free(gba); free(oldab); free(oldba); free(rab); free(rba);
     return niter;
 }
+
+/* ------------------------------------------------------------------------
+ * dRPA / rCCD (source/dRPA.f90): the ring-only CCD on the single spin-summed
+ * T2 that DrvdRPA iterates (source/Main.f90:85-91 passes T2aaaa).
+ */
+
+/* CCEnergy of the dRPA module, source/dRPA.f90:181-216. */
+double orc_drpa_energy(const orc_system* s, const double* t2, double csosex) {
+    const int o = s->nocc, n = s->nao;
+    double ecorr = 0.0;
+    for (int i = 1; i <= o; ++i)
+        for (int j = 1; j <= o; ++j)
+            for (int a = o + 1; a <= n; ++a) {
+                int b = find_index(s, i, j, a);
+                if (b <= o) continue;
+                double vijab = eri(s, a, b, i, j, 0);
+                double vijba = eri(s, b, a, i, j, 0);
+                size_t x = IDX(i, j, a);
+                double de = vijab * t2[x] * 2 - vijba * t2[x] * csosex;
+                ecorr = ecorr + de;
+            }
+    return ecorr;
+}
+
+/* TermsWrapper of the dRPA module, source/dRPA.f90:146-175: GetG2Drivers
+ * (source/dRPA.f90:364-396), GetG2Mosaics (399-450), GetG2Rings (453-551),
+ * then GetRes (282-312).  term_mask bit0 drivers, bit1 mosaics, bit3 rings,
+ * each further gated by the Do* flags as in TermsWrapper. */
+void orc_drpa_terms(const orc_system* s, int term_mask, const double* t2, double* g2,
+                    double* r2) {
+    const int o = s->nocc, n = s->nao;
+    const size_t sz = (size_t)o * o * (n - o);
+    memset(g2, 0, sz * sizeof(double));
+    if (term_mask & 1)
+        for (int i = 1; i <= o; ++i)
+            for (int j = 1; j <= o; ++j)
+                for (int a = o + 1; a <= n; ++a) {
+                    int b = find_index(s, i, j, a);
+                    if (b <= o) continue;
+                    double v = eri(s, i, j, a, b, s->p.irange[R_DRVDIR]);
+                    if (v < -80.0) continue;
+                    g2[IDX(i, j, a)] = v;
+                }
+    if ((term_mask & 2) && s->p.do_mosaic) {
+        double* joo = (double*)calloc((size_t)o + 1, sizeof(double));
+        double* jvv = (double*)calloc((size_t)n + 1, sizeof(double));
+        for (int k = 1; k <= o; ++k)
+            for (int l = 1; l <= o; ++l)
+                for (int c = o + 1; c <= n; ++c) {
+                    int d = find_index(s, k, l, c);
+                    if (d <= o) continue;
+                    double v = eri(s, c, d, k, l, s->p.irange[R_MOSAIC]);
+                    if (v < -80.0) continue;
+                    jvv[c] = jvv[c] + 2 * v * t2[IDX(k, l, c)];
+                    joo[k] = joo[k] + 2 * v * t2[IDX(k, l, c)];
+                }
+        for (int i = 1; i <= o; ++i)
+            for (int j = 1; j <= o; ++j)
+                for (int a = o + 1; a <= n; ++a) {
+                    int b = find_index(s, i, j, a);
+                    if (b <= o) continue;
+                    size_t x = IDX(i, j, a);
+                    g2[x] = g2[x] - 0.5 * (jvv[a] * t2[x] + jvv[b] * t2[x]);
+                    g2[x] = g2[x] - 0.5 * (joo[i] * t2[x] + joo[j] * t2[x]);
+                }
+        free(joo);
+        free(jvv);
+    }
+    if ((term_mask & 8) && s->p.do_ring) {
+        const int flin = imax3(s->p.irange[R_RING], s->p.irange[R_DRING], s->p.irange[R_LINRING]);
+        const int fquad = imax3(s->p.irange[R_RING], s->p.irange[R_DRING], s->p.irange[R_QUADRING]);
+        for (int i = 1; i <= o; ++i)
+            for (int j = 1; j <= o; ++j)
+                for (int a = o + 1; a <= n; ++a) {
+                    int b = find_index(s, i, j, a);
+                    if (b <= o) continue;
+                    size_t x = IDX(i, j, a);
+                    for (int k = 1; k <= o; ++k) {
+                        int c = find_index(s, i, k, a);
+                        if (c <= o) continue;
+                        double v = eri(s, c, j, k, b, flin);
+                        if (v < -80.0) continue;
+                        g2[x] = g2[x] + 2 * v * t2[IDX(i, k, a)];
+                    }
+                    for (int k = 1; k <= o; ++k) {
+                        int c = find_index(s, j, k, b);
+                        if (c <= o) continue;
+                        double v = eri(s, c, i, k, a, flin);
+                        if (v < -80.0) continue;
+                        g2[x] = g2[x] + 2 * v * t2[IDX(j, k, b)];
+                    }
+                }
+        for (int i = 1; i <= o; ++i)
+            for (int a = o + 1; a <= n; ++a)
+                for (int l = 1; l <= o; ++l) {
+                    double jidal = 0.0;
+                    for (int k = 1; k <= o; ++k) {
+                        int c = find_index(s, i, k, a);
+                        if (c <= o) continue;
+                        int d = find_index(s, k, l, c);
+                        if (d <= o) continue;
+                        double v = eri(s, c, d, k, l, fquad);
+                        if (v < -80.0) continue;
+                        jidal = jidal + 2 * t2[IDX(i, k, a)] * v;
+                    }
+                    for (int j = 1; j <= o; ++j) {
+                        int b = find_index(s, i, j, a);
+                        if (b <= o) continue;
+                        int d = find_index(s, j, l, b);
+                        if (d <= o) continue;
+                        g2[IDX(i, j, a)] = g2[IDX(i, j, a)] + 2 * jidal * t2[IDX(l, j, d)];
+                    }
+                }
+    }
+    if (r2) get_res(s, t2, g2, r2);
+}
+
+/* DrvdRPA, source/dRPA.f90:23-143, with the DIIS module's GetTFromDIIS
+ * (Max_N_DIIS = 3, source/DIISCC.f90:19, 52-131) and the dRPA module's own
+ * SolveCC (219-250) and CnvrgCC (315-344).  Returns NIter; *ecorr is
+ * Ec(dRPA), *esosex Ec(SOSEX) of the final amplitudes. */
+int orc_drpa(const orc_system* s, double tolmax, int maxiter, double* t2, double* ecorr_out,
+             double* esosex_out, double* maxres_out, int* failed, double* trace_e,
+             double* trace_dt) {
+    const int o = s->nocc, n = s->nao, ndiis = 3;
+    const size_t sz = (size_t)o * o * (n - o);
+    if (tolmax <= 0.0) tolmax = s->tolmax;
+    if (maxiter <= 0) maxiter = 1000;
+    double* g2 = (double*)calloc(sz, sizeof(double));
+    double* old = (double*)calloc(sz * ndiis, sizeof(double));
+    double* r2 = (double*)calloc(sz * ndiis, sizeof(double));
+    memcpy(old + sz * (ndiis - 1), t2, sz * sizeof(double));
+    int niter = 1;
+    double dt = 0.0;
+    for (size_t x = 0; x < sz; ++x) dt = fmax(dt, fabs(t2[x]));
+    double ecorr = orc_drpa_energy(s, t2, 0.0);
+    if (trace_e) trace_e[0] = ecorr;
+    if (trace_dt) trace_dt[0] = dt;
+    while (dt >= tolmax && niter < maxiter) {
+        get_t_from_diis(sz, ndiis, niter, t2, old, r2);
+        orc_drpa_terms(s, 11, t2, g2, r2 + sz * (ndiis - 1));
+        orc_solvecc(s, t2, g2);
+        niter = niter + 1;
+        memmove(old, old + sz, sz * (ndiis - 1) * sizeof(double));
+        dt = 0.0;
+        for (size_t x = 0; x < sz; ++x) dt = fmax(dt, fabs(t2[x] - old[x + sz * (ndiis - 1)]));
+        memcpy(old + sz * (ndiis - 1), t2, sz * sizeof(double));
+        ecorr = orc_drpa_energy(s, t2, 0.0);
+        if (trace_e && niter <= maxiter) trace_e[niter - 1] = ecorr;
+        if (trace_dt && niter <= maxiter) trace_dt[niter - 1] = dt;
+    }
+    orc_drpa_terms(s, 11, t2, g2, r2 + sz * (ndiis - 1));
+    double maxres = 0.0;
+    for (size_t x = 0; x < sz; ++x) maxres = fmax(maxres, fabs(r2[x + sz * (ndiis - 1)]));
+    if (ecorr_out) *ecorr_out = ecorr;
+    if (esosex_out) *esosex_out = orc_drpa_energy(s, t2, 1.0);
+    if (maxres_out) *maxres_out = maxres;
+    if (failed) *failed = maxres > 10000.0 * tolmax;
+    free(g2); free(old); free(r2);
+    return niter;
+}

@@ -78,6 +78,18 @@ int    orc_ccd(const orc_system* s, double tolmax, int maxiter,
 /* One damped-Jacobi step as SolveCC (source/CCD.f90:354-386), in place. */
 void   orc_solvecc(const orc_system* s, double* t2, const double* g2);
 
+/* dRPA / rCCD on the spin-summed T2 (source/dRPA.f90).  Energy with the
+ * SOSEX coefficient csosex (0: dRPA, 1: SOSEX; source/dRPA.f90:181-216). */
+double orc_drpa_energy(const orc_system* s, const double* t2, double csosex);
+/* The dRPA TermsWrapper (source/dRPA.f90:146-175): G and R = D*T - G (r2 may
+ * be NULL); term_mask bit0 drivers, bit1 mosaics, bit3 rings. */
+void   orc_drpa_terms(const orc_system* s, int term_mask, const double* t2, double* g2,
+                      double* r2);
+/* DrvdRPA (source/dRPA.f90:23-143): as orc_ccd, from t2 in place; *ecorr is
+ * Ec(dRPA), *esosex Ec(SOSEX) of the final amplitudes. */
+int    orc_drpa(const orc_system* s, double tolmax, int maxiter, double* t2, double* ecorr,
+                double* esosex, double* maxres, int* failed, double* trace_e, double* trace_dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@ Sources (paths relative to the reference root):
                           twist-averaging MP2 blocks and the final TA CCD
   source/Test/fort.60     basis n-vectors / eigenvalues at r_s=1 and 5, Madelung
   source/Test/fort.59     the 10 twists, averaged eigenvalues, special twist
-  benchmarks/ZRun-dRPA/Output, fort.60      N=14 cutoff 5 (57 orbitals)
+  benchmarks/ZRun-dRPA/Output, fort.60      N=14 cutoff 5 (57 orbitals): MP2 and dRPA/SOSEX
   benchmarks/ZRun-v6-N14/fort.59, Output    N=14 cutoff 2, 100 twists
 """
 from __future__ import annotations
@@ -60,6 +60,8 @@ def output_blocks(lines):
                 cur["niter"] = int(re.findall(r"in\s+(\d+)", ln)[0])
             elif "Ec(CCD) is" in ln or "Ec(dRPA) is" in ln:
                 cur["ecorr"] = f(re.findall(FLOAT, ln)[0])
+            elif "Ec(SOSEX) is" in ln:
+                cur["esosex"] = f(re.findall(FLOAT, ln)[0])
             elif "Max CCD Residual" in ln:
                 cur["maxres"] = f(re.findall(FLOAT, ln)[0])
     return blocks
@@ -124,6 +126,11 @@ def main(root="/root/reference"):
         "params": {"nel": 14, "maxkpoint": 5, "rs": 1.0, "twist": [0.0, 0.0, 0.0]},
         "ehf": blk[0]["ehf"], "eigen": blk[0]["eig"], "e2": blk[0]["e2"],
         "madelung": mads[0] if mads else None, "basis": tables[0] if tables else None,
+        # the dRPA/SOSEX run of that Input (DodRPASOSEX T, mosaics off): the
+        # printed trajectory (E, NIter, dT), Ec(dRPA), Ec(SOSEX), max residual
+        "drpa_trace": blk[1]["trace"], "drpa_niter": blk[1]["niter"],
+        "drpa_ecorr": blk[1]["ecorr"], "drpa_esosex": blk[1]["esosex"],
+        "drpa_maxres": blk[1]["maxres"],
     }
     # --- N=14, r_s=1, cutoff 2, first twist of the v6 run (benchmarks/ZRun-v6-N14) ---
     rows = [ln.split() for ln in read(root, "benchmarks/ZRun-v6-N14/fort.59")]

@@ -207,3 +207,74 @@ def test_each_term_moves_the_fixture(n24, drop):
             assert abs(e - ref) < F15_10
         else:
             assert abs(e - ref) > 1e-6
+
+
+def test_c0_drpa_sosex(fixtures):
+    """DrvdRPA on the reference's dRPA benchmark Input (N=14, r_s=1, cutoff 5,
+    DodRPASOSEX, mosaics off): the printed trajectory (E, NIter, dT),
+    Ec(dRPA), Ec(SOSEX) and the max residual of benchmarks/ZRun-dRPA/Output."""
+    c = fixtures["n14_cut5"]
+    s = _sys(c["params"], do_mosaic=False)
+    r = s.drpa()
+    tr = np.array(c["drpa_trace"])
+    assert r["niter"] == c["drpa_niter"] == len(tr)
+    np.testing.assert_array_equal(tr[:, 1], np.arange(1, len(tr) + 1))
+    np.testing.assert_allclose(r["trace_e"], tr[:, 0], rtol=0, atol=F15_10)
+    np.testing.assert_allclose(r["trace_dt"], tr[:, 2], rtol=0, atol=F15_10)
+    assert abs(r["ecorr"] - c["drpa_ecorr"]) < F18_12
+    assert abs(r["esosex"] - c["drpa_esosex"]) < F18_12
+    assert abs(r["maxres"] - c["drpa_maxres"]) < 5e-21
+    assert not r["failed"]
+
+
+def _drpa_reduced(s, t2):
+    """The dRPA G in the form the device computes it: every ring term carries
+    v(n_i - n_a), so with R(i,a) = sum_k T(i,k,a) and Q(j, q) = sum over
+    (l, d) with n_d - n_l = q of T(l,j,d),
+      G = v_ia (1 + 2 R(i,a) + 2 R(j,b) + 4 R(i,a) Q(j, n_a - n_i))
+          - 1/2 (Jvv(a) + Jvv(b) + Joo(i) + Joo(j)) T,
+    Jvv(c) = sum_k 2 v(n_c - n_k) R(k,c), Joo(k) = sum_c 2 v(n_c - n_k) R(k,c)."""
+    o, nv = s.nocc, s.nvir
+    nvec = s.nvec.astype(int)
+    idx = {tuple(nvec[p]): p for p in range(s.nao)}
+    def v(p, q):  # v(n_p - n_q), 0-based orbitals
+        return s.eri(p + 1, q + 1, q + 1, p + 1)
+    R = t2.sum(axis=1)  # R[i, a-o]
+    qsum = {}
+    for l in range(o):
+        for d in range(o, s.nao):
+            q = tuple(nvec[d] - nvec[l])
+            qsum.setdefault(q, np.zeros(o))
+            qsum[q] += t2[l, :, d - o]
+    jvv = np.array([sum(2 * v(o + c, k) * R[k, c] for k in range(o)) for c in range(nv)])
+    joo = np.array([sum(2 * v(o + c, k) * R[k, c] for c in range(nv)) for k in range(o)])
+    g = np.zeros_like(t2)
+    for i in range(o):
+        for j in range(o):
+            for av in range(nv):
+                a = o + av
+                b = idx.get(tuple(nvec[i] + nvec[j] - nvec[a]), -1)
+                if b < o:
+                    continue
+                vq = v(i, a)
+                qj = qsum.get(tuple(nvec[a] - nvec[i]), np.zeros(o))[j]
+                ring = vq * (2 * R[i, av] + 2 * R[j, b - o] + 4 * R[i, av] * qj)
+                mos = 0.5 * (jvv[av] + jvv[b - o] + joo[i] + joo[j]) * t2[i, j, av]
+                g[i, j, av] = vq + (ring if s.params.do_ring else 0.0) - (mos if s.params.do_mosaic else 0.0)
+    return g
+
+
+@pytest.mark.parametrize("mosaic", [False, True])
+def test_drpa_reduced_form(fixtures, mosaic):
+    """The row-sum / transfer-sum form of the dRPA terms (what the device
+    computes) against the oracle's loop-literal GetG2Drivers/Mosaics/Rings on
+    random amplitudes of the valid elements."""
+    c = fixtures["n14_cut5"]
+    s = _sys(c["params"], do_mosaic=mosaic)
+    _, taa, _, _ = s.mp2()
+    rng = np.random.default_rng(7)
+    t2 = np.where(taa != 0.0, rng.standard_normal(taa.shape) * 0.05, 0.0)
+    g, _ = s.drpa_terms(np.asfortranarray(t2))
+    gr = _drpa_reduced(s, t2)
+    assert np.abs(g).max() > 0.1
+    np.testing.assert_allclose(gr, g, rtol=0, atol=1e-14 * np.abs(g).max())
