@@ -178,6 +178,19 @@ int uegccd_trace(const uegccd_ctx* ctx, double* e, double* dt, int32_t n);
 /* DrvMBPT amplitudes and E(2) (source/MP2.f90:56-98).  Any array may be NULL. */
 int uegccd_mp2(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, double* t2abba, double* e2);
 
+/* Replaces DrvdRPA (source/dRPA.f90:23-143, called from source/Main.f90:91,
+ * 139, 253 when DodRPASOSEX): the ring-only CCD (dRPA + SOSEX) on the
+ * spin-summed amplitudes T2 (the reference passes its T2aaaa array), with the
+ * ring and mosaic terms the system enables, iterated as the reference does
+ * (damped Jacobi; its DIIS is inert) until max|dT| < TolMax, then the residual
+ * check (failed when max|R| > 10000 TolMax).  t2 (layout as the other
+ * amplitude arrays): in, the starting amplitudes (NULL: the MP2 T2aaaa); out,
+ * the final ones.  res->ecorr is Ec(dRPA) of the last iteration, *esosex (may
+ * be NULL) Ec(SOSEX) of the final amplitudes; uegccd_trace holds the
+ * (E, dT) lines.  The context must be FP64 in UEGCCD_MODE_REFERENCE, else
+ * UEGCCD_ERR_UNSUPPORTED.  Returns the status as uegccd_solve. */
+int uegccd_drpa(uegccd_ctx* ctx, double* t2, uegccd_result* res, double* esosex);
+
 /* DrvMBPT's connectivity (source/MP2.f90:75-85): counts[k] is the number of
  * (i,j,a,b) excitations with |n_i - n_a|^2 = k+1 plus those with
  * |n_i - n_b|^2 = k+1, over the valid (i,j,a); counted on the device.

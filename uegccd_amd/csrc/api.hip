@@ -94,7 +94,7 @@ struct uegccd_ctx {
   std::vector<int4> nv4, kb_nva;
   std::vector<double> eig, vtab, voo, vso;  // Coulomb table; v(n_k - n_j) over occupied k, j; v(n_s - n_k)
   std::vector<int4> lad_items;  // (0, K, row0, col0)
-  std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow;  // hole-particle sets P(q)
+  std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow, qb_of;  // hole-particle sets P(q)
   std::vector<int4> qb_info;
   std::vector<int4> con_items;  // (0, kind, arg, arg) of k_contract
   // device
@@ -279,10 +279,12 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->qb_neg.assign(nq, -1);
     c->qb_d.assign((size_t)nq * o, -1);
     c->qrow.assign((size_t)o * v, -1);
+    c->qb_of.assign((size_t)o * v, -1);
     for (int q = 0; q < nq; ++q) {
       for (int r : qrows[q]) {
         const int i = r % o, pv = r / o;
         c->qrow[(size_t)pv * o + i] = (int)c->qb_rows.size();
+        c->qb_of[(size_t)pv * o + i] = q;
         c->qb_rows.push_back(r);
       }
       c->qb_roff[q + 1] = (int)c->qb_rows.size();
@@ -391,7 +393,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
            aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->con_items.size()));
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->qb_info.size()));
-  bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qrow.size()));
+  bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->qrow.size()));  // qrow, qb_of
   bytes += aligned(sizeof(double4) * n3);                              // quad
   bytes += 4096;
   return bytes;
@@ -595,6 +597,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.qb_d = im.put(c->qb_d);
   S.qb_info = im.put(c->qb_info);
   S.qrow = im.put(c->qrow);
+  S.qb_of = im.put(c->qb_of);
   c->d_sd = im.reserve<SolveDev>(1);
   // the solve's ladder tiles and contraction items (any batch reads them here)
   S.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
@@ -615,6 +618,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.elem_prefix = d_pref;
   b.lad_prefix = d_pref + 2;
   b.con_prefix = d_pref + 4;
+  b.drpa_sum_blocks = c->v + (int)c->qb_neg.size();
   b.elem_blocks = eb;
   b.slice_blocks = sb;
   for (int k = 0; k < kSliceClasses; ++k) {
@@ -800,6 +804,7 @@ int uegccd_mp2(uegccd_ctx* c, double* taa, double* tab, double* tba, double* e2)
   if (e2) *e2 = st.ecorr;
   return UEGCCD_OK;
 }
+
 
 int uegccd_trace(const uegccd_ctx* c, double* e, double* dt, int32_t n) {
   if (!c) return UEGCCD_ERR_INVALID;
@@ -1111,6 +1116,66 @@ int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
 }
 
 }  // namespace
+
+// DrvdRPA (source/dRPA.f90:23-143) on the context's basis: the rings-only CCD
+// (with mosaics when the system asks for them) on the spin-summed T2,
+// iterated as the reference does (damped Jacobi; its DIIS is inert) until
+// max|dT| < TolMax, then the residual check with FailRatio 10000.
+extern "C" int uegccd_drpa(uegccd_ctx* c, double* t2, uegccd_result* res, double* esosex) {
+  if (!c || !res) return UEGCCD_ERR_INVALID;
+  if (c->opt.mode != UEGCCD_MODE_REFERENCE || c->opt.precision != UEGCCD_PREC_FP64)
+    return UEGCCD_ERR_UNSUPPORTED;
+  HIPCHK(hipSetDevice(c->device));
+  const auto t_start = std::chrono::steady_clock::now();
+  const BatchDev& b = c->batch;
+  hipStream_t s = c->stream;
+  launch_state_reset(b, PH_ITER, 1, s);
+  if (t2) {
+    int rc = copy_in_amplitudes(c, t2, nullptr);  // spin-summed T2 in the first component
+    if (rc) return rc;
+  } else {
+    launch_mp2(b, s, 1);
+  }
+  launch_init(b, s, 1);
+  launch_finalize(b, FIN_INIT, s, 1);
+  HIPCHK(hipGetLastError());
+  State st{};
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  // passes in chunks between state polls; passes after the last one return at once
+  for (int passes = 0; st.phase != PH_DONE && passes <= c->maxiter + 2;) {
+    const int chunk = std::min(16, std::max(1, c->maxiter + 2 - passes));
+    for (int k = 0; k < chunk; ++k) launch_drpa_pass(b, s);
+    HIPCHK(hipGetLastError());
+    passes += chunk;
+    rc = read_state(c, &st);
+    if (rc) return rc;
+  }
+  const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  c->last_niter = st.niter;  // the (E, dT) lines uegccd_trace returns
+  const int status = fill_result(c, st, ms, res);
+  res->failed = (st.maxres > 10000.0 * c->tolmax) || status != UEGCCD_OK;  // source/dRPA.f90:103-104
+  // Ec(SOSEX) of the final amplitudes (CCEnergy with CSosex 1, source/dRPA.f90:200-208)
+  const size_t n3 = c->n3;
+  std::vector<double> h(2 * n3);
+  HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (esosex) {
+    const int o = c->o;
+    double e = 0.0;
+    for (const uint2& m : c->cmap) {
+      const int x = (int)m.x, i = x % o, av = x / (o * o), bv = elem_partner(m) - o;
+      const double t = h[2 * (size_t)x];
+      e = e + (c->vso[(size_t)av * o + i] * t * 2 - c->vso[(size_t)bv * o + i] * t);
+    }
+    *esosex = e;
+  }
+  if (t2) {
+    std::fill(t2, t2 + n3, 0.0);
+    for (const uint2& m : c->cmap) t2[m.x] = h[2 * (size_t)m.x];
+  }
+  return status;
+}
 
 extern "C" int uegccd_solve_batch(uegccd_ctx** cs, int32_t n, double** tab, double** tba,
                                   uegccd_result* res) {

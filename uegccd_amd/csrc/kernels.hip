@@ -190,7 +190,8 @@ __device__ __forceinline__ GlobalPtr<double> partial(const SolveG& S, int q, int
 // Initial state: E(T0) and max|T0| (source/CCD.f90:62-64); also lays the
 // starting amplitudes out by K block (tk), as k_slice does for later passes.
 __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restrict__ solves,
-                                                         const int* __restrict__ prefix, int n) {
+                                                         const int* __restrict__ prefix, int n,
+                                                         int drpa) {
   __shared__ Lds L;
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
@@ -207,7 +208,10 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
     const double2 t = S.tcur[x];
     st2(S.tk + elem_pos(cm), t);
     const double taa = t.x + t.y;
-    e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
+    if (drpa)  // the dRPA module's CCEnergy with CSosex 0 (source/dRPA.f90:200-208)
+      e = VPQ(a, i) * t.x * 2;
+    else
+      e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
     tmax = fmax(fabs(t.x), fabs(t.y));
     if (!isfinite(t.x) || !isfinite(t.y)) bad = 1.0;
   }
@@ -224,7 +228,8 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
 
 // DrvMBPT amplitudes and E(2) (source/MP2.f90:56-74); writes tcur.
 __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restrict__ solves,
-                                                        const int* __restrict__ prefix, int n) {
+                                                        const int* __restrict__ prefix, int n,
+                                                        int drpa) {
   __shared__ Lds L;
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
@@ -243,7 +248,8 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
     const double tab = vab / den, tba = -vba / den;
     const double taa = tab + tba;
     e = 0.5 * taa * (vab - vba) + 0.5 * tab * vab - 0.5 * tba * vba;
-    S.tcur[x] = make_double2(tab, tba);
+    // dRPA starts from T2aaaa = T2abab + T2abba (source/Main.f90:85,91)
+    S.tcur[x] = drpa ? make_double2(taa, 0.0) : make_double2(tab, tba);
   }
   e = block_sum(e, L.red);
   if (threadIdx.x == 0) {
@@ -1415,6 +1421,108 @@ int connectivity_on_device(const BatchDev& b, int nvalid, int nbins, int* hist, 
 }
 
 // --------------------------------------------------------------------------
+// dRPA / rCCD (source/dRPA.f90): the ring-only CCD on the spin-summed T2
+// (held in the first component of tcur).  Every ring term of GetG2Rings
+// (source/dRPA.f90:474-548) carries v(n_i - n_a) (the momentum of each
+// Coulomb element there is n_i - n_a or its negative), so with the row sums
+// R(i,a) = sum_k T(i,k,a) and the transfer sums Q(j,q) = sum over the pairs
+// (l,d) of P(q) of T(l,j,d),
+//   G(i,j,a) = v_ia [1 + 2 R(i,a) + 2 R(j,b) + 4 R(i,a) Q(j, n_a - n_i)]
+//              - 1/2 (Jvv(a) + Jvv(b) + Joo(i) + Joo(j)) T(i,j,a)
+// with the mosaic sums (source/dRPA.f90:420-436) Jvv(c) = sum_k 2 v(n_c - n_k)
+// R(k,c) and Joo(k) = sum_c 2 v(n_c - n_k) R(k,c).  Work buffers: R in rsum,
+// the Joo parts in joo_part, Jvv in jvv, Q in lad (nqb*o doubles <= 2*n3).
+
+// Block a < v (one wave): R(:,a), the Joo parts and Jvv(a) of slab a.
+// Block v + q: Q(:, q) of transfer q.
+__global__ __launch_bounds__(64) void k_drpa_sums(const SolveDev* __restrict__ solves) {
+  const SolveG& S = as_global(solves)[0];
+  if (S.st->phase == PH_DONE) return;
+  const int o = S.o, o2 = o * o, lane = threadIdx.x;
+  const auto T = S.tcur;
+  if ((int)blockIdx.x < S.v) {
+    const int av = blockIdx.x;
+    double r = 0.0;
+    if (lane < o)
+      for (int k = 0; k < o; ++k) r += T[(size_t)lane + o * k + (size_t)o2 * av].x;
+    const double p = lane < o ? 2.0 * S.vso[av * o + lane] * r : 0.0;
+    if (lane < o) {
+      S.rsum[av * o + lane] = r;
+      S.joo_part[av * o + lane] = p;
+    }
+    const double jv = wave_sum(p);
+    if (lane == 0) S.jvv[av] = jv;
+  } else {
+    const int q = blockIdx.x - S.v;
+    const int r0 = S.qb_roff[q], r1 = S.qb_roff[q + 1];
+    double acc = 0.0;
+    if (lane < o)
+      for (int r = r0; r < r1; ++r) {
+        const int pr = S.qb_rows[r], l = pr % o, dv = pr / o;
+        acc += T[(size_t)l + o * lane + (size_t)o2 * dv].x;
+      }
+    if (lane < o) reinterpret_cast<GlobalPtr<double>>(S.lad)[(size_t)q * o + lane] = acc;
+  }
+}
+
+// One thread per valid element: G, R = D*T - G (GetRes, source/dRPA.f90:282-312),
+// the damped update (SolveCC, 219-250), dT and the energy of the new
+// amplitudes (CCEnergy with CSosex 0, 181-216).  In the final pass (the
+// residual check, 100-103) the amplitudes stay.
+__global__ __launch_bounds__(kElemThreads) void k_drpa_main(const SolveDev* __restrict__ solves) {
+  __shared__ double red[(kElemThreads / 64) * 32];
+  __shared__ double joo_l[kMaxOcc];
+  const SolveG& S = as_global(solves)[0];
+  const int phase = S.st->phase;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int o = S.o, cp = lb * kElemPerBlock + threadIdx.x;
+  int x = 0, b = -1;
+  if (cp < S.nvalid) {
+    const uint2 cm = ld(S.cmap + cp);
+    x = (int)cm.x;
+    b = elem_partner(cm);
+  }
+  if (S.do_mosaic)
+    for (int k = threadIdx.x; k < o; k += blockDim.x) {
+      double a = 0.0;
+      for (int c = 0; c < S.v; ++c) a += S.joo_part[c * o + k];
+      joo_l[k] = a;
+    }
+  __syncthreads();
+  if (phase == PH_DONE) return;
+  const bool lead = cp < S.nvalid;
+  double e_part = 0.0, dt_part = 0.0, r_part = 0.0, bad = 0.0;
+  if (lead) {
+    const int i = x % o, j = (x / o) % o, av = x / (o * o), a = o + av, bv = b - o;
+    const double t = S.tcur[x].x;
+    const double vq = S.vso[av * o + i];  // v(n_i - n_a) = v(n_j - n_b)
+    double g = vq;
+    if (S.do_mosaic) {
+      g = g - 0.5 * (S.jvv[av] * t + S.jvv[bv] * t);
+      g = g - 0.5 * (joo_l[i] * t + joo_l[j] * t);
+    }
+    if (S.do_ring) {
+      const double ria = S.rsum[av * o + i], rjb = S.rsum[bv * o + j];
+      const double qj = reinterpret_cast<GlobalPtr<const double>>(S.lad)[(size_t)S.qb_of[av * o + i] * o + j];
+      g += 2 * vq * ria + 2 * vq * rjb + 4 * vq * ria * qj;
+    }
+    const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
+    const double res = -g + den * t;
+    r_part = fabs(res);
+    double tn = t;
+    if (phase == PH_ITER) {
+      const double df = S.df;
+      tn = (g / den - (1.0 - df) * t) / df;
+      dt_part = fabs(tn - t);
+      S.tcur[x] = make_double2(tn, 0.0);
+    }
+    e_part = vq * tn * 2;
+    if (!isfinite(tn) || !isfinite(res)) bad = 1.0;
+  }
+  main_partials<16>(S, lb, 0, 0, lead, dt_part, r_part, e_part, bad, 0.0, 0.0, S.histE, 0, red);
+}
+
+// --------------------------------------------------------------------------
 // Plane-wave basis on the device (init_HEG + SetupBasis + change_rs,
 // source/HEG.f90:150-324, 494-533, 581-615).  Every floating-point step is an
 // explicitly rounded operation in the host builder's order (no contraction
@@ -1627,13 +1735,20 @@ static void launch_contract(const BatchDev& b, hipStream_t s, hipEvent_t stop = 
            (const int*)b.con_prefix, b.n, b.ncon);
 }
 
-void launch_init(const BatchDev& b, hipStream_t s) {
+void launch_init(const BatchDev& b, hipStream_t s, int drpa) {
   hipLaunchKernelGGL(k_init, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
-                     b.elem_prefix, b.n);
+                     b.elem_prefix, b.n, drpa);
 }
-void launch_mp2(const BatchDev& b, hipStream_t s) {
+void launch_mp2(const BatchDev& b, hipStream_t s, int drpa) {
   hipLaunchKernelGGL(k_mp2, dim3(b.elem_blocks), dim3(kElemPerBlock), 0, s, b.solves,
-                     b.elem_prefix, b.n);
+                     b.elem_prefix, b.n, drpa);
+}
+
+void launch_drpa_pass(const BatchDev& b, hipStream_t s) {
+  // a batch of one: the sums over its slabs and transfers, then its elements
+  hipLaunchKernelGGL(k_drpa_sums, dim3(b.drpa_sum_blocks), dim3(64), 0, s, b.solves);
+  hipLaunchKernelGGL(k_drpa_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, b.solves, (int)FIN_ITER, 1);
 }
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64) {
   hipLaunchKernelGGL(k_finalize, dim3(b.n), dim3(kFinThreads), 0, s, b.solves, kind, prec64);

@@ -106,6 +106,7 @@ struct SolveDevT {
   Ptr<const int> qb_d;      // [nqb*o] virtual index of n_l + q for each l, or -1
   Ptr<const int4> qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
   Ptr<const int> qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
+  Ptr<const int> qb_of;     // [v*o] transfer block of pair (i, p), at (p - o)*o + i
   // work items of the tile launches: (0, K, row0, col0) ladder tiles and
   // (0, kind, arg, arg) contraction items
   Ptr<const int4> lad_items;
@@ -156,6 +157,7 @@ struct BatchDev {
   int* lad_prefix;      // [n+1] ladder tiles of each solve (one block each)
   int nlad;
   int elem_blocks;
+  int drpa_sum_blocks;               // k_drpa_sums of a batch of one: v + nqb
   int slice_blocks;                  // over all classes
   int slice_cls_blocks[kSliceClasses];
   int slice_lds[kSliceClasses];      // dynamic LDS bytes (largest NOcc of the class)
@@ -181,8 +183,13 @@ inline int slice_threads(int o) {
 constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 
 // Kernel launchers (kernels.hip).
-void launch_init(const BatchDev& b, hipStream_t s);
-void launch_mp2(const BatchDev& b, hipStream_t s);
+// drpa: the spin-summed amplitudes of the dRPA iteration (in the first
+// component of tcur, the second zero) and its energy
+void launch_init(const BatchDev& b, hipStream_t s, int drpa = 0);
+void launch_mp2(const BatchDev& b, hipStream_t s, int drpa = 0);
+// One dRPA pass (source/dRPA.f90:80-96) of a batch-of-one: the row and
+// transfer sums, the per-element update, the reductions.
+void launch_drpa_pass(const BatchDev& b, hipStream_t s);
 // Events timing one kernel of a pass (which: as launch_one, or 4 for the
 // reductions and DIIS solve): `after` is written when it completes, `before`
 // when the kernel launched just before it completes.  When `which` is the

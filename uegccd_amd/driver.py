@@ -174,6 +174,24 @@ class CCDContext:
                 raise L.UegccdError(st, "uegccd_solve")
         return SolveResult.from_c(r, taa, tab, tba, self.trace(r.niter))
 
+    def drpa(self, t2=None, allow_unconverged=False):
+        """DrvdRPA (source/dRPA.f90:23-143) on this basis: the ring-only CCD on
+        the spin-summed T2 (default start: the MP2 T2aaaa), iterated as the
+        reference does.  Returns (SolveResult with ecorr = Ec(dRPA) and t2 in
+        its tab field, Ec(SOSEX))."""
+        t2 = self.zeros() if t2 is None else np.array(self._f(t2), order="F", copy=True)
+        start = L.dptr(t2) if np.any(t2) else None
+        r = L.Result()
+        es = ctypes.c_double()
+        st = self._lib.uegccd_drpa(self._h, start, ctypes.byref(r), ctypes.byref(es))
+        if st != L.OK and not (allow_unconverged and st == L.ERR_NOT_CONVERGED):
+            raise L.UegccdError(st, "uegccd_drpa")
+        if start is None:
+            L.check(self._lib.uegccd_get_amplitudes(self._h, None, L.dptr(t2), None),
+                    "uegccd_get_amplitudes")
+        zero = self.zeros()
+        return SolveResult.from_c(r, t2, t2, zero, self.trace(r.niter)), es.value
+
     def connectivity(self, nbins=400):
         """DrvMBPT's connectivity histogram (uegccd_connectivity): bin k-1
         counts the excitations with |n_i - n_a|^2 = k and |n_i - n_b|^2 = k."""
