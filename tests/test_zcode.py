@@ -42,8 +42,10 @@ def parse(lines):
             out["trace"].append((float(t[1]), int(t[2]), float(t[3])))
         elif "converged in" in ln:
             out["niter"] = int(re.findall(r"in\s+(\d+)", ln)[0])
-        elif "Ec(CCD) is" in ln:
+        elif "Ec(CCD) is" in ln or "Ec(dRPA) is" in ln:
             out["ecorr"] = float(re.findall(FLOAT, ln)[0])
+        elif "Ec(SOSEX) is" in ln:
+            out["esosex"] = float(re.findall(FLOAT, ln)[0])
         elif "Max CCD Residual" in ln:
             out["maxres"] = float(re.findall(FLOAT, ln)[0])
     return out
@@ -85,3 +87,24 @@ def test_zcode_diis_mode(tmp_path):
     o = parse(lines)
     assert o["ecorr"] == pytest.approx(cfg["n14_cut5"]["ecorr_tight"], abs=1e-9)
     assert o["niter"] < cfg["n14_cut5"]["niter_reference_mode"]
+
+
+def test_zcode_drpa_reference_output(tmp_path, fixtures):
+    """The reference's dRPA benchmark Input (DodRPASOSEX T, mosaics off):
+    ZCode's Output carries the benchmark's dRPA trajectory, Ec(dRPA),
+    Ec(SOSEX) and max residual (benchmarks/ZRun-dRPA/Output)."""
+    print("[zcode] dRPA benchmark", flush=True)
+    fx = fixtures["n14_cut5"]
+    lines = run_zcode(tmp_path, standard_input(14, 5, 1.0, **{"Do Mosaics": False, "DodRPASOSEX": True}))
+    assert any("dRPA summary follows" in ln for ln in lines)
+    o = parse(lines)
+    assert o["e2"] == pytest.approx(fx["e2"], abs=1e-12)
+    ref = np.array(fx["drpa_trace"])
+    got = np.array(o["trace"])
+    assert o["niter"] == fx["drpa_niter"] == len(got)
+    np.testing.assert_array_equal(got[:, 1], ref[:, 1])
+    np.testing.assert_allclose(got[:, 0], ref[:, 0], rtol=0, atol=1.1e-10)
+    np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=0, atol=1.1e-10)
+    assert o["ecorr"] == pytest.approx(fx["drpa_ecorr"], abs=1e-12)
+    assert o["esosex"] == pytest.approx(fx["drpa_esosex"], abs=1e-12)
+    assert o["maxres"] == pytest.approx(fx["drpa_maxres"], rel=1e-6)

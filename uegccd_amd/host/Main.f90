@@ -1,7 +1,8 @@
 ! ZCode: the single-point driver of the reference (Program TermedCC,
 ! source/Main.f90:2-97) on the MI355X solver: read Input, build the basis,
-! MP2 guess, CCD.  Twist averaging (DoSkipTA F) and dRPA (DodRPASOSEX T) are
-! not part of this host and stop with a message.
+! MP2 guess, then CCD or, with DodRPASOSEX, dRPA/SOSEX on T2aaaa
+! (source/Main.f90:84-92).  Twist averaging (DoSkipTA F) is not part of this
+! host and stops with a message.
 Program TermedCC
   Use Precision
   Use Types, Only: UEGInfoType, HEGDataType
@@ -9,6 +10,7 @@ Program TermedCC
   Use HEG, Only: InitHEG
   Use MP2, Only: DrvMBPT
   Use CCD, Only: DrvCCD, ShutDownSolver
+  Use dRPA, Only: DrvdRPA
   Implicit None
   Type (UEGInfoType) :: UEGInfo
   Type (HEGDataType) :: HEGData
@@ -26,9 +28,12 @@ Program TermedCC
   Close(7)
 
   If (UEGInfo%DoSingleCalc) Then
-    If (UEGInfo%DodRPASOSEX) Stop 'dRPA/SOSEX is not part of this build'
     Call DrvMBPT(HEGData, UEGInfo, T2aaaa, T2abab, T2abba)
-    Call DrvCCD(HEGData, UEGInfo, T2aaaa, T2abab, T2abba)
+    If (.not. UEGInfo%DodRPASOSEX) Then
+      Call DrvCCD(HEGData, UEGInfo, T2aaaa, T2abab, T2abba)
+    Else
+      Call DrvdRPA(HEGData, UEGInfo, T2aaaa)
+    End If
   End If
   Call ShutDownSolver()
   If (UEGInfo%DoSkipTA) Stop 'Finishing calculation after single point calculation'

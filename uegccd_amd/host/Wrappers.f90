@@ -85,6 +85,14 @@ Module Wrappers
       Type(uegccd_result), Intent(Out) :: res
     End Function
 
+    Integer(c_int) Function uegccd_drpa(ctx, t2, res, esosex) Bind(C, name="uegccd_drpa")
+      Import :: c_int, c_ptr, c_double, uegccd_result
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(InOut) :: t2(*)
+      Type(uegccd_result), Intent(Out) :: res
+      Real(c_double), Intent(Out) :: esosex
+    End Function
+
     Integer(c_int) Function uegccd_trace(ctx, e, dt, n) Bind(C, name="uegccd_trace")
       Import :: c_int, c_ptr, c_double, c_int32_t
       Type(c_ptr), Value :: ctx
