@@ -185,6 +185,33 @@ def served(reqs, arrivals, precision, Server, options, max_active=64, iters_per_
     return wall, out
 
 
+def drpa_c2(lib, L, heg, info, CCDContext, options, reps, with_cpu):
+    """dRPA/SOSEX (uegccd_drpa, the reference's DrvdRPA iteration) on the same
+    N=54 |n|^2<=16 system from the MP2 guess: median device-inclusive time per
+    solve over `reps` solves after a warm-up, NIter, Ec(dRPA), Ec(SOSEX); with
+    with_cpu the serial CPU oracle's DrvdRPA is timed beside it and the
+    energies compared."""
+    ctx = CCDContext(heg, info, options(mode="reference"))
+    ctx.drpa()
+    ms = []
+    for _ in range(reps):
+        r, esosex = ctx.drpa()
+        ms.append(r.solve_ms)
+    out = {"workload": "N=54 r_s=2.0 Gamma |n|^2<=16 dRPA/SOSEX (rings and mosaics), reference iteration",
+           "ms_per_solve": float(np.median(ms)), "niter": r.niter, "ecorr_drpa": r.ecorr,
+           "ecorr_sosex": esosex, "maxres": r.maxres}
+    if with_cpu:
+        from tests.helpers import oracle_for
+        orc = oracle_for(info)
+        t0 = time.perf_counter()
+        o = orc.drpa()
+        cpu_s = time.perf_counter() - t0
+        out.update({"cpu_oracle_s": cpu_s, "cpu_cores": 1, "speedup_vs_cpu": cpu_s * 1000.0 / out["ms_per_solve"],
+                    "abs_dE_drpa_vs_oracle": abs(r.ecorr - o["ecorr"]),
+                    "abs_dE_sosex_vs_oracle": abs(esosex - o["esosex"]), "niter_oracle": o["niter"]})
+    return out
+
+
 def twist_stream(lib, L, n, rate):
     from uegccd_amd.driver import CCDContext, Server, init_heg, options
     reqs, arrivals = stream_requests(n, 1234, rate)
@@ -239,6 +266,8 @@ def main():
     ap.add_argument("--rate", type=float, default=700.0, help="Poisson arrivals per second")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the twist stream (for profiling it)")
+    ap.add_argument("--drpa-only", action="store_true",
+                    help="run only the dRPA/SOSEX solves (for profiling them)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="three 256 MiB device copies (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -261,6 +290,9 @@ def main():
         return
     info = workload_info()
     heg = init_heg(info, device=0)
+    if args.drpa_only:
+        print(json.dumps({"drpa": drpa_c2(lib, L, heg, info, CCDContext, options, 5, False)}))
+        return
     ctx = CCDContext(heg, info, options(mode="diis", ndiis=8, precision=args.precision))
     res = L.Result()
 
@@ -337,6 +369,7 @@ def main():
 
     mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
     stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
+    drpa = drpa_c2(lib, L, heg, info, CCDContext, options, 10, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -360,6 +393,7 @@ def main():
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "mixed_precision": mixed,
         "twist_stream": stream,
+        "drpa": drpa,
     }
     print(json.dumps(out))
 

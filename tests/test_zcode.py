@@ -69,11 +69,13 @@ def test_zcode_n24_reference_output(tmp_path, fixtures):
     assert o["maxres"] == pytest.approx(fx["ccd_maxres"], rel=1e-6)
 
 
-def test_zcode_matches_python_host(tmp_path):
-    """Same Input through ZCode (Fortran) and the Python host: same Output text."""
-    print("[zcode] n14 cut5 vs python host", flush=True)
+@pytest.mark.parametrize("drpa", [False, True])
+def test_zcode_matches_python_host(tmp_path, drpa):
+    """Same Input through ZCode (Fortran) and the Python host: same Output text
+    (CCD, and dRPA/SOSEX with mosaics on)."""
+    print(f"[zcode] n14 cut5 vs python host drpa={drpa}", flush=True)
     from uegccd_amd.driver import single_point
-    text = standard_input(14, 5, 1.0)
+    text = standard_input(14, 5, 1.0, DodRPASOSEX=drpa)
     zlines = run_zcode(tmp_path, text)
     ptext, heg, e2, r = single_point(text)
     assert zlines == ptext.splitlines()

@@ -618,7 +618,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.elem_prefix = d_pref;
   b.lad_prefix = d_pref + 2;
   b.con_prefix = d_pref + 4;
-  b.drpa_sum_blocks = c->v + (int)c->qb_neg.size();
+  b.drpa_sum_blocks = kDrpaJooParts * c->o + c->v + (int)c->qb_neg.size();
   b.elem_blocks = eb;
   b.slice_blocks = sb;
   for (int k = 0; k < kSliceClasses; ++k) {
@@ -1138,18 +1138,25 @@ extern "C" int uegccd_drpa(uegccd_ctx* c, double* t2, uegccd_result* res, double
   }
   launch_init(b, s, 1);
   launch_finalize(b, FIN_INIT, s, 1);
+  launch_drpa_start(b, (int64_t)c->n3, s);
   HIPCHK(hipGetLastError());
   State st{};
   int rc = read_state(c, &st);
   if (rc) return rc;
-  // passes in chunks between state polls; passes after the last one return at once
-  for (int passes = 0; st.phase != PH_DONE && passes <= c->maxiter + 2;) {
-    const int chunk = std::min(16, std::max(1, c->maxiter + 2 - passes));
+  // passes in chunks between state polls, sized from the decay of max|dT| as
+  // in the CCD iteration; passes after the last one return at once
+  const int first = std::max(1, c->opt.iters_per_sync), cap = std::max(first, kMaxChunk);
+  std::vector<double> prev_dt(1, 0.0);
+  std::vector<State> polled(1);
+  for (int passes = 0, chunk = first; st.phase != PH_DONE && passes <= c->maxiter + 2;) {
+    chunk = std::min(chunk, std::max(1, c->maxiter + 3 - passes));
     for (int k = 0; k < chunk; ++k) launch_drpa_pass(b, s);
     HIPCHK(hipGetLastError());
     passes += chunk;
     rc = read_state(c, &st);
     if (rc) return rc;
+    polled[0] = st;
+    if (st.phase != PH_DONE) chunk = next_chunk(&c, 1, polled, prev_dt, chunk, first, cap);
   }
   const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   c->last_niter = st.niter;  // the (E, dT) lines uegccd_trace returns

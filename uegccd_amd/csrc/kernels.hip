@@ -1431,35 +1431,71 @@ int connectivity_on_device(const BatchDev& b, int nvalid, int nbins, int* hist, 
 //              - 1/2 (Jvv(a) + Jvv(b) + Joo(i) + Joo(j)) T(i,j,a)
 // with the mosaic sums (source/dRPA.f90:420-436) Jvv(c) = sum_k 2 v(n_c - n_k)
 // R(k,c) and Joo(k) = sum_c 2 v(n_c - n_k) R(k,c).  Work buffers: R in rsum,
-// the Joo parts in joo_part, Jvv in jvv, Q in lad (nqb*o doubles <= 2*n3).
+// Joo in joo, Jvv in jvv, Q in lad (nqb*o doubles <= n3), and the transposed
+// amplitudes tT(j,i,a) = T(i,j,a) in tv (n3 doubles), which make the sums
+// over the first index contiguous reads.
 
-// Block a < v (one wave): R(:,a), the Joo parts and Jvv(a) of slab a.
-// Block v + q: Q(:, q) of transfer q.
+__device__ inline GlobalPtr<double> drpa_tt(const SolveG& S) {
+  return reinterpret_cast<GlobalPtr<double>>(S.tv);
+}
+
+// tT of the starting amplitudes (every position, zeros included), once per solve.
+__global__ __launch_bounds__(256) void k_drpa_transpose(const SolveDev* __restrict__ solves) {
+  const SolveG& S = as_global(solves)[0];
+  const int o = S.o;
+  const int64_t n3 = (int64_t)o * o * S.v, e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n3) return;
+  const int j = (int)(e % o), i = (int)((e / o) % o);
+  const int64_t slab = e / ((int64_t)o * o);
+  drpa_tt(S)[e] = S.tcur[(size_t)i + (size_t)o * j + (size_t)o * o * slab].x;
+}
+
+// Block g*o + k (g < kDrpaJooParts, one wave): the part of Joo(k) from the
+// slabs c of range g, sum over (c, j) of 2 v(n_c - n_k) tT(j,k,c), the lanes
+// taking the pairs in turn.  Block P*o + a: R(:,a) and Jvv(a) of slab a.
+// Block P*o + v + q: Q(:,q) of transfer q, lane j reading tT(j,l,d) of each
+// pair (l,d) of P(q).
 __global__ __launch_bounds__(64) void k_drpa_sums(const SolveDev* __restrict__ solves) {
   const SolveG& S = as_global(solves)[0];
   if (S.st->phase == PH_DONE) return;
-  const int o = S.o, o2 = o * o, lane = threadIdx.x;
+  const int o = S.o, o2 = o * o, lane = threadIdx.x, nj = kDrpaJooParts * o;
   const auto T = S.tcur;
-  if ((int)blockIdx.x < S.v) {
-    const int av = blockIdx.x;
-    double r = 0.0;
-    if (lane < o)
-      for (int k = 0; k < o; ++k) r += T[(size_t)lane + o * k + (size_t)o2 * av].x;
-    const double p = lane < o ? 2.0 * S.vso[av * o + lane] * r : 0.0;
-    if (lane < o) {
-      S.rsum[av * o + lane] = r;
-      S.joo_part[av * o + lane] = p;
+  const auto tT = drpa_tt(S);
+  int blk = blockIdx.x;
+  if (blk < nj) {
+    if (!S.do_mosaic) return;
+    const int k = blk % o, g = blk / o;
+    const int c0 = (int)((int64_t)S.v * g / kDrpaJooParts), c1 = (int)((int64_t)S.v * (g + 1) / kDrpaJooParts);
+    const int cnt = (c1 - c0) * o;
+    double p = 0.0;
+#pragma unroll 4
+    for (int m = lane; m < cnt; m += 64) {
+      const int c = c0 + m / o, j = m - (m / o) * o;
+      p += S.vso[c * o + k] * tT[(size_t)j + (size_t)o * k + (size_t)o2 * c];
     }
-    const double jv = wave_sum(p);
+    const double jo = wave_sum(p);
+    if (lane == 0) S.joo[g * o + k] = 2.0 * jo;
+    return;
+  }
+  blk -= nj;
+  if (blk < S.v) {
+    const int av = blk;
+    double r = 0.0;
+    if (lane < o) {
+#pragma unroll 4
+      for (int k = 0; k < o; ++k) r += T[(size_t)lane + o * k + (size_t)o2 * av].x;
+    }
+    if (lane < o) S.rsum[av * o + lane] = r;
+    const double jv = wave_sum(lane < o ? 2.0 * S.vso[av * o + lane] * r : 0.0);
     if (lane == 0) S.jvv[av] = jv;
   } else {
-    const int q = blockIdx.x - S.v;
+    const int q = blk - S.v;
     const int r0 = S.qb_roff[q], r1 = S.qb_roff[q + 1];
     double acc = 0.0;
     if (lane < o)
       for (int r = r0; r < r1; ++r) {
         const int pr = S.qb_rows[r], l = pr % o, dv = pr / o;
-        acc += T[(size_t)l + o * lane + (size_t)o2 * dv].x;
+        acc += tT[(size_t)lane + o * l + (size_t)o2 * dv];
       }
     if (lane < o) reinterpret_cast<GlobalPtr<double>>(S.lad)[(size_t)q * o + lane] = acc;
   }
@@ -1471,9 +1507,9 @@ __global__ __launch_bounds__(64) void k_drpa_sums(const SolveDev* __restrict__ s
 // residual check, 100-103) the amplitudes stay.
 __global__ __launch_bounds__(kElemThreads) void k_drpa_main(const SolveDev* __restrict__ solves) {
   __shared__ double red[(kElemThreads / 64) * 32];
-  __shared__ double joo_l[kMaxOcc];
   const SolveG& S = as_global(solves)[0];
   const int phase = S.st->phase;
+  if (phase == PH_DONE) return;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int o = S.o, cp = lb * kElemPerBlock + threadIdx.x;
   int x = 0, b = -1;
@@ -1482,14 +1518,13 @@ __global__ __launch_bounds__(kElemThreads) void k_drpa_main(const SolveDev* __re
     x = (int)cm.x;
     b = elem_partner(cm);
   }
-  if (S.do_mosaic)
-    for (int k = threadIdx.x; k < o; k += blockDim.x) {
-      double a = 0.0;
-      for (int c = 0; c < S.v; ++c) a += S.joo_part[c * o + k];
-      joo_l[k] = a;
-    }
+  __shared__ double joo_l[kMaxOcc];
+  if (S.do_mosaic && (int)threadIdx.x < o) {
+    double t = 0.0;
+    for (int g = 0; g < kDrpaJooParts; ++g) t += S.joo[g * o + threadIdx.x];
+    joo_l[threadIdx.x] = t;
+  }
   __syncthreads();
-  if (phase == PH_DONE) return;
   const bool lead = cp < S.nvalid;
   double e_part = 0.0, dt_part = 0.0, r_part = 0.0, bad = 0.0;
   if (lead) {
@@ -1515,6 +1550,7 @@ __global__ __launch_bounds__(kElemThreads) void k_drpa_main(const SolveDev* __re
       tn = (g / den - (1.0 - df) * t) / df;
       dt_part = fabs(tn - t);
       S.tcur[x] = make_double2(tn, 0.0);
+      drpa_tt(S)[(size_t)j + (size_t)o * i + (size_t)o * o * av] = tn;
     }
     e_part = vq * tn * 2;
     if (!isfinite(tn) || !isfinite(res)) bad = 1.0;
@@ -1744,6 +1780,10 @@ void launch_mp2(const BatchDev& b, hipStream_t s, int drpa) {
                      b.elem_prefix, b.n, drpa);
 }
 
+void launch_drpa_start(const BatchDev& b, int64_t n3, hipStream_t s) {
+  hipLaunchKernelGGL(k_drpa_transpose, dim3((unsigned)std::max<int64_t>(1, (n3 + 255) / 256)), dim3(256), 0, s,
+                     b.solves);
+}
 void launch_drpa_pass(const BatchDev& b, hipStream_t s) {
   // a batch of one: the sums over its slabs and transfers, then its elements
   hipLaunchKernelGGL(k_drpa_sums, dim3(b.drpa_sum_blocks), dim3(64), 0, s, b.solves);

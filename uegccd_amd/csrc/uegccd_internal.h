@@ -157,7 +157,7 @@ struct BatchDev {
   int* lad_prefix;      // [n+1] ladder tiles of each solve (one block each)
   int nlad;
   int elem_blocks;
-  int drpa_sum_blocks;               // k_drpa_sums of a batch of one: v + nqb
+  int drpa_sum_blocks;               // k_drpa_sums of a batch of one: kDrpaJooParts*o + v + nqb
   int slice_blocks;                  // over all classes
   int slice_cls_blocks[kSliceClasses];
   int slice_lds[kSliceClasses];      // dynamic LDS bytes (largest NOcc of the class)
@@ -170,6 +170,8 @@ struct BatchDev {
 // the mosaic Joo sums of one solve.
 enum ContractKind : int { CK_RING = 0, CK_HH = 1, CK_JOO = 2, CK_XRING = 3 };
 constexpr int kJooParts = 32;  // slice ranges the mosaic Joo sum is split over (one wave each)
+constexpr int kDrpaJooParts = 8;  // slab ranges of the dRPA Joo sum (one wave each)
+static_assert(kDrpaJooParts <= kJooParts, "the dRPA Joo parts fit the joo buffer");
 constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
 inline int slice_lds_bytes(int o) { return (3 * o * o + 4 * o) * 8; }
@@ -189,6 +191,7 @@ void launch_init(const BatchDev& b, hipStream_t s, int drpa = 0);
 void launch_mp2(const BatchDev& b, hipStream_t s, int drpa = 0);
 // One dRPA pass (source/dRPA.f90:80-96) of a batch-of-one: the row and
 // transfer sums, the per-element update, the reductions.
+void launch_drpa_start(const BatchDev& b, int64_t n3, hipStream_t s);
 void launch_drpa_pass(const BatchDev& b, hipStream_t s);
 // Events timing one kernel of a pass (which: as launch_one, or 4 for the
 // reductions and DIIS solve): `after` is written when it completes, `before`

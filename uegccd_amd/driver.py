@@ -386,7 +386,21 @@ def DrvCCD(heg: HEGData, info: UEGInfo, T2aaaa, T2abab, T2abba, opts: L.Options 
     return r
 
 
-# --- Output blocks, formats of source/MP2.f90:143-150 and source/CCD.f90:127-140 ---
+def DrvdRPA(heg: HEGData, info: UEGInfo, T2, opts: L.Options | None = None,
+            ctx: CCDContext | None = None):
+    """DrvdRPA (source/dRPA.f90:23-143) in place from the spin-summed T2 (the
+    reference passes T2aaaa); sets heg.ECorr as the reference leaves it
+    (Ec(SOSEX) after a converged run, else Ec(dRPA)); returns (SolveResult,
+    Ec(SOSEX))."""
+    c = ctx or CCDContext(heg, info, opts)
+    r, esosex = c.drpa(T2, allow_unconverged=True)
+    T2[...] = r.tab
+    heg.ECorr = r.ecorr if r.failed else esosex
+    return r, esosex
+
+
+# --- Output blocks, formats of source/MP2.f90:143-150, source/CCD.f90:127-140
+# and source/dRPA.f90:122-139 ---
 _STAR = " " * 14 + "*" * 50
 _DASH = " " * 14 + "*" + "-" * 48 + "*"
 
@@ -429,17 +443,43 @@ def format_ccd_block(heg: HEGData, info: UEGInfo, r: SolveResult) -> str:
     return "\n".join(out) + "\n"
 
 
+def format_drpa_block(heg: HEGData, info: UEGInfo, r: SolveResult, esosex: float) -> str:
+    out = [" " * 14 + "*              dRPA summary follows              *", _DASH]
+    if info.DoRing:
+        out.append(" " * 14 + "* Including ring diagrams...                     *")
+    if info.DoMosaic:
+        out.append(" " * 14 + "* Including mosaic diagrams...                   *")
+    out += [_DASH, " " * 14 + "* dRPA Energy      Iteration    Biggest change   *"]
+    for k in range(r.niter):
+        out.append(" " * 14 + f"* {r.trace_e[k]:15.10f}  {k + 1:5d}{'':7}{r.trace_dt[k]:14.10f}    *")
+    out.append(_DASH)
+    if r.failed:
+        out.append(" " * 14 + "* Final dRPA Energy is   DID NOT CONVERGE        *")
+    else:
+        out.append(" " * 14 + f"* dRPA has converged in {r.niter:3d} iterations{'':11}*")
+        out.append(" " * 14 + f"* Ec(dRPA) is {'':9}{r.ecorr:18.12f} a.u.   *")
+        out.append(" " * 14 + f"* Final dRPA Energy is {r.ecorr + heg.EHF:18.12f} a.u.   *")
+        out.append(" " * 14 + f"* Ec(SOSEX) is {'':8}{esosex:18.12f} a.u.   *")
+        out.append(" " * 14 + f"* Final SOSEX Energy is{esosex + heg.EHF:18.12f} a.u.   *")
+    out.append(" " * 14 + f"* Max CCD Residual is  {'':4}{r.maxres:18.12E}    *")
+    out.append(_STAR)
+    return "\n".join(out) + "\n"
+
+
 def single_point(input_text: str, opts: L.Options | None = None):
-    """Main's DoSingleCalc path (source/Main.f90:44-97): basis, MP2, CCD; returns
+    """Main's DoSingleCalc path (source/Main.f90:44-97): basis, MP2, then CCD
+    or, with DodRPASOSEX, dRPA/SOSEX on T2aaaa (source/Main.f90:84-92); returns
     (Output text, HEGData, E(2), SolveResult)."""
     info = parse_input(input_text)
-    if info.DodRPASOSEX:
-        raise L.UegccdError(L.ERR_UNSUPPORTED, "dRPA/SOSEX driver")
     heg = init_heg(info, device=0)
     ctx = CCDContext(heg, info, opts)
     taa, tab, tba = DrvMBPT(heg, info, ctx)
     e2 = heg.ECorr
     text = format_mp2_block(heg, e2)
-    r = DrvCCD(heg, info, taa, tab, tba, ctx=ctx)
-    text += format_ccd_block(heg, info, r)
+    if info.DodRPASOSEX:
+        r, esosex = DrvdRPA(heg, info, taa, ctx=ctx)
+        text += format_drpa_block(heg, info, r, esosex)
+    else:
+        r = DrvCCD(heg, info, taa, tab, tba, ctx=ctx)
+        text += format_ccd_block(heg, info, r)
     return text, heg, e2, r
