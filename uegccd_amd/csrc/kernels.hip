@@ -1489,14 +1489,20 @@ __global__ __launch_bounds__(64) void k_drpa_sums(const SolveDev* __restrict__ s
     const double jv = wave_sum(lane < o ? 2.0 * S.vso[av * o + lane] * r : 0.0);
     if (lane == 0) S.jvv[av] = jv;
   } else {
+    // the pairs of P(q) are loaded 64 at a time, one per lane, and each step
+    // takes its row offset by readlane, so the row loads do not wait on the
+    // pair list (the loop is uniform over the wave; lanes past o read row
+    // o - 1 and store nothing)
     const int q = blk - S.v;
-    const int r0 = S.qb_roff[q], r1 = S.qb_roff[q + 1];
+    const int r0 = S.qb_roff[q], r1 = S.qb_roff[q + 1], jl = min(lane, o - 1);
     double acc = 0.0;
-    if (lane < o)
-      for (int r = r0; r < r1; ++r) {
-        const int pr = S.qb_rows[r], l = pr % o, dv = pr / o;
-        acc += tT[(size_t)lane + o * l + (size_t)o2 * dv];
-      }
+    for (int c0 = r0; c0 < r1; c0 += 64) {
+      const int nr = min(64, r1 - c0);
+      const int pr = lane < nr ? S.qb_rows[c0 + lane] : 0;
+      const int off = o * (pr % o) + o2 * (pr / o);  // o*l + o^2*d of pair (l, d)
+#pragma unroll 8
+      for (int r = 0; r < nr; ++r) acc += tT[(size_t)jl + __builtin_amdgcn_readlane(off, r)];
+    }
     if (lane < o) reinterpret_cast<GlobalPtr<double>>(S.lad)[(size_t)q * o + lane] = acc;
   }
 }
