@@ -96,6 +96,8 @@ def _lib():
         lib.orc_ccd.argtypes = [P, ctypes.c_double, ctypes.c_int, D, D, D, D, D, I, D, D]
         lib.orc_solvecc.argtypes = [P, D, D]
         lib.orc_drpa_energy.restype = ctypes.c_double
+        lib.orc_sf.restype = ctypes.c_int
+        lib.orc_sf.argtypes = [P, D, D, D, ctypes.c_int, ctypes.POINTER(ctypes.c_int), D, D]
         lib.orc_drpa_energy.argtypes = [P, D, ctypes.c_double]
         lib.orc_drpa_terms.argtypes = [P, ctypes.c_int, D, D, D]
         lib.orc_drpa.restype = ctypes.c_int
@@ -175,6 +177,20 @@ class OracleSystem:
     def energy(self, taa, tab, tba):
         taa, tab, tba = (np.asfortranarray(x, dtype=np.float64) for x in (taa, tab, tba))
         return self._lib.orc_energy(self._h, _dp(taa), _dp(tab), _dp(tba))
+
+    def structure_factor(self, tab, tba):
+        """CCEnergyAndSF: (q [n,3] int, vals [n,5] = SfG, SfSum, SfV, SfDSum,
+        SfXSum, energy), rows in the reference's print order."""
+        tab = np.asfortranarray(tab, dtype=np.float64)
+        tba = np.asfortranarray(tba, dtype=np.float64)
+        taa = np.asfortranarray(tab + tba)
+        e = ctypes.c_double(0.0)
+        n = self._lib.orc_sf(self._h, _dp(taa), _dp(tab), _dp(tba), 0, None, None, ctypes.byref(e))
+        q = np.zeros((max(n, 1), 3), dtype=np.int32)
+        vals = np.zeros((max(n, 1), 5))
+        self._lib.orc_sf(self._h, _dp(taa), _dp(tab), _dp(tba), n,
+                         q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(vals), ctypes.byref(e))
+        return q[:n], vals[:n], e.value
 
     def terms(self, tab, tba, mask=31):
         """G and R for the given amplitudes (TermsWrapper)."""

@@ -395,6 +395,75 @@ double orc_energy(const orc_system* s, const double* taaaa, const double* tabab,
     return ecorr;
 }
 
+/* CCEnergyAndSF, source/CCD.f90:240-351 (with dK and dK2scaled,
+   source/HEG.f90:1012-1054, and SendForAllocation, 1056-1070): the energy
+   and the five grids over q = n_a - n_i (direct) and n_a - n_j (exchange) of
+   the valid elements, written in the reference's print order (q_x slowest)
+   for the cells with SfG > 1e-10.  Rows k < cap go to q[3k..] and
+   vals[5k..] = (SfG, SfSum, SfV, SfDSum, SfXSum); returns the row count. */
+int orc_sf(const orc_system* s, const double* taaaa, const double* tabab, const double* tabba,
+           int cap, int* q, double* vals, double* ecorr) {
+    const int o = s->nocc, n = s->nao;
+    const int fl = s->p.irange[R_ENERGY];
+    int mx = s->nvec[0];
+    for (int k = 0; k < 3 * n; ++k) mx = s->nvec[k] > mx ? s->nvec[k] : mx;
+    const int len = mx * 2 + 1, w = 2 * len + 1;
+    const size_t cells = (size_t)w * w * w;
+    double* g[5];
+    for (int k = 0; k < 5; ++k) g[k] = (double*)calloc(cells, sizeof(double));
+    enum { SUM, G, V, DSUM, XSUM };
+    const double sc = pow(2.0 * 3.14159265358979323846 / s->L, 2.0);
+    double e = 0.0;
+    for (int i = 1; i <= o; ++i)
+        for (int j = 1; j <= o; ++j)
+            for (int a = o + 1; a <= n; ++a) {
+                int b = find_index(s, i, j, a);
+                if (b <= o) continue;
+                double vab = eri(s, a, b, i, j, fl);
+                double vba = eri(s, a, b, j, i, fl);
+                if (fmin(vab, vba) < -80.0) continue;
+                size_t x = IDX(i, j, a);
+                double de = vab * (taaaa[x] + tabab[x]) - vba * (taaaa[x] + tabba[x]);
+                e = e + 0.5 * de;
+                for (int ex = 0; ex < 2; ++ex) {
+                    const int r = ex ? j : i;  /* dK(a,b,i,j) or dK(a,b,j,i) */
+                    int qv[3], q2 = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        qv[d] = s->nvec[(a - 1) * 3 + d] - s->nvec[(r - 1) * 3 + d];
+                        q2 += qv[d] * qv[d];
+                    }
+                    const size_t c = ((size_t)(qv[0] + len) * w + (qv[1] + len)) * w + (qv[2] + len);
+                    if (!ex) {
+                        g[SUM][c] += taaaa[x] + tabab[x];
+                        g[DSUM][c] += 2.0 * tabab[x];
+                        g[XSUM][c] += taaaa[x] - tabab[x];
+                        g[V][c] = vab;
+                    } else {
+                        g[SUM][c] -= taaaa[x] + tabba[x];
+                        g[DSUM][c] -= 2.0 * tabba[x];
+                        g[XSUM][c] -= taaaa[x] - tabba[x];
+                        g[V][c] = vba;
+                    }
+                    g[G][c] = sqrt(q2 * sc);
+                }
+            }
+    int rows = 0;
+    for (size_t c = 0; c < cells; ++c) {
+        if (!(g[G][c] > 1.0e-10)) continue;
+        if (rows < cap) {
+            q[3 * rows] = (int)(c / ((size_t)w * w)) - len;
+            q[3 * rows + 1] = (int)((c / w) % w) - len;
+            q[3 * rows + 2] = (int)(c % w) - len;
+            double* r = vals + 5 * (size_t)rows;
+            r[0] = g[G][c]; r[1] = g[SUM][c]; r[2] = g[V][c]; r[3] = g[DSUM][c]; r[4] = g[XSUM][c];
+        }
+        ++rows;
+    }
+    for (int k = 0; k < 5; ++k) free(g[k]);
+    if (ecorr) *ecorr = e;
+    return rows;
+}
+
 /* GetG2Drivers, source/CCD.f90:465-504. */
 static void g2_drivers(const orc_system* s, double* gab, double* gba) {
     const int o = s->nocc, n = s->nao;

@@ -56,6 +56,8 @@ double orc_eri(const orc_system* s, int a, int b, int i, int j, int flag);
 double orc_mp2(const orc_system* s, double* taaaa, double* tabab, double* tabba);
 
 /* CCEnergy (source/CCD.f90:186-237). */
+int    orc_sf(const orc_system* s, const double* taaaa, const double* tabab, const double* tabba,
+              int cap, int* q, double* vals, double* ecorr);
 double orc_energy(const orc_system* s, const double* taaaa, const double* tabab,
                   const double* tabba);
 

@@ -278,3 +278,54 @@ def test_drpa_reduced_form(fixtures, mosaic):
     gr = _drpa_reduced(s, t2)
     assert np.abs(g).max() > 0.1
     np.testing.assert_allclose(gr, g, rtol=0, atol=1e-14 * np.abs(g).max())
+
+
+@pytest.mark.parametrize("amps", ["mp2", "random"])
+def test_structure_factor_identities(fixtures, amps):
+    """CCEnergyAndSF (source/CCD.f90:240-351) restated in the oracle, pinned by
+    what the binning fixes: the cells are exactly the transfers n_a - n_i and
+    n_a - n_j of the valid elements (brute force over the n-vectors), in the
+    reference's print order; SfG = 2 pi |q| / L and SfV = v(q) = 1/(pi L |q|^2)
+    in closed form; sum_q SfV SfSum = 2 E (the energy pinned to the reference's
+    Output); sum_q SfV SfDSum and sum_q SfV SfXSum equal the element sums
+    2 (v_ia T^ab - v_ja T^ba) and v_ia T^ba - v_ja T^ab; SfDSum + SfXSum =
+    SfSum cell by cell."""
+    c = fixtures["n14_cut5"]
+    s = _sys(c["params"])
+    _, taa, tab, tba = s.mp2()
+    if amps == "random":
+        rng = np.random.default_rng(11)
+        mask = taa != 0.0
+        tab = np.where(mask, rng.standard_normal(taa.shape) * 0.05, 0.0)
+        tba = np.where(mask, rng.standard_normal(taa.shape) * 0.05, 0.0)
+        taa = tab + tba
+    q, vals, e = s.structure_factor(tab, tba)
+    o = s.nocc
+    nvec = s.nvec.astype(int)
+    idx = {tuple(nvec[p]): p for p in range(s.nao)}
+    cells = set()
+    sd = sx = 0.0
+    for i in range(o):
+        for j in range(o):
+            for a in range(o, s.nao):
+                if idx.get(tuple(nvec[i] + nvec[j] - nvec[a]), -1) < o:
+                    continue
+                qi, qj = tuple(nvec[a] - nvec[i]), tuple(nvec[a] - nvec[j])
+                cells.add(qi)
+                cells.add(qj)
+                vi, vj = 1 / (np.pi * s.L * np.dot(qi, qi)), 1 / (np.pi * s.L * np.dot(qj, qj))
+                x = (i, j, a - o)
+                sd += 2 * (vi * tab[x] - vj * tba[x])
+                sx += vi * tba[x] - vj * tab[x]
+    assert [tuple(r) for r in q] == sorted(cells)
+    q2 = (q.astype(float) ** 2).sum(axis=1)
+    np.testing.assert_allclose(vals[:, 0], 2 * np.pi / s.L * np.sqrt(q2), rtol=1e-15)
+    np.testing.assert_allclose(vals[:, 2], 1 / (np.pi * s.L * q2), rtol=1e-14)
+    e_ref = s.energy(taa, tab, tba)
+    assert abs(e - e_ref) < 1e-15
+    assert abs(vals[:, 2] @ vals[:, 1] - 2 * e_ref) < 1e-13
+    assert abs(vals[:, 2] @ vals[:, 3] - sd) < 1e-13
+    assert abs(vals[:, 2] @ vals[:, 4] - sx) < 1e-13
+    np.testing.assert_allclose(vals[:, 3] + vals[:, 4], vals[:, 1], rtol=0, atol=1e-15)
+    if amps == "mp2":
+        assert abs(e_ref - c["e2"]) < 1e-10
