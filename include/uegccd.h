@@ -133,6 +133,19 @@ int uegccd_residual(uegccd_ctx* ctx, const double* t2abab, const double* t2abba,
 /* CCEnergy (source/CCD.f90:186-237). */
 int uegccd_energy(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, double* ecorr);
 
+/* Structure factor of the amplitudes (host arrays, the layout above): replaces
+ * CCEnergyAndSF (source/CCD.f90:240-351), which the reference calls on the MP2
+ * guess (source/CCD.f90:65, DoSFCalcMP2) and on the converged amplitudes
+ * (source/CCD.f90:109, DoSFCalcCCD).  One row per momentum transfer q = n_a - n_i
+ * or n_a - n_j of a valid element, in the reference's print order (q_x, then q_y,
+ * then q_z ascending): qvec[3k..3k+2] = q and vals[5k..5k+4] = (SfG = 2 pi |q| / L,
+ * SfSum, SfV = v(q), SfDSum, SfXSum).  With qvec or vals NULL, *ncell receives the
+ * row count and nothing is computed; otherwise *ncell is the capacity in rows on
+ * entry (UEGCCD_ERR_INVALID if too small) and the row count on return.  *ecorr,
+ * when given, receives the energy the reference routine also sets. */
+int uegccd_structure_factor(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, int32_t* ncell,
+                            int32_t* qvec, double* vals, double* ecorr);
+
 /* DrvCCD (source/CCD.f90:18-183): iterate from the given amplitudes (the MP2
  * guess in the reference) to convergence.  t2aaaa is written as
  * t2abab + t2abba on output (may be NULL). */

@@ -86,6 +86,7 @@ struct uegccd_ctx {
   int o = 0, v = 0, nao = 0, n3 = 0, nK = 0, n2max = 0, wtot = 0, maxiter = 1000;
   int64_t nvalid = 0;
   double tolmax = 1e-8, failratio = 100, df = 2.0;
+  double boxL = 0.0;  // box length L
   // host copies of the maps
   std::vector<int> kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
   std::vector<int> kb_toff;        // K-blocked amplitude layout: block offsets
@@ -124,6 +125,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   c->o = o;
   c->v = v;
   c->nao = nao;
+  c->boxL = s.L;
   c->n3 = o * o * v;
   c->nv4.resize(nao);
   int mx = 0;
@@ -780,6 +782,66 @@ int uegccd_energy(uegccd_ctx* c, const double* tab, const double* tba, double* e
   rc = read_state(c, &st);
   if (rc) return rc;
   *ecorr = st.ecorr;
+  return st.nonfinite ? UEGCCD_ERR_NONFINITE : UEGCCD_OK;
+}
+
+int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba, int32_t* ncell,
+                            int32_t* qvec, double* vals, double* ecorr) {
+  if (!c || !ncell) return UEGCCD_ERR_INVALID;
+  const int o = c->o, nq = (int)c->qb_neg.size();
+  // the cells printed: transfers n_a - n_i or n_a - n_j of a valid element
+  std::vector<char> touched(nq, 0);
+  for (const uint2& m : c->cmap) {
+    const int x = (int)m.x, i = x % o, j = (x / o) % o, av = x / (o * o);
+    touched[c->qb_of[(size_t)av * o + i]] = 1;
+    touched[c->qb_of[(size_t)av * o + j]] = 1;
+  }
+  struct Cell { int x, y, z, q; };
+  std::vector<Cell> cells;
+  for (int q = 0; q < nq; ++q) {
+    if (!touched[q]) continue;
+    const int r = c->qb_rows[c->qb_roff[q]], l = r % o, dv = r / o;
+    const int4 na = c->nv4[o + dv], nl = c->nv4[l];
+    cells.push_back({na.x - nl.x, na.y - nl.y, na.z - nl.z, q});
+  }
+  std::sort(cells.begin(), cells.end(), [](const Cell& a, const Cell& b) {
+    return a.x != b.x ? a.x < b.x : a.y != b.y ? a.y < b.y : a.z < b.z;
+  });
+  const int n = (int)cells.size();
+  if (!qvec || !vals) {  // size query
+    *ncell = n;
+    return UEGCCD_OK;
+  }
+  if (!tab || !tba || *ncell < n) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = copy_in_amplitudes(c, tab, tba);
+  if (rc) return rc;
+  launch_state_reset(c->batch, PH_ITER, 1, c->stream);
+  launch_init(c->batch, c->stream);
+  launch_finalize(c->batch, FIN_SUM, c->stream);
+  launch_sf(c->batch, c->v, nq, c->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> sums(3 * (size_t)std::max(1, nq));
+  HIPCHK(hipMemcpyAsync(sums.data(), c->sd.jxring, sizeof(double) * 3 * nq, hipMemcpyDeviceToHost, c->stream));
+  State st;
+  rc = read_state(c, &st);
+  if (rc) return rc;
+  const double sc = std::pow(2.0 * M_PI / c->boxL, 2.0);  // dK2scaled, source/HEG.f90:1025
+  for (int k = 0; k < n; ++k) {
+    const Cell& e = cells[k];
+    const int r = c->qb_rows[c->qb_roff[e.q]], l = r % o, dv = r / o;
+    qvec[3 * k] = e.x;
+    qvec[3 * k + 1] = e.y;
+    qvec[3 * k + 2] = e.z;
+    double* out = vals + 5 * (size_t)k;
+    out[0] = std::sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z) * sc);
+    out[1] = sums[3 * (size_t)e.q];
+    out[2] = c->vso[(size_t)dv * o + l];  // v(n_a - n_l) = v(q)
+    out[3] = sums[3 * (size_t)e.q + 1];
+    out[4] = sums[3 * (size_t)e.q + 2];
+  }
+  *ncell = n;
+  if (ecorr) *ecorr = st.ecorr;
   return st.nonfinite ? UEGCCD_ERR_NONFINITE : UEGCCD_OK;
 }
 

@@ -1565,6 +1565,58 @@ __global__ __launch_bounds__(kElemThreads) void k_drpa_main(const SolveDev* __re
 }
 
 // --------------------------------------------------------------------------
+// Structure factor (CCEnergyAndSF, source/CCD.f90:240-351).  Each valid
+// element (i,j,a) adds to the cell q = n_a - n_i (direct) and subtracts at
+// q = n_a - n_j (exchange); with T^aa = T^ab + T^ba the three sums are
+//   Sum: +(2 T^ab + T^ba), -(T^ab + 2 T^ba)   DSum: +2 T^ab, -2 T^ba
+//   XSum: +T^ba, -T^ab.
+// So a pair (l,d) of the ring sets P(q) = {(l,d) : n_d - n_l = q} brings its
+// row sums over j (direct) less its column sums over i (exchange), and a cell
+// is the sum over its pairs.  Invalid amplitudes are zero.  Scratch: the pair
+// sums in jring (3 o v doubles), the cell sums in jxring (3 nq doubles).
+
+// Block a (one wave, lane l < o): the three sums of pair (l, a).
+__global__ __launch_bounds__(64) void k_sf_pairs(const SolveDev* __restrict__ solves) {
+  const SolveG& S = as_global(solves)[0];
+  const int o = S.o, o2 = o * o, av = blockIdx.x, l = threadIdx.x;
+  if (l >= o) return;
+  const auto T = S.tcur;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = 0; k < o; ++k) {
+    const double2 d = T[(size_t)l + o * k + (size_t)o2 * av];  // (l,k,a): the direct row of l
+    const double2 x = T[(size_t)k + o * l + (size_t)o2 * av];  // (k,l,a): the exchange column of l
+    s0 += (2.0 * d.x + d.y) - (x.x + 2.0 * x.y);
+    s1 += 2.0 * d.x - 2.0 * x.y;
+    s2 += d.y - x.x;
+  }
+  const size_t w = 3 * ((size_t)av * o + l);
+  S.jring[w] = s0;
+  S.jring[w + 1] = s1;
+  S.jring[w + 2] = s2;
+}
+
+// Block q (one wave): the cell sums of transfer q over the pairs of P(q).
+__global__ __launch_bounds__(64) void k_sf_cells(const SolveDev* __restrict__ solves) {
+  const SolveG& S = as_global(solves)[0];
+  const int q = blockIdx.x, r0 = S.qb_roff[q], r1 = S.qb_roff[q + 1];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int r = r0 + (int)threadIdx.x; r < r1; r += 64) {
+    const size_t w = 3 * (size_t)S.qb_rows[r];  // row l + o*d of pair (l, d)
+    a0 += S.jring[w];
+    a1 += S.jring[w + 1];
+    a2 += S.jring[w + 2];
+  }
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  a2 = wave_sum(a2);
+  if (threadIdx.x == 0) {
+    S.jxring[3 * (size_t)q] = a0;
+    S.jxring[3 * (size_t)q + 1] = a1;
+    S.jxring[3 * (size_t)q + 2] = a2;
+  }
+}
+
+// --------------------------------------------------------------------------
 // Plane-wave basis on the device (init_HEG + SetupBasis + change_rs,
 // source/HEG.f90:150-324, 494-533, 581-615).  Every floating-point step is an
 // explicitly rounded operation in the host builder's order (no contraction
@@ -1786,6 +1838,10 @@ void launch_mp2(const BatchDev& b, hipStream_t s, int drpa) {
                      b.elem_prefix, b.n, drpa);
 }
 
+void launch_sf(const BatchDev& b, int v, int nq, hipStream_t s) {
+  hipLaunchKernelGGL(k_sf_pairs, dim3(v), dim3(64), 0, s, b.solves);
+  hipLaunchKernelGGL(k_sf_cells, dim3(nq), dim3(64), 0, s, b.solves);
+}
 void launch_drpa_start(const BatchDev& b, int64_t n3, hipStream_t s) {
   hipLaunchKernelGGL(k_drpa_transpose, dim3((unsigned)std::max<int64_t>(1, (n3 + 255) / 256)), dim3(256), 0, s,
                      b.solves);

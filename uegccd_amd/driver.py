@@ -174,6 +174,24 @@ class CCDContext:
                 raise L.UegccdError(st, "uegccd_solve")
         return SolveResult.from_c(r, taa, tab, tba, self.trace(r.niter))
 
+    def structure_factor(self, tab, tba):
+        """uegccd_structure_factor: (q [n,3] int, vals [n,5] = SfG, SfSum, SfV,
+        SfDSum, SfXSum, energy) of the given amplitudes (CCEnergyAndSF,
+        source/CCD.f90:240-351)."""
+        tab = np.asfortranarray(tab, dtype=np.float64)
+        tba = np.asfortranarray(tba, dtype=np.float64)
+        n = ctypes.c_int32(0)
+        L.check(self._lib.uegccd_structure_factor(self._h, None, None, ctypes.byref(n), None, None, None),
+                "uegccd_structure_factor")
+        q = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+        vals = np.zeros((max(n.value, 1), 5))
+        e = ctypes.c_double(0.0)
+        L.check(self._lib.uegccd_structure_factor(
+            self._h, L.dptr(tab), L.dptr(tba), ctypes.byref(n),
+            q.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), L.dptr(vals), ctypes.byref(e)),
+            "uegccd_structure_factor")
+        return q[:n.value], vals[:n.value], e.value
+
     def drpa(self, t2=None, allow_unconverged=False):
         """DrvdRPA (source/dRPA.f90:23-143) on this basis: the ring-only CCD on
         the spin-summed T2 (default start: the MP2 T2aaaa), iterated as the
@@ -375,15 +393,33 @@ def DrvMBPT(heg: HEGData, info: UEGInfo, ctx: CCDContext | None = None):
 
 
 def DrvCCD(heg: HEGData, info: UEGInfo, T2aaaa, T2abab, T2abba, opts: L.Options | None = None,
-           ctx: CCDContext | None = None):
-    """Solve in place from the given amplitudes; sets heg.ECorr; returns SolveResult."""
+           ctx: CCDContext | None = None, sf_out: list | None = None):
+    """Solve in place from the given amplitudes; sets heg.ECorr; returns SolveResult.
+    With sf_out, the unit-80 structure-factor blocks the reference writes with
+    DoSFCalcMP2 (of the starting amplitudes, source/CCD.f90:65) and DoSFCalcCCD
+    (of the converged ones, source/CCD.f90:109) are appended to it."""
     c = ctx or CCDContext(heg, info, opts)
+    if sf_out is not None and info.DoSFCalcMP2:
+        sf_out.append(format_sf_block(*c.structure_factor(T2abab, T2abba)[:2]))
     r = c.solve(T2abab, T2abba, allow_unconverged=True)
+    if sf_out is not None and info.DoSFCalcCCD:
+        sf_out.append(format_sf_block(*c.structure_factor(r.tab, r.tba)[:2]))
     T2abab[...] = r.tab
     T2abba[...] = r.tba
     T2aaaa[...] = r.tab + r.tba
     heg.ECorr = r.ecorr
     return r
+
+
+def format_sf_block(q, vals) -> str:
+    """Unit 80 of CCEnergyAndSF (source/CCD.f90:265-267, 334-342): three header
+    lines, then one list-directed row I, J, A, SfG, SfSum, SfV, SfDSum, SfXSum
+    per cell (integers in 12 columns, reals to 17 significant digits; the
+    spacing of list-directed output is the compiler's)."""
+    out = [" --------", " Starting a new structure factor", " --------"]
+    for (x, y, z), r in zip(q, vals):
+        out.append(f"{x:12d}{y:12d}{z:12d}" + "".join(f"{v:26.17E}" for v in r))
+    return "\n".join(out) + "\n"
 
 
 def DrvdRPA(heg: HEGData, info: UEGInfo, T2, opts: L.Options | None = None,

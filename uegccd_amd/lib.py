@@ -29,6 +29,7 @@ EXPORTS = (
     "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
     "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
     "uegccd_heg_build_device", "uegccd_connectivity", "uegccd_drpa",
+    "uegccd_structure_factor",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -123,6 +124,8 @@ def load():
     lib.uegccd_trace.argtypes = [P, D, D, ctypes.c_int32]
     lib.uegccd_mp2.argtypes = [P, D, D, D, D]
     lib.uegccd_drpa.argtypes = [P, D, ctypes.POINTER(Result), D]
+    I32 = ctypes.POINTER(ctypes.c_int32)
+    lib.uegccd_structure_factor.argtypes = [P, D, D, I32, I32, D, D]
     lib.uegccd_heg_build.argtypes = [ctypes.POINTER(HegParams), I32, I32, D, D]
     lib.uegccd_heg_build_device.argtypes = [ctypes.POINTER(HegParams), ctypes.c_int32, I32, I32, D, D]
     lib.uegccd_strerror.argtypes = [ctypes.c_int]
@@ -152,7 +155,8 @@ def load():
                  "uegccd_kernel_time", "uegccd_footprint", "uegccd_server_create",
                  "uegccd_server_submit", "uegccd_server_step", "uegccd_server_drain",
                  "uegccd_server_poll", "uegccd_time_kernel", "uegccd_kernel_stats",
-                 "uegccd_heg_build_device", "uegccd_connectivity", "uegccd_drpa"):
+                 "uegccd_heg_build_device", "uegccd_connectivity", "uegccd_drpa",
+                 "uegccd_structure_factor"):
         getattr(lib, name).restype = ctypes.c_int
     _LIB = lib
     return lib
