@@ -212,6 +212,34 @@ def drpa_c2(lib, L, heg, info, CCDContext, options, reps, with_cpu):
     return out
 
 
+def sf_c2(heg, info, CCDContext, options, reps, with_cpu):
+    """Structure factor (uegccd_structure_factor, the reference's
+    CCEnergyAndSF) of the MP2 amplitudes of the same N=54 |n|^2<=16 system:
+    median time per call over `reps` calls with host amplitudes in and the
+    cells out (PCIe included), the number of cells; with with_cpu the serial
+    CPU oracle's restatement is timed beside it and the cells compared."""
+    ctx = CCDContext(heg, info, options(mode="reference"))
+    _, _, tab, tba = ctx.mp2()
+    ctx.structure_factor(tab, tba)
+    ms = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        q, vals, e = ctx.structure_factor(tab, tba)
+        ms.append((time.perf_counter() - t0) * 1000.0)
+    out = {"workload": "N=54 r_s=2.0 Gamma |n|^2<=16 structure factor of the MP2 amplitudes",
+           "ms_per_call": float(np.median(ms)), "cells": int(len(q)), "ecorr": e}
+    if with_cpu:
+        from tests.helpers import oracle_for
+        orc = oracle_for(info)
+        t0 = time.perf_counter()
+        qo, vo, eo = orc.structure_factor(tab, tba)
+        cpu_s = time.perf_counter() - t0
+        same = qo.shape == q.shape and bool((qo == q).all())
+        out.update({"cpu_oracle_s": cpu_s, "cpu_cores": 1, "same_cells_as_oracle": same,
+                    "max_abs_diff_vs_oracle": float(np.abs(vals - vo).max()) if same else None})
+    return out
+
+
 def twist_stream(lib, L, n, rate):
     from uegccd_amd.driver import CCDContext, Server, init_heg, options
     reqs, arrivals = stream_requests(n, 1234, rate)
@@ -370,6 +398,7 @@ def main():
     mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
     stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
     drpa = drpa_c2(lib, L, heg, info, CCDContext, options, 10, not args.no_cpu)
+    sf = sf_c2(heg, info, CCDContext, options, 10, not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
@@ -394,6 +423,7 @@ def main():
         "mixed_precision": mixed,
         "twist_stream": stream,
         "drpa": drpa,
+        "structure_factor": sf,
     }
     print(json.dumps(out))
 
