@@ -110,3 +110,36 @@ def test_zcode_drpa_reference_output(tmp_path, fixtures):
     assert o["ecorr"] == pytest.approx(fx["drpa_ecorr"], abs=1e-12)
     assert o["esosex"] == pytest.approx(fx["drpa_esosex"], abs=1e-12)
     assert o["maxres"] == pytest.approx(fx["drpa_maxres"], rel=1e-6)
+
+
+def test_zcode_structure_factor_unit80(tmp_path):
+    """DoSFCalcMP2 and DoSFCalcCCD: ZCode writes the two unit-80 blocks of
+    CCEnergyAndSF (source/CCD.f90:65,109), equal to the library's structure
+    factor of the MP2 guess and of the converged amplitudes."""
+    print("[zcode] n14 cut5 structure factor", flush=True)
+    from uegccd_amd.driver import CCDContext, init_heg, options
+    from uegccd_amd.inputfile import parse_input
+    text = standard_input(14, 5, 1.0, DoSFCalcMP2=True, DoSFCalcCCD=True)
+    run_zcode(tmp_path, text)
+    # list-directed records may wrap over several lines: a block is every
+    # number after its header, eight to a cell
+    blocks, cur = [], None
+    for line in (tmp_path / "fort.80").read_text().splitlines():
+        if "Starting" in line:
+            cur = []
+            blocks.append(cur)
+        elif "--------" not in line and cur is not None:
+            cur.extend(float(x) for x in line.split())
+    assert len(blocks) == 2
+    blocks = [np.array(b).reshape(-1, 8) for b in blocks]
+    info = parse_input(text)
+    ctx = CCDContext(init_heg(info), info, options(mode="reference"))
+    _, _, tab, tba = ctx.mp2()
+    expect = [ctx.structure_factor(tab, tba)]
+    r = ctx.solve(tab, tba)
+    expect.append(ctx.structure_factor(r.tab, r.tba))
+    for got, (q, vals, _) in zip(blocks, expect):
+        got = np.array(got)
+        assert got.shape == (len(q), 8)
+        assert (got[:, :3].astype(int) == q).all()
+        np.testing.assert_allclose(got[:, 3:], vals, rtol=0, atol=1e-13 * np.abs(vals).max())

@@ -70,9 +70,11 @@ Contains
 
     Write(6,*) 'Doing CCD...'
     If (.not. c_associated(Ctx)) Call SetUpSolver(HEGData, UEGInfo)
+    If (UEGInfo%DoSFCalcMP2) Call WriteSF(T2abab, T2abba)
     St = uegccd_solve(Ctx, T2aaaa, T2abab, T2abba, Res)
     If (St /= UEGCCD_OK .and. St /= UEGCCD_ERR_NOT_CONVERGED) Call Check(St, 'uegccd_solve')
     HEGData%ECorr = Res%ecorr
+    If (UEGInfo%DoSFCalcCCD) Call WriteSF(T2abab, T2abba)
     Allocate(E(Max(Res%niter,1)), DT(Max(Res%niter,1)))
     Call Check(uegccd_trace(Ctx, E, DT, Res%niter), 'uegccd_trace')
 
@@ -116,5 +118,30 @@ Contains
 1130  Format(14x,'* Including mosaic diagrams...                   *')
 2000  Format(14x,'* Final CCD Energy is   DID NOT CONVERGE         *')
   End Subroutine DrvCCD
+
+  ! CCEnergyAndSF (source/CCD.f90:240-351): the structure factor of the
+  ! amplitudes from the library, written to unit 80 in the reference's order
+  Subroutine WriteSF(T2abab, T2abba)
+    Real (Kind=pr), Intent(In) :: T2abab(*), T2abba(*)
+    Integer(c_int32_t), Allocatable, Target :: Q(:,:)
+    Real(c_double), Allocatable, Target :: V(:,:)
+    Integer(c_int32_t) :: N
+    Real(c_double) :: E
+    Integer :: K
+
+    N = 0
+    Call Check(uegccd_structure_factor(Ctx, T2abab, T2abba, N, c_null_ptr, c_null_ptr, E), &
+               'uegccd_structure_factor')
+    Allocate(Q(3, Max(N, 1)), V(5, Max(N, 1)))
+    Call Check(uegccd_structure_factor(Ctx, T2abab, T2abba, N, c_loc(Q), c_loc(V), E), &
+               'uegccd_structure_factor')
+    Write(80,*) "--------"
+    Write(80,*) "Starting a new structure factor"
+    Write(80,*) "--------"
+    Do K = 1, N
+      Write(80,*) Q(1,K), Q(2,K), Q(3,K), V(1,K), V(2,K), V(3,K), V(4,K), V(5,K)
+    End Do
+    Flush(80)
+  End Subroutine WriteSF
 
 End Module CCD

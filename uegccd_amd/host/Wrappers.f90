@@ -104,6 +104,16 @@ Module Wrappers
       Import :: c_ptr, c_int
       Integer(c_int), Value :: st
     End Function
+    Integer(c_int) Function uegccd_structure_factor(ctx, tab, tba, ncell, qvec, vals, ecorr) &
+        Bind(C, name="uegccd_structure_factor")
+      Import :: c_int, c_int32_t, c_ptr, c_double
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(In) :: tab(*), tba(*)
+      Integer(c_int32_t), Intent(InOut) :: ncell
+      Type(c_ptr), Value :: qvec, vals
+      Real(c_double), Intent(Out) :: ecorr
+    End Function
+
   End Interface
 
 Contains
