@@ -77,6 +77,9 @@ typedef struct {
     int32_t safe_eri;      /* UEGInfo%SafeERI                                */
     int32_t irange[15];    /* IRange* in Input-key order (source/IO.f90:88-102) */
     int32_t do_ring, do_xring, do_ladder, do_mosaic;
+    double screening;      /* HEGData%ScreeningDistance (source/HEG.f90:512);
+                              read only when a range flag selects the short- or
+                              long-range kernel (flags 1, 2 under CasedERI)  */
 } uegccd_system;
 
 typedef struct {

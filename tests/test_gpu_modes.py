@@ -24,8 +24,10 @@ def _ctx(info, **opt):
     return CCDContext(heg, info, options(**opt))
 
 
-@pytest.mark.parametrize("name", ["n14_cut5", "n54_cut9", "n38_cut9"])
+@pytest.mark.parametrize("name", ["n14_cut5", "n54_cut9", "n38_cut9", "n54_cut16"])
 def test_mixed_precision_energy(name):
+    """n54_cut16 is the mixed-precision workload itself: N=54, r_s=2, |n|^2<=16,
+    FP32 ladders while max|R| >= 1e-5, then FP64."""
     print(f"[mixed] {name}", flush=True)
     info = info_for(name)
     c64 = _ctx(info, mode="diis", ndiis=8)

@@ -140,3 +140,52 @@ def test_admission_respects_budget_and_refuses_oversize():
     with pytest.raises(L.UegccdError) as e:
         Server().poll(12345)
     assert e.value.status == L.ERR_NOT_FOUND
+
+
+def test_full_twist_stream_poisson():
+    """The whole served twist-averaging workload: 256 requests of the seeded
+    stream (seed 1234; N 14/38/54, r_s 0.5-5, cutoffs 9/16/25, twists in
+    [-0.5, 0.5)^3) submitted to the server at their Poisson arrival times
+    (700/s), mixed precision.  Every request converges, each served energy
+    equals the same request solved alone in mixed precision (1e-12 Ha, same
+    NIter), and each is within 1e-6 Ha of the request's plain FP64 solve."""
+    import time
+    from uegccd_amd.driver import CCDContext, Server, init_heg, options
+    from uegccd_amd.twist_stream import request_info, stream_requests
+    reqs, arrivals = stream_requests(256, 1234, 700.0)
+    infos = [request_info(q) for q in reqs]
+    omx = options(mode="diis", ndiis=8, precision="mixed")
+    o64 = options(mode="diis", ndiis=8)
+    alone_mx, alone_64 = [], []
+    print("[stream256] alone solves", flush=True)
+    for info in infos:
+        heg = init_heg(info)
+        for opts, out in ((omx, alone_mx), (o64, alone_64)):
+            c = CCDContext(heg, info, opts)
+            _, _, tab, tba = c.mp2()
+            out.append(c.solve(tab, tba))
+            c.close()
+    print("[stream256] served at Poisson arrivals", flush=True)
+    srv = Server(max_active=64, iters_per_step=4)
+    ids, i, t0 = [], 0, time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while i < len(infos) and arrivals[i] <= now:
+            ids.append(srv.submit(infos[i], omx))
+            i += 1
+        na, nq = srv.step()
+        if i == len(infos) and na == 0 and nq == 0:
+            break
+        if na == 0 and nq == 0:
+            time.sleep(max(0.0, min(arrivals[i] - (time.perf_counter() - t0), 0.001)))
+    out = [srv.poll(k) for k in ids]
+    srv.close()
+    assert len(out) == 256
+    overlap, _ = _overlaps(out)
+    assert overlap > 1  # requests that arrived while others ran shared the device
+    for q, mx, e64, s in zip(reqs, alone_mx, alone_64, out):
+        assert mx.status == 0 and e64.status == 0, q
+        assert s.state == L.REQ_DONE and s.status == 0, (q, s)
+        assert s.niter == mx.niter and s.iters_fp32 == mx.iters_fp32, (q, s.niter, mx.niter)
+        assert abs(s.ecorr - mx.ecorr) < 1e-12, (q, s.ecorr, mx.ecorr)
+        assert abs(s.ecorr - e64.ecorr) < 1e-6, (q, s.ecorr, e64.ecorr)

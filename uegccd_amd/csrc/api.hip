@@ -63,6 +63,86 @@ EriKind eri_kind(const uegccd_system& s) {
   return ERI_BASE;
 }
 
+// The range flag of each Coulomb role: the DummyFlag the reference passes at
+// that call, a Max over the term's IRange keys (source/CCD.f90:216-217,
+// 491-492, 555-556, 620-641, 666-672, 726-752, 782-788, 855-881, 910-916);
+// only CasedERI reads it, every other variant computes flag 0.  irange[] is
+// in Input-key order (source/IO.f90:88-102): Rings, XRings, Ladders,
+// Mosaics, DriverDir, DriverEx, Energy, LinRings, QuadRings, DRings,
+// ExRings, LinLadd, QuadLadd, DLadders, ExLadders.
+void role_flags(const uegccd_system& s, EriKind kind, int* flag) {
+  for (int r = 0; r < kRoles; ++r) flag[r] = 0;
+  if (kind != ERI_CASED) return;
+  const int* R = s.irange;
+  enum { RING, XRING, LADDER, MOSAIC, DRVDIR, DRVEX, ENERGY, LINRING, QUADRING, DRING, EXRING,
+         LINLADD, QUADLADD, DLADD, EXLADD };
+  auto mx = [](int a, int b, int c) { return std::max(a, std::max(b, c)); };
+  flag[RO_ENERGY] = R[ENERGY];
+  flag[RO_DRVDIR] = R[DRVDIR];
+  flag[RO_DRVEX] = R[DRVEX];
+  flag[RO_MOSAIC] = R[MOSAIC];
+  flag[RO_LAD_LD] = mx(R[LADDER], R[DLADD], R[LINLADD]);
+  flag[RO_LAD_LX] = mx(R[LADDER], R[EXLADD], R[LINLADD]);
+  flag[RO_LAD_QD] = mx(R[LADDER], R[DLADD], R[QUADLADD]);
+  flag[RO_LAD_QX] = mx(R[LADDER], R[EXLADD], R[QUADLADD]);
+  flag[RO_RING_LD] = mx(R[RING], R[DRING], R[LINRING]);
+  flag[RO_RING_LX] = mx(R[RING], R[EXRING], R[LINRING]);
+  flag[RO_RING_QD] = mx(R[RING], R[DRING], R[QUADRING]);
+  flag[RO_RING_QX] = mx(R[RING], R[EXRING], R[QUADRING]);
+  flag[RO_XR_LD] = mx(R[XRING], R[DRING], R[LINRING]);
+  flag[RO_XR_LX] = mx(R[XRING], R[EXRING], R[LINRING]);
+  flag[RO_XR_QD] = mx(R[XRING], R[DRING], R[QUADRING]);
+  flag[RO_XR_QX] = mx(R[XRING], R[EXRING], R[QUADRING]);
+}
+
+// CoreN masks every integral with an index <= CoreN (source/HEG.f90:760-763,
+// 819-820, 883-884) in CasedERI, NoCaseERI and CoreERI.
+bool core_masked(const uegccd_system& s, EriKind kind) {
+  return s.coren > 0 && (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_CORE);
+}
+
+// Whether the roles read by k_slice (V slabs of the ring and crossed-ring
+// linear products; v(s - k) of the mosaic and both quadratic directs; the Y
+// tables of both quadratic exchanges) need more than one table per array.
+bool slice_split_of(const int* rs) {
+  return rs[RO_RING_LX] != rs[RO_XR_LX] || rs[RO_MOSAIC] != rs[RO_RING_QD] ||
+         rs[RO_MOSAIC] != rs[RO_XR_QD] || rs[RO_RING_QX] != rs[RO_XR_QX];
+}
+// Whether the v(n_s - n_k) factors k_main reads (drivers, linear ring and
+// crossed-ring directs, energy) come from more than one table.
+bool main_split_of(const int* rs) {
+  const int e = rs[RO_ENERGY];
+  return rs[RO_DRVDIR] != e || rs[RO_DRVEX] != e || rs[RO_RING_LD] != e || rs[RO_XR_LD] != e;
+}
+
+// One entry of the Coulomb table of range flag `flag` at |dn|^2 = q
+// (source/HEG.f90:727-757: flag 0 the full kernel, 1 its short-range Yukawa
+// part, 2 the long-range rest, 3 zero), then the GapII screening of the
+// variants that apply it (source/HEG.f90:770-772, 829-831, 938-940).
+double coulomb(const uegccd_system& s, EriKind kind, int flag, int q) {
+  const double L = s.L, mu2 = std::pow(s.screening, 2.0);
+  double v;
+  if (q == 0) {
+    switch (flag) {
+      case 0: v = -s.madelung; break;
+      case 1: v = (1.0 / std::pow(L, 3.0)) * 4.0 * kPi / mu2; break;
+      case 2: v = -s.madelung - (1.0 / std::pow(L, 3.0)) * 4.0 * kPi / mu2; break;
+      default: v = 0.0; break;
+    }
+  } else {
+    const double qs = (double)q * std::pow(2.0 * kPi / L, 2.0);
+    switch (flag) {
+      case 0: v = 1.0 / (L * kPi * (double)q); break;
+      case 1: v = (1.0 / std::pow(L, 3.0)) * 4.0 * kPi / (mu2 + qs); break;
+      case 2: v = (1.0 / std::pow(L, 3.0)) * 4.0 * kPi / (mu2 + qs) * mu2 / qs; break;
+      default: v = 0.0; break;
+    }
+  }
+  const bool gap = kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_GAP;
+  if (gap && s.gapii > 0.0 && v > s.gapii) v = 0.0;
+  return v;
+}
+
 struct DeviceArena {
   char* base = nullptr;
   size_t size = 0, used = 0;
@@ -93,7 +173,13 @@ struct uegccd_ctx {
   std::vector<int> slab_coff;        // history layout: first compact index of each virtual slice
   std::vector<uint2> cmap;           // compact index -> elem_map(element, K-blocked position, partner)
   std::vector<int4> nv4, kb_nva;
-  std::vector<double> eig, vtab, voo, vso;  // Coulomb table; v(n_k - n_j) over occupied k, j; v(n_s - n_k)
+  std::vector<double> eig, vtab, voo, vso;  // Coulomb tables by slot: v(|dn|^2); v(n_k - n_j) over
+                                            // occupied k, j; v(n_s - n_k)
+  int ntab = 1;                    // table slots (one per range flag in use; slot 0 is flag 0)
+  int rslot[kRoles] = {};          // slot of each Coulomb role
+  int coren = 0;                   // core orbitals frozen by the CoreN masking (0: none)
+  std::vector<int> core_elems;     // valid elements with a core index (held at zero)
+  double screening = 0.0;          // HEGData%ScreeningDistance
   std::vector<int4> lad_items;  // (0, K, row0, col0)
   std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow, qb_of;  // hole-particle sets P(q)
   std::vector<int4> qb_info;
@@ -223,6 +309,13 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       pn[pr] = c->kb_poff[K + 1] - c->kb_poff[K];
       pbase[pr] = c->kb_toff[K] + c->pair_pos[pr];
     }
+    // With CoreN masking, the elements with a core index (i or j <= CoreN)
+    // leave the valid set: every masked integral of the reference multiplies
+    // such an amplitude or lands on such an element, so they stay exactly
+    // zero and the others see the unmasked terms (the frozen core).
+    const EriKind kind0 = eri_kind(s);
+    c->coren = core_masked(s, kind0) ? std::min(s.coren, o) : 0;
+    c->core_elems.clear();
     uint2* cm = c->cmap.data();
     int nc = 0;
     for (int av = 0, x = 0; av < v; ++av) {
@@ -230,10 +323,16 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       for (int pr = 0; pr < o * o; ++pr, ++x) {
         const int K = pk[pr], ci = civ[(size_t)K * v + av];
         if (ci < 0) continue;
+        if (pr % o < c->coren || pr / o < c->coren) {
+          c->core_elems.push_back(x);
+          continue;
+        }
         cm[nc++] = elem_map(x, pbase[pr] + ci * pn[pr], c->kb_vb[c->kb_voff[K] + ci]);
       }
     }
     c->slab_coff[v] = nc;
+    c->cmap.resize(nc);
+    c->nvalid = nc;
   }
   // ladder tiles: 16 pair rows x 16 columns (a list then pair list)
   c->lad_items.clear();
@@ -254,9 +353,6 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
                                   (a.z - b.z) * (a.z - b.z));
     }
   if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
-  if (slice_lds_bytes(o) + (int)sizeof(int4) * kMaxNao +
-      (int)sizeof(double) * (kMaxN2 + 16 + kMaxDiis * kMaxDiis + kMaxDiis) > kMaxDynLds)
-    return UEGCCD_ERR_UNSUPPORTED;
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
   // occurrence over (p, i), for the quadratic ring products
   {
@@ -328,27 +424,62 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       }
   }
   c->n2max = n2max;
+  // Coulomb tables: slot 0 holds flag 0, each other range flag a role uses
+  // gets a slot of its own (usually there is none)
   const EriKind kind = eri_kind(s);
-  const bool gap_applies = (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_GAP);
-  c->vtab.assign(n2max + 1, 0.0);
-  for (int q = 0; q <= n2max; ++q) {
-    double val = (q == 0) ? -s.madelung : 1.0 / (s.L * kPi * (double)q);
-    if (gap_applies && val > s.gapii && s.gapii > 0.0) val = 0.0;
-    c->vtab[q] = val;
+  int flag[kRoles];
+  role_flags(s, kind, flag);
+  int slot_of_flag[kMaxTabSlots] = {0, -1, -1, -1};
+  int flags_of_slot[kMaxTabSlots] = {0, 0, 0, 0};
+  c->ntab = 1;
+  for (int r = 0; r < kRoles; ++r) {
+    const int f = flag[r];
+    if (f < 0 || f > 3) return UEGCCD_ERR_INVALID;
+    if (slot_of_flag[f] < 0) {
+      slot_of_flag[f] = c->ntab;
+      flags_of_slot[c->ntab++] = f;
+    }
+    c->rslot[r] = slot_of_flag[f];
   }
-  // the Coulomb elements every slice block needs, from the same table:
+  for (int t = 0; t < c->ntab; ++t)
+    if ((flags_of_slot[t] == 1 || flags_of_slot[t] == 2) && !(s.screening > 0.0)) return UEGCCD_ERR_INVALID;
+  const size_t nt = (size_t)n2max + 1;
+  c->vtab.assign(c->ntab * nt, 0.0);
+  for (int t = 0; t < c->ntab; ++t)
+    for (int q = 0; q <= n2max; ++q) c->vtab[t * nt + q] = coulomb(s, kind, flags_of_slot[t], q);
+  // A negative integral is the checked variants' -100 sentinel, on which the
+  // reference skips the excitation ("Disallowed excitation not trapped!").
+  // Only the roles whose transfer can vanish (an occupied or a virtual pair
+  // with itself) ever meet v(0); refuse a negative one there.
+  if (kind != ERI_BASE) {
+    const int zero_roles[] = {RO_LAD_LD, RO_LAD_LX, RO_RING_LX, RO_XR_LX};
+    const int on[] = {s.do_ladder, s.do_ladder, s.do_ring, s.do_xring};
+    for (int k = 0; k < 4; ++k)
+      if (on[k] && c->vtab[c->rslot[zero_roles[k]] * nt] < 0.0) return UEGCCD_ERR_UNSUPPORTED;
+  }
+  // the Coulomb elements every slice block needs, from the same tables:
   // V(k, j) = v(n_k - n_j) over occupied k, j (k fastest) and v(n_s - n_k)
   auto d2 = [&](int p, int q) {
     const int4 a = c->nv4[p], b = c->nv4[q];
     const int x = a.x - b.x, y = a.y - b.y, z = a.z - b.z;
     return x * x + y * y + z * z;
   };
-  c->voo.assign((size_t)o * o, 0.0);
-  for (int j = 0; j < o; ++j)
-    for (int k = 0; k < o; ++k) c->voo[k + (size_t)o * j] = c->vtab[d2(k, j)];
-  c->vso.assign((size_t)v * o, 0.0);
-  for (int sv = 0; sv < v; ++sv)
-    for (int k = 0; k < o; ++k) c->vso[(size_t)sv * o + k] = c->vtab[d2(o + sv, k)];
+  c->voo.assign((size_t)c->ntab * o * o, 0.0);
+  c->vso.assign((size_t)c->ntab * v * o, 0.0);
+  for (int t = 0; t < c->ntab; ++t) {
+    const double* vt = c->vtab.data() + t * nt;
+    double* voo = c->voo.data() + (size_t)t * o * o;
+    double* vso = c->vso.data() + (size_t)t * v * o;
+    for (int j = 0; j < o; ++j)
+      for (int k = 0; k < o; ++k) voo[k + (size_t)o * j] = vt[d2(k, j)];
+    for (int sv = 0; sv < v; ++sv)
+      for (int k = 0; k < o; ++k) vso[(size_t)sv * o + k] = vt[d2(o + sv, k)];
+  }
+  // LDS of the slice blocks (one table; two with split slice roles)
+  const bool split = slice_split_of(c->rslot);
+  if (slice_lds_bytes(o, split) + (int)sizeof(int4) * kMaxNao +
+      (int)sizeof(double) * ((split ? 2 : 1) * kMaxN2 + 16 + kMaxDiis * kMaxDiis + kMaxDiis) > kMaxDynLds)
+    return UEGCCD_ERR_UNSUPPORTED;
   c->eig.assign(s.eigen, s.eigen + nao);
   return UEGCCD_OK;
 }
@@ -366,8 +497,8 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   const size_t ovo = (size_t)c->o * c->v * c->o;
   size_t bytes = 0;
   bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
-  bytes += aligned(sizeof(double) * (c->n2max + 1)) + aligned(sizeof(float) * (c->n2max + 1));
-  bytes += aligned(sizeof(double) * c->o * c->o) + aligned(sizeof(double) * c->v * c->o);  // voo, vso
+  bytes += aligned(sizeof(double) * c->vtab.size());                    // vtab
+  bytes += aligned(sizeof(double) * c->voo.size()) + aligned(sizeof(double) * c->vso.size());
   bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
   bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->kb_nva.size()));
@@ -401,17 +532,12 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   return bytes;
 }
 
+// Input values the reference accepts (range flags 0-3; CoreN at least 0);
+// which integrals they select is worked out in build_maps.
 int check_supported(const uegccd_system& s) {
-  const EriKind kind = eri_kind(s);
-  // CoreN > 0 masks integrals by orbital index (CAS), which the Coulomb table
-  // does not express; range flags 1-3 change the kernel per term.
-  if (s.coren > 0 && (kind == ERI_CASED || kind == ERI_NOCASE || kind == ERI_CORE))
-    return UEGCCD_ERR_UNSUPPORTED;
-  if (kind == ERI_CASED)
-    for (int k = 0; k < 15; ++k)
-      if (s.irange[k] != 0) return UEGCCD_ERR_UNSUPPORTED;
-  // a negative v(0) makes the checked ERI variants return their -100 sentinel
-  if (kind != ERI_BASE && -s.madelung < 0.0) return UEGCCD_ERR_UNSUPPORTED;
+  if (s.coren < 0) return UEGCCD_ERR_INVALID;
+  for (int k = 0; k < 15; ++k)
+    if (s.irange[k] < 0 || s.irange[k] > 3) return UEGCCD_ERR_INVALID;
   return UEGCCD_OK;
 }
 
@@ -442,8 +568,13 @@ struct ArenaImage {
 };
 
 // Host (i,j,a) arrays of each spin -> interleaved device amplitudes; entries
-// with no virtual momentum partner are zeroed.
+// with no virtual momentum partner are zeroed.  Under CoreN masking the
+// elements with a core index must be zero on input (DrvMBPT makes them so):
+// the library keeps them at zero, which is the reference's iteration only
+// from such amplitudes.
 int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
+  for (int x : c->core_elems)
+    if ((tab && tab[x] != 0.0) || (tba && tba[x] != 0.0)) return UEGCCD_ERR_INVALID;
   std::vector<double> h(2 * (size_t)c->n3, 0.0);
   for (const uint2& m : c->cmap) {
     const size_t x = (size_t)m.x;
@@ -572,13 +703,32 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.tol_e = opt.tol_e;
   S.tol_r = opt.tol_r;
   S.mixed_switch = opt.precision == UEGCCD_PREC_MIXED ? opt.mixed_switch : 0.0;
-  const std::vector<float> vt32(c->vtab.begin(), c->vtab.end());
+  S.ntab = c->ntab;
+  for (int r = 0; r < kRoles; ++r) S.rslot[r] = c->rslot[r];
+  S.slice_split = slice_split_of(c->rslot) ? 1 : 0;
+  S.main_split = main_split_of(c->rslot) ? 1 : 0;
+  {
+    // the ladder tiles stage the distinct tables of their four roles
+    const int roles[4] = {RO_LAD_LD, RO_LAD_LX, RO_LAD_QD, RO_LAD_QX};
+    S.lad_ntab = 0;
+    for (int k = 0; k < 4; ++k) {
+      const int sl = c->rslot[roles[k]];
+      int idx = -1;
+      for (int m = 0; m < S.lad_ntab; ++m)
+        if (S.lad_slot[m] == sl) idx = m;
+      if (idx < 0) {
+        idx = S.lad_ntab++;
+        S.lad_slot[idx] = sl;
+      }
+      S.lad_tab[k] = idx;
+    }
+    for (int k = S.lad_ntab; k < 4; ++k) S.lad_slot[k] = 0;
+  }
   S.nv4 = im.put(c->nv4);
   S.eig = im.put(c->eig);
   S.vtab = im.put(c->vtab);
   S.voo = im.put(c->voo);
   S.vso = im.put(c->vso);
-  S.vtab32 = im.put(vt32);
   S.kb_poff = im.put(c->kb_poff);
   S.kb_pairs = im.put(c->kb_pairs);
   S.kb_voff = im.put(c->kb_voff);
@@ -623,12 +773,14 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.drpa_sum_blocks = kDrpaJooParts * c->o + c->v + (int)c->qb_neg.size();
   b.elem_blocks = eb;
   b.slice_blocks = sb;
+  b.slice_split = S.slice_split;
   for (int k = 0; k < kSliceClasses; ++k) {
     b.slice_prefix[k] = d_pref + 6 + 2 * k;
     b.slice_cls_blocks[k] = k == cls ? sb : 0;
-    b.slice_lds[k] = k == cls ? slice_lds_bytes(c->o) : 0;
+    b.slice_lds[k] = k == cls ? slice_lds_bytes(c->o, S.slice_split) : 0;
   }
   b.nlad = S.nlad;
+  b.lad_lds = ladder_lds_bytes(S);
   b.ncon = S.ncon;
   // work buffers
   c->arena.used = aligned(c->arena.used);
@@ -790,12 +942,24 @@ int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba,
   if (!c || !ncell) return UEGCCD_ERR_INVALID;
   const int o = c->o, nq = (int)c->qb_neg.size();
   // the cells printed: transfers n_a - n_i or n_a - n_j of a valid element
+  // (those with a core index too: the reference bins them, with zero
+  // amplitudes and, under CoreN masking, a zero integral).  SfV of a cell is
+  // the integral of its last writer in the reference's loop order (i, then
+  // j, then a; direct before exchange): v(q), or 0 when that writer is a
+  // masked core element.
   std::vector<char> touched(nq, 0);
-  for (const uint2& m : c->cmap) {
-    const int x = (int)m.x, i = x % o, j = (x / o) % o, av = x / (o * o);
-    touched[c->qb_of[(size_t)av * o + i]] = 1;
-    touched[c->qb_of[(size_t)av * o + j]] = 1;
-  }
+  std::vector<int64_t> last(nq, -1);
+  std::vector<char> last_core(nq, 0);
+  auto bin = [&](int x, bool core) {
+    const int i = x % o, j = (x / o) % o, av = x / (o * o);
+    const int64_t key = (((int64_t)i * o + j) * c->v + av) * 2;
+    const int qd = c->qb_of[(size_t)av * o + i], qx = c->qb_of[(size_t)av * o + j];
+    touched[qd] = touched[qx] = 1;
+    if (key > last[qd]) { last[qd] = key; last_core[qd] = core; }
+    if (key + 1 > last[qx]) { last[qx] = key + 1; last_core[qx] = core; }
+  };
+  for (const uint2& m : c->cmap) bin((int)m.x, false);
+  for (int x : c->core_elems) bin(x, true);
   struct Cell { int x, y, z, q; };
   std::vector<Cell> cells;
   for (int q = 0; q < nq; ++q) {
@@ -836,7 +1000,8 @@ int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba,
     double* out = vals + 5 * (size_t)k;
     out[0] = std::sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z) * sc);
     out[1] = sums[3 * (size_t)e.q];
-    out[2] = c->vso[(size_t)dv * o + l];  // v(n_a - n_l) = v(q)
+    // v(n_a - n_l) = v(q) from the energy's table (IRangeEnergy, source/CCD.f90:290-291)
+    out[2] = last_core[e.q] ? 0.0 : c->vso[(size_t)c->rslot[RO_ENERGY] * c->v * o + (size_t)dv * o + l];
     out[3] = sums[3 * (size_t)e.q + 1];
     out[4] = sums[3 * (size_t)e.q + 2];
   }
@@ -969,14 +1134,18 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   std::vector<int> sp[kSliceClasses];
   int slds[kSliceClasses] = {};
   for (auto& v : sp) v.assign(n + 1, 0);
+  // one k_slice instantiation serves the launch: the split one if any solve needs it
+  int slice_split = 0, lad_lds = 0;
+  for (int k = 0; k < n; ++k) slice_split |= cs[k]->sd.slice_split;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
     ep[k + 1] = ep[k] + elem_blocks_of(c);
     const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
     const int cls = slice_threads(c->o) / 256 - 1;
     for (int q = 0; q < kSliceClasses; ++q) sp[q][k + 1] = sp[q][k] + (need_slice && q == cls ? c->v : 0);
-    if (need_slice) slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o));
+    if (need_slice) slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o, slice_split));
     lp[k + 1] = lp[k] + c->sd.nlad;
+    lad_lds = std::max(lad_lds, ladder_lds_bytes(c->sd));
     cp[k + 1] = cp[k] + c->sd.ncon;
   }
   size_t bytes = aligned(sizeof(SolveDev) * n) + (3 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
@@ -1006,6 +1175,8 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     b.slice_blocks += sp[q][n];
   }
   b.lad_prefix = const_cast<int*>(im.put(lp));
+  b.lad_lds = lad_lds;
+  b.slice_split = slice_split;
   b.con_prefix = const_cast<int*>(im.put(cp));
   b.ncon = cp[n];
   out->states = a.take<State>(n);
@@ -1187,6 +1358,9 @@ extern "C" int uegccd_drpa(uegccd_ctx* c, double* t2, uegccd_result* res, double
   if (!c || !res) return UEGCCD_ERR_INVALID;
   if (c->opt.mode != UEGCCD_MODE_REFERENCE || c->opt.precision != UEGCCD_PREC_FP64)
     return UEGCCD_ERR_UNSUPPORTED;
+  // the reduced ring form takes every Coulomb element from one table: range
+  // flags other than 0 are not run here
+  if (c->ntab != 1) return UEGCCD_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
   const auto t_start = std::chrono::steady_clock::now();
   const BatchDev& b = c->batch;
@@ -1677,6 +1851,7 @@ int plan_request(Request& r) {
   s.do_xring = q.do_xring;
   s.do_ladder = q.do_ladder;
   s.do_mosaic = q.do_mosaic;
+  s.screening = sc[4];
   r.nao = nao;
   r.ehf = sc[3];
   rc = plan_ctx(&s, &q.opt, &r.ctx);

@@ -173,9 +173,11 @@ struct Lds {
   double red[16];
 };
 
-__device__ __forceinline__ void stage(const SolveG& S, Lds& L) {
+// Stages the n-vectors and the Coulomb table of one slot.
+__device__ __forceinline__ void stage(const SolveG& S, Lds& L, int slot) {
+  const auto vt = S.vtab + (size_t)slot * (S.n2max + 1);
   for (int k = threadIdx.x; k < S.nao; k += blockDim.x) L.nv[k] = S.nv4[k];
-  for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) L.vt[k] = S.vtab[k];
+  for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) L.vt[k] = vt[k];
   __syncthreads();
 }
 
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
-  stage(S, L);
+  stage(S, L, S.rslot[RO_ENERGY]);
   const int o = S.o, o2 = o * o;
   const int c = lb * blockDim.x + threadIdx.x;  // compact index of a valid element
   double e = 0.0, tmax = 0.0, bad = 0.0;
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
-  stage(S, L);
+  stage(S, L, 0);  // DrvMBPT's ERI calls pass no flag (source/MP2.f90:61-62): flag 0
   const int o = S.o, o2 = o * o;
   const int c = lb * blockDim.x + threadIdx.x;  // compact index of a valid element
   double e = 0.0;
@@ -498,9 +500,13 @@ __device__ __forceinline__ void slab_place(const SolveG& S, double* Tab, double*
 // One block of BT threads per virtual slice s (BT = slice_threads of the
 // largest NOcc among the launch's solves): one slab entry and one J output per thread
 // (a larger slab loops over the rest).
-template <int BT>
+// SPLIT: the ring and crossed-ring roles read different Coulomb tables
+// (range flags): a second V slab Vx for the crossed-ring products, v(n_s -
+// n_k) of the quadratic ring and crossed-ring directs (vsr, vsx) beside the
+// mosaic's (vs), and a second Y table vt2 for the crossed rings.
+template <int BT, bool SPLIT>
 __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                            int n, int bid, Lds& L, SliceDiis& D, double* dyn) {
+                            int n, int bid, Lds& L, SliceDiis& D, double* dyn, double* vt2) {
   const int s = find_solve(prefix, n, bid);
   const SolveG& S = as_global(solves)[s];
   const int sv = bid - prefix[s];
@@ -512,6 +518,12 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   double* cs = rs + o;
   double* jp = cs + o;
   double* vs = jp + o;  // v(n_s - n_k)
+  double* Vx = SPLIT ? vs + o : Vo;
+  double* vsr = SPLIT ? Vx + o2 : vs;
+  double* vsx = SPLIT ? vsr + o : vs;
+  const size_t vo = (size_t)S.v * o;
+  const auto voo_r = S.voo + (size_t)S.rslot[RO_RING_LX] * o2;
+  const auto vso_m = S.vso + S.rslot[RO_MOSAIC] * vo;
   // The pass's amplitudes on the slab (source/CCD.f90:80-99 with
   // source/DIISCC.f90:52-131): tcur as it stands after a start, else
   // sum_k c_k histT[k]: the DIIS extrapolation, whose coefficients wave 0 of
@@ -534,20 +546,31 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   const int t0 = threadIdx.x;
   const auto HC = S.histT + c0 + t0;
   const uint2 cm = t0 < nvs ? ld(S.cmap + c0 + t0) : make_uint2(0u, 0u);
-  const double vr = t0 < o2 ? S.voo[t0] : 0.0;
+  const double vr = t0 < o2 ? voo_r[t0] : 0.0;
   double2 h[kPre];
 #pragma unroll
   for (int k = 0; k < kPre; ++k)
     h[k] = (t0 < nvs && k < ndiis) ? ld(HC + (size_t)k * nval) : make_double2(0.0, 0.0);
-  const double vsr = t0 < o ? S.vso[sv * o + t0] : 0.0;
+  const double vsm = t0 < o ? vso_m[sv * o + t0] : 0.0;
   if (t0 < kMaxDiis * kMaxDiis) D.bm[t0] = bw;
   if (t0 < kMaxDiis) D.coef[t0] = t0 == slot_cur ? 1.0 : 0.0;
   for (int idx = t0; idx < o2; idx += BT) {
     Tab[idx] = 0.0;
     Tba[idx] = 0.0;
-    Vo[idx] = idx == t0 ? vr : S.voo[idx];
+    Vo[idx] = idx == t0 ? vr : voo_r[idx];
   }
-  stage(S, L);
+  if (SPLIT) {
+    const auto voo_x = S.voo + (size_t)S.rslot[RO_XR_LX] * o2;
+    const auto vso_r = S.vso + S.rslot[RO_RING_QD] * vo, vso_x = S.vso + S.rslot[RO_XR_QD] * vo;
+    const auto vt_x = S.vtab + (size_t)S.rslot[RO_XR_QX] * (S.n2max + 1);
+    for (int idx = t0; idx < o2; idx += BT) Vx[idx] = voo_x[idx];
+    for (int k = t0; k < o; k += BT) {
+      vsr[k] = vso_r[sv * o + k];
+      vsx[k] = vso_x[sv * o + k];
+    }
+    for (int k = t0; k < kMaxN2; k += BT) vt2[k] = k <= S.n2max ? vt_x[k] : 0.0;
+  }
+  stage(S, L, S.rslot[RO_RING_QX]);
   if (phase == PH_DONE) return;  // the whole block: the solve has finished
   if (src == SRC_DIIS) {  // a failed solve leaves the plain update
     if (t0 < 64) {
@@ -581,7 +604,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       slab_place(S, Tab, Tba, xs, ld(S.cmap + c0 + c), combine_history<kPre>(cf, hm, D.coef, ndiis, HCc, nval));
     }
   }
-  if ((int)threadIdx.x < o) vs[threadIdx.x] = vsr;
+  if ((int)threadIdx.x < o) vs[threadIdx.x] = vsm;
   __syncthreads();
   // Row/column sums (for J) by groups of 8 lanes per row i, each lane a
   // stride-8 share of k, then a 3-step shuffle tree; stored with the mosaic
@@ -629,7 +652,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
         // in-range LDS addresses, masked values: no branch between the loads
         const int k = k0 + kq, kc = min(k, o - 1);
         const bool ak = ra < o && k < o, bk = cb < o && k < o;
-        const double ta_rk = Tab[rac + o * kc], tb_rk = Tba[rac + o * kc], v_rk = Vo[rac + o * kc];
+        const double ta_rk = Tab[rac + o * kc], tb_rk = Tba[rac + o * kc], v_rk = Vx[rac + o * kc];
         const double v_kc = Vo[kc + o * cbc], ta_kc = Tab[kc + o * cbc], tb_kc = Tba[kc + o * cbc];
         tva = MfmaTraits<double>::mma(ak ? ta_rk : 0.0, bk ? v_kc : 0.0, tva);
         tvb = MfmaTraits<double>::mma(ak ? tb_rk : 0.0, bk ? v_kc : 0.0, tvb);
@@ -664,23 +687,25 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
     for (int k = 0; k < o; ++k) {
       const int4 nk = L.nv[k];
       const int dx = bx + nk.x, dy = by + nk.y, dz = bz + nk.z;
-      const double y = L.vt[min(__mul24(dx, dx) + __mul24(dy, dy) + __mul24(dz, dz), kMaxN2 - 1)];
+      const int yi = min(__mul24(dx, dx) + __mul24(dy, dy) + __mul24(dz, dz), kMaxN2 - 1);
+      const double y = L.vt[yi];
+      const double yx = SPLIT ? vt2[yi] : y;
       const double ta = Tab[i + o * k], tb = Tba[i + o * k];
       const double ua = Tab[k + o * i], ub = Tba[k + o * i];
       a1 += (ta + tb) * y;
       a2 += ta * y;
       a3 += tb * y;
-      a4 += (ua + ub) * y;
-      a5 += ub * y;
-      a6 += ua * y;
+      a4 += (ua + ub) * yx;
+      a5 += ub * yx;
+      a6 += ua * yx;
     }
-    const double vq = vs[i];
+    const double vq = vsr[i], vqx = vsx[i];
     const size_t jx = 3 * ((size_t)S.qrow[sv * o + i] * o + l);
     S.jring[jx] = vq * rs[i] - a1;
     S.jring[jx + 1] = vq * rs[i] - a2;
     S.jring[jx + 2] = -a3;
-    S.jxring[jx] = vq * cs[i] - a4;
-    S.jxring[jx + 1] = vq * cs[i] - a5;
+    S.jxring[jx] = vqx * cs[i] - a4;
+    S.jxring[jx + 1] = vqx * cs[i] - a5;
     S.jxring[jx + 2] = a6;
   }
 }
@@ -707,6 +732,11 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
 // partial sums are combined in a fixed order.  vt is the Coulomb table and
 // cls the block's virtual list (n_c, c), both in LDS, so each group of steps
 // costs one round of global loads (the amplitudes its list entries select).
+// vt holds the solve's distinct ladder tables (n2max + 1 entries each);
+// lad_tab[] picks the table of each role: the particle-particle columns
+// read the linear direct (v(c-a)) and exchange (v(c-b)) tables, the W
+// columns the quadratic ones (v(c-k), v(c-l)) and, for v(i-k) and v(i-l),
+// the linear ones.
 template <typename Real>
 __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
                             const int4* cls, double* red) {
@@ -739,6 +769,9 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
 #pragma unroll
   for (int r = 0; r < 4; ++r) xpo[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<Real>::row(kq, r), noo - 1)];
   const int4 nx = S.nv4[cx], ny = S.nv4[cy];
+  const int nt = S.n2max + 1;
+  const double* vtx = vt + nt * (col < nvv ? S.lad_tab[0] : S.lad_tab[2]);
+  const double* vty = vt + nt * (col < nvv ? S.lad_tab[1] : S.lad_tab[3]);
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
   // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
   // their amplitude loads (unconditional, in-range addresses, masked values)
@@ -761,8 +794,8 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
       const int4 nc = cls[min(ci, nvv - 1)];
       Real bs = 0, bp = 0;
       if (ci < nvv && colv) {
-        const Real vx = (Real)vt[d2(nc, nx)];
-        const Real vy = (Real)vt[d2(nc, ny)];
+        const Real vx = (Real)vtx[d2(nc, nx)];
+        const Real vy = (Real)vty[d2(nc, ny)];
         bs = vx - vy;
         bp = vx + vy;
       }
@@ -798,7 +831,7 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
       const int q = col - nvv;
       const int i = xpo_r % o;
       const int4 ni = S.nv4[i];
-      const double vik = vt[d2(ni, nx)], vil = vt[d2(ni, ny)];
+      const double vik = vt[nt * S.lad_tab[0] + d2(ni, nx)], vil = vt[nt * S.lad_tab[1] + d2(ni, ny)];
       st2(S.w + S.kb_woff[K] + rowo * noo + q, make_double2(vik - vil + 0.5 * as, vik + vil + 0.5 * ap));
     }
   }
@@ -811,7 +844,11 @@ __device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __r
   const int4 it = ld(S.lad_items + (blk - prefix[s]));
   const int phase = S.st->phase;  // requested with the tables, tested after them
   const int K = it.y, voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
-  for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) vt[k] = S.vtab[k];
+  const int nt = S.n2max + 1;
+  for (int m = 0; m < S.lad_ntab; ++m) {
+    const auto src = S.vtab + (size_t)S.lad_slot[m] * nt;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) vt[m * nt + k] = src[k];
+  }
   for (int k = threadIdx.x; k < nvv; k += blockDim.x) cls[k] = ld(S.kb_nva + voff + k);
   __syncthreads();
   if (phase == PH_DONE) return;
@@ -821,19 +858,20 @@ __device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __r
     ladder_tile<float>(S, K, it.z, it.w, vt, cls, red);
 }
 
-template <int BT>
+template <int BT, bool SPLIT>
 __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solves,
                                               const int* __restrict__ prefix, int n) {
   __shared__ Lds L;
   __shared__ SliceDiis D;
+  __shared__ double vt2[SPLIT ? kMaxN2 : 1];
   extern __shared__ double dyn[];
-  slice_block<BT>(solves, prefix, n, blockIdx.x, L, D, dyn);
+  slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2);
 }
 
 __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
                                                 const int* __restrict__ prefix, int n) {
-  __shared__ double vt[kMaxN2];
-  __shared__ int4 cls[kMaxNao];  // a K block's virtual list has at most v < kMaxNao entries
+  extern __shared__ double vt[];  // the solve's ladder tables (ladder_lds_bytes)
+  __shared__ int4 cls[kMaxNao];   // a K block's virtual list has at most v < kMaxNao entries
   __shared__ double red[3 * 64 * 8];
   ladder_block(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
 }
@@ -1123,12 +1161,27 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
     const double2 t0 = T[x];
     const double t_ab = t0.x, t_ba = t0.y;
     // the four Coulomb elements of the element, v(n_p - n_q) for p occupied
-    // and q virtual, from the per-solve virtual-occupied block
-    const double v_ia = S.vso[av * o + i], v_ib = S.vso[bv * o + i];
-    const double v_ja = S.vso[av * o + j], v_jb = S.vso[bv * o + j];
+    // and q virtual, from the per-solve virtual-occupied block; with split
+    // range flags each role reads its own table (drivers direct/exchange,
+    // ring and crossed-ring linear directs, energy)
+    const size_t vo = (size_t)S.v * o;
+    const auto VE = S.vso + S.rslot[RO_ENERGY] * vo;
+    const double v_ia = VE[av * o + i], v_ib = VE[bv * o + i];
+    const double v_ja = VE[av * o + j], v_jb = VE[bv * o + j];
+    double d_ia = v_ia, d_ib = v_ib, r_ia = v_ia, r_jb = v_jb, x_ja = v_ja, x_ib = v_ib;
+    if (S.main_split) {
+      const auto VD = S.vso + S.rslot[RO_DRVDIR] * vo, VX = S.vso + S.rslot[RO_DRVEX] * vo;
+      const auto VR = S.vso + S.rslot[RO_RING_LD] * vo, VC = S.vso + S.rslot[RO_XR_LD] * vo;
+      d_ia = VD[av * o + i];
+      d_ib = VX[bv * o + i];
+      r_ia = VR[av * o + i];
+      r_jb = VR[bv * o + j];
+      x_ja = VC[av * o + j];
+      x_ib = VC[bv * o + i];
+    }
     // drivers (source/CCD.f90:484-501)
-    double gab = v_ia;
-    double gba = -v_ib;
+    double gab = d_ia;
+    double gba = -d_ib;
     // mosaics (source/CCD.f90:571-582)
     if (S.do_mosaic) {
       const auto jvv = S.jvv;
@@ -1150,17 +1203,17 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
       const double4 tb = S.tv[(size_t)j + o * i + (size_t)o2 * bv];
       if (S.do_ring) {
         // linear rings (source/CCD.f90:716-763): v(q) rowsum - (T V), a- and b-slice
-        gab += v_ia * S.rsum[av * o + i] - ta.x;
+        gab += r_ia * S.rsum[av * o + i] - ta.x;
         gba -= ta.y;
-        gab += v_jb * S.rsum[bv * o + j] - tb.x;
+        gab += r_jb * S.rsum[bv * o + j] - tb.x;
         gba -= tb.y;
       }
       if (S.do_xring) {
         // linear crossed rings (source/CCD.f90:845-892): (V T) of the b- and a-slices
         gab -= tb.z;
-        gba += v_ja * S.csum[bv * o + i] - tb.w;
+        gba += x_ja * S.csum[bv * o + i] - tb.w;
         gab -= ta.z;
-        gba += v_ib * S.csum[av * o + j] - ta.w;
+        gba += x_ib * S.csum[av * o + j] - ta.w;
       }
       const double4 qd = S.quad[x];
       if (S.do_ring) { gab += qd.x; gba += qd.y; }
@@ -1781,11 +1834,19 @@ int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const do
 static void allow_large_lds() {
   static bool done = false;
   if (done) return;
-  const void* fs[] = {reinterpret_cast<const void*>(&k_slice<256>), reinterpret_cast<const void*>(&k_slice<512>),
-                      reinterpret_cast<const void*>(&k_slice<768>)};
+  const void* fs[] = {reinterpret_cast<const void*>(&k_slice<256, false>),
+                      reinterpret_cast<const void*>(&k_slice<512, false>),
+                      reinterpret_cast<const void*>(&k_slice<768, false>)};
   for (const void* f : fs)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                               kMaxDynLds - (int)sizeof(Lds) - (int)sizeof(SliceDiis));
+  const void* fsplit[] = {reinterpret_cast<const void*>(&k_slice<256, true>),
+                          reinterpret_cast<const void*>(&k_slice<512, true>),
+                          reinterpret_cast<const void*>(&k_slice<768, true>)};
+  for (const void* f : fsplit)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kMaxDynLds - (int)sizeof(Lds) - (int)sizeof(SliceDiis) -
+                                  (int)sizeof(double) * kMaxN2);
   done = true;
 }
 
@@ -1812,14 +1873,20 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
     const dim3 g(b.slice_cls_blocks[q]), t(256 * (q + 1));
     const auto* pf = (const int*)b.slice_prefix[q];
     const hipEvent_t ev = q == last ? stop : nullptr;
-    if (q == 0) launch_k(k_slice<256>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
-    else if (q == 1) launch_k(k_slice<512>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
-    else launch_k(k_slice<768>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+    if (b.slice_split) {
+      if (q == 0) launch_k(k_slice<256, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+      else if (q == 1) launch_k(k_slice<512, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+      else launch_k(k_slice<768, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+    } else {
+      if (q == 0) launch_k(k_slice<256, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+      else if (q == 1) launch_k(k_slice<512, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+      else launch_k(k_slice<768, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+    }
   }
 }
 
 static void launch_ladder(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
-  launch_k(k_ladder, stop, dim3(b.nlad), dim3(256), 0, s, (const SolveDev*)b.solves,
+  launch_k(k_ladder, stop, dim3(b.nlad), dim3(256), (uint32_t)b.lad_lds, s, (const SolveDev*)b.solves,
            (const int*)b.lad_prefix, b.n);
 }
 

@@ -35,6 +35,31 @@ __host__ __device__ inline uint2 elem_map(int x, int pos, int b) {
 __host__ __device__ inline int elem_pos(uint2 m) { return (int)(m.y & ((1u << kPosBits) - 1u)); }
 __host__ __device__ inline int elem_partner(uint2 m) { return (int)(m.y >> kPosBits); }
 
+// The places where a Coulomb factor enters the reformulated terms, each with
+// its own range flag (source/CCD.f90: the DummyFlag of every ERI call, a Max
+// over the term's IRange keys).  Each role reads the table slot rslot[role]
+// (slot 0 is always flag 0, the one DrvMBPT uses).
+enum Role : int {
+  RO_ENERGY = 0,  // v(a-i), v(a-j) of CCEnergy (IRangeEnergy)
+  RO_DRVDIR,      // v(i-a) of the drivers (IRangeDriverDirect)
+  RO_DRVEX,       // v(i-b) of the drivers (IRangeDriverExchange)
+  RO_MOSAIC,      // v(c-k), v(c-l) of the mosaics (IRangeMosaic)
+  RO_LAD_LD,      // ladders, linear direct: pp v(c-a), hh v(i-k)
+  RO_LAD_LX,      // ladders, linear exchange: pp v(c-b), hh v(i-l)
+  RO_LAD_QD,      // ladders, quadratic direct: hh v(c-k)
+  RO_LAD_QX,      // ladders, quadratic exchange: hh v(c-l)
+  RO_RING_LD,     // rings, linear direct: v(i-a) row-sum factor
+  RO_RING_LX,     // rings, linear exchange: v(k-j) of the slice products TV
+  RO_RING_QD,     // rings, quadratic direct: v(q) of J1/J2
+  RO_RING_QX,     // rings, quadratic exchange: Y(k,l)
+  RO_XR_LD,       // crossed rings, linear direct: v(j-a) column-sum factor
+  RO_XR_LX,       // crossed rings, linear exchange: v(j-k) of the slice products VT
+  RO_XR_QD,       // crossed rings, quadratic direct: v(q') of J1x/J2x
+  RO_XR_QX,       // crossed rings, quadratic exchange: Y of the crossed rings
+  kRoles
+};
+constexpr int kMaxTabSlots = 4;  // flags 0..3
+
 enum Phase : int { PH_ITER = 0, PH_FINAL = 1, PH_DONE = 2 };
 enum FinKind : int { FIN_ITER = 0, FIN_INIT = 1, FIN_SUM = 2 };
 enum MainFlags : int { MF_ITERATE = 1, MF_WRITE_GR = 2 };
@@ -77,13 +102,21 @@ struct SolveDevT {
   int do_ring, do_xring, do_ladder, do_mosaic;
   double df;        // DenomFactor
   double tol_dt, tol_e, tol_r, mixed_switch;
+  // Coulomb tables, one slot per range flag in use (slot 0: flag 0); role r
+  // reads slot rslot[r].  slice_split / main_split: the roles of k_slice /
+  // k_main do not all share one slot per array (some range flag differs).
+  int ntab;
+  int rslot[kRoles];
+  int slice_split, main_split;
+  int lad_ntab;             // distinct slots of the four ladder roles
+  int lad_tab[4];           // LAD_LD, LAD_LX, LAD_QD, LAD_QX -> index among those slots
+  int lad_slot[4];          // the distinct slots, in that index order
   // constant per basis
   Ptr<const int4> nv4;      // [nao] (n1,n2,n3,0)
   Ptr<const double> eig;    // [nao]
-  Ptr<const double> vtab;   // [n2max+1] Coulomb table v(|dn|^2)
-  Ptr<const double> voo;    // [o*o] v(n_k - n_j) over occupied k, j (k fastest)
-  Ptr<const double> vso;    // [v*o] v(n_s - n_k), slice (virtual) s, occupied k
-  Ptr<const float> vtab32;  // same, FP32
+  Ptr<const double> vtab;   // [ntab][n2max+1] Coulomb table v(|dn|^2)
+  Ptr<const double> voo;    // [ntab][o*o] v(n_k - n_j) over occupied k, j (k fastest)
+  Ptr<const double> vso;    // [ntab][v*o] v(n_s - n_k), slice (virtual) s, occupied k
   Ptr<const int> kb_poff;   // [nK+1]
   Ptr<const int> kb_pairs;  // [o*o] pair ids x = i + o*j grouped by K
   Ptr<const int> kb_voff;   // [nK+1]
@@ -143,6 +176,9 @@ struct SolveDevT {
 template <typename T> using HostPtr = T*;
 using SolveDev = SolveDevT<HostPtr>;
 
+// Dynamic LDS of a k_ladder block: the solve's distinct ladder tables.
+inline int ladder_lds_bytes(const SolveDev& S) { return S.lad_ntab * (S.n2max + 1) * 8; }
+
 constexpr int kSliceClasses = 3;
 
 // Launch geometry of a batch of solves iterated together.
@@ -156,6 +192,8 @@ struct BatchDev {
   int* slice_prefix[kSliceClasses];  // [n+1] each
   int* lad_prefix;      // [n+1] ladder tiles of each solve (one block each)
   int nlad;
+  int lad_lds;                       // dynamic LDS bytes of k_ladder (its Coulomb tables)
+  int slice_split;                   // some solve's slice roles use different tables
   int elem_blocks;
   int drpa_sum_blocks;               // k_drpa_sums of a batch of one: kDrpaJooParts*o + v + nqb
   int slice_blocks;                  // over all classes
@@ -174,7 +212,11 @@ constexpr int kDrpaJooParts = 8;  // slab ranges of the dRPA Joo sum (one wave e
 static_assert(kDrpaJooParts <= kJooParts, "the dRPA Joo parts fit the joo buffer");
 constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
 
-inline int slice_lds_bytes(int o) { return (3 * o * o + 4 * o) * 8; }
+// Dynamic LDS of a k_slice block: T^ab, T^ba, V(k,j) slabs and four o-vectors;
+// with split tables also the crossed-ring V(j,k) and two more o-vectors.
+inline int slice_lds_bytes(int o, bool split = false) {
+  return (3 * o * o + 4 * o) * 8 + (split ? (o * o + 2 * o) * 8 : 0);
+}
 // Threads of a k_slice block (one block per virtual slice, one slab entry
 // and J output per thread): 256 per 256 of the o*o entries, at most 768
 // (a larger slab is looped over).

@@ -104,6 +104,7 @@ class CCDContext:
             s.irange[k] = int(info.irange[k])
         s.do_ring, s.do_xring = int(info.DoRing), int(info.DoXRing)
         s.do_ladder, s.do_mosaic = int(info.DoLadder), int(info.DoMosaic)
+        s.screening = heg.ScreeningDistance
         self.opts = opts if opts is not None else options()
         h = ctypes.c_void_p()
         L.check(self._lib.uegccd_ctx_create(ctypes.byref(s), ctypes.byref(self.opts),
@@ -281,6 +282,7 @@ def _system(heg: HEGData, info: UEGInfo):
         s.irange[k] = int(info.irange[k])
     s.do_ring, s.do_xring = int(info.DoRing), int(info.DoXRing)
     s.do_ladder, s.do_mosaic = int(info.DoLadder), int(info.DoMosaic)
+    s.screening = heg.ScreeningDistance
     return s, (nvec, eig)
 
 

@@ -48,6 +48,7 @@ Contains
     Sys%do_xring = Merge(1, 0, UEGInfo%DoXRing)
     Sys%do_ladder = Merge(1, 0, UEGInfo%DoLadder)
     Sys%do_mosaic = Merge(1, 0, UEGInfo%DoMosaic)
+    Sys%screening = HEGData%ScreeningDistance
     Call SolverOptions(Opt)
     Call Check(uegccd_ctx_create(Sys, Opt, Ctx), 'uegccd_ctx_create')
   End Subroutine SetUpSolver

@@ -15,6 +15,7 @@ Module Wrappers
     Integer(c_int32_t) :: coren, safe_eri
     Integer(c_int32_t) :: irange(15)
     Integer(c_int32_t) :: do_ring, do_xring, do_ladder, do_mosaic
+    Real(c_double) :: screening
   End Type uegccd_system
 
   Type, Bind(C) :: uegccd_options

@@ -41,7 +41,7 @@ class System(ctypes.Structure):
                 ("gapii", ctypes.c_double), ("coren", ctypes.c_int32), ("safe_eri", ctypes.c_int32),
                 ("irange", ctypes.c_int32 * 15), ("do_ring", ctypes.c_int32),
                 ("do_xring", ctypes.c_int32), ("do_ladder", ctypes.c_int32),
-                ("do_mosaic", ctypes.c_int32)]
+                ("do_mosaic", ctypes.c_int32), ("screening", ctypes.c_double)]
 
 
 class Options(ctypes.Structure):
