@@ -42,7 +42,9 @@ enum {
     UEGCCD_ERR_NOT_CONVERGED = -5,/* solve stopped at max_iter (soft failure)  */
     UEGCCD_ERR_NONFINITE = -6,    /* NaN/Inf in amplitudes or residual         */
     UEGCCD_ERR_BUSY = -7,         /* server: no free slot / HBM budget now     */
-    UEGCCD_ERR_NOT_FOUND = -8     /* server: unknown request id               */
+    UEGCCD_ERR_NOT_FOUND = -8,    /* server: unknown request id               */
+    UEGCCD_ERR_SCRATCH = -9       /* larger than the fixed device scratch of
+                                     uegccd_solve_noalloc                      */
 };
 
 /* Iteration schemes. */
@@ -244,6 +246,27 @@ int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t device, int32_t*
                             int32_t* nvec, double* eigen, double* scalars);
 
 const char* uegccd_strerror(int status);
+
+/* ------------------------------------------------------------------------
+ * Allocation-free solve.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same solve as
+ * uegccd_ctx_create + uegccd_solve + uegccd_ctx_destroy (same kernels, same
+ * results), with the context's device memory (maps, amplitudes, DIIS
+ * history, intermediates, state) taken from a fixed per-device scratch array
+ * of UEGCCD_NOALLOC_SCRATCH_MB MiB that the library holds from the first call
+ * on a device until the process ends.  It calls no hipMalloc/hipFree, makes
+ * no stream, event or pinned buffer (it runs on the null stream and polls
+ * into pageable memory); host-side planning uses the C++ heap.  A system
+ * whose context needs more than the scratch returns UEGCCD_ERR_SCRATCH
+ * (uegccd_footprint gives the size; the N=54 |n|^2<=16 solve with DIIS 8
+ * needs 56 MB).  Every call on a device uses the same array and leaves its
+ * contents behind (each call clears what it needs): calls that overlap on one
+ * device race and must be serialised by the caller.  Arguments as
+ * uegccd_ctx_create (sys, opt) and uegccd_solve (amplitudes in/out, res). */
+#define UEGCCD_NOALLOC_SCRATCH_MB 64
+int uegccd_solve_noalloc(const uegccd_system* sys, const uegccd_options* opt, double* t2aaaa,
+                         double* t2abab, double* t2abba, uegccd_result* res);
 
 /* ------------------------------------------------------------------------
  * Batched solves (twist averaging / r_s scans): several independent systems

@@ -202,6 +202,9 @@ struct uegccd_ctx {
   int64_t timed_launches = 0;
   bool owns_memory = false;  // arena from hipMalloc (else carved from a server pool)
   bool owns_stream = false;
+  // the allocation-free solve: no events, no pinned buffers (polls into
+  // pageable memory, host-clock timing)
+  bool no_alloc = false;
 };
 
 namespace {
@@ -624,6 +627,8 @@ const char* uegccd_strerror(int st) {
     case UEGCCD_ERR_NONFINITE: return "non-finite amplitudes or residual";
     case UEGCCD_ERR_BUSY: return "no free slot or HBM budget";
     case UEGCCD_ERR_NOT_FOUND: return "unknown request";
+    case UEGCCD_ERR_SCRATCH:
+      return "system larger than the allocation-free solve's device scratch (UEGCCD_NOALLOC_SCRATCH_MB)";
     default: return "unknown status";
   }
 }
@@ -1225,18 +1230,28 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   int maxiter = 0;
   for (int k = 0; k < n; ++k) maxiter = std::max(maxiter, cs[k]->maxiter);
   // events and the pinned state buffer are made once per context and reused
-  if (!c0->span[0]) HIPCHK(hipEventCreate(&c0->span[0]));
-  if (!c0->span[1]) HIPCHK(hipEventCreate(&c0->span[1]));
-  if (c0->h_states_cap < n) {
-    if (c0->h_states) hipHostFree(c0->h_states);
-    c0->h_states = nullptr;
-    c0->h_states_cap = 0;
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c0->h_states), sizeof(State) * n, hipHostMallocDefault));
-    c0->h_states_cap = n;
+  // (the allocation-free solve makes neither: host clock, pageable polls)
+  const bool no_alloc = c0->no_alloc;
+  std::vector<State> pageable;
+  State* hst = nullptr;
+  if (no_alloc) {
+    pageable.resize(n);
+    hst = pageable.data();
+  } else {
+    if (!c0->span[0]) HIPCHK(hipEventCreate(&c0->span[0]));
+    if (!c0->span[1]) HIPCHK(hipEventCreate(&c0->span[1]));
+    if (c0->h_states_cap < n) {
+      if (c0->h_states) hipHostFree(c0->h_states);
+      c0->h_states = nullptr;
+      c0->h_states_cap = 0;
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c0->h_states), sizeof(State) * n, hipHostMallocDefault));
+      c0->h_states_cap = n;
+    }
+    hst = c0->h_states;
   }
-  State* hst = c0->h_states;
   const hipEvent_t e0 = c0->span[0], e1 = c0->span[1];
-  HIPCHK(hipEventRecord(e0, s));
+  const auto host_t0 = std::chrono::steady_clock::now();
+  if (!no_alloc) HIPCHK(hipEventRecord(e0, s));
   launch_init(b, s);
   launch_finalize(b, FIN_INIT, s, prec64);
   HIPCHK(hipGetLastError());
@@ -1247,7 +1262,7 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   std::vector<double> prev_r(n, 0.0);
   int passes = 0;
   bool all_done = false;
-  const bool timing = c0->timed_kernel >= 0;
+  const bool timing = c0->timed_kernel >= 0 && !no_alloc;
   if (timing && (int)c0->tev.size() < 2 * cap) {
     for (size_t k = c0->tev.size(); k < (size_t)(2 * cap); ++k) {
       hipEvent_t e;
@@ -1302,10 +1317,15 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     for (int k = 0; k < n; ++k) all_done = all_done && st[k].phase == PH_DONE;
     if (!all_done) chunk = next_chunk(cs, n, st, prev_r, chunk, first, cap);
   }
-  HIPCHK(hipEventRecord(e1, s));
-  HIPCHK(hipEventSynchronize(e1));
   float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
+  if (no_alloc) {
+    HIPCHK(hipStreamSynchronize(s));
+    ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - host_t0).count();
+  } else {
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    hipEventElapsedTime(&ms, e0, e1);
+  }
   if (ms_out) *ms_out = ms;
   for (int k = 0; k < n; ++k) cs[k]->last_niter = st[k].niter;
   return UEGCCD_OK;
@@ -2215,4 +2235,79 @@ extern "C" int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t devic
     return done(UEGCCD_ERR_HIP);
   if (scalars) heg_scalars(p, g, ehf, scalars);
   return done(rc);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free solve (uegccd_solve_noalloc).  The context that
+// uegccd_ctx_create would allocate (maps, descriptor, amplitudes, DIIS
+// history, intermediates, partials, state) is laid out in `scratch`, a
+// file-scope device array the code-object loader places on every device and
+// keeps until the process ends.  Host code never uses the array's name (a
+// host-side stand-in): the per-device address comes from hipGetSymbolAddress
+// on the first call on that device, with that device current.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(UEGCCD_NOALLOC_SCRATCH_MB) << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch` (device `dev` must be current).
+int scratch_of_device(int dev, char** out) {
+  if (dev < 0 || dev >= kMaxScratchDevices) return UEGCCD_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      std::fprintf(stderr, "uegccd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return UEGCCD_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" int uegccd_solve_noalloc(const uegccd_system* sys, const uegccd_options* opt_in,
+                                    double* taa, double* tab, double* tba, uegccd_result* res) {
+  if (!sys || !tab || !tba) return UEGCCD_ERR_INVALID;
+  uegccd_ctx* c = nullptr;
+  int rc = plan_ctx(sys, opt_in, &c);
+  if (rc) return rc;
+  std::unique_ptr<uegccd_ctx, void (*)(uegccd_ctx*)> guard(c, uegccd_ctx_destroy);
+  const size_t bytes = arena_bytes(c, ndiis_of(c->opt));
+  if (bytes > kNoAllocScratchBytes) return UEGCCD_ERR_SCRATCH;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return UEGCCD_ERR_HIP;
+  if (c->device < 0 || c->device >= ndev) return UEGCCD_ERR_INVALID;
+  HIPCHK(hipSetDevice(c->device));
+  char* base = nullptr;
+  rc = scratch_of_device(c->device, &base);
+  if (rc) return rc;
+  c->no_alloc = true;
+  c->stream = nullptr;  // the null stream: no stream object is created
+  std::vector<char> image;
+  size_t work_off = 0;
+  layout_ctx(c, sys, base, bytes, image, &work_off, true);
+  if (c->arena.used > c->arena.size) return UEGCCD_ERR_SCRATCH;
+  // the constant section, then the work buffers cleared by a fill kernel (the
+  // last call's contents are still there)
+  HIPCHK(hipMemcpyAsync(base, image.data(), image.size(), hipMemcpyHostToDevice, c->stream));
+  launch_zero(base + work_off, bytes - work_off, c->stream);
+  HIPCHK(hipGetLastError());
+  rc = copy_in_amplitudes(c, tab, tba);
+  if (rc) return rc;
+  std::vector<State> st;
+  float ms = 0.f;
+  uegccd_ctx* cs[1] = {c};
+  rc = iterate(cs, 1, c->batch, nullptr, st, &ms);
+  if (rc) return rc;
+  rc = copy_out_amplitudes(c, taa, tab, tba);
+  if (rc) return rc;
+  return fill_result(c, st[0], ms, res);
 }

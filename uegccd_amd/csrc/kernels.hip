@@ -1926,6 +1926,18 @@ void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s)
   hipLaunchKernelGGL(k_state_reset, dim3(b.n), dim3(64), 0, s, b.solves, phase, prec64);
 }
 
+__global__ void k_zero(double* __restrict__ p, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    p[k] = 0.0;
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / sizeof(double);
+  if (n == 0) return;
+  const unsigned grid = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, s, static_cast<double*>(p), n);
+}
+
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_states, dim3(b.n), dim3(64), 0, s, b.solves, out);
 }

@@ -255,6 +255,10 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
 // One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main.
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
+// Clears `bytes` (a multiple of 8) of device memory at p with a fill kernel
+// (no memset node: usable where a caller captures the stream).
+void launch_zero(void* p, size_t bytes, hipStream_t s);
+
 // MP2 connectivity histogram of a batch-of-one solve (nbins <= 4096 bins,
 // bin k counts |dn|^2 = k+1); hist and overflow zeroed by the caller.
 int connectivity_on_device(const BatchDev& b, int nvalid, int nbins, int* hist, int* overflow,

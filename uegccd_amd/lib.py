@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_PKG, "libuegccd.so")
 OK = 0
 ERR_INVALID, ERR_ALLOC, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3, -4
 ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_BUSY, ERR_NOT_FOUND = -5, -6, -7, -8
+ERR_SCRATCH = -9
+NOALLOC_SCRATCH_MB = 64
 MODE_REFERENCE, MODE_DIIS = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
@@ -29,7 +31,7 @@ EXPORTS = (
     "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
     "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
     "uegccd_heg_build_device", "uegccd_connectivity", "uegccd_drpa",
-    "uegccd_structure_factor",
+    "uegccd_structure_factor", "uegccd_solve_noalloc",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -121,6 +123,8 @@ def load():
     lib.uegccd_residual.argtypes = [P, D, D, D, D, D, D]
     lib.uegccd_energy.argtypes = [P, D, D, D]
     lib.uegccd_solve.argtypes = [P, D, D, D, ctypes.POINTER(Result)]
+    lib.uegccd_solve_noalloc.argtypes = [ctypes.POINTER(System), ctypes.POINTER(Options), D, D, D,
+                                         ctypes.POINTER(Result)]
     lib.uegccd_trace.argtypes = [P, D, D, ctypes.c_int32]
     lib.uegccd_mp2.argtypes = [P, D, D, D, D]
     lib.uegccd_drpa.argtypes = [P, D, ctypes.POINTER(Result), D]
