@@ -9,9 +9,11 @@ second over the K timed steps; p50/p99 are the per-solve latencies.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 Also reported: the roofline of the dominant kernel (HIP events on the
-library's stream) and the CPU baseline (the loop-literal C port of the
-reference, timed on a bounded sample of residual passes of the same system on
-this host and scaled by the reference-mode iteration count).
+library's stream), every kernel's algorithmic work against its measured HBM
+traffic, and the CPU baseline: one full reference-mode solve of the same
+system by the loop-literal C port of the reference on one pinned core, and by
+its OpenMP build on all cores, with the speed-up of the GPU's reference-mode
+solve over it (the same iteration on both sides).
 """
 from __future__ import annotations
 
@@ -34,34 +36,37 @@ WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_finalize"}
-def algo_work(kernel, o, v, nvalid, sum_noo2, spp, shh):
-    """Algorithmic (flops, bytes) of one launch, each datum moved once.
-    k_slice: the pass's amplitudes from 8 history entries of each valid element
-    (4 flops each), written to tcur and to the K-blocked copy; per slice, TV/VT
-    (8 o^3 flops) and the quadratic J intermediates (12 o^3); writes TV/VT and
-    the two J arrays.  k_ladder:
-    8 S_pp + 8 S_hh flops (pp ladder, W); reads T, writes the ladder and W.  k_contract reads the J rows (3 doubles, rings
-    and crossed rings) and the T columns of every transfer q, writes the
-    quadratic sums (2 x 4 flops per element and l), reads the W blocks and T
-    and updates the ladder (4 flops per element and pair).  k_main reads T, the
-    ladder, two slice products, the quadratic sums and 8 DIIS error entries and
-    writes one T and one error entry; k_finalize reads the 4 + 8 partials of
-    every element block."""
-    n3 = o * o * v
+def algo_work(kernel, o, v, nvalid, ndiis, spp, shh):
+    """Algorithmic (flops, bytes) of one launch in the terms of the reference's
+    loops (source/CCD.f90): valid elements only, each compulsory datum moved
+    once, nothing for the intermediates the kernel split passes between
+    launches (slice products, J rows, ladder sums, W blocks, partials) nor for
+    the K-blocked duplicate of the amplitudes.
+      k_slice:    the pass's amplitudes, the sum over the ndiis history entries
+                  of every valid element (4 flops each; 16 B in per entry,
+                  16 B out); the linear ring and crossed-ring slice products
+                  over the valid amplitudes (4 products, 2 o flops per valid
+                  element each) and the quadratic J sums (6 accumulations of
+                  2 flops per valid element and l).
+      k_ladder:   8 S_pp (particle-particle ladder) + 8 S_hh (the W
+                  intermediates, half the hole-hole ladder's 16 S_hh); reads T.
+      k_contract: the quadratic ring and crossed-ring contractions (3 + 3
+                  products of 2 flops per valid element and l) and the other
+                  8 S_hh of the hole-hole ladder; reads T.
+      k_main:     60 flops per valid element; reads T, writes the new T, reads
+                  the ndiis DIIS error entries of the new B-matrix row and
+                  writes one.
+      k_finalize: reductions of the block partials (intermediates only)."""
     if kernel == "k_slice":
-        return (20.0 * o * n3 + 32.0 * nvalid,
-                (8 * 16 + 16 + 16) * nvalid + 32 * n3 + 2 * 24 * n3)
+        return (20.0 * o * nvalid + 4.0 * ndiis * nvalid, 16 * (ndiis + 1) * nvalid)
     if kernel == "k_ladder":
-        return (8.0 * spp + 8.0 * shh, 16 * nvalid + 16 * nvalid + 16 * sum_noo2)
+        return (8.0 * spp + 8.0 * shh, 16 * nvalid)
     if kernel == "k_contract":
-        return (16.0 * nvalid * o + 8.0 * shh,
-                2 * 24 * n3 + 2 * 16 * n3 + 32 * nvalid + 16 * sum_noo2 + 16 * nvalid
-                + 32 * nvalid + 8 * o * v)
+        return (12.0 * o * nvalid + 8.0 * shh, 16 * nvalid)
     if kernel == "k_main":
-        return (60.0 * nvalid, (16 + 16 + 64 + 32 + 8 * 16 + 32) * nvalid)
+        return (60.0 * nvalid, (16 + 16 + 16 * ndiis + 16) * nvalid)
     if kernel == "k_finalize":
-        nblk = -(-n3 // 256)
-        return (12.0 * nblk, 8 * 12 * nblk)
+        return (0.0, 0)
     raise KeyError(kernel)
 
 
@@ -122,25 +127,52 @@ def calibrate_copy():
     print(json.dumps({"copied_bytes": n * 8, "copies": 3}))
 
 
-def cpu_baseline(info, niter_ref, budget_s=20.0):
+def cpu_model():
+    """The host CPU's model name, logical CPUs and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count(), sorted(os.sched_getaffinity(0))
+
+
+def cpu_baseline(info, gpu_ref_ms):
+    """One full reference-mode solve (the reference's damped iteration and
+    stopping rule, from the MP2 guess) of the timed system by the serial -O3 C
+    port of the reference loops, this thread pinned to one core; then the same
+    solve by the OpenMP build of the port on every core this process may use."""
+    from oracle import OracleSystem
     from tests.helpers import oracle_for
     orc = oracle_for(info)
-    _, taa, tab, tba = orc.mp2()
+    model, ncpu, allowed = cpu_model()
+    core = allowed[0]
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        r = orc.ccd()
+        solve_s = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, set(allowed))
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(allowed)))
+    omp = OracleSystem(orc.params, variant="omp")
     t0 = time.perf_counter()
-    passes = 0
-    while True:
-        orc.terms(tab, tba)
-        passes += 1
-        if time.perf_counter() - t0 > budget_s / 2 or passes >= 3:
-            break
-    per_pass = (time.perf_counter() - t0) / passes
-    solve_s = per_pass * (niter_ref + 1)
+    ro = omp.ccd()
+    omp_s = time.perf_counter() - t0
     return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{passes} residual passes (TermsWrapper) of the same N=54 |n|^2<=16 system "
-                      f"by the serial -O3 C port, {per_pass:.3f} s each, x {niter_ref + 1} "
-                      f"passes of the reference-mode solve ({niter_ref} iterations + final "
-                      f"residual); solve {solve_s:.1f} s",
-            "solve_s": solve_s}
+            "sample": f"one full reference-mode solve ({r['niter']} iterations + final residual) of "
+                      f"the same N=54 |n|^2<=16 system from the MP2 guess by the serial -O3 C port, "
+                      f"pinned to core {core} of {ncpu} ({model})",
+            "solve_s": solve_s, "niter": r["niter"], "ecorr": r["ecorr"], "cpu_model": model,
+            "host_cpus": ncpu, "pinned_core": core,
+            "speedup_same_iteration": solve_s * 1000.0 / gpu_ref_ms,
+            "all_cores": {"value": 1.0 / omp_s, "unit": "solves/s", "cores": threads,
+                          "solve_s": omp_s, "niter": ro["niter"],
+                          "same_energy": ro["ecorr"] == r["ecorr"],
+                          "speedup_same_iteration": omp_s * 1000.0 / gpu_ref_ms}}
 
 
 def sequential_fp64(reqs, lib, L, CCDContext, init_heg, options):
@@ -157,6 +189,25 @@ def sequential_fp64(reqs, lib, L, CCDContext, init_heg, options):
         out.append((st, res.ecorr, res.niter))
         ctx.close()
     return time.perf_counter() - t0, out
+
+
+def sequential_fp64_prebuilt(reqs, lib, L, CCDContext, init_heg, options):
+    """The same sequential FP64 solves with every context built beforehand:
+    only the MP2 guess and the solve of each request are timed."""
+    res = L.Result()
+    ctxs = []
+    for q in reqs:
+        info = request_info(q)
+        ctxs.append(CCDContext(init_heg(info), info, options(mode="diis", ndiis=8)))
+    out = []
+    t0 = time.perf_counter()
+    for c in ctxs:
+        st = lib.uegccd_solve_resident(c.handle, 1, ctypes.byref(res))
+        out.append((st, res.ecorr, res.niter))
+    wall = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return wall, out
 
 
 def served(reqs, arrivals, precision, Server, options, max_active=64, iters_per_step=4):
@@ -244,6 +295,7 @@ def twist_stream(lib, L, n, rate):
     from uegccd_amd.driver import CCDContext, Server, init_heg, options
     reqs, arrivals = stream_requests(n, 1234, rate)
     seq_s, seq = sequential_fp64(reqs, lib, L, CCDContext, init_heg, options)
+    pre_s, pre = sequential_fp64_prebuilt(reqs, lib, L, CCDContext, init_heg, options)
     e64 = np.array([e for _, e, _ in seq])
     ok64 = np.array([st == L.OK for st, _, _ in seq])
 
@@ -263,7 +315,12 @@ def twist_stream(lib, L, n, rate):
     return {
         "requests": n, "seed": 1234, "poisson_rate_per_s": rate,
         "sequential_fp64": {"wall_s": seq_s, "solves_per_s": n / seq_s,
-                            "converged": int(ok64.sum())},
+                            "converged": int(ok64.sum()),
+                            "note": "basis, maps and context of each request built in the loop"},
+        "sequential_fp64_prebuilt": {"wall_s": pre_s, "solves_per_s": n / pre_s,
+                                     "converged": int(sum(st == L.OK for st, _, _ in pre)),
+                                     "same_energies": bool(all(a[1] == b[1] for a, b in zip(seq, pre))),
+                                     "note": "contexts built before the timed loop"},
         "batched_mixed": summary(batch_wall, batch_out),
         "served_mixed": summary(srv_wall, srv_out),
     }
@@ -378,10 +435,22 @@ def main():
 
     # roofline of the dominant kernel, from its live time in the timed region:
     # its algorithmic flops and bytes per launch against the FP64 (MFMA and
-    # vector: 78.6 TFLOP/s) and HBM roofs; the bound is the roof it is closer to
+    # vector: 78.6 TFLOP/s) and HBM roofs; the bound is the roof it is closer
+    # to.  Every kernel also gets its algorithmic work, its in-solve time and
+    # the measured HBM traffic per launch over the algorithmic bytes.
     spp, shh, nvalid, noo2 = block_sums(heg.nVec, heg.NOcc)
-    traffic = measured_traffic().get(dominant)
-    flops, nbytes = algo_work(dominant, heg.NOcc, heg.NAO - heg.NOcc, nvalid, noo2, spp, shh)
+    ndiis = 8
+    traffic_all = measured_traffic()
+    per_kernel = {}
+    for name, us in kt.items():
+        fl, by = algo_work(name, heg.NOcc, heg.NAO - heg.NOcc, nvalid, ndiis, spp, shh)
+        tr = traffic_all.get(name)
+        per_kernel[name] = {"us": us, "algo_flops": fl, "algo_bytes": by, "traffic_bytes": tr,
+                            "traffic_over_algo": (tr / by) if (tr and by) else None,
+                            "frac_hbm": by / (us * 1e-6) / 1e9 / HBM_PEAK_GBS if us else None,
+                            "frac_fp64": fl / (us * 1e-6) / 1e12 / FP64_MFMA_PEAK_TFS if us else None}
+    traffic = traffic_all.get(dominant)
+    flops, nbytes = algo_work(dominant, heg.NOcc, heg.NAO - heg.NOcc, nvalid, ndiis, spp, shh)
     t = dom_us * 1e-6
     f_mfma = flops / t / 1e12 / FP64_MFMA_PEAK_TFS
     f_hbm = nbytes / t / 1e9 / HBM_PEAK_GBS
@@ -393,7 +462,8 @@ def main():
                 "unit": "GB/s", "frac": f_hbm, "traffic": traffic}
     roof.update({"kernel": dominant, "avg_launch_us": dom_us, "launches": dom_launches,
                  "flops_per_launch": flops, "bytes_per_launch": nbytes,
-                 "frac_other_roof": min(f_mfma, f_hbm)})
+                 "frac_other_roof": min(f_mfma, f_hbm),
+                 "traffic_over_algo": (traffic / nbytes) if traffic else None})
 
     mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
     stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
@@ -402,7 +472,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(info, niter_ref)
+        cpu = cpu_baseline(info, ref_mode_ms)
 
     lat_sorted = np.sort(np.array(lat))
     out = {
@@ -417,9 +487,14 @@ def main():
                    "reference_mode_solve_ms": ref_mode_ms},
         "p50_ms": float(np.percentile(lat_sorted, 50)), "p99_ms": float(np.percentile(lat_sorted, 99)),
         "kernel_us_in_solve": kt,
+        "kernels": per_kernel,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        # the same algorithm on both sides: the reference-mode solve (the
+        # reference's iteration and stopping rule) on the GPU against the CPU port
+        "speedup_vs_cpu": cpu["speedup_same_iteration"] if cpu else None,
+        # DIIS's fewer iterations, a gain of the algorithm rather than the hardware
+        "algorithmic_gain_diis": ref_mode_ms / (1000.0 * wall / args.steps),
         "mixed_precision": mixed,
         "twist_stream": stream,
         "drpa": drpa,

@@ -25,7 +25,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
+# "serial": liboracle.so, the -O3 single-thread build (the checker and the
+# one-core CPU baseline); "omp": liboracle_omp.so, the same code with the term
+# loops over OpenMP threads (the all-cores baseline; the same results).
+_VARIANTS = {"serial": "liboracle.so", "omp": "liboracle_omp.so"}
 
 IRANGE_KEYS = ("Rings Range", "XRings Range", "Ladders Range", "Mosaics Range",
                "DriverDir Range", "DriverEx Range", "Energy Range", "LinRings Range",
@@ -66,10 +70,9 @@ class OracleParams:
     irange: tuple = field(default_factory=lambda: (0,) * 15)
 
 
-def _lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+def _lib(variant="serial"):
+    if variant not in _LIBS:
+        path = os.path.join(_HERE, _VARIANTS[variant])
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
         lib = ctypes.CDLL(path)
@@ -102,8 +105,8 @@ def _lib():
         lib.orc_drpa_terms.argtypes = [P, ctypes.c_int, D, D, D]
         lib.orc_drpa.restype = ctypes.c_int
         lib.orc_drpa.argtypes = [P, ctypes.c_double, ctypes.c_int, D, D, D, D, I, D, D]
-        _LIB = lib
-    return _LIB
+        _LIBS[variant] = lib
+    return _LIBS[variant]
 
 
 def _dp(a):
@@ -113,7 +116,7 @@ def _dp(a):
 class OracleSystem:
     """One HEG system (HEGDataType) built by the oracle's restated init_HEG+change_rs."""
 
-    def __init__(self, p: OracleParams):
+    def __init__(self, p: OracleParams, variant: str = "serial"):
         self.params = p
         cp = _Params()
         cp.nel, cp.maxkpoint, cp.coren = p.nel, p.maxkpoint, p.coren
@@ -128,7 +131,7 @@ class OracleSystem:
         cp.fcut, cp.corefcut = p.fcut, p.corefcut
         cp.madelungfactor, cp.gap, cp.gapii = p.madelungfactor, p.gap, p.gapii
         cp.exchangefactor = p.exchangefactor
-        self._lib = _lib()
+        self._lib = _lib(variant)
         self._h = self._lib.orc_build(ctypes.byref(cp))
         if not self._h:
             raise ValueError("oracle: basis construction failed")

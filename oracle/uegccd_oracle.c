@@ -14,6 +14,16 @@
 #include <string.h>
 
 #define PI 3.14159265358979323846264338327950288
+
+/* The all-cores CPU baseline (liboracle_omp.so, built with -DORC_OMP
+ * -fopenmp) splits the outer occupied loop of the term routines over
+ * threads.  Every element (i,j,a) is written only by the iteration of its own
+ * i, in the same order as the serial loop, so the results are the same bits. */
+#ifdef ORC_OMP
+#define ORC_PAR_I _Pragma("omp parallel for schedule(dynamic)")
+#else
+#define ORC_PAR_I
+#endif
 enum { ERI_CASED, ERI_NOCASE, ERI_CORE, ERI_GAP, ERI_BASE };
 /* irange[] slots, in Input-key order (source/IO.f90:88-102) */
 enum { R_RING, R_XRING, R_LADDER, R_MOSAIC, R_DRVDIR, R_DRVEX, R_ENERGY, R_LINRING,
@@ -502,6 +512,7 @@ static void g2_mosaics(const orc_system* s, double* gab, double* gba, const doub
                 jvv[c] = jvv[c] + vkl * (taa[x] + tab[x]) - vlk * (taa[x] + tba[x]);
                 joo[k] = joo[k] + vkl * (taa[x] + tab[x]) - vlk * (taa[x] + tba[x]);
             }
+    ORC_PAR_I
     for (int i = 1; i <= o; ++i)
         for (int j = 1; j <= o; ++j)
             for (int a = o + 1; a <= n; ++a) {
@@ -526,6 +537,7 @@ static void g2_ladders(const orc_system* s, double* gab, double* gba, const doub
     const int f_j3 = imax3(r[R_LADDER], r[R_EXLADD], r[R_LINLADD]);
     const int f_qd = imax3(r[R_LADDER], r[R_DLADD], r[R_QUADLADD]);
     const int f_qx = imax3(r[R_LADDER], r[R_EXLADD], r[R_QUADLADD]);
+    ORC_PAR_I
     for (int i = 1; i <= o; ++i)
         for (int j = 1; j <= o; ++j) {
             for (int k = 1; k <= o; ++k) {
@@ -579,6 +591,7 @@ static void g2_rings(const orc_system* s, double* gab, double* gba, const double
     const int f_lx = imax3(r[R_RING], r[R_EXRING], r[R_LINRING]);
     const int f_qd = imax3(r[R_RING], r[R_DRING], r[R_QUADRING]);
     const int f_qx = imax3(r[R_RING], r[R_EXRING], r[R_QUADRING]);
+    ORC_PAR_I
     for (int i = 1; i <= o; ++i)
         for (int j = 1; j <= o; ++j)
             for (int a = o + 1; a <= n; ++a) {
@@ -606,6 +619,7 @@ static void g2_rings(const orc_system* s, double* gab, double* gba, const double
                     gba[x] = gba[x] - vak * tba[y];
                 }
             }
+    ORC_PAR_I
     for (int i = 1; i <= o; ++i)
         for (int a = o + 1; a <= n; ++a)
             for (int l = 1; l <= o; ++l) {
@@ -644,6 +658,7 @@ static void g2_xrings(const orc_system* s, double* gab, double* gba, const doubl
     const int f_ld = imax3(r[R_XRING], r[R_DRING], r[R_LINRING]);
     const int f_qd = imax3(r[R_XRING], r[R_DRING], r[R_QUADRING]);
     const int f_qx = imax3(r[R_XRING], r[R_EXRING], r[R_QUADRING]);
+    ORC_PAR_I
     for (int i = 1; i <= o; ++i)
         for (int j = 1; j <= o; ++j)
             for (int a = o + 1; a <= n; ++a) {
@@ -671,6 +686,7 @@ static void g2_xrings(const orc_system* s, double* gab, double* gba, const doubl
                     gba[x] = gba[x] + (vbk - vkb) * tba[y] + vbk * taa[y];
                 }
             }
+    ORC_PAR_I
     for (int i = 1; i <= o; ++i)
         for (int b = o + 1; b <= n; ++b)
             for (int l = 1; l <= o; ++l) {

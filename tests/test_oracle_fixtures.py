@@ -329,3 +329,18 @@ def test_structure_factor_identities(fixtures, amps):
     np.testing.assert_allclose(vals[:, 3] + vals[:, 4], vals[:, 1], rtol=0, atol=1e-15)
     if amps == "mp2":
         assert abs(e_ref - c["e2"]) < 1e-10
+
+
+def test_openmp_build_gives_the_same_bits():
+    """liboracle_omp.so (the all-cores CPU baseline) splits the term loops
+    over threads by the outer occupied index; each element keeps its serial
+    summation order, so residuals and the whole trajectory are identical."""
+    from tests.helpers import info_for, oracle_for
+    s = oracle_for(info_for("n14_cut5_tw"))
+    omp = OracleSystem(s.params, variant="omp")
+    _, _, tab, tba = s.mp2()
+    for a, b in zip(s.terms(tab, tba), omp.terms(tab, tba)):
+        np.testing.assert_array_equal(a, b)
+    r, ro = s.ccd(), omp.ccd()
+    assert r["niter"] == ro["niter"] and r["ecorr"] == ro["ecorr"]
+    np.testing.assert_array_equal(r["trace_e"], ro["trace_e"])
