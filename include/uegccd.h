@@ -151,6 +151,16 @@ int uegccd_energy(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, d
 int uegccd_structure_factor(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, int32_t* ncell,
                             int32_t* qvec, double* vals, double* ecorr);
 
+/* DrvMBPT's structure factor (source/MP2.f90:86-117, written to unit 81 on
+ * every MP2 call): the MP2 amplitudes are written on the device (as
+ * uegccd_mp2, which they replace in the context) and binned as
+ * uegccd_structure_factor bins, with three values per cell, vals[3k..3k+2] =
+ * (SfG, SfSum, SfV); SfV is DrvMBPT's integral, which carries no range flag.
+ * Size query and capacity as uegccd_structure_factor; *e2 (may be NULL)
+ * receives E(2). */
+int uegccd_mp2_structure_factor(uegccd_ctx* ctx, int32_t* ncell, int32_t* qvec, double* vals,
+                                double* e2);
+
 /* DrvCCD (source/CCD.f90:18-183): iterate from the given amplitudes (the MP2
  * guess in the reference) to convergence.  t2aaaa is written as
  * t2abab + t2abba on output (may be NULL). */
@@ -244,6 +254,18 @@ int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao, int32_t* nvec, do
  * with (2*floor(sqrt(maxkpoint)) + 7)^3 > 8192 return UEGCCD_ERR_UNSUPPORTED. */
 int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t device, int32_t* nao,
                             int32_t* nvec, double* eigen, double* scalars);
+
+/* print_ex_SF (source/HEG.f90:617-674), the exchange structure factor
+ * print_HF writes to unit 82 after each basis set-up and change of r_s: for
+ * the basis of p at r_s = p->rs, over ordered pairs (l1, l2) of distinct
+ * occupied orbitals, count[k] pairs with n_l1 - n_l2 = qvec[3k..3k+2] and
+ * exv[k] = (1/Omega) 4 pi / |k_l1 - k_l2|^2 of the cell's last pair in the
+ * reference's loop order, cells in its print order (q_x, then q_y, then q_z
+ * ascending).  With qvec, count or exv NULL, *ncell receives the cell count;
+ * else *ncell is the capacity on entry and the count on return.  A host
+ * computation (no device). */
+int uegccd_exchange_structure_factor(const uegccd_heg_params* p, int32_t* ncell, int32_t* qvec,
+                                     int32_t* count, double* exv);
 
 const char* uegccd_strerror(int status);
 

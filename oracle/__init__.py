@@ -102,6 +102,11 @@ def _lib(variant="serial"):
         lib.orc_sf.restype = ctypes.c_int
         lib.orc_sf.argtypes = [P, D, D, D, ctypes.c_int, ctypes.POINTER(ctypes.c_int), D, D]
         lib.orc_drpa_energy.argtypes = [P, D, ctypes.c_double]
+        lib.orc_mp2_sf.restype = ctypes.c_int
+        lib.orc_mp2_sf.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), D, D]
+        lib.orc_exsf.restype = ctypes.c_int
+        lib.orc_exsf.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int), D]
         lib.orc_drpa_terms.argtypes = [P, ctypes.c_int, D, D, D]
         lib.orc_drpa.restype = ctypes.c_int
         lib.orc_drpa.argtypes = [P, ctypes.c_double, ctypes.c_int, D, D, D, D, I, D, D]
@@ -194,6 +199,26 @@ class OracleSystem:
         self._lib.orc_sf(self._h, _dp(taa), _dp(tab), _dp(tba), n,
                          q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(vals), ctypes.byref(e))
         return q[:n], vals[:n], e.value
+
+    def mp2_structure_factor(self):
+        """DrvMBPT's structure factor: (q [n,3], vals [n,3] = SfG, SfSum, SfV, E(2))."""
+        e = ctypes.c_double(0.0)
+        n = self._lib.orc_mp2_sf(self._h, 0, None, None, ctypes.byref(e))
+        q = np.zeros((max(n, 1), 3), dtype=np.int32)
+        vals = np.zeros((max(n, 1), 3))
+        self._lib.orc_mp2_sf(self._h, n, q.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(vals),
+                             ctypes.byref(e))
+        return q[:n], vals[:n], e.value
+
+    def exchange_structure_factor(self):
+        """print_ex_SF at this system's r_s: (q [n,3], count [n], ExV [n])."""
+        I = ctypes.POINTER(ctypes.c_int)
+        n = self._lib.orc_exsf(self._h, 0, None, None, None)
+        q = np.zeros((max(n, 1), 3), dtype=np.int32)
+        cnt = np.zeros(max(n, 1), dtype=np.int32)
+        v = np.zeros(max(n, 1))
+        self._lib.orc_exsf(self._h, n, q.ctypes.data_as(I), cnt.ctypes.data_as(I), _dp(v))
+        return q[:n], cnt[:n], v[:n]
 
     def terms(self, tab, tba, mask=31):
         """G and R for the given amplitudes (TermsWrapper)."""

@@ -474,6 +474,114 @@ int orc_sf(const orc_system* s, const double* taaaa, const double* tabab, const 
     return rows;
 }
 
+/* DrvMBPT's structure factor, source/MP2.f90:56-117: the MP2 amplitudes of
+   each valid (i,j,a) binned at q = n_a - n_i (+(T^aa + T^ab)) and at
+   q = n_a - n_j (-(T^aa + T^ba)), SfG = |q| 2 pi / L and SfV the integral of
+   the cell's last writer; the cells with SfG > 0 in print order, vals[3k..] =
+   (SfG, SfSum, SfV).  Returns the row count; *e2 the energy. */
+int orc_mp2_sf(const orc_system* s, int cap, int* q, double* vals, double* e2) {
+    const int o = s->nocc, n = s->nao;
+    int mx = s->nvec[0];
+    for (int k = 0; k < 3 * n; ++k) mx = s->nvec[k] > mx ? s->nvec[k] : mx;
+    const int len = mx * 2 + 1, w = 2 * len + 1;
+    const size_t cells = (size_t)w * w * w;
+    double* sum = (double*)calloc(cells, sizeof(double));
+    double* sg = (double*)calloc(cells, sizeof(double));
+    double* sv = (double*)calloc(cells, sizeof(double));
+    const double sc = pow(2.0 * PI / s->L, 2.0);
+    double e = 0.0;
+    for (int i = 1; i <= o; ++i)
+        for (int j = 1; j <= o; ++j)
+            for (int a = o + 1; a <= n; ++a) {
+                int b = find_index(s, i, j, a);
+                if (b <= o) continue;
+                double vab = eri(s, i, j, a, b, 0);
+                double vba = eri(s, i, j, b, a, 0);
+                if (fmin(vab, vba) < -80.0) continue;
+                double den = s->eigen[i - 1] + s->eigen[j - 1] - s->eigen[a - 1] - s->eigen[b - 1];
+                double tab = vab / den, tba = -vba / den, taa = tab + tba;
+                e = e + 0.5 * taa * (vab - vba) + 0.5 * tab * vab - 0.5 * tba * vba;
+                for (int ex = 0; ex < 2; ++ex) {
+                    const int r = ex ? j : i;
+                    int qv[3], q2 = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        qv[d] = s->nvec[(a - 1) * 3 + d] - s->nvec[(r - 1) * 3 + d];
+                        q2 += qv[d] * qv[d];
+                    }
+                    const size_t c = ((size_t)(qv[0] + len) * w + (qv[1] + len)) * w + (qv[2] + len);
+                    if (!ex) { sum[c] += taa + tab; sv[c] = vab; }
+                    else { sum[c] -= taa + tba; sv[c] = vba; }
+                    sg[c] = sqrt(q2 * sc);
+                }
+            }
+    int rows = 0;
+    for (size_t c = 0; c < cells; ++c) {
+        if (!(sg[c] > 0.0)) continue;
+        if (rows < cap) {
+            q[3 * rows] = (int)(c / ((size_t)w * w)) - len;
+            q[3 * rows + 1] = (int)((c / w) % w) - len;
+            q[3 * rows + 2] = (int)(c % w) - len;
+            vals[3 * rows] = sg[c]; vals[3 * rows + 1] = sum[c]; vals[3 * rows + 2] = sv[c];
+        }
+        ++rows;
+    }
+    free(sum); free(sg); free(sv);
+    if (e2) *e2 = e;
+    return rows;
+}
+
+/* print_ex_SF, source/HEG.f90:617-674, at the system's current r_s: over
+   ordered pairs of distinct occupied orbitals, the count at q = n_l1 - n_l2
+   and ExV = (1/Omega) 4 pi / |k_l1 - k_l2|^2 (the last pair's), cells with a
+   count in print order.  Returns the row count. */
+int orc_exsf(const orc_system* s, int cap, int* q, int* count, double* exv) {
+    const int no = s->nel / 2;
+    int qmax[3] = {0, 0, 0};
+    for (int l1 = 1; l1 <= no; ++l1)
+        for (int l2 = 1; l2 <= no; ++l2) {
+            if (l2 == l1) continue;
+            for (int d = 0; d < 3; ++d) {
+                int t = abs(s->nvec[(l1 - 1) * 3 + d] - s->nvec[(l2 - 1) * 3 + d]);
+                if (t > qmax[d]) qmax[d] = t;
+            }
+        }
+    int maxq = qmax[0];
+    if (qmax[1] > maxq) maxq = qmax[1];
+    if (qmax[2] > maxq) maxq = qmax[2];
+    maxq += 1;
+    const int w = 2 * maxq + 1;
+    int* cnt = (int*)calloc((size_t)w * w * w, sizeof(int));
+    double* v = (double*)calloc((size_t)w * w * w, sizeof(double));
+    for (int l1 = 1; l1 <= no; ++l1)
+        for (int l2 = 1; l2 <= no; ++l2) {
+            if (l2 == l1) continue;
+            int qv[3];
+            double dk[3], dd = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                qv[d] = s->nvec[(l1 - 1) * 3 + d] - s->nvec[(l2 - 1) * 3 + d];
+                dk[d] = s->kvec[(l1 - 1) * 3 + d] - s->kvec[(l2 - 1) * 3 + d];
+            }
+            dd = dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2];
+            const size_t c = ((size_t)(qv[0] + maxq) * w + (qv[1] + maxq)) * w + (qv[2] + maxq);
+            cnt[c] += 1;
+            v[c] = (1.0 / s->omega) * (4.0 * PI / dd);
+        }
+    int rows = 0;
+    for (size_t c = 0; c < (size_t)w * w * w; ++c) {
+        if (!(cnt[c] > 1.0e-10)) continue;
+        if (rows < cap) {
+            q[3 * rows] = (int)(c / ((size_t)w * w)) - maxq;
+            q[3 * rows + 1] = (int)((c / w) % w) - maxq;
+            q[3 * rows + 2] = (int)(c % w) - maxq;
+            count[rows] = cnt[c];
+            exv[rows] = v[c];
+        }
+        ++rows;
+    }
+    free(cnt); free(v);
+    return rows;
+}
+
 /* GetG2Drivers, source/CCD.f90:465-504. */
 static void g2_drivers(const orc_system* s, double* gab, double* gba) {
     const int o = s->nocc, n = s->nao;

@@ -58,6 +58,10 @@ double orc_mp2(const orc_system* s, double* taaaa, double* tabab, double* tabba)
 /* CCEnergy (source/CCD.f90:186-237). */
 int    orc_sf(const orc_system* s, const double* taaaa, const double* tabab, const double* tabba,
               int cap, int* q, double* vals, double* ecorr);
+/* DrvMBPT's structure factor (source/MP2.f90:56-117): rows (SfG, SfSum, SfV). */
+int    orc_mp2_sf(const orc_system* s, int cap, int* q, double* vals, double* e2);
+/* print_ex_SF (source/HEG.f90:617-674) at the current r_s: count and ExV per cell. */
+int    orc_exsf(const orc_system* s, int cap, int* q, int* count, double* exv);
 double orc_energy(const orc_system* s, const double* taaaa, const double* tabab,
                   const double* tabba);
 

@@ -185,3 +185,19 @@ def test_fort59_records_reproduce_reference_bytes(fixtures):
         else:
             out = record(*items)
         assert out == line, (out, line)
+
+
+@pytest.mark.parametrize("name", ["n14_cut5", "n24_rs5", "n54_cut16", "n14_cut5_tw"])
+def test_exchange_structure_factor_vs_oracle(name):
+    """uegccd_exchange_structure_factor (print_ex_SF, unit 82; host code)
+    against the oracle at the Input's r_s and at r_s = 1, the two blocks
+    print_HF writes per basis (source/HEG.f90:319,531)."""
+    from uegccd_amd.driver import exchange_structure_factor
+    info = info_for(name)
+    for rs in (info.rSMin, 1.0):
+        q, cnt, v = exchange_structure_factor(info, rs=rs)
+        o = oracle_for(info_for(name, rSMin=rs, rSMax=rs))
+        oq, ocnt, ov = o.exchange_structure_factor()
+        np.testing.assert_array_equal(q, oq)
+        np.testing.assert_array_equal(cnt, ocnt)
+        np.testing.assert_allclose(v, ov, rtol=1e-14)

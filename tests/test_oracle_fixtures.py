@@ -344,3 +344,44 @@ def test_openmp_build_gives_the_same_bits():
     r, ro = s.ccd(), omp.ccd()
     assert r["niter"] == ro["niter"] and r["ecorr"] == ro["ecorr"]
     np.testing.assert_array_equal(r["trace_e"], ro["trace_e"])
+
+
+def test_mp2_structure_factor_pins(fixtures):
+    """DrvMBPT's binning (unit 81): the cells and (SfG, SfSum, SfV) equal the
+    first columns of CCEnergyAndSF's on the MP2 amplitudes (pinned above),
+    sum_q SfV SfSum = 2 E(2), and E(2) is the reference's printed value."""
+    n24 = fixtures["n24"]
+    s = _sys(n24["params"])
+    q, vals, e2 = s.mp2_structure_factor()
+    assert abs(e2 - n24["e2"]) < F18_12
+    _, _, tab, tba = s.mp2()
+    qs, vs, _ = s.structure_factor(tab, tba)
+    np.testing.assert_array_equal(q, qs)
+    np.testing.assert_array_equal(vals, vs[:, :3])
+    assert abs((vals[:, 2] * vals[:, 1]).sum() - 2.0 * e2) < 1e-13
+
+
+def test_exchange_structure_factor_pins(fixtures):
+    """print_ex_SF (unit 82): the counts are the ordered occupied pairs by
+    transfer (brute force), ExV is the Coulomb kernel 1/(pi L |q|^2), and
+    sum_q count ExV is minus the HF exchange energy, which with the kinetic
+    energy and the Madelung term gives the reference's printed E(SCF)."""
+    n24 = fixtures["n24"]
+    s = _sys(n24["params"])
+    q, cnt, exv = s.exchange_structure_factor()
+    no = s.nocc
+    n = s.nvec[:no]
+    pairs = {}
+    for a in range(no):
+        for b in range(no):
+            if a != b:
+                k = tuple(n[a] - n[b])
+                pairs[k] = pairs.get(k, 0) + 1
+    assert sorted(pairs) == [tuple(x) for x in q]
+    assert [pairs[tuple(x)] for x in q] == list(cnt)
+    q2 = (q.astype(float) ** 2).sum(axis=1)
+    np.testing.assert_allclose(exv, 1.0 / (np.pi * s.L * q2), rtol=1e-13)
+    tw = np.array(n24["params"]["twist"])
+    kin = ((2.0 * np.pi / s.L) ** 2 * ((n + tw) ** 2).sum(axis=1)).sum()  # 2 x sum 1/2 |k|^2
+    xhf = -(cnt * exv).sum()
+    assert abs(kin + xhf + s.madelung * 0.5 * 2 * no - n24["ehf"]) < F18_12

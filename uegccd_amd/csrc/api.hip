@@ -942,8 +942,17 @@ int uegccd_energy(uegccd_ctx* c, const double* tab, const double* tba, double* e
   return st.nonfinite ? UEGCCD_ERR_NONFINITE : UEGCCD_OK;
 }
 
-int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba, int32_t* ncell,
-                            int32_t* qvec, double* vals, double* ecorr) {
+}  // extern "C"
+
+namespace {
+
+// The structure factor of the amplitudes in tcur (from_mp2: first written as
+// DrvMBPT's guess) on the cells the reference prints: with qvec or vals NULL
+// a size query, else nval values per cell (SfG, SfSum, SfV[, SfDSum,
+// SfXSum]) with SfV from table slot vslot.  Shared by CCEnergyAndSF (unit
+// 80) and DrvMBPT's binning (unit 81).
+int structure_factor_cells(uegccd_ctx* c, const double* tab, const double* tba, bool from_mp2, int vslot,
+                           int nval, int32_t* ncell, int32_t* qvec, double* vals, double* energy) {
   if (!c || !ncell) return UEGCCD_ERR_INVALID;
   const int o = c->o, nq = (int)c->qb_neg.size();
   // the cells printed: transfers n_a - n_i or n_a - n_j of a valid element
@@ -981,12 +990,17 @@ int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba,
     *ncell = n;
     return UEGCCD_OK;
   }
-  if (!tab || !tba || *ncell < n) return UEGCCD_ERR_INVALID;
+  if (*ncell < n || (!from_mp2 && (!tab || !tba))) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(c->device));
-  int rc = copy_in_amplitudes(c, tab, tba);
-  if (rc) return rc;
+  int rc = UEGCCD_OK;
   launch_state_reset(c->batch, PH_ITER, 1, c->stream);
-  launch_init(c->batch, c->stream);
+  if (from_mp2) {
+    launch_mp2(c->batch, c->stream);
+  } else {
+    rc = copy_in_amplitudes(c, tab, tba);
+    if (rc) return rc;
+    launch_init(c->batch, c->stream);
+  }
   launch_finalize(c->batch, FIN_SUM, c->stream);
   launch_sf(c->batch, c->v, nq, c->stream);
   HIPCHK(hipGetLastError());
@@ -996,23 +1010,41 @@ int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba,
   rc = read_state(c, &st);
   if (rc) return rc;
   const double sc = std::pow(2.0 * M_PI / c->boxL, 2.0);  // dK2scaled, source/HEG.f90:1025
+  const double* vso = c->vso.data() + (size_t)vslot * c->v * o;
   for (int k = 0; k < n; ++k) {
     const Cell& e = cells[k];
     const int r = c->qb_rows[c->qb_roff[e.q]], l = r % o, dv = r / o;
     qvec[3 * k] = e.x;
     qvec[3 * k + 1] = e.y;
     qvec[3 * k + 2] = e.z;
-    double* out = vals + 5 * (size_t)k;
+    double* out = vals + (size_t)nval * k;
     out[0] = std::sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z) * sc);
     out[1] = sums[3 * (size_t)e.q];
-    // v(n_a - n_l) = v(q) from the energy's table (IRangeEnergy, source/CCD.f90:290-291)
-    out[2] = last_core[e.q] ? 0.0 : c->vso[(size_t)c->rslot[RO_ENERGY] * c->v * o + (size_t)dv * o + l];
-    out[3] = sums[3 * (size_t)e.q + 1];
-    out[4] = sums[3 * (size_t)e.q + 2];
+    out[2] = last_core[e.q] ? 0.0 : vso[(size_t)dv * o + l];  // v(n_a - n_l) = v(q)
+    if (nval > 3) {
+      out[3] = sums[3 * (size_t)e.q + 1];
+      out[4] = sums[3 * (size_t)e.q + 2];
+    }
   }
   *ncell = n;
-  if (ecorr) *ecorr = st.ecorr;
+  if (energy) *energy = st.ecorr;
   return st.nonfinite ? UEGCCD_ERR_NONFINITE : UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uegccd_structure_factor(uegccd_ctx* c, const double* tab, const double* tba, int32_t* ncell,
+                            int32_t* qvec, double* vals, double* ecorr) {
+  // SfV from the energy's table (IRangeEnergy, source/CCD.f90:290-291)
+  if (!c) return UEGCCD_ERR_INVALID;
+  return structure_factor_cells(c, tab, tba, false, c->rslot[RO_ENERGY], 5, ncell, qvec, vals, ecorr);
+}
+
+int uegccd_mp2_structure_factor(uegccd_ctx* c, int32_t* ncell, int32_t* qvec, double* vals, double* e2) {
+  // DrvMBPT's ERI calls pass no flag (source/MP2.f90:61-62): SfV from flag 0
+  return structure_factor_cells(c, nullptr, nullptr, true, 0, 3, ncell, qvec, vals, e2);
 }
 
 int uegccd_mp2(uegccd_ctx* c, double* taa, double* tab, double* tba, double* e2) {
@@ -1418,22 +1450,22 @@ extern "C" int uegccd_drpa(uegccd_ctx* c, double* t2, uegccd_result* res, double
   c->last_niter = st.niter;  // the (E, dT) lines uegccd_trace returns
   const int status = fill_result(c, st, ms, res);
   res->failed = (st.maxres > 10000.0 * c->tolmax) || status != UEGCCD_OK;  // source/dRPA.f90:103-104
-  // Ec(SOSEX) of the final amplitudes (CCEnergy with CSosex 1, source/dRPA.f90:200-208)
-  const size_t n3 = c->n3;
-  std::vector<double> h(2 * n3);
-  HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  // Ec(SOSEX) of the final amplitudes (CCEnergy with CSosex 1, source/dRPA.f90:200-208),
+  // reduced on the device like every other energy
   if (esosex) {
-    const int o = c->o;
-    double e = 0.0;
-    for (const uint2& m : c->cmap) {
-      const int x = (int)m.x, i = x % o, av = x / (o * o), bv = elem_partner(m) - o;
-      const double t = h[2 * (size_t)x];
-      e = e + (c->vso[(size_t)av * o + i] * t * 2 - c->vso[(size_t)bv * o + i] * t);
-    }
-    *esosex = e;
+    launch_init(b, s, 2);
+    launch_finalize(b, FIN_SUM, s);
+    HIPCHK(hipGetLastError());
+    State se{};
+    rc = read_state(c, &se);
+    if (rc) return rc;
+    *esosex = se.ecorr;
   }
   if (t2) {
+    const size_t n3 = c->n3;
+    std::vector<double> h(2 * n3);
+    HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     std::fill(t2, t2 + n3, 0.0);
     for (const uint2& m : c->cmap) t2[m.x] = h[2 * (size_t)m.x];
   }
@@ -1696,6 +1728,60 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
     ehf = ehf * 2.0 + mad * 0.5 * nel;
     heg_scalars(p, g, ehf, scalars);
   }
+  return UEGCCD_OK;
+}
+
+// print_ex_SF (source/HEG.f90:617-674), which print_HF writes to unit 82
+// after every basis set-up (at r_s = 1, source/HEG.f90:319) and every change
+// of r_s (source/HEG.f90:531): over ordered pairs of distinct occupied
+// orbitals, the count at q = n_l1 - n_l2 and the exchange integral
+// (1/Omega) 4 pi / |k_l1 - k_l2|^2 of the cell's last pair in loop order.
+// A host routine: O(NOcc^2) bookkeeping beside the basis builder.
+extern "C" int uegccd_exchange_structure_factor(const uegccd_heg_params* p, int32_t* ncell, int32_t* qvec,
+                                                int32_t* count, double* exv) {
+  if (!p || !ncell) return UEGCCD_ERR_INVALID;
+  int32_t nao = 0;
+  int rc = uegccd_heg_build(p, &nao, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  std::vector<int32_t> nvec((size_t)nao * 3);
+  rc = uegccd_heg_build(p, &nao, nvec.data(), nullptr, nullptr);
+  if (rc) return rc;
+  const int no = p->nel / 2;
+  const HegGeom g = heg_geometry(p);
+  std::vector<double> kv((size_t)no * 3);
+  for (int q = 0; q < no; ++q)
+    for (int d = 0; d < 3; ++d) {
+      const double nt = nvec[(size_t)d * nao + q] + p->twist[d];
+      kv[3 * q + d] = (nt * (2.0 * kPi / g.L1)) * (1.0 / g.rescale);
+    }
+  std::map<std::tuple<int, int, int>, std::pair<int, double>> cells;  // (q_x, q_y, q_z) order
+  for (int l1 = 0; l1 < no; ++l1)
+    for (int l2 = 0; l2 < no; ++l2) {
+      if (l1 == l2) continue;
+      const auto key = std::make_tuple(nvec[l1] - nvec[l2], nvec[(size_t)nao + l1] - nvec[(size_t)nao + l2],
+                                       nvec[2 * (size_t)nao + l1] - nvec[2 * (size_t)nao + l2]);
+      const double d0 = kv[3 * l1] - kv[3 * l2], d1 = kv[3 * l1 + 1] - kv[3 * l2 + 1],
+                   d2v = kv[3 * l1 + 2] - kv[3 * l2 + 2];
+      auto& cell = cells[key];
+      cell.first += 1;
+      cell.second = (1.0 / g.omega) * (4.0 * kPi / (d0 * d0 + d1 * d1 + d2v * d2v));
+    }
+  const int n = (int)cells.size();
+  if (!qvec || !count || !exv) {
+    *ncell = n;
+    return UEGCCD_OK;
+  }
+  if (*ncell < n) return UEGCCD_ERR_INVALID;
+  int k = 0;
+  for (const auto& e : cells) {
+    qvec[3 * k] = std::get<0>(e.first);
+    qvec[3 * k + 1] = std::get<1>(e.first);
+    qvec[3 * k + 2] = std::get<2>(e.first);
+    count[k] = e.second.first;
+    exv[k] = e.second.second;
+    ++k;
+  }
+  *ncell = n;
   return UEGCCD_OK;
 }
 

@@ -210,8 +210,10 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
     const double2 t = S.tcur[x];
     st2(S.tk + elem_pos(cm), t);
     const double taa = t.x + t.y;
-    if (drpa)  // the dRPA module's CCEnergy with CSosex 0 (source/dRPA.f90:200-208)
+    if (drpa == 1)  // the dRPA module's CCEnergy with CSosex 0 (source/dRPA.f90:200-208)
       e = VPQ(a, i) * t.x * 2;
+    else if (drpa == 2)  // the same with CSosex 1: Ec(SOSEX)
+      e = VPQ(a, i) * t.x * 2 - VPQ(elem_partner(cm), i) * t.x;
     else
       e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
     tmax = fmax(fabs(t.x), fabs(t.y));

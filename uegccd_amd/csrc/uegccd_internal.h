@@ -228,7 +228,8 @@ constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
 
 // Kernel launchers (kernels.hip).
 // drpa: the spin-summed amplitudes of the dRPA iteration (in the first
-// component of tcur, the second zero) and its energy
+// component of tcur, the second zero) and its energy (1: Ec(dRPA), 2:
+// Ec(SOSEX))
 void launch_init(const BatchDev& b, hipStream_t s, int drpa = 0);
 void launch_mp2(const BatchDev& b, hipStream_t s, int drpa = 0);
 // One dRPA pass (source/dRPA.f90:80-96) of a batch-of-one: the row and

@@ -73,6 +73,29 @@ def init_heg(info: UEGInfo, twist=None, device: int | None = None) -> HEGData:
                    DenomFactor=sc[5], TolMax=sc[6], FailRatio=sc[7])
 
 
+def exchange_structure_factor(info: UEGInfo, rs: float | None = None, twist=None):
+    """print_ex_SF (source/HEG.f90:617-674; unit 82) of the basis at r_s (the
+    Input's rSMin by default): (q [n,3], count [n], ExV [n])."""
+    lib = L.load()
+    p = L.HegParams()
+    p.nel, p.maxkpoint = info.NElectron, info.MaxKPoint
+    p.rs = info.rSMin if rs is None else rs
+    tw = info.NTwist if twist is None else twist
+    for d in range(3):
+        p.twist[d] = float(tw[d])
+    p.fcut, p.corefcut = info.fcut, info.corefcut
+    p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+    n = ctypes.c_int32(0)
+    L.check(lib.uegccd_exchange_structure_factor(ctypes.byref(p), ctypes.byref(n), None, None, None),
+            "uegccd_exchange_structure_factor")
+    q = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+    cnt = np.zeros(max(n.value, 1), dtype=np.int32)
+    v = np.zeros(max(n.value, 1))
+    L.check(lib.uegccd_exchange_structure_factor(ctypes.byref(p), ctypes.byref(n), L.iptr(q), L.iptr(cnt),
+                                                 L.dptr(v)), "uegccd_exchange_structure_factor")
+    return q[:n.value], cnt[:n.value], v[:n.value]
+
+
 def options(mode="reference", ndiis=8, precision="fp64", **kw) -> L.Options:
     lib = L.load()
     o = L.Options()
@@ -191,6 +214,21 @@ class CCDContext:
             self._h, L.dptr(tab), L.dptr(tba), ctypes.byref(n),
             q.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), L.dptr(vals), ctypes.byref(e)),
             "uegccd_structure_factor")
+        return q[:n.value], vals[:n.value], e.value
+
+    def mp2_structure_factor(self):
+        """uegccd_mp2_structure_factor: DrvMBPT's binning of the MP2 amplitudes
+        (source/MP2.f90:86-117; unit 81): (q [n,3], vals [n,3] = SfG, SfSum,
+        SfV, E(2))."""
+        n = ctypes.c_int32(0)
+        L.check(self._lib.uegccd_mp2_structure_factor(self._h, ctypes.byref(n), None, None, None),
+                "uegccd_mp2_structure_factor")
+        q = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+        vals = np.zeros((max(n.value, 1), 3))
+        e = ctypes.c_double(0.0)
+        L.check(self._lib.uegccd_mp2_structure_factor(
+            self._h, ctypes.byref(n), q.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), L.dptr(vals),
+            ctypes.byref(e)), "uegccd_mp2_structure_factor")
         return q[:n.value], vals[:n.value], e.value
 
     def drpa(self, t2=None, allow_unconverged=False):

@@ -51,6 +51,37 @@ Contains
     Allocate(HEGData%nVec(NAO,3), HEGData%Eigen(NAO))
     HEGData%nVec = Reshape(NVecC, [Int(NAO), 3])
     HEGData%Eigen = Eig
+    ! print_HF's exchange structure factor: once for the set-up at r_s = 1
+    ! and once after the change to r_s (source/HEG.f90:319,531,613)
+    Call WriteExSF(P, 1.0_c_double)
+    Call WriteExSF(P, UEGInfo%rSMin)
   End Subroutine InitHEG
+
+  ! print_ex_SF (source/HEG.f90:617-674) to unit 82, from the library.
+  Subroutine WriteExSF(P0, RS)
+    Type (uegccd_heg_params), Intent(In) :: P0
+    Real(c_double), Intent(In) :: RS
+    Type (uegccd_heg_params) :: P
+    Integer(c_int32_t), Allocatable, Target :: Q(:,:), Cnt(:)
+    Real(c_double), Allocatable, Target :: V(:)
+    Integer(c_int32_t) :: N
+    Integer :: K
+
+    P = P0
+    P%rs = RS
+    N = 0
+    Call Check(uegccd_exchange_structure_factor(P, N, c_null_ptr, c_null_ptr, c_null_ptr), &
+               'uegccd_exchange_structure_factor')
+    Allocate(Q(3, Max(N, 1)), Cnt(Max(N, 1)), V(Max(N, 1)))
+    Call Check(uegccd_exchange_structure_factor(P, N, c_loc(Q), c_loc(Cnt), c_loc(V)), &
+               'uegccd_exchange_structure_factor')
+    Write(82,*) "--------"
+    Write(82,*) "Starting a new structure factor"
+    Write(82,*) "--------"
+    Do K = 1, N
+      Write(82,*) Q(1,K), Q(2,K), Q(3,K), Cnt(K), V(K)
+    End Do
+    Flush(82)
+  End Subroutine WriteExSF
 
 End Module HEG

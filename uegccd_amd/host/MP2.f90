@@ -45,6 +45,7 @@ Contains
     Write(7,1020) HEGData%ECorr + HEGData%EHF
     Write(7,1000)
     Close(7)
+    Call WriteMP2SF()
 
 1000  Format(14x,'**************************************************')
 1005  Format(14x,'* E(SCF) is ',12x,F18.12,' a.u.',2x,'*')
@@ -57,5 +58,29 @@ Contains
 
     If (UEGInfo%DoOnlyMP2Grid) Stop 'DoOnlyMP2Grid flag is true, so calculation stopping here'
   End Subroutine DrvMBPT
+
+  ! DrvMBPT's structure factor (source/MP2.f90:86-117), written to unit 81 on
+  ! every call as the reference does: cells, SfG, SfSum, SfV from the library.
+  Subroutine WriteMP2SF()
+    Integer(c_int32_t), Allocatable, Target :: Q(:,:)
+    Real(c_double), Allocatable, Target :: V(:,:)
+    Integer(c_int32_t) :: N
+    Real(c_double) :: E
+    Integer :: K
+
+    N = 0
+    Call Check(uegccd_mp2_structure_factor(SolverContext(), N, c_null_ptr, c_null_ptr, E), &
+               'uegccd_mp2_structure_factor')
+    Allocate(Q(3, Max(N, 1)), V(3, Max(N, 1)))
+    Call Check(uegccd_mp2_structure_factor(SolverContext(), N, c_loc(Q), c_loc(V), E), &
+               'uegccd_mp2_structure_factor')
+    Write(81,*) "--------"
+    Write(81,*) "Starting a new structure factor"
+    Write(81,*) "--------"
+    Do K = 1, N
+      Write(81,*) Q(1,K), Q(2,K), Q(3,K), V(1,K), V(2,K), V(3,K)
+    End Do
+    Flush(81)
+  End Subroutine WriteMP2SF
 
 End Module MP2

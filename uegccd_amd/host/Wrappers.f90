@@ -115,6 +115,23 @@ Module Wrappers
       Real(c_double), Intent(Out) :: ecorr
     End Function
 
+    Integer(c_int) Function uegccd_mp2_structure_factor(ctx, ncell, qvec, vals, e2) &
+        Bind(C, name="uegccd_mp2_structure_factor")
+      Import :: c_int, c_int32_t, c_ptr, c_double
+      Type(c_ptr), Value :: ctx
+      Integer(c_int32_t), Intent(InOut) :: ncell
+      Type(c_ptr), Value :: qvec, vals
+      Real(c_double), Intent(Out) :: e2
+    End Function
+
+    Integer(c_int) Function uegccd_exchange_structure_factor(p, ncell, qvec, count, exv) &
+        Bind(C, name="uegccd_exchange_structure_factor")
+      Import :: c_int, c_int32_t, c_ptr, uegccd_heg_params
+      Type(uegccd_heg_params), Intent(In) :: p
+      Integer(c_int32_t), Intent(InOut) :: ncell
+      Type(c_ptr), Value :: qvec, count, exv
+    End Function
+
   End Interface
 
 Contains

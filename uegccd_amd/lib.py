@@ -31,7 +31,8 @@ EXPORTS = (
     "uegccd_server_step", "uegccd_server_drain", "uegccd_server_poll",
     "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
     "uegccd_heg_build_device", "uegccd_connectivity", "uegccd_drpa",
-    "uegccd_structure_factor", "uegccd_solve_noalloc",
+    "uegccd_structure_factor", "uegccd_solve_noalloc", "uegccd_mp2_structure_factor",
+    "uegccd_exchange_structure_factor",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -130,6 +131,8 @@ def load():
     lib.uegccd_drpa.argtypes = [P, D, ctypes.POINTER(Result), D]
     I32 = ctypes.POINTER(ctypes.c_int32)
     lib.uegccd_structure_factor.argtypes = [P, D, D, I32, I32, D, D]
+    lib.uegccd_mp2_structure_factor.argtypes = [P, I32, I32, D, D]
+    lib.uegccd_exchange_structure_factor.argtypes = [ctypes.POINTER(HegParams), I32, I32, I32, D]
     lib.uegccd_heg_build.argtypes = [ctypes.POINTER(HegParams), I32, I32, D, D]
     lib.uegccd_heg_build_device.argtypes = [ctypes.POINTER(HegParams), ctypes.c_int32, I32, I32, D, D]
     lib.uegccd_strerror.argtypes = [ctypes.c_int]
