@@ -465,6 +465,18 @@ def main():
                  "frac_other_roof": min(f_mfma, f_hbm),
                  "traffic_over_algo": (traffic / nbytes) if traffic else None})
 
+    # the same solve with every pass launched kernel by kernel (no graph replay)
+    nctx = CCDContext(heg, info, options(mode="diis", ndiis=8, precision=args.precision, graph=0))
+    nlat = []
+    for k in range(11):
+        s0 = time.perf_counter()
+        L.check(lib.uegccd_solve_resident(nctx.handle, 1, ctypes.byref(res)), "solve without graphs")
+        if k:
+            nlat.append(1000.0 * (time.perf_counter() - s0))
+    nctx.close()
+    graph_ab = {"ms_per_solve_graph_replay": 1000.0 * wall / args.steps,
+                "ms_per_solve_kernel_launches": float(np.mean(nlat)), "same_energy": res.ecorr == ecorr}
+
     mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
     stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
     drpa = drpa_c2(lib, L, heg, info, CCDContext, options, 10, not args.no_cpu)
@@ -488,6 +500,7 @@ def main():
         "p50_ms": float(np.percentile(lat_sorted, 50)), "p99_ms": float(np.percentile(lat_sorted, 99)),
         "kernel_us_in_solve": kt,
         "kernels": per_kernel,
+        "graph": graph_ab,
         "roofline": roof,
         "cpu_baseline": cpu,
         # the same algorithm on both sides: the reference-mode solve (the

@@ -98,6 +98,10 @@ typedef struct {
     double mixed_switch;   /* MIXED: FP32 ladders while max|R| >= this       */
     int32_t device;        /* HIP device ordinal                             */
     int32_t iters_per_sync;/* iterations launched between host polls (>=1)  */
+    int32_t graph;         /* 1: each chunk of passes between host polls is
+                              captured once as a HIP graph and replayed
+                              (single-context solves); 0: launched kernel by
+                              kernel                                        */
 } uegccd_options;
 
 typedef struct {

@@ -89,3 +89,26 @@ def test_nonfinite_is_isolated_in_a_batch():
     res = solve_batch([good, bad], [tab, btab], [tba, btab])
     assert res[0].status == 0 and abs(res[0].ecorr - ref.ecorr) < 1e-12
     assert res[1].status != 0
+
+
+@pytest.mark.parametrize("name,opt", [("n54_cut16", dict(mode="diis", ndiis=8)),
+                                      ("n24_rs5", dict(mode="reference")),
+                                      ("n38_cut9", dict(mode="diis", ndiis=6, precision="mixed"))])
+def test_graph_replay_equals_launches(name, opt):
+    """Chunks of passes replayed from captured HIP graphs give the same bits
+    as the same passes launched kernel by kernel.  Each solve replays its
+    chunk graphs several times with the host's state polls (copies and
+    synchronisations) between them, and three solves in a row reuse the
+    captured graphs."""
+    print(f"[graph] {name} {opt}", flush=True)
+    info = info_for(name)
+    ref = _ctx(info, graph=0, **opt)
+    _, _, tab, tba = ref.mp2()
+    r0 = ref.solve(tab, tba)
+    g = _ctx(info, graph=1, **opt)
+    for _ in range(3):
+        r = g.solve(tab, tba)
+        assert r.niter == r0.niter and r.ecorr == r0.ecorr
+        np.testing.assert_array_equal(r.tab, r0.tab)
+        np.testing.assert_array_equal(r.tba, r0.tba)
+        np.testing.assert_array_equal(r.trace_e, r0.trace_e)

@@ -205,6 +205,8 @@ struct uegccd_ctx {
   // the allocation-free solve: no events, no pinned buffers (polls into
   // pageable memory, host-clock timing)
   bool no_alloc = false;
+  // captured chunks of passes of the context's own batch, by chunk length
+  std::map<int, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -614,6 +616,7 @@ void uegccd_default_options(uegccd_options* opt) {
   opt->mixed_switch = 1e-5;
   opt->device = 0;
   opt->iters_per_sync = 8;
+  opt->graph = 1;
 }
 
 const char* uegccd_strerror(int st) {
@@ -875,6 +878,7 @@ int uegccd_footprint(const uegccd_system* sys, const uegccd_options* opt, int64_
 void uegccd_ctx_destroy(uegccd_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
+  for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
   for (hipEvent_t e : c->tev) hipEventDestroy(e);
   for (hipEvent_t e : c->span)
     if (e) hipEventDestroy(e);
@@ -1250,6 +1254,39 @@ int next_chunk(uegccd_ctx** cs, int n, const std::vector<State>& st, std::vector
   return std::min(std::max(want, 1), cap);
 }
 
+// The chunk of `len` passes of the context's own batch as an instantiated HIP
+// graph, captured on first use and kept (the kernels' arguments are the
+// context's fixed device tables, so a replay is the same launches).  The
+// passes hold no memset or copy nodes.
+int chunk_graph(uegccd_ctx* c, int len, hipGraphExec_t* out) {
+  auto it = c->graphs.find(len);
+  if (it != c->graphs.end()) {
+    *out = it->second;
+    return UEGCCD_OK;
+  }
+  prepare_launches();  // per-kernel attributes are set outside the capture
+  hipStream_t s = c->stream;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < len; ++k) launch_iteration(c->batch, MF_ITERATE, s);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  if (ec != hipSuccess || !g) {
+    std::fprintf(stderr, "uegccd: graph capture failed: %s\n", hipGetErrorString(ec));
+    if (g) hipGraphDestroy(g);
+    return UEGCCD_ERR_HIP;
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    std::fprintf(stderr, "uegccd: graph instantiation failed: %s\n", hipGetErrorString(ei));
+    return UEGCCD_ERR_HIP;
+  }
+  c->graphs[len] = ex;
+  *out = ex;
+  return UEGCCD_OK;
+}
+
 // Runs the iteration of a batch already holding its starting amplitudes in
 // tcur: init (E, dT of the guess), then passes until every system is done.
 int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vector<State>& st,
@@ -1306,8 +1343,18 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   // a timed first kernel of the pass takes its `before` event from the
   // previous pass of the chunk, so the first pass of each chunk is not timed
   const bool timed_first = timing && c0->timed_kernel == first_pass_kernel(b, MF_ITERATE);
+  // a single context's passes replay as captured graphs (not while a kernel
+  // is timed with events, nor without device allocation: instantiating a
+  // graph allocates)
+  const bool graphs = n == 1 && &b == &c0->batch && c0->opt.graph && !timing && !no_alloc;
   while (!all_done && passes <= maxiter + 2) {
-    for (int k = 0; k < chunk; ++k) {
+    if (graphs) {
+      hipGraphExec_t ex = nullptr;
+      const int rc = chunk_graph(c0, chunk, &ex);
+      if (rc) return rc;
+      HIPCHK(hipGraphLaunch(ex, s));
+    }
+    for (int k = 0; k < chunk && !graphs; ++k) {
       if (timing) {
         const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
         const hipEvent_t tail = (timed_first && k + 1 < chunk) ? c0->tev[2 * (k + 1)] : nullptr;

@@ -1864,6 +1864,8 @@ static void launch_k(F kernel, hipEvent_t stop, dim3 grid, dim3 block, uint32_t 
   hipExtLaunchKernelGGL(kernel, grid, block, lds, s, nullptr, stop, 0, args...);
 }
 
+void prepare_launches() { allow_large_lds(); }
+
 static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   allow_large_lds();
   const auto* sd = (const SolveDev*)b.solves;

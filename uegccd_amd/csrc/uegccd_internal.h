@@ -246,6 +246,9 @@ struct KernelTimer {
   int which;
   hipEvent_t before, after;
 };
+// One-time per-kernel set-up (dynamic LDS limits), done before any stream
+// capture.
+void prepare_launches();
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s,
                       const KernelTimer* timer = nullptr, hipEvent_t tail_stop = nullptr);
 int first_pass_kernel(const BatchDev& b, int main_flags);
