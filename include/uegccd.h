@@ -102,6 +102,10 @@ typedef struct {
                               captured once as a HIP graph and replayed
                               (single-context solves); 0: launched kernel by
                               kernel                                        */
+    int32_t persistent;    /* 1: each chunk of passes of a single-context solve
+                              runs as one cooperative launch whose phases are
+                              separated by grid-wide barriers (the persistent
+                              form of the five launches of a pass); 0: off    */
 } uegccd_options;
 
 typedef struct {

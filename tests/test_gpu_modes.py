@@ -112,3 +112,23 @@ def test_graph_replay_equals_launches(name, opt):
         np.testing.assert_array_equal(r.tab, r0.tab)
         np.testing.assert_array_equal(r.tba, r0.tba)
         np.testing.assert_array_equal(r.trace_e, r0.trace_e)
+
+
+@pytest.mark.parametrize("name,opt", [("n54_cut16", dict(mode="diis", ndiis=8)),
+                                      ("n14_cut5", dict(mode="reference")),
+                                      ("n38_cut9", dict(mode="diis", ndiis=6, precision="mixed"))])
+def test_persistent_kernel_equals_launches(name, opt):
+    """The persistent pass kernel (a chunk of passes in one cooperative launch,
+    grid barriers between the phases) gives the same bits as the five
+    launches per pass: the same work items, each computed in the same order."""
+    print(f"[persistent] {name} {opt}", flush=True)
+    info = info_for(name)
+    ref = _ctx(info, graph=0, **opt)
+    _, _, tab, tba = ref.mp2()
+    r0 = ref.solve(tab, tba)
+    p = _ctx(info, persistent=1, **opt)
+    for _ in range(2):
+        r = p.solve(tab, tba)
+        assert r.niter == r0.niter and r.ecorr == r0.ecorr
+        np.testing.assert_array_equal(r.tab, r0.tab)
+        np.testing.assert_array_equal(r.tba, r0.tba)

@@ -1346,15 +1346,23 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   // a single context's passes replay as captured graphs (not while a kernel
   // is timed with events, nor without device allocation: instantiating a
   // graph allocates)
-  const bool graphs = n == 1 && &b == &c0->batch && c0->opt.graph && !timing && !no_alloc;
+  const bool persist = n == 1 && &b == &c0->batch && c0->opt.persistent && !timing && !no_alloc &&
+                       !c0->sd.slice_split;
+  const bool graphs = !persist && n == 1 && &b == &c0->batch && c0->opt.graph && !timing && !no_alloc;
+  const int persist_lds = std::max(slice_lds_bytes(c0->o), ladder_lds_bytes(c0->sd));
   while (!all_done && passes <= maxiter + 2) {
-    if (graphs) {
+    if (persist) {
+      if (launch_persistent(b, persist_lds, chunk, s)) {
+        std::fprintf(stderr, "uegccd: the persistent pass kernel could not be launched\n");
+        return UEGCCD_ERR_HIP;
+      }
+    } else if (graphs) {
       hipGraphExec_t ex = nullptr;
       const int rc = chunk_graph(c0, chunk, &ex);
       if (rc) return rc;
       HIPCHK(hipGraphLaunch(ex, s));
     }
-    for (int k = 0; k < chunk && !graphs; ++k) {
+    for (int k = 0; k < chunk && !graphs && !persist; ++k) {
       if (timing) {
         const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
         const hipEvent_t tail = (timed_first && k + 1 < chunk) ? c0->tev[2 * (k + 1)] : nullptr;

@@ -24,6 +24,7 @@
 //   jring / jxring                   3 doubles (J1, J2, J3) per (row of P(q), l)
 //   histT / histE                    valid elements only, in element order (compact
 //                                    index cpos[x]), slot-major: k*nvalid + cpos[x]
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
 
 #include "uegccd_internal.h"
@@ -1060,16 +1061,9 @@ __device__ void joo_wave(const SolveG& S, int part) {
   }
 }
 
-// One work item per wave; the four waves of a block take four consecutive
-// items of the XCD-ordered list.
-__global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
-                                                            const int* __restrict__ prefix, int n,
-                                                            int nitems) {
-  // the wave index is uniform over the wave: saying so lets the item, the
-  // descriptor and its pointers live in scalar registers (scalar loads)
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + wave;
-  if (item >= nitems) return;
+// One contraction work item, by one wave.
+__device__ void contract_item(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
+                              int item) {
   const int s = find_solve(prefix, n, item);
   const SolveG& S = as_global(solves)[s];
   const int4 it = ld(S.con_items + (item - prefix[s]));
@@ -1082,6 +1076,19 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
     hh_tile(S, it.z, it.w);
   else
     joo_wave(S, it.z);
+}
+
+// One work item per wave; the four waves of a block take four consecutive
+// items of the XCD-ordered list.
+__global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
+                                                            const int* __restrict__ prefix, int n,
+                                                            int nitems) {
+  // the wave index is uniform over the wave: saying so lets the item, the
+  // descriptor and its pointers live in scalar registers (scalar loads)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + wave;
+  if (item >= nitems) return;
+  contract_item(solves, prefix, n, item);
 }
 
 // Block partials of the residual pass: max dT, max |R|, sum E, the non-finite
@@ -1121,11 +1128,10 @@ __device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, i
 // mosaics, ladders (pp + hh from the ladder and contraction passes), rings and
 // crossed rings (slice products and the quadratic sums), then R = D*T - G,
 // the damped update, dT, the energy and the DIIS dot products.
-__global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restrict__ solves,
-                                                        const int* __restrict__ prefix, int n,
-                                                        int flags) {
-  __shared__ double red[(kElemThreads / 64) * 32];
-  const int gb = xcd_block(blockIdx.x, gridDim.x);
+// One 256-thread element block gb of the launch (block index after the XCD
+// renumbering); red and joo_l are the block's LDS.
+__device__ void main_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
+                           int gb, int flags, double* red, double* joo_l) {
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
   // the state, the element (one valid element per thread, by compact index
@@ -1141,7 +1147,6 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
     x = (int)cm.x;
     b = elem_partner(cm);
   }
-  __shared__ double joo_l[kMaxOcc];
   if (S.do_mosaic)  // Joo from its slice-range parts, in part order
     for (int k = threadIdx.x; k < o; k += blockDim.x) {
       double a = 0.0;
@@ -1253,6 +1258,14 @@ __global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restric
     main_partials<32>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, S.nvalid, red);
 }
 
+__global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restrict__ solves,
+                                                        const int* __restrict__ prefix, int n,
+                                                        int flags) {
+  __shared__ double red[(kElemThreads / 64) * 32];
+  __shared__ double joo_l[kMaxOcc];
+  main_block(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), flags, red, joo_l);
+}
+
 // Sums and maxima of one solve's element-block partials by one block, in a
 // fixed order.  Slot-major partials: quantity q of every element block is
 // contiguous; q = 1 (max dT) and q = 2 (max |R|) are maxima, E (0), the
@@ -1352,14 +1365,13 @@ __device__ bool advance_state(const SolveG& S, State* st, const double* vals, in
 // FIN_SUM after a plain energy evaluation.  The iteration state is worked on
 // in LDS: one parallel copy in, one out, instead of a chain of dependent
 // global accesses by a single thread.
-__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind,
-                                                          int prec64) {
-  __shared__ double vals[kPartialStride];
-  __shared__ double red[(kFinThreads / 64) * 32];
-  __shared__ State sst;
+// The reductions and state update of solve sidx by one kFinThreads block;
+// vals, red and sst are the block's LDS.
+__device__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, int kind, int prec64,
+                               double* vals, double* red, State& sst) {
   constexpr int kStateWords = sizeof(State) / sizeof(double);
   static_assert(kStateWords <= 2 * kFinThreads, "two state words per thread");
-  const SolveG& S = as_global(solves)[blockIdx.x];
+  const SolveG& S = as_global(solves)[sidx];
   // One round of loads: each thread's state words go out with its partials
   // (the DIIS dots are read whatever the phase; a finishing pass ignores them).
   const double* src = reinterpret_cast<const double*>(S.st);
@@ -1409,6 +1421,14 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
   double* out = reinterpret_cast<double*>(S.st);
   if (w0 < kStateWords) out[w0] = dst[w0];
   if (w1 < kStateWords) out[w1] = dst[w1];
+}
+
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind,
+                                                          int prec64) {
+  __shared__ double vals[kPartialStride];
+  __shared__ double red[(kFinThreads / 64) * 32];
+  __shared__ State sst;
+  finalize_block(solves, blockIdx.x, kind, prec64, vals, red, sst);
 }
 
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {
@@ -1829,6 +1849,94 @@ int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const do
   hipLaunchKernelGGL(k_basis_orbitals, dim3((nb + 255) / 256), dim3(256), 0, s, A, idx, nvec, eigen,
                      ehf);
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// --------------------------------------------------------------------------
+// Persistent pass kernel: a chunk of passes of one solve in one cooperative
+// launch.  Every block (256 threads) loops over each phase's work items
+// (slice blocks, ladder tiles, contraction items four to a block, element
+// blocks, the solve's finalize) and a grid-wide barrier separates the
+// phases, in place of the five launches of a pass.  The LDS of the phases
+// share one buffer; a block barrier opens every work item, since the waves
+// of a block leave an item's epilogue at different times.
+struct PersistLds {
+  union {
+    struct { Lds L; SliceDiis D; } sl;
+    struct { int4 cls[kMaxNao]; double red[3 * 64 * 8]; } la;
+    struct { double red[(kElemThreads / 64) * 32]; double joo[kMaxOcc]; } mn;
+    struct { double vals[kPartialStride]; double red[(kFinThreads / 64) * 32]; State sst; } fi;
+  } u;
+};
+
+__global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ solves,
+                                                 const int* __restrict__ slice_prefix,
+                                                 const int* __restrict__ lad_prefix,
+                                                 const int* __restrict__ con_prefix,
+                                                 const int* __restrict__ elem_prefix, int nslice, int nlad,
+                                                 int ncon, int nelem, int passes) {
+  __shared__ PersistLds P;
+  extern __shared__ double pdyn[];  // slice slabs, then the ladder tables
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int pass = 0; pass < passes; ++pass) {
+    for (int it = blockIdx.x; it < nslice; it += gridDim.x) {
+      __syncthreads();
+      slice_block<256, false>(solves, slice_prefix, 1, it, P.u.sl.L, P.u.sl.D, pdyn, nullptr);
+    }
+    grid.sync();
+    for (int it = blockIdx.x; it < nlad; it += gridDim.x) {
+      __syncthreads();
+      ladder_block(solves, lad_prefix, 1, it, pdyn, P.u.la.cls, P.u.la.red);
+    }
+    grid.sync();
+    for (int base = blockIdx.x * kHHTilesPerBlock; base < ncon; base += gridDim.x * kHHTilesPerBlock)
+      if (base + wave < ncon) contract_item(solves, con_prefix, 1, base + wave);
+    grid.sync();
+    for (int it = blockIdx.x; it < nelem; it += gridDim.x) {
+      __syncthreads();
+      main_block(solves, elem_prefix, 1, it, MF_ITERATE, P.u.mn.red, P.u.mn.joo);
+    }
+    grid.sync();
+    if (blockIdx.x == 0) finalize_block(solves, 0, FIN_ITER, 1, P.u.fi.vals, P.u.fi.red, P.u.fi.sst);
+    grid.sync();
+  }
+}
+
+int launch_persistent(const BatchDev& b, int dyn_lds, int passes, hipStream_t s) {
+  static int grid = 0, grid_lds = -1;
+  if (b.n != 1 || b.slice_split) return -1;
+  if (grid_lds != dyn_lds) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
+    if (dyn_lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_persist), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            dyn_lds) != hipSuccess)
+      return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persist, 256, dyn_lds) != hipSuccess ||
+        per_cu < 1)
+      return -1;
+    grid = per_cu * prop.multiProcessorCount;
+    grid_lds = dyn_lds;
+  }
+  const int nslice = b.slice_blocks;
+  const int nlad = b.nlad, ncon = b.ncon, nelem = b.elem_blocks;
+  // no more blocks than the largest phase has work items: a grid barrier's
+  // cost grows with the blocks that take part
+  const int need = std::max(std::max(nslice, nlad), std::max((ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock, nelem));
+  const int g = std::max(1, std::min(grid, need));
+  int cls = 0;
+  for (int q = 0; q < kSliceClasses; ++q)
+    if (b.slice_cls_blocks[q] > 0) cls = q;
+  const SolveDev* sd = b.solves;
+  const int* sp = b.slice_prefix[cls];
+  const int* lp = b.lad_prefix;
+  const int* cp = b.con_prefix;
+  const int* ep = b.elem_prefix;
+  void* args[] = {(void*)&sd, (void*)&sp, (void*)&lp, (void*)&cp, (void*)&ep, (void*)&nslice,
+                  (void*)&nlad, (void*)&ncon, (void*)&nelem, (void*)&passes};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_persist), dim3(g), dim3(256), args,
+                                    (unsigned)dyn_lds, s) == hipSuccess ? 0 : -1;
 }
 
 // --------------------------------------------------------------------------

@@ -246,6 +246,11 @@ struct KernelTimer {
   int which;
   hipEvent_t before, after;
 };
+// A chunk of `passes` passes of a batch of one as one cooperative,
+// grid-synchronised launch (dyn_lds: the larger of the slice slabs and the
+// ladder tables).  Returns nonzero when it cannot run (split slice tables,
+// launch refused).
+int launch_persistent(const BatchDev& b, int dyn_lds, int passes, hipStream_t s);
 // One-time per-kernel set-up (dynamic LDS limits), done before any stream
 // capture.
 void prepare_launches();

@@ -23,7 +23,7 @@ Module Wrappers
     Real(c_double) :: tol_dt, tol_e, tol_r, damping
     Integer(c_int32_t) :: precision
     Real(c_double) :: mixed_switch
-    Integer(c_int32_t) :: device, iters_per_sync, graph
+    Integer(c_int32_t) :: device, iters_per_sync, graph, persistent
   End Type uegccd_options
 
   Type, Bind(C) :: uegccd_result

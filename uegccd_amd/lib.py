@@ -53,7 +53,8 @@ class Options(ctypes.Structure):
                 ("tol_dt", ctypes.c_double), ("tol_e", ctypes.c_double), ("tol_r", ctypes.c_double),
                 ("damping", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("mixed_switch", ctypes.c_double), ("device", ctypes.c_int32),
-                ("iters_per_sync", ctypes.c_int32), ("graph", ctypes.c_int32)]
+                ("iters_per_sync", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("persistent", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
