@@ -188,9 +188,25 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
  * one kernel of the residual pass over the context's current amplitudes:
  * 0 per-slice amplitudes and ring/mosaic products, 1 contractions (quadratic
  * rings per momentum transfer, hole-hole ladder tiles, Joo), 2
- * particle-particle ladder and W tiles, 3 residual assembly + update.  No
- * counterpart in the reference; measurement support. */
+ * particle-particle ladder and W tiles, 3 residual assembly + update; the
+ * ladders in the context's starting precision.  No counterpart in the
+ * reference; measurement support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
+
+/* Tile choices of the tuned kernels: k-steps per group of loads in the
+ * ladder tiles (1, 2 or 4) and per round of loads in the ring (4 or 8) and
+ * hole-hole (4 or 8) contraction tiles.  A context starts from the tuned
+ * table compiled into the library (uegccd_amd/tuned_tiles.json, written by
+ * the offline sweep python -m uegccd_amd.tune), by the class of its largest
+ * pair-momentum block and its precision; uegccd_set_tiles overrides that
+ * (NULL restores it), uegccd_get_tiles reports it with the block sizes that
+ * select the class.  A batch runs the tiles of its solve with the most ladder
+ * tiles.  No counterpart in the reference; tuning support. */
+typedef struct {
+    int32_t ladder_group, quad_steps, hh_steps;
+} uegccd_tiles;
+int uegccd_set_tiles(uegccd_ctx* ctx, const uegccd_tiles* tiles);
+int uegccd_get_tiles(const uegccd_ctx* ctx, uegccd_tiles* tiles, int32_t* noo_max, int32_t* nvv_max);
 
 /* In-solve timing of one kernel (numbering as uegccd_kernel_time, plus 4 for
  * the per-solve reductions, convergence test and DIIS solve; -1 turns it

@@ -132,3 +132,25 @@ def test_persistent_kernel_equals_launches(name, opt):
         assert r.niter == r0.niter and r.ecorr == r0.ecorr
         np.testing.assert_array_equal(r.tab, r0.tab)
         np.testing.assert_array_equal(r.tba, r0.tba)
+
+
+def test_tile_choices_give_the_same_bits():
+    """The tuned kernels' tile choices change how loads are grouped, not the
+    order of the accumulation: every choice gives the solve's bits; invalid
+    choices are refused."""
+    import ctypes
+    from uegccd_amd import lib as L
+    info = info_for("n38_cut9")
+    ref = _ctx(info, mode="diis", ndiis=8)
+    _, _, tab, tba = ref.mp2()
+    r0 = ref.solve(tab, tba)
+    lib = L.load()
+    for g in (1, 2, 4):
+        for qs in (4, 8):
+            for hs in (4, 8):
+                L.check(lib.uegccd_set_tiles(ref.handle, ctypes.byref(L.Tiles(g, qs, hs))), "set_tiles")
+                r = ref.solve(tab, tba)
+                assert r.niter == r0.niter and r.ecorr == r0.ecorr, (g, qs, hs)
+                np.testing.assert_array_equal(r.tab, r0.tab)
+    assert lib.uegccd_set_tiles(ref.handle, ctypes.byref(L.Tiles(3, 8, 8))) == L.ERR_INVALID
+    L.check(lib.uegccd_set_tiles(ref.handle, None), "set_tiles reset")

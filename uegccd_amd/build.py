@@ -51,8 +51,30 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+TUNED_JSON = os.path.join(PKG, "tuned_tiles.json")
+TUNED_INC = os.path.join(CSRC, "tuned_tiles.inc")
+
+
+def write_tuned_tiles():
+    """The tile table of the offline sweep (tuned_tiles.json, written by
+    `python -m uegccd_amd.tune`) as the C initialisers api.hip includes; an
+    empty table (every class takes the defaults) before the first sweep."""
+    import json
+    rows = []
+    if os.path.exists(TUNED_JSON):
+        for e in json.load(open(TUNED_JSON))["classes"]:
+            rows.append("    {%d, %d, %d, %d, %d, %d}," % (
+                e["noo_max"], e["nvv_max"], 1 if e["precision"] == "mixed" else 0,
+                e["ladder_group"], e["quad_steps"], e["hh_steps"]))
+    text = "// generated by uegccd_amd/build.py from tuned_tiles.json\n" + "\n".join(rows) + "\n"
+    if not os.path.exists(TUNED_INC) or open(TUNED_INC).read() != text:
+        with open(TUNED_INC, "w") as f:
+            f.write(text)
+
+
 def build_lib():
     os.makedirs(BUILD, exist_ok=True)
+    write_tuned_tiles()
     objs = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)

@@ -27,6 +27,40 @@ using namespace uegccd;
 
 namespace {
 
+// The tile table of the offline sweep (python -m uegccd_amd.tune writes
+// uegccd_amd/tuned_tiles.json; the build turns it into this include): per
+// class of systems, bounded by the largest pair-momentum block (pairs), the
+// largest virtual list of a block and the precision, the tile choice that
+// ran fastest.  Empty before the first sweep: every class takes the defaults.
+struct TunedTiles {
+  int noo_max, nvv_max, mixed;
+  int ladder_group, quad_steps, hh_steps;
+};
+const TunedTiles kTunedTiles[] = {
+#include "tuned_tiles.inc"
+    {0, 0, -1, 2, 8, 8}  // end marker
+};
+
+// The tuned tiles of a system whose largest block has noo pairs and nvv
+// virtual columns: the smallest class that covers it.
+Tiles tuned_tiles(int noo, int nvv, bool mixed) {
+  Tiles t;
+  int best = -1;
+  for (int k = 0; kTunedTiles[k].mixed >= 0; ++k) {
+    const TunedTiles& e = kTunedTiles[k];
+    if (e.mixed != (mixed ? 1 : 0) || noo > e.noo_max || nvv > e.nvv_max) continue;
+    if (best < 0 || e.noo_max < kTunedTiles[best].noo_max ||
+        (e.noo_max == kTunedTiles[best].noo_max && e.nvv_max < kTunedTiles[best].nvv_max))
+      best = k;
+  }
+  if (best >= 0) {
+    t.ladder_group = kTunedTiles[best].ladder_group;
+    t.quad_steps = kTunedTiles[best].quad_steps;
+    t.hh_steps = kTunedTiles[best].hh_steps;
+  }
+  return t;
+}
+
 constexpr double kPi = 3.14159265358979323846264338327950288;
 
 #define HIPCHK(expr)                                                    \
@@ -207,6 +241,8 @@ struct uegccd_ctx {
   bool no_alloc = false;
   // captured chunks of passes of the context's own batch, by chunk length
   std::map<int, hipGraphExec_t> graphs;
+  Tiles tiles;               // tile choice of the tuned kernels (tuned table or uegccd_set_tiles)
+  int noo_max = 0, nvv_max = 0;  // largest pair-momentum block: pairs, virtual columns
 };
 
 namespace {
@@ -294,6 +330,11 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->kb_woff[K + 1] = c->kb_woff[K] + noo * noo;
   }
   c->wtot = c->kb_woff[c->nK];
+  c->noo_max = c->nvv_max = 0;
+  for (int K = 0; K < c->nK; ++K) {
+    c->noo_max = std::max(c->noo_max, c->kb_poff[K + 1] - c->kb_poff[K]);
+    c->nvv_max = std::max(c->nvv_max, c->kb_voff[K + 1] - c->kb_voff[K]);
+  }
   // K-blocked layout of the valid amplitudes: block K holds, for each entry ci
   // of its virtual list, its noo pairs contiguously (position toff + ci*noo + p);
   // cmap gives each compact index its element, this position and its partner
@@ -667,6 +708,7 @@ int plan_ctx(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx*
   if (opt.damping > 0.0) c->df = opt.damping;
   else if (opt.mode == UEGCCD_MODE_DIIS) c->df = 1.0;
   c->maxiter = opt.max_iter > 0 ? opt.max_iter : 1000;
+  c->tiles = tuned_tiles(c->noo_max, c->nvv_max, opt.precision == UEGCCD_PREC_MIXED);
   *out = c;
   return UEGCCD_OK;
 }
@@ -789,6 +831,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   }
   b.nlad = S.nlad;
   b.lad_lds = ladder_lds_bytes(S);
+  b.tiles = c->tiles;
   b.ncon = S.ncon;
   // work buffers
   c->arena.used = aligned(c->arena.used);
@@ -1217,6 +1260,13 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   }
   b.lad_prefix = const_cast<int*>(im.put(lp));
   b.lad_lds = lad_lds;
+  // one instantiation per launch: the tiles of the solve with the most ladder tiles
+  {
+    int big = 0;
+    for (int k = 1; k < n; ++k)
+      if (cs[k]->sd.nlad > cs[big]->sd.nlad) big = k;
+    b.tiles = cs[big]->tiles;
+  }
   b.slice_split = slice_split;
   b.con_prefix = const_cast<int*>(im.put(cp));
   b.ncon = cp[n];
@@ -1619,11 +1669,41 @@ extern "C" int uegccd_kernel_stats(uegccd_ctx* c, double* total_us, int64_t* lau
   return UEGCCD_OK;
 }
 
+extern "C" int uegccd_set_tiles(uegccd_ctx* c, const uegccd_tiles* t) {
+  if (!c) return UEGCCD_ERR_INVALID;
+  Tiles x = tuned_tiles(c->noo_max, c->nvv_max, c->opt.precision == UEGCCD_PREC_MIXED);
+  if (t) {
+    if ((t->ladder_group != 1 && t->ladder_group != 2 && t->ladder_group != 4) ||
+        (t->quad_steps != 4 && t->quad_steps != 8) || (t->hh_steps != 4 && t->hh_steps != 8))
+      return UEGCCD_ERR_INVALID;
+    x.ladder_group = t->ladder_group;
+    x.quad_steps = t->quad_steps;
+    x.hh_steps = t->hh_steps;
+  }
+  c->tiles = x;
+  c->batch.tiles = x;
+  for (auto& g : c->graphs) hipGraphExecDestroy(g.second);  // captured with the old kernels
+  c->graphs.clear();
+  return UEGCCD_OK;
+}
+
+extern "C" int uegccd_get_tiles(const uegccd_ctx* c, uegccd_tiles* t, int32_t* noo_max, int32_t* nvv_max) {
+  if (!c || !t) return UEGCCD_ERR_INVALID;
+  t->ladder_group = c->tiles.ladder_group;
+  t->quad_steps = c->tiles.quad_steps;
+  t->hh_steps = c->tiles.hh_steps;
+  if (noo_max) *noo_max = c->noo_max;
+  if (nvv_max) *nvv_max = c->nvv_max;
+  return UEGCCD_OK;
+}
+
 extern "C" int uegccd_kernel_time(uegccd_ctx* c, int32_t which, int32_t reps, double* avg_us) {
   if (!c || !avg_us || reps <= 0 || which < 0 || which > 3) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  launch_state_reset(c->batch, PH_ITER, 1, s);
+  // the ladders in the context's starting precision (FP32 for mixed precision)
+  const int prec64 = c->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
+  launch_state_reset(c->batch, PH_ITER, prec64, s);
   launch_one(c->batch, which, s);  // warm
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

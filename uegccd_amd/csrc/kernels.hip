@@ -740,7 +740,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
 // read the linear direct (v(c-a)) and exchange (v(c-b)) tables, the W
 // columns the quadratic ones (v(c-k), v(c-l)) and, for v(i-k) and v(i-l),
 // the linear ones.
-template <typename Real>
+template <typename Real, int G>
 __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
                             const int4* cls, double* red) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
@@ -779,9 +779,7 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
   // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
   // their amplitude loads (unconditional, in-range addresses, masked values)
   // before the MFMAs, which run for every step (masked steps add zeros).
-  // G = 2 measured fastest (G = 8: 17.9 us, 4: 16.2, 2: 15.2 on the N=54,
-  // |n|^2 <= 16 pass): the pass is bound by memory requests, not latency.
-  constexpr int G = 2;
+  // G comes from the tuned tile table (uegccd_amd/tuned_tiles.json).
   for (int g0 = 4 * wave; g0 < nvv; g0 += 16 * G) {
     double2 t[G];
 #pragma unroll
@@ -840,6 +838,7 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
   }
 }
 
+template <int G>
 __device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
                              int n, int blk, double* vt, int4* cls, double* red) {
   const int s = find_solve(prefix, n, blk);
@@ -856,9 +855,9 @@ __device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __r
   __syncthreads();
   if (phase == PH_DONE) return;
   if (S.st->prec64)
-    ladder_tile<double>(S, K, it.z, it.w, vt, cls, red);
+    ladder_tile<double, G>(S, K, it.z, it.w, vt, cls, red);
   else
-    ladder_tile<float>(S, K, it.z, it.w, vt, cls, red);
+    ladder_tile<float, G>(S, K, it.z, it.w, vt, cls, red);
 }
 
 template <int BT, bool SPLIT>
@@ -871,12 +870,13 @@ __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solve
   slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2);
 }
 
+template <int G>
 __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
                                                 const int* __restrict__ prefix, int n) {
   extern __shared__ double vt[];  // the solve's ladder tables (ladder_lds_bytes)
   __shared__ int4 cls[kMaxNao];   // a K block's virtual list has at most v < kMaxNao entries
   __shared__ double red[3 * 64 * 8];
-  ladder_block(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
+  ladder_block<G>(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
 }
 
 // --------------------------------------------------------------------------
@@ -898,9 +898,8 @@ __global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ sol
 // two rounds with no branch between them (masked values, clamped in-range
 // addresses): the J rows with every index (the column's pair, the four output
 // rows' pairs, the partner slices d_q(l)), then the T columns they select.
-constexpr int kQuadSteps = 8;  // k-steps of 4 l values per round of loads (l < 32)
-
-template <bool XRING>
+// QS: k-steps of 4 l values per round of loads (from the tuned tile table).
+template <bool XRING, int QS>
 __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int4 info = S.qb_info[qb];  // rows of q: [r0, r0+n); columns (rows of -q): [c0, c0+m)
@@ -918,11 +917,11 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   const auto T = S.tcur;
   acc_t cab = {0, 0, 0, 0}, cba = {0, 0, 0, 0};
   int prow[4], j = 0, pcv = 0;
-  for (int g0 = 0; g0 < o; g0 += 4 * kQuadSteps) {
-    double a1[kQuadSteps], a2[kQuadSteps], a3[kQuadSteps];
-    int dv[kQuadSteps];
+  for (int g0 = 0; g0 < o; g0 += 4 * QS) {
+    double a1[QS], a2[QS], a3[QS];
+    int dv[QS];
 #pragma unroll
-    for (int st = 0; st < kQuadSteps; ++st) {
+    for (int st = 0; st < QS; ++st) {
       const int l = g0 + 4 * st + kq;
       const auto a = J + 3 * min(l, o - 1);
       const bool ok = l < o && rv;
@@ -940,9 +939,9 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
       pcv = pc / o;
     }
     const size_t tcol = (size_t)o * j;
-    double2 b[kQuadSteps];
+    double2 b[QS];
 #pragma unroll
-    for (int st = 0; st < kQuadSteps; ++st) {
+    for (int st = 0; st < QS; ++st) {
       const int l = g0 + 4 * st + kq;
       const int slice = XRING ? pcv : max(dv[st], 0);
       const bool ok = l < o && cv && (XRING || dv[st] >= 0);
@@ -952,7 +951,7 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
     // (all steps, the masked ones add zeros: a branch here would let the
     // compiler sink each step's loads behind the previous step's MFMAs)
 #pragma unroll
-    for (int st = 0; st < kQuadSteps; ++st) {
+    for (int st = 0; st < QS; ++st) {
       if (!XRING) {
         // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab,  ba += J3 T^ba
         cab = MfmaTraits<double>::mma(a2[st], b[st].x + b[st].y, cab);
@@ -986,8 +985,8 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
 // the particle-particle ladder of the same elements.  W and the K-blocked
 // amplitudes (a column's pairs contiguous) load in one round; the four
 // output elements are read together before they are written.
-constexpr int kHHSteps = 8;  // k-steps of 4 pairs per round of loads
-
+// HS: k-steps of 4 pairs per round of loads (from the tuned tile table).
+template <int HS>
 __device__ void hh_tile(const SolveG& S, int K, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
@@ -1009,19 +1008,19 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
   const size_t tslab = (size_t)o2 * av;
   const auto Wrow = W + (size_t)(rv ? rowA : 0) * noo;
   acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-  for (int g0 = 0; g0 < noo; g0 += 4 * kHHSteps) {
-    double ws[kHHSteps], wp[kHHSteps];
+  for (int g0 = 0; g0 < noo; g0 += 4 * HS) {
+    double ws[HS], wp[HS];
 #pragma unroll
-    for (int st = 0; st < kHHSteps; ++st) {
+    for (int st = 0; st < HS; ++st) {
       const int q = g0 + 4 * st + kq;
       const double2 w = ld(Wrow + min(q, noo - 1));
       const bool ok = q < noo && rv;
       ws[st] = ok ? w.x : 0.0;
       wp[st] = ok ? w.y : 0.0;
     }
-    double bs[kHHSteps], bp[kHHSteps];
+    double bs[HS], bp[HS];
 #pragma unroll
-    for (int st = 0; st < kHHSteps; ++st) {
+    for (int st = 0; st < HS; ++st) {
       const int q = g0 + 4 * st + kq;
       const double2 t = ld(TK + min(q, noo - 1));
       const bool ok = q < noo && cv;
@@ -1029,7 +1028,7 @@ __device__ void hh_tile(const SolveG& S, int K, int tile) {
       bp[st] = ok ? t.x - t.y : 0.0;
     }
 #pragma unroll
-    for (int st = 0; st < kHHSteps; ++st) {  // all steps, as in quad_tile
+    for (int st = 0; st < HS; ++st) {  // all steps, as in quad_tile
       accs = MfmaTraits<double>::mma(ws[st], bs[st], accs);
       accp = MfmaTraits<double>::mma(wp[st], bp[st], accp);
     }
@@ -1062,6 +1061,7 @@ __device__ void joo_wave(const SolveG& S, int part) {
 }
 
 // One contraction work item, by one wave.
+template <int QS, int HS>
 __device__ void contract_item(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
                               int item) {
   const int s = find_solve(prefix, n, item);
@@ -1069,17 +1069,18 @@ __device__ void contract_item(const SolveDev* __restrict__ solves, const int* __
   const int4 it = ld(S.con_items + (item - prefix[s]));
   // (no phase test: the contraction outputs of a finished solve are never read)
   if (it.y == CK_RING)
-    quad_tile<false>(S, it.z, it.w);
+    quad_tile<false, QS>(S, it.z, it.w);
   else if (it.y == CK_XRING)
-    quad_tile<true>(S, it.z, it.w);
+    quad_tile<true, QS>(S, it.z, it.w);
   else if (it.y == CK_HH)
-    hh_tile(S, it.z, it.w);
+    hh_tile<HS>(S, it.z, it.w);
   else
     joo_wave(S, it.z);
 }
 
 // One work item per wave; the four waves of a block take four consecutive
 // items of the XCD-ordered list.
+template <int QS, int HS>
 __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
                                                             const int* __restrict__ prefix, int n,
                                                             int nitems) {
@@ -1088,7 +1089,7 @@ __global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __res
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + wave;
   if (item >= nitems) return;
-  contract_item(solves, prefix, n, item);
+  contract_item<QS, HS>(solves, prefix, n, item);
 }
 
 // Block partials of the residual pass: max dT, max |R|, sum E, the non-finite
@@ -1886,11 +1887,11 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
     grid.sync();
     for (int it = blockIdx.x; it < nlad; it += gridDim.x) {
       __syncthreads();
-      ladder_block(solves, lad_prefix, 1, it, pdyn, P.u.la.cls, P.u.la.red);
+      ladder_block<2>(solves, lad_prefix, 1, it, pdyn, P.u.la.cls, P.u.la.red);
     }
     grid.sync();
     for (int base = blockIdx.x * kHHTilesPerBlock; base < ncon; base += gridDim.x * kHHTilesPerBlock)
-      if (base + wave < ncon) contract_item(solves, con_prefix, 1, base + wave);
+      if (base + wave < ncon) contract_item<8, 8>(solves, con_prefix, 1, base + wave);
     grid.sync();
     for (int it = blockIdx.x; it < nelem; it += gridDim.x) {
       __syncthreads();
@@ -1998,14 +1999,25 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
 }
 
 static void launch_ladder(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
-  launch_k(k_ladder, stop, dim3(b.nlad), dim3(256), (uint32_t)b.lad_lds, s, (const SolveDev*)b.solves,
-           (const int*)b.lad_prefix, b.n);
+  const auto* sd = (const SolveDev*)b.solves;
+  const auto* lp = (const int*)b.lad_prefix;
+  const uint32_t lds = (uint32_t)b.lad_lds;
+  switch (b.tiles.ladder_group) {
+    case 1: launch_k(k_ladder<1>, stop, dim3(b.nlad), dim3(256), lds, s, sd, lp, b.n); break;
+    case 4: launch_k(k_ladder<4>, stop, dim3(b.nlad), dim3(256), lds, s, sd, lp, b.n); break;
+    default: launch_k(k_ladder<2>, stop, dim3(b.nlad), dim3(256), lds, s, sd, lp, b.n); break;
+  }
 }
 
 static void launch_contract(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
   const int grid = (b.ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock;
-  launch_k(k_contract, stop, dim3(grid), dim3(kElemThreads), 0, s, (const SolveDev*)b.solves,
-           (const int*)b.con_prefix, b.n, b.ncon);
+  const auto* sd = (const SolveDev*)b.solves;
+  const auto* cp = (const int*)b.con_prefix;
+  const int q8 = b.tiles.quad_steps == 4 ? 0 : 1, h8 = b.tiles.hh_steps == 4 ? 0 : 1;
+  if (q8 && h8) launch_k(k_contract<8, 8>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
+  else if (q8) launch_k(k_contract<8, 4>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
+  else if (h8) launch_k(k_contract<4, 8>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
+  else launch_k(k_contract<4, 4>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
 }
 
 void launch_init(const BatchDev& b, hipStream_t s, int drpa) {

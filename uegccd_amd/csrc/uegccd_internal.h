@@ -181,6 +181,13 @@ inline int ladder_lds_bytes(const SolveDev& S) { return S.lad_ntab * (S.n2max + 
 
 constexpr int kSliceClasses = 3;
 
+// Tile choices of the tuned kernels: k-steps per group of loads in the ladder
+// tiles (1, 2, 4) and per round of loads in the ring (4, 8) and hole-hole
+// (4, 8) contraction tiles.
+struct Tiles {
+  int ladder_group = 2, quad_steps = 8, hh_steps = 8;
+};
+
 // Launch geometry of a batch of solves iterated together.
 struct BatchDev {
   int n;
@@ -201,6 +208,7 @@ struct BatchDev {
   int slice_lds[kSliceClasses];      // dynamic LDS bytes (largest NOcc of the class)
   int* con_prefix;      // [n+1] k_contract work items of each solve (one wave each)
   int ncon;
+  Tiles tiles;          // the tuned kernels' tile choice for the launch
 };
 
 // k_contract work items, one per wave: a 16x16 tile of the quadratic ring

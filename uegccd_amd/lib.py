@@ -32,7 +32,7 @@ EXPORTS = (
     "uegccd_server_clock_ms", "uegccd_time_kernel", "uegccd_kernel_stats",
     "uegccd_heg_build_device", "uegccd_connectivity", "uegccd_drpa",
     "uegccd_structure_factor", "uegccd_solve_noalloc", "uegccd_mp2_structure_factor",
-    "uegccd_exchange_structure_factor",
+    "uegccd_exchange_structure_factor", "uegccd_set_tiles", "uegccd_get_tiles",
 )
 REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
@@ -62,6 +62,11 @@ class Result(ctypes.Structure):
                 ("dt", ctypes.c_double), ("maxres", ctypes.c_double), ("niter", ctypes.c_int32),
                 ("failed", ctypes.c_int32), ("iters_fp32", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("solve_ms", ctypes.c_double)]
+
+
+class Tiles(ctypes.Structure):
+    _fields_ = [("ladder_group", ctypes.c_int32), ("quad_steps", ctypes.c_int32),
+                ("hh_steps", ctypes.c_int32)]
 
 
 class HegParams(ctypes.Structure):
@@ -143,6 +148,8 @@ def load():
     lib.uegccd_solve_resident.argtypes = [P, ctypes.c_int32, ctypes.POINTER(Result)]
     lib.uegccd_get_amplitudes.argtypes = [P, D, D, D]
     lib.uegccd_kernel_time.argtypes = [P, ctypes.c_int32, ctypes.c_int32, D]
+    lib.uegccd_set_tiles.argtypes = [P, ctypes.POINTER(Tiles)]
+    lib.uegccd_get_tiles.argtypes = [P, ctypes.POINTER(Tiles), I32, I32]
     I64 = ctypes.POINTER(ctypes.c_int64)
     lib.uegccd_footprint.argtypes = [ctypes.POINTER(System), ctypes.POINTER(Options), I64]
     lib.uegccd_server_create.argtypes = [ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]
