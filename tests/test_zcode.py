@@ -183,3 +183,78 @@ def test_zcode_structure_factor_units_81_82(tmp_path):
         q, cnt, v = exchange_structure_factor(info, rs=rs)
         assert (blk[:, :3].astype(int) == q).all() and (blk[:, 3].astype(int) == cnt).all()
         np.testing.assert_allclose(blk[:, 4], v, rtol=1e-13)
+
+
+def test_zcode_twist_average_reference_run(tmp_path, fixtures):
+    """DoSkipTA F: ZCode runs the reference's twist averaging
+    (source/Main.f90:99-255) on the reference's own TA Input (N=24, r_s=5,
+    cutoff 40, fcut 2) over its ten recorded twists (source/Test/fort.59):
+    every fort.59 number (per-twist E_HF and E(2), averages, squared
+    averages, errors, averaged eigenvalues and connectivity, connectivity
+    distances, the special twist with its tie between twists 6 and 7), the
+    per-twist connectivity in fort.58 and the final MP2 + 84-iteration CCD
+    with the averaged eigenvalues in Output."""
+    print("[zcode] n24 twist averaging", flush=True)
+    fx = fixtures["n24"]
+    (tmp_path / "twists").write_text("\n".join(" ".join(repr(x) for x in t) for t in fx["ta_twists"]) + "\n")
+    text = standard_input(24, 40, 5.0, (0.1, 0.2, 0.45), fcut=2.0, DoSkipTA=False, TACalcN=10)
+    (tmp_path / "Input").write_text(text)
+    e = dict(os.environ, UEGCCD_TWISTS=str(tmp_path / "twists"))
+    r = subprocess.run([ZCODE], cwd=tmp_path, capture_output=True, text=True, timeout=300, env=e)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ours = (tmp_path / "fort.59").read_text().splitlines()
+    ref = fx["ta_fort59_lines"]
+    assert len(ours) == len(ref) == 27
+    for k in range(27):   # gfortran's list-directed layout, field by field
+        assert re.sub(r"\d", "0", ours[k]) == re.sub(r"\d", "0", ref[k]), (k, ours[k], ref[k])
+    num = re.compile(r"^[-+]?\d*\.\d+(E[-+]\d+)?$")
+    nums = lambda ln: [float(t) for t in ln.split() if num.match(t)]  # noqa: E731
+    for k in range(10):   # per twist: the twist (as printed), E_HF, E(2), E(2) again (no TA CCD)
+        assert ours[k].split()[:3] == ref[k].split()[:3]
+        np.testing.assert_allclose(nums(ours[k])[3:], nums(ref[k])[3:], rtol=0, atol=1e-13)
+    for k in (10, 11):    # averages, squared averages
+        np.testing.assert_allclose(nums(ours[k]), nums(ref[k]), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(nums(ours[12]), nums(ref[12]), rtol=1e-10)      # errors
+    np.testing.assert_allclose(nums(ours[13]), nums(ref[13]), rtol=0, atol=1e-14)  # eigenvalues
+    assert nums(ours[14]) == nums(ref[14])                                      # connectivity
+    for k in range(15, 25):
+        assert int(ours[k].split()[0]) == k - 14
+        np.testing.assert_allclose(nums(ours[k])[-1:], nums(ref[k])[-1:], rtol=1e-12)
+    assert ours[25].split() == ref[25].split()      # special twist angle found (twist 6)
+    assert abs(nums(ours[26])[0] - nums(ref[26])[0]) < 1e-12 * nums(ref[26])[0]
+    assert len((tmp_path / "fort.58").read_text().splitlines()) == 10
+    # the final MP2 and CCD (the last blocks of Output)
+    lines = (tmp_path / "Output").read_text().splitlines()
+    last = max(k for k, ln in enumerate(lines) if "E(2) is" in ln)
+    o = parse(lines[last - 2:])
+    assert o["e2"] == pytest.approx(fx["ta_final_e2"], abs=1e-12)
+    assert o["niter"] == fx["ta_final_niter"] == 84
+    tr = np.array(o["trace"])
+    refp = np.array(fx["ta_final_trace"])
+    np.testing.assert_allclose(tr[:, 0], refp[:, 0], rtol=0, atol=1.1e-10)
+    np.testing.assert_allclose(tr[:, 2], refp[:, 2], rtol=0, atol=1.1e-10)
+    assert o["ecorr"] == pytest.approx(fx["ta_final_ecorr"], abs=1e-12)
+
+
+def test_zcode_twist_average_served_ccd(tmp_path):
+    """DoCalcTACCD T: every twist's CCD runs through the library's request
+    server; each per-twist record's CCD energy equals that twist's solve
+    alone."""
+    print("[zcode] twist averaging with served CCD", flush=True)
+    from uegccd_amd.driver import CCDContext, init_heg, options
+    from uegccd_amd.inputfile import parse_input
+    twists = [(0.1, -0.2, 0.3), (-0.25, 0.05, 0.4), (0.33, 0.12, -0.41)]
+    (tmp_path / "twists").write_text("\n".join(" ".join(repr(x) for x in t) for t in twists) + "\n")
+    text = standard_input(14, 5, 1.0, DoSkipTA=False, DoCalcTACCD=True, TACalcN=3)
+    (tmp_path / "Input").write_text(text)
+    e = dict(os.environ, UEGCCD_TWISTS=str(tmp_path / "twists"))
+    r = subprocess.run([ZCODE], cwd=tmp_path, capture_output=True, text=True, timeout=300, env=e)
+    assert r.returncode == 0, r.stdout + r.stderr
+    info = parse_input(text)
+    rec = (tmp_path / "fort.59").read_text().splitlines()
+    for k, tw in enumerate(twists):
+        ctx = CCDContext(init_heg(info, twist=tw, device=0), info, options(mode="reference"))
+        _, _, tab, tba = ctx.mp2()
+        alone = ctx.solve(tab, tba)
+        assert abs(float(rec[k].split()[5]) - alone.ecorr) < 1e-12
+        ctx.close()

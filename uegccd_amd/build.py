@@ -33,7 +33,7 @@ HIPCC_FLAGS = [
 # Fortran host: the reference's file set (Main, IO, HEG-facing types, CCD,
 # MP2, DIISCC, Wrappers) compiled in dependency order.
 FORTRAN_SOURCES = ["Precision.f90", "Types.f90", "Wrappers.f90", "IO.f90", "HEG.f90",
-                   "DIISCC.f90", "CCD.f90", "dRPA.f90", "MP2.f90", "Main.f90"]
+                   "DIISCC.f90", "CCD.f90", "dRPA.f90", "MP2.f90", "TwistAverage.f90", "Main.f90"]
 
 
 def _run(cmd, cwd=None):

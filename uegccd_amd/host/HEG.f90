@@ -12,9 +12,14 @@ Module HEG
 
 Contains
 
-  Subroutine InitHEG(UEGInfo, HEGData)
+  ! Twist, when given, replaces the Input's NTwist: the basis of
+  ! ChangeTwist + SetupBasis at the current r_s (source/HEG.f90:551-578), which
+  ! prints one exchange structure factor; without it, init_HEG + change_rs,
+  ! which print two (at r_s = 1 and at r_s).
+  Subroutine InitHEG(UEGInfo, HEGData, Twist)
     Type (UEGInfoType), Intent(InOut) :: UEGInfo
     Type (HEGDataType), Intent(InOut) :: HEGData
+    Real (Kind=pr), Intent(In), Optional :: Twist(3)
     Type (uegccd_heg_params) :: P
     Integer(c_int32_t) :: NAO
     Integer(c_int32_t), Allocatable, Target :: NVecC(:)
@@ -25,6 +30,7 @@ Contains
     P%maxkpoint = UEGInfo%MaxKPoint
     P%rs = UEGInfo%rSMin
     P%twist = UEGInfo%NTwist
+    If (Present(Twist)) P%twist = Twist
     P%fcut = UEGInfo%fcut
     P%corefcut = UEGInfo%corefcut
     P%madelungfactor = UEGInfo%MadelungFactor
@@ -53,7 +59,7 @@ Contains
     HEGData%Eigen = Eig
     ! print_HF's exchange structure factor: once for the set-up at r_s = 1
     ! and once after the change to r_s (source/HEG.f90:319,531,613)
-    Call WriteExSF(P, 1.0_c_double)
+    If (.not. Present(Twist)) Call WriteExSF(P, 1.0_c_double)
     Call WriteExSF(P, UEGInfo%rSMin)
   End Subroutine InitHEG
 

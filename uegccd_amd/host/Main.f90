@@ -1,8 +1,7 @@
-! ZCode: the single-point driver of the reference (Program TermedCC,
-! source/Main.f90:2-97) on the MI355X solver: read Input, build the basis,
-! MP2 guess, then CCD or, with DodRPASOSEX, dRPA/SOSEX on T2aaaa
-! (source/Main.f90:84-92).  Twist averaging (DoSkipTA F) is not part of this
-! host and stops with a message.
+! ZCode: the driver of the reference (Program TermedCC, source/Main.f90) on
+! the MI355X solver: read Input, build the basis, MP2 guess, then CCD or,
+! with DodRPASOSEX, dRPA/SOSEX on T2aaaa (source/Main.f90:84-92); then, unless
+! DoSkipTA, twist averaging (source/Main.f90:99-255, module TwistAverage).
 Program TermedCC
   Use Precision
   Use Types, Only: UEGInfoType, HEGDataType
@@ -11,6 +10,7 @@ Program TermedCC
   Use MP2, Only: DrvMBPT
   Use CCD, Only: DrvCCD, ShutDownSolver
   Use dRPA, Only: DrvdRPA
+  Use TwistAverage, Only: DrvTwistAverage
   Implicit None
   Type (UEGInfoType) :: UEGInfo
   Type (HEGDataType) :: HEGData
@@ -37,5 +37,5 @@ Program TermedCC
   End If
   Call ShutDownSolver()
   If (UEGInfo%DoSkipTA) Stop 'Finishing calculation after single point calculation'
-  Stop 'Twist averaging is not part of this build (set DoSkipTA: T)'
+  Call DrvTwistAverage(UEGInfo, HEGData, T2aaaa, T2abab, T2abba)
 End Program TermedCC

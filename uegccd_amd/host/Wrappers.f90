@@ -37,6 +37,27 @@ Module Wrappers
     Real(c_double) :: rs, twist(3), fcut, corefcut, madelungfactor, gap, exchangefactor
   End Type uegccd_heg_params
 
+  Type, Bind(C) :: uegccd_server_options
+    Integer(c_int32_t) :: device
+    Integer(c_int64_t) :: hbm_budget_bytes
+    Integer(c_int32_t) :: max_active, iters_per_step, planner_threads
+  End Type uegccd_server_options
+
+  Type, Bind(C) :: uegccd_request
+    Type(uegccd_heg_params) :: heg
+    Real(c_double) :: gapii
+    Integer(c_int32_t) :: coren, safe_eri
+    Integer(c_int32_t) :: irange(15)
+    Integer(c_int32_t) :: do_ring, do_xring, do_ladder, do_mosaic
+    Type(uegccd_options) :: opt
+  End Type uegccd_request
+
+  Type, Bind(C) :: uegccd_request_result
+    Integer(c_int32_t) :: state, status, nao, nocc, niter, iters_fp32
+    Real(c_double) :: ehf, e2, ecorr, maxres, t_submit_ms, t_start_ms, t_done_ms
+    Integer(c_int64_t) :: footprint_bytes
+  End Type uegccd_request_result
+
   Interface
     Subroutine uegccd_default_options(opt) Bind(C, name="uegccd_default_options")
       Import :: uegccd_options
@@ -130,6 +151,43 @@ Module Wrappers
       Type(uegccd_heg_params), Intent(In) :: p
       Integer(c_int32_t), Intent(InOut) :: ncell
       Type(c_ptr), Value :: qvec, count, exv
+    End Function
+
+    Integer(c_int) Function uegccd_connectivity(ctx, counts, nbins) Bind(C, name="uegccd_connectivity")
+      Import :: c_int, c_int32_t, c_ptr
+      Type(c_ptr), Value :: ctx
+      Integer(c_int32_t), Intent(Out) :: counts(*)
+      Integer(c_int32_t), Value :: nbins
+    End Function
+
+    Integer(c_int) Function uegccd_server_create(opt, srv) Bind(C, name="uegccd_server_create")
+      Import :: c_int, c_ptr, uegccd_server_options
+      Type(uegccd_server_options), Intent(In) :: opt
+      Type(c_ptr), Intent(Out) :: srv
+    End Function
+
+    Subroutine uegccd_server_destroy(srv) Bind(C, name="uegccd_server_destroy")
+      Import :: c_ptr
+      Type(c_ptr), Value :: srv
+    End Subroutine
+
+    Integer(c_int) Function uegccd_server_submit(srv, req, id) Bind(C, name="uegccd_server_submit")
+      Import :: c_int, c_ptr, c_int64_t, uegccd_request
+      Type(c_ptr), Value :: srv
+      Type(uegccd_request), Intent(In) :: req
+      Integer(c_int64_t), Intent(Out) :: id
+    End Function
+
+    Integer(c_int) Function uegccd_server_drain(srv) Bind(C, name="uegccd_server_drain")
+      Import :: c_int, c_ptr
+      Type(c_ptr), Value :: srv
+    End Function
+
+    Integer(c_int) Function uegccd_server_poll(srv, id, res) Bind(C, name="uegccd_server_poll")
+      Import :: c_int, c_ptr, c_int64_t, uegccd_request_result
+      Type(c_ptr), Value :: srv
+      Integer(c_int64_t), Value :: id
+      Type(uegccd_request_result), Intent(Out) :: res
     End Function
 
   End Interface
