@@ -536,11 +536,18 @@ int elem_blocks_of(const uegccd_ctx* c) {
   return (int)std::max<int64_t>(1, (c->nvalid + kElemPerBlock - 1) / kElemPerBlock);
 }
 
+// Doubles of jring / jxring: (J2, J3) pairs per (row of P(q), l); the
+// structure-factor sums reuse them for 3 doubles per hole-particle pair and
+// per transfer.
+static size_t jring_doubles(const uegccd_ctx* c) {
+  const size_t ovo = (size_t)c->o * c->v * c->o, ov3 = 3 * (size_t)c->o * c->v;
+  return 2 * ((std::max<size_t>(2 * ovo, ov3) + 1) / 2 + 1);
+}
+
 // Device bytes of a context: the arena ctx_create carves its buffers from.
 size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   const size_t n3 = c->n3;
   const size_t eb = elem_blocks_of(c);
-  const size_t ovo = (size_t)c->o * c->v * c->o;
   size_t bytes = 0;
   bytes += aligned(sizeof(int4) * c->nao) + aligned(sizeof(double) * c->nao);
   bytes += aligned(sizeof(double) * c->vtab.size());                    // vtab
@@ -557,8 +564,8 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(int) * (c->nK + 1));                         // kb_toff
   bytes += aligned(sizeof(double2) * n3);                              // lad
   bytes += aligned(sizeof(double2) * std::max(1, c->wtot));            // w
-  bytes += 2 * aligned(sizeof(double) * 3 * std::max<size_t>(1, ovo));  // jring, jxring
-  bytes += aligned(sizeof(double4) * n3);                              // tv
+  bytes += 2 * aligned(sizeof(double) * jring_doubles(c));             // jring, jxring
+  bytes += aligned(sizeof(double2) * n3);                              // tv
   bytes += 2 * aligned(sizeof(double) * c->o * c->v);                  // rsum, csum
   bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
   bytes += aligned(sizeof(double) * c->o * kJooParts);
@@ -728,7 +735,6 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   const int ndiis = ndiis_of(opt);
   const size_t n3 = c->n3;
   const int eb = elem_blocks_of(c);
-  const size_t ovo = (size_t)c->o * c->v * c->o;
   c->arena.base = base;
   c->arena.size = bytes;
   c->arena.used = 0;
@@ -845,9 +851,9 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.lad = c->arena.take<double2>(n3);
   S.wtot = c->wtot;
   S.w = c->arena.take<double2>(std::max(1, c->wtot));
-  S.jring = c->arena.take<double>(3 * std::max<size_t>(1, ovo));
-  S.jxring = c->arena.take<double>(3 * std::max<size_t>(1, ovo));
-  S.tv = c->arena.take<double4>(n3);
+  S.jring = c->arena.take<double2>(jring_doubles(c) / 2);
+  S.jxring = c->arena.take<double2>(jring_doubles(c) / 2);
+  S.tv = c->arena.take<double2>(n3);
   S.quad = c->arena.take<double4>(n3);
   S.rsum = c->arena.take<double>((size_t)c->o * c->v);
   S.csum = c->arena.take<double>((size_t)c->o * c->v);

@@ -20,8 +20,8 @@
 //
 // Interleaved layouts keep each gathered access one 16/32/64-byte load:
 //   tcur, lad, histT/histE entries   double2  (abab, abba) or (S, P) per (i,j,a)
-//   tv                               double4  (TV_ab, TV_ba, VT_ab, VT_ba)
-//   jring / jxring                   3 doubles (J1, J2, J3) per (row of P(q), l)
+//   tv                               double2  (TV_ab + VT_ab, TV_ba + VT_ba) per (i,j,a)
+//   jring / jxring                   double2  (J2, J3) per (row of P(q), l)
 //   histT / histE                    valid elements only, in element order (compact
 //                                    index cpos[x]), slot-major: k*nvalid + cpos[x]
 #include <hip/hip_cooperative_groups.h>
@@ -303,7 +303,11 @@ struct MfmaTraits<float> {
 //   J1 = v(q) rsum(i) - sum_k T^aa(i,k,s) Y    J1x = v(q) csum(i) - sum_k T^aa(k,i,s) Y
 //   J2 = v(q) rsum(i) - sum_k T^ab(i,k,s) Y    J2x = v(q) csum(i) - sum_k T^ba(k,i,s) Y
 //   J3 =              - sum_k T^ba(i,k,s) Y    J3x =                sum_k T^ab(k,i,s) Y
-// (the momentum test on d is applied where J is contracted).  The mosaic
+// (the momentum test on d is applied where J is contracted).  T^aa = T^ab +
+// T^ba makes J1 = J2 + J3 and J1x = J2x - J3x, so only J2, J3 are stored.
+// A residual element takes the sum TV + VT of each spin at (i,j) of its slice
+// a and at (j,i) of its partner's slice b, so that sum is what is stored.
+// The mosaic
 // sums of source/CCD.f90:548-567 are per-slice reductions as well:
 //   Jvv(s) = sum_k P_s(k),  Joo(k) = sum_s P_s(k)  (k_joo),
 //   P_s(k) = sum_l v(n_s-n_k)(T^aa+T^ab)(k,l,s) - v(n_s-n_l)(T^aa+T^ba)(k,l,s).
@@ -639,34 +643,38 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
       S.joo_part[sv * o + i] = p;
     }
   }
-  // The slice products TV = T V and VT = V T of both spins as 16x16 FP64
-  // MFMA tiles over (r, c), dealt over the block's waves.
+  // The slice products TV = T V and VT = V T as 16x16 FP64 MFMA tiles over
+  // (r, c), one spin per work unit (T^ab units write the first component of
+  // tv, T^ba units the second), stored as the sum of the products the ring
+  // flags turn on.  The units go to the block's last waves, so that they run
+  // beside the row sums above (its first waves).
   {
+    const double fr = S.do_ring ? 1.0 : 0.0, fx = S.do_xring ? 1.0 : 0.0;
     typedef MfmaTraits<double>::acc_t acc_t;
-    const int nt = (o + 15) >> 4, ntiles = nt * nt;
+    const int nt = (o + 15) >> 4, nunits = 2 * nt * nt;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
     constexpr int nw = BT / 64;
-    for (int t = wave; t < ntiles; t += nw) {
+    const auto tvd = reinterpret_cast<GlobalPtr<double>>(S.tv) + 2 * (size_t)o2 * sv;
+    for (int u = nw - 1 - wave; u < nunits; u += nw) {
+      const int t = u >> 1, half = u & 1;
+      const double* TT = half ? Tba : Tab;
       const int rt = t / nt, ct = t % nt;
       const int ra = rt * 16 + r16, cb = ct * 16 + r16;
       const int rac = min(ra, o - 1), cbc = min(cb, o - 1);
-      acc_t tva = {0, 0, 0, 0}, tvb = {0, 0, 0, 0}, vta = {0, 0, 0, 0}, vtb = {0, 0, 0, 0};
+      acc_t tv = {0, 0, 0, 0}, vt = {0, 0, 0, 0};
       for (int k0 = 0; k0 < o; k0 += 4) {
         // in-range LDS addresses, masked values: no branch between the loads
         const int k = k0 + kq, kc = min(k, o - 1);
         const bool ak = ra < o && k < o, bk = cb < o && k < o;
-        const double ta_rk = Tab[rac + o * kc], tb_rk = Tba[rac + o * kc], v_rk = Vx[rac + o * kc];
-        const double v_kc = Vo[kc + o * cbc], ta_kc = Tab[kc + o * cbc], tb_kc = Tba[kc + o * cbc];
-        tva = MfmaTraits<double>::mma(ak ? ta_rk : 0.0, bk ? v_kc : 0.0, tva);
-        tvb = MfmaTraits<double>::mma(ak ? tb_rk : 0.0, bk ? v_kc : 0.0, tvb);
-        vta = MfmaTraits<double>::mma(ak ? v_rk : 0.0, bk ? ta_kc : 0.0, vta);
-        vtb = MfmaTraits<double>::mma(ak ? v_rk : 0.0, bk ? tb_kc : 0.0, vtb);
+        const double t_rk = TT[rac + o * kc], v_rk = Vx[rac + o * kc];
+        const double v_kc = Vo[kc + o * cbc], t_kc = TT[kc + o * cbc];
+        tv = MfmaTraits<double>::mma(ak ? t_rk : 0.0, bk ? v_kc : 0.0, tv);
+        vt = MfmaTraits<double>::mma(ak ? v_rk : 0.0, bk ? t_kc : 0.0, vt);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = rt * 16 + MfmaTraits<double>::row(kq, r);
-        if (row < o && cb < o)
-          S.tv[(size_t)o2 * sv + row + o * cb] = make_double4(tva[r], tvb[r], vta[r], vtb[r]);
+        if (row < o && cb < o) tvd[2 * (row + o * cb) + half] = fr * tv[r] + fx * vt[r];
       }
     }
   }
@@ -677,39 +685,55 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
     jv = wave_sum(jv);
     if (threadIdx.x == 0) S.jvv[sv] = jv;
   }
+  // The quadratic intermediates J(i, l), kJL values of l per thread (l = g,
+  // g + G, ...): the four amplitudes of (i, k) and (k, i) are read once per k
+  // for all of them.  Consecutive threads take consecutive g of one i, so
+  // their stores fill runs of one J row.
+  constexpr int kJL = 3;
+  const int G = (o + kJL - 1) / kJL;
   const int4 ns = L.nv[so];
-  for (int idx = threadIdx.x; idx < o2; idx += BT) {
-    // quadratic intermediates for i = r, l = c
-    const int i = idx % o, l = idx / o;
-    const int4 ni = L.nv[i], nl = L.nv[l];
-    const int bx = ni.x - ns.x - nl.x, by = ni.y - ns.y - nl.y, bz = ni.z - ns.z - nl.z;
-    double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0;
+  for (int idx = threadIdx.x; idx < o * G; idx += BT) {
+    const int i = idx / G, g = idx - i * G;
+    const int4 ni = L.nv[i];
+    int bx[kJL], by[kJL], bz[kJL];
+#pragma unroll
+    for (int m = 0; m < kJL; ++m) {
+      const int4 nl = L.nv[min(g + m * G, o - 1)];
+      bx[m] = ni.x - ns.x - nl.x;
+      by[m] = ni.y - ns.y - nl.y;
+      bz[m] = ni.z - ns.z - nl.z;
+    }
+    double a2[kJL], a3[kJL], a5[kJL], a6[kJL];
+#pragma unroll
+    for (int m = 0; m < kJL; ++m) a2[m] = a3[m] = a5[m] = a6[m] = 0.0;
     // (unrolled so that several table lookups, each waiting on its n_k, are
     // in flight at once; |d| < 2^11, so 24-bit multiplies are exact)
-#pragma unroll 4
+#pragma unroll 2
     for (int k = 0; k < o; ++k) {
       const int4 nk = L.nv[k];
-      const int dx = bx + nk.x, dy = by + nk.y, dz = bz + nk.z;
-      const int yi = min(__mul24(dx, dx) + __mul24(dy, dy) + __mul24(dz, dz), kMaxN2 - 1);
-      const double y = L.vt[yi];
-      const double yx = SPLIT ? vt2[yi] : y;
       const double ta = Tab[i + o * k], tb = Tba[i + o * k];
       const double ua = Tab[k + o * i], ub = Tba[k + o * i];
-      a1 += (ta + tb) * y;
-      a2 += ta * y;
-      a3 += tb * y;
-      a4 += (ua + ub) * yx;
-      a5 += ub * yx;
-      a6 += ua * yx;
+#pragma unroll
+      for (int m = 0; m < kJL; ++m) {
+        const int dx = bx[m] + nk.x, dy = by[m] + nk.y, dz = bz[m] + nk.z;
+        const int yi = min(__mul24(dx, dx) + __mul24(dy, dy) + __mul24(dz, dz), kMaxN2 - 1);
+        const double y = L.vt[yi];
+        const double yx = SPLIT ? vt2[yi] : y;
+        a2[m] += ta * y;
+        a3[m] += tb * y;
+        a5[m] += ub * yx;
+        a6[m] += ua * yx;
+      }
     }
     const double vq = vsr[i], vqx = vsx[i];
-    const size_t jx = 3 * ((size_t)S.qrow[sv * o + i] * o + l);
-    S.jring[jx] = vq * rs[i] - a1;
-    S.jring[jx + 1] = vq * rs[i] - a2;
-    S.jring[jx + 2] = -a3;
-    S.jxring[jx] = vqx * cs[i] - a4;
-    S.jxring[jx + 1] = vqx * cs[i] - a5;
-    S.jxring[jx + 2] = a6;
+    const size_t jrow = (size_t)S.qrow[sv * o + i] * o;
+#pragma unroll
+    for (int m = 0; m < kJL; ++m) {
+      const int l = g + m * G;
+      if (l >= o) continue;
+      st2(S.jring + jrow + l, make_double2(vq * rs[i] - a2[m], -a3[m]));
+      st2(S.jxring + jrow + l, make_double2(vqx * cs[i] - a5[m], a6[m]));
+    }
   }
 }
 
@@ -912,22 +936,21 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   // lane supplies A[row r16][l] (J rows, contiguous per q) and B[l][column r16]
   const int rowA = row0 + r16, colB = col0 + r16;
   const bool rv = rowA < n, cv = colB < m;
-  const auto J = (XRING ? S.jxring : S.jring) + 3 * (size_t)(r0 + (rv ? rowA : 0)) * o;
+  const auto J = (XRING ? S.jxring : S.jring) + (size_t)(r0 + (rv ? rowA : 0)) * o;
   const auto dq = S.qb_d + (size_t)qb * o;
   const auto T = S.tcur;
   acc_t cab = {0, 0, 0, 0}, cba = {0, 0, 0, 0};
   int prow[4], j = 0, pcv = 0;
   for (int g0 = 0; g0 < o; g0 += 4 * QS) {
-    double a1[QS], a2[QS], a3[QS];
+    double a2[QS], a3[QS];
     int dv[QS];
 #pragma unroll
     for (int st = 0; st < QS; ++st) {
       const int l = g0 + 4 * st + kq;
-      const auto a = J + 3 * min(l, o - 1);
+      const double2 a = ld(J + min(l, o - 1));
       const bool ok = l < o && rv;
-      a1[st] = ok ? a[0] : 0.0;
-      a2[st] = ok ? a[1] : 0.0;
-      a3[st] = ok ? a[2] : 0.0;
+      a2[st] = ok ? a.x : 0.0;
+      a3[st] = ok ? a.y : 0.0;
       if (!XRING) dv[st] = dq[min(l, o - 1)];
     }
     if (g0 == 0) {
@@ -953,14 +976,16 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
 #pragma unroll
     for (int st = 0; st < QS; ++st) {
       if (!XRING) {
-        // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab,  ba += J3 T^ba
-        cab = MfmaTraits<double>::mma(a2[st], b[st].x + b[st].y, cab);
-        cab = MfmaTraits<double>::mma(a1[st], b[st].x, cab);
+        // ring:  ab += J2 (T^ab + T^ba) + J1 T^ab = J2 (2 T^ab + T^ba) + J3 T^ab,
+        //        ba += J3 T^ba
+        cab = MfmaTraits<double>::mma(a2[st], 2.0 * b[st].x + b[st].y, cab);
+        cab = MfmaTraits<double>::mma(a3[st], b[st].x, cab);
         cba = MfmaTraits<double>::mma(a3[st], b[st].y, cba);
       } else {
-        // crossed ring:  ba -= J1 T^ba + J2 (T^ab + T^ba),  ab += J3 T^ab
-        cba = MfmaTraits<double>::mma(-a1[st], b[st].y, cba);
-        cba = MfmaTraits<double>::mma(-a2[st], b[st].x + b[st].y, cba);
+        // crossed ring:  ba -= J1 T^ba + J2 (T^ab + T^ba) = J2 (T^ab + 2 T^ba) - J3 T^ba,
+        //                ab += J3 T^ab
+        cba = MfmaTraits<double>::mma(-a2[st], b[st].x + 2.0 * b[st].y, cba);
+        cba = MfmaTraits<double>::mma(a3[st], b[st].y, cba);
         cab = MfmaTraits<double>::mma(a3[st], b[st].x, cab);
       }
     }
@@ -1207,22 +1232,15 @@ __device__ void main_block(const SolveDev* __restrict__ solves, const int* __res
     }
     // slice products: element (i,j) of slice a, (j,i) of slice b
     if (S.do_ring || S.do_xring) {
-      const double4 ta = S.tv[x];
-      const double4 tb = S.tv[(size_t)j + o * i + (size_t)o2 * bv];
-      if (S.do_ring) {
-        // linear rings (source/CCD.f90:716-763): v(q) rowsum - (T V), a- and b-slice
-        gab += r_ia * S.rsum[av * o + i] - ta.x;
-        gba -= ta.y;
-        gab += r_jb * S.rsum[bv * o + j] - tb.x;
-        gba -= tb.y;
-      }
-      if (S.do_xring) {
-        // linear crossed rings (source/CCD.f90:845-892): (V T) of the b- and a-slices
-        gab -= tb.z;
-        gba += x_ja * S.csum[bv * o + i] - tb.w;
-        gab -= ta.z;
-        gba += x_ib * S.csum[av * o + j] - ta.w;
-      }
+      // linear rings (source/CCD.f90:716-763): v(q) rowsum - (T V), and
+      // crossed rings (source/CCD.f90:845-892): v(q) colsum - (V T), of the
+      // a-slice at (i,j) and the b-slice at (j,i)
+      const double2 ua = S.tv[x];
+      const double2 ub = S.tv[(size_t)j + o * i + (size_t)o2 * bv];
+      gab -= ua.x + ub.x;
+      gba -= ua.y + ub.y;
+      if (S.do_ring) gab += r_ia * S.rsum[av * o + i] + r_jb * S.rsum[bv * o + j];
+      if (S.do_xring) gba += x_ja * S.csum[bv * o + i] + x_ib * S.csum[av * o + j];
       const double4 qd = S.quad[x];
       if (S.do_ring) { gab += qd.x; gba += qd.y; }
       if (S.do_xring) { gab += qd.z; gba += qd.w; }
@@ -1666,29 +1684,32 @@ __global__ __launch_bounds__(64) void k_sf_pairs(const SolveDev* __restrict__ so
     s2 += d.y - x.x;
   }
   const size_t w = 3 * ((size_t)av * o + l);
-  S.jring[w] = s0;
-  S.jring[w + 1] = s1;
-  S.jring[w + 2] = s2;
+  const auto JP = reinterpret_cast<GlobalPtr<double>>(S.jring);
+  JP[w] = s0;
+  JP[w + 1] = s1;
+  JP[w + 2] = s2;
 }
 
 // Block q (one wave): the cell sums of transfer q over the pairs of P(q).
 __global__ __launch_bounds__(64) void k_sf_cells(const SolveDev* __restrict__ solves) {
   const SolveG& S = as_global(solves)[0];
   const int q = blockIdx.x, r0 = S.qb_roff[q], r1 = S.qb_roff[q + 1];
+  const auto JP = reinterpret_cast<GlobalPtr<const double>>(S.jring);
+  const auto JC = reinterpret_cast<GlobalPtr<double>>(S.jxring);
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for (int r = r0 + (int)threadIdx.x; r < r1; r += 64) {
     const size_t w = 3 * (size_t)S.qb_rows[r];  // row l + o*d of pair (l, d)
-    a0 += S.jring[w];
-    a1 += S.jring[w + 1];
-    a2 += S.jring[w + 2];
+    a0 += JP[w];
+    a1 += JP[w + 1];
+    a2 += JP[w + 2];
   }
   a0 = wave_sum(a0);
   a1 = wave_sum(a1);
   a2 = wave_sum(a2);
   if (threadIdx.x == 0) {
-    S.jxring[3 * (size_t)q] = a0;
-    S.jxring[3 * (size_t)q + 1] = a1;
-    S.jxring[3 * (size_t)q + 2] = a2;
+    JC[3 * (size_t)q] = a0;
+    JC[3 * (size_t)q + 1] = a1;
+    JC[3 * (size_t)q + 2] = a2;
   }
 }
 

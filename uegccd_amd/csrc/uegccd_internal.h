@@ -158,10 +158,10 @@ struct SolveDevT {
   Ptr<double2> lad;         // [n3] particle-particle ladder, (S, P) parts
   Ptr<double2> w;           // [wtot] hole-hole W intermediates, (S, P) parts
   int wtot;
-  Ptr<double> jring;        // [o*v rows][o][3] quadratic ring intermediates (J1, J2, J3),
-                        // rows in the grouped P(q) order (qrow)
-  Ptr<double> jxring;       // [o*v rows][o][3] quadratic crossed-ring intermediates, same rows
-  Ptr<double4> tv;          // [n3] slice products (TV_ab, TV_ba, VT_ab, VT_ba)
+  Ptr<double2> jring;       // [o*v rows][o] quadratic ring intermediates (J2, J3), rows
+                            // in the grouped P(q) order (qrow)
+  Ptr<double2> jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
+  Ptr<double2> tv;          // [n3] slice products (TV_ab + VT_ab, TV_ba + VT_ba)
   Ptr<double4> quad;        // [n3] quadratic ring (ab, ba) and crossed-ring (ab, ba) sums
   Ptr<double> rsum;         // [v*o] slice row sums
   Ptr<double> csum;         // [v*o] slice column sums
