@@ -35,7 +35,7 @@ METRIC = "CCD solve time & twist-solves/sec at |ΔE|<1e-6 Ha vs FP64; p50/p99 la
 WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
-KERNELS = {0: "k_slice", 1: "k_contract", 2: "k_ladder", 3: "k_main", 4: "k_finalize"}
+KERNELS = {0: "k_slice", 1: "k_contract", 3: "k_main", 4: "k_finalize"}
 def algo_work(kernel, o, v, nvalid, ndiis, spp, shh):
     """Algorithmic (flops, bytes) of one launch in the terms of the reference's
     loops (source/CCD.f90): valid elements only, each compulsory datum moved
@@ -48,21 +48,18 @@ def algo_work(kernel, o, v, nvalid, ndiis, spp, shh):
                   over the valid amplitudes (4 products, 2 o flops per valid
                   element each) and the quadratic J sums (6 accumulations of
                   2 flops per valid element and l).
-      k_ladder:   8 S_pp (particle-particle ladder) + 8 S_hh (the W
-                  intermediates, half the hole-hole ladder's 16 S_hh); reads T.
-      k_contract: the quadratic ring and crossed-ring contractions (3 + 3
-                  products of 2 flops per valid element and l) and the other
-                  8 S_hh of the hole-hole ladder; reads T.
+      k_contract: the ladders, 8 S_pp (particle-particle) + 16 S_hh (the W
+                  intermediates and the hole-hole products), and the quadratic
+                  ring and crossed-ring contractions (3 + 3 products of 2 flops
+                  per valid element and l); reads T.
       k_main:     60 flops per valid element; reads T, writes the new T, reads
                   the ndiis DIIS error entries of the new B-matrix row and
                   writes one.
       k_finalize: reductions of the block partials (intermediates only)."""
     if kernel == "k_slice":
         return (20.0 * o * nvalid + 4.0 * ndiis * nvalid, 16 * (ndiis + 1) * nvalid)
-    if kernel == "k_ladder":
-        return (8.0 * spp + 8.0 * shh, 16 * nvalid)
     if kernel == "k_contract":
-        return (12.0 * o * nvalid + 8.0 * shh, 16 * nvalid)
+        return (8.0 * spp + 16.0 * shh + 12.0 * o * nvalid, 16 * nvalid)
     if kernel == "k_main":
         return (60.0 * nvalid, (16 + 16 + 16 * ndiis + 16) * nvalid)
     if kernel == "k_finalize":

@@ -186,11 +186,12 @@ int uegccd_get_amplitudes(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, doubl
 
 /* Average device time (microseconds, HIP events on the context's stream) of
  * one kernel of the residual pass over the context's current amplitudes:
- * 0 per-slice amplitudes and ring/mosaic products, 1 contractions (quadratic
- * rings per momentum transfer, hole-hole ladder tiles, Joo), 2
- * particle-particle ladder and W tiles, 3 residual assembly + update; the
- * ladders in the context's starting precision.  No counterpart in the
- * reference; measurement support. */
+ * 0 per-slice amplitudes and ring/mosaic products, 1 contractions
+ * (particle-particle ladder tiles, hole-hole W rows and ladder, quadratic
+ * rings per momentum transfer, Joo), 3 residual assembly + update (2, the
+ * former separate ladder kernel, is rejected); the ladders in the context's
+ * starting precision.  No counterpart in the reference; measurement
+ * support. */
 int uegccd_kernel_time(uegccd_ctx* ctx, int32_t kernel, int32_t reps, double* avg_us);
 
 /* Tile choices of the tuned kernels: k-steps per group of loads in the

@@ -197,12 +197,12 @@ struct uegccd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   uegccd_options opt{};
-  int o = 0, v = 0, nao = 0, n3 = 0, nK = 0, n2max = 0, wtot = 0, maxiter = 1000;
+  int o = 0, v = 0, nao = 0, n3 = 0, nK = 0, n2max = 0, maxiter = 1000;
   int64_t nvalid = 0;
   double tolmax = 1e-8, failratio = 100, df = 2.0;
   double boxL = 0.0;  // box length L
   // host copies of the maps
-  std::vector<int> kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, kb_woff, pair_kb, pair_pos;
+  std::vector<int> kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, pair_kb, pair_pos;
   std::vector<int> kb_toff;        // K-blocked amplitude layout: block offsets
   std::vector<int> slab_coff;        // history layout: first compact index of each virtual slice
   std::vector<uint2> cmap;           // compact index -> elem_map(element, K-blocked position, partner)
@@ -214,7 +214,8 @@ struct uegccd_ctx {
   int coren = 0;                   // core orbitals frozen by the CoreN masking (0: none)
   std::vector<int> core_elems;     // valid elements with a core index (held at zero)
   double screening = 0.0;          // HEGData%ScreeningDistance
-  std::vector<int4> lad_items;  // (0, K, row0, col0)
+  std::vector<int4> hh_items;   // (0, K, row0, 0) hole-hole units
+  std::vector<int4> lad_items;  // (0, K, row0, col0) particle-particle tiles
   std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow, qb_of;  // hole-particle sets P(q)
   std::vector<int4> qb_info;
   std::vector<int4> con_items;  // (0, kind, arg, arg) of k_contract
@@ -299,7 +300,6 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   c->nK = (int)pairs.size();
   c->kb_poff.assign(c->nK + 1, 0);
   c->kb_voff.assign(c->nK + 1, 0);
-  c->kb_woff.assign(c->nK + 1, 0);
   c->kb_pairs.clear();
   c->kb_va.clear();
   c->kb_vb.clear();
@@ -326,10 +326,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     }
     c->kb_poff[K + 1] = (int)c->kb_pairs.size();
     c->kb_voff[K + 1] = (int)c->kb_va.size();
-    const int noo = (int)pairs[K].size();
-    c->kb_woff[K + 1] = c->kb_woff[K] + noo * noo;
   }
-  c->wtot = c->kb_woff[c->nK];
   c->noo_max = c->nvv_max = 0;
   for (int K = 0; K < c->nK; ++K) {
     c->noo_max = std::max(c->noo_max, c->kb_poff[K + 1] - c->kb_poff[K]);
@@ -380,16 +377,27 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->cmap.resize(nc);
     c->nvalid = nc;
   }
-  // ladder tiles: 16 pair rows x 16 columns (a list then pair list)
+  // ladder work: per 16 pair rows of a block, one hole-hole unit (its W rows
+  // and the hole-hole products of those rows) and the particle-particle
+  // tiles of 16 a columns; the largest blocks' units first
+  c->hh_items.clear();
   c->lad_items.clear();
   for (int K = 0; K < c->nK; ++K) {
     const int noo = c->kb_poff[K + 1] - c->kb_poff[K];
     const int nvv = c->kb_voff[K + 1] - c->kb_voff[K];
     if (nvv == 0) continue;
-    const int ncol = nvv + noo;
-    for (int r0 = 0; r0 < noo; r0 += 16)
-      for (int c0 = 0; c0 < ncol; c0 += 16) c->lad_items.push_back(make_int4(0, K, r0, c0));
+    for (int r0 = 0; r0 < noo; r0 += 16) {
+      c->hh_items.push_back(make_int4(0, K, r0, 0));
+      for (int c0 = 0; c0 < nvv; c0 += 16) c->lad_items.push_back(make_int4(0, K, r0, c0));
+    }
   }
+  std::stable_sort(c->hh_items.begin(), c->hh_items.end(), [c](const int4& a, const int4& b) {
+    auto work = [c](int K) {
+      const int64_t noo = c->kb_poff[K + 1] - c->kb_poff[K], nvv = c->kb_voff[K + 1] - c->kb_voff[K];
+      return noo * nvv;
+    };
+    return work(a.y) > work(b.y);
+  });
   // Coulomb table, flag 0 of the ERI variants (source/HEG.f90:729-772)
   int n2max = 0;
   for (int p = 0; p < nao; ++p)
@@ -445,11 +453,10 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
                                 qn >= 0 ? c->qb_roff[qn] : 0,
                                 qn >= 0 ? c->qb_roff[qn + 1] - c->qb_roff[qn] : 0);
     }
-    // work items of the contraction pass
+    // wave items of the contraction pass: the Joo sums over slice ranges
+    // (first: they read the most data per wave), 16x16 tiles of the
+    // per-transfer ring products
     c->con_items.clear();
-    // one wave per item: the Joo sums over slice ranges (first: they read the
-    // most data per wave), 16x16 tiles of the per-transfer ring products,
-    // 16x16 hole-hole ladder tiles
     if (s.do_mosaic)
       for (int p = 0; p < kJooParts; ++p) c->con_items.push_back(make_int4(0, CK_JOO, p, 0));
     if (s.do_ring || s.do_xring)
@@ -460,13 +467,6 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
           if (s.do_ring) c->con_items.push_back(make_int4(0, CK_RING, q, t));
           if (s.do_xring) c->con_items.push_back(make_int4(0, CK_XRING, q, t));
         }
-      }
-    if (s.do_ladder)
-      for (int K = 0; K < c->nK; ++K) {
-        const int noo = c->kb_poff[K + 1] - c->kb_poff[K];
-        const int nvv = c->kb_voff[K + 1] - c->kb_voff[K];
-        const int nt = ((noo + 15) / 16) * ((nvv + 15) / 16);
-        for (int t = 0; t < nt; ++t) c->con_items.push_back(make_int4(0, CK_HH, K, t));
       }
   }
   c->n2max = n2max;
@@ -545,6 +545,9 @@ static size_t jring_doubles(const uegccd_ctx* c) {
 }
 
 // Device bytes of a context: the arena ctx_create carves its buffers from.
+// Blocks of a k_contract range, padded to a multiple of the 8 XCDs.
+static int pad8(int n) { return (n + 7) & ~7; }
+
 size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   const size_t n3 = c->n3;
   const size_t eb = elem_blocks_of(c);
@@ -563,7 +566,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double2) * std::max<size_t>(1, c->kb_toff.back()));  // tk
   bytes += aligned(sizeof(int) * (c->nK + 1));                         // kb_toff
   bytes += aligned(sizeof(double2) * n3);                              // lad
-  bytes += aligned(sizeof(double2) * std::max(1, c->wtot));            // w
+  bytes += aligned(sizeof(double2) * std::max(1, c->kb_toff.back()));  // hh
   bytes += 2 * aligned(sizeof(double) * jring_doubles(c));             // jring, jxring
   bytes += aligned(sizeof(double2) * n3);                              // tv
   bytes += 2 * aligned(sizeof(double) * c->o * c->v);                  // rsum, csum
@@ -572,8 +575,9 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double) * eb * kPartialStride);
   bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
-  bytes += aligned(sizeof(int) * (6 + 2 * kSliceClasses)) +
-           aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size()));
+  bytes += aligned(sizeof(int) * (8 + 2 * kSliceClasses)) +
+           aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size())) +
+           aligned(sizeof(int4) * std::max<size_t>(1, c->hh_items.size()));
   bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
            aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
@@ -791,7 +795,6 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.kb_va = im.put(c->kb_va);
   S.kb_vb = im.put(c->kb_vb);
   S.kb_nva = im.put(c->kb_nva);
-  S.kb_woff = im.put(c->kb_woff);
   S.kb_toff = im.put(c->kb_toff);
   S.slab_coff = im.put(c->slab_coff);
   S.cmap = im.put(c->cmap);
@@ -808,6 +811,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.qb_of = im.put(c->qb_of);
   c->d_sd = im.reserve<SolveDev>(1);
   // the solve's ladder tiles and contraction items (any batch reads them here)
+  S.nhh = sys->do_ladder ? (int)c->hh_items.size() : 0;
+  S.hh_items = im.put(c->hh_items);
   S.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
   S.lad_items = im.put(c->lad_items);
   S.ncon = (int)c->con_items.size();
@@ -817,26 +822,32 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
   const int sb = need_slice ? c->v : 0;
   const int cls = slice_threads(c->o) / 256 - 1;
-  int h_pref[6 + 2 * kSliceClasses] = {0, eb, 0, S.nlad, 0, S.ncon};
-  h_pref[6 + 2 * cls + 1] = sb;
-  int* d_pref = im.reserve<int>(6 + 2 * kSliceClasses);
-  im.write(d_pref, h_pref, 6 + 2 * kSliceClasses);
+  int h_pref[8 + 2 * kSliceClasses] = {0, eb, 0, S.nlad, 0, S.ncon, 0, S.nhh};
+  h_pref[8 + 2 * cls + 1] = sb;
+  int* d_pref = im.reserve<int>(8 + 2 * kSliceClasses);
+  im.write(d_pref, h_pref, 8 + 2 * kSliceClasses);
   b.n = 1;
   b.solves = c->d_sd;
   b.elem_prefix = d_pref;
   b.lad_prefix = d_pref + 2;
   b.con_prefix = d_pref + 4;
+  b.hh_prefix = d_pref + 6;
   b.drpa_sum_blocks = kDrpaJooParts * c->o + c->v + (int)c->qb_neg.size();
   b.elem_blocks = eb;
   b.slice_blocks = sb;
   b.slice_split = S.slice_split;
   for (int k = 0; k < kSliceClasses; ++k) {
-    b.slice_prefix[k] = d_pref + 6 + 2 * k;
+    b.slice_prefix[k] = d_pref + 8 + 2 * k;
     b.slice_cls_blocks[k] = k == cls ? sb : 0;
     b.slice_lds[k] = k == cls ? slice_lds_bytes(c->o, S.slice_split) : 0;
   }
+  b.nhh = S.nhh;
+  b.nhh_pad = pad8(S.nhh);
   b.nlad = S.nlad;
-  b.lad_lds = ladder_lds_bytes(S);
+  b.nlad_pad = pad8(S.nlad);
+  b.tab_doubles = ladder_tab_doubles(S);
+  b.cls_max = std::max(1, c->nvv_max);
+  b.con_lds = contract_lds_bytes(b.tab_doubles, b.cls_max, c->noo_max);
   b.tiles = c->tiles;
   b.ncon = S.ncon;
   // work buffers
@@ -849,8 +860,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.gout = c->arena.take<double>(2 * n3);
   S.rout = c->arena.take<double>(2 * n3);
   S.lad = c->arena.take<double2>(n3);
-  S.wtot = c->wtot;
-  S.w = c->arena.take<double2>(std::max(1, c->wtot));
+  S.hh = c->arena.take<double2>(std::max(1, c->kb_toff[c->nK]));
   S.jring = c->arena.take<double2>(jring_doubles(c) / 2);
   S.jxring = c->arena.take<double2>(jring_doubles(c) / 2);
   S.tv = c->arena.take<double2>(n3);
@@ -1220,12 +1230,12 @@ int upload_image(char* dst, const std::vector<char>& image, hipStream_t stream, 
 // gather buffer, in one (reused) allocation uploaded in one copy.
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = nullptr,
                PinnedStage* stage = nullptr) {
-  std::vector<int> ep(n + 1, 0), lp(n + 1, 0), cp(n + 1, 0);
+  std::vector<int> ep(n + 1, 0), lp(n + 1, 0), cp(n + 1, 0), hp(n + 1, 0);
   std::vector<int> sp[kSliceClasses];
   int slds[kSliceClasses] = {};
   for (auto& v : sp) v.assign(n + 1, 0);
   // one k_slice instantiation serves the launch: the split one if any solve needs it
-  int slice_split = 0, lad_lds = 0;
+  int slice_split = 0, tab_doubles = 0, cls_max = 1, noo_max = 0;
   for (int k = 0; k < n; ++k) slice_split |= cs[k]->sd.slice_split;
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
@@ -1235,10 +1245,13 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     for (int q = 0; q < kSliceClasses; ++q) sp[q][k + 1] = sp[q][k] + (need_slice && q == cls ? c->v : 0);
     if (need_slice) slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o, slice_split));
     lp[k + 1] = lp[k] + c->sd.nlad;
-    lad_lds = std::max(lad_lds, ladder_lds_bytes(c->sd));
+    hp[k + 1] = hp[k] + c->sd.nhh;
+    tab_doubles = std::max(tab_doubles, ladder_tab_doubles(c->sd));
+    cls_max = std::max(cls_max, c->nvv_max);
+    noo_max = std::max(noo_max, c->noo_max);
     cp[k + 1] = cp[k] + c->sd.ncon;
   }
-  size_t bytes = aligned(sizeof(SolveDev) * n) + (3 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
+  size_t bytes = aligned(sizeof(SolveDev) * n) + (4 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(State) * n);
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
@@ -1265,7 +1278,13 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     b.slice_blocks += sp[q][n];
   }
   b.lad_prefix = const_cast<int*>(im.put(lp));
-  b.lad_lds = lad_lds;
+  b.hh_prefix = const_cast<int*>(im.put(hp));
+  b.nhh = hp[n];
+  b.nhh_pad = pad8(hp[n]);
+  b.nlad_pad = pad8(lp[n]);
+  b.tab_doubles = tab_doubles;
+  b.cls_max = cls_max;
+  b.con_lds = contract_lds_bytes(tab_doubles, cls_max, noo_max);
   // one instantiation per launch: the tiles of the solve with the most ladder tiles
   {
     int big = 0;
@@ -1405,7 +1424,7 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   const bool persist = n == 1 && &b == &c0->batch && c0->opt.persistent && !timing && !no_alloc &&
                        !c0->sd.slice_split;
   const bool graphs = !persist && n == 1 && &b == &c0->batch && c0->opt.graph && !timing && !no_alloc;
-  const int persist_lds = std::max(slice_lds_bytes(c0->o), ladder_lds_bytes(c0->sd));
+  const int persist_lds = std::max(slice_lds_bytes(c0->o), b.con_lds);
   while (!all_done && passes <= maxiter + 2) {
     if (persist) {
       if (launch_persistent(b, persist_lds, chunk, s)) {
@@ -1659,7 +1678,7 @@ extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, do
 }
 
 extern "C" int uegccd_time_kernel(uegccd_ctx* c, int32_t which) {
-  if (!c || which < -1 || which > 4) return UEGCCD_ERR_INVALID;
+  if (!c || which < -1 || which > 4 || which == 2) return UEGCCD_ERR_INVALID;
   c->timed_kernel = which;
   c->timed_us = 0.0;
   c->timed_launches = 0;
@@ -1704,7 +1723,7 @@ extern "C" int uegccd_get_tiles(const uegccd_ctx* c, uegccd_tiles* t, int32_t* n
 }
 
 extern "C" int uegccd_kernel_time(uegccd_ctx* c, int32_t which, int32_t reps, double* avg_us) {
-  if (!c || !avg_us || reps <= 0 || which < 0 || which > 3) return UEGCCD_ERR_INVALID;
+  if (!c || !avg_us || reps <= 0 || which < 0 || which > 3 || which == 2) return UEGCCD_ERR_INVALID;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   // the ladders in the context's starting precision (FP32 for mixed precision)

@@ -1,17 +1,16 @@
 // CDNA4 (gfx950) kernels of the CCD amplitude iteration.
 //
 // One residual pass (TermsWrapper, source/CCD.f90:146-181) plus the update
-// (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is five launches:
+// (SolveCC/GetRes/CnvrgCC/CCEnergy, source/CCD.f90:186-462) is four launches:
 //   k_slice         per virtual slice: the pass's amplitudes (the last update
 //                   or its DIIS extrapolation, source/DIISCC.f90:52-131),
 //                   linear ring / crossed-ring products, quadratic ring
 //                   intermediates, mosaic partial sums
 //                   (source/CCD.f90:507-585, 689-940)
-//   k_ladder        the particle-particle ladder and the hole-hole W
-//                   intermediates as pair-momentum-blocked MFMA tiles
-//                   (source/CCD.f90:611-681)
-//   k_contract      quadratic ring / crossed-ring products per momentum
-//                   transfer, hole-hole ladder tiles (MFMA), mosaic Joo
+//   k_contract      the particle-particle ladder, the hole-hole W rows and
+//                   ladder as pair-momentum-blocked MFMA tiles
+//                   (source/CCD.f90:611-681), the quadratic ring /
+//                   crossed-ring products per momentum transfer, mosaic Joo
 //   k_main          residual assembly, R = D*T - G, damped update, dT, energy,
 //                   DIIS dot products
 //   k_finalize      deterministic reductions, convergence test, DIIS solve
@@ -20,6 +19,8 @@
 //
 // Interleaved layouts keep each gathered access one 16/32/64-byte load:
 //   tcur, lad, histT/histE entries   double2  (abab, abba) or (S, P) per (i,j,a)
+//   tk, hh                           double2  the valid elements by K block (column
+//                                    ci of block K holds its pairs contiguously)
 //   tv                               double2  (TV_ab + VT_ab, TV_ba + VT_ba) per (i,j,a)
 //   jring / jxring                   double2  (J2, J3) per (row of P(q), l)
 //   histT / histE                    valid elements only, in element order (compact
@@ -748,58 +749,38 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
 // in place of (a,b):
 //   W_S[p,q] = v(i-k) - v(i-l) + 1/2 sum_c S[p,c] (v(c-k) - v(c-l))
 //   W_P[p,q] = v(i-k) + v(i-l) + 1/2 sum_c P[p,c] (v(c-k) + v(c-l))
-// so one tile kernel serves both: rows = pairs of K (16 per tile), columns =
-// the block's a list followed by its pair list.  The B operand is never
-// stored: each lane synthesises its element from two n-vectors and the
-// Coulomb table.  One wave64 per 16x16 tile, v_mfma_f64_16x16x4f64 (or the
-// f32 form while the mixed-precision schedule is in its FP32 phase).
+// so one tile routine serves both: rows = pairs of K (16 per tile), columns
+// = the block's a list (particle-particle tiles) or its pair list (W).  The
+// B operand is never stored: each lane synthesises its element from two
+// n-vectors and the Coulomb table.  One wave64 per 16x16 tile,
+// v_mfma_f64_16x16x4f64 (or the f32 form while the mixed-precision schedule
+// is in its FP32 phase).  The W columns of a row tile are reduced inside one
+// block, the hole-hole unit, which keeps its W rows in LDS and contracts the
+// hole-hole ladder from there (source/CCD.f90:651-656):
+//   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a].
 
-// One 16x16 tile, the reduction over the K block's virtual list split over
-// the four waves of the block (wave w takes steps w, w+4, ...); the waves'
-// partial sums are combined in a fixed order.  vt is the Coulomb table and
-// cls the block's virtual list (n_c, c), both in LDS, so each group of steps
-// costs one round of global loads (the amplitudes its list entries select).
-// vt holds the solve's distinct ladder tables (n2max + 1 entries each);
-// lad_tab[] picks the table of each role: the particle-particle columns
-// read the linear direct (v(c-a)) and exchange (v(c-b)) tables, the W
-// columns the quadratic ones (v(c-k), v(c-l)) and, for v(i-k) and v(i-l),
-// the linear ones.
+// The sums of one 16x16 tile (column lane r16 synthesises its B column from
+// the orbitals cx, cy and the tables vtx, vty), the reduction over the K
+// block's virtual list split over the block's four waves (wave w takes steps
+// w, w+4, ...) and combined in a fixed order.  cls is the block's virtual list
+// (n_c, c) in LDS, so each group of steps costs one round of global loads
+// (the amplitudes its list entries select).  Every thread of the block calls
+// it; wave 0 returns true with the tile's sums in accs, accp.
 template <typename Real, int G>
-__device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
-                            const int4* cls, double* red) {
-  typedef typename MfmaTraits<Real>::acc_t acc_t;
-  const int o = S.o, o2 = o * o;
+__device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, int cx, int cy, bool colv,
+                                            const double* vtx, const double* vty, const int4* cls,
+                                            double* red, typename MfmaTraits<Real>::acc_t& accs,
+                                            typename MfmaTraits<Real>::acc_t& accp) {
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
-  const int ncol = nvv + noo;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   // A operand from the K-blocked amplitudes: column ci of the block holds its
   // noo pairs contiguously, so a step's 16 rows are one contiguous run
   const auto TK = S.tk + S.kb_toff[K];
-
   const int rowA = row0 + r16;
-  const int col = col0 + r16;
-  const bool colv = col < ncol;
-  int cx = 0, cy = 0;
-  if (colv) {
-    if (col < nvv) {
-      cx = S.kb_va[voff + col];
-      cy = S.kb_vb[voff + col];
-    } else {
-      const int xq = S.kb_pairs[poff + col - nvv];
-      cx = xq % o;
-      cy = xq / o;
-    }
-  }
-  // the output rows' pairs, requested now (used after the reduction)
-  int xpo[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) xpo[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<Real>::row(kq, r), noo - 1)];
   const int4 nx = S.nv4[cx], ny = S.nv4[cy];
-  const int nt = S.n2max + 1;
-  const double* vtx = vt + nt * (col < nvv ? S.lad_tab[0] : S.lad_tab[2]);
-  const double* vty = vt + nt * (col < nvv ? S.lad_tab[1] : S.lad_tab[3]);
-  acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
+  accs = {0, 0, 0, 0};
+  accp = {0, 0, 0, 0};
   // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
   // their amplitude loads (unconditional, in-range addresses, masked values)
   // before the MFMAs, which run for every step (masked steps add zeros).
@@ -836,7 +817,7 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
     }
   }
   __syncthreads();
-  if (wave != 0 || !colv) return;
+  if (wave != 0) return false;
   for (int w = 1; w < 4; ++w) {
     const double* theirs = red + ((w - 1) * 64 + lane) * 8;
     for (int r = 0; r < 4; ++r) {
@@ -844,44 +825,269 @@ __device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const do
       accp[r] = accp[r] + (Real)theirs[4 + r];
     }
   }
+  return true;
+}
+
+// One 16x16 particle-particle tile (rows row0.., columns col0.. of the a
+// list) into lad.  vt holds the solve's distinct ladder tables (n2max + 1
+// entries each); lad_tab[] picks the table of each role: these columns read
+// the linear direct (v(c-a)) and exchange (v(c-b)) tables.
+template <typename Real, int G>
+__device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
+                            const int4* cls, double* red) {
+  typedef typename MfmaTraits<Real>::acc_t acc_t;
+  const int o = S.o, o2 = o * o;
+  const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
+  const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
+  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int col = col0 + r16;
+  const bool colv = col < nvv;
+  const int cx = colv ? S.kb_va[voff + col] : 0, cy = colv ? S.kb_vb[voff + col] : 0;
+  // the output rows' pairs, requested now (used after the reduction)
+  int xpo[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) xpo[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<Real>::row(kq, r), noo - 1)];
+  const int nt = S.n2max + 1;
+  acc_t accs, accp;
+  if (!ladder_sums<Real, G>(S, K, row0, cx, cy, colv, vt + nt * S.lad_tab[0], vt + nt * S.lad_tab[1], cls, red,
+                            accs, accp) ||
+      !colv)
+    return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int rowo = row0 + MfmaTraits<Real>::row(kq, r);
     if (rowo >= noo) continue;
-    const int xpo_r = xpo[r];
-    const double as = (double)accs[r], ap = (double)accp[r];
-    if (col < nvv) {
-      st2(S.lad + (size_t)xpo_r + (size_t)o2 * (cx - o), make_double2(0.5 * as, 0.5 * ap));
-    } else {
-      const int q = col - nvv;
-      const int i = xpo_r % o;
-      const int4 ni = S.nv4[i];
-      const double vik = vt[nt * S.lad_tab[0] + d2(ni, nx)], vil = vt[nt * S.lad_tab[1] + d2(ni, ny)];
-      st2(S.w + S.kb_woff[K] + rowo * noo + q, make_double2(vik - vil + 0.5 * as, vik + vil + 0.5 * ap));
+    st2(S.lad + (size_t)xpo[r] + (size_t)o2 * (cx - o), make_double2(0.5 * (double)accs[r], 0.5 * (double)accp[r]));
+  }
+}
+
+// The W rows row0..row0+15 of block K over NC tiles of 16 pair columns at
+// once (columns cbase + 16c + r16, c < NC): one pass over the block's virtual list,
+// each group of amplitude loads shared by the NC tiles, every tile with the
+// same MFMA sequence as a tile of its own; then each tile's four wave sums
+// are combined in a fixed order and W = v(i-k) -+ v(i-l) + 1/2 sums goes to
+// Wl (16 x noo, (W_S, W_P); zeros in rows past the block).  The W columns
+// read the quadratic tables v(c-k), v(c-l), and v(i-k), v(i-l) the linear
+// ones.  Every thread of the block calls it.
+template <typename Real, int G, int NC>
+__device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cbase, const double* vt,
+                                       const int4* cls, double* red, double2* Wl) {
+  typedef typename MfmaTraits<Real>::acc_t acc_t;
+  const int o = S.o;
+  const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
+  const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int nt = S.n2max + 1;
+  const double* vtx = vt + nt * S.lad_tab[2];
+  const double* vty = vt + nt * S.lad_tab[3];
+  const auto TK = S.tk + S.kb_toff[K];
+  const int rowA = row0 + r16;
+  int4 nx[NC], ny[NC];
+  bool colv[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int q = cbase + 16 * c + r16;
+    colv[c] = q < noo;
+    const int xq = colv[c] ? S.kb_pairs[poff + q] : 0;
+    nx[c] = S.nv4[xq % o];
+    ny[c] = S.nv4[xq / o];
+  }
+  acc_t accs[NC], accp[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    accs[c] = {0, 0, 0, 0};
+    accp[c] = {0, 0, 0, 0};
+  }
+  for (int g0 = 4 * wave; g0 < nvv; g0 += 16 * G) {
+    double2 t[G];
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      const int ci = g0 + 16 * st + kq;
+      const bool ok = ci < nvv && rowA < noo;
+      const double2 v = ld(TK + (ok ? ci * noo + rowA : 0));
+      t[st] = ok ? v : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int st = 0; st < G; ++st) {
+      const int ci = g0 + 16 * st + kq;
+      const int4 nc = cls[min(ci, nvv - 1)];
+      const Real as = (Real)(t[st].x + t[st].y), ap = (Real)(t[st].x - t[st].y);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        Real bs = 0, bp = 0;
+        if (ci < nvv && colv[c]) {
+          const Real vx = (Real)vtx[d2(nc, nx[c])];
+          const Real vy = (Real)vty[d2(nc, ny[c])];
+          bs = vx - vy;
+          bp = vx + vy;
+        }
+        accs[c] = MfmaTraits<Real>::mma(as, bs, accs[c]);
+        accp[c] = MfmaTraits<Real>::mma(ap, bp, accp[c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (cbase + 16 * c >= noo) break;
+    if (wave > 0) {
+      double* mine = red + ((wave - 1) * 64 + lane) * 8;
+      for (int r = 0; r < 4; ++r) {
+        mine[r] = (double)accs[c][r];
+        mine[4 + r] = (double)accp[c][r];
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {
+      acc_t ss = accs[c], pp = accp[c];
+      for (int w = 1; w < 4; ++w) {
+        const double* theirs = red + ((w - 1) * 64 + lane) * 8;
+        for (int r = 0; r < 4; ++r) {
+          ss[r] = ss[r] + (Real)theirs[r];
+          pp[r] = pp[r] + (Real)theirs[4 + r];
+        }
+      }
+      if (colv[c]) {
+        const int q = cbase + 16 * c + r16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int lr = MfmaTraits<Real>::row(kq, r), rowo = row0 + lr;
+          double2 wv = make_double2(0.0, 0.0);
+          if (rowo < noo) {
+            const int4 ni = S.nv4[S.kb_pairs[poff + rowo] % o];
+            const double vik = vt[nt * S.lad_tab[0] + d2(ni, nx[c])], vil = vt[nt * S.lad_tab[1] + d2(ni, ny[c])];
+            wv = make_double2(vik - vil + 0.5 * (double)ss[r], vik + vil + 0.5 * (double)pp[r]);
+          }
+          Wl[lr * noo + q] = wv;
+        }
+      }
+    }
+    __syncthreads();  // red is reused by the next tile; Wl is complete after the last
+  }
+}
+
+// The hole-hole unit of rows row0..row0+15 of block K: their W rows over all
+// noo pair columns into Wl (w_rows, two column tiles per pass over the
+// virtual list), then hh = W . (S, P) as 16x16 FP64 MFMA tiles over the
+// block's a columns, dealt over the waves, A from Wl and B from the
+// K-blocked amplitudes (a column's pairs contiguous), into hh at the
+// amplitudes' K-blocked positions.  HS: k-steps of 4 pairs per round of
+// loads (from the tuned tile table).
+constexpr int kWGroup = 8;
+template <typename Real, int G, int HS>
+__device__ void hh_unit(const SolveG& S, int K, int row0, const double* vt, const int4* cls, double* red,
+                        double2* Wl) {
+  typedef MfmaTraits<double>::acc_t acc64_t;
+  const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
+  const int nvv = S.kb_voff[K + 1] - S.kb_voff[K];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  // (the W pass is the unit's longest chain of loads: kWGroup steps per round)
+  if (noo <= 16)
+    w_rows<Real, kWGroup, 1>(S, K, row0, 0, vt, cls, red, Wl);
+  else
+    for (int cb = 0; cb < noo; cb += 32) w_rows<Real, kWGroup, 2>(S, K, row0, cb, vt, cls, red, Wl);
+  const int rowA = row0 + r16;
+  const bool rv = rowA < noo;
+  const double2* Wrow = Wl + r16 * noo;
+  const auto TKb = S.tk + S.kb_toff[K];
+  const auto HH = S.hh + S.kb_toff[K];
+  for (int c0 = 16 * wave; c0 < nvv; c0 += 64) {
+    const int colB = c0 + r16;
+    const bool cv = colB < nvv;
+    const auto TK = TKb + (size_t)(cv ? colB : 0) * noo;
+    acc64_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
+    for (int g0 = 0; g0 < noo; g0 += 4 * HS) {
+      double ws[HS], wp[HS];
+#pragma unroll
+      for (int st = 0; st < HS; ++st) {
+        const int q = g0 + 4 * st + kq;
+        const double2 w = Wrow[min(q, noo - 1)];
+        const bool ok = q < noo && rv;
+        ws[st] = ok ? w.x : 0.0;
+        wp[st] = ok ? w.y : 0.0;
+      }
+      double bs[HS], bp[HS];
+#pragma unroll
+      for (int st = 0; st < HS; ++st) {
+        const int q = g0 + 4 * st + kq;
+        const double2 t = ld(TK + min(q, noo - 1));
+        const bool ok = q < noo && cv;
+        bs[st] = ok ? t.x + t.y : 0.0;
+        bp[st] = ok ? t.x - t.y : 0.0;
+      }
+      // (all steps, the masked ones add zeros: a branch here would let the
+      // compiler sink each step's loads behind the previous step's MFMAs)
+#pragma unroll
+      for (int st = 0; st < HS; ++st) {
+        accs = MfmaTraits<double>::mma(ws[st], bs[st], accs);
+        accp = MfmaTraits<double>::mma(wp[st], bp[st], accp);
+      }
+    }
+    if (!cv) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int p = row0 + MfmaTraits<double>::row(kq, r);
+      if (p < noo) st2(HH + (size_t)colB * noo + p, make_double2(0.5 * accs[r], 0.5 * accp[r]));
     }
   }
 }
 
-template <int G>
-__device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                             int n, int blk, double* vt, int4* cls, double* red) {
-  const int s = find_solve(prefix, n, blk);
-  const SolveG& S = as_global(solves)[s];
-  const int4 it = ld(S.lad_items + (blk - prefix[s]));
-  const int phase = S.st->phase;  // requested with the tables, tested after them
-  const int K = it.y, voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
+// Dynamic LDS of a ladder-tile or hole-hole block: the solve's distinct
+// ladder tables (n2max + 1 doubles each), then (16-byte aligned) a K block's
+// virtual list, then the hole-hole unit's W rows.
+struct LadderLds {
+  double* vt;
+  int4* cls;
+  double2* wl;
+};
+__device__ __forceinline__ LadderLds ladder_lds(double* dyn, int tab_doubles, int cls_max) {
+  LadderLds l;
+  l.vt = dyn;
+  l.cls = reinterpret_cast<int4*>(dyn + ((tab_doubles + 1) & ~1));
+  l.wl = reinterpret_cast<double2*>(l.cls + cls_max);
+  return l;
+}
+
+// Stages the solve's ladder tables and block K's virtual list in LDS; returns
+// the solve's state fields the block needs (phase, precision).
+__device__ __forceinline__ int2 stage_ladder(const SolveG& S, int K, const LadderLds& l) {
+  const int phase = S.st->phase, prec64 = S.st->prec64;  // requested with the tables, tested after them
+  const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int nt = S.n2max + 1;
   for (int m = 0; m < S.lad_ntab; ++m) {
     const auto src = S.vtab + (size_t)S.lad_slot[m] * nt;
-    for (int k = threadIdx.x; k < nt; k += blockDim.x) vt[m * nt + k] = src[k];
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) l.vt[m * nt + k] = src[k];
   }
-  for (int k = threadIdx.x; k < nvv; k += blockDim.x) cls[k] = ld(S.kb_nva + voff + k);
+  for (int k = threadIdx.x; k < nvv; k += blockDim.x) l.cls[k] = ld(S.kb_nva + voff + k);
   __syncthreads();
-  if (phase == PH_DONE) return;
-  if (S.st->prec64)
-    ladder_tile<double, G>(S, K, it.z, it.w, vt, cls, red);
+  return make_int2(phase, prec64);
+}
+
+template <int G>
+__device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+                             int n, int blk, const LadderLds& l, double* red) {
+  const int s = find_solve(prefix, n, blk);
+  const SolveG& S = as_global(solves)[s];
+  const int4 it = ld(S.lad_items + (blk - prefix[s]));
+  const int2 st = stage_ladder(S, it.y, l);
+  if (st.x == PH_DONE) return;
+  if (st.y)
+    ladder_tile<double, G>(S, it.y, it.z, it.w, l.vt, l.cls, red);
   else
-    ladder_tile<float, G>(S, K, it.z, it.w, vt, cls, red);
+    ladder_tile<float, G>(S, it.y, it.z, it.w, l.vt, l.cls, red);
+}
+
+template <int G, int HS>
+__device__ void hh_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+                         int n, int blk, const LadderLds& l, double* red) {
+  const int s = find_solve(prefix, n, blk);
+  const SolveG& S = as_global(solves)[s];
+  const int4 it = ld(S.hh_items + (blk - prefix[s]));
+  const int2 st = stage_ladder(S, it.y, l);
+  if (st.x == PH_DONE) return;
+  if (st.y)
+    hh_unit<double, G, HS>(S, it.y, it.z, l.vt, l.cls, red, l.wl);
+  else
+    hh_unit<float, G, HS>(S, it.y, it.z, l.vt, l.cls, red, l.wl);
 }
 
 template <int BT, bool SPLIT>
@@ -892,15 +1098,6 @@ __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solve
   __shared__ double vt2[SPLIT ? kMaxN2 : 1];
   extern __shared__ double dyn[];
   slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2);
-}
-
-template <int G>
-__global__ __launch_bounds__(256) void k_ladder(const SolveDev* __restrict__ solves,
-                                                const int* __restrict__ prefix, int n) {
-  extern __shared__ double vt[];  // the solve's ladder tables (ladder_lds_bytes)
-  __shared__ int4 cls[kMaxNao];   // a K block's virtual list has at most v < kMaxNao entries
-  __shared__ double red[3 * 64 * 8];
-  ladder_block<G>(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), vt, cls, red);
 }
 
 // --------------------------------------------------------------------------
@@ -1004,72 +1201,6 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
   }
 }
 
-// Hole-hole ladder contraction of one K block (source/CCD.f90:651-656):
-//   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a]
-// one wave64 per 16x16 (pair, a) tile on v_mfma_f64_16x16x4f64, added into
-// the particle-particle ladder of the same elements.  W and the K-blocked
-// amplitudes (a column's pairs contiguous) load in one round; the four
-// output elements are read together before they are written.
-// HS: k-steps of 4 pairs per round of loads (from the tuned tile table).
-template <int HS>
-__device__ void hh_tile(const SolveG& S, int K, int tile) {
-  typedef MfmaTraits<double>::acc_t acc_t;
-  const int o = S.o, o2 = o * o;
-  const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
-  const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
-  const int nct = (nvv + 15) >> 4;
-  if (tile >= ((noo + 15) >> 4) * nct) return;
-  const int row0 = (tile / nct) * 16, col0 = (tile % nct) * 16;
-  const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const auto W = S.w + S.kb_woff[K];
-  const int rowA = row0 + r16, colB = col0 + r16;
-  const bool rv = rowA < noo, cv = colB < nvv;
-  const int av = S.kb_va[voff + (cv ? colB : 0)] - o;
-  // B operand from the K-blocked amplitudes: column colB holds its pairs q
-  const auto TK = S.tk + S.kb_toff[K] + (size_t)(cv ? colB : 0) * noo;
-  int prow[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) prow[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<double>::row(kq, r), noo - 1)];
-  const size_t tslab = (size_t)o2 * av;
-  const auto Wrow = W + (size_t)(rv ? rowA : 0) * noo;
-  acc_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-  for (int g0 = 0; g0 < noo; g0 += 4 * HS) {
-    double ws[HS], wp[HS];
-#pragma unroll
-    for (int st = 0; st < HS; ++st) {
-      const int q = g0 + 4 * st + kq;
-      const double2 w = ld(Wrow + min(q, noo - 1));
-      const bool ok = q < noo && rv;
-      ws[st] = ok ? w.x : 0.0;
-      wp[st] = ok ? w.y : 0.0;
-    }
-    double bs[HS], bp[HS];
-#pragma unroll
-    for (int st = 0; st < HS; ++st) {
-      const int q = g0 + 4 * st + kq;
-      const double2 t = ld(TK + min(q, noo - 1));
-      const bool ok = q < noo && cv;
-      bs[st] = ok ? t.x + t.y : 0.0;
-      bp[st] = ok ? t.x - t.y : 0.0;
-    }
-#pragma unroll
-    for (int st = 0; st < HS; ++st) {  // all steps, as in quad_tile
-      accs = MfmaTraits<double>::mma(ws[st], bs[st], accs);
-      accp = MfmaTraits<double>::mma(wp[st], bp[st], accp);
-    }
-  }
-  if (!cv) return;
-  double2 cur[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) cur[r] = ld(S.lad + prow[r] + tslab);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int p = row0 + MfmaTraits<double>::row(kq, r);
-    if (p >= noo) continue;
-    st2(S.lad + prow[r] + tslab, make_double2(cur[r].x + 0.5 * accs[r], cur[r].y + 0.5 * accp[r]));
-  }
-}
-
 // Mosaic Joo(k) = sum over slices of the slice blocks' partials
 // (source/CCD.f90:563-567), split over kJooParts waves: wave `part` sums the
 // slices of its range in order (lane k, column k) into joo[part][k]; k_main
@@ -1085,8 +1216,8 @@ __device__ void joo_wave(const SolveG& S, int part) {
   }
 }
 
-// One contraction work item, by one wave.
-template <int QS, int HS>
+// One ring / crossed-ring / Joo work item, by one wave.
+template <int QS>
 __device__ void contract_item(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
                               int item) {
   const int s = find_solve(prefix, n, item);
@@ -1097,24 +1228,42 @@ __device__ void contract_item(const SolveDev* __restrict__ solves, const int* __
     quad_tile<false, QS>(S, it.z, it.w);
   else if (it.y == CK_XRING)
     quad_tile<true, QS>(S, it.z, it.w);
-  else if (it.y == CK_HH)
-    hh_tile<HS>(S, it.z, it.w);
   else
     joo_wave(S, it.z);
 }
 
-// One work item per wave; the four waves of a block take four consecutive
-// items of the XCD-ordered list.
-template <int QS, int HS>
-__global__ __launch_bounds__(kElemThreads) void k_contract(const SolveDev* __restrict__ solves,
-                                                            const int* __restrict__ prefix, int n,
-                                                            int nitems) {
+// The contraction pass, one launch of three kinds of 256-thread blocks:
+//   [0, nhh_pad)             hole-hole units (the longest blocks, so they go
+//                            first and, not renumbered, round-robin over the XCDs)
+//   [nhh_pad, + nlad_pad)    particle-particle ladder tiles
+//   the rest                 four ring / crossed-ring / Joo items per block
+// The pads are multiples of 8, so a block's XCD is still blockIdx mod 8 inside
+// each range, which the ladder tiles and items renumber XCD-contiguously.
+template <int G, int QS, int HS>
+__global__ __launch_bounds__(256) void k_contract(const SolveDev* __restrict__ solves,
+                                                  const int* __restrict__ hh_prefix, int nhh, int nhh_pad,
+                                                  const int* __restrict__ lad_prefix, int nlad, int nlad_pad,
+                                                  const int* __restrict__ con_prefix, int ncon, int n,
+                                                  int tab_doubles, int cls_max) {
+  extern __shared__ double dyn[];  // ladder tables, a virtual list, W rows
+  __shared__ double red[3 * 64 * 8];
+  const int bid = blockIdx.x;
+  if (bid < nhh_pad) {
+    if (bid < nhh) hh_block<G, HS>(solves, hh_prefix, n, bid, ladder_lds(dyn, tab_doubles, cls_max), red);
+    return;
+  }
+  int rb = bid - nhh_pad;
+  if (rb < nlad_pad) {
+    const int t = xcd_block(rb, nlad_pad);
+    if (t < nlad) ladder_block<G>(solves, lad_prefix, n, t, ladder_lds(dyn, tab_doubles, cls_max), red);
+    return;
+  }
+  rb -= nlad_pad;
   // the wave index is uniform over the wave: saying so lets the item, the
   // descriptor and its pointers live in scalar registers (scalar loads)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int item = xcd_block(blockIdx.x, gridDim.x) * kHHTilesPerBlock + wave;
-  if (item >= nitems) return;
-  contract_item<QS, HS>(solves, prefix, n, item);
+  const int item = xcd_block(rb, gridDim.x - nhh_pad - nlad_pad) * kItemsPerBlock + wave;
+  if (item < ncon) contract_item<QS>(solves, con_prefix, n, item);
 }
 
 // Block partials of the residual pass: max dT, max |R|, sum E, the non-finite
@@ -1167,11 +1316,12 @@ __device__ void main_block(const SolveDev* __restrict__ solves, const int* __res
   const int lb = gb - prefix[s];
   const int o = S.o, o2 = o * o, n3 = S.n3;
   const int cp = lb * kElemPerBlock + threadIdx.x;
-  int x = 0, b = -1;
+  int x = 0, b = -1, pos = 0;
   if (cp < S.nvalid) {
     const uint2 cm = ld(S.cmap + cp);
     x = (int)cm.x;
     b = elem_partner(cm);
+    pos = elem_pos(cm);
   }
   if (S.do_mosaic)  // Joo from its slice-range parts, in part order
     for (int k = threadIdx.x; k < o; k += blockDim.x) {
@@ -1224,9 +1374,10 @@ __device__ void main_block(const SolveDev* __restrict__ solves, const int* __res
       gab = gab - 0.5 * (joo[i] * t_ab + joo[j] * t_ab);
       gba = gba - 0.5 * (joo[i] * t_ba + joo[j] * t_ba);
     }
-    // ladders: particle-particle part (k_ladder) with the hole-hole sum (k_contract)
+    // ladders: particle-particle part with the hole-hole sum (k_contract)
     if (S.do_ladder) {
-      const double2 l2 = S.lad[x];
+      const double2 lp = S.lad[x], lh = S.hh[pos];
+      const double2 l2 = make_double2(lp.x + lh.x, lp.y + lh.y);
       gab += 0.5 * (l2.x + l2.y);
       gba += 0.5 * (l2.x - l2.y);
     }
@@ -1884,7 +2035,7 @@ int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const do
 struct PersistLds {
   union {
     struct { Lds L; SliceDiis D; } sl;
-    struct { int4 cls[kMaxNao]; double red[3 * 64 * 8]; } la;
+    struct { double red[3 * 64 * 8]; } la;
     struct { double red[(kElemThreads / 64) * 32]; double joo[kMaxOcc]; } mn;
     struct { double vals[kPartialStride]; double red[(kFinThreads / 64) * 32]; State sst; } fi;
   } u;
@@ -1892,27 +2043,35 @@ struct PersistLds {
 
 __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ solves,
                                                  const int* __restrict__ slice_prefix,
+                                                 const int* __restrict__ hh_prefix,
                                                  const int* __restrict__ lad_prefix,
                                                  const int* __restrict__ con_prefix,
-                                                 const int* __restrict__ elem_prefix, int nslice, int nlad,
-                                                 int ncon, int nelem, int passes) {
+                                                 const int* __restrict__ elem_prefix, int nslice, int nhh,
+                                                 int nlad, int ncon, int nelem, int passes, int tab_doubles,
+                                                 int cls_max) {
   __shared__ PersistLds P;
-  extern __shared__ double pdyn[];  // slice slabs, then the ladder tables
+  extern __shared__ double pdyn[];  // slice slabs, then the ladder tables, a virtual list, W rows
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ncb = (ncon + kItemsPerBlock - 1) / kItemsPerBlock;
+  const LadderLds ll = ladder_lds(pdyn, tab_doubles, cls_max);
   for (int pass = 0; pass < passes; ++pass) {
     for (int it = blockIdx.x; it < nslice; it += gridDim.x) {
       __syncthreads();
       slice_block<256, false>(solves, slice_prefix, 1, it, P.u.sl.L, P.u.sl.D, pdyn, nullptr);
     }
     grid.sync();
-    for (int it = blockIdx.x; it < nlad; it += gridDim.x) {
+    for (int it = blockIdx.x; it < nhh + nlad + ncb; it += gridDim.x) {
       __syncthreads();
-      ladder_block<2>(solves, lad_prefix, 1, it, pdyn, P.u.la.cls, P.u.la.red);
+      if (it < nhh) {
+        hh_block<2, 8>(solves, hh_prefix, 1, it, ll, P.u.la.red);
+      } else if (it < nhh + nlad) {
+        ladder_block<2>(solves, lad_prefix, 1, it - nhh, ll, P.u.la.red);
+      } else {
+        const int item = (it - nhh - nlad) * kItemsPerBlock + wave;
+        if (item < ncon) contract_item<8>(solves, con_prefix, 1, item);
+      }
     }
-    grid.sync();
-    for (int base = blockIdx.x * kHHTilesPerBlock; base < ncon; base += gridDim.x * kHHTilesPerBlock)
-      if (base + wave < ncon) contract_item<8, 8>(solves, con_prefix, 1, base + wave);
     grid.sync();
     for (int it = blockIdx.x; it < nelem; it += gridDim.x) {
       __syncthreads();
@@ -1942,21 +2101,25 @@ int launch_persistent(const BatchDev& b, int dyn_lds, int passes, hipStream_t s)
     grid_lds = dyn_lds;
   }
   const int nslice = b.slice_blocks;
-  const int nlad = b.nlad, ncon = b.ncon, nelem = b.elem_blocks;
+  const int nhh = b.nhh, nlad = b.nlad, ncon = b.ncon, nelem = b.elem_blocks;
+  const int tab_doubles = b.tab_doubles, cls_max = b.cls_max;
   // no more blocks than the largest phase has work items: a grid barrier's
   // cost grows with the blocks that take part
-  const int need = std::max(std::max(nslice, nlad), std::max((ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock, nelem));
+  const int need = std::max(std::max(nslice, nhh + nlad + (ncon + kItemsPerBlock - 1) / kItemsPerBlock), nelem);
   const int g = std::max(1, std::min(grid, need));
   int cls = 0;
   for (int q = 0; q < kSliceClasses; ++q)
     if (b.slice_cls_blocks[q] > 0) cls = q;
   const SolveDev* sd = b.solves;
   const int* sp = b.slice_prefix[cls];
+  const int* hp = b.hh_prefix;
   const int* lp = b.lad_prefix;
   const int* cp = b.con_prefix;
   const int* ep = b.elem_prefix;
-  void* args[] = {(void*)&sd, (void*)&sp, (void*)&lp, (void*)&cp, (void*)&ep, (void*)&nslice,
-                  (void*)&nlad, (void*)&ncon, (void*)&nelem, (void*)&passes};
+  void* args[] = {(void*)&sd,    (void*)&sp,     (void*)&hp,          (void*)&lp,
+                  (void*)&cp,    (void*)&ep,     (void*)&nslice,      (void*)&nhh,
+                  (void*)&nlad,  (void*)&ncon,   (void*)&nelem,       (void*)&passes,
+                  (void*)&tab_doubles, (void*)&cls_max};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_persist), dim3(g), dim3(256), args,
                                     (unsigned)dyn_lds, s) == hipSuccess ? 0 : -1;
 }
@@ -1979,6 +2142,16 @@ static void allow_large_lds() {
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                               kMaxDynLds - (int)sizeof(Lds) - (int)sizeof(SliceDiis) -
                                   (int)sizeof(double) * kMaxN2);
+  const void* fc[] = {
+      reinterpret_cast<const void*>(&k_contract<1, 4, 4>), reinterpret_cast<const void*>(&k_contract<1, 4, 8>),
+      reinterpret_cast<const void*>(&k_contract<1, 8, 4>), reinterpret_cast<const void*>(&k_contract<1, 8, 8>),
+      reinterpret_cast<const void*>(&k_contract<2, 4, 4>), reinterpret_cast<const void*>(&k_contract<2, 4, 8>),
+      reinterpret_cast<const void*>(&k_contract<2, 8, 4>), reinterpret_cast<const void*>(&k_contract<2, 8, 8>),
+      reinterpret_cast<const void*>(&k_contract<4, 4, 4>), reinterpret_cast<const void*>(&k_contract<4, 4, 8>),
+      reinterpret_cast<const void*>(&k_contract<4, 8, 4>), reinterpret_cast<const void*>(&k_contract<4, 8, 8>)};
+  for (const void* f : fc)
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kMaxDynLds - (int)sizeof(double) * 3 * 64 * 8);
   done = true;
 }
 
@@ -2019,26 +2192,30 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
   }
 }
 
-static void launch_ladder(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
-  const auto* sd = (const SolveDev*)b.solves;
-  const auto* lp = (const int*)b.lad_prefix;
-  const uint32_t lds = (uint32_t)b.lad_lds;
-  switch (b.tiles.ladder_group) {
-    case 1: launch_k(k_ladder<1>, stop, dim3(b.nlad), dim3(256), lds, s, sd, lp, b.n); break;
-    case 4: launch_k(k_ladder<4>, stop, dim3(b.nlad), dim3(256), lds, s, sd, lp, b.n); break;
-    default: launch_k(k_ladder<2>, stop, dim3(b.nlad), dim3(256), lds, s, sd, lp, b.n); break;
-  }
+template <int G, int QS, int HS>
+static void launch_contract_t(const BatchDev& b, hipStream_t s, hipEvent_t stop) {
+  const int grid = b.nhh_pad + b.nlad_pad + (b.ncon + kItemsPerBlock - 1) / kItemsPerBlock;
+  launch_k(k_contract<G, QS, HS>, stop, dim3(grid), dim3(256), (uint32_t)b.con_lds, s, (const SolveDev*)b.solves,
+           (const int*)b.hh_prefix, b.nhh, b.nhh_pad, (const int*)b.lad_prefix, b.nlad, b.nlad_pad,
+           (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max);
+}
+
+template <int G>
+static void launch_contract_g(const BatchDev& b, hipStream_t s, hipEvent_t stop) {
+  const int q8 = b.tiles.quad_steps == 4 ? 0 : 1, h8 = b.tiles.hh_steps == 4 ? 0 : 1;
+  if (q8 && h8) launch_contract_t<G, 8, 8>(b, s, stop);
+  else if (q8) launch_contract_t<G, 8, 4>(b, s, stop);
+  else if (h8) launch_contract_t<G, 4, 8>(b, s, stop);
+  else launch_contract_t<G, 4, 4>(b, s, stop);
 }
 
 static void launch_contract(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
-  const int grid = (b.ncon + kHHTilesPerBlock - 1) / kHHTilesPerBlock;
-  const auto* sd = (const SolveDev*)b.solves;
-  const auto* cp = (const int*)b.con_prefix;
-  const int q8 = b.tiles.quad_steps == 4 ? 0 : 1, h8 = b.tiles.hh_steps == 4 ? 0 : 1;
-  if (q8 && h8) launch_k(k_contract<8, 8>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
-  else if (q8) launch_k(k_contract<8, 4>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
-  else if (h8) launch_k(k_contract<4, 8>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
-  else launch_k(k_contract<4, 4>, stop, dim3(grid), dim3(kElemThreads), 0, s, sd, cp, b.n, b.ncon);
+  allow_large_lds();
+  switch (b.tiles.ladder_group) {
+    case 1: launch_contract_g<1>(b, s, stop); break;
+    case 4: launch_contract_g<4>(b, s, stop); break;
+    default: launch_contract_g<2>(b, s, stop); break;
+  }
 }
 
 void launch_init(const BatchDev& b, hipStream_t s, int drpa) {
@@ -2091,8 +2268,7 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s) {
 static int pass_kernels(const BatchDev& b, int main_flags, int* order) {
   int n = 0;
   if (b.slice_blocks > 0) order[n++] = 0;
-  if (b.nlad > 0) order[n++] = 2;
-  if (b.ncon > 0) order[n++] = 1;
+  if (b.nhh + b.nlad + b.ncon > 0) order[n++] = 1;
   order[n++] = 3;
   if (main_flags & MF_ITERATE) order[n++] = 4;
   return n;
@@ -2110,10 +2286,7 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
       if (b.slice_blocks > 0) launch_slice(b, s);
       break;
     case 1:
-      if (b.ncon > 0) launch_contract(b, s);
-      break;
-    case 2:
-      if (b.nlad > 0) launch_ladder(b, s);
+      if (b.nhh + b.nlad + b.ncon > 0) launch_contract(b, s);
       break;
     case 3:
       hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
@@ -2138,7 +2311,6 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
   for (int i = 0; i < n; ++i) {
     switch (order[i]) {
       case 0: launch_slice(b, s, stop[i]); break;
-      case 2: launch_ladder(b, s, stop[i]); break;
       case 1: launch_contract(b, s, stop[i]); break;
       case 3:
         launch_k(k_main, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,

@@ -123,7 +123,6 @@ struct SolveDevT {
   Ptr<const int> kb_va;     // virtual columns a (orbital index) grouped by K
   Ptr<const int> kb_vb;     // their partners b
   Ptr<const int4> kb_nva;   // (n_a, a) of the same list
-  Ptr<const int> kb_woff;   // [nK+1] offsets of the n_oo x n_oo W blocks
   Ptr<const int> kb_toff;   // [nK+1] offsets of the K blocks in tk
   // the valid elements in element order (compact index c; the history layout)
   int nvalid;
@@ -140,8 +139,11 @@ struct SolveDevT {
   Ptr<const int4> qb_info;  // [nqb] (first row, rows, first row of -q, rows of -q)
   Ptr<const int> qrow;      // [v*o] row of pair (i, p) in the grouped order, at (p - o)*o + i
   Ptr<const int> qb_of;     // [v*o] transfer block of pair (i, p), at (p - o)*o + i
-  // work items of the tile launches: (0, K, row0, col0) ladder tiles and
-  // (0, kind, arg, arg) contraction items
+  // work items of the contraction launch: (0, K, row0, 0) hole-hole units,
+  // (0, K, row0, col0) particle-particle ladder tiles, (0, kind, arg, arg)
+  // ring / crossed-ring / Joo items
+  Ptr<const int4> hh_items;
+  int nhh;
   Ptr<const int4> lad_items;
   int nlad;
   Ptr<const int4> con_items;
@@ -156,8 +158,7 @@ struct SolveDevT {
   Ptr<double> gout;         // [2*n3] G of the last pass (when MF_WRITE_GR)
   Ptr<double> rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
   Ptr<double2> lad;         // [n3] particle-particle ladder, (S, P) parts
-  Ptr<double2> w;           // [wtot] hole-hole W intermediates, (S, P) parts
-  int wtot;
+  Ptr<double2> hh;          // [nvalid] hole-hole ladder, (S, P) parts, laid out as tk
   Ptr<double2> jring;       // [o*v rows][o] quadratic ring intermediates (J2, J3), rows
                             // in the grouped P(q) order (qrow)
   Ptr<double2> jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
@@ -176,8 +177,15 @@ struct SolveDevT {
 template <typename T> using HostPtr = T*;
 using SolveDev = SolveDevT<HostPtr>;
 
-// Dynamic LDS of a k_ladder block: the solve's distinct ladder tables.
-inline int ladder_lds_bytes(const SolveDev& S) { return S.lad_ntab * (S.n2max + 1) * 8; }
+// Doubles of the solve's distinct ladder tables (n2max + 1 each), rounded up
+// to keep what follows them in LDS 16-byte aligned.
+inline int ladder_tab_doubles(const SolveDev& S) { return (S.lad_ntab * (S.n2max + 1) + 1) & ~1; }
+// Dynamic LDS of a k_contract block: the ladder tables, a K block's virtual
+// list (at most cls_max entries) and the hole-hole unit's 16 W rows (at most
+// noo_max pairs).
+inline int contract_lds_bytes(int tab_doubles, int cls_max, int noo_max) {
+  return tab_doubles * 8 + cls_max * 16 + 16 * noo_max * 16;
+}
 
 constexpr int kSliceClasses = 3;
 
@@ -197,28 +205,34 @@ struct BatchDev {
   // slice_threads / 256 - 1) over the solves of that size; one block per
   // virtual slice, the other solves have no blocks in the class's prefix
   int* slice_prefix[kSliceClasses];  // [n+1] each
-  int* lad_prefix;      // [n+1] ladder tiles of each solve (one block each)
-  int nlad;
-  int lad_lds;                       // dynamic LDS bytes of k_ladder (its Coulomb tables)
+  // k_contract: hole-hole units, ladder tiles (one block each) and ring /
+  // crossed-ring / Joo items (one wave each) of each solve; the unit and tile
+  // ranges of the launch are padded to multiples of 8 blocks
+  int* hh_prefix;       // [n+1]
+  int nhh, nhh_pad;
+  int* lad_prefix;      // [n+1]
+  int nlad, nlad_pad;
+  int tab_doubles;      // largest ladder_tab_doubles of the batch
+  int cls_max;          // largest K-block virtual list of the batch
+  int con_lds;          // dynamic LDS bytes of k_contract
   int slice_split;                   // some solve's slice roles use different tables
   int elem_blocks;
   int drpa_sum_blocks;               // k_drpa_sums of a batch of one: kDrpaJooParts*o + v + nqb
   int slice_blocks;                  // over all classes
   int slice_cls_blocks[kSliceClasses];
   int slice_lds[kSliceClasses];      // dynamic LDS bytes (largest NOcc of the class)
-  int* con_prefix;      // [n+1] k_contract work items of each solve (one wave each)
+  int* con_prefix;      // [n+1]
   int ncon;
   Tiles tiles;          // the tuned kernels' tile choice for the launch
 };
 
-// k_contract work items, one per wave: a 16x16 tile of the quadratic ring
-// products of one transfer q, a 16x16 hole-hole ladder tile of one K block,
-// the mosaic Joo sums of one solve.
-enum ContractKind : int { CK_RING = 0, CK_HH = 1, CK_JOO = 2, CK_XRING = 3 };
+// k_contract wave items: a 16x16 tile of the quadratic ring (crossed-ring)
+// products of one transfer q, the mosaic Joo sums of a slice range.
+enum ContractKind : int { CK_RING = 0, CK_JOO = 2, CK_XRING = 3 };
 constexpr int kJooParts = 32;  // slice ranges the mosaic Joo sum is split over (one wave each)
 constexpr int kDrpaJooParts = 8;  // slab ranges of the dRPA Joo sum (one wave each)
 static_assert(kDrpaJooParts <= kJooParts, "the dRPA Joo parts fit the joo buffer");
-constexpr int kHHTilesPerBlock = kElemThreads / 64;  // items (waves) per block
+constexpr int kItemsPerBlock = kElemThreads / 64;  // k_contract wave items per block
 
 // Dynamic LDS of a k_slice block: T^ab, T^ba, V(k,j) slabs and four o-vectors;
 // with split tables also the crossed-ring V(j,k) and two more o-vectors.
@@ -269,7 +283,8 @@ int first_pass_kernel(const BatchDev& b, int main_flags);
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64 = 1);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
-// One kernel of the pass: 0 slice, 1 contract, 2 ladder, 3 main.
+// One kernel of the pass: 0 slice, 1 contract, 3 main (2 is no longer a
+// kernel of its own: the ladder tiles run in k_contract).
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
 // Clears `bytes` (a multiple of 8) of device memory at p with a fill kernel

@@ -3,11 +3,11 @@
     python -m uegccd_amd.tune [--reps 40] [--out uegccd_amd/tuned_tiles.json]
 
 For every representative system (N 14/38/54, cutoffs 5/9/16/25, the Gamma
-point and a twist) it solves the system once, then times the ladder kernel
-for each k-step group (1, 2, 4) and the contraction kernel for each pair of
-k-steps per round of loads (ring tiles 4/8, hole-hole tiles 4/8) with
-uegccd_kernel_time, in FP64 and (ladder only) in the FP32 phase of mixed
-precision.  Systems are grouped into classes by their largest pair-momentum
+point and a twist) it solves the system once, then times the contraction
+kernel (ladder tiles, hole-hole units, ring items) with uegccd_kernel_time
+for each ladder k-step group (1, 2, 4), in FP64 and in the FP32 phase of
+mixed precision, and for each pair of k-steps per round of loads (ring
+tiles 4/8, hole-hole tiles 4/8) in FP64.  Systems are grouped into classes by their largest pair-momentum
 block (pairs <= 8/16/32/64) and its virtual list (<= 64/128/256/512/1024);
 each class takes the choice with the lowest time summed over its systems,
 each relative to its own default time.  The table is written as JSON; the
@@ -70,7 +70,7 @@ def sweep_system(lib, nel, cut, rs, tw, reps):
         lad = {}
         for g in LADDER_GROUPS:
             set_tiles(lib, ctx, g, 8, 8)
-            lad[str(g)] = kernel_us(lib, ctx, 2, reps)
+            lad[str(g)] = kernel_us(lib, ctx, 1, reps)
         out[f"ladder_us_{prec}"] = lad
         if prec == "fp64":
             con = {}
