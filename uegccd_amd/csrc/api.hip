@@ -547,6 +547,10 @@ static size_t jring_doubles(const uegccd_ctx* c) {
 // Device bytes of a context: the arena ctx_create carves its buffers from.
 // Blocks of a k_contract range, padded to a multiple of the 8 XCDs.
 static int pad8(int n) { return (n + 7) & ~7; }
+// Partials width k_main needs for a solve: 4 + its DIIS dots, 16 or 32.
+static int main_nq_of(const SolveDev& S) {
+  return (S.mode == UEGCCD_MODE_DIIS && S.ndiis > 12) ? 32 : 16;
+}
 
 size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   const size_t n3 = c->n3;
@@ -849,6 +853,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.cls_max = std::max(1, c->nvv_max);
   b.con_lds = contract_lds_bytes(b.tab_doubles, b.cls_max, c->noo_max);
   b.tiles = c->tiles;
+  b.main_nq = main_nq_of(S);
   b.ncon = S.ncon;
   // work buffers
   c->arena.used = aligned(c->arena.used);
@@ -1293,6 +1298,8 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     b.tiles = cs[big]->tiles;
   }
   b.slice_split = slice_split;
+  b.main_nq = 16;
+  for (int k = 0; k < n; ++k) b.main_nq = std::max(b.main_nq, main_nq_of(cs[k]->sd));
   b.con_prefix = const_cast<int*>(im.put(cp));
   b.ncon = cp[n];
   out->states = a.take<State>(n);

@@ -1239,8 +1239,9 @@ __device__ void contract_item(const SolveDev* __restrict__ solves, const int* __
 //   the rest                 four ring / crossed-ring / Joo items per block
 // The pads are multiples of 8, so a block's XCD is still blockIdx mod 8 inside
 // each range, which the ladder tiles and items renumber XCD-contiguously.
+// (held to 128 VGPRs, four waves per SIMD: its tiles wait on gathers)
 template <int G, int QS, int HS>
-__global__ __launch_bounds__(256) void k_contract(const SolveDev* __restrict__ solves,
+__global__ __launch_bounds__(256, 4) void k_contract(const SolveDev* __restrict__ solves,
                                                   const int* __restrict__ hh_prefix, int nhh, int nhh_pad,
                                                   const int* __restrict__ lad_prefix, int nlad, int nlad_pad,
                                                   const int* __restrict__ con_prefix, int ncon, int n,
@@ -1305,6 +1306,9 @@ __device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, i
 // the damped update, dT, the energy and the DIIS dot products.
 // One 256-thread element block gb of the launch (block index after the XCD
 // renumbering); red and joo_l are the block's LDS.
+// NQ: the partials' width (16 serves up to 12 DIIS entries, 32 up to 28),
+// or 0 to choose it per solve.
+template <int NQ>
 __device__ void main_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
                            int gb, int flags, double* red, double* joo_l) {
   const int s = find_solve(prefix, n, gb);
@@ -1422,18 +1426,21 @@ __device__ void main_block(const SolveDev* __restrict__ solves, const int* __res
   // block partials: max dT, max |R|, sum E, non-finite count, DIIS dots
   const int nd = (S.mode == UEGCCD_MODE_DIIS) ? nh : 0;
   const double2* HE = S.histE + (lead ? cp : 0);
-  if (nd <= 12)
+  if (NQ == 16 || (NQ == 0 && nd <= 12))
     main_partials<16>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, S.nvalid, red);
   else
     main_partials<32>(S, lb, nd, slot, lead, dt_part, r_part, e_part, bad, eab, eba, HE, S.nvalid, red);
 }
 
-__global__ __launch_bounds__(kElemThreads) void k_main(const SolveDev* __restrict__ solves,
-                                                        const int* __restrict__ prefix, int n,
-                                                        int flags) {
+// (NQ = 16, the usual DIIS length, is held to 128 VGPRs: four waves per SIMD
+// to hide the element gathers)
+template <int NQ>
+__global__ __launch_bounds__(kElemThreads, NQ == 16 ? 4 : 1) void k_main(const SolveDev* __restrict__ solves,
+                                                                        const int* __restrict__ prefix, int n,
+                                                                        int flags) {
   __shared__ double red[(kElemThreads / 64) * 32];
   __shared__ double joo_l[kMaxOcc];
-  main_block(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), flags, red, joo_l);
+  main_block<NQ>(solves, prefix, n, xcd_block(blockIdx.x, gridDim.x), flags, red, joo_l);
 }
 
 // Sums and maxima of one solve's element-block partials by one block, in a
@@ -2075,7 +2082,7 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
     grid.sync();
     for (int it = blockIdx.x; it < nelem; it += gridDim.x) {
       __syncthreads();
-      main_block(solves, elem_prefix, 1, it, MF_ITERATE, P.u.mn.red, P.u.mn.joo);
+      main_block<0>(solves, elem_prefix, 1, it, MF_ITERATE, P.u.mn.red, P.u.mn.joo);
     }
     grid.sync();
     if (blockIdx.x == 0) finalize_block(solves, 0, FIN_ITER, 1, P.u.fi.vals, P.u.fi.red, P.u.fi.sst);
@@ -2289,8 +2296,12 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
       if (b.nhh + b.nlad + b.ncon > 0) launch_contract(b, s);
       break;
     case 3:
-      hipLaunchKernelGGL(k_main, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
-                         b.elem_prefix, b.n, (int)MF_ITERATE);
+      if (b.main_nq == 16)
+        hipLaunchKernelGGL(k_main<16>, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
+                           b.elem_prefix, b.n, (int)MF_ITERATE);
+      else
+        hipLaunchKernelGGL(k_main<32>, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
+                           b.elem_prefix, b.n, (int)MF_ITERATE);
       break;
     default:
       break;
@@ -2313,8 +2324,12 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
       case 0: launch_slice(b, s, stop[i]); break;
       case 1: launch_contract(b, s, stop[i]); break;
       case 3:
-        launch_k(k_main, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
-                 (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
+        if (b.main_nq == 16)
+          launch_k(k_main<16>, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
+                   (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
+        else
+          launch_k(k_main<32>, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
+                   (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
         break;
       case 4:
         launch_k(k_finalize, stop[i], dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,

@@ -224,6 +224,7 @@ struct BatchDev {
   int* con_prefix;      // [n+1]
   int ncon;
   Tiles tiles;          // the tuned kernels' tile choice for the launch
+  int main_nq;          // k_main's partials width: 16 (DIIS lengths up to 12) or 32
 };
 
 // k_contract wave items: a 16x16 tile of the quadratic ring (crossed-ring)
