@@ -35,7 +35,7 @@ METRIC = "CCD solve time & twist-solves/sec at |ΔE|<1e-6 Ha vs FP64; p50/p99 la
 WORKLOAD = "n54_cut16"
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
-KERNELS = {0: "k_slice", 1: "k_contract", 3: "k_main", 4: "k_finalize"}
+KERNELS = {0: "k_slice", 1: "k_contract", 3: "k_main"}
 def algo_work(kernel, o, v, nvalid, ndiis, spp, shh):
     """Algorithmic (flops, bytes) of one launch in the terms of the reference's
     loops (source/CCD.f90): valid elements only, each compulsory datum moved

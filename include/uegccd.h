@@ -209,9 +209,9 @@ typedef struct {
 int uegccd_set_tiles(uegccd_ctx* ctx, const uegccd_tiles* tiles);
 int uegccd_get_tiles(const uegccd_ctx* ctx, uegccd_tiles* tiles, int32_t* noo_max, int32_t* nvv_max);
 
-/* In-solve timing of one kernel (numbering as uegccd_kernel_time, plus 4 for
- * the per-solve reductions, convergence test and DIIS solve; -1 turns it
- * off): in every pass of the context's later solves, HIP stop events on the
+/* In-solve timing of one kernel (numbering as uegccd_kernel_time; -1 turns
+ * it off; the per-solve reductions, convergence test and DIIS bookkeeping of
+ * a pass run at the start of the next pass's kernel 0): in every pass of the context's later solves, HIP stop events on the
  * context's stream are written by that kernel's dispatch and by the dispatch
  * before it (kernels of a stream run back to back, so the difference is the
  * kernel's duration as a kernel trace shows it; when it is the first kernel

@@ -524,7 +524,8 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   // LDS of the slice blocks (one table; two with split slice roles)
   const bool split = slice_split_of(c->rslot);
   if (slice_lds_bytes(o, split) + (int)sizeof(int4) * kMaxNao +
-      (int)sizeof(double) * ((split ? 2 : 1) * kMaxN2 + 16 + kMaxDiis * kMaxDiis + kMaxDiis) > kMaxDynLds)
+      (int)sizeof(double) * ((split ? 2 : 1) * kMaxN2 + 16 + kMaxDiis * kMaxDiis + kMaxDiis) +
+      slice_fin_lds(slice_threads(o)) > kMaxDynLds)
     return UEGCCD_ERR_UNSUPPORTED;
   c->eig.assign(s.eigen, s.eigen + nao);
   return UEGCCD_OK;
@@ -577,7 +578,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double) * c->v) + aligned(sizeof(double) * c->v * c->o);
   bytes += aligned(sizeof(double) * c->o * kJooParts);
   bytes += aligned(sizeof(double) * eb * kPartialStride);
-  bytes += aligned(sizeof(State)) + aligned(sizeof(SolveDev));
+  bytes += 2 * aligned(sizeof(State)) + aligned(sizeof(SolveDev));
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
   bytes += aligned(sizeof(int) * (8 + 2 * kSliceClasses)) +
            aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size())) +
@@ -823,8 +824,9 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.con_items = im.put(c->con_items);
   // batch of one: block prefixes of the element, slice, ladder and contraction launches
   BatchDev& b = c->batch;
-  const bool need_slice = sys->do_mosaic || sys->do_ring || sys->do_xring;
-  const int sb = need_slice ? c->v : 0;
+  // (every pass needs its slice blocks: they build the pass's amplitudes
+  // from the history, whichever terms are on)
+  const int sb = c->v;
   const int cls = slice_threads(c->o) / 256 - 1;
   int h_pref[8 + 2 * kSliceClasses] = {0, eb, 0, S.nlad, 0, S.ncon, 0, S.nhh};
   h_pref[8 + 2 * cls + 1] = sb;
@@ -879,6 +881,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.partials = c->arena.take<double>((size_t)eb * kPartialStride /* slot-major: [q][block] */);
   c->d_state = c->arena.take<State>(1);
   S.st = c->d_state;
+  S.st2 = c->arena.take<State>(1);
   S.trace = c->arena.take<double>(2 * (size_t)c->maxiter);
   im.write(c->d_sd, &S, 1);
 }
@@ -1245,10 +1248,9 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   for (int k = 0; k < n; ++k) {
     uegccd_ctx* c = cs[k];
     ep[k + 1] = ep[k] + elem_blocks_of(c);
-    const bool need_slice = c->sd.do_mosaic || c->sd.do_ring || c->sd.do_xring;
     const int cls = slice_threads(c->o) / 256 - 1;
-    for (int q = 0; q < kSliceClasses; ++q) sp[q][k + 1] = sp[q][k] + (need_slice && q == cls ? c->v : 0);
-    if (need_slice) slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o, slice_split));
+    for (int q = 0; q < kSliceClasses; ++q) sp[q][k + 1] = sp[q][k] + (q == cls ? c->v : 0);
+    slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o, slice_split));
     lp[k + 1] = lp[k] + c->sd.nlad;
     hp[k + 1] = hp[k] + c->sd.nhh;
     tab_doubles = std::max(tab_doubles, ladder_tab_doubles(c->sd));
@@ -1349,7 +1351,8 @@ int chunk_graph(uegccd_ctx* c, int len, hipGraphExec_t* out) {
   prepare_launches();  // per-kernel attributes are set outside the capture
   hipStream_t s = c->stream;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < len; ++k) launch_iteration(c->batch, MF_ITERATE, s);
+  for (int k = 0; k < len; ++k) launch_iteration(c->batch, MF_ITERATE, s, nullptr, nullptr, k > 0);
+  launch_finalize(c->batch, FIN_ITER, s);
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s, &g);
   if (ec != hipSuccess || !g) {
@@ -1444,15 +1447,18 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
       if (rc) return rc;
       HIPCHK(hipGraphLaunch(ex, s));
     }
+    // (each pass's k_slice runs the reductions of the pass before it; the
+    // chunk's last pass has a finalize launch of its own)
     for (int k = 0; k < chunk && !graphs && !persist; ++k) {
       if (timing) {
         const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
         const hipEvent_t tail = (timed_first && k + 1 < chunk) ? c0->tev[2 * (k + 1)] : nullptr;
-        launch_iteration(b, MF_ITERATE, s, &t, tail);
+        launch_iteration(b, MF_ITERATE, s, &t, tail, k > 0);
       } else {
-        launch_iteration(b, MF_ITERATE, s);
+        launch_iteration(b, MF_ITERATE, s, nullptr, nullptr, k > 0);
       }
     }
+    if (!graphs && !persist) launch_finalize(b, FIN_ITER, s);
     passes += chunk;
     HIPCHK(hipGetLastError());
     if (gather) {
@@ -1685,7 +1691,7 @@ extern "C" int uegccd_get_amplitudes(uegccd_ctx* c, double* taa, double* tab, do
 }
 
 extern "C" int uegccd_time_kernel(uegccd_ctx* c, int32_t which) {
-  if (!c || which < -1 || which > 4 || which == 2) return UEGCCD_ERR_INVALID;
+  if (!c || which < -1 || which > 3 || which == 2) return UEGCCD_ERR_INVALID;
   c->timed_kernel = which;
   c->timed_us = 0.0;
   c->timed_launches = 0;
@@ -2357,7 +2363,9 @@ int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
       s->batch_stale = false;
       s->nbatch = cs.size();
     }
-    for (int k = 0; k < s->opt.iters_per_step; ++k) launch_iteration(s->batch.b, MF_ITERATE, s->stream);
+    for (int k = 0; k < s->opt.iters_per_step; ++k)
+      launch_iteration(s->batch.b, MF_ITERATE, s->stream, nullptr, nullptr, k > 0);
+    launch_finalize(s->batch.b, FIN_ITER, s->stream);
     launch_gather_states(s->batch.b, s->batch.states, s->stream);
     HIPCHK(hipMemcpyAsync(s->h_states, s->batch.states, sizeof(State) * s->nbatch,
                           hipMemcpyDeviceToHost, s->stream));

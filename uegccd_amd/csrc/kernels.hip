@@ -57,6 +57,9 @@ __device__ __forceinline__ void st2(GlobalPtr<double2> p, double2 v) { *p = v; }
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+constexpr int kStateWords = sizeof(State) / sizeof(double);
+static_assert(kStateWords <= 2 * kFinThreads, "two state words per thread");
+
 __device__ __forceinline__ int find_solve(const int* __restrict__ prefix, int n, int blk) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
@@ -512,9 +515,18 @@ __device__ __forceinline__ void slab_place(const SolveG& S, double* Tab, double*
 // (range flags): a second V slab Vx for the crossed-ring products, v(n_s -
 // n_k) of the quadratic ring and crossed-ring directs (vsr, vsx) beside the
 // mosaic's (vs), and a second Y table vt2 for the crossed rings.
+// fin_prev: the block first runs the reductions and state update of the
+// previous pass (finalize_state, the same in every block; the block of slice
+// 0 writes the new state to st2), with fvals, fred ((BT/64)*32 doubles) and
+// fsst as its LDS.
+__device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int prec64, double* vals, double* red,
+                                              State& sst, bool write_trace);
+__device__ __forceinline__ void put_state(const State& sst, GlobalPtr<State> to);
+
 template <int BT, bool SPLIT>
 __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                            int n, int bid, Lds& L, SliceDiis& D, double* dyn, double* vt2) {
+                            int n, int bid, Lds& L, SliceDiis& D, double* dyn, double* vt2, int fin_prev,
+                            double* fvals, double* fred, State* fsst) {
   const int s = find_solve(prefix, n, bid);
   const SolveG& S = as_global(solves)[s];
   const int sv = bid - prefix[s];
@@ -544,9 +556,6 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   // history (up to kPre slots), the occupied Coulomb block V(k, j), the state
   // and its B matrix go out in the same round of loads as the tables.
   constexpr int kPre = 8;
-  const State& st = *S.st;
-  const int phase = st.phase, src = st.next_src, nh = st.nhist, slot_cur = st.slot_cur;
-  const double bw = threadIdx.x < kMaxDiis * kMaxDiis ? st.bmat[threadIdx.x] : 0.0;
   const int ndiis = S.ndiis;
   const size_t nval = S.nvalid;
   const int xs = o2 * sv;  // the slab's first element
@@ -560,6 +569,26 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   for (int k = 0; k < kPre; ++k)
     h[k] = (t0 < nvs && k < ndiis) ? ld(HC + (size_t)k * nval) : make_double2(0.0, 0.0);
   const double vsm = t0 < o ? vso_m[sv * o + t0] : 0.0;
+  // the state of this pass (requested after the history, which does not
+  // depend on it)
+  int phase, src, nh, slot_cur;
+  double bw;
+  if (fin_prev) {
+    finalize_state(S, FIN_ITER, 1, fvals, fred, *fsst, sv == 0);
+    if (sv == 0) put_state(*fsst, S.st2);
+    phase = fsst->phase;
+    src = fsst->next_src;
+    nh = fsst->nhist;
+    slot_cur = fsst->slot_cur;
+    bw = t0 < kMaxDiis * kMaxDiis ? fsst->bmat[t0] : 0.0;
+  } else {
+    const State& st = *S.st;
+    phase = st.phase;
+    src = st.next_src;
+    nh = st.nhist;
+    slot_cur = st.slot_cur;
+    bw = t0 < kMaxDiis * kMaxDiis ? st.bmat[t0] : 0.0;
+  }
   if (t0 < kMaxDiis * kMaxDiis) D.bm[t0] = bw;
   if (t0 < kMaxDiis) D.coef[t0] = t0 == slot_cur ? 1.0 : 0.0;
   for (int idx = t0; idx < o2; idx += BT) {
@@ -1050,7 +1079,7 @@ __device__ __forceinline__ LadderLds ladder_lds(double* dyn, int tab_doubles, in
 // Stages the solve's ladder tables and block K's virtual list in LDS; returns
 // the solve's state fields the block needs (phase, precision).
 __device__ __forceinline__ int2 stage_ladder(const SolveG& S, int K, const LadderLds& l) {
-  const int phase = S.st->phase, prec64 = S.st->prec64;  // requested with the tables, tested after them
+  const int phase = S.st2->phase, prec64 = S.st2->prec64;  // requested with the tables, tested after them
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int nt = S.n2max + 1;
   for (int m = 0; m < S.lad_ntab; ++m) {
@@ -1092,12 +1121,15 @@ __device__ void hh_block(const SolveDev* __restrict__ solves, const int* __restr
 
 template <int BT, bool SPLIT>
 __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solves,
-                                              const int* __restrict__ prefix, int n) {
+                                              const int* __restrict__ prefix, int n, int fin_prev) {
   __shared__ Lds L;
   __shared__ SliceDiis D;
   __shared__ double vt2[SPLIT ? kMaxN2 : 1];
+  __shared__ double fvals[kPartialStride];
+  __shared__ double fred[(BT / 64) * 32];
+  __shared__ State fsst;
   extern __shared__ double dyn[];
-  slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2);
+  slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2, fin_prev, fvals, fred, &fsst);
 }
 
 // --------------------------------------------------------------------------
@@ -1316,8 +1348,13 @@ __device__ void main_block(const SolveDev* __restrict__ solves, const int* __res
   // the state, the element (one valid element per thread, by compact index
   // cp, which is also its history position) and the Joo parts are requested
   // together; a finished solve leaves after they arrive
-  const int phase = S.st->phase, slot = S.st->slot;
+  const int phase = S.st2->phase, slot = S.st2->slot;
   const int lb = gb - prefix[s];
+  if (lb == 0) {  // the pass's state becomes the state between passes
+    const auto from = reinterpret_cast<GlobalPtr<const double>>(S.st2);
+    const auto to = reinterpret_cast<GlobalPtr<double>>(S.st);
+    for (int w = threadIdx.x; w < kStateWords; w += blockDim.x) to[w] = from[w];
+  }
   const int o = S.o, o2 = o * o, n3 = S.n3;
   const int cp = lb * kElemPerBlock + threadIdx.x;
   int x = 0, b = -1, pos = 0;
@@ -1446,19 +1483,21 @@ __global__ __launch_bounds__(kElemThreads, NQ == 16 ? 4 : 1) void k_main(const S
 // Sums and maxima of one solve's element-block partials by one block, in a
 // fixed order.  Slot-major partials: quantity q of every element block is
 // contiguous; q = 1 (max dT) and q = 2 (max |R|) are maxima, E (0), the
-// non-finite count (3) and the nd DIIS dots (4..) sums.  Each thread takes
-// blocks t, t + blockDim, ... and loads every quantity of a block at once, so
-// the loads of a round are in flight together (the partials were written on
-// other XCDs and miss this L2).  vals[0..3+nd] (E, max dT, max |R|,
-// non-finite count, dots) are written by threads 0..3+nd: valid after a
-// barrier.  red holds (blockDim/64)*32 doubles.
+// non-finite count (3) and the nd DIIS dots (4..) sums.  Each of the first
+// kFinThreads threads takes blocks t, t + kFinThreads, ... (further threads
+// add zeros, so a block of any size of at least kFinThreads sums in the same
+// order) and loads every quantity of a block at once, so the loads of a
+// round are in flight together (the partials were written on other XCDs and
+// miss this L2).  vals[0..3+nd] (E, max dT, max |R|, non-finite count, dots)
+// are written by threads 0..3+nd: valid after a barrier.  red holds
+// (blockDim/64)*32 doubles.
 template <int NQ>
-__device__ void reduce_partials_n(const SolveG& S, int nd, double* vals, double* red) {
+__device__ __forceinline__ void reduce_partials_n(const SolveG& S, int nd, double* vals, double* red) {
   const int nq = 4 + nd, nb = S.nblk_elem;
   double acc[NQ];  // dT, max|R|, E, non-finite, dots
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+  for (int k = threadIdx.x; k < nb && threadIdx.x < kFinThreads; k += kFinThreads) {
     double p[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -1472,7 +1511,7 @@ __device__ void reduce_partials_n(const SolveG& S, int nd, double* vals, double*
   const int q = threadIdx.x;
   if (q < nq) vals[q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q] = r;
 }
-__device__ void reduce_partials(const SolveG& S, int nd, double* vals, double* red) {
+__device__ __forceinline__ void reduce_partials(const SolveG& S, int nd, double* vals, double* red) {
   if (nd <= 12)
     reduce_partials_n<16>(S, nd, vals, red);
   else
@@ -1483,7 +1522,8 @@ __device__ void reduce_partials(const SolveG& S, int nd, double* vals, double* r
 // the state copy in LDS, by one thread: NIter, E, dT, convergence, the mixed
 // precision switch, the DIIS history and its B matrix row, the trace line.
 // Returns whether the next amplitudes are to be extrapolated.
-__device__ bool advance_state(const SolveG& S, State* st, const double* vals, int phase0) {
+__device__ __forceinline__ bool advance_state(const SolveG& S, State* st, const double* vals, int phase0,
+                                              bool write_trace) {
   const double e = vals[0], dt = vals[1], rmax = vals[2];
   if (vals[3] > 0.0) st->nonfinite = 1;
   if (phase0 == PH_FINAL) {
@@ -1496,7 +1536,7 @@ __device__ bool advance_state(const SolveG& S, State* st, const double* vals, in
   st->ecorr = e;
   st->dt = dt;
   st->maxres_in = rmax;
-  if (S.trace && st->niter <= S.maxiter) {
+  if (write_trace && S.trace && st->niter <= S.maxiter) {
     S.trace[2 * (st->niter - 1)] = e;
     S.trace[2 * (st->niter - 1) + 1] = dt;
   }
@@ -1536,38 +1576,38 @@ __device__ bool advance_state(const SolveG& S, State* st, const double* vals, in
   return want;
 }
 
-// Reductions of a pass and the state that follows, one block per solve:
+// Reductions of a pass and the state that follows:
 // FIN_ITER after a residual pass (bookkeeping and DIIS solve), FIN_INIT after
 // the first pass (E and dT of the starting amplitudes, source/CCD.f90:62-64),
 // FIN_SUM after a plain energy evaluation.  The iteration state is worked on
 // in LDS: one parallel copy in, one out, instead of a chain of dependent
 // global accesses by a single thread.
-// The reductions and state update of solve sidx by one kFinThreads block;
-// vals, red and sst are the block's LDS.
-__device__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, int kind, int prec64,
-                               double* vals, double* red, State& sst) {
-  constexpr int kStateWords = sizeof(State) / sizeof(double);
-  static_assert(kStateWords <= 2 * kFinThreads, "two state words per thread");
-  const SolveG& S = as_global(solves)[sidx];
+// finalize_state leaves the new state of solve S in sst (LDS) for a block of
+// kFinThreads threads or more (vals and red are its LDS); it returns the
+// phase the state had (a FIN_ITER on a finished solve changes nothing).
+// write_trace: this block writes the trace line (one block per solve does).
+__device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int prec64, double* vals, double* red,
+                                              State& sst, bool write_trace) {
   // One round of loads: each thread's state words go out with its partials
   // (the DIIS dots are read whatever the phase; a finishing pass ignores them).
   const double* src = reinterpret_cast<const double*>(S.st);
   const int w0 = threadIdx.x, w1 = threadIdx.x + kFinThreads;
+  const bool two = threadIdx.x < kFinThreads;
   const double sw0 = w0 < kStateWords ? src[w0] : 0.0;
-  const double sw1 = w1 < kStateWords ? src[w1] : 0.0;
+  const double sw1 = two && w1 < kStateWords ? src[w1] : 0.0;
   const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS) ? S.ndiis : 0;
   reduce_partials(S, nd, vals, red);
   double* dst = reinterpret_cast<double*>(&sst);
   if (w0 < kStateWords) dst[w0] = sw0;
-  if (w1 < kStateWords) dst[w1] = sw1;
+  if (two && w1 < kStateWords) dst[w1] = sw1;
   __syncthreads();
   const int phase0 = sst.phase;
-  if (kind == FIN_ITER && phase0 == PH_DONE) return;  // the whole block
+  if (kind == FIN_ITER && phase0 == PH_DONE) return phase0;  // the whole block
   if (threadIdx.x == 0) {
     State* st = &sst;
     const double e = vals[0], dt = vals[1];
     if (kind == FIN_ITER) {
-      st->next_src = advance_state(S, st, vals, phase0) ? SRC_DIIS : SRC_PLAIN;
+      st->next_src = advance_state(S, st, vals, phase0, write_trace) ? SRC_DIIS : SRC_PLAIN;
     } else {
       if (kind == FIN_INIT) {  // a fresh state: nothing carries over from an earlier solve
         st->nonfinite = 0;
@@ -1587,7 +1627,7 @@ __device__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, in
         st->nhist = 0;
         st->next_src = SRC_TCUR;
         st->converged = 0;
-        if (S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
+        if (write_trace && S.trace && S.maxiter > 0) { S.trace[0] = e; S.trace[1] = dt; }
         const bool go = S.mode == UEGCCD_MODE_REFERENCE ? (dt >= S.tol_dt) : true;
         st->phase = (go && st->niter < S.maxiter && !st->nonfinite) ? PH_ITER : PH_FINAL;
         if (!go) st->converged = 1;
@@ -1595,9 +1635,25 @@ __device__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, in
     }
   }
   __syncthreads();
-  double* out = reinterpret_cast<double*>(S.st);
-  if (w0 < kStateWords) out[w0] = dst[w0];
-  if (w1 < kStateWords) out[w1] = dst[w1];
+  return phase0;
+}
+
+// Writes an LDS state to a device state (each thread its words).
+__device__ __forceinline__ void put_state(const State& sst, GlobalPtr<State> to) {
+  const double* src = reinterpret_cast<const double*>(&sst);
+  auto out = reinterpret_cast<GlobalPtr<double>>(to);
+  for (int w = threadIdx.x; w < kStateWords; w += blockDim.x) out[w] = src[w];
+}
+
+// The reductions and state update of solve sidx by one kFinThreads block:
+// the new state goes to st and to the pass copy st2.
+__device__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, int kind, int prec64,
+                               double* vals, double* red, State& sst) {
+  const SolveG& S = as_global(solves)[sidx];
+  const int phase0 = finalize_state(S, kind, prec64, vals, red, sst, true);
+  if (kind == FIN_ITER && phase0 == PH_DONE) return;
+  put_state(sst, S.st);
+  put_state(sst, S.st2);
 }
 
 __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __restrict__ solves, int kind,
@@ -1610,8 +1666,8 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const SolveDev* __rest
 
 __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, int prec64) {
   const SolveG& S = as_global(solves)[blockIdx.x];
-  if (threadIdx.x != 0) return;
-  const auto st = S.st;
+  if (threadIdx.x > 1) return;
+  const auto st = threadIdx.x == 0 ? S.st : S.st2;  // both copies
   st->niter = 0;
   st->phase = phase;
   st->slot = 0;
@@ -1627,7 +1683,6 @@ __global__ void k_state_reset(const SolveDev* __restrict__ solves, int phase, in
 
 // Copies every solve's state into one contiguous buffer (one host copy per poll).
 __global__ void k_gather_states(const SolveDev* __restrict__ solves, State* __restrict__ out) {
-  constexpr int kStateWords = sizeof(State) / sizeof(double);
   const double* src = reinterpret_cast<const double*>(solves[blockIdx.x].st);
   double* dst = reinterpret_cast<double*>(out + blockIdx.x);
   for (int k = threadIdx.x; k < kStateWords; k += blockDim.x) dst[k] = src[k];
@@ -2065,7 +2120,8 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
   for (int pass = 0; pass < passes; ++pass) {
     for (int it = blockIdx.x; it < nslice; it += gridDim.x) {
       __syncthreads();
-      slice_block<256, false>(solves, slice_prefix, 1, it, P.u.sl.L, P.u.sl.D, pdyn, nullptr);
+      slice_block<256, false>(solves, slice_prefix, 1, it, P.u.sl.L, P.u.sl.D, pdyn, nullptr, 0, nullptr,
+                              nullptr, nullptr);
     }
     grid.sync();
     for (int it = blockIdx.x; it < nhh + nlad + ncb; it += gridDim.x) {
@@ -2133,32 +2189,31 @@ int launch_persistent(const BatchDev& b, int dyn_lds, int passes, hipStream_t s)
 
 // --------------------------------------------------------------------------
 // Dynamic LDS above the 64 KiB default needs an explicit per-kernel limit.
+// The limit of each kernel is the LDS of a CU less the kernel's own static
+// LDS (as the runtime reports it).
+static bool allow_dyn_lds(const void* f) {
+  hipFuncAttributes a;
+  if (hipFuncGetAttributes(&a, f) != hipSuccess) return false;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynLds - (int)a.sharedSizeBytes) ==
+         hipSuccess;
+}
+
 static void allow_large_lds() {
   static bool done = false;
   if (done) return;
-  const void* fs[] = {reinterpret_cast<const void*>(&k_slice<256, false>),
-                      reinterpret_cast<const void*>(&k_slice<512, false>),
-                      reinterpret_cast<const void*>(&k_slice<768, false>)};
-  for (const void* f : fs)
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              kMaxDynLds - (int)sizeof(Lds) - (int)sizeof(SliceDiis));
-  const void* fsplit[] = {reinterpret_cast<const void*>(&k_slice<256, true>),
-                          reinterpret_cast<const void*>(&k_slice<512, true>),
-                          reinterpret_cast<const void*>(&k_slice<768, true>)};
-  for (const void* f : fsplit)
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              kMaxDynLds - (int)sizeof(Lds) - (int)sizeof(SliceDiis) -
-                                  (int)sizeof(double) * kMaxN2);
-  const void* fc[] = {
+  const void* fs[] = {
+      reinterpret_cast<const void*>(&k_slice<256, false>), reinterpret_cast<const void*>(&k_slice<512, false>),
+      reinterpret_cast<const void*>(&k_slice<768, false>), reinterpret_cast<const void*>(&k_slice<256, true>),
+      reinterpret_cast<const void*>(&k_slice<512, true>),  reinterpret_cast<const void*>(&k_slice<768, true>),
       reinterpret_cast<const void*>(&k_contract<1, 4, 4>), reinterpret_cast<const void*>(&k_contract<1, 4, 8>),
       reinterpret_cast<const void*>(&k_contract<1, 8, 4>), reinterpret_cast<const void*>(&k_contract<1, 8, 8>),
       reinterpret_cast<const void*>(&k_contract<2, 4, 4>), reinterpret_cast<const void*>(&k_contract<2, 4, 8>),
       reinterpret_cast<const void*>(&k_contract<2, 8, 4>), reinterpret_cast<const void*>(&k_contract<2, 8, 8>),
       reinterpret_cast<const void*>(&k_contract<4, 4, 4>), reinterpret_cast<const void*>(&k_contract<4, 4, 8>),
       reinterpret_cast<const void*>(&k_contract<4, 8, 4>), reinterpret_cast<const void*>(&k_contract<4, 8, 8>)};
-  for (const void* f : fc)
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              kMaxDynLds - (int)sizeof(double) * 3 * 64 * 8);
+  bool refused = false;
+  for (const void* f : fs) refused |= !allow_dyn_lds(f);
+  if (refused) (void)hipGetLastError();  // a refused limit leaves the default, and no error behind
   done = true;
 }
 
@@ -2176,7 +2231,7 @@ static void launch_k(F kernel, hipEvent_t stop, dim3 grid, dim3 block, uint32_t 
 
 void prepare_launches() { allow_large_lds(); }
 
-static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr) {
+static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr, int fin_prev = 0) {
   allow_large_lds();
   const auto* sd = (const SolveDev*)b.solves;
   int last = -1;  // the stop event goes with the last launch
@@ -2188,13 +2243,13 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
     const auto* pf = (const int*)b.slice_prefix[q];
     const hipEvent_t ev = q == last ? stop : nullptr;
     if (b.slice_split) {
-      if (q == 0) launch_k(k_slice<256, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
-      else if (q == 1) launch_k(k_slice<512, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
-      else launch_k(k_slice<768, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+      if (q == 0) launch_k(k_slice<256, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
+      else if (q == 1) launch_k(k_slice<512, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
+      else launch_k(k_slice<768, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
     } else {
-      if (q == 0) launch_k(k_slice<256, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
-      else if (q == 1) launch_k(k_slice<512, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
-      else launch_k(k_slice<768, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n);
+      if (q == 0) launch_k(k_slice<256, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
+      else if (q == 1) launch_k(k_slice<512, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
+      else launch_k(k_slice<768, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
     }
   }
 }
@@ -2273,11 +2328,11 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s) {
 
 // The kernels of one pass in launch order (numbering as KernelTimer::which).
 static int pass_kernels(const BatchDev& b, int main_flags, int* order) {
+  (void)main_flags;
   int n = 0;
   if (b.slice_blocks > 0) order[n++] = 0;
   if (b.nhh + b.nlad + b.ncon > 0) order[n++] = 1;
   order[n++] = 3;
-  if (main_flags & MF_ITERATE) order[n++] = 4;
   return n;
 }
 
@@ -2309,7 +2364,7 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
 }
 
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const KernelTimer* timer,
-                      hipEvent_t tail_stop) {
+                      hipEvent_t tail_stop, int fin_prev) {
   int order[5];
   const int n = pass_kernels(b, main_flags, order);
   hipEvent_t stop[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -2321,7 +2376,7 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
   if (tail_stop) stop[n - 1] = tail_stop;
   for (int i = 0; i < n; ++i) {
     switch (order[i]) {
-      case 0: launch_slice(b, s, stop[i]); break;
+      case 0: launch_slice(b, s, stop[i], fin_prev); break;
       case 1: launch_contract(b, s, stop[i]); break;
       case 3:
         if (b.main_nq == 16)
@@ -2330,10 +2385,6 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
         else
           launch_k(k_main<32>, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
                    (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
-        break;
-      case 4:
-        launch_k(k_finalize, stop[i], dim3(b.n), dim3(kFinThreads), 0, s, (const SolveDev*)b.solves,
-                 (int)FIN_ITER, 1);
         break;
     }
   }

@@ -171,7 +171,10 @@ struct SolveDevT {
   Ptr<double> joo;          // [kJooParts][o] mosaic Joo summed over each slice range
   Ptr<double> partials;     // [nblk_elem * kPartialStride]
   int nblk_elem;
-  Ptr<State> st;
+  Ptr<State> st;            // the state between passes
+  Ptr<State> st2;           // the state of the running pass: equal to st between passes;
+                            // k_slice writes it when it runs the previous pass's reductions,
+                            // k_contract and k_main read it, k_main copies it back to st
   Ptr<double> trace;        // [2*maxiter] (E, dT) by NIter
 };
 template <typename T> using HostPtr = T*;
@@ -248,6 +251,11 @@ inline int slice_threads(int o) {
   return 256 * (g < 1 ? 1 : (g > 3 ? 3 : g));
 }
 constexpr int kMaxDynLds = 160 * 1024;        // LDS per CU on gfx950
+// Static LDS of a k_slice block of bt threads for the previous pass's
+// reductions: the partial sums, the reduction scratch and a state copy.
+constexpr int slice_fin_lds(int bt) {
+  return (int)sizeof(double) * (kPartialStride + (bt / 64) * 32) + (int)sizeof(State);
+}
 
 // Kernel launchers (kernels.hip).
 // drpa: the spin-summed amplitudes of the dRPA iteration (in the first
@@ -277,8 +285,12 @@ int launch_persistent(const BatchDev& b, int dyn_lds, int passes, hipStream_t s)
 // One-time per-kernel set-up (dynamic LDS limits), done before any stream
 // capture.
 void prepare_launches();
+// One residual pass: k_slice, k_contract, k_main.  fin_prev: the pass's
+// k_slice first runs the reductions and state update of the pass before it
+// (every block the same; one writes the state), so that only the last pass
+// of a chunk needs a launch_finalize(FIN_ITER) of its own.
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s,
-                      const KernelTimer* timer = nullptr, hipEvent_t tail_stop = nullptr);
+                      const KernelTimer* timer = nullptr, hipEvent_t tail_stop = nullptr, int fin_prev = 0);
 int first_pass_kernel(const BatchDev& b, int main_flags);
 // FIN_INIT starts a fresh state (every field; prec64 the starting precision).
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64 = 1);
