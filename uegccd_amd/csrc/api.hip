@@ -271,7 +271,10 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   // center + lin[p] + lin[q] - lin[r]: FindIndex is one add and one load.
   const int h = 3 * mx + 1, w = 2 * h + 1;
   const int center = h + w * h + w * w * h;
-  std::vector<int> box((size_t)w * w * w, -1);
+  // (the temporaries of planning are kept per planning thread: a request
+  // then touches no freshly mapped pages for them)
+  thread_local std::vector<int> box, kid, civ, qid;
+  box.assign((size_t)w * w * w, -1);
   std::vector<int> lin(nao);
   for (int p = 0; p < nao; ++p) {
     const int4 n = c->nv4[p];
@@ -282,7 +285,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   }
   // pair-momentum blocks, in order of first occurrence of K over x = i + o*j
   // (block ids looked up by the box index of K)
-  std::vector<int> kid((size_t)w * w * w, -1);
+  kid.assign((size_t)w * w * w, -1);
   std::vector<std::vector<int>> pairs;
   c->pair_kb.assign(o * o, 0);
   c->pair_pos.assign(o * o, 0);
@@ -308,24 +311,48 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   // b = n_i + n_j - n_a depends on (i,j) only through K = n_i + n_j, so a
   // block's virtual list holds the a whose partner K - n_a is a virtual
   // orbital, and element (i,j,a) is valid exactly when a is on the list of
-  // the block of (i,j).  civ[K*v + av]: entry of a in block K's list, or -1.
-  std::vector<int> civ((size_t)c->nK * v, -1);
-  for (int K = 0; K < c->nK; ++K) {
+  // the block of (i,j).  civ[av*nK + K]: entry of a in block K's list, or -1
+  // (slice-major, so that the element walk below reads one row per slice).
+  const int nK = c->nK;
+  civ.assign((size_t)nK * v, -1);
+  std::vector<int> kbase(nK), nvk(nK, 0);
+  c->kb_pairs.reserve((size_t)o * o);
+  for (int K = 0; K < nK; ++K) {
     const int x0 = pairs[K][0];
-    const int kbase = center + lin[x0 % o] + lin[x0 / o];
+    kbase[K] = center + lin[x0 % o] + lin[x0 / o];
     for (int x : pairs[K]) c->kb_pairs.push_back(x);
+    c->kb_poff[K + 1] = (int)c->kb_pairs.size();
+  }
+  // partners slice by slice (civ holds the partner b for now), counted per block
+  for (int av = 0; av < v; ++av) {
+    int* crow = civ.data() + (size_t)av * nK;
+    const int la = lin[o + av];
+    for (int K = 0; K < nK; ++K) {
+      const int b = box[kbase[K] - la];
+      crow[K] = b >= o ? b : -1;
+      nvk[K] += b >= o;
+    }
+  }
+  for (int K = 0; K < nK; ++K) c->kb_voff[K + 1] = c->kb_voff[K] + nvk[K];
+  const int nva = c->kb_voff[nK];
+  c->kb_va.assign(nva, 0);
+  c->kb_vb.assign(nva, 0);
+  c->kb_nva.assign(nva, make_int4(0, 0, 0, 0));
+  {
+    std::vector<int> fill(c->kb_voff.begin(), c->kb_voff.end() - 1);
     for (int av = 0; av < v; ++av) {
-      const int b = box[kbase - lin[o + av]];
-      if (b >= o) {
-        civ[(size_t)K * v + av] = (int)c->kb_va.size() - c->kb_voff[K];
-        c->kb_va.push_back(o + av);
-        c->kb_vb.push_back(b);
-        const int4 na = c->nv4[o + av];
-        c->kb_nva.push_back(make_int4(na.x, na.y, na.z, o + av));
+      int* crow = civ.data() + (size_t)av * nK;
+      const int4 na = c->nv4[o + av];
+      for (int K = 0; K < nK; ++K) {
+        const int b = crow[K];
+        if (b < 0) continue;
+        const int e = fill[K]++;
+        c->kb_va[e] = o + av;
+        c->kb_vb[e] = b;
+        c->kb_nva[e] = make_int4(na.x, na.y, na.z, o + av);
+        crow[K] = e - c->kb_voff[K];  // now the entry of a in block K's list
       }
     }
-    c->kb_poff[K + 1] = (int)c->kb_pairs.size();
-    c->kb_voff[K + 1] = (int)c->kb_va.size();
   }
   c->noo_max = c->nvv_max = 0;
   for (int K = 0; K < c->nK; ++K) {
@@ -341,7 +368,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   for (int K = 0; K < c->nK; ++K)
     c->kb_toff[K + 1] = c->kb_toff[K] + (c->kb_voff[K + 1] - c->kb_voff[K]) * (c->kb_poff[K + 1] - c->kb_poff[K]);
   c->nvalid = c->kb_toff[c->nK];
-  c->cmap.resize((size_t)c->nvalid);
+  c->cmap.resize((size_t)c->nvalid + 1);
   c->slab_coff.assign((size_t)v + 1, 0);
   {
     // per pair: its block, the block's pair count and the pair's first tk position
@@ -357,23 +384,42 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     // such an amplitude or lands on such an element, so they stay exactly
     // zero and the others see the unmasked terms (the frozen core).
     const EriKind kind0 = eri_kind(s);
-    c->coren = core_masked(s, kind0) ? std::min(s.coren, o) : 0;
+    const int coren = core_masked(s, kind0) ? std::min(s.coren, o) : 0;
+    c->coren = coren;
     c->core_elems.clear();
-    uint2* cm = c->cmap.data();
+    std::vector<char> core(o * o);
+    for (int pr = 0; pr < o * o; ++pr) core[pr] = (pr % o < coren || pr / o < coren) ? 1 : 0;
+    uint2* const cm = c->cmap.data();
+    int* const slab = c->slab_coff.data();
+    const int* const vb = c->kb_vb.data();
+    const int* const voff = c->kb_voff.data();
+    const int o2 = o * o;
     int nc = 0;
     for (int av = 0, x = 0; av < v; ++av) {
-      c->slab_coff[av] = nc;
-      for (int pr = 0; pr < o * o; ++pr, ++x) {
-        const int K = pk[pr], ci = civ[(size_t)K * v + av];
-        if (ci < 0) continue;
-        if (pr % o < c->coren || pr / o < c->coren) {
-          c->core_elems.push_back(x);
-          continue;
+      slab[av] = nc;
+      const int* crow = civ.data() + (size_t)av * nK;
+      if (coren) {
+        for (int pr = 0; pr < o2; ++pr, ++x) {
+          const int K = pk[pr], ci = crow[K];
+          if (ci < 0) continue;
+          if (core[pr]) {
+            c->core_elems.push_back(x);
+            continue;
+          }
+          cm[nc++] = elem_map(x, pbase[pr] + ci * pn[pr], vb[voff[K] + ci]);
         }
-        cm[nc++] = elem_map(x, pbase[pr] + ci * pn[pr], c->kb_vb[c->kb_voff[K] + ci]);
+      } else {
+        // (branch-free: every element writes the next entry, the valid ones
+        // keep it; cmap has one spare entry for the last write)
+        for (int pr = 0; pr < o2; ++pr, ++x) {
+          const int K = pk[pr], ci = crow[K];
+          const int cj = ci < 0 ? 0 : ci;
+          cm[nc] = elem_map(x, pbase[pr] + cj * pn[pr], vb[voff[K] + cj]);
+          nc += ci >= 0;
+        }
       }
     }
-    c->slab_coff[v] = nc;
+    slab[v] = nc;
     c->cmap.resize(nc);
     c->nvalid = nc;
   }
@@ -400,46 +446,55 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   });
   // Coulomb table, flag 0 of the ERI variants (source/HEG.f90:729-772)
   int n2max = 0;
-  for (int p = 0; p < nao; ++p)
-    for (int q = 0; q < nao; ++q) {
-      const int4 a = c->nv4[p], b = c->nv4[q];
-      n2max = std::max(n2max, (a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) +
-                                  (a.z - b.z) * (a.z - b.z));
+  for (int p = 0; p < nao; ++p) {
+    const int4 a = c->nv4[p];
+    int m = 0;
+    for (int q = p + 1; q < nao; ++q) {
+      const int4 b = c->nv4[q];
+      const int dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+      m = std::max(m, dx * dx + dy * dy + dz * dz);
     }
+    n2max = std::max(n2max, m);
+  }
   if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
   // occurrence over (p, i), for the quadratic ring products
   {
-    std::vector<int> qid((size_t)w * w * w, -1);   // block of transfer q by box index
-    std::vector<std::vector<int>> qrows;
-    std::vector<int> qlin;
+    // (a counting sort: transfer ids and counts in one pass, rows in another)
+    qid.assign((size_t)w * w * w, -1);   // block of transfer q by box index
+    std::vector<int> qlin, qcount;
+    qlin.reserve((size_t)o * v);
+    qcount.reserve((size_t)o * v);
+    c->qb_of.assign((size_t)o * v, -1);
     for (int pv = 0; pv < v; ++pv)
       for (int i = 0; i < o; ++i) {
         const int ql = lin[o + pv] - lin[i];
         int& q = qid[center + ql];
         if (q < 0) {
-          q = (int)qrows.size();
-          qrows.emplace_back();
+          q = (int)qlin.size();
           qlin.push_back(ql);
+          qcount.push_back(0);
         }
-        qrows[q].push_back(i + o * pv);
+        ++qcount[q];
+        c->qb_of[(size_t)pv * o + i] = q;
       }
-    const int nq = (int)qrows.size();
+    const int nq = (int)qlin.size();
     c->qb_roff.assign(nq + 1, 0);
-    c->qb_rows.clear();
-    c->qb_rows.reserve((size_t)o * v);
+    for (int q = 0; q < nq; ++q) c->qb_roff[q + 1] = c->qb_roff[q] + qcount[q];
+    c->qb_rows.assign((size_t)o * v, 0);
+    c->qrow.assign((size_t)o * v, -1);
+    {
+      std::vector<int> fill(c->qb_roff.begin(), c->qb_roff.end() - 1);
+      for (int pv = 0; pv < v; ++pv)
+        for (int i = 0; i < o; ++i) {
+          const int q = c->qb_of[(size_t)pv * o + i], r = fill[q]++;
+          c->qrow[(size_t)pv * o + i] = r;
+          c->qb_rows[r] = i + o * pv;
+        }
+    }
     c->qb_neg.assign(nq, -1);
     c->qb_d.assign((size_t)nq * o, -1);
-    c->qrow.assign((size_t)o * v, -1);
-    c->qb_of.assign((size_t)o * v, -1);
     for (int q = 0; q < nq; ++q) {
-      for (int r : qrows[q]) {
-        const int i = r % o, pv = r / o;
-        c->qrow[(size_t)pv * o + i] = (int)c->qb_rows.size();
-        c->qb_of[(size_t)pv * o + i] = q;
-        c->qb_rows.push_back(r);
-      }
-      c->qb_roff[q + 1] = (int)c->qb_rows.size();
       c->qb_neg[q] = qid[center - qlin[q]];
       for (int l = 0; l < o; ++l) {
         const int p = box[center + lin[l] + qlin[q]];
