@@ -104,8 +104,9 @@ typedef struct {
                               kernel                                        */
     int32_t persistent;    /* 1: each chunk of passes of a single-context solve
                               runs as one cooperative launch whose phases are
-                              separated by grid-wide barriers (the persistent
-                              form of the five launches of a pass); 0: off    */
+                              separated by grid-wide barriers (the slice,
+                              contraction, residual and state-update phases
+                              of a pass); 0: off                            */
 } uegccd_options;
 
 typedef struct {
