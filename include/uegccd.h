@@ -256,6 +256,10 @@ int uegccd_connectivity(uegccd_ctx* ctx, int32_t* counts, int32_t nbins);
 /* Plane-wave basis of init_HEG + change_rs (source/HEG.f90:16-43,150-324,
  * 494-533): enumeration |n|^2 <= maxkpoint, stable ordering by |n+twist|^2,
  * fcut/corefcut truncation, HF eigenvalues with the Madelung term, E_HF.
+ * cutoff_twisted = 1 (no counterpart in the reference, which cuts on the
+ * untwisted |n|^2 at source/HEG.f90:190-194) enumerates |n+twist|^2 <=
+ * maxkpoint instead: a sphere of the same radius around -twist, so that the
+ * basis of every twist holds the same kinetic-energy window.
  * Call with nvec == NULL to query nao; then nvec holds nao*3 (column-major
  * like nVec(NAO,3)) and eigen nao entries.  scalars (may be NULL) receives
  * L, Omega, Madelung, EHF, ScreeningDistance, DenomFactor, TolMax, FailRatio. */
@@ -266,6 +270,7 @@ typedef struct {
     double twist[3];
     double fcut, corefcut;
     double madelungfactor, gap, exchangefactor;
+    int32_t cutoff_twisted; /* 0: |n|^2 <= maxkpoint; 1: |n+twist|^2 <= maxkpoint */
 } uegccd_heg_params;
 
 int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao, int32_t* nvec, double* eigen,

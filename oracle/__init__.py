@@ -45,7 +45,8 @@ class _Params(ctypes.Structure):
                 ("rs", ctypes.c_double), ("twist", ctypes.c_double * 3),
                 ("fcut", ctypes.c_double), ("corefcut", ctypes.c_double),
                 ("madelungfactor", ctypes.c_double), ("gap", ctypes.c_double),
-                ("gapii", ctypes.c_double), ("exchangefactor", ctypes.c_double)]
+                ("gapii", ctypes.c_double), ("exchangefactor", ctypes.c_double),
+                ("cutoff_twisted", ctypes.c_int)]
 
 
 @dataclass
@@ -68,6 +69,7 @@ class OracleParams:
     do_ladder: bool = True
     do_mosaic: bool = True
     irange: tuple = field(default_factory=lambda: (0,) * 15)
+    cutoff_twisted: bool = False
 
 
 def _lib(variant="serial"):
@@ -136,6 +138,7 @@ class OracleSystem:
         cp.fcut, cp.corefcut = p.fcut, p.corefcut
         cp.madelungfactor, cp.gap, cp.gapii = p.madelungfactor, p.gap, p.gapii
         cp.exchangefactor = p.exchangefactor
+        cp.cutoff_twisted = int(p.cutoff_twisted)
         self._lib = _lib(variant)
         self._h = self._lib.orc_build(ctypes.byref(cp))
         if not self._h:

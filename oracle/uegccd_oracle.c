@@ -141,7 +141,11 @@ static int setup_basis(orc_system* s) {
                 g.nt[1] = l2 + s->p.twist[1];
                 g.nt[2] = l3 + s->p.twist[2];
                 g.nt2 = g.nt[0] * g.nt[0] + g.nt[1] * g.nt[1] + g.nt[2] * g.nt[2];
-                if (l1 * l1 + l2 * l2 + l3 * l3 <= s->maxk) g1[M++] = g;
+                /* source/HEG.f90:190-194 cuts on the untwisted |n|^2; the
+                   opt-in twisted sphere cuts on |n+twist|^2 instead */
+                const int in = s->p.cutoff_twisted ? (g.nt2 <= (double)s->maxk)
+                                                   : (l1 * l1 + l2 * l2 + l3 * l3 <= s->maxk);
+                if (in) g1[M++] = g;
             }
     int nbasis = M;
     /* stable ascending sort by twisted |n+t|^2: the bubble sort at
@@ -279,7 +283,14 @@ static inline int find_index(const orc_system* s, int m, int n, int l) {
     const int* nl = &s->nvec[(l - 1) * 3];
     int k0 = nm[0] + nn[0] - nl[0], k1 = nm[1] + nn[1] - nl[1], k2 = nm[2] + nn[2] - nl[2];
     if (abs(k0) > s->cell || abs(k1) > s->cell || abs(k2) > s->cell) return 0;
-    if (k0 * k0 + k1 * k1 + k2 * k2 > s->maxk) return 0;
+    /* the reference's early exit outside its cutoff sphere |n|^2 <= MaxKPoint;
+       with the opt-in twisted sphere the same test on |n + twist|^2 */
+    if (s->p.cutoff_twisted) {
+        const double x = k0 + s->p.twist[0], y = k1 + s->p.twist[1], z = k2 + s->p.twist[2];
+        if (x * x + y * y + z * z > (double)s->maxk) return 0;
+    } else if (k0 * k0 + k1 * k1 + k2 * k2 > s->maxk) {
+        return 0;
+    }
     int k = s->kp2bf[kidx(s, k0, k1, k2)];
     if (k > s->nao) return 0;
     return k;

@@ -32,6 +32,8 @@ typedef struct {
     double twist[3];       /* NTwist                                          */
     double fcut, corefcut;
     double madelungfactor, gap, gapii, exchangefactor;
+    int    cutoff_twisted; /* 0: |n|^2 <= maxkpoint (the reference's cutoff);
+                              1: |n+twist|^2 <= maxkpoint (opt-in)            */
 } orc_params;
 
 /* Opaque system (HEGDataType + the ERI pointer choice). */

@@ -36,7 +36,7 @@ def oracle_for(info, eigen=None) -> OracleSystem:
                      exchangefactor=info.exchangefactor, coren=info.CoreN,
                      safe_eri=info.SafeERI, do_ring=info.DoRing, do_xring=info.DoXRing,
                      do_ladder=info.DoLadder, do_mosaic=info.DoMosaic,
-                     irange=tuple(info.irange))
+                     irange=tuple(info.irange), cutoff_twisted=bool(info.CutoffTwisted))
     s = OracleSystem(p)
     if eigen is not None:
         s.set_eigen(eigen)

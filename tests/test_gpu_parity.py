@@ -196,6 +196,11 @@ def _basis_cases():
     cases += [parse_input(standard_input(nel, ec, rs, tw)) for nel, ec, rs, tw in reqs]
     cases.append(parse_input(standard_input(14, 40, 1.0, (0.5, 0.5, 0.5))))   # maximal ties
     cases.append(parse_input(standard_input(38, 16, 3.0, (0.25, -0.125, 0.0), fcut=1.5)))
+    # the opt-in twisted sphere |n + twist|^2 <= E_c
+    for nel, ec, rs, tw in reqs[:8] + [(14, 40, 1.0, (0.5, 0.5, 0.5))]:
+        info = parse_input(standard_input(nel, ec, rs, tw))
+        info.CutoffTwisted = True
+        cases.append(info)
     return cases
 
 
@@ -221,3 +226,25 @@ def test_device_basis_rejects_oversized_cube():
     with pytest.raises(L.UegccdError) as e:
         init_heg(parse_input(standard_input(14, 81, 1.0, (0, 0, 0))), device=0)
     assert e.value.status == L.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("nel,ec,rs,twist", [(14, 5, 1.0, (0.11, -0.23, 0.05)), (38, 9, 1.0, (-0.31, 0.42, 0.07))])
+def test_twisted_cutoff_solve(nel, ec, rs, twist):
+    """A CCD solve on a twisted-sphere basis (|n + twist|^2 <= E_c): the
+    reference iteration of the oracle on the same basis, with the same NIter
+    and every (E, dT) of the trajectory within 1e-12."""
+    from uegccd_amd.twist_stream import request_info
+    print(f"[twisted cutoff] N={nel} Ec={ec}", flush=True)
+    info = request_info((nel, ec, rs, twist), cutoff_twisted=True)
+    heg, ctx = _product(info)
+    orc = oracle_for(info)
+    assert heg.NAO == orc.nao
+    np.testing.assert_array_equal(heg.nVec, orc.nvec)
+    _, _, tab, tba = ctx.mp2()
+    r = ctx.solve(tab, tba)
+    o = orc.ccd()
+    assert r.status == 0 and r.niter == o["niter"]
+    assert abs(r.ecorr - o["ecorr"]) < 1e-12
+    e, dt = ctx.trace(r.niter)
+    np.testing.assert_allclose(e, o["trace_e"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dt, o["trace_dt"], rtol=0, atol=1e-12)

@@ -201,3 +201,90 @@ def test_exchange_structure_factor_vs_oracle(name):
         np.testing.assert_array_equal(q, oq)
         np.testing.assert_array_equal(cnt, ocnt)
         np.testing.assert_allclose(v, ov, rtol=1e-14)
+
+
+def _twisted_sphere(ec, twist):
+    """Every integer n with |n + twist|^2 <= ec, in the stable order of
+    |n + twist|^2 over the enumeration (a, then b, then c ascending): the
+    basis of the twisted cutoff before any fcut truncation, by brute force."""
+    cell = int(np.sqrt(ec)) + 3
+    pts = []
+    for a in range(-cell, cell + 1):
+        for b in range(-cell, cell + 1):
+            for c in range(-cell, cell + 1):
+                x, y, z = a + twist[0], b + twist[1], c + twist[2]
+                k2 = x * x + y * y + z * z
+                if k2 <= ec:
+                    pts.append((k2, len(pts), (a, b, c)))
+    pts.sort(key=lambda p: (p[0], p[1]))
+    return np.array([p[2] for p in pts], dtype=np.int32)
+
+
+TWISTED = [(14, 5, (0.11, -0.23, 0.05)), (38, 9, (0.5, 0.5, 0.5)), (54, 16, (-0.31, 0.42, 0.07)),
+           (14, 25, (0.5, 0.0, -0.25))]
+
+
+@pytest.mark.parametrize("nel,ec,twist", TWISTED)
+def test_twisted_cutoff_basis(nel, ec, twist):
+    """The opt-in twisted sphere |n + twist|^2 <= E_c: the oracle's basis is
+    the brute-force lattice sphere around -twist in SetupBasis's stable order,
+    the host builder gives the oracle's n-vectors and eigenvalues, and every
+    orbital lies inside the sphere, none outside."""
+    from uegccd_amd.driver import init_heg
+    from uegccd_amd.twist_stream import request_info
+    info = request_info((nel, ec, 1.5, twist), cutoff_twisted=True)
+    orc = oracle_for(info)
+    want = _twisted_sphere(ec, twist)
+    assert orc.nao == len(want)
+    np.testing.assert_array_equal(orc.nvec, want)
+    heg = init_heg(info)
+    assert heg.NAO == orc.nao
+    np.testing.assert_array_equal(heg.nVec, orc.nvec)
+    np.testing.assert_allclose(heg.Eigen, orc.eigen, rtol=0, atol=1e-14)
+    assert heg.EHF == pytest.approx(orc.ehf, rel=1e-13, abs=1e-13)
+    nt2 = ((heg.nVec + np.array(twist)) ** 2).sum(axis=1)
+    assert nt2.max() <= ec
+    # the untwisted cutoff of the same request is a different set of plane waves
+    plain = init_heg(request_info((nel, ec, 1.5, twist)))
+    assert {tuple(v) for v in plain.nVec} != {tuple(v) for v in heg.nVec}
+
+
+def test_twisted_cutoff_at_gamma_is_the_reference_cutoff():
+    from uegccd_amd.driver import init_heg
+    from uegccd_amd.twist_stream import request_info
+    for nel, ec in ((14, 5), (54, 16)):
+        a = init_heg(request_info((nel, ec, 2.0, (0.0, 0.0, 0.0))))
+        b = init_heg(request_info((nel, ec, 2.0, (0.0, 0.0, 0.0)), cutoff_twisted=True))
+        np.testing.assert_array_equal(a.nVec, b.nVec)
+        np.testing.assert_array_equal(a.Eigen, b.Eigen)
+
+
+def test_cutoff_flag_is_checked():
+    lib = L.load()
+    p = L.HegParams()
+    p.nel, p.maxkpoint, p.rs, p.fcut, p.corefcut = 14, 5, 1.0, 100.0, 1.0
+    p.madelungfactor, p.exchangefactor = 1.0, 1.0
+    p.cutoff_twisted = 2
+    n = ctypes.c_int32()
+    assert lib.uegccd_heg_build(ctypes.byref(p), ctypes.byref(n), None, None, None) == L.ERR_INVALID
+
+
+@pytest.mark.parametrize("twisted", [False, True])
+def test_oracle_findindex_on_both_cutoffs(twisted):
+    """The oracle's FindIndex returns the (1-based) orbital of n_m + n_n - n_l
+    whenever that momentum is in the basis and 0 otherwise, by brute force
+    over every triple of a twisted system under either cutoff (the twisted
+    sphere holds plane waves outside |n|^2 <= E_c, which the reference's early
+    exit would miss)."""
+    from uegccd_amd.twist_stream import request_info
+    info = request_info((14, 5, 1.0, (0.41, -0.23, 0.35)), cutoff_twisted=twisted)
+    orc = oracle_for(info)
+    where = {tuple(v): k + 1 for k, v in enumerate(orc.nvec)}
+    if twisted:
+        assert max(int((v ** 2).sum()) for v in orc.nvec) > 5  # outside the untwisted sphere
+    n = orc.nao
+    for m in range(1, n + 1):
+        for l in range(1, n + 1):
+            for nn in range(1, 8):
+                want = where.get(tuple(orc.nvec[m - 1] + orc.nvec[nn - 1] - orc.nvec[l - 1]), 0)
+                assert orc.findindex(m, nn, l) == want

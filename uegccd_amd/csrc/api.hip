@@ -1397,6 +1397,15 @@ int next_chunk(uegccd_ctx** cs, int n, const std::vector<State>& st, std::vector
 // graph, captured on first use and kept (the kernels' arguments are the
 // context's fixed device tables, so a replay is the same launches).  The
 // passes hold no memset or copy nodes.
+// Where a pass's reductions and state update run: folded into the next
+// pass's k_slice, where every slice block reduces its solve's element-block
+// partials for itself (about 1 us of a C2 pass, against 5 us for a launch of
+// its own), for a single solve; in a finalize launch after every pass for a
+// batch, where the slice blocks of its large solves would re-read hundreds
+// of KB of partials each (a finalize launch per pass: batched stream 1,278
+// -> 1,365 solves/s).
+bool fold_state_update(int nsolves) { return nsolves == 1; }
+
 int chunk_graph(uegccd_ctx* c, int len, hipGraphExec_t* out) {
   auto it = c->graphs.find(len);
   if (it != c->graphs.end()) {
@@ -1502,18 +1511,21 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
       if (rc) return rc;
       HIPCHK(hipGraphLaunch(ex, s));
     }
-    // (each pass's k_slice runs the reductions of the pass before it; the
-    // chunk's last pass has a finalize launch of its own)
+    // (a single solve's k_slice runs the reductions of the pass before it,
+    // and the chunk's last pass has a finalize launch of its own; a batch
+    // has a finalize launch after every pass)
+    const bool fold = fold_state_update(n);
     for (int k = 0; k < chunk && !graphs && !persist; ++k) {
       if (timing) {
         const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
         const hipEvent_t tail = (timed_first && k + 1 < chunk) ? c0->tev[2 * (k + 1)] : nullptr;
-        launch_iteration(b, MF_ITERATE, s, &t, tail, k > 0);
+        launch_iteration(b, MF_ITERATE, s, &t, tail, fold && k > 0);
       } else {
-        launch_iteration(b, MF_ITERATE, s, nullptr, nullptr, k > 0);
+        launch_iteration(b, MF_ITERATE, s, nullptr, nullptr, fold && k > 0);
       }
+      if (!fold) launch_finalize(b, FIN_ITER, s);
     }
-    if (!graphs && !persist) launch_finalize(b, FIN_ITER, s);
+    if (!graphs && !persist && fold) launch_finalize(b, FIN_ITER, s);
     passes += chunk;
     HIPCHK(hipGetLastError());
     if (gather) {
@@ -1886,7 +1898,8 @@ int fcut_count(double f, int nel) {
 
 extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, int32_t* nvec,
                                 double* eigen, double* scalars) {
-  if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0))
+  if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0) ||
+      (p->cutoff_twisted != 0 && p->cutoff_twisted != 1))
     return UEGCCD_ERR_INVALID;
   const int nel = p->nel, no = nel / 2;
   const int cell = (int)std::sqrt((double)p->maxkpoint) + 3;
@@ -1895,13 +1908,13 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
   for (int a = -cell; a <= cell; ++a)
     for (int b = -cell; b <= cell; ++b)
       for (int c = -cell; c <= cell; ++c) {
-        if (a * a + b * b + c * c > p->maxkpoint) continue;
         PW w;
         w.n[0] = a; w.n[1] = b; w.n[2] = c;
         w.nt[0] = a + p->twist[0];
         w.nt[1] = b + p->twist[1];
         w.nt[2] = c + p->twist[2];
         w.nt2 = w.nt[0] * w.nt[0] + w.nt[1] * w.nt[1] + w.nt[2] * w.nt[2];
+        if (p->cutoff_twisted ? w.nt2 > (double)p->maxkpoint : a * a + b * b + c * c > p->maxkpoint) continue;
         pw.push_back(w);
       }
   // ties keep enumeration order, as the reference's bubble sort on strict '<'
@@ -2418,9 +2431,12 @@ int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
       s->batch_stale = false;
       s->nbatch = cs.size();
     }
-    for (int k = 0; k < s->opt.iters_per_step; ++k)
-      launch_iteration(s->batch.b, MF_ITERATE, s->stream, nullptr, nullptr, k > 0);
-    launch_finalize(s->batch.b, FIN_ITER, s->stream);
+    const bool fold = fold_state_update((int)s->nbatch);
+    for (int k = 0; k < s->opt.iters_per_step; ++k) {
+      launch_iteration(s->batch.b, MF_ITERATE, s->stream, nullptr, nullptr, fold && k > 0);
+      if (!fold) launch_finalize(s->batch.b, FIN_ITER, s->stream);
+    }
+    if (fold) launch_finalize(s->batch.b, FIN_ITER, s->stream);
     launch_gather_states(s->batch.b, s->batch.states, s->stream);
     HIPCHK(hipMemcpyAsync(s->h_states, s->batch.states, sizeof(State) * s->nbatch,
                           hipMemcpyDeviceToHost, s->stream));
@@ -2486,7 +2502,8 @@ double uegccd_server_clock_ms(const uegccd_server* s) { return s ? server_now(s)
 extern "C" int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t device,
                                        int32_t* nao_out, int32_t* nvec, double* eigen,
                                        double* scalars) {
-  if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0))
+  if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0) ||
+      (p->cutoff_twisted != 0 && p->cutoff_twisted != 1))
     return UEGCCD_ERR_INVALID;
   const int nel = p->nel, no = nel / 2;
   if (no > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
@@ -2526,7 +2543,7 @@ extern "C" int uegccd_heg_build_device(const uegccd_heg_params* p, int32_t devic
   };
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return done(UEGCCD_ERR_HIP);
   int count = 0;
-  if (basis_on_device(cell, p->maxkpoint, p->twist, npad, d_key, d_idx, d_count, s) ||
+  if (basis_on_device(cell, p->maxkpoint, p->cutoff_twisted, p->twist, npad, d_key, d_idx, d_count, s) ||
       hipMemcpyAsync(&count, d_count, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return done(UEGCCD_ERR_HIP);

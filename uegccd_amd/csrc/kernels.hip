@@ -1935,7 +1935,7 @@ __global__ __launch_bounds__(64) void k_sf_cells(const SolveDev* __restrict__ so
 
 // Keys |n + twist|^2 of the cube [-cell, cell]^3 in enumeration order (x
 // slowest), +inf outside |n|^2 <= maxk and in the padding.
-__global__ void k_basis_keys(int cell, int maxk, double tx, double ty, double tz, int npad,
+__global__ void k_basis_keys(int cell, int maxk, int twisted, double tx, double ty, double tz, int npad,
                              double* __restrict__ key, int* __restrict__ idx) {
 #pragma clang fp contract(off)
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1944,10 +1944,10 @@ __global__ void k_basis_keys(int cell, int maxk, double tx, double ty, double tz
   double kv = __longlong_as_double(0x7ff0000000000000LL);  // +inf
   if (e < w * w * w) {
     const int a = e / (w * w) - cell, b = (e / w) % w - cell, c = e % w - cell;
-    if (a * a + b * b + c * c <= maxk) {
-      const double x = a + tx, y = b + ty, z = c + tz;
-      kv = x * x + y * y + z * z;
-    }
+    const double x = a + tx, y = b + ty, z = c + tz;
+    const double k2 = x * x + y * y + z * z;
+    // the reference's sphere |n|^2 <= maxk, or the twisted one |n+t|^2 <= maxk
+    if (twisted ? k2 <= (double)maxk : a * a + b * b + c * c <= maxk) kv = k2;
   }
   key[e] = kv;
   idx[e] = e;
@@ -2065,9 +2065,9 @@ __global__ __launch_bounds__(256) void k_basis_orbitals(BasisArgs A, const int* 
   }
 }
 
-int basis_on_device(int cell, int maxk, const double* twist, int npad, double* key, int* idx,
+int basis_on_device(int cell, int maxk, int twisted, const double* twist, int npad, double* key, int* idx,
                     int* count, hipStream_t s) {
-  hipLaunchKernelGGL(k_basis_keys, dim3((npad + 255) / 256), dim3(256), 0, s, cell, maxk, twist[0],
+  hipLaunchKernelGGL(k_basis_keys, dim3((npad + 255) / 256), dim3(256), 0, s, cell, maxk, twisted, twist[0],
                      twist[1], twist[2], npad, key, idx);
   const int lds = npad * (int)(sizeof(double) + sizeof(int));
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_basis_sort),

@@ -314,7 +314,7 @@ int connectivity_on_device(const BatchDev& b, int nvalid, int nbins, int* hist, 
 // enumeration index, the count of plane waves inside the cutoff; then the nb
 // orbitals (n-vectors column-major, eigenvalues) and E_HF.
 constexpr int kMaxBasisPad = 8192;
-int basis_on_device(int cell, int maxk, const double* twist, int npad, double* key, int* idx,
+int basis_on_device(int cell, int maxk, int twisted, const double* twist, int npad, double* key, int* idx,
                     int* count, hipStream_t s);
 int basis_orbitals_on_device(int cell, int nb, int no, int mc, int nel, const double* twist,
                              double L1, double rescale, double omega, double mad, double gap,

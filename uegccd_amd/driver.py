@@ -53,6 +53,7 @@ def init_heg(info: UEGInfo, twist=None, device: int | None = None) -> HEGData:
         p.twist[d] = float(tw[d])
     p.fcut, p.corefcut = info.fcut, info.corefcut
     p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+    p.cutoff_twisted = int(info.CutoffTwisted)
     if device is None:
         build = lambda *a: lib.uegccd_heg_build(ctypes.byref(p), *a)  # noqa: E731
         name = "uegccd_heg_build"
@@ -85,6 +86,7 @@ def exchange_structure_factor(info: UEGInfo, rs: float | None = None, twist=None
         p.twist[d] = float(tw[d])
     p.fcut, p.corefcut = info.fcut, info.corefcut
     p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+    p.cutoff_twisted = int(info.CutoffTwisted)
     n = ctypes.c_int32(0)
     L.check(lib.uegccd_exchange_structure_factor(ctypes.byref(p), ctypes.byref(n), None, None, None),
             "uegccd_exchange_structure_factor")
@@ -394,6 +396,7 @@ class Server:
             p.twist[d] = float(tw[d])
         p.fcut, p.corefcut = info.fcut, info.corefcut
         p.madelungfactor, p.gap, p.exchangefactor = info.madelungfactor, info.Gap, info.exchangefactor
+        p.cutoff_twisted = int(info.CutoffTwisted)
         q.gapii, q.coren, q.safe_eri = info.GapII, info.CoreN, int(info.SafeERI)
         for k in range(15):
             q.irange[k] = int(info.irange[k])

@@ -36,6 +36,7 @@ Contains
     P%madelungfactor = UEGInfo%MadelungFactor
     P%gap = UEGInfo%Gap
     P%exchangefactor = UEGInfo%ExchangeFactor
+    P%cutoff_twisted = 0
     Call Check(uegccd_heg_build_device(P, 0_c_int32_t, NAO, c_null_ptr, c_null_ptr, c_null_ptr), &
                'uegccd_heg_build_device')
     Allocate(NVecC(3*NAO), Eig(NAO))

@@ -100,6 +100,7 @@ Contains
     Req%heg%madelungfactor = UEGInfo%MadelungFactor
     Req%heg%gap = UEGInfo%Gap
     Req%heg%exchangefactor = UEGInfo%ExchangeFactor
+    Req%heg%cutoff_twisted = 0
     Req%gapii = UEGInfo%GapII
     Req%coren = UEGInfo%CoreN
     Req%safe_eri = Merge(1, 0, UEGInfo%SafeERI)

@@ -35,6 +35,7 @@ Module Wrappers
   Type, Bind(C) :: uegccd_heg_params
     Integer(c_int32_t) :: nel, maxkpoint
     Real(c_double) :: rs, twist(3), fcut, corefcut, madelungfactor, gap, exchangefactor
+    Integer(c_int32_t) :: cutoff_twisted = 0   ! 0: the reference's |n|^2 cutoff
   End Type uegccd_heg_params
 
   Type, Bind(C) :: uegccd_server_options

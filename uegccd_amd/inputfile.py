@@ -68,6 +68,10 @@ class UEGInfo:
     DoSFCalcCCD: bool = False
     DoSkipTA: bool = False
     NAO: int = 0  # set by the basis construction
+    # Not an Input keyword (the reference cuts on the untwisted |n|^2 <= MaxKPoint,
+    # source/HEG.f90:190-194): callers opt in to the twisted sphere
+    # |n + twist|^2 <= MaxKPoint by setting it.
+    CutoffTwisted: bool = False
 
     @property
     def rs(self) -> float:

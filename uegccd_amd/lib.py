@@ -73,7 +73,8 @@ class HegParams(ctypes.Structure):
     _fields_ = [("nel", ctypes.c_int32), ("maxkpoint", ctypes.c_int32), ("rs", ctypes.c_double),
                 ("twist", ctypes.c_double * 3), ("fcut", ctypes.c_double),
                 ("corefcut", ctypes.c_double), ("madelungfactor", ctypes.c_double),
-                ("gap", ctypes.c_double), ("exchangefactor", ctypes.c_double)]
+                ("gap", ctypes.c_double), ("exchangefactor", ctypes.c_double),
+                ("cutoff_twisted", ctypes.c_int32)]
 
 
 class ServerOptions(ctypes.Structure):

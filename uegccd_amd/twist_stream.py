@@ -34,7 +34,12 @@ def stream_requests(n=256, seed=1234, rate=None, electrons=ELECTRONS, rs_values=
     return reqs, arrivals
 
 
-def request_info(req) -> UEGInfo:
-    """The Input of one request (CCD only, the reference's default keys)."""
+def request_info(req, cutoff_twisted=False) -> UEGInfo:
+    """The Input of one request (CCD only, the reference's default keys).  The
+    basis cutoff is the reference's |n|^2 <= E_c; with cutoff_twisted the
+    basis is the twisted sphere |n + twist|^2 <= E_c instead (an opt-in of
+    the library's basis builders)."""
     nel, ec, rs, tw = req
-    return parse_input(standard_input(nel, ec, rs, tw))
+    info = parse_input(standard_input(nel, ec, rs, tw))
+    info.CutoffTwisted = bool(cutoff_twisted)
+    return info
