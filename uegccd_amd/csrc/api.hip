@@ -202,11 +202,13 @@ struct uegccd_ctx {
   double tolmax = 1e-8, failratio = 100, df = 2.0;
   double boxL = 0.0;  // box length L
   // host copies of the maps
-  std::vector<int> kb_poff, kb_pairs, kb_voff, kb_va, kb_vb, pair_kb, pair_pos;
+  std::vector<int> kb_poff, kb_pairs, kb_voff, pair_kb, pair_pos;
+  int nva = 0;                     // entries of the blocks' virtual lists (built on the device)
+  std::vector<uint64_t> vbits;     // bit av*nK + K: virtual a has a virtual partner in block K
+  int box_w = 0, box_center = 0;   // momentum box of the device-built maps
   std::vector<int> kb_toff;        // K-blocked amplitude layout: block offsets
   std::vector<int> slab_coff;        // history layout: first compact index of each virtual slice
-  std::vector<uint2> cmap;           // compact index -> elem_map(element, K-blocked position, partner)
-  std::vector<int4> nv4, kb_nva;
+  std::vector<int4> nv4;
   std::vector<double> eig, vtab, voo, vso;  // Coulomb tables by slot: v(|dn|^2); v(n_k - n_j) over
                                             // occupied k, j; v(n_s - n_k)
   int ntab = 1;                    // table slots (one per range flag in use; slot 0 is flag 0)
@@ -216,11 +218,13 @@ struct uegccd_ctx {
   double screening = 0.0;          // HEGData%ScreeningDistance
   std::vector<int4> hh_items;   // (0, K, row0, 0) hole-hole units
   std::vector<int4> lad_items;  // (0, K, row0, col0) particle-particle tiles
-  std::vector<int> qb_roff, qb_rows, qb_neg, qb_d, qrow, qb_of;  // hole-particle sets P(q)
+  std::vector<int> qb_roff, qb_rows, qb_neg, qrow, qb_of;  // hole-particle sets P(q)
   std::vector<int4> qb_info;
   std::vector<int4> con_items;  // (0, kind, arg, arg) of k_contract
   // device
   DeviceArena arena;
+  int* d_box = nullptr;    // scratch of the device-built maps (launch_plan)
+  int* d_civ = nullptr;
   SolveDev sd{};           // host copy of the device descriptor
   SolveDev* d_sd = nullptr;
   BatchDev batch{};        // batch of one
@@ -248,6 +252,31 @@ struct uegccd_ctx {
 
 namespace {
 
+// The elements (i, j, a) that have a virtual momentum partner, in element
+// order x = i + o j + o^2 (a - o), from the host's valid bits: f(x, core),
+// core when i or j is a core orbital of the CoreN masking (those stay out of
+// the valid set).  The device's element map cmap lists the others in the
+// same order.
+template <typename F>
+void for_each_element(const uegccd_ctx* c, F f) {
+  const int o = c->o, o2 = o * o, nK = c->nK, coren = c->coren;
+  const uint64_t* bits = c->vbits.data();
+  for (int av = 0; av < c->v; ++av) {
+    const size_t row = (size_t)av * nK;
+    for (int pr = 0; pr < o2; ++pr) {
+      const size_t e = row + c->pair_kb[pr];
+      if ((bits[e >> 6] >> (e & 63)) & 1ull) f(pr + o2 * av, pr % o < coren || pr / o < coren);
+    }
+  }
+}
+// The valid elements: f(x) for each, in element order (the order of cmap).
+template <typename F>
+void for_each_valid(const uegccd_ctx* c, F f) {
+  for_each_element(c, [&f](int x, bool core) {
+    if (!core) f(x);
+  });
+}
+
 int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   const int o = s.nocc, nao = s.nao, v = nao - o;
   c->o = o;
@@ -266,11 +295,17 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     c->nv4[p] = q;
     mx = std::max({mx, std::abs(q.x), std::abs(q.y), std::abs(q.z)});
   }
-  // momentum -> orbital lookup over a box holding every n_p + n_q - n_r.  The
-  // box index is affine in the momentum, so index(n_p + n_q - n_r) =
-  // center + lin[p] + lin[q] - lin[r]: FindIndex is one add and one load.
-  const int h = 3 * mx + 1, w = 2 * h + 1;
+  int mo = 0;  // largest |component| of an occupied momentum
+  for (int i = 0; i < o; ++i)
+    mo = std::max({mo, std::abs(c->nv4[i].x), std::abs(c->nv4[i].y), std::abs(c->nv4[i].z)});
+  // momentum -> orbital lookup over a box holding every n_i + n_j - n_p and
+  // n_i + n_p - n_j (i, j occupied: half-width max|n| + 2 mo).  The box index is
+  // affine in the momentum, so index(n_p + n_q - n_r) = center + lin[p] +
+  // lin[q] - lin[r]: FindIndex is one add and one load.
+  const int h = mx + 2 * mo + 1, w = 2 * h + 1;
   const int center = h + w * h + w * w * h;
+  c->box_w = w;
+  c->box_center = center;
   // (the temporaries of planning are kept per planning thread: a request
   // then touches no freshly mapped pages for them)
   thread_local std::vector<int> box, kid, civ, qid;
@@ -284,145 +319,104 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     slot = p;
   }
   // pair-momentum blocks, in order of first occurrence of K over x = i + o*j
-  // (block ids looked up by the box index of K)
-  kid.assign((size_t)w * w * w, -1);
-  std::vector<std::vector<int>> pairs;
+  // (block ids looked up in a box over the sums of two occupied momenta, of
+  // half-width 2 mo)
+  const int wk = 4 * mo + 1, ck = 2 * mo * (1 + wk + wk * wk);
+  kid.assign((size_t)wk * wk * wk, -1);
+  thread_local std::vector<int> lk, kcnt, kfirst;
+  lk.resize(o);
+  for (int i = 0; i < o; ++i) lk[i] = c->nv4[i].x + wk * c->nv4[i].y + wk * wk * c->nv4[i].z;
   c->pair_kb.assign(o * o, 0);
   c->pair_pos.assign(o * o, 0);
+  kcnt.clear();
+  kfirst.clear();
   for (int x = 0; x < o * o; ++x) {
-    const int i = x % o, j = x / o;
-    int& K = kid[center + lin[i] + lin[j]];
+    int& K = kid[ck + lk[x % o] + lk[x / o]];
     if (K < 0) {
-      K = (int)pairs.size();
-      pairs.emplace_back();
+      K = (int)kcnt.size();
+      kcnt.push_back(0);
+      kfirst.push_back(x);
     }
     c->pair_kb[x] = K;
-    c->pair_pos[x] = (int)pairs[K].size();
-    pairs[K].push_back(x);
+    c->pair_pos[x] = kcnt[K]++;
   }
-  c->nK = (int)pairs.size();
+  c->nK = (int)kcnt.size();
   c->kb_poff.assign(c->nK + 1, 0);
+  for (int K = 0; K < c->nK; ++K) c->kb_poff[K + 1] = c->kb_poff[K] + kcnt[K];
+  c->kb_pairs.assign(o * o, 0);
+  for (int x = 0; x < o * o; ++x) c->kb_pairs[c->kb_poff[c->pair_kb[x]] + c->pair_pos[x]] = x;
   c->kb_voff.assign(c->nK + 1, 0);
-  c->kb_pairs.clear();
-  c->kb_va.clear();
-  c->kb_vb.clear();
-  c->kb_nva.clear();
   // The partner test of every loop in source/CCD.f90, FindIndex(i,j,a) > NOcc:
   // b = n_i + n_j - n_a depends on (i,j) only through K = n_i + n_j, so a
   // block's virtual list holds the a whose partner K - n_a is a virtual
   // orbital, and element (i,j,a) is valid exactly when a is on the list of
-  // the block of (i,j).  civ[av*nK + K]: entry of a in block K's list, or -1
-  // (slice-major, so that the element walk below reads one row per slice).
+  // the block of (i,j).  The host keeps which (a, K) are valid (the bit set
+  // vbits) and the counts the layout needs; the lists themselves and the
+  // element map are built on the device at placement (launch_plan).
   const int nK = c->nK;
-  civ.assign((size_t)nK * v, -1);
+  civ.resize((size_t)nK * v);
+  c->vbits.assign(((size_t)nK * v + 63) / 64, 0ull);
   std::vector<int> kbase(nK), nvk(nK, 0);
-  c->kb_pairs.reserve((size_t)o * o);
-  for (int K = 0; K < nK; ++K) {
-    const int x0 = pairs[K][0];
-    kbase[K] = center + lin[x0 % o] + lin[x0 / o];
-    for (int x : pairs[K]) c->kb_pairs.push_back(x);
-    c->kb_poff[K + 1] = (int)c->kb_pairs.size();
-  }
-  // partners slice by slice (civ holds the partner b for now), counted per block
+  for (int K = 0; K < nK; ++K) kbase[K] = center + lin[kfirst[K] % o] + lin[kfirst[K] / o];
+  // With CoreN masking, the elements with a core index (i or j <= CoreN)
+  // leave the valid set: every masked integral of the reference multiplies
+  // such an amplitude or lands on such an element, so they stay exactly
+  // zero and the others see the unmasked terms (the frozen core).
+  const EriKind kind0 = eri_kind(s);
+  const int coren = core_masked(s, kind0) ? std::min(s.coren, o) : 0;
+  c->coren = coren;
+  std::vector<int> nfree(nK, 0);  // pairs of each block with no core index
+  for (int pr = 0; pr < o * o; ++pr)
+    nfree[c->pair_kb[pr]] += (pr % o < coren || pr / o < coren) ? 0 : 1;
+  // partners slice by slice, counted per block and (the valid elements of
+  // each slice) per slice (two loops: each vectorises)
+  c->slab_coff.assign((size_t)v + 1, 0);
   for (int av = 0; av < v; ++av) {
     int* crow = civ.data() + (size_t)av * nK;
     const int la = lin[o + av];
     for (int K = 0; K < nK; ++K) {
       const int b = box[kbase[K] - la];
       crow[K] = b >= o ? b : -1;
-      nvk[K] += b >= o;
+    }
+    int nslab = 0;
+    for (int K = 0; K < nK; ++K) {
+      const int ok = crow[K] >= 0;
+      nvk[K] += ok;
+      nslab += ok * nfree[K];
+    }
+    c->slab_coff[av + 1] = c->slab_coff[av] + nslab;
+  }
+  {
+    const size_t ne = (size_t)nK * v;
+    uint64_t* const bits = c->vbits.data();
+    for (size_t w0 = 0; w0 < ne; w0 += 64) {  // (branch-free: half the tests pass, at random)
+      const size_t n = std::min<size_t>(64, ne - w0);
+      uint64_t m = 0;
+      for (size_t k = 0; k < n; ++k) m |= (uint64_t)(civ[w0 + k] >= 0) << k;
+      bits[w0 >> 6] = m;
     }
   }
   for (int K = 0; K < nK; ++K) c->kb_voff[K + 1] = c->kb_voff[K] + nvk[K];
-  const int nva = c->kb_voff[nK];
-  c->kb_va.assign(nva, 0);
-  c->kb_vb.assign(nva, 0);
-  c->kb_nva.assign(nva, make_int4(0, 0, 0, 0));
-  {
-    std::vector<int> fill(c->kb_voff.begin(), c->kb_voff.end() - 1);
-    for (int av = 0; av < v; ++av) {
-      int* crow = civ.data() + (size_t)av * nK;
-      const int4 na = c->nv4[o + av];
-      for (int K = 0; K < nK; ++K) {
-        const int b = crow[K];
-        if (b < 0) continue;
-        const int e = fill[K]++;
-        c->kb_va[e] = o + av;
-        c->kb_vb[e] = b;
-        c->kb_nva[e] = make_int4(na.x, na.y, na.z, o + av);
-        crow[K] = e - c->kb_voff[K];  // now the entry of a in block K's list
-      }
-    }
-  }
+  c->nva = c->kb_voff[nK];
   c->noo_max = c->nvv_max = 0;
   for (int K = 0; K < c->nK; ++K) {
     c->noo_max = std::max(c->noo_max, c->kb_poff[K + 1] - c->kb_poff[K]);
     c->nvv_max = std::max(c->nvv_max, c->kb_voff[K + 1] - c->kb_voff[K]);
   }
   // K-blocked layout of the valid amplitudes: block K holds, for each entry ci
-  // of its virtual list, its noo pairs contiguously (position toff + ci*noo + p);
-  // cmap gives each compact index its element, this position and its partner
+  // of its virtual list, its noo pairs contiguously (position toff + ci*noo + p).
   // The valid elements in element order (their compact index there is the
-  // layout of the DIIS history), with the count before each virtual slice.
+  // layout of the DIIS history) start each virtual slice at slab_coff.
   c->kb_toff.assign(c->nK + 1, 0);
   for (int K = 0; K < c->nK; ++K)
     c->kb_toff[K + 1] = c->kb_toff[K] + (c->kb_voff[K + 1] - c->kb_voff[K]) * (c->kb_poff[K + 1] - c->kb_poff[K]);
-  c->nvalid = c->kb_toff[c->nK];
-  c->cmap.resize((size_t)c->nvalid + 1);
-  c->slab_coff.assign((size_t)v + 1, 0);
-  {
-    // per pair: its block, the block's pair count and the pair's first tk position
-    std::vector<int> pk(o * o), pn(o * o), pbase(o * o);
-    for (int pr = 0; pr < o * o; ++pr) {
-      const int K = c->pair_kb[pr];
-      pk[pr] = K;
-      pn[pr] = c->kb_poff[K + 1] - c->kb_poff[K];
-      pbase[pr] = c->kb_toff[K] + c->pair_pos[pr];
-    }
-    // With CoreN masking, the elements with a core index (i or j <= CoreN)
-    // leave the valid set: every masked integral of the reference multiplies
-    // such an amplitude or lands on such an element, so they stay exactly
-    // zero and the others see the unmasked terms (the frozen core).
-    const EriKind kind0 = eri_kind(s);
-    const int coren = core_masked(s, kind0) ? std::min(s.coren, o) : 0;
-    c->coren = coren;
-    c->core_elems.clear();
-    std::vector<char> core(o * o);
-    for (int pr = 0; pr < o * o; ++pr) core[pr] = (pr % o < coren || pr / o < coren) ? 1 : 0;
-    uint2* const cm = c->cmap.data();
-    int* const slab = c->slab_coff.data();
-    const int* const vb = c->kb_vb.data();
-    const int* const voff = c->kb_voff.data();
-    const int o2 = o * o;
-    int nc = 0;
-    for (int av = 0, x = 0; av < v; ++av) {
-      slab[av] = nc;
-      const int* crow = civ.data() + (size_t)av * nK;
-      if (coren) {
-        for (int pr = 0; pr < o2; ++pr, ++x) {
-          const int K = pk[pr], ci = crow[K];
-          if (ci < 0) continue;
-          if (core[pr]) {
-            c->core_elems.push_back(x);
-            continue;
-          }
-          cm[nc++] = elem_map(x, pbase[pr] + ci * pn[pr], vb[voff[K] + ci]);
-        }
-      } else {
-        // (branch-free: every element writes the next entry, the valid ones
-        // keep it; cmap has one spare entry for the last write)
-        for (int pr = 0; pr < o2; ++pr, ++x) {
-          const int K = pk[pr], ci = crow[K];
-          const int cj = ci < 0 ? 0 : ci;
-          cm[nc] = elem_map(x, pbase[pr] + cj * pn[pr], vb[voff[K] + cj]);
-          nc += ci >= 0;
-        }
-      }
-    }
-    slab[v] = nc;
-    c->cmap.resize(nc);
-    c->nvalid = nc;
-  }
+  c->nvalid = c->slab_coff[v];
+  // the valid elements with a core index (held at zero), for the input checks
+  c->core_elems.clear();
+  if (coren)
+    for_each_element(c, [c](int x, bool core) {
+      if (core) c->core_elems.push_back(x);
+    });
   // ladder work: per 16 pair rows of a block, one hole-hole unit (its W rows
   // and the hole-hole products of those rows) and the particle-particle
   // tiles of 16 a columns; the largest blocks' units first
@@ -444,32 +438,64 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     };
     return work(a.y) > work(b.y);
   });
-  // Coulomb table, flag 0 of the ERI variants (source/HEG.f90:729-772)
+  // Coulomb table, flag 0 of the ERI variants (source/HEG.f90:729-772), up to
+  // the largest |n_p - n_q|^2 of the basis.  |n_p - n_q| <= |n_p| + |n_q|, so
+  // with the plane waves in decreasing |n| the pairs of p can raise the
+  // maximum only while (|n_p| + |n_q|)^2 exceeds it; the scan stops at the
+  // first that cannot (d2 is an integer: a bound below best + 1 is enough,
+  // which also covers the rounding of the square roots).
   int n2max = 0;
-  for (int p = 0; p < nao; ++p) {
-    const int4 a = c->nv4[p];
-    int m = 0;
-    for (int q = p + 1; q < nao; ++q) {
-      const int4 b = c->nv4[q];
-      const int dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-      m = std::max(m, dx * dx + dy * dy + dz * dz);
+  {
+    thread_local std::vector<int> byn, cnt;
+    thread_local std::vector<double> rn;
+    int n2top = 0;
+    for (int p = 0; p < nao; ++p) {
+      const int4 a = c->nv4[p];
+      n2top = std::max(n2top, a.x * a.x + a.y * a.y + a.z * a.z);
     }
-    n2max = std::max(n2max, m);
+    cnt.assign(n2top + 2, 0);  // counting sort by |n|^2, largest first
+    for (int p = 0; p < nao; ++p) {
+      const int4 a = c->nv4[p];
+      ++cnt[n2top - (a.x * a.x + a.y * a.y + a.z * a.z) + 1];
+    }
+    for (int k = 1; k <= n2top + 1; ++k) cnt[k] += cnt[k - 1];
+    byn.assign(nao, 0);
+    rn.assign(nao, 0.0);
+    for (int p = 0; p < nao; ++p) {
+      const int4 a = c->nv4[p];
+      const int e = cnt[n2top - (a.x * a.x + a.y * a.y + a.z * a.z)]++;
+      byn[e] = p;
+      rn[e] = std::sqrt((double)(a.x * a.x + a.y * a.y + a.z * a.z));
+    }
+    for (int u = 0; u < nao; ++u) {
+      if ((2.0 * rn[u]) * (2.0 * rn[u]) < n2max + 1.0 - 1e-6) break;
+      const int4 a = c->nv4[byn[u]];
+      for (int t = 0; t < nao; ++t) {
+        if ((rn[u] + rn[t]) * (rn[u] + rn[t]) < n2max + 1.0 - 1e-6) break;
+        const int4 b = c->nv4[byn[t]];
+        const int dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        n2max = std::max(n2max, dx * dx + dy * dy + dz * dz);
+      }
+    }
   }
   if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
   // occurrence over (p, i), for the quadratic ring products
   {
-    // (a counting sort: transfer ids and counts in one pass, rows in another)
-    qid.assign((size_t)w * w * w, -1);   // block of transfer q by box index
-    std::vector<int> qlin, qcount;
-    qlin.reserve((size_t)o * v);
-    qcount.reserve((size_t)o * v);
+    // (a counting sort: transfer ids and counts in one pass, rows in another;
+    // transfer ids looked up in a box of half-width max|n| + mo)
+    const int hq = mx + mo, wq = 2 * hq + 1, cq = hq * (1 + wq + wq * wq);
+    qid.assign((size_t)wq * wq * wq, -1);   // block of transfer q by box index
+    thread_local std::vector<int> qlin, qcount, lq;
+    lq.resize(nao);
+    for (int p = 0; p < nao; ++p) lq[p] = c->nv4[p].x + wq * c->nv4[p].y + wq * wq * c->nv4[p].z;
+    qlin.clear();
+    qcount.clear();
     c->qb_of.assign((size_t)o * v, -1);
     for (int pv = 0; pv < v; ++pv)
       for (int i = 0; i < o; ++i) {
-        const int ql = lin[o + pv] - lin[i];
-        int& q = qid[center + ql];
+        const int ql = lq[o + pv] - lq[i];
+        int& q = qid[cq + ql];
         if (q < 0) {
           q = (int)qlin.size();
           qlin.push_back(ql);
@@ -492,15 +518,10 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
           c->qb_rows[r] = i + o * pv;
         }
     }
+    // (the partner slices d_q(l) of each transfer, qb_d, are looked up on the
+    // device at placement)
     c->qb_neg.assign(nq, -1);
-    c->qb_d.assign((size_t)nq * o, -1);
-    for (int q = 0; q < nq; ++q) {
-      c->qb_neg[q] = qid[center - qlin[q]];
-      for (int l = 0; l < o; ++l) {
-        const int p = box[center + lin[l] + qlin[q]];
-        c->qb_d[(size_t)q * o + l] = p >= o ? p - o : -1;
-      }
-    }
+    for (int q = 0; q < nq; ++q) c->qb_neg[q] = qid[cq - qlin[q]];
     c->qb_info.assign(nq, make_int4(0, 0, 0, 0));
     for (int q = 0; q < nq; ++q) {
       const int qn = c->qb_neg[q];
@@ -512,6 +533,14 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     // (first: they read the most data per wave), 16x16 tiles of the
     // per-transfer ring products
     c->con_items.clear();
+    {
+      size_t n = s.do_mosaic ? kJooParts : 0;
+      if (s.do_ring || s.do_xring)
+        for (int q = 0; q < nq; ++q)
+          n += (size_t)((c->qb_info[q].y + 15) / 16) * ((c->qb_info[q].w + 15) / 16) *
+               ((s.do_ring ? 1 : 0) + (s.do_xring ? 1 : 0));
+      c->con_items.reserve(n);
+    }
     if (s.do_mosaic)
       for (int p = 0; p < kJooParts; ++p) c->con_items.push_back(make_int4(0, CK_JOO, p, 0));
     if (s.do_ring || s.do_xring)
@@ -608,6 +637,11 @@ static int main_nq_of(const SolveDev& S) {
   return (S.mode == UEGCCD_MODE_DIIS && S.ndiis > 12) ? 32 : 16;
 }
 
+// Scratch of the device-built maps: the momentum box and the partner table.
+static size_t plan_scratch_ints(const uegccd_ctx* c) {
+  return (size_t)c->box_w * c->box_w * c->box_w + (size_t)c->v * c->nK;
+}
+
 size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   const size_t n3 = c->n3;
   const size_t eb = elem_blocks_of(c);
@@ -616,13 +650,14 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double) * c->vtab.size());                    // vtab
   bytes += aligned(sizeof(double) * c->voo.size()) + aligned(sizeof(double) * c->vso.size());
   bytes += 3 * aligned(sizeof(int) * (c->nK + 1)) + aligned(sizeof(int) * c->kb_pairs.size());
-  bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->kb_va.size()));
-  bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->kb_nva.size()));
+  bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->nva));   // kb_va, kb_vb
+  bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->nva));      // kb_nva
   bytes += 2 * aligned(sizeof(int) * c->o * c->o);
   bytes += aligned(sizeof(double2) * n3) * 3;                         // tcur, gout, rout
   bytes += 2 * aligned(sizeof(double2) * std::max<size_t>(1, (size_t)c->nvalid * ndiis));  // histT, histE
   bytes += aligned(sizeof(int) * (c->v + 1));                          // slab_coff
-  bytes += aligned(sizeof(uint2) * std::max<size_t>(1, c->cmap.size()));   // cmap
+  bytes += aligned(sizeof(uint2) * std::max<size_t>(1, c->nvalid));      // cmap
+  bytes += aligned(sizeof(int) * plan_scratch_ints(c));                // box, civ (launch_plan)
   bytes += aligned(sizeof(double2) * std::max<size_t>(1, c->kb_toff.back()));  // tk
   bytes += aligned(sizeof(int) * (c->nK + 1));                         // kb_toff
   bytes += aligned(sizeof(double2) * n3);                              // lad
@@ -640,7 +675,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
            aligned(sizeof(int4) * std::max<size_t>(1, c->hh_items.size()));
   bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
-           aligned(sizeof(int) * std::max<size_t>(1, c->qb_d.size()));
+           aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size() * (size_t)c->o));  // qb_d
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->con_items.size()));
   bytes += aligned(sizeof(int4) * std::max<size_t>(1, c->qb_info.size()));
   bytes += 2 * aligned(sizeof(int) * std::max<size_t>(1, c->qrow.size()));  // qrow, qb_of
@@ -693,11 +728,10 @@ int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
   for (int x : c->core_elems)
     if ((tab && tab[x] != 0.0) || (tba && tba[x] != 0.0)) return UEGCCD_ERR_INVALID;
   std::vector<double> h(2 * (size_t)c->n3, 0.0);
-  for (const uint2& m : c->cmap) {
-    const size_t x = (size_t)m.x;
-    h[2 * x] = tab ? tab[x] : 0.0;
-    h[2 * x + 1] = tba ? tba[x] : 0.0;
-  }
+  for_each_valid(c, [&](int x) {
+    h[2 * (size_t)x] = tab ? tab[x] : 0.0;
+    h[2 * (size_t)x + 1] = tba ? tba[x] : 0.0;
+  });
   HIPCHK(hipMemcpyAsync(c->sd.tcur, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -786,6 +820,23 @@ int plan_ctx(const uegccd_system* sys, const uegccd_options* opt_in, uegccd_ctx*
 
 int ndiis_of(const uegccd_options& opt) { return opt.mode == UEGCCD_MODE_DIIS ? opt.ndiis : 1; }
 
+// An upper bound of the bytes of a context's constant section (each array of
+// layout_ctx's image, rounded up to the arena's 256-byte alignment).
+size_t const_section_bound(const uegccd_ctx* c) {
+  size_t b = 0;
+  auto add = [&b](size_t bytes) { b += (std::max<size_t>(bytes, 1) + 255) & ~size_t(255); };
+  add(sizeof(int4) * c->nv4.size());
+  add(sizeof(double) * c->eig.size());
+  add(sizeof(double) * (c->vtab.size() + c->voo.size() + c->vso.size()));
+  add(sizeof(int) * (c->kb_poff.size() + c->kb_pairs.size() + c->kb_voff.size() + c->kb_toff.size()));
+  add(sizeof(int) * (c->slab_coff.size() + c->pair_kb.size() + c->pair_pos.size()));
+  add(sizeof(int) * (c->qb_roff.size() + c->qb_rows.size() + c->qb_neg.size() + c->qrow.size() +
+                     c->qb_of.size()));
+  add(sizeof(int4) * (c->qb_info.size() + c->hh_items.size() + c->lad_items.size() + c->con_items.size()));
+  add(sizeof(SolveDev) + sizeof(int) * (8 + 2 * kSliceClasses));
+  return b + 24 * 256;  // (one alignment pad per array)
+}
+
 // Lays a planned context out in `bytes` (= arena_bytes) of device memory at
 // `base`: the maps, the device descriptor and the batch-of-one tables form a
 // constant section written into `image` (uploaded by the caller in one copy),
@@ -804,7 +855,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   c->arena.used = 0;
   if (fill) {
     image.clear();
-    image.reserve(bytes);  // one allocation: the constant section is a prefix of the arena
+    image.reserve(const_section_bound(c));  // one allocation for the constant section
   }
   ArenaImage im{&c->arena, &image, fill};
 
@@ -852,20 +903,15 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.kb_poff = im.put(c->kb_poff);
   S.kb_pairs = im.put(c->kb_pairs);
   S.kb_voff = im.put(c->kb_voff);
-  S.kb_va = im.put(c->kb_va);
-  S.kb_vb = im.put(c->kb_vb);
-  S.kb_nva = im.put(c->kb_nva);
   S.kb_toff = im.put(c->kb_toff);
   S.slab_coff = im.put(c->slab_coff);
-  S.cmap = im.put(c->cmap);
-  S.nvalid = (int)c->cmap.size();
+  S.nvalid = (int)c->nvalid;
   S.pair_kb = im.put(c->pair_kb);
   S.pair_pos = im.put(c->pair_pos);
   S.nqb = (int)c->qb_neg.size();
   S.qb_roff = im.put(c->qb_roff);
   S.qb_rows = im.put(c->qb_rows);
   S.qb_neg = im.put(c->qb_neg);
-  S.qb_d = im.put(c->qb_d);
   S.qb_info = im.put(c->qb_info);
   S.qrow = im.put(c->qrow);
   S.qb_of = im.put(c->qb_of);
@@ -912,9 +958,16 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.tiles = c->tiles;
   b.main_nq = main_nq_of(S);
   b.ncon = S.ncon;
-  // work buffers
+  // work buffers (cleared at placement), the device-built maps first
   c->arena.used = aligned(c->arena.used);
   *work_off = c->arena.used;
+  S.kb_va = c->arena.take<int>(std::max(1, c->nva));
+  S.kb_vb = c->arena.take<int>(std::max(1, c->nva));
+  S.kb_nva = c->arena.take<int4>(std::max(1, c->nva));
+  S.cmap = c->arena.take<uint2>(std::max<int64_t>(1, c->nvalid));
+  S.qb_d = c->arena.take<int>(std::max<size_t>(1, (size_t)S.nqb * c->o));
+  c->d_box = c->arena.take<int>(plan_scratch_ints(c));
+  c->d_civ = c->d_box + (size_t)c->box_w * c->box_w * c->box_w;
   S.tcur = c->arena.take<double2>(n3);
   S.tk = c->arena.take<double2>(std::max<int64_t>(1, c->kb_toff[c->nK]));
   S.histT = c->arena.take<double2>(std::max<size_t>(1, (size_t)c->nvalid * ndiis));
@@ -941,6 +994,18 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   im.write(c->d_sd, &S, 1);
 }
 
+// The device-built maps of a placed context (its work buffers cleared).
+void launch_plan_maps(const uegccd_ctx* c, hipStream_t s) {
+  PlanArgs p;
+  p.sd = c->d_sd;
+  p.w = c->box_w;
+  p.center = c->box_center;
+  p.coren = c->coren;
+  p.box = c->d_box;
+  p.civ = c->d_civ;
+  launch_plan(p, c->o, c->v, c->nao, c->nK, (int)c->qb_neg.size(), s);
+}
+
 // Puts a planned context on its device in memory of its own: one allocation,
 // one copy of the constant section, one clear of the work buffers.  Deletes c
 // on failure.
@@ -965,8 +1030,12 @@ int place_ctx(uegccd_ctx* c, const uegccd_system* sys, uegccd_ctx** out) {
   }
   c->owns_stream = true;
   if (hipMemcpyAsync(base, image.data(), image.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipMemsetAsync(base + work_off, 0, bytes - work_off, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess) {
+      hipMemsetAsync(base + work_off, 0, bytes - work_off, c->stream) != hipSuccess) {
+    uegccd_ctx_destroy(c);
+    return UEGCCD_ERR_HIP;
+  }
+  launch_plan_maps(c, c->stream);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
     uegccd_ctx_destroy(c);
     return UEGCCD_ERR_HIP;
   }
@@ -1042,13 +1111,12 @@ int uegccd_residual(uegccd_ctx* c, const double* tab, const double* tba, double*
   // the valid elements' values; zeros elsewhere
   for (double* out : {gab, gba, rab, rba})
     if (out) std::fill(out, out + n3, 0.0);
-  for (const uint2& m : c->cmap) {
-    const size_t x = (size_t)m.x;
+  for_each_valid(c, [&](int x) {
     if (gab) gab[x] = h[x];
     if (gba) gba[x] = h[n3 + x];
     if (rab) rab[x] = h[2 * n3 + x];
     if (rba) rba[x] = h[3 * n3 + x];
-  }
+  });
   return UEGCCD_OK;
 }
 
@@ -1098,7 +1166,7 @@ int structure_factor_cells(uegccd_ctx* c, const double* tab, const double* tba, 
     if (key > last[qd]) { last[qd] = key; last_core[qd] = core; }
     if (key + 1 > last[qx]) { last[qx] = key + 1; last_core[qx] = core; }
   };
-  for (const uint2& m : c->cmap) bin((int)m.x, false);
+  for_each_valid(c, [&](int x) { bin(x, false); });
   for (int x : c->core_elems) bin(x, true);
   struct Cell { int x, y, z, q; };
   std::vector<Cell> cells;
@@ -1600,13 +1668,12 @@ int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
   // the valid elements' amplitudes; zeros elsewhere
   for (double* out : {taa, tab, tba})
     if (out) std::fill(out, out + n3, 0.0);
-  for (const uint2& m : c->cmap) {
-    const size_t x = (size_t)m.x;
-    const double a = h[2 * x], b = h[2 * x + 1];
+  for_each_valid(c, [&](int x) {
+    const double a = h[2 * (size_t)x], b = h[2 * (size_t)x + 1];
     if (tab) tab[x] = a;
     if (tba) tba[x] = b;
     if (taa) taa[x] = a + b;
-  }
+  });
   return UEGCCD_OK;
 }
 
@@ -1677,7 +1744,7 @@ extern "C" int uegccd_drpa(uegccd_ctx* c, double* t2, uegccd_result* res, double
     HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     std::fill(t2, t2 + n3, 0.0);
-    for (const uint2& m : c->cmap) t2[m.x] = h[2 * (size_t)m.x];
+    for_each_valid(c, [&](int x) { t2[x] = h[2 * (size_t)x]; });
   }
   return status;
 }
@@ -1855,6 +1922,27 @@ double madelung_ewald(double omega, double L) {
   return real + recip + term2 + term4;
 }
 
+// The Ewald sum depends on (Omega, L) alone, and a twist-averaging or r_s-scan
+// stream repeats few of those: each pair's value is kept for the process (a
+// few thousand at most), and a repeated pair gets the same bits.
+double madelung_cached(double omega, double L) {
+  static std::mutex mu;
+  static std::map<std::pair<uint64_t, uint64_t>, double> memo;
+  uint64_t ko, kl;
+  std::memcpy(&ko, &omega, sizeof ko);
+  std::memcpy(&kl, &L, sizeof kl);
+  const auto key = std::make_pair(ko, kl);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+  }
+  const double v = madelung_ewald(omega, L);
+  std::lock_guard<std::mutex> lk(mu);
+  if (memo.size() < 4096) memo.emplace(key, v);
+  return v;
+}
+
 // Geometry at r_s = 1, then rescaled to r_s as change_rs does
 // (source/HEG.f90:494-533), and the Madelung constant.
 struct HegGeom {
@@ -1867,7 +1955,7 @@ HegGeom heg_geometry(const uegccd_heg_params* p) {
   g.rescale = p->rs / 1.0;
   g.L = g.L1 * g.rescale;
   g.omega = std::pow(g.L1, 3) * std::pow(g.rescale, 3.0);
-  g.mad = madelung_ewald(g.omega, g.L) * p->madelungfactor;
+  g.mad = madelung_cached(g.omega, g.L) * p->madelungfactor;
   return g;
 }
 
@@ -1894,10 +1982,10 @@ int fcut_count(double f, int nel) {
   return m;
 }
 
-}  // namespace
-
-extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, int32_t* nvec,
-                                double* eigen, double* scalars) {
+// The basis builder: out(nb) gives the n-vector (nb*3, column-major) and
+// eigenvalue buffers for the nb orbitals found (either may be null).
+template <typename Out>
+int heg_build_core(const uegccd_heg_params* p, int32_t* nao_out, Out out, double* scalars) {
   if (!p || !nao_out || p->nel <= 0 || (p->nel % 2) != 0 || p->maxkpoint < 0 || !(p->rs > 0.0) ||
       (p->cutoff_twisted != 0 && p->cutoff_twisted != 1))
     return UEGCCD_ERR_INVALID;
@@ -1927,6 +2015,9 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
   nb = nb - mc + no;
   if (nb <= no) return UEGCCD_ERR_INVALID;
   *nao_out = nb;
+  int32_t* nvec;
+  double* eigen;
+  std::tie(nvec, eigen) = out(nb);
   if (!nvec && !eigen && !scalars) return UEGCCD_OK;
   const HegGeom g = heg_geometry(p);
   const double L1 = g.L1, rescale = g.rescale, L = g.L, omega = g.omega, mad = g.mad;
@@ -1970,6 +2061,13 @@ extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, in
     heg_scalars(p, g, ehf, scalars);
   }
   return UEGCCD_OK;
+}
+
+}  // namespace
+
+extern "C" int uegccd_heg_build(const uegccd_heg_params* p, int32_t* nao_out, int32_t* nvec,
+                                double* eigen, double* scalars) {
+  return heg_build_core(p, nao_out, [&](int) { return std::make_pair(nvec, eigen); }, scalars);
 }
 
 // print_ex_SF (source/HEG.f90:617-674), which print_HF writes to unit 82
@@ -2133,6 +2231,9 @@ struct uegccd_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request*> to_plan;    // guarded by mu
+  // uploaded constant images kept for the planners to lay the next ones out
+  // in (memory already mapped: no page faults), at most lookahead of them (mu)
+  std::vector<std::vector<char>> spare_images;
   int planned_waiting = 0;         // claimed for planning, not yet admitted (mu)
   int lookahead = 128;
   bool stopping = false;           // mu
@@ -2175,12 +2276,12 @@ void finish_request(uegccd_server* s, Request& r, int status) {
 int plan_request(Request& r) {
   const uegccd_request& q = r.req;
   int32_t nao = 0;
-  int rc = uegccd_heg_build(&q.heg, &nao, nullptr, nullptr, nullptr);
-  if (rc) return rc;
-  r.nvec.assign((size_t)nao * 3, 0);
-  r.eigen.assign(nao, 0.0);
   double sc[8];
-  rc = uegccd_heg_build(&q.heg, &nao, r.nvec.data(), r.eigen.data(), sc);
+  int rc = heg_build_core(&q.heg, &nao, [&r](int nb) {
+    r.nvec.assign((size_t)nb * 3, 0);
+    r.eigen.assign(nb, 0.0);
+    return std::make_pair(r.nvec.data(), r.eigen.data());
+  }, sc);
   if (rc) return rc;
   uegccd_system& s = r.sys;
   s.nocc = q.heg.nel / 2;
@@ -2207,7 +2308,7 @@ int plan_request(Request& r) {
   // the constant section at a placeholder base: admission writes the descriptor
   size_t work_off = 0;
   layout_ctx(r.ctx, &r.sys, nullptr, (size_t)r.footprint, r.image, &work_off, true);
-  r.image.shrink_to_fit();
+
   return UEGCCD_OK;
 }
 
@@ -2225,6 +2326,10 @@ void planner_loop(uegccd_server* s) {
       int expect = PLAN_NONE;
       if (!r->plan_state.compare_exchange_strong(expect, PLAN_BUSY)) continue;
       ++s->planned_waiting;
+      if (!s->spare_images.empty()) {
+        r->image.swap(s->spare_images.back());
+        s->spare_images.pop_back();
+      }
     }
     r->plan_rc = plan_request(*r);
     {
@@ -2314,9 +2419,14 @@ int admit(uegccd_server* s) {
     layout_ctx(c, &r.sys, mem, fp, r.image, &work_off, false);  // the planner built the image
     r.placed = true;
     int rc = upload_image(mem, r.image, s->stream, &s->stage);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);  // (the staged copy is independent of the image)
+      if ((int)s->spare_images.size() < s->lookahead) s->spare_images.push_back(std::move(r.image));
+    }
     std::vector<char>().swap(r.image);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(mem + work_off, 0, fp - work_off, s->stream));
+    launch_plan_maps(c, s->stream);
     const int prec64 = r.ctx->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
     launch_mp2(r.ctx->batch, s->stream);
     launch_init(r.ctx->batch, s->stream);
@@ -2632,6 +2742,7 @@ extern "C" int uegccd_solve_noalloc(const uegccd_system* sys, const uegccd_optio
   // last call's contents are still there)
   HIPCHK(hipMemcpyAsync(base, image.data(), image.size(), hipMemcpyHostToDevice, c->stream));
   launch_zero(base + work_off, bytes - work_off, c->stream);
+  launch_plan_maps(c, c->stream);
   HIPCHK(hipGetLastError());
   rc = copy_in_amplitudes(c, tab, tba);
   if (rc) return rc;

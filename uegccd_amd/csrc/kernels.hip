@@ -2310,6 +2310,126 @@ void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s)
   hipLaunchKernelGGL(k_state_reset, dim3(b.n), dim3(64), 0, s, b.solves, phase, prec64);
 }
 
+// --------------------------------------------------------------------------
+// Maps of a context on the device (launch_plan).  The box index of momentum n
+// is center + n.x + w n.y + w^2 n.z; every lookup n_p + n_q - n_r of these
+// kernels, n_i + n_j - n_a and n_l + n_p - n_i with i, j, l occupied, stays
+// inside the box (its half-width is max|n| + 2 max|n_occupied| + 1).
+__device__ __forceinline__ int box_lin(int4 n, int w) { return n.x + w * n.y + w * w * n.z; }
+
+// Exclusive prefix of `flag` over a 256-thread block in thread order (wave
+// ballots, then the four wave counts); *total receives the block's count.
+__device__ __forceinline__ int block_rank(bool flag, int* wsum, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(flag);
+  const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[wave] = __popcll(m);
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k < wave) before += wsum[k];
+    all += wsum[k];
+  }
+  __syncthreads();  // wsum is written again by the next call
+  *total = all;
+  return before + in_wave;
+}
+
+// The box: orbital p + 1 at the box index of n_p.
+__global__ __launch_bounds__(256) void k_plan_box(PlanArgs P) {
+  const SolveDev& S = *P.sd;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < S.nao) P.box[P.center + box_lin(S.nv4[p], P.w)] = p + 1;
+}
+
+// Blocks [0, nK): block K's virtual list, the virtuals a whose partner
+// b = K - n_a is a virtual orbital (source/CCD.f90: FindIndex(i,j,a) > NOcc),
+// in a order, and civ[a][K].  The rest: qb_d[q][l], the virtual index of
+// n_l + q (or -1), one thread per (q, l).
+__global__ __launch_bounds__(256) void k_plan_blocks(PlanArgs P) {
+  __shared__ int wsum[4];
+  const SolveDev& S = *P.sd;
+  const int o = S.o, v = S.v, nK = S.nK, w = P.w;
+  if ((int)blockIdx.x < nK) {
+    const int K = blockIdx.x;
+    const int x0 = S.kb_pairs[S.kb_poff[K]];
+    const int kb = P.center + box_lin(S.nv4[x0 % o], w) + box_lin(S.nv4[x0 / o], w);
+    const int voff = S.kb_voff[K];
+    int* const va = const_cast<int*>(S.kb_va);
+    int* const vb = const_cast<int*>(S.kb_vb);
+    int4* const nva = const_cast<int4*>(S.kb_nva);
+    int run = 0;
+    for (int a0 = 0; a0 < v; a0 += 256) {
+      const int av = a0 + threadIdx.x;
+      int b = -1;
+      int4 na = make_int4(0, 0, 0, 0);
+      if (av < v) {
+        na = S.nv4[o + av];
+        b = P.box[kb - box_lin(na, w)] - 1;
+      }
+      const bool valid = av < v && b >= o;
+      int tot;
+      const int rank = block_rank(valid, wsum, &tot);
+      if (valid) {
+        const int e = voff + run + rank;
+        va[e] = o + av;
+        vb[e] = b;
+        nva[e] = make_int4(na.x, na.y, na.z, o + av);
+      }
+      if (av < v) P.civ[(size_t)av * nK + K] = valid ? run + rank : -1;
+      run += tot;
+    }
+    return;
+  }
+  const int t = (blockIdx.x - nK) * blockDim.x + threadIdx.x;
+  if (t >= S.nqb * o) return;
+  const int q = t / o, l = t - q * o;
+  const int r = S.qb_rows[S.qb_roff[q]];  // a pair (i, p) of the transfer: q = n_p - n_i
+  const int i = r % o, pv = r / o;
+  const int ql = box_lin(S.nv4[o + pv], w) - box_lin(S.nv4[i], w);
+  const int p = P.box[P.center + box_lin(S.nv4[l], w) + ql] - 1;
+  const_cast<int*>(S.qb_d)[t] = p >= o ? p - o : -1;
+}
+
+// One block per virtual slice a: its valid elements (i, j, a) in element
+// order, pair x = i + o j ascending, from slab_coff[a] on: the element, its
+// K-blocked position and its partner.  Elements with a core index (i or j <
+// coren, the CoreN masking) are left out.
+__global__ __launch_bounds__(256) void k_plan_cmap(PlanArgs P) {
+  __shared__ int wsum[4];
+  const SolveDev& S = *P.sd;
+  const int o = S.o, o2 = o * o, nK = S.nK, av = blockIdx.x;
+  const int base = S.slab_coff[av];
+  const int* civ = P.civ + (size_t)av * nK;
+  uint2* const cmap = const_cast<uint2*>(S.cmap);
+  int run = 0;
+  for (int p0 = 0; p0 < o2; p0 += 256) {
+    const int pr = p0 + threadIdx.x;
+    int K = 0, ci = -1;
+    if (pr < o2) {
+      K = S.pair_kb[pr];
+      ci = civ[K];
+      if (pr % o < P.coren || pr / o < P.coren) ci = -1;
+    }
+    int tot;
+    const int rank = block_rank(ci >= 0, wsum, &tot);
+    if (ci >= 0) {
+      const int noo = S.kb_poff[K + 1] - S.kb_poff[K];
+      const int pos = S.kb_toff[K] + ci * noo + S.pair_pos[pr];
+      cmap[base + run + rank] = elem_map(pr + o2 * av, pos, S.kb_vb[S.kb_voff[K] + ci]);
+    }
+    run += tot;
+  }
+}
+
+void launch_plan(const PlanArgs& p, int o, int v, int nao, int nK, int nq, hipStream_t s) {
+  hipLaunchKernelGGL(k_plan_box, dim3((nao + 255) / 256), dim3(256), 0, s, p);
+  const int qblocks = (nq * o + 255) / 256;
+  hipLaunchKernelGGL(k_plan_blocks, dim3(nK + qblocks), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(k_plan_cmap, dim3(v), dim3(256), 0, s, p);
+}
+
 __global__ void k_zero(double* __restrict__ p, size_t n) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
     p[k] = 0.0;

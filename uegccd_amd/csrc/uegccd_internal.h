@@ -300,6 +300,24 @@ void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
 // kernel of its own: the ladder tiles run in k_contract).
 void launch_one(const BatchDev& b, int which, hipStream_t s);
 
+// Maps of a context built on the device at placement, after its work buffers
+// are cleared (the reference finds every momentum partner with FindIndex,
+// source/HEG.f90:357-404; here one momentum box answers all of them):
+//   box  [w^3]     orbital p + 1 at box index center + n.x + w n.y + w^2 n.z
+//                  of every plane wave of the basis, 0 elsewhere (zero on entry)
+//   civ  [v * nK]  entry of virtual a in block K's virtual list, or -1 (scratch)
+// and, into the descriptor's arrays: each pair-momentum block's virtual list
+// (kb_va, kb_vb, kb_nva), the element map cmap of the valid elements in
+// element order (slab_coff gives each slice's first compact index), and the
+// partner slices qb_d of the transfer sets.
+struct PlanArgs {
+  const SolveDev* sd;  // the context's device descriptor
+  int w, center, coren;
+  int* box;
+  int* civ;
+};
+void launch_plan(const PlanArgs& p, int o, int v, int nao, int nK, int nq, hipStream_t s);
+
 // Clears `bytes` (a multiple of 8) of device memory at p with a fill kernel
 // (no memset node: usable where a caller captures the stream).
 void launch_zero(void* p, size_t bytes, hipStream_t s);
