@@ -305,7 +305,11 @@ def twist_stream(lib, L, n, rate):
                 "max_abs_dE_vs_fp64": float(de.max()) if de.size else None,
                 "p99_abs_dE_vs_fp64": float(np.percentile(de, 99)) if de.size else None,
                 "converged": good, "fp32_iters": int(sum(r.iters_fp32 for r in out)),
-                "iters": int(sum(r.niter for r in out))}
+                "iters": int(sum(r.niter for r in out)),
+                # host time a planner thread spent on each request (basis, maps, image)
+                "plan_ms_mean": float(np.mean([r.plan_ms for r in out])),
+                "plan_ms_p50": float(np.median([r.plan_ms for r in out])),
+                "plan_ms_max": float(np.max([r.plan_ms for r in out]))}
 
     batch_wall, batch_out = served(reqs, np.zeros(n), "mixed", Server, options, max_active=n)
     srv_wall, srv_out = served(reqs, arrivals, "mixed", Server, options)

@@ -374,6 +374,8 @@ typedef struct {
     double ehf, e2, ecorr, maxres;
     double t_submit_ms, t_start_ms, t_done_ms;   /* server clock                 */
     int64_t footprint_bytes;
+    double plan_ms;            /* host time spent planning the request (basis,
+                                  maps, constant image) on a planner thread    */
 } uegccd_request_result;
 
 int uegccd_server_create(const uegccd_server_options* opt, uegccd_server** out);

@@ -2151,6 +2151,7 @@ struct Request {
   int32_t nao = 0;                 // results of planning, copied into res on admission
   double ehf = 0.0;
   int64_t footprint = 0;
+  double plan_ms = 0.0;            // host time of plan_request
   std::vector<char> image;         // constant section laid out by the planner (descriptor at admission)
 };
 
@@ -2331,7 +2332,9 @@ void planner_loop(uegccd_server* s) {
         s->spare_images.pop_back();
       }
     }
+    const auto t0 = Clock::now();
     r->plan_rc = plan_request(*r);
+    r->plan_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     {
       std::lock_guard<std::mutex> lk(s->mu);
       r->plan_state.store(PLAN_DONE, std::memory_order_release);
@@ -2353,7 +2356,9 @@ bool head_planned(uegccd_server* s, Request& r) {
         std::lock_guard<std::mutex> lk(s->mu);
         ++s->planned_waiting;
       }
+      const auto t0 = Clock::now();
       r.plan_rc = plan_request(r);
+      r.plan_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
       r.plan_state.store(PLAN_DONE, std::memory_order_release);
       return true;
     }
@@ -2388,6 +2393,7 @@ int admit(uegccd_server* s) {
     r.res.nocc = r.req.heg.nel / 2;
     r.res.ehf = r.ehf;
     r.res.footprint_bytes = r.footprint;
+    r.res.plan_ms = r.plan_ms;
     if (r.plan_rc) {
       finish_request(s, r, r.plan_rc);
       release_plan_slot(s);

@@ -354,6 +354,7 @@ class RequestResult:
     t_start_ms: float
     t_done_ms: float
     footprint_bytes: int
+    plan_ms: float
 
     @property
     def latency_ms(self):

@@ -57,6 +57,7 @@ Module Wrappers
     Integer(c_int32_t) :: state, status, nao, nocc, niter, iters_fp32
     Real(c_double) :: ehf, e2, ecorr, maxres, t_submit_ms, t_start_ms, t_done_ms
     Integer(c_int64_t) :: footprint_bytes
+    Real(c_double) :: plan_ms
   End Type uegccd_request_result
 
   Interface

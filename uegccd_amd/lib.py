@@ -98,7 +98,7 @@ class RequestResult(ctypes.Structure):
                 ("ehf", ctypes.c_double), ("e2", ctypes.c_double), ("ecorr", ctypes.c_double),
                 ("maxres", ctypes.c_double), ("t_submit_ms", ctypes.c_double),
                 ("t_start_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double),
-                ("footprint_bytes", ctypes.c_int64)]
+                ("footprint_bytes", ctypes.c_int64), ("plan_ms", ctypes.c_double)]
 
 
 class UegccdError(RuntimeError):
