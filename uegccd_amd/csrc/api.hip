@@ -1472,7 +1472,11 @@ int next_chunk(uegccd_ctx** cs, int n, const std::vector<State>& st, std::vector
 // batch, where the slice blocks of its large solves would re-read hundreds
 // of KB of partials each (a finalize launch per pass: batched stream 1,278
 // -> 1,365 solves/s).
-bool fold_state_update(int nsolves) { return nsolves == 1; }
+
+bool fold_state_update(int nsolves, const uegccd_ctx* c0) {
+  // (the slice blocks reduce partials of DIIS histories up to 12 entries)
+  return nsolves == 1 && !(c0->opt.mode == UEGCCD_MODE_DIIS && c0->opt.ndiis > 12);
+}
 
 int chunk_graph(uegccd_ctx* c, int len, hipGraphExec_t* out) {
   auto it = c->graphs.find(len);
@@ -1483,8 +1487,12 @@ int chunk_graph(uegccd_ctx* c, int len, hipGraphExec_t* out) {
   prepare_launches();  // per-kernel attributes are set outside the capture
   hipStream_t s = c->stream;
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < len; ++k) launch_iteration(c->batch, MF_ITERATE, s, nullptr, nullptr, k > 0);
-  launch_finalize(c->batch, FIN_ITER, s);
+  const bool fold = fold_state_update(1, c);
+  for (int k = 0; k < len; ++k) {
+    launch_iteration(c->batch, MF_ITERATE, s, nullptr, nullptr, fold && k > 0);
+    if (!fold) launch_finalize(c->batch, FIN_ITER, s);
+  }
+  if (fold) launch_finalize(c->batch, FIN_ITER, s);
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s, &g);
   if (ec != hipSuccess || !g) {
@@ -1582,7 +1590,7 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     // (a single solve's k_slice runs the reductions of the pass before it,
     // and the chunk's last pass has a finalize launch of its own; a batch
     // has a finalize launch after every pass)
-    const bool fold = fold_state_update(n);
+    const bool fold = fold_state_update(n, c0);
     for (int k = 0; k < chunk && !graphs && !persist; ++k) {
       if (timing) {
         const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
@@ -2547,7 +2555,7 @@ int uegccd_server_step(uegccd_server* s, int32_t* n_active, int32_t* n_queued) {
       s->batch_stale = false;
       s->nbatch = cs.size();
     }
-    const bool fold = fold_state_update((int)s->nbatch);
+    const bool fold = false;  // (a batch: a finalize launch after every pass)
     for (int k = 0; k < s->opt.iters_per_step; ++k) {
       launch_iteration(s->batch.b, MF_ITERATE, s->stream, nullptr, nullptr, fold && k > 0);
       if (!fold) launch_finalize(s->batch.b, FIN_ITER, s->stream);

@@ -519,12 +519,13 @@ __device__ __forceinline__ void slab_place(const SolveG& S, double* Tab, double*
 // previous pass (finalize_state, the same in every block; the block of slice
 // 0 writes the new state to st2), with fvals, fred ((BT/64)*32 doubles) and
 // fsst as its LDS.
+template <int MAXQ>
 __device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int prec64, double* vals, double* red,
                                               State& sst, bool write_trace);
 __device__ __forceinline__ void put_state(const State& sst, GlobalPtr<State> to);
 
 template <int BT, bool SPLIT>
-__device__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+__device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
                             int n, int bid, Lds& L, SliceDiis& D, double* dyn, double* vt2, int fin_prev,
                             double* fvals, double* fred, State* fsst) {
   const int s = find_solve(prefix, n, bid);
@@ -574,7 +575,7 @@ __device__ void slice_block(const SolveDev* __restrict__ solves, const int* __re
   int phase, src, nh, slot_cur;
   double bw;
   if (fin_prev) {
-    finalize_state(S, FIN_ITER, 1, fvals, fred, *fsst, sv == 0);
+    finalize_state<16>(S, FIN_ITER, 1, fvals, fred, *fsst, sv == 0);
     if (sv == 0) put_state(*fsst, S.st2);
     phase = fsst->phase;
     src = fsst->next_src;
@@ -862,7 +863,7 @@ __device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, in
 // entries each); lad_tab[] picks the table of each role: these columns read
 // the linear direct (v(c-a)) and exchange (v(c-b)) tables.
 template <typename Real, int G>
-__device__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
+__device__ __forceinline__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
                             const int4* cls, double* red) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
   const int o = S.o, o2 = o * o;
@@ -1003,7 +1004,7 @@ __device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cba
 // loads (from the tuned tile table).
 constexpr int kWGroup = 8;
 template <typename Real, int G, int HS>
-__device__ void hh_unit(const SolveG& S, int K, int row0, const double* vt, const int4* cls, double* red,
+__device__ __forceinline__ void hh_unit(const SolveG& S, int K, int row0, const double* vt, const int4* cls, double* red,
                         double2* Wl) {
   typedef MfmaTraits<double>::acc_t acc64_t;
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
@@ -1092,7 +1093,7 @@ __device__ __forceinline__ int2 stage_ladder(const SolveG& S, int K, const Ladde
 }
 
 template <int G>
-__device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+__device__ __forceinline__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
                              int n, int blk, const LadderLds& l, double* red) {
   const int s = find_solve(prefix, n, blk);
   const SolveG& S = as_global(solves)[s];
@@ -1106,7 +1107,7 @@ __device__ void ladder_block(const SolveDev* __restrict__ solves, const int* __r
 }
 
 template <int G, int HS>
-__device__ void hh_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
+__device__ __forceinline__ void hh_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
                          int n, int blk, const LadderLds& l, double* red) {
   const int s = find_solve(prefix, n, blk);
   const SolveG& S = as_global(solves)[s];
@@ -1153,7 +1154,7 @@ __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solve
 // rows' pairs, the partner slices d_q(l)), then the T columns they select.
 // QS: k-steps of 4 l values per round of loads (from the tuned tile table).
 template <bool XRING, int QS>
-__device__ void quad_tile(const SolveG& S, int qb, int tile) {
+__device__ __forceinline__ void quad_tile(const SolveG& S, int qb, int tile) {
   typedef MfmaTraits<double>::acc_t acc_t;
   const int4 info = S.qb_info[qb];  // rows of q: [r0, r0+n); columns (rows of -q): [c0, c0+m)
   const int r0 = info.x, n = info.y, c0 = info.z, m = info.w;
@@ -1237,7 +1238,7 @@ __device__ void quad_tile(const SolveG& S, int qb, int tile) {
 // (source/CCD.f90:563-567), split over kJooParts waves: wave `part` sums the
 // slices of its range in order (lane k, column k) into joo[part][k]; k_main
 // adds the parts in order.
-__device__ void joo_wave(const SolveG& S, int part) {
+__device__ __forceinline__ void joo_wave(const SolveG& S, int part) {
   if (!S.do_mosaic) return;
   const int o = S.o, lane = threadIdx.x & 63;
   const int s0 = (S.v * part) / kJooParts, s1 = (S.v * (part + 1)) / kJooParts;
@@ -1250,7 +1251,7 @@ __device__ void joo_wave(const SolveG& S, int part) {
 
 // One ring / crossed-ring / Joo work item, by one wave.
 template <int QS>
-__device__ void contract_item(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
+__device__ __forceinline__ void contract_item(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
                               int item) {
   const int s = find_solve(prefix, n, item);
   const SolveG& S = as_global(solves)[s];
@@ -1341,7 +1342,7 @@ __device__ __forceinline__ void main_partials(const SolveG& S, int lb, int nd, i
 // NQ: the partials' width (16 serves up to 12 DIIS entries, 32 up to 28),
 // or 0 to choose it per solve.
 template <int NQ>
-__device__ void main_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
+__device__ __forceinline__ void main_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix, int n,
                            int gb, int flags, double* red, double* joo_l) {
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
@@ -1497,25 +1498,39 @@ __device__ __forceinline__ void reduce_partials_n(const SolveG& S, int nd, doubl
   double acc[NQ];  // dT, max|R|, E, non-finite, dots
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  for (int k = threadIdx.x; k < nb && threadIdx.x < kFinThreads; k += kFinThreads) {
-    double p[NQ];
+  // blocks k, k + kFinThreads, ... in that order; the 16-wide form takes two
+  // at a time, so that both loads of a pair go out in one round (the partials
+  // of up to 2 kFinThreads blocks); the 32-wide form one at a time (registers)
+  constexpr int kStep = NQ <= 16 ? 2 : 1;
+  for (int k = threadIdx.x; k < nb && threadIdx.x < kFinThreads; k += kStep * kFinThreads) {
+    const int k2 = k + kFinThreads;
+    const bool two = kStep == 2 && k2 < nb;
+    double p[NQ], p2[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int src = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q;
       p[q] = src < nq ? *partial(S, src, k) : 0.0;
+      p2[q] = src < nq && two ? *partial(S, src, k2) : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], p[q]) : acc[q] + p[q];
+    if (two) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], p2[q]) : acc[q] + p2[q];
+    }
   }
   const double r = block_reduce_scatter<NQ, 2>(acc, red);
   const int q = threadIdx.x;
   if (q < nq) vals[q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q] = r;
 }
+// MAXQ: the widest partials the caller reduces (16: DIIS histories up to 12,
+// the only width the slice blocks' folded update instantiates).
+template <int MAXQ>
 __device__ __forceinline__ void reduce_partials(const SolveG& S, int nd, double* vals, double* red) {
-  if (nd <= 12)
+  if (MAXQ == 16 || nd <= 12)
     reduce_partials_n<16>(S, nd, vals, red);
   else
-    reduce_partials_n<32>(S, nd, vals, red);
+    reduce_partials_n<MAXQ == 16 ? 16 : 32>(S, nd, vals, red);
 }
 
 // The iteration bookkeeping of source/CCD.f90:80-99 after a residual pass, on
@@ -1586,6 +1601,7 @@ __device__ __forceinline__ bool advance_state(const SolveG& S, State* st, const 
 // kFinThreads threads or more (vals and red are its LDS); it returns the
 // phase the state had (a FIN_ITER on a finished solve changes nothing).
 // write_trace: this block writes the trace line (one block per solve does).
+template <int MAXQ>
 __device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int prec64, double* vals, double* red,
                                               State& sst, bool write_trace) {
   // One round of loads: each thread's state words go out with its partials
@@ -1596,7 +1612,7 @@ __device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int pre
   const double sw0 = w0 < kStateWords ? src[w0] : 0.0;
   const double sw1 = two && w1 < kStateWords ? src[w1] : 0.0;
   const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS) ? S.ndiis : 0;
-  reduce_partials(S, nd, vals, red);
+  reduce_partials<MAXQ>(S, nd, vals, red);
   double* dst = reinterpret_cast<double*>(&sst);
   if (w0 < kStateWords) dst[w0] = sw0;
   if (two && w1 < kStateWords) dst[w1] = sw1;
@@ -1647,10 +1663,10 @@ __device__ __forceinline__ void put_state(const State& sst, GlobalPtr<State> to)
 
 // The reductions and state update of solve sidx by one kFinThreads block:
 // the new state goes to st and to the pass copy st2.
-__device__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, int kind, int prec64,
+__device__ __forceinline__ void finalize_block(const SolveDev* __restrict__ solves, int sidx, int kind, int prec64,
                                double* vals, double* red, State& sst) {
   const SolveG& S = as_global(solves)[sidx];
-  const int phase0 = finalize_state(S, kind, prec64, vals, red, sst, true);
+  const int phase0 = finalize_state<32>(S, kind, prec64, vals, red, sst, true);
   if (kind == FIN_ITER && phase0 == PH_DONE) return;
   put_state(sst, S.st);
   put_state(sst, S.st2);
