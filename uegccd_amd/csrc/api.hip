@@ -225,6 +225,7 @@ struct uegccd_ctx {
   DeviceArena arena;
   int* d_box = nullptr;    // scratch of the device-built maps (launch_plan)
   int* d_civ = nullptr;
+  PlanArgs pa{};           // host copy of the planning arguments (batches collect them)
   SolveDev sd{};           // host copy of the device descriptor
   SolveDev* d_sd = nullptr;
   BatchDev batch{};        // batch of one
@@ -864,6 +865,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.o = c->o; S.v = c->v; S.nao = c->nao; S.nK = c->nK; S.n3 = c->n3; S.n2max = c->n2max;
   S.ndiis = ndiis;
   S.mode = opt.mode;
+  S.prec0 = opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
   S.maxiter = c->maxiter;
   S.diis_start = std::max(1, opt.diis_start);
   S.diis_err_is_dt = 1;
@@ -957,6 +959,19 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.con_lds = contract_lds_bytes(b.tab_doubles, b.cls_max, c->noo_max);
   b.tiles = c->tiles;
   b.main_nq = main_nq_of(S);
+  // the planning launches of the batch of one (their arguments are written
+  // once the scratch is placed, below)
+  b.plan = im.reserve<PlanArgs>(1);
+  {
+    int* d_ppref = im.reserve<int>(6);
+    const int hpp[6] = {0, plan_box_blocks(c->nao), 0, plan_list_blocks(c->nK, (int)c->qb_neg.size(), c->o), 0,
+                        c->v};
+    im.write(d_ppref, hpp, 6);
+    for (int k = 0; k < 3; ++k) {
+      b.plan_prefix[k] = d_ppref + 2 * k;
+      b.plan_blocks[k] = hpp[2 * k + 1];
+    }
+  }
   b.ncon = S.ncon;
   // work buffers (cleared at placement), the device-built maps first
   c->arena.used = aligned(c->arena.used);
@@ -992,19 +1007,17 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.st2 = c->arena.take<State>(1);
   S.trace = c->arena.take<double>(2 * (size_t)c->maxiter);
   im.write(c->d_sd, &S, 1);
+  c->pa.sd = c->d_sd;
+  c->pa.w = c->box_w;
+  c->pa.center = c->box_center;
+  c->pa.coren = c->coren;
+  c->pa.box = c->d_box;
+  c->pa.civ = c->d_civ;
+  im.write(const_cast<PlanArgs*>(b.plan), &c->pa, 1);
 }
 
 // The device-built maps of a placed context (its work buffers cleared).
-void launch_plan_maps(const uegccd_ctx* c, hipStream_t s) {
-  PlanArgs p;
-  p.sd = c->d_sd;
-  p.w = c->box_w;
-  p.center = c->box_center;
-  p.coren = c->coren;
-  p.box = c->d_box;
-  p.civ = c->d_civ;
-  launch_plan(p, c->o, c->v, c->nao, c->nK, (int)c->qb_neg.size(), s);
-}
+void launch_plan_maps(const uegccd_ctx* c, hipStream_t s) { launch_plan(c->batch, s); }
 
 // Puts a planned context on its device in memory of its own: one allocation,
 // one copy of the constant section, one clear of the work buffers.  Deletes c
@@ -1381,8 +1394,19 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     noo_max = std::max(noo_max, c->noo_max);
     cp[k + 1] = cp[k] + c->sd.ncon;
   }
+  // the planning launches of the solves (used when a batch is planned at once)
+  std::vector<PlanArgs> pas(n);
+  std::vector<int> pp[3];
+  for (auto& v : pp) v.assign(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    const uegccd_ctx* c = cs[k];
+    pas[k] = c->pa;
+    pp[0][k + 1] = pp[0][k] + plan_box_blocks(c->nao);
+    pp[1][k + 1] = pp[1][k] + plan_list_blocks(c->nK, (int)c->qb_neg.size(), c->o);
+    pp[2][k + 1] = pp[2][k] + c->v;
+  }
   size_t bytes = aligned(sizeof(SolveDev) * n) + (4 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
-                 aligned(sizeof(State) * n);
+                 aligned(sizeof(State) * n) + aligned(sizeof(PlanArgs) * n) + 3 * aligned(sizeof(int) * (n + 1));
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
     out->base = nullptr;
@@ -1426,6 +1450,11 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.main_nq = 16;
   for (int k = 0; k < n; ++k) b.main_nq = std::max(b.main_nq, main_nq_of(cs[k]->sd));
   b.con_prefix = const_cast<int*>(im.put(cp));
+  b.plan = im.put(pas);
+  for (int q = 0; q < 3; ++q) {
+    b.plan_prefix[q] = const_cast<int*>(im.put(pp[q]));
+    b.plan_blocks[q] = pp[q][n];
+  }
   b.ncon = cp[n];
   out->states = a.take<State>(n);
   b.nlad = lp[n];
@@ -2226,6 +2255,75 @@ struct DevicePool {
 // for the next step.
 constexpr size_t kStagePerStep = size_t(64) << 20;
 
+// One step's admissions travel to the device in one copy: the constant
+// images of the admitted contexts packed into a pinned buffer, then the table
+// of copy jobs; a scatter kernel puts each image in its arena and clears
+// each work region (one API call and one launch, not two per request).
+struct AdmitUpload {
+  char* host = nullptr;  // pinned, kStagePerStep + the job table
+  char* dev = nullptr;   // the same size on the device
+  size_t cap = 0, used = 0;
+  std::vector<CopyJob> jobs;
+  std::vector<size_t> src_off;  // each job's source offset in the buffer (kNone: a clear)
+  static constexpr size_t kNone = ~size_t(0);
+  static size_t table_bytes() { return aligned(sizeof(CopyJob) * 2 * 4096); }
+  int ensure() {
+    if (host) return UEGCCD_OK;
+    cap = kStagePerStep;
+    if (hipHostMalloc(reinterpret_cast<void**>(&host), cap + table_bytes(), hipHostMallocDefault) != hipSuccess)
+      return UEGCCD_ERR_ALLOC;
+    if (hipMalloc(&dev, cap + table_bytes()) != hipSuccess) return UEGCCD_ERR_ALLOC;
+    return UEGCCD_OK;
+  }
+  bool fits(size_t bytes) const { return used + aligned(bytes) <= cap && jobs.size() + 2 <= 2 * 4096; }
+  // an image to dst (its size rounded up to 16 bytes: the arena's next
+  // buffer starts 256-aligned past it) and a clear of `clear` bytes at zdst
+  void add(char* dst, const std::vector<char>& image, char* zdst, size_t clear) {
+    const size_t n = (image.size() + 15) & ~size_t(15);
+    std::memcpy(host + used, image.data(), image.size());
+    std::memset(host + used + image.size(), 0, n - image.size());
+    if (n) {
+      jobs.push_back({dst, nullptr, n});
+      src_off.push_back(used);
+    }
+    used += aligned(n);
+    if (clear) {
+      jobs.push_back({zdst, nullptr, clear});
+      src_off.push_back(kNone);
+    }
+  }
+  // Sends what was added since the last send (so that the device works on
+  // the first admissions while the host prepares the next); `last` ends the
+  // step's upload.
+  size_t sent = 0, jobs_sent = 0;
+  int send(hipStream_t s, bool last) {
+    if (jobs.size() > jobs_sent) {
+      const size_t nj = jobs.size() - jobs_sent;
+      for (size_t k = jobs_sent; k < jobs.size(); ++k)
+        jobs[k].src = src_off[k] == kNone ? nullptr : dev + src_off[k];
+      CopyJob* table = reinterpret_cast<CopyJob*>(host + cap) + jobs_sent;
+      std::memcpy(table, jobs.data() + jobs_sent, sizeof(CopyJob) * nj);
+      if (used > sent) HIPCHK(hipMemcpyAsync(dev + sent, host + sent, used - sent, hipMemcpyHostToDevice, s));
+      CopyJob* dtable = reinterpret_cast<CopyJob*>(dev + cap) + jobs_sent;
+      HIPCHK(hipMemcpyAsync(dtable, table, sizeof(CopyJob) * nj, hipMemcpyHostToDevice, s));
+      launch_scatter(dtable, (int)nj, s);
+      sent = used;
+      jobs_sent = jobs.size();
+    }
+    if (last) {
+      jobs.clear();
+      src_off.clear();
+      used = sent = jobs_sent = 0;
+    }
+    return UEGCCD_OK;
+  }
+  void release() {
+    if (host) hipHostFree(host);
+    if (dev) hipFree(dev);
+    host = dev = nullptr;
+  }
+};
+
 }  // namespace
 
 struct uegccd_server {
@@ -2251,6 +2349,8 @@ struct uegccd_server {
   std::vector<int64_t> active;     // iterated together
   size_t active_bytes = 0;
   BatchScratch batch;
+  BatchScratch admitted;           // the solves admitted by one step, set up by batched launches
+  AdmitUpload upload;              // their images and clears, one copy and one launch per step
   bool batch_stale = true;
   size_t nbatch = 0;               // the first nbatch active solves form the batch
   State* h_states = nullptr;       // pinned, one per batch member
@@ -2390,10 +2490,12 @@ void release_plan_slot(uegccd_server* s) {
   s->cv_work.notify_one();
 }
 
-// Admits queued requests in arrival order while slots and memory allow; each
-// admitted solve gets its MP2 guess, E(2) and first dT on the server stream.
+// Admits queued requests in arrival order while slots and memory allow; the
+// solves admitted together get their device-built maps, MP2 guess, E(2) and
+// first dT on the server stream from one launch of each kernel.
 int admit(uegccd_server* s) {
   const size_t budget = (size_t)s->opt.hbm_budget_bytes;
+  std::vector<uegccd_ctx*> fresh;
   while (s->qhead < s->queue.size() && (int)s->active.size() < s->opt.max_active) {
     Request& r = *s->reqs[s->queue[s->qhead]];
     if (!head_planned(s, r)) break;
@@ -2416,7 +2518,14 @@ int admit(uegccd_server* s) {
       continue;
     }
     if (s->active_bytes + fp > budget) break;
-    if (!s->active.empty() && s->stage.used >= kStagePerStep) break;
+    if (int rc = s->upload.ensure()) return rc;
+    if (!s->upload.fits(r.image.size())) {
+      if (!fresh.empty() || r.image.size() + 256 <= kStagePerStep) break;  // (the next step takes the rest)
+      finish_request(s, r, UEGCCD_ERR_ALLOC);  // an image larger than a step's upload
+      release_plan_slot(s);
+      ++s->qhead;
+      continue;
+    }
     char* mem = s->pool.alloc(fp);
     if (!mem) {
       if (!s->active.empty()) break;  // retried once solves have left
@@ -2432,20 +2541,16 @@ int admit(uegccd_server* s) {
     size_t work_off = 0;
     layout_ctx(c, &r.sys, mem, fp, r.image, &work_off, false);  // the planner built the image
     r.placed = true;
-    int rc = upload_image(mem, r.image, s->stream, &s->stage);
+    s->upload.add(mem, r.image, mem + work_off, fp - work_off);
     {
       std::lock_guard<std::mutex> lk(s->mu);  // (the staged copy is independent of the image)
       if ((int)s->spare_images.size() < s->lookahead) s->spare_images.push_back(std::move(r.image));
     }
     std::vector<char>().swap(r.image);
-    if (rc) return rc;
-    HIPCHK(hipMemsetAsync(mem + work_off, 0, fp - work_off, s->stream));
-    launch_plan_maps(c, s->stream);
-    const int prec64 = r.ctx->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
-    launch_mp2(r.ctx->batch, s->stream);
-    launch_init(r.ctx->batch, s->stream);
-    launch_finalize(r.ctx->batch, FIN_INIT, s->stream, prec64);
-    HIPCHK(hipGetLastError());
+    fresh.push_back(c);
+    if (fresh.size() % 32 == 0) {
+      if (int rc = s->upload.send(s->stream, false)) return rc;
+    }
     r.res.state = UEGCCD_REQ_RUNNING;
     r.res.t_start_ms = server_now(s);
     s->active.push_back(s->queue[s->qhead]);
@@ -2453,6 +2558,20 @@ int admit(uegccd_server* s) {
     release_plan_slot(s);
     s->batch_stale = true;
     ++s->qhead;
+  }
+  if (!fresh.empty()) {
+    // (their images and clears in one copy and one launch, then one launch of
+    // each set-up kernel over the admitted solves, in their own batch
+    // tables; each solve starts in its own precision)
+    if (int rc = s->upload.send(s->stream, true)) return rc;
+    int rc = make_batch(fresh.data(), (int)fresh.size(), &s->admitted, s->stream, &s->stage);
+    if (rc) return rc;
+    const BatchDev& b = s->admitted.b;
+    launch_plan(b, s->stream);
+    launch_mp2(b, s->stream);
+    launch_init(b, s->stream);
+    launch_finalize(b, FIN_INIT, s->stream, -1);
+    HIPCHK(hipGetLastError());
   }
   if (s->qhead > 1024 && s->qhead * 2 > s->queue.size()) {
     s->queue.erase(s->queue.begin(), s->queue.begin() + s->qhead);
@@ -2513,7 +2632,9 @@ void uegccd_server_destroy(uegccd_server* s) {
   }
   s->pool.clear();
   s->stage.release();
+  s->upload.release();
   if (s->batch.base) hipFree(s->batch.base);
+  if (s->admitted.base) hipFree(s->admitted.base);
   if (s->h_states) hipHostFree(s->h_states);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;

@@ -1627,7 +1627,7 @@ __device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int pre
     } else {
       if (kind == FIN_INIT) {  // a fresh state: nothing carries over from an earlier solve
         st->nonfinite = 0;
-        st->prec64 = prec64;
+        st->prec64 = prec64 < 0 ? S.prec0 : prec64;
         st->iters_fp32 = 0;
         st->maxres = st->maxres_in = 0.0;
       }
@@ -2352,23 +2352,31 @@ __device__ __forceinline__ int block_rank(bool flag, int* wsum, int* total) {
   return before + in_wave;
 }
 
-// The box: orbital p + 1 at the box index of n_p.
-__global__ __launch_bounds__(256) void k_plan_box(PlanArgs P) {
+// The box: orbital p + 1 at the box index of n_p.  Every planning kernel
+// covers a batch: a block finds its solve by the launch's block prefix.
+__global__ __launch_bounds__(256) void k_plan_box(const PlanArgs* __restrict__ pa, const int* __restrict__ prefix,
+                                                  int n) {
+  const int s = find_solve(prefix, n, blockIdx.x);
+  const PlanArgs& P = pa[s];
   const SolveDev& S = *P.sd;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = (blockIdx.x - prefix[s]) * blockDim.x + threadIdx.x;
   if (p < S.nao) P.box[P.center + box_lin(S.nv4[p], P.w)] = p + 1;
 }
 
-// Blocks [0, nK): block K's virtual list, the virtuals a whose partner
-// b = K - n_a is a virtual orbital (source/CCD.f90: FindIndex(i,j,a) > NOcc),
-// in a order, and civ[a][K].  The rest: qb_d[q][l], the virtual index of
-// n_l + q (or -1), one thread per (q, l).
-__global__ __launch_bounds__(256) void k_plan_blocks(PlanArgs P) {
+// Blocks [0, nK) of a solve: block K's virtual list, the virtuals a whose
+// partner b = K - n_a is a virtual orbital (source/CCD.f90: FindIndex(i,j,a)
+// > NOcc), in a order, and civ[a][K].  The rest: qb_d[q][l], the virtual
+// index of n_l + q (or -1), one thread per (q, l).
+__global__ __launch_bounds__(256) void k_plan_blocks(const PlanArgs* __restrict__ pa,
+                                                     const int* __restrict__ prefix, int n) {
   __shared__ int wsum[4];
+  const int s = find_solve(prefix, n, blockIdx.x);
+  const PlanArgs& P = pa[s];
   const SolveDev& S = *P.sd;
   const int o = S.o, v = S.v, nK = S.nK, w = P.w;
-  if ((int)blockIdx.x < nK) {
-    const int K = blockIdx.x;
+  const int blk = blockIdx.x - prefix[s];
+  if (blk < nK) {
+    const int K = blk;
     const int x0 = S.kb_pairs[S.kb_poff[K]];
     const int kb = P.center + box_lin(S.nv4[x0 % o], w) + box_lin(S.nv4[x0 / o], w);
     const int voff = S.kb_voff[K];
@@ -2398,7 +2406,7 @@ __global__ __launch_bounds__(256) void k_plan_blocks(PlanArgs P) {
     }
     return;
   }
-  const int t = (blockIdx.x - nK) * blockDim.x + threadIdx.x;
+  const int t = (blk - nK) * blockDim.x + threadIdx.x;
   if (t >= S.nqb * o) return;
   const int q = t / o, l = t - q * o;
   const int r = S.qb_rows[S.qb_roff[q]];  // a pair (i, p) of the transfer: q = n_p - n_i
@@ -2408,14 +2416,17 @@ __global__ __launch_bounds__(256) void k_plan_blocks(PlanArgs P) {
   const_cast<int*>(S.qb_d)[t] = p >= o ? p - o : -1;
 }
 
-// One block per virtual slice a: its valid elements (i, j, a) in element
-// order, pair x = i + o j ascending, from slab_coff[a] on: the element, its
-// K-blocked position and its partner.  Elements with a core index (i or j <
-// coren, the CoreN masking) are left out.
-__global__ __launch_bounds__(256) void k_plan_cmap(PlanArgs P) {
+// One block per virtual slice a of a solve: its valid elements (i, j, a) in
+// element order, pair x = i + o j ascending, from slab_coff[a] on: the
+// element, its K-blocked position and its partner.  Elements with a core
+// index (i or j < coren, the CoreN masking) are left out.
+__global__ __launch_bounds__(256) void k_plan_cmap(const PlanArgs* __restrict__ pa, const int* __restrict__ prefix,
+                                                   int n) {
   __shared__ int wsum[4];
+  const int s = find_solve(prefix, n, blockIdx.x);
+  const PlanArgs& P = pa[s];
   const SolveDev& S = *P.sd;
-  const int o = S.o, o2 = o * o, nK = S.nK, av = blockIdx.x;
+  const int o = S.o, o2 = o * o, nK = S.nK, av = blockIdx.x - prefix[s];
   const int base = S.slab_coff[av];
   const int* civ = P.civ + (size_t)av * nK;
   uint2* const cmap = const_cast<uint2*>(S.cmap);
@@ -2439,11 +2450,25 @@ __global__ __launch_bounds__(256) void k_plan_cmap(PlanArgs P) {
   }
 }
 
-void launch_plan(const PlanArgs& p, int o, int v, int nao, int nK, int nq, hipStream_t s) {
-  hipLaunchKernelGGL(k_plan_box, dim3((nao + 255) / 256), dim3(256), 0, s, p);
-  const int qblocks = (nq * o + 255) / 256;
-  hipLaunchKernelGGL(k_plan_blocks, dim3(nK + qblocks), dim3(256), 0, s, p);
-  hipLaunchKernelGGL(k_plan_cmap, dim3(v), dim3(256), 0, s, p);
+void launch_plan(const BatchDev& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_plan_box, dim3(b.plan_blocks[0]), dim3(256), 0, s, b.plan, b.plan_prefix[0], b.n);
+  hipLaunchKernelGGL(k_plan_blocks, dim3(b.plan_blocks[1]), dim3(256), 0, s, b.plan, b.plan_prefix[1], b.n);
+  hipLaunchKernelGGL(k_plan_cmap, dim3(b.plan_blocks[2]), dim3(256), 0, s, b.plan, b.plan_prefix[2], b.n);
+}
+
+// Job blockIdx.y of a scatter launch, its 16-byte words strided over the
+// launch's blocks in x.
+__global__ __launch_bounds__(256) void k_scatter(const CopyJob* __restrict__ jobs) {
+  const CopyJob J = jobs[blockIdx.y];
+  uint4* const d = reinterpret_cast<uint4*>(J.dst);
+  const uint4* const src = reinterpret_cast<const uint4*>(J.src);
+  const size_t n16 = J.bytes / 16;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (size_t)gridDim.x * blockDim.x)
+    d[k] = src ? src[k] : make_uint4(0u, 0u, 0u, 0u);
+}
+
+void launch_scatter(const CopyJob* jobs, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(64, n), dim3(256), 0, s, jobs);
 }
 
 __global__ void k_zero(double* __restrict__ p, size_t n) {

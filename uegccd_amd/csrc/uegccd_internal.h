@@ -98,6 +98,7 @@ struct SolveDevT {
   int n2max;        // largest |n_p - n_q|^2 in the basis
   int ndiis;        // history length (1 in reference mode)
   int mode;         // UEGCCD_MODE_*
+  int prec0;        // ladder precision a solve starts in: 1 FP64, 0 FP32 (mixed precision)
   int maxiter, diis_start, diis_err_is_dt;
   int do_ring, do_xring, do_ladder, do_mosaic;
   double df;        // DenomFactor
@@ -200,6 +201,8 @@ struct Tiles {
 };
 
 // Launch geometry of a batch of solves iterated together.
+struct PlanArgs;
+
 struct BatchDev {
   int n;
   SolveDev* solves;     // device array [n]
@@ -228,6 +231,12 @@ struct BatchDev {
   int ncon;
   Tiles tiles;          // the tuned kernels' tile choice for the launch
   int main_nq;          // k_main's partials width: 16 (DIIS lengths up to 12) or 32
+  // the device-built maps of the solves (launch_plan): each solve's PlanArgs
+  // and the block prefixes of the planning launches (box; block lists and
+  // transfer partners; element map)
+  const PlanArgs* plan;     // [n]
+  int* plan_prefix[3];      // [n+1] each
+  int plan_blocks[3];
 };
 
 // k_contract wave items: a 16x16 tile of the quadratic ring (crossed-ring)
@@ -292,7 +301,8 @@ void prepare_launches();
 void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s,
                       const KernelTimer* timer = nullptr, hipEvent_t tail_stop = nullptr, int fin_prev = 0);
 int first_pass_kernel(const BatchDev& b, int main_flags);
-// FIN_INIT starts a fresh state (every field; prec64 the starting precision).
+// FIN_INIT starts a fresh state (every field; prec64 the starting precision,
+// or each solve's own prec0 when negative).
 void launch_finalize(const BatchDev& b, int kind, hipStream_t s, int prec64 = 1);
 void launch_state_reset(const BatchDev& b, int phase, int prec64, hipStream_t s);
 void launch_gather_states(const BatchDev& b, State* out, hipStream_t s);
@@ -316,7 +326,21 @@ struct PlanArgs {
   int* box;
   int* civ;
 };
-void launch_plan(const PlanArgs& p, int o, int v, int nao, int nK, int nq, hipStream_t s);
+// Blocks of the three planning launches for one solve.
+inline int plan_box_blocks(int nao) { return (nao + 255) / 256; }
+inline int plan_list_blocks(int nK, int nq, int o) { return nK + (nq * o + 255) / 256; }
+// The maps of every solve of a batch, three launches.
+void launch_plan(const BatchDev& b, hipStream_t s);
+
+// Copies (src non-null) or clears (src null) `bytes` (a multiple of 16) at
+// dst, for n such jobs in one launch: the admissions of a server step (each
+// context's constant image and the clear of its work buffers).
+struct CopyJob {
+  char* dst;
+  const char* src;
+  size_t bytes;
+};
+void launch_scatter(const CopyJob* jobs, int n, hipStream_t s);
 
 // Clears `bytes` (a multiple of 8) of device memory at p with a fill kernel
 // (no memset node: usable where a caller captures the stream).
