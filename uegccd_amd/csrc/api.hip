@@ -217,7 +217,8 @@ struct uegccd_ctx {
   std::vector<int> core_elems;     // valid elements with a core index (held at zero)
   double screening = 0.0;          // HEGData%ScreeningDistance
   std::vector<int4> hh_items;   // (0, K, row0, 0) hole-hole units
-  std::vector<int4> lad_items;  // (0, K, row0, col0) particle-particle tiles
+  std::vector<int4> lad_items;  // (0, K, row0, col0) particle-particle tiles of 16 columns
+  std::vector<int4> lad_items2; // the same of 32 columns (two tiles per block, in batches)
   std::vector<int> qb_roff, qb_rows, qb_neg, qrow, qb_of;  // hole-particle sets P(q)
   std::vector<int4> qb_info;
   std::vector<int4> con_items;  // (0, kind, arg, arg) of k_contract
@@ -423,6 +424,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   // tiles of 16 a columns; the largest blocks' units first
   c->hh_items.clear();
   c->lad_items.clear();
+  c->lad_items2.clear();
   for (int K = 0; K < c->nK; ++K) {
     const int noo = c->kb_poff[K + 1] - c->kb_poff[K];
     const int nvv = c->kb_voff[K + 1] - c->kb_voff[K];
@@ -430,6 +432,7 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     for (int r0 = 0; r0 < noo; r0 += 16) {
       c->hh_items.push_back(make_int4(0, K, r0, 0));
       for (int c0 = 0; c0 < nvv; c0 += 16) c->lad_items.push_back(make_int4(0, K, r0, c0));
+      for (int c0 = 0; c0 < nvv; c0 += 32) c->lad_items2.push_back(make_int4(0, K, r0, c0));
     }
   }
   std::stable_sort(c->hh_items.begin(), c->hh_items.end(), [c](const int4& a, const int4& b) {
@@ -673,6 +676,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double) * 2 * c->maxiter);
   bytes += aligned(sizeof(int) * (8 + 2 * kSliceClasses)) +
            aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size())) +
+           aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items2.size())) + aligned(sizeof(int) * 2) +
            aligned(sizeof(int4) * std::max<size_t>(1, c->hh_items.size()));
   bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
@@ -834,6 +838,7 @@ size_t const_section_bound(const uegccd_ctx* c) {
   add(sizeof(int) * (c->qb_roff.size() + c->qb_rows.size() + c->qb_neg.size() + c->qrow.size() +
                      c->qb_of.size()));
   add(sizeof(int4) * (c->qb_info.size() + c->hh_items.size() + c->lad_items.size() + c->con_items.size()));
+  add(sizeof(int4) * c->lad_items2.size() + sizeof(int) * 2);
   add(sizeof(SolveDev) + sizeof(int) * (8 + 2 * kSliceClasses));
   return b + 24 * 256;  // (one alignment pad per array)
 }
@@ -923,6 +928,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.hh_items = im.put(c->hh_items);
   S.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
   S.lad_items = im.put(c->lad_items);
+  S.nlad2 = sys->do_ladder ? (int)c->lad_items2.size() : 0;
+  S.lad_items2 = im.put(c->lad_items2);
   S.ncon = (int)c->con_items.size();
   S.con_items = im.put(c->con_items);
   // batch of one: block prefixes of the element, slice, ladder and contraction launches
@@ -954,6 +961,13 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   b.nhh_pad = pad8(S.nhh);
   b.nlad = S.nlad;
   b.nlad_pad = pad8(S.nlad);
+  {
+    int* d_l2 = im.reserve<int>(2);
+    const int h_l2[2] = {0, S.nlad2};
+    im.write(d_l2, h_l2, 2);
+    b.lad_prefix2 = d_l2;
+    b.nlad2 = S.nlad2;
+  }
   b.tab_doubles = ladder_tab_doubles(S);
   b.cls_max = std::max(1, c->nvv_max);
   b.con_lds = contract_lds_bytes(b.tab_doubles, b.cls_max, c->noo_max);
@@ -1374,7 +1388,7 @@ int upload_image(char* dst, const std::vector<char>& image, hipStream_t stream, 
 // gather buffer, in one (reused) allocation uploaded in one copy.
 int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = nullptr,
                PinnedStage* stage = nullptr) {
-  std::vector<int> ep(n + 1, 0), lp(n + 1, 0), cp(n + 1, 0), hp(n + 1, 0);
+  std::vector<int> ep(n + 1, 0), lp(n + 1, 0), lp2(n + 1, 0), cp(n + 1, 0), hp(n + 1, 0);
   std::vector<int> sp[kSliceClasses];
   int slds[kSliceClasses] = {};
   for (auto& v : sp) v.assign(n + 1, 0);
@@ -1388,6 +1402,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     for (int q = 0; q < kSliceClasses; ++q) sp[q][k + 1] = sp[q][k] + (q == cls ? c->v : 0);
     slds[cls] = std::max(slds[cls], slice_lds_bytes(c->o, slice_split));
     lp[k + 1] = lp[k] + c->sd.nlad;
+    lp2[k + 1] = lp2[k] + c->sd.nlad2;
     hp[k + 1] = hp[k] + c->sd.nhh;
     tab_doubles = std::max(tab_doubles, ladder_tab_doubles(c->sd));
     cls_max = std::max(cls_max, c->nvv_max);
@@ -1405,7 +1420,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     pp[1][k + 1] = pp[1][k] + plan_list_blocks(c->nK, (int)c->qb_neg.size(), c->o);
     pp[2][k + 1] = pp[2][k] + c->v;
   }
-  size_t bytes = aligned(sizeof(SolveDev) * n) + (4 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
+  size_t bytes = aligned(sizeof(SolveDev) * n) + (5 + kSliceClasses) * aligned(sizeof(int) * (n + 1)) +
                  aligned(sizeof(State) * n) + aligned(sizeof(PlanArgs) * n) + 3 * aligned(sizeof(int) * (n + 1));
   if (out->cap < bytes) {
     if (out->base) hipFree(out->base);
@@ -1432,6 +1447,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     b.slice_blocks += sp[q][n];
   }
   b.lad_prefix = const_cast<int*>(im.put(lp));
+  b.lad_prefix2 = const_cast<int*>(im.put(lp2));
   b.hh_prefix = const_cast<int*>(im.put(hp));
   b.nhh = hp[n];
   b.nhh_pad = pad8(hp[n]);
@@ -1458,6 +1474,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.ncon = cp[n];
   out->states = a.take<State>(n);
   b.nlad = lp[n];
+  b.nlad2 = lp2[n];
   b.elem_blocks = ep[n];
   return upload_image(out->base, image, stream, stage);
 }

@@ -795,12 +795,16 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
 // w, w+4, ...) and combined in a fixed order.  cls is the block's virtual list
 // (n_c, c) in LDS, so each group of steps costs one round of global loads
 // (the amplitudes its list entries select).  Every thread of the block calls
-// it; wave 0 returns true with the tile's sums in accs, accp.
-template <typename Real, int G>
-__device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, int cx, int cy, bool colv,
-                                            const double* vtx, const double* vty, const int4* cls,
-                                            double* red, typename MfmaTraits<Real>::acc_t& accs,
-                                            typename MfmaTraits<Real>::acc_t& accp) {
+// it; wave 0 returns true with the tiles' sums in accs, accp.  NC column
+// tiles (columns colv[c] apart, each synthesised from its orbitals cx[c],
+// cy[c]) share each k-step's amplitude loads, every tile with the same MFMA
+// sequence as a tile of its own (the same bits for every NC).
+template <typename Real, int G, int NC>
+__device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, const int (&cx)[NC],
+                                            const int (&cy)[NC], const bool (&colv)[NC], const double* vtx,
+                                            const double* vty, const int4* cls, double* red,
+                                            typename MfmaTraits<Real>::acc_t (&accs)[NC],
+                                            typename MfmaTraits<Real>::acc_t (&accp)[NC]) {
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
@@ -808,9 +812,14 @@ __device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, in
   // noo pairs contiguously, so a step's 16 rows are one contiguous run
   const auto TK = S.tk + S.kb_toff[K];
   const int rowA = row0 + r16;
-  const int4 nx = S.nv4[cx], ny = S.nv4[cy];
-  accs = {0, 0, 0, 0};
-  accp = {0, 0, 0, 0};
+  int4 nx[NC], ny[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    nx[c] = S.nv4[cx[c]];
+    ny[c] = S.nv4[cy[c]];
+    accs[c] = {0, 0, 0, 0};
+    accp[c] = {0, 0, 0, 0};
+  }
   // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
   // their amplitude loads (unconditional, in-range addresses, masked values)
   // before the MFMAs, which run for every step (masked steps add zeros).
@@ -828,41 +837,55 @@ __device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, in
     for (int st = 0; st < G; ++st) {
       const int ci = g0 + 16 * st + kq;
       const int4 nc = cls[min(ci, nvv - 1)];
-      Real bs = 0, bp = 0;
-      if (ci < nvv && colv) {
-        const Real vx = (Real)vtx[d2(nc, nx)];
-        const Real vy = (Real)vty[d2(nc, ny)];
-        bs = vx - vy;
-        bp = vx + vy;
+      const Real as = (Real)(t[st].x + t[st].y), ap = (Real)(t[st].x - t[st].y);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        Real bs = 0, bp = 0;
+        if (ci < nvv && colv[c]) {
+          const Real vx = (Real)vtx[d2(nc, nx[c])];
+          const Real vy = (Real)vty[d2(nc, ny[c])];
+          bs = vx - vy;
+          bp = vx + vy;
+        }
+        accs[c] = MfmaTraits<Real>::mma(as, bs, accs[c]);
+        accp[c] = MfmaTraits<Real>::mma(ap, bp, accp[c]);
       }
-      accs = MfmaTraits<Real>::mma((Real)(t[st].x + t[st].y), bs, accs);
-      accp = MfmaTraits<Real>::mma((Real)(t[st].x - t[st].y), bp, accp);
     }
   }
-  if (wave > 0) {
-    double* mine = red + ((wave - 1) * 64 + lane) * 8;
-    for (int r = 0; r < 4; ++r) {
-      mine[r] = (double)accs[r];
-      mine[4 + r] = (double)accp[r];
+  // each tile's four wave sums, combined in wave order (red holds 3 waves'
+  // partials of one tile at a time)
+  bool lead = true;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (wave > 0) {
+      double* mine = red + ((wave - 1) * 64 + lane) * 8;
+      for (int r = 0; r < 4; ++r) {
+        mine[r] = (double)accs[c][r];
+        mine[4 + r] = (double)accp[c][r];
+      }
     }
-  }
-  __syncthreads();
-  if (wave != 0) return false;
-  for (int w = 1; w < 4; ++w) {
-    const double* theirs = red + ((w - 1) * 64 + lane) * 8;
-    for (int r = 0; r < 4; ++r) {
-      accs[r] = accs[r] + (Real)theirs[r];
-      accp[r] = accp[r] + (Real)theirs[4 + r];
+    __syncthreads();
+    if (wave == 0) {
+      for (int w = 1; w < 4; ++w) {
+        const double* theirs = red + ((w - 1) * 64 + lane) * 8;
+        for (int r = 0; r < 4; ++r) {
+          accs[c][r] = accs[c][r] + (Real)theirs[r];
+          accp[c][r] = accp[c][r] + (Real)theirs[4 + r];
+        }
+      }
+    } else {
+      lead = false;
     }
+    if (c + 1 < NC) __syncthreads();  // red is reused by the next tile
   }
-  return true;
+  return lead;
 }
 
-// One 16x16 particle-particle tile (rows row0.., columns col0.. of the a
-// list) into lad.  vt holds the solve's distinct ladder tables (n2max + 1
-// entries each); lad_tab[] picks the table of each role: these columns read
-// the linear direct (v(c-a)) and exchange (v(c-b)) tables.
-template <typename Real, int G>
+// NC 16x16 particle-particle tiles (rows row0.., columns col0 + 16c.. of
+// the a list) into lad.  vt holds the solve's distinct ladder tables
+// (n2max + 1 entries each); lad_tab[] picks the table of each role: these
+// columns read the linear direct (v(c-a)) and exchange (v(c-b)) tables.
+template <typename Real, int G, int NC>
 __device__ __forceinline__ void ladder_tile(const SolveG& S, int K, int row0, int col0, const double* vt,
                             const int4* cls, double* red) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
@@ -870,24 +893,34 @@ __device__ __forceinline__ void ladder_tile(const SolveG& S, int K, int row0, in
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
   const int voff = S.kb_voff[K], nvv = S.kb_voff[K + 1] - voff;
   const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
-  const int col = col0 + r16;
-  const bool colv = col < nvv;
-  const int cx = colv ? S.kb_va[voff + col] : 0, cy = colv ? S.kb_vb[voff + col] : 0;
+  int cx[NC], cy[NC];
+  bool colv[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int col = col0 + 16 * c + r16;
+    colv[c] = col < nvv;
+    cx[c] = colv[c] ? S.kb_va[voff + col] : 0;
+    cy[c] = colv[c] ? S.kb_vb[voff + col] : 0;
+  }
   // the output rows' pairs, requested now (used after the reduction)
   int xpo[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) xpo[r] = S.kb_pairs[poff + min(row0 + MfmaTraits<Real>::row(kq, r), noo - 1)];
   const int nt = S.n2max + 1;
-  acc_t accs, accp;
-  if (!ladder_sums<Real, G>(S, K, row0, cx, cy, colv, vt + nt * S.lad_tab[0], vt + nt * S.lad_tab[1], cls, red,
-                            accs, accp) ||
-      !colv)
+  acc_t accs[NC], accp[NC];
+  if (!ladder_sums<Real, G, NC>(S, K, row0, cx, cy, colv, vt + nt * S.lad_tab[0], vt + nt * S.lad_tab[1], cls,
+                                red, accs, accp))
     return;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int rowo = row0 + MfmaTraits<Real>::row(kq, r);
-    if (rowo >= noo) continue;
-    st2(S.lad + (size_t)xpo[r] + (size_t)o2 * (cx - o), make_double2(0.5 * (double)accs[r], 0.5 * (double)accp[r]));
+  for (int c = 0; c < NC; ++c) {
+    if (!colv[c]) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rowo = row0 + MfmaTraits<Real>::row(kq, r);
+      if (rowo >= noo) continue;
+      st2(S.lad + (size_t)xpo[r] + (size_t)o2 * (cx[c] - o),
+          make_double2(0.5 * (double)accs[c][r], 0.5 * (double)accp[c][r]));
+    }
   }
 }
 
@@ -1092,18 +1125,21 @@ __device__ __forceinline__ int2 stage_ladder(const SolveG& S, int K, const Ladde
   return make_int2(phase, prec64);
 }
 
-template <int G>
+// A particle-particle block: ladder item `item`, NC column tiles (the solve's
+// 16-column items for NC = 1, its 32-column items for NC = 2).
+template <int G, int NC>
 __device__ __forceinline__ void ladder_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                             int n, int blk, const LadderLds& l, double* red) {
-  const int s = find_solve(prefix, n, blk);
+                             int n, int item, const LadderLds& l, double* red) {
+  static_assert(NC == 1 || NC == 2, "one or two column tiles");
+  const int s = find_solve(prefix, n, item);
   const SolveG& S = as_global(solves)[s];
-  const int4 it = ld(S.lad_items + (blk - prefix[s]));
+  const int4 it = ld((NC == 2 ? S.lad_items2 : S.lad_items) + (item - prefix[s]));
   const int2 st = stage_ladder(S, it.y, l);
   if (st.x == PH_DONE) return;
   if (st.y)
-    ladder_tile<double, G>(S, it.y, it.z, it.w, l.vt, l.cls, red);
+    ladder_tile<double, G, NC>(S, it.y, it.z, it.w, l.vt, l.cls, red);
   else
-    ladder_tile<float, G>(S, it.y, it.z, it.w, l.vt, l.cls, red);
+    ladder_tile<float, G, NC>(S, it.y, it.z, it.w, l.vt, l.cls, red);
 }
 
 template <int G, int HS>
@@ -1268,12 +1304,13 @@ __device__ __forceinline__ void contract_item(const SolveDev* __restrict__ solve
 // The contraction pass, one launch of three kinds of 256-thread blocks:
 //   [0, nhh_pad)             hole-hole units (the longest blocks, so they go
 //                            first and, not renumbered, round-robin over the XCDs)
-//   [nhh_pad, + nlad_pad)    particle-particle ladder tiles
+//   [nhh_pad, + nlad_pad)    particle-particle ladder blocks, NC column tiles
+//                            each (nlad blocks over the ladder items)
 //   the rest                 four ring / crossed-ring / Joo items per block
 // The pads are multiples of 8, so a block's XCD is still blockIdx mod 8 inside
 // each range, which the ladder tiles and items renumber XCD-contiguously.
 // (held to 128 VGPRs, four waves per SIMD: its tiles wait on gathers)
-template <int G, int QS, int HS>
+template <int G, int QS, int HS, int NC>
 __global__ __launch_bounds__(256, 4) void k_contract(const SolveDev* __restrict__ solves,
                                                   const int* __restrict__ hh_prefix, int nhh, int nhh_pad,
                                                   const int* __restrict__ lad_prefix, int nlad, int nlad_pad,
@@ -1289,7 +1326,7 @@ __global__ __launch_bounds__(256, 4) void k_contract(const SolveDev* __restrict_
   int rb = bid - nhh_pad;
   if (rb < nlad_pad) {
     const int t = xcd_block(rb, nlad_pad);
-    if (t < nlad) ladder_block<G>(solves, lad_prefix, n, t, ladder_lds(dyn, tab_doubles, cls_max), red);
+    if (t < nlad) ladder_block<G, NC>(solves, lad_prefix, n, t, ladder_lds(dyn, tab_doubles, cls_max), red);
     return;
   }
   rb -= nlad_pad;
@@ -2145,7 +2182,7 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
       if (it < nhh) {
         hh_block<2, 8>(solves, hh_prefix, 1, it, ll, P.u.la.red);
       } else if (it < nhh + nlad) {
-        ladder_block<2>(solves, lad_prefix, 1, it - nhh, ll, P.u.la.red);
+        ladder_block<2, 1>(solves, lad_prefix, 1, it - nhh, ll, P.u.la.red);
       } else {
         const int item = (it - nhh - nlad) * kItemsPerBlock + wave;
         if (item < ncon) contract_item<8>(solves, con_prefix, 1, item);
@@ -2221,12 +2258,13 @@ static void allow_large_lds() {
       reinterpret_cast<const void*>(&k_slice<256, false>), reinterpret_cast<const void*>(&k_slice<512, false>),
       reinterpret_cast<const void*>(&k_slice<768, false>), reinterpret_cast<const void*>(&k_slice<256, true>),
       reinterpret_cast<const void*>(&k_slice<512, true>),  reinterpret_cast<const void*>(&k_slice<768, true>),
-      reinterpret_cast<const void*>(&k_contract<1, 4, 4>), reinterpret_cast<const void*>(&k_contract<1, 4, 8>),
-      reinterpret_cast<const void*>(&k_contract<1, 8, 4>), reinterpret_cast<const void*>(&k_contract<1, 8, 8>),
-      reinterpret_cast<const void*>(&k_contract<2, 4, 4>), reinterpret_cast<const void*>(&k_contract<2, 4, 8>),
-      reinterpret_cast<const void*>(&k_contract<2, 8, 4>), reinterpret_cast<const void*>(&k_contract<2, 8, 8>),
-      reinterpret_cast<const void*>(&k_contract<4, 4, 4>), reinterpret_cast<const void*>(&k_contract<4, 4, 8>),
-      reinterpret_cast<const void*>(&k_contract<4, 8, 4>), reinterpret_cast<const void*>(&k_contract<4, 8, 8>)};
+#define UEGCCD_CONTRACT_NC(G, Q, H) reinterpret_cast<const void*>(&k_contract<G, Q, H, 1>), \
+                                    reinterpret_cast<const void*>(&k_contract<G, Q, H, 2>)
+      UEGCCD_CONTRACT_NC(1, 4, 4), UEGCCD_CONTRACT_NC(1, 4, 8), UEGCCD_CONTRACT_NC(1, 8, 4),
+      UEGCCD_CONTRACT_NC(1, 8, 8), UEGCCD_CONTRACT_NC(2, 4, 4), UEGCCD_CONTRACT_NC(2, 4, 8),
+      UEGCCD_CONTRACT_NC(2, 8, 4), UEGCCD_CONTRACT_NC(2, 8, 8), UEGCCD_CONTRACT_NC(4, 4, 4),
+      UEGCCD_CONTRACT_NC(4, 4, 8), UEGCCD_CONTRACT_NC(4, 8, 4), UEGCCD_CONTRACT_NC(4, 8, 8)};
+#undef UEGCCD_CONTRACT_NC
   bool refused = false;
   for (const void* f : fs) refused |= !allow_dyn_lds(f);
   if (refused) (void)hipGetLastError();  // a refused limit leaves the default, and no error behind
@@ -2272,10 +2310,19 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
 
 template <int G, int QS, int HS>
 static void launch_contract_t(const BatchDev& b, hipStream_t s, hipEvent_t stop) {
-  const int grid = b.nhh_pad + b.nlad_pad + (b.ncon + kItemsPerBlock - 1) / kItemsPerBlock;
-  launch_k(k_contract<G, QS, HS>, stop, dim3(grid), dim3(256), (uint32_t)b.con_lds, s, (const SolveDev*)b.solves,
-           (const int*)b.hh_prefix, b.nhh, b.nhh_pad, (const int*)b.lad_prefix, b.nlad, b.nlad_pad,
-           (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max);
+  // (a batch of several solves runs two column tiles per ladder block, a
+  // solve alone one: fewer amplitude loads against more parallelism)
+  const int nc = b.n > 1 ? 2 : 1, nlad = nc == 2 ? b.nlad2 : b.nlad, nlad_pad = (nlad + 7) & ~7;
+  const int* lad_prefix = nc == 2 ? b.lad_prefix2 : b.lad_prefix;
+  const int grid = b.nhh_pad + nlad_pad + (b.ncon + kItemsPerBlock - 1) / kItemsPerBlock;
+  if (nc == 2)
+    launch_k(k_contract<G, QS, HS, 2>, stop, dim3(grid), dim3(256), (uint32_t)b.con_lds, s,
+             (const SolveDev*)b.solves, (const int*)b.hh_prefix, b.nhh, b.nhh_pad, lad_prefix,
+             nlad, nlad_pad, (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max);
+  else
+    launch_k(k_contract<G, QS, HS, 1>, stop, dim3(grid), dim3(256), (uint32_t)b.con_lds, s,
+             (const SolveDev*)b.solves, (const int*)b.hh_prefix, b.nhh, b.nhh_pad, lad_prefix,
+             nlad, nlad_pad, (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max);
 }
 
 template <int G>

@@ -145,8 +145,10 @@ struct SolveDevT {
   // ring / crossed-ring / Joo items
   Ptr<const int4> hh_items;
   int nhh;
-  Ptr<const int4> lad_items;
+  Ptr<const int4> lad_items;   // one 16-column tile each
   int nlad;
+  Ptr<const int4> lad_items2;  // two 16-column tiles each (the batches' blocks)
+  int nlad2;
   Ptr<const int4> con_items;
   int ncon;
   // mutable buffers
@@ -193,6 +195,7 @@ inline int contract_lds_bytes(int tab_doubles, int cls_max, int noo_max) {
 
 constexpr int kSliceClasses = 3;
 
+
 // Tile choices of the tuned kernels: k-steps per group of loads in the ladder
 // tiles (1, 2, 4) and per round of loads in the ring (4, 8) and hole-hole
 // (4, 8) contraction tiles.
@@ -218,6 +221,8 @@ struct BatchDev {
   int nhh, nhh_pad;
   int* lad_prefix;      // [n+1]
   int nlad, nlad_pad;
+  int* lad_prefix2;     // [n+1] the same over the two-tile items
+  int nlad2;
   int tab_doubles;      // largest ladder_tab_doubles of the batch
   int cls_max;          // largest K-block virtual list of the batch
   int con_lds;          // dynamic LDS bytes of k_contract
