@@ -312,6 +312,9 @@ def twist_stream(lib, L, n, rate):
                 "plan_ms_max": float(np.max([r.plan_ms for r in out]))}
 
     batch_wall, batch_out = served(reqs, np.zeros(n), "mixed", Server, options, max_active=n)
+    # the same batch with FP64 ladders throughout (what mixed precision buys
+    # where the GPU is full)
+    b64_wall, b64_out = served(reqs, np.zeros(n), "fp64", Server, options, max_active=n)
     srv_wall, srv_out = served(reqs, arrivals, "mixed", Server, options)
     return {
         "requests": n, "seed": 1234, "poisson_rate_per_s": rate,
@@ -322,6 +325,7 @@ def twist_stream(lib, L, n, rate):
                                      "converged": int(sum(st == L.OK for st, _, _ in pre)),
                                      "same_energies": bool(all(a[1] == b[1] for a, b in zip(seq, pre))),
                                      "note": "contexts built before the timed loop"},
+        "batched_fp64": summary(b64_wall, b64_out),
         "batched_mixed": summary(batch_wall, batch_out),
         "served_mixed": summary(srv_wall, srv_out),
     }

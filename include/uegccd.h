@@ -140,11 +140,15 @@ int uegccd_ctx_info(const uegccd_ctx* ctx, int32_t* nocc, int32_t* nao, int64_t*
 
 /* One residual evaluation, TermsWrapper (source/CCD.f90:146-181):
  * G = drivers + mosaics + ladders + rings + xrings, R = D*T - G.
- * Any of g2abab/g2abba/r2abab/r2abba may be NULL. */
+ * Any of g2abab/g2abba/r2abab/r2abba may be NULL.  Like uegccd_energy and
+ * uegccd_structure_factor, it copies its amplitudes into the context: they
+ * become the amplitudes uegccd_solve_resident(from_mp2 = 0) starts from, and
+ * the iteration state is reset. */
 int uegccd_residual(uegccd_ctx* ctx, const double* t2abab, const double* t2abba,
                     double* g2abab, double* g2abba, double* r2abab, double* r2abba);
 
-/* CCEnergy (source/CCD.f90:186-237). */
+/* CCEnergy (source/CCD.f90:186-237).  The amplitudes become the context's
+ * resident amplitudes (see uegccd_residual). */
 int uegccd_energy(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, double* ecorr);
 
 /* Structure factor of the amplitudes (host arrays, the layout above): replaces
@@ -156,7 +160,8 @@ int uegccd_energy(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, d
  * SfSum, SfV = v(q), SfDSum, SfXSum).  With qvec or vals NULL, *ncell receives the
  * row count and nothing is computed; otherwise *ncell is the capacity in rows on
  * entry (UEGCCD_ERR_INVALID if too small) and the row count on return.  *ecorr,
- * when given, receives the energy the reference routine also sets. */
+ * when given, receives the energy the reference routine also sets.  The
+ * amplitudes become the context's resident amplitudes (see uegccd_residual). */
 int uegccd_structure_factor(uegccd_ctx* ctx, const double* t2abab, const double* t2abba, int32_t* ncell,
                             int32_t* qvec, double* vals, double* ecorr);
 
@@ -179,7 +184,9 @@ int uegccd_solve(uegccd_ctx* ctx, double* t2aaaa, double* t2abab, double* t2abba
 /* As uegccd_solve, but the amplitudes stay in device memory: from_mp2 != 0
  * first writes the MP2 guess on the device (the caller-side copy of
  * source/Main.f90:85-87 without a host round trip); otherwise the solve starts
- * from the amplitudes the context holds (the last solve's or uegccd_mp2's).
+ * from the amplitudes the context holds: those of the last call that left
+ * amplitudes there (a solve's result, uegccd_mp2's guess, or the input of
+ * uegccd_residual, uegccd_energy or uegccd_structure_factor).
  * Read them with uegccd_get_amplitudes.  No counterpart in the reference; for
  * callers that keep T2 resident across solves (serving, benchmarks). */
 int uegccd_solve_resident(uegccd_ctx* ctx, int32_t from_mp2, uegccd_result* res);

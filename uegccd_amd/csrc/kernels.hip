@@ -1608,9 +1608,11 @@ __device__ __forceinline__ bool advance_state(const SolveG& S, State* st, const 
   const bool stop = conv || st->niter >= S.maxiter;
   bool want = false;
   if (S.mode == UEGCCD_MODE_DIIS) {
-    if (switched) {
-      st->nhist = 0;  // residuals of the FP32 phase do not enter FP64 extrapolation
-    } else {
+    // (the history carries over the precision switch: the FP32 phase's error
+    // vectors only shape the extrapolation, the FP64 residuals decide
+    // convergence; restarting it there cost a plain step and a rebuilt
+    // history, 5% more iterations over the 256-request stream)
+    {
       st->nhist = st->nhist < S.ndiis ? st->nhist + 1 : S.ndiis;
       for (int k = 0; k < S.ndiis; ++k) {
         st->bmat[slot * kMaxDiis + k] = vals[4 + k];
