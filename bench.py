@@ -260,6 +260,32 @@ def drpa_c2(lib, L, heg, info, CCDContext, options, reps, with_cpu):
     return out
 
 
+def drpa_scan_top(CCDContext, options, init_heg, reps):
+    """dRPA/SOSEX on the largest basis of the reference's dRPA basis scan
+    (N=14, r_s=1, Gamma, mosaics off, |n|^2 <= 665: 71,999 orbitals): the
+    host planning and placement of its context, the median device-inclusive
+    time per solve, and both energies against the scan's printed values."""
+    from uegccd_amd.inputfile import parse_input, standard_input
+    with open(os.path.join(ROOT, "tests", "golden", "reference_fixtures.json")) as f:
+        scan = json.load(f)["drpa_scan"]
+    p = scan["params"]
+    cut, ed, es, two_m = scan["rows"][-1]
+    info = parse_input(standard_input(p["nel"], cut, p["rs"], tuple(p["twist"]), **{
+        "Do Mosaics": p["do_mosaic"], "DodRPASOSEX": True, "fcut": p["fcut"]}))
+    t0 = time.perf_counter()
+    heg = init_heg(info)
+    ctx = CCDContext(heg, info, options(mode="reference"))
+    setup_ms = (time.perf_counter() - t0) * 1000.0
+    ms = []
+    for _ in range(reps):
+        r, esosex = ctx.drpa()
+        ms.append(r.solve_ms)
+    return {"workload": f"N=14 r_s=1.0 Gamma |n|^2<={cut} dRPA/SOSEX ({heg.NAO} orbitals), reference iteration",
+            "nao": heg.NAO, "two_m_scan": two_m, "setup_ms": setup_ms, "ms_per_solve": float(np.median(ms)),
+            "niter": r.niter, "ecorr_drpa": r.ecorr, "ecorr_sosex": esosex,
+            "abs_dE_drpa_vs_scan": abs(r.ecorr - ed), "abs_dE_sosex_vs_scan": abs(esosex - es)}
+
+
 def sf_c2(heg, info, CCDContext, options, reps, with_cpu):
     """Structure factor (uegccd_structure_factor, the reference's
     CCEnergyAndSF) of the MP2 amplitudes of the same N=54 |n|^2<=16 system:
@@ -489,6 +515,7 @@ def main():
     mixed = mixed_c2(lib, L, ecorr, heg, info, CCDContext, options, 10)
     stream = twist_stream(lib, L, args.stream, args.rate) if args.stream > 0 else None
     drpa = drpa_c2(lib, L, heg, info, CCDContext, options, 10, not args.no_cpu)
+    drpa_top = drpa_scan_top(CCDContext, options, init_heg, 3)
     sf = sf_c2(heg, info, CCDContext, options, 10, not args.no_cpu)
 
     cpu = None
@@ -520,6 +547,7 @@ def main():
         "mixed_precision": mixed,
         "twist_stream": stream,
         "drpa": drpa,
+        "drpa_scan_top": drpa_top,
         "structure_factor": sf,
     }
     print(json.dumps(out))

@@ -14,6 +14,8 @@ Sources (paths relative to the reference root):
   source/Test/fort.59     the 10 twists, averaged eigenvalues, special twist
   benchmarks/ZRun-dRPA/Output, fort.60      N=14 cutoff 5 (57 orbitals): MP2 and dRPA/SOSEX
   benchmarks/ZRun-v6-N14/fort.59, Output    N=14 cutoff 2, 100 twists
+  benchmarks/ZRun-dRPA/old-dRPA/*-data      N=14 r_s=1 dRPA/SOSEX basis scan, cutoffs
+                          5..665 (cutoff, Ec(dRPA), Ec(SOSEX), a run time, 2M)
 """
 from __future__ import annotations
 
@@ -141,6 +143,21 @@ def main(root="/root/reference"):
         "ehf": f(rows[0][3]), "e2": f(rows[0][4]), "ccd_ecorr": f(rows[0][5]),
         "eigen": blk[0]["eig"], "ccd_trace": blk[1]["trace"], "ccd_niter": blk[1]["niter"],
         "ccd_maxres": blk[1]["maxres"],
+    }
+    # --- N=14, r_s=1, Gamma dRPA/SOSEX scan over the cutoff (benchmarks/ZRun-dRPA/old-dRPA):
+    # three runs (high/med/low) whose energies agree row for row (the 2M
+    # column of the low run reads 0 in its cutoff-610 row); the fourth column
+    # (a run time) is left out ---
+    scans = [[ln.split() for ln in read(root, f"benchmarks/ZRun-dRPA/old-dRPA/{k}-1.0-data") if ln.strip()]
+             for k in ("high", "med", "low")]
+    for other in scans[1:]:
+        assert [r[:3] for r in other] == [r[:3] for r in scans[0]][:len(other)]
+    # (the Input has no fcut key, and 2M is the whole cutoff basis on every
+    # row: fcut 1000, 221,359 orbitals for N=14, truncates none of them)
+    out["drpa_scan"] = {
+        "params": {"nel": 14, "rs": 1.0, "twist": [0.0, 0.0, 0.0], "do_mosaic": False, "fcut": 1000.0},
+        "rows": [[int(r[0]), f(r[1]), f(r[2]), int(r[4])] for r in scans[0]],
+        "columns": ["cutoff", "ecorr_drpa", "ecorr_sosex", "two_m"],
     }
     path = os.path.join(HERE, "reference_fixtures.json")
     with open(path, "w") as fh:

@@ -288,3 +288,30 @@ def test_oracle_findindex_on_both_cutoffs(twisted):
             for nn in range(1, 8):
                 want = where.get(tuple(orc.nvec[m - 1] + orc.nvec[nn - 1] - orc.nvec[l - 1]), 0)
                 assert orc.findindex(m, nn, l) == want
+
+
+def _scan_info(cut, fixtures):
+    p = fixtures["drpa_scan"]["params"]
+    return parse_input(standard_input(p["nel"], cut, p["rs"], tuple(p["twist"]), **{
+        "Do Mosaics": p["do_mosaic"], "DodRPASOSEX": True, "fcut": p["fcut"]}))
+
+
+def test_basis_sizes_of_the_drpa_scan(fixtures):
+    """Every row of the reference's dRPA basis scan (N=14, r_s=1, cutoffs
+    5..665): the host builder's basis has the scan's 2M spin orbitals."""
+    from uegccd_amd.driver import init_heg
+    for cut, _, _, two_m in fixtures["drpa_scan"]["rows"]:
+        assert 2 * init_heg(_scan_info(cut, fixtures)).NAO == two_m, cut
+
+
+def test_planning_takes_the_largest_scan_basis(fixtures):
+    """A context of the scan's largest basis (71,999 orbitals) plans on the
+    host: its footprint is known without a device, and the amplitude arrays
+    (o*o*v entries) dominate it."""
+    from uegccd_amd.driver import footprint, init_heg, options
+    info = _scan_info(665, fixtures)
+    heg = init_heg(info)
+    assert heg.NAO == 71999
+    n3 = heg.NOcc ** 2 * (heg.NAO - heg.NOcc)
+    b = footprint(heg, info, options(mode="reference"))
+    assert 16 * n3 * 4 < b < 16 * n3 * 40

@@ -385,3 +385,20 @@ def test_exchange_structure_factor_pins(fixtures):
     kin = ((2.0 * np.pi / s.L) ** 2 * ((n + tw) ** 2).sum(axis=1)).sum()  # 2 x sum 1/2 |k|^2
     xhf = -(cnt * exv).sum()
     assert abs(kin + xhf + s.madelung * 0.5 * 2 * no - n24["ehf"]) < F18_12
+
+
+def test_drpa_scan_energies(fixtures):
+    """The reference's dRPA/SOSEX basis scan (benchmarks/ZRun-dRPA/old-dRPA:
+    N=14, r_s=1, Gamma, mosaics off): Ec(dRPA) and Ec(SOSEX) within their
+    twelve printed decimals, and the basis size, on the rows up to cutoff 50
+    (1,503 orbitals)."""
+    sc = fixtures["drpa_scan"]
+    rows = [r for r in sc["rows"] if r[0] <= 50]
+    assert len(rows) == 10
+    for cut, ed, es, two_m in rows:
+        s = _sys(sc["params"], maxkpoint=cut)
+        assert 2 * s.nao == two_m
+        r = s.drpa()
+        assert not r["failed"]
+        assert abs(r["ecorr"] - ed) < F18_12, cut
+        assert abs(r["esosex"] - es) < F18_12, cut

@@ -250,6 +250,8 @@ struct uegccd_ctx {
   std::map<int, hipGraphExec_t> graphs;
   Tiles tiles;               // tile choice of the tuned kernels (tuned table or uegccd_set_tiles)
   int noo_max = 0, nvv_max = 0;  // largest pair-momentum block: pairs, virtual columns
+  ElemCode ecode{};          // bit widths of the element map entries
+  int vo2max = 0;            // largest |n_a - n_k|^2 of a virtual a and an occupied k
 };
 
 namespace {
@@ -281,6 +283,8 @@ void for_each_valid(const uegccd_ctx* c, F f) {
 
 int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   const int o = s.nocc, nao = s.nao, v = nao - o;
+  // (NOcc is staged per slice in LDS; element indices are ints)
+  if (o > kMaxOcc || (int64_t)o * o * v >= (int64_t(1) << 31)) return UEGCCD_ERR_UNSUPPORTED;
   c->o = o;
   c->v = v;
   c->nao = nao;
@@ -413,6 +417,10 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
   for (int K = 0; K < c->nK; ++K)
     c->kb_toff[K + 1] = c->kb_toff[K] + (c->kb_voff[K + 1] - c->kb_voff[K]) * (c->kb_poff[K + 1] - c->kb_poff[K]);
   c->nvalid = c->slab_coff[v];
+  // element map entries: element, position in tk and partner in 64 bits
+  c->ecode.xb = bits_for(c->n3);
+  c->ecode.pb = bits_for(c->kb_toff[c->nK]);
+  if (c->ecode.xb + c->ecode.pb + bits_for(nao) > 64) return UEGCCD_ERR_UNSUPPORTED;
   // the valid elements with a core index (held at zero), for the input checks
   c->core_elems.clear();
   if (coren)
@@ -482,7 +490,6 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
       }
     }
   }
-  if (n2max >= kMaxN2 - 1 || nao > kMaxNao || o > kMaxOcc) return UEGCCD_ERR_UNSUPPORTED;
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, in order of first
   // occurrence over (p, i), for the quadratic ring products
   {
@@ -609,9 +616,12 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     for (int sv = 0; sv < v; ++sv)
       for (int k = 0; k < o; ++k) vso[(size_t)sv * o + k] = vt[d2(o + sv, k)];
   }
+  c->vo2max = 0;
+  for (int sv = 0; sv < v; ++sv)
+    for (int k = 0; k < o; ++k) c->vo2max = std::max(c->vo2max, d2(o + sv, k));
   // LDS of the slice blocks (one table; two with split slice roles)
   const bool split = slice_split_of(c->rslot);
-  if (slice_lds_bytes(o, split) + (int)sizeof(int4) * kMaxNao +
+  if (slice_lds_bytes(o, split) + (int)sizeof(int4) * kMaxOcc +
       (int)sizeof(double) * ((split ? 2 : 1) * kMaxN2 + 16 + kMaxDiis * kMaxDiis + kMaxDiis) +
       slice_fin_lds(slice_threads(o)) > kMaxDynLds)
     return UEGCCD_ERR_UNSUPPORTED;
@@ -743,6 +753,17 @@ int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
   return UEGCCD_OK;
 }
 
+// Whether the CCD pass kernels take a context's basis: the slice blocks' LDS
+// Coulomb table must hold every virtual-occupied difference of momenta, and a
+// contraction block's LDS the ladder tables, the largest pair-momentum
+// block's virtual list and its W rows.  (dRPA, MP2, the energy and the
+// structure factor run on any basis a context holds.)
+int ccd_check(const uegccd_ctx* c) {
+  if (c->vo2max > kMaxN2 - 2) return UEGCCD_ERR_UNSUPPORTED;
+  const int lds = contract_lds_bytes(ladder_tab_doubles(c->sd), std::max(1, c->nvv_max), c->noo_max);
+  return lds <= contract_lds_limit() ? UEGCCD_OK : UEGCCD_ERR_UNSUPPORTED;
+}
+
 int read_state(uegccd_ctx* c, State* st) {
   HIPCHK(hipMemcpyAsync(st, c->d_state, sizeof(State), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -868,6 +889,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   SolveDev& S = c->sd;
   S = SolveDev{};
   S.o = c->o; S.v = c->v; S.nao = c->nao; S.nK = c->nK; S.n3 = c->n3; S.n2max = c->n2max;
+  S.ecode = c->ecode;
   S.ndiis = ndiis;
   S.mode = opt.mode;
   S.prec0 = opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
@@ -1122,6 +1144,7 @@ int uegccd_ctx_info(const uegccd_ctx* c, int32_t* nocc, int32_t* nao, int64_t* n
 int uegccd_residual(uegccd_ctx* c, const double* tab, const double* tba, double* gab,
                     double* gba, double* rab, double* rba) {
   if (!c || !tab || !tba) return UEGCCD_ERR_INVALID;
+  if (int rc = ccd_check(c)) return rc;
   HIPCHK(hipSetDevice(c->device));
   int rc = copy_in_amplitudes(c, tab, tba);
   if (rc) return rc;
@@ -1565,8 +1588,10 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   uegccd_ctx* c0 = cs[0];
   hipStream_t s = c0->stream;
   const int prec64 = c0->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1;
-  for (int k = 0; k < n; ++k)
+  for (int k = 0; k < n; ++k) {
     if ((cs[k]->opt.precision == UEGCCD_PREC_MIXED ? 0 : 1) != prec64) return UEGCCD_ERR_INVALID;
+    if (int rc = ccd_check(cs[k])) return rc;
+  }
   int maxiter = 0;
   for (int k = 0; k < n; ++k) maxiter = std::max(maxiter, cs[k]->maxiter);
   // events and the pinned state buffer are made once per context and reused
@@ -1925,6 +1950,7 @@ extern "C" int uegccd_get_tiles(const uegccd_ctx* c, uegccd_tiles* t, int32_t* n
 
 extern "C" int uegccd_kernel_time(uegccd_ctx* c, int32_t which, int32_t reps, double* avg_us) {
   if (!c || !avg_us || reps <= 0 || which < 0 || which > 3 || which == 2) return UEGCCD_ERR_INVALID;
+  if (int rc = ccd_check(c)) return rc;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   // the ladders in the context's starting precision (FP32 for mixed precision)
@@ -2434,8 +2460,7 @@ int plan_request(Request& r) {
   // the constant section at a placeholder base: admission writes the descriptor
   size_t work_off = 0;
   layout_ctx(r.ctx, &r.sys, nullptr, (size_t)r.footprint, r.image, &work_off, true);
-
-  return UEGCCD_OK;
+  return ccd_check(r.ctx);
 }
 
 void planner_loop(uegccd_server* s) {

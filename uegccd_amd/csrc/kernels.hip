@@ -28,6 +28,8 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "uegccd_internal.h"
 #include "../../include/uegccd.h"
 
@@ -171,22 +173,27 @@ __device__ __forceinline__ double block_reduce_scatter(double (&v)[NQ], double* 
   return r;
 }
 
-// Coulomb table and n-vectors staged in LDS.
+// The slice blocks' LDS tables: the occupied n-vectors and the Coulomb table
+// v(|dn|^2) up to slice_n2(S) (zeros above it).
 struct Lds {
-  int4 nv[kMaxNao];
+  int4 nv[kMaxOcc];
   double vt[kMaxN2];
   double red[16];
 };
 
-// Stages the n-vectors and the Coulomb table of one slot.
+// The last Coulomb-table entry a slice block stages: the table's own end, or
+// kMaxN2 - 2 for a larger basis (the slice blocks read it at the differences
+// of a virtual and an occupied momentum, which the host keeps below that).
+__device__ __forceinline__ int slice_n2(const SolveG& S) { return min(S.n2max, kMaxN2 - 2); }
+
+// Stages the occupied n-vectors and the Coulomb table of one slot.
 __device__ __forceinline__ void stage(const SolveG& S, Lds& L, int slot) {
   const auto vt = S.vtab + (size_t)slot * (S.n2max + 1);
-  for (int k = threadIdx.x; k < S.nao; k += blockDim.x) L.nv[k] = S.nv4[k];
-  for (int k = threadIdx.x; k <= S.n2max; k += blockDim.x) L.vt[k] = vt[k];
+  const int n2 = slice_n2(S);
+  for (int k = threadIdx.x; k < S.o; k += blockDim.x) L.nv[k] = S.nv4[k];
+  for (int k = threadIdx.x; k <= n2; k += blockDim.x) L.vt[k] = vt[k];
   __syncthreads();
 }
-
-#define VPQ(p, q) (L.vt[d2(L.nv[p], L.nv[q])])
 
 // Slot-major block partials: quantity q of element block lb.
 __device__ __forceinline__ GlobalPtr<double> partial(const SolveG& S, int q, int lb) {
@@ -199,34 +206,36 @@ __device__ __forceinline__ GlobalPtr<double> partial(const SolveG& S, int q, int
 __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restrict__ solves,
                                                          const int* __restrict__ prefix, int n,
                                                          int drpa) {
-  __shared__ Lds L;
+  __shared__ double red[16];
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
-  stage(S, L, S.rslot[RO_ENERGY]);
   const int o = S.o, o2 = o * o;
+  // v(n_p - n_k) of a virtual p and an occupied k, from the energy's table
+  const auto vs = S.vso + (size_t)S.rslot[RO_ENERGY] * S.v * o;
   const int c = lb * blockDim.x + threadIdx.x;  // compact index of a valid element
   double e = 0.0, tmax = 0.0, bad = 0.0;
   if (c < S.nvalid) {
     const uint2 cm = ld(S.cmap + c);
-    const int x = (int)cm.x;
-    const int i = x % o, j = (x / o) % o, a = o + x / o2;
+    const int x = elem_x(S.ecode, cm);
+    const int i = x % o, j = (x / o) % o, av = x / o2;
     const double2 t = S.tcur[x];
-    st2(S.tk + elem_pos(cm), t);
+    st2(S.tk + elem_pos(S.ecode, cm), t);
     const double taa = t.x + t.y;
+    const double vai = vs[av * o + i];
     if (drpa == 1)  // the dRPA module's CCEnergy with CSosex 0 (source/dRPA.f90:200-208)
-      e = VPQ(a, i) * t.x * 2;
+      e = vai * t.x * 2;
     else if (drpa == 2)  // the same with CSosex 1: Ec(SOSEX)
-      e = VPQ(a, i) * t.x * 2 - VPQ(elem_partner(cm), i) * t.x;
+      e = vai * t.x * 2 - vs[(elem_partner(S.ecode, cm) - o) * o + i] * t.x;
     else
-      e = 0.5 * (VPQ(a, i) * (taa + t.x) - VPQ(a, j) * (taa + t.y));
+      e = 0.5 * (vai * (taa + t.x) - vs[av * o + j] * (taa + t.y));
     tmax = fmax(fabs(t.x), fabs(t.y));
     if (!isfinite(t.x) || !isfinite(t.y)) bad = 1.0;
   }
-  e = block_sum(e, L.red);
-  tmax = block_max(tmax, L.red);
-  bad = block_sum(bad, L.red);
+  e = block_sum(e, red);
+  tmax = block_max(tmax, red);
+  bad = block_sum(bad, red);
   if (threadIdx.x == 0) {
     *partial(S, 0, lb) = e;
     *partial(S, 1, lb) = tmax;
@@ -239,20 +248,21 @@ __global__ __launch_bounds__(kElemPerBlock) void k_init(const SolveDev* __restri
 __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restrict__ solves,
                                                         const int* __restrict__ prefix, int n,
                                                         int drpa) {
-  __shared__ Lds L;
+  __shared__ double red[16];
   const int gb = xcd_block(blockIdx.x, gridDim.x);
   const int s = find_solve(prefix, n, gb);
   const SolveG& S = as_global(solves)[s];
   const int lb = gb - prefix[s];
-  stage(S, L, 0);  // DrvMBPT's ERI calls pass no flag (source/MP2.f90:61-62): flag 0
   const int o = S.o, o2 = o * o;
+  // DrvMBPT's ERI calls pass no flag (source/MP2.f90:61-62): table slot 0
+  const auto vs = S.vso;
   const int c = lb * blockDim.x + threadIdx.x;  // compact index of a valid element
   double e = 0.0;
   if (c < S.nvalid) {
     const uint2 cm = ld(S.cmap + c);
-    const int x = (int)cm.x, b = elem_partner(cm);
-    const int i = x % o, j = (x / o) % o, a = o + x / o2;
-    const double vab = VPQ(i, a), vba = VPQ(i, b);
+    const int x = elem_x(S.ecode, cm), b = elem_partner(S.ecode, cm);
+    const int i = x % o, j = (x / o) % o, av = x / o2, a = o + av;
+    const double vab = vs[av * o + i], vba = vs[(b - o) * o + i];
     const double den = S.eig[i] + S.eig[j] - S.eig[a] - S.eig[b];
     const double tab = vab / den, tba = -vba / den;
     const double taa = tab + tba;
@@ -260,7 +270,7 @@ __global__ __launch_bounds__(kElemPerBlock) void k_mp2(const SolveDev* __restric
     // dRPA starts from T2aaaa = T2abab + T2abba (source/Main.f90:85,91)
     S.tcur[x] = drpa ? make_double2(taa, 0.0) : make_double2(tab, tba);
   }
-  e = block_sum(e, L.red);
+  e = block_sum(e, red);
   if (threadIdx.x == 0) {
     *partial(S, 0, lb) = e;
     *partial(S, 1, lb) = 0.0;
@@ -501,11 +511,11 @@ __device__ __forceinline__ double2 combine_history(const double (&cf)[P], const 
 // cm = elem_map(element, position in tk, partner), xs the slab's first element.
 __device__ __forceinline__ void slab_place(const SolveG& S, double* Tab, double* Tba, int xs, uint2 cm,
                                            double2 t) {
-  const int x = (int)cm.x;
+  const int x = elem_x(S.ecode, cm);
   Tab[x - xs] = t.x;
   Tba[x - xs] = t.y;
   st2(S.tcur + x, t);
-  st2(S.tk + elem_pos(cm), t);
+  st2(S.tk + elem_pos(S.ecode, cm), t);
 }
 
 // One block of BT threads per virtual slice s (BT = slice_threads of the
@@ -606,7 +616,7 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
       vsr[k] = vso_r[sv * o + k];
       vsx[k] = vso_x[sv * o + k];
     }
-    for (int k = t0; k < kMaxN2; k += BT) vt2[k] = k <= S.n2max ? vt_x[k] : 0.0;
+    for (int k = t0; k < kMaxN2; k += BT) vt2[k] = k <= slice_n2(S) ? vt_x[k] : 0.0;
   }
   stage(S, L, S.rslot[RO_RING_QX]);
   if (phase == PH_DONE) return;  // the whole block: the solve has finished
@@ -619,10 +629,12 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
     }
     __syncthreads();
   }
-  // Y below is looked up at n_i + n_k - n_s - n_l, which leaves the table
-  // whenever (i,k,s) has no partner; those terms multiply a zero amplitude,
-  // so the table reads as zero there (the last entry is always past n2max).
-  for (int k = S.n2max + 1 + t0; k < kMaxN2; k += BT) L.vt[k] = 0.0;
+  // Y below is looked up at n_i + n_k - n_s - n_l = n_b - n_l, b the partner
+  // of (i,k,s): a virtual-occupied difference, within the staged table.
+  // Without a partner the index can leave the table; those terms multiply a
+  // zero amplitude, so the table reads as zero there (the last entry is
+  // always past the staged ones).
+  for (int k = slice_n2(S) + 1 + t0; k < kMaxN2; k += BT) L.vt[k] = 0.0;
   if (src == SRC_TCUR) {
     for (int idx = t0; idx < o2; idx += BT) {
       const double2 t = ld(S.tcur + xs + idx);
@@ -722,7 +734,7 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
   // their stores fill runs of one J row.
   constexpr int kJL = 3;
   const int G = (o + kJL - 1) / kJL;
-  const int4 ns = L.nv[so];
+  const int4 ns = S.nv4[so];
   for (int idx = threadIdx.x; idx < o * G; idx += BT) {
     const int i = idx / G, g = idx - i * G;
     const int4 ni = L.nv[i];
@@ -1398,9 +1410,9 @@ __device__ __forceinline__ void main_block(const SolveDev* __restrict__ solves, 
   int x = 0, b = -1, pos = 0;
   if (cp < S.nvalid) {
     const uint2 cm = ld(S.cmap + cp);
-    x = (int)cm.x;
-    b = elem_partner(cm);
-    pos = elem_pos(cm);
+    x = elem_x(S.ecode, cm);
+    b = elem_partner(S.ecode, cm);
+    pos = elem_pos(S.ecode, cm);
   }
   if (S.do_mosaic)  // Joo from its slice-range parts, in part order
     for (int k = threadIdx.x; k < o; k += blockDim.x) {
@@ -1758,7 +1770,7 @@ __global__ __launch_bounds__(256) void k_connectivity(const SolveDev* __restrict
   const int c = blockIdx.x * blockDim.x + threadIdx.x;  // compact index of a valid element
   if (c < S.nvalid) {
     const uint2 cm = ld(S.cmap + c);
-    const int x = (int)cm.x, b = elem_partner(cm);
+    const int x = elem_x(S.ecode, cm), b = elem_partner(S.ecode, cm);
     const int o = S.o, i = x % o, a = o + x / (o * o);
     const int4 ni = S.nv4[i];
     const int d1 = d2(ni, S.nv4[a]), d2b = d2(ni, S.nv4[b]);
@@ -1883,8 +1895,8 @@ __global__ __launch_bounds__(kElemThreads) void k_drpa_main(const SolveDev* __re
   int x = 0, b = -1;
   if (cp < S.nvalid) {
     const uint2 cm = ld(S.cmap + cp);
-    x = (int)cm.x;
-    b = elem_partner(cm);
+    x = elem_x(S.ecode, cm);
+    b = elem_partner(S.ecode, cm);
   }
   __shared__ double joo_l[kMaxOcc];
   if (S.do_mosaic && (int)threadIdx.x < o) {
@@ -2287,6 +2299,30 @@ static void launch_k(F kernel, hipEvent_t stop, dim3 grid, dim3 block, uint32_t 
 
 void prepare_launches() { allow_large_lds(); }
 
+int contract_lds_limit() {
+  static const int lim = [] {
+    const void* fs[] = {
+#define UEGCCD_CONTRACT_NC(G, Q, H) reinterpret_cast<const void*>(&k_contract<G, Q, H, 1>), \
+                                    reinterpret_cast<const void*>(&k_contract<G, Q, H, 2>)
+        UEGCCD_CONTRACT_NC(1, 4, 4), UEGCCD_CONTRACT_NC(1, 4, 8), UEGCCD_CONTRACT_NC(1, 8, 4),
+        UEGCCD_CONTRACT_NC(1, 8, 8), UEGCCD_CONTRACT_NC(2, 4, 4), UEGCCD_CONTRACT_NC(2, 4, 8),
+        UEGCCD_CONTRACT_NC(2, 8, 4), UEGCCD_CONTRACT_NC(2, 8, 8), UEGCCD_CONTRACT_NC(4, 4, 4),
+        UEGCCD_CONTRACT_NC(4, 4, 8), UEGCCD_CONTRACT_NC(4, 8, 4), UEGCCD_CONTRACT_NC(4, 8, 8)};
+#undef UEGCCD_CONTRACT_NC
+    int m = kMaxDynLds;
+    for (const void* f : fs) {
+      hipFuncAttributes a;
+      if (hipFuncGetAttributes(&a, f) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+      }
+      m = std::min(m, kMaxDynLds - (int)a.sharedSizeBytes);
+    }
+    return m;
+  }();
+  return lim;
+}
+
 static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nullptr, int fin_prev = 0) {
   allow_large_lds();
   const auto* sd = (const SolveDev*)b.solves;
@@ -2493,7 +2529,7 @@ __global__ __launch_bounds__(256) void k_plan_cmap(const PlanArgs* __restrict__ 
     if (ci >= 0) {
       const int noo = S.kb_poff[K + 1] - S.kb_poff[K];
       const int pos = S.kb_toff[K] + ci * noo + S.pair_pos[pr];
-      cmap[base + run + rank] = elem_map(pr + o2 * av, pos, S.kb_vb[S.kb_voff[K] + ci]);
+      cmap[base + run + rank] = elem_map(S.ecode, pr + o2 * av, pos, S.kb_vb[S.kb_voff[K] + ci]);
     }
     run += tot;
   }

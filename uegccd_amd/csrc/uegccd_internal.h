@@ -21,19 +21,36 @@ constexpr int kPartialStride = 4 + kMaxDiis;  // E, dT, max|R|, non-finite count
 constexpr int kElemThreads = 256;                 // threads of the element-wise blocks
 constexpr int kElemPerBlock = kElemThreads;       // amplitudes per element block (one per thread)
 constexpr int kFinThreads = 256;                  // threads of the per-solve reduction block
-constexpr int kMaxNao = 1024;                 // orbitals staged in LDS
-constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the Coulomb table
+constexpr int kMaxN2 = 1024;                  // |dn|^2 range of the slice blocks' LDS Coulomb table
 constexpr int kMaxOcc = 64;                   // NOcc staged per slice in LDS
 
-// Map entry of a valid element: (element x, position in tk | partner b << 22).
-constexpr int kPosBits = 22;
-static_assert((long long)kMaxOcc * kMaxOcc * kMaxNao <= (1LL << kPosBits), "tk positions fit 22 bits");
-static_assert(kMaxNao <= (1 << (32 - kPosBits)), "orbital indices fit 10 bits");
-__host__ __device__ inline uint2 elem_map(int x, int pos, int b) {
-  return make_uint2((unsigned)x, (unsigned)pos | ((unsigned)b << kPosBits));
+// Map entry of a valid element, 64 bits (x low word, y high word): the
+// element x in the low xb bits, its position in tk in the next pb bits, its
+// partner b above them.  The widths are per solve (xb the bits of n3 - 1,
+// pb those of the tk size - 1), so a basis fits while xb + pb + the bits of
+// nao - 1 stay within 64.
+struct ElemCode {
+  int xb, pb;
+};
+__host__ __device__ inline uint2 elem_map(ElemCode c, int x, int pos, int b) {
+  const uint64_t m = (uint64_t)(unsigned)x | ((uint64_t)(unsigned)pos << c.xb) |
+                     ((uint64_t)(unsigned)b << (c.xb + c.pb));
+  return make_uint2((unsigned)m, (unsigned)(m >> 32));
 }
-__host__ __device__ inline int elem_pos(uint2 m) { return (int)(m.y & ((1u << kPosBits) - 1u)); }
-__host__ __device__ inline int elem_partner(uint2 m) { return (int)(m.y >> kPosBits); }
+__host__ __device__ inline uint64_t elem_word(uint2 m) { return (uint64_t)m.x | ((uint64_t)m.y << 32); }
+__host__ __device__ inline int elem_x(ElemCode c, uint2 m) { return (int)(m.x & ((1u << c.xb) - 1u)); }
+__host__ __device__ inline int elem_pos(ElemCode c, uint2 m) {
+  return (int)((elem_word(m) >> c.xb) & ((1ull << c.pb) - 1ull));
+}
+__host__ __device__ inline int elem_partner(ElemCode c, uint2 m) {
+  return (int)(elem_word(m) >> (c.xb + c.pb));
+}
+// Bits that hold the values 0 .. n - 1 (at least one).
+inline int bits_for(int64_t n) {
+  int k = 1;
+  while (k < 63 && (int64_t(1) << k) < n) ++k;
+  return k;
+}
 
 // The places where a Coulomb factor enters the reformulated terms, each with
 // its own range flag (source/CCD.f90: the DummyFlag of every ERI call, a Max
@@ -127,8 +144,9 @@ struct SolveDevT {
   Ptr<const int> kb_toff;   // [nK+1] offsets of the K blocks in tk
   // the valid elements in element order (compact index c; the history layout)
   int nvalid;
+  ElemCode ecode;           // bit widths of the cmap entries
   Ptr<const int> slab_coff; // [v+1] first compact index of each virtual slice
-  Ptr<const uint2> cmap;    // [nvalid] elem_map(element x, position in tk, partner b)
+  Ptr<const uint2> cmap;    // [nvalid] elem_map(ecode, element x, position in tk, partner b)
   Ptr<const int> pair_kb;   // [o*o] block of pair x
   Ptr<const int> pair_pos;  // [o*o] position of pair x inside its block
   // hole-particle pair sets P(q) = {(i,p): n_p - n_i = q}, one per transfer q
@@ -299,6 +317,9 @@ int launch_persistent(const BatchDev& b, int dyn_lds, int passes, hipStream_t s)
 // One-time per-kernel set-up (dynamic LDS limits), done before any stream
 // capture.
 void prepare_launches();
+// Dynamic LDS every k_contract instance can take (the LDS of a CU less the
+// largest static LDS among them; 0 when the runtime cannot say).
+int contract_lds_limit();
 // One residual pass: k_slice, k_contract, k_main.  fin_prev: the pass's
 // k_slice first runs the reductions and state update of the pass before it
 // (every block the same; one writes the state), so that only the last pass
