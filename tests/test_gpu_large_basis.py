@@ -11,9 +11,13 @@
   the DIIS solve's amplitudes a fixed point of the oracle's equations (its
   max|R| there below the solve's tolerance, its energy there within 1e-12 Ha
   of the solve's).
-- A basis whose pair-momentum blocks outgrow a contraction block's LDS
-  (cutoff 400, 33,401 orbitals) is refused for CCD with
-  UEGCCD_ERR_UNSUPPORTED, while dRPA, MP2 and the energy run on it.
+- CCD on N=2 |n|^2 <= 120 (5,497 orbitals): its one pair-momentum block's
+  virtual list (5,496 columns) goes through LDS in two parts; the residual
+  terms at the MP2 amplitudes and the converged DIIS solve against the
+  oracle as above.
+- A basis with a virtual-occupied |dn|^2 past the slice blocks' LDS
+  Coulomb table (N=2 |n|^2 <= 1050, 142,945 orbitals) is refused for CCD
+  with UEGCCD_ERR_UNSUPPORTED, while MP2 and the energy run on it.
 """
 import numpy as np
 import pytest
@@ -49,12 +53,9 @@ def test_drpa_basis_scan(fixtures, cut):
     assert abs(esosex - row[2]) < SCAN_TOL
 
 
-def test_ccd_past_a_thousand_orbitals(fixtures):
+def _ccd_against_oracle(info):
     from uegccd_amd.driver import CCDContext, init_heg, options
-    print("[large] CCD N=14 cutoff 50", flush=True)
-    info = _scan_info(50, fixtures, **{"Do Mosaics": True, "DodRPASOSEX": False})
     heg = init_heg(info)
-    assert heg.NAO == 1503
     orc = oracle_for(info)
     e2o, _, tab0, tba0 = orc.mp2()
     ctx = CCDContext(heg, info, options(mode="diis", ndiis=8))
@@ -70,14 +71,28 @@ def test_ccd_past_a_thousand_orbitals(fixtures):
     _, _, o_rab, o_rba = orc.terms(r.tab, r.tba, mask=31)
     assert max(np.abs(o_rab).max(), np.abs(o_rba).max()) < 1e-7
     assert abs(orc.energy(r.tab + r.tba, r.tab, r.tba) - r.ecorr) < 1e-12
+    return heg
 
 
-def test_ccd_refused_where_the_blocks_outgrow_lds(fixtures):
+def test_ccd_past_a_thousand_orbitals(fixtures):
+    print("[large] CCD N=14 cutoff 50", flush=True)
+    heg = _ccd_against_oracle(_scan_info(50, fixtures, **{"Do Mosaics": True, "DodRPASOSEX": False}))
+    assert heg.NAO == 1503
+
+
+def test_ccd_virtual_list_in_parts():
+    print("[large] CCD N=2 cutoff 120", flush=True)
+    heg = _ccd_against_oracle(parse_input(standard_input(2, 120, 1.0, fcut=1000.0)))
+    assert (heg.NAO, heg.NOcc) == (5497, 1)
+
+
+def test_ccd_refused_past_the_slice_table():
     from uegccd_amd import lib as L
     from uegccd_amd.driver import CCDContext, init_heg, options
-    print("[large] CCD refusal at cutoff 400", flush=True)
-    info = _scan_info(400, fixtures)
+    print("[large] CCD refusal N=2 cutoff 1050", flush=True)
+    info = parse_input(standard_input(2, 1050, 1.0, fcut=10000.0))
     heg = init_heg(info)
+    assert heg.NAO > 140000
     ctx = CCDContext(heg, info, options(mode="reference"))
     e2, _, tab, tba = ctx.mp2()
     assert e2 < 0.0

@@ -753,14 +753,18 @@ int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
   return UEGCCD_OK;
 }
 
+// Entries of a context's K-block virtual lists a contraction block holds in
+// LDS at once: the largest list, up to kClsChunk (longer ones go in parts).
+int cls_entries(const uegccd_ctx* c) { return std::max(1, std::min(c->nvv_max, kClsChunk)); }
+
 // Whether the CCD pass kernels take a context's basis: the slice blocks' LDS
 // Coulomb table must hold every virtual-occupied difference of momenta, and a
-// contraction block's LDS the ladder tables, the largest pair-momentum
+// contraction block's LDS the ladder tables, a part of a pair-momentum
 // block's virtual list and its W rows.  (dRPA, MP2, the energy and the
 // structure factor run on any basis a context holds.)
 int ccd_check(const uegccd_ctx* c) {
   if (c->vo2max > kMaxN2 - 2) return UEGCCD_ERR_UNSUPPORTED;
-  const int lds = contract_lds_bytes(ladder_tab_doubles(c->sd), std::max(1, c->nvv_max), c->noo_max);
+  const int lds = contract_lds_bytes(ladder_tab_doubles(c->sd), cls_entries(c), c->noo_max);
   return lds <= contract_lds_limit() ? UEGCCD_OK : UEGCCD_ERR_UNSUPPORTED;
 }
 
@@ -991,7 +995,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
     b.nlad2 = S.nlad2;
   }
   b.tab_doubles = ladder_tab_doubles(S);
-  b.cls_max = std::max(1, c->nvv_max);
+  b.cls_max = cls_entries(c);
   b.con_lds = contract_lds_bytes(b.tab_doubles, b.cls_max, c->noo_max);
   b.tiles = c->tiles;
   b.main_nq = main_nq_of(S);
@@ -1428,7 +1432,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
     lp2[k + 1] = lp2[k] + c->sd.nlad2;
     hp[k + 1] = hp[k] + c->sd.nhh;
     tab_doubles = std::max(tab_doubles, ladder_tab_doubles(c->sd));
-    cls_max = std::max(cls_max, c->nvv_max);
+    cls_max = std::max(cls_max, cls_entries(c));
     noo_max = std::max(noo_max, c->noo_max);
     cp[k + 1] = cp[k] + c->sd.ncon;
   }

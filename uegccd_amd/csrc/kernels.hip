@@ -801,6 +801,22 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
 // hole-hole ladder from there (source/CCD.f90:651-656):
 //   hh_S[p,a] = sum_q W_S[p,q] S[q,a],  hh_P[p,a] = sum_q W_P[p,q] P[q,a].
 
+// The part of block K's virtual list from entry base on, in LDS for the
+// tile loops: a list of up to kClsChunk entries is staged once with the
+// ladder tables (stage_ladder); a longer one part by part here, behind block
+// barriers (every thread of the block calls this, base uniform).  Returns
+// the entries of the part.
+__device__ __forceinline__ int restage_cls(const SolveG& S, int K, int base, int nvv, const int4* cls) {
+  if (nvv > kClsChunk) {
+    const int voff = S.kb_voff[K], n = min(kClsChunk, nvv - base);
+    int4* const part = const_cast<int4*>(cls);  // (the block's own LDS)
+    __syncthreads();  // every wave is done with the part before
+    for (int k = threadIdx.x; k < n; k += blockDim.x) part[k] = ld(S.kb_nva + voff + base + k);
+    __syncthreads();
+  }
+  return min(nvv - base, kClsChunk);
+}
+
 // The sums of one 16x16 tile (column lane r16 synthesises its B column from
 // the orbitals cx, cy and the tables vtx, vty), the reduction over the K
 // block's virtual list split over the block's four waves (wave w takes steps
@@ -835,32 +851,40 @@ __device__ __forceinline__ bool ladder_sums(const SolveG& S, int K, int row0, co
   // this wave's k-steps are c0 = 4*wave + 16*m; groups of G steps issue all
   // their amplitude loads (unconditional, in-range addresses, masked values)
   // before the MFMAs, which run for every step (masked steps add zeros).
-  // G comes from the tuned tile table (uegccd_amd/tuned_tiles.json).
-  for (int g0 = 4 * wave; g0 < nvv; g0 += 16 * G) {
-    double2 t[G];
+  // G comes from the tuned tile table (uegccd_amd/tuned_tiles.json).  A list
+  // longer than kClsChunk is staged a part at a time (parts a multiple of
+  // every group long, so each wave takes the same steps in the same order,
+  // and no group straddles two parts).
+  for (int base = 0; base < nvv; base += kClsChunk) {
+    // (indices from base: entries left, entries of this part, their amplitudes)
+    const int nleft = nvv - base, end = restage_cls(S, K, base, nvv, cls);
+    const auto TKp = TK + (size_t)base * noo;
+    for (int g0 = 4 * wave; g0 < end; g0 += 16 * G) {
+      double2 t[G];
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
-      const int ci = g0 + 16 * st + kq;
-      const bool ok = ci < nvv && rowA < noo;
-      const double2 v = ld(TK + (ok ? ci * noo + rowA : 0));
-      t[st] = ok ? v : make_double2(0.0, 0.0);
-    }
+      for (int st = 0; st < G; ++st) {
+        const int ci = g0 + 16 * st + kq;
+        const bool ok = ci < nleft && rowA < noo;
+        const double2 v = ld(TKp + (ok ? ci * noo + rowA : 0));
+        t[st] = ok ? v : make_double2(0.0, 0.0);
+      }
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
-      const int ci = g0 + 16 * st + kq;
-      const int4 nc = cls[min(ci, nvv - 1)];
-      const Real as = (Real)(t[st].x + t[st].y), ap = (Real)(t[st].x - t[st].y);
+      for (int st = 0; st < G; ++st) {
+        const int ci = g0 + 16 * st + kq;
+        const int4 nc = cls[min(ci, end - 1)];
+        const Real as = (Real)(t[st].x + t[st].y), ap = (Real)(t[st].x - t[st].y);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        Real bs = 0, bp = 0;
-        if (ci < nvv && colv[c]) {
-          const Real vx = (Real)vtx[d2(nc, nx[c])];
-          const Real vy = (Real)vty[d2(nc, ny[c])];
-          bs = vx - vy;
-          bp = vx + vy;
+        for (int c = 0; c < NC; ++c) {
+          Real bs = 0, bp = 0;
+          if (ci < nleft && colv[c]) {
+            const Real vx = (Real)vtx[d2(nc, nx[c])];
+            const Real vy = (Real)vty[d2(nc, ny[c])];
+            bs = vx - vy;
+            bp = vx + vy;
+          }
+          accs[c] = MfmaTraits<Real>::mma(as, bs, accs[c]);
+          accp[c] = MfmaTraits<Real>::mma(ap, bp, accp[c]);
         }
-        accs[c] = MfmaTraits<Real>::mma(as, bs, accs[c]);
-        accp[c] = MfmaTraits<Real>::mma(ap, bp, accp[c]);
       }
     }
   }
@@ -973,31 +997,36 @@ __device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cba
     accs[c] = {0, 0, 0, 0};
     accp[c] = {0, 0, 0, 0};
   }
-  for (int g0 = 4 * wave; g0 < nvv; g0 += 16 * G) {
-    double2 t[G];
+  for (int base = 0; base < nvv; base += kClsChunk) {  // (list parts as in ladder_sums)
+    // (indices from base: entries left, entries of this part, their amplitudes)
+    const int nleft = nvv - base, end = restage_cls(S, K, base, nvv, cls);
+    const auto TKp = TK + (size_t)base * noo;
+    for (int g0 = 4 * wave; g0 < end; g0 += 16 * G) {
+      double2 t[G];
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
-      const int ci = g0 + 16 * st + kq;
-      const bool ok = ci < nvv && rowA < noo;
-      const double2 v = ld(TK + (ok ? ci * noo + rowA : 0));
-      t[st] = ok ? v : make_double2(0.0, 0.0);
-    }
+      for (int st = 0; st < G; ++st) {
+        const int ci = g0 + 16 * st + kq;
+        const bool ok = ci < nleft && rowA < noo;
+        const double2 v = ld(TKp + (ok ? ci * noo + rowA : 0));
+        t[st] = ok ? v : make_double2(0.0, 0.0);
+      }
 #pragma unroll
-    for (int st = 0; st < G; ++st) {
-      const int ci = g0 + 16 * st + kq;
-      const int4 nc = cls[min(ci, nvv - 1)];
-      const Real as = (Real)(t[st].x + t[st].y), ap = (Real)(t[st].x - t[st].y);
+      for (int st = 0; st < G; ++st) {
+        const int ci = g0 + 16 * st + kq;
+        const int4 nc = cls[min(ci, end - 1)];
+        const Real as = (Real)(t[st].x + t[st].y), ap = (Real)(t[st].x - t[st].y);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        Real bs = 0, bp = 0;
-        if (ci < nvv && colv[c]) {
-          const Real vx = (Real)vtx[d2(nc, nx[c])];
-          const Real vy = (Real)vty[d2(nc, ny[c])];
-          bs = vx - vy;
-          bp = vx + vy;
+        for (int c = 0; c < NC; ++c) {
+          Real bs = 0, bp = 0;
+          if (ci < nleft && colv[c]) {
+            const Real vx = (Real)vtx[d2(nc, nx[c])];
+            const Real vy = (Real)vty[d2(nc, ny[c])];
+            bs = vx - vy;
+            bp = vx + vy;
+          }
+          accs[c] = MfmaTraits<Real>::mma(as, bs, accs[c]);
+          accp[c] = MfmaTraits<Real>::mma(ap, bp, accp[c]);
         }
-        accs[c] = MfmaTraits<Real>::mma(as, bs, accs[c]);
-        accp[c] = MfmaTraits<Real>::mma(ap, bp, accp[c]);
       }
     }
   }
@@ -1132,7 +1161,8 @@ __device__ __forceinline__ int2 stage_ladder(const SolveG& S, int K, const Ladde
     const auto src = S.vtab + (size_t)S.lad_slot[m] * nt;
     for (int k = threadIdx.x; k < nt; k += blockDim.x) l.vt[m * nt + k] = src[k];
   }
-  for (int k = threadIdx.x; k < nvv; k += blockDim.x) l.cls[k] = ld(S.kb_nva + voff + k);
+  if (nvv <= kClsChunk)  // (a longer list is staged part by part by the tile loops)
+    for (int k = threadIdx.x; k < nvv; k += blockDim.x) l.cls[k] = ld(S.kb_nva + voff + k);
   __syncthreads();
   return make_int2(phase, prec64);
 }

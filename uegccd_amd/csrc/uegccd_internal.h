@@ -210,6 +210,9 @@ inline int ladder_tab_doubles(const SolveDev& S) { return (S.lad_ntab * (S.n2max
 inline int contract_lds_bytes(int tab_doubles, int cls_max, int noo_max) {
   return tab_doubles * 8 + cls_max * 16 + 16 * noo_max * 16;
 }
+// Entries of a K block's virtual list held in LDS at once (a multiple of 16
+// times every k-step group of the tile loops); cls_max never exceeds it.
+constexpr int kClsChunk = 4096;
 
 constexpr int kSliceClasses = 3;
 
