@@ -252,6 +252,14 @@ struct uegccd_ctx {
   int noo_max = 0, nvv_max = 0;  // largest pair-momentum block: pairs, virtual columns
   ElemCode ecode{};          // bit widths of the element map entries
   int vo2max = 0;            // largest |n_a - n_k|^2 of a virtual a and an occupied k
+  // the structure factor's cells (transfer n_a - n_l, its P(q), and whether
+  // its last writer in the reference's loop order is a masked core element),
+  // in the reference's print order; built by the first structure-factor call
+  struct SfCell {
+    int x, y, z, q, core;
+  };
+  std::vector<SfCell> sf_cells;
+  bool sf_ready = false;
 };
 
 namespace {
@@ -739,16 +747,39 @@ struct ArenaImage {
 // elements with a core index must be zero on input (DrvMBPT makes them so):
 // the library keeps them at zero, which is the reference's iteration only
 // from such amplitudes.
+// The caller's dense amplitude arrays into tcur: both copied as they are into
+// the gout work buffer (2 n3 doubles, free outside residual calls), then each
+// valid element's pair placed into the cleared tcur on the device (the
+// entries of invalid elements are ignored, as the reference's loops skip
+// them).
 int copy_in_amplitudes(uegccd_ctx* c, const double* tab, const double* tba) {
   for (int x : c->core_elems)
     if ((tab && tab[x] != 0.0) || (tba && tba[x] != 0.0)) return UEGCCD_ERR_INVALID;
-  std::vector<double> h(2 * (size_t)c->n3, 0.0);
-  for_each_valid(c, [&](int x) {
-    h[2 * (size_t)x] = tab ? tab[x] : 0.0;
-    h[2 * (size_t)x + 1] = tba ? tba[x] : 0.0;
-  });
-  HIPCHK(hipMemcpyAsync(c->sd.tcur, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
-                        c->stream));
+  const size_t n3 = c->n3;
+  double* const sab = c->sd.gout;
+  double* const sba = c->sd.gout + n3;
+  HIPCHK(hipMemsetAsync(c->sd.tcur, 0, sizeof(double2) * n3, c->stream));
+  if (tab) HIPCHK(hipMemcpyAsync(sab, tab, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+  if (tba) HIPCHK(hipMemcpyAsync(sba, tba, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+  launch_place_amplitudes(c->batch, c->nvalid, tab ? sab : nullptr, tba ? sba : nullptr, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return UEGCCD_OK;
+}
+
+// tcur into the caller's dense arrays (the valid elements' amplitudes, zeros
+// elsewhere: tcur holds zeros at invalid elements), unpacked on the device
+// into the gout and rout work buffers and copied from there.
+int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
+  const size_t n3 = c->n3;
+  double* const sab = c->sd.gout;
+  double* const sba = c->sd.gout + n3;
+  double* const saa = c->sd.rout;
+  launch_split_amplitudes(c->batch, (int64_t)n3, sab, sba, saa, c->stream);
+  HIPCHK(hipGetLastError());
+  if (tab) HIPCHK(hipMemcpyAsync(tab, sab, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+  if (tba) HIPCHK(hipMemcpyAsync(tba, sba, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+  if (taa) HIPCHK(hipMemcpyAsync(taa, saa, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return UEGCCD_OK;
 }
@@ -1203,36 +1234,40 @@ int structure_factor_cells(uegccd_ctx* c, const double* tab, const double* tba, 
                            int nval, int32_t* ncell, int32_t* qvec, double* vals, double* energy) {
   if (!c || !ncell) return UEGCCD_ERR_INVALID;
   const int o = c->o, nq = (int)c->qb_neg.size();
-  // the cells printed: transfers n_a - n_i or n_a - n_j of a valid element
-  // (those with a core index too: the reference bins them, with zero
-  // amplitudes and, under CoreN masking, a zero integral).  SfV of a cell is
-  // the integral of its last writer in the reference's loop order (i, then
-  // j, then a; direct before exchange): v(q), or 0 when that writer is a
-  // masked core element.
-  std::vector<char> touched(nq, 0);
-  std::vector<int64_t> last(nq, -1);
-  std::vector<char> last_core(nq, 0);
-  auto bin = [&](int x, bool core) {
-    const int i = x % o, j = (x / o) % o, av = x / (o * o);
-    const int64_t key = (((int64_t)i * o + j) * c->v + av) * 2;
-    const int qd = c->qb_of[(size_t)av * o + i], qx = c->qb_of[(size_t)av * o + j];
-    touched[qd] = touched[qx] = 1;
-    if (key > last[qd]) { last[qd] = key; last_core[qd] = core; }
-    if (key + 1 > last[qx]) { last[qx] = key + 1; last_core[qx] = core; }
-  };
-  for_each_valid(c, [&](int x) { bin(x, false); });
-  for (int x : c->core_elems) bin(x, true);
-  struct Cell { int x, y, z, q; };
-  std::vector<Cell> cells;
-  for (int q = 0; q < nq; ++q) {
-    if (!touched[q]) continue;
-    const int r = c->qb_rows[c->qb_roff[q]], l = r % o, dv = r / o;
-    const int4 na = c->nv4[o + dv], nl = c->nv4[l];
-    cells.push_back({na.x - nl.x, na.y - nl.y, na.z - nl.z, q});
+  if (!c->sf_ready) {
+    // the cells printed: transfers n_a - n_i or n_a - n_j of a valid element
+    // (those with a core index too: the reference bins them, with zero
+    // amplitudes and, under CoreN masking, a zero integral).  SfV of a cell
+    // is the integral of its last writer in the reference's loop order (i,
+    // then j, then a; direct before exchange): v(q), or 0 when that writer
+    // is a masked core element.
+    std::vector<char> touched(nq, 0);
+    std::vector<int64_t> last(nq, -1);
+    std::vector<char> last_core(nq, 0);
+    auto bin = [&](int x, bool core) {
+      const int i = x % o, j = (x / o) % o, av = x / (o * o);
+      const int64_t key = (((int64_t)i * o + j) * c->v + av) * 2;
+      const int qd = c->qb_of[(size_t)av * o + i], qx = c->qb_of[(size_t)av * o + j];
+      touched[qd] = touched[qx] = 1;
+      if (key > last[qd]) { last[qd] = key; last_core[qd] = core; }
+      if (key + 1 > last[qx]) { last[qx] = key + 1; last_core[qx] = core; }
+    };
+    for_each_valid(c, [&](int x) { bin(x, false); });
+    for (int x : c->core_elems) bin(x, true);
+    auto& cells = c->sf_cells;
+    cells.clear();
+    for (int q = 0; q < nq; ++q) {
+      if (!touched[q]) continue;
+      const int r = c->qb_rows[c->qb_roff[q]], l = r % o, dv = r / o;
+      const int4 na = c->nv4[o + dv], nl = c->nv4[l];
+      cells.push_back({na.x - nl.x, na.y - nl.y, na.z - nl.z, q, last_core[q]});
+    }
+    std::sort(cells.begin(), cells.end(), [](const uegccd_ctx::SfCell& a, const uegccd_ctx::SfCell& b) {
+      return a.x != b.x ? a.x < b.x : a.y != b.y ? a.y < b.y : a.z < b.z;
+    });
+    c->sf_ready = true;
   }
-  std::sort(cells.begin(), cells.end(), [](const Cell& a, const Cell& b) {
-    return a.x != b.x ? a.x < b.x : a.y != b.y ? a.y < b.y : a.z < b.z;
-  });
+  const auto& cells = c->sf_cells;
   const int n = (int)cells.size();
   if (!qvec || !vals) {  // size query
     *ncell = n;
@@ -1260,7 +1295,7 @@ int structure_factor_cells(uegccd_ctx* c, const double* tab, const double* tba, 
   const double sc = std::pow(2.0 * M_PI / c->boxL, 2.0);  // dK2scaled, source/HEG.f90:1025
   const double* vso = c->vso.data() + (size_t)vslot * c->v * o;
   for (int k = 0; k < n; ++k) {
-    const Cell& e = cells[k];
+    const auto& e = cells[k];
     const int r = c->qb_rows[c->qb_roff[e.q]], l = r % o, dv = r / o;
     qvec[3 * k] = e.x;
     qvec[3 * k + 1] = e.y;
@@ -1268,7 +1303,7 @@ int structure_factor_cells(uegccd_ctx* c, const double* tab, const double* tba, 
     double* out = vals + (size_t)nval * k;
     out[0] = std::sqrt((double)(e.x * e.x + e.y * e.y + e.z * e.z) * sc);
     out[1] = sums[3 * (size_t)e.q];
-    out[2] = last_core[e.q] ? 0.0 : vso[(size_t)dv * o + l];  // v(n_a - n_l) = v(q)
+    out[2] = e.core ? 0.0 : vso[(size_t)dv * o + l];  // v(n_a - n_l) = v(q)
     if (nval > 3) {
       out[3] = sums[3 * (size_t)e.q + 1];
       out[4] = sums[3 * (size_t)e.q + 2];
@@ -1302,17 +1337,10 @@ int uegccd_mp2(uegccd_ctx* c, double* taa, double* tab, double* tba, double* e2)
   launch_mp2(c->batch, c->stream);
   launch_finalize(c->batch, FIN_SUM, c->stream);
   HIPCHK(hipGetLastError());
-  const size_t n3 = c->n3;
-  std::vector<double> h(2 * n3);
-  HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, c->stream));
   State st;
   int rc = read_state(c, &st);
+  if (!rc) rc = copy_out_amplitudes(c, taa, tab, tba);
   if (rc) return rc;
-  for (size_t x = 0; x < n3; ++x) {
-    if (tab) tab[x] = h[2 * x];
-    if (tba) tba[x] = h[2 * x + 1];
-    if (taa) taa[x] = h[2 * x] + h[2 * x + 1];
-  }
   if (e2) *e2 = st.ecorr;
   return UEGCCD_OK;
 }
@@ -1742,23 +1770,6 @@ int fill_result(uegccd_ctx* c, const State& st, float ms, uegccd_result* res) {
   return status;
 }
 
-int copy_out_amplitudes(uegccd_ctx* c, double* taa, double* tab, double* tba) {
-  const size_t n3 = c->n3;
-  std::vector<double> h(2 * n3);
-  HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  // the valid elements' amplitudes; zeros elsewhere
-  for (double* out : {taa, tab, tba})
-    if (out) std::fill(out, out + n3, 0.0);
-  for_each_valid(c, [&](int x) {
-    const double a = h[2 * (size_t)x], b = h[2 * (size_t)x + 1];
-    if (tab) tab[x] = a;
-    if (tba) tba[x] = b;
-    if (taa) taa[x] = a + b;
-  });
-  return UEGCCD_OK;
-}
 
 }  // namespace
 
@@ -1822,12 +1833,8 @@ extern "C" int uegccd_drpa(uegccd_ctx* c, double* t2, uegccd_result* res, double
     *esosex = se.ecorr;
   }
   if (t2) {
-    const size_t n3 = c->n3;
-    std::vector<double> h(2 * n3);
-    HIPCHK(hipMemcpyAsync(h.data(), c->sd.tcur, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::fill(t2, t2 + n3, 0.0);
-    for_each_valid(c, [&](int x) { t2[x] = h[2 * (size_t)x]; });
+    rc = copy_out_amplitudes(c, nullptr, t2, nullptr);
+    if (rc) return rc;
   }
   return status;
 }

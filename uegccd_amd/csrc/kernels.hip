@@ -2424,6 +2424,43 @@ void launch_sf(const BatchDev& b, int v, int nq, hipStream_t s) {
   hipLaunchKernelGGL(k_sf_pairs, dim3(v), dim3(64), 0, s, b.solves);
   hipLaunchKernelGGL(k_sf_cells, dim3(nq), dim3(64), 0, s, b.solves);
 }
+// Amplitudes between a solve's dense (i,j,a) arrays, staged in device
+// memory, and tcur.  k_place writes each valid element's (T^ab, T^ba) into
+// tcur (whose other entries the caller has cleared; a NULL array reads as
+// zeros); k_split unpacks every entry of tcur into T^ab, T^ba and
+// T^aa = T^ab + T^ba (invalid entries hold zeros).
+__global__ __launch_bounds__(256) void k_place_amplitudes(const SolveDev* __restrict__ solves,
+                                                          const double* __restrict__ ab,
+                                                          const double* __restrict__ ba) {
+  const SolveG& S = as_global(solves)[0];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.nvalid) return;
+  const int x = elem_x(S.ecode, ld(S.cmap + c));
+  st2(S.tcur + x, make_double2(ab ? ab[x] : 0.0, ba ? ba[x] : 0.0));
+}
+
+__global__ __launch_bounds__(256) void k_split_amplitudes(const SolveDev* __restrict__ solves, double* __restrict__ ab,
+                                                          double* __restrict__ ba, double* __restrict__ aa) {
+  const SolveG& S = as_global(solves)[0];
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= S.n3) return;
+  const double2 t = S.tcur[x];
+  ab[x] = t.x;
+  ba[x] = t.y;
+  aa[x] = t.x + t.y;
+}
+
+void launch_place_amplitudes(const BatchDev& b, int64_t nvalid, const double* ab, const double* ba,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_place_amplitudes, dim3((unsigned)std::max<int64_t>(1, (nvalid + 255) / 256)), dim3(256), 0,
+                     s, b.solves, ab, ba);
+}
+
+void launch_split_amplitudes(const BatchDev& b, int64_t n3, double* ab, double* ba, double* aa, hipStream_t s) {
+  hipLaunchKernelGGL(k_split_amplitudes, dim3((unsigned)std::max<int64_t>(1, (n3 + 255) / 256)), dim3(256), 0, s,
+                     b.solves, ab, ba, aa);
+}
+
 void launch_drpa_start(const BatchDev& b, int64_t n3, hipStream_t s) {
   hipLaunchKernelGGL(k_drpa_transpose, dim3((unsigned)std::max<int64_t>(1, (n3 + 255) / 256)), dim3(256), 0, s,
                      b.solves);

@@ -301,6 +301,9 @@ void launch_mp2(const BatchDev& b, hipStream_t s, int drpa = 0);
 // One dRPA pass (source/dRPA.f90:80-96) of a batch-of-one: the row and
 // transfer sums, the per-element update, the reductions.
 void launch_drpa_start(const BatchDev& b, int64_t n3, hipStream_t s);
+// A batch of one's amplitudes in and out of tcur through staged dense arrays.
+void launch_place_amplitudes(const BatchDev& b, int64_t nvalid, const double* ab, const double* ba, hipStream_t s);
+void launch_split_amplitudes(const BatchDev& b, int64_t n3, double* ab, double* ba, double* aa, hipStream_t s);
 void launch_sf(const BatchDev& b, int v, int nq, hipStream_t s);
 void launch_drpa_pass(const BatchDev& b, hipStream_t s);
 // Events timing one kernel of a pass (which: as launch_one, or 4 for the
