@@ -186,6 +186,15 @@ struct Lds {
 // of a virtual and an occupied momentum, which the host keeps below that).
 __device__ __forceinline__ int slice_n2(const SolveG& S) { return min(S.n2max, kMaxN2 - 2); }
 
+// Whether a pass has nothing to compute for a solve: it has finished, or it
+// has converged in DIIS mode, where the last residual pass already holds the
+// converged amplitudes' residual and the final pass only lays out the
+// closing plain step (k_slice writes those amplitudes to tcur; the
+// reference-mode final pass is the reference's residual check and runs).
+__device__ __forceinline__ bool pass_idle(const SolveG& S, int phase) {
+  return phase == PH_DONE || (phase == PH_FINAL && S.mode == UEGCCD_MODE_DIIS);
+}
+
 // Stages the occupied n-vectors and the Coulomb table of one slot.
 __device__ __forceinline__ void stage(const SolveG& S, Lds& L, int slot) {
   const auto vt = S.vtab + (size_t)slot * (S.n2max + 1);
@@ -654,6 +663,7 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
       slab_place(S, Tab, Tba, xs, ld(S.cmap + c0 + c), combine_history<kPre>(cf, hm, D.coef, ndiis, HCc, nval));
     }
   }
+  if (pass_idle(S, phase)) return;  // the closing step's amplitudes are in tcur (DIIS mode)
   if ((int)threadIdx.x < o) vs[threadIdx.x] = vsm;
   __syncthreads();
   // Row/column sums (for J) by groups of 8 lanes per row i, each lane a
@@ -1177,7 +1187,7 @@ __device__ __forceinline__ void ladder_block(const SolveDev* __restrict__ solves
   const SolveG& S = as_global(solves)[s];
   const int4 it = ld((NC == 2 ? S.lad_items2 : S.lad_items) + (item - prefix[s]));
   const int2 st = stage_ladder(S, it.y, l);
-  if (st.x == PH_DONE) return;
+  if (pass_idle(S, st.x)) return;
   if (st.y)
     ladder_tile<double, G, NC>(S, it.y, it.z, it.w, l.vt, l.cls, red);
   else
@@ -1191,7 +1201,7 @@ __device__ __forceinline__ void hh_block(const SolveDev* __restrict__ solves, co
   const SolveG& S = as_global(solves)[s];
   const int4 it = ld(S.hh_items + (blk - prefix[s]));
   const int2 st = stage_ladder(S, it.y, l);
-  if (st.x == PH_DONE) return;
+  if (pass_idle(S, st.x)) return;
   if (st.y)
     hh_unit<double, G, HS>(S, it.y, it.z, l.vt, l.cls, red, l.wl);
   else
@@ -1334,7 +1344,8 @@ __device__ __forceinline__ void contract_item(const SolveDev* __restrict__ solve
   const int s = find_solve(prefix, n, item);
   const SolveG& S = as_global(solves)[s];
   const int4 it = ld(S.con_items + (item - prefix[s]));
-  // (no phase test: the contraction outputs of a finished solve are never read)
+  // (the phase is requested beside the item, so the test adds no round trip)
+  if (pass_idle(S, S.st2->phase)) return;
   if (it.y == CK_RING)
     quad_tile<false, QS>(S, it.z, it.w);
   else if (it.y == CK_XRING)
@@ -1452,7 +1463,7 @@ __device__ __forceinline__ void main_block(const SolveDev* __restrict__ solves, 
       joo_l[k] = a;
     }
   __syncthreads();  // publishes joo_l
-  if (phase == PH_DONE) return;
+  if (pass_idle(S, phase)) return;
   const int nh = (flags & MF_ITERATE) ? S.ndiis : 0;
   const auto T = S.tcur;
   const bool valid = b >= 0;
@@ -1621,7 +1632,9 @@ __device__ __forceinline__ bool advance_state(const SolveG& S, State* st, const 
   const double e = vals[0], dt = vals[1], rmax = vals[2];
   if (vals[3] > 0.0) st->nonfinite = 1;
   if (phase0 == PH_FINAL) {
-    st->maxres = rmax;
+    // the final residual check (reference mode), or in DIIS mode, whose final
+    // pass computes nothing, the residual of the converged pass
+    st->maxres = S.mode == UEGCCD_MODE_DIIS ? st->maxres_in : rmax;
     st->phase = PH_DONE;
     return false;
   }
