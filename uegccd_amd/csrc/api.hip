@@ -235,7 +235,7 @@ struct uegccd_ctx {
   int last_niter = 0;
   // in-solve timing of one kernel (uegccd_time_kernel): -1 off
   int timed_kernel = -1;
-  std::vector<hipEvent_t> tev;   // before/after pairs, one per pass of a chunk
+  std::vector<hipEvent_t> tev;   // before/after pair of the timed pass of a chunk
   hipEvent_t span[2] = {nullptr, nullptr};  // brackets of a solve's iteration (created once)
   State* h_states = nullptr;      // pinned host copies of the polled states
   int h_states_cap = 0;
@@ -1660,17 +1660,21 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
   int passes = 0;
   bool all_done = false;
   const bool timing = c0->timed_kernel >= 0 && !no_alloc;
-  if (timing && (int)c0->tev.size() < 2 * cap) {
-    for (size_t k = c0->tev.size(); k < (size_t)(2 * cap); ++k) {
+  if (timing && c0->tev.size() < 2) {
+    for (size_t k = c0->tev.size(); k < 2; ++k) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
       c0->tev.push_back(e);
     }
   }
-  std::vector<int> prev_niter(n, 1), prev_phase(n, PH_ITER);  // as left by the init
-  // a timed first kernel of the pass takes its `before` event from the
-  // previous pass of the chunk, so the first pass of each chunk is not timed
+  std::vector<int> prev_niter(n, 1);  // as left by the init
+  // The timed kernel carries its events in one pass per chunk, pass k0: an
+  // event on a dispatch holds the next one back by a microsecond or two, so
+  // the timed solves keep them few.  A timed first kernel of the pass takes
+  // its `before` event from the pass before it, so it is timed in the second
+  // pass of a chunk.
   const bool timed_first = timing && c0->timed_kernel == first_pass_kernel(b, MF_ITERATE);
+  const int k0 = timed_first ? 1 : 0;
   // a single context's passes replay as captured graphs (not while a kernel
   // is timed with events, nor without device allocation: instantiating a
   // graph allocates)
@@ -1695,12 +1699,12 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     // has a finalize launch after every pass)
     const bool fold = fold_state_update(n, c0);
     for (int k = 0; k < chunk && !graphs && !persist; ++k) {
-      if (timing) {
-        const KernelTimer t{c0->timed_kernel, c0->tev[2 * k], c0->tev[2 * k + 1]};
-        const hipEvent_t tail = (timed_first && k + 1 < chunk) ? c0->tev[2 * (k + 1)] : nullptr;
-        launch_iteration(b, MF_ITERATE, s, &t, tail, fold && k > 0);
+      if (timing && k == k0) {
+        const KernelTimer t{c0->timed_kernel, c0->tev[0], c0->tev[1]};
+        launch_iteration(b, MF_ITERATE, s, &t, nullptr, fold && k > 0);
       } else {
-        launch_iteration(b, MF_ITERATE, s, nullptr, nullptr, fold && k > 0);
+        const hipEvent_t tail = (timing && k + 1 == k0 && k0 < chunk) ? c0->tev[0] : nullptr;
+        launch_iteration(b, MF_ITERATE, s, nullptr, tail, fold && k > 0);
       }
       if (!fold) launch_finalize(b, FIN_ITER, s);
     }
@@ -1717,19 +1721,16 @@ int iterate(uegccd_ctx** cs, int n, const BatchDev& b, State* gather, std::vecto
     HIPCHK(hipStreamSynchronize(s));
     std::copy(hst, hst + n, st.begin());
     if (timing) {
-      // passes of this chunk in which some solve was still iterating: its
-      // NIter advance plus its final residual pass (later passes return at once)
-      int real = 0;
+      // the timed pass counts when some solve was still iterating in it (the
+      // chunk's iterating passes come first: its NIter advance)
+      int iterating = 0;
       for (int k = 0; k < n; ++k) {
-        int r = st[k].niter - prev_niter[k];
-        if (st[k].phase == PH_DONE && prev_phase[k] != PH_DONE && !st[k].nonfinite) r += 1;
-        real = std::max(real, r);
+        iterating = std::max(iterating, st[k].niter - prev_niter[k]);
         prev_niter[k] = st[k].niter;
-        prev_phase[k] = st[k].phase;
       }
-      for (int k = timed_first ? 1 : 0; k < std::min(real, chunk); ++k) {
+      if (k0 < chunk && iterating > k0) {
         float us = 0.f;
-        HIPCHK(hipEventElapsedTime(&us, c0->tev[2 * k], c0->tev[2 * k + 1]));
+        HIPCHK(hipEventElapsedTime(&us, c0->tev[0], c0->tev[1]));
         c0->timed_us += 1000.0 * us;
         c0->timed_launches += 1;
       }
