@@ -589,6 +589,18 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
   for (int k = 0; k < kPre; ++k)
     h[k] = (t0 < nvs && k < ndiis) ? ld(HC + (size_t)k * nval) : make_double2(0.0, 0.0);
   const double vsm = t0 < o ? vso_m[sv * o + t0] : 0.0;
+  // the LDS tables' entries (the occupied n-vectors and the Y table up to
+  // slice_n2, zeros above it), requested with the history too and stored
+  // once the state is known
+  // (two entries a thread in blocks of 512 and 768 threads, whose registers
+  // have room: the whole table; 256-thread blocks load theirs after the update)
+  constexpr int kVt = BT >= 512 ? 2 : 0;
+  const auto vtq = S.vtab + (size_t)S.rslot[RO_RING_QX] * (S.n2max + 1);
+  const int n2s = slice_n2(S);
+  double vtr[kVt > 0 ? kVt : 1];
+#pragma unroll
+  for (int m = 0; m < kVt; ++m) vtr[m] = t0 + m * BT <= n2s ? vtq[t0 + m * BT] : 0.0;
+  const int4 nvo = kVt > 0 && t0 < o ? ld(S.nv4 + t0) : make_int4(0, 0, 0, 0);
   // the state of this pass (requested after the history, which does not
   // depend on it)
   int phase, src, nh, slot_cur;
@@ -627,7 +639,12 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
     }
     for (int k = t0; k < kMaxN2; k += BT) vt2[k] = k <= slice_n2(S) ? vt_x[k] : 0.0;
   }
-  stage(S, L, S.rslot[RO_RING_QX]);
+#pragma unroll
+  for (int m = 0; m < kVt; ++m)
+    if (t0 + m * BT < kMaxN2) L.vt[t0 + m * BT] = vtr[m];
+  for (int k = t0 + kVt * BT; k < kMaxN2; k += BT) L.vt[k] = k <= n2s ? vtq[k] : 0.0;
+  if (t0 < o) L.nv[t0] = kVt > 0 ? nvo : ld(S.nv4 + t0);
+  __syncthreads();
   if (phase == PH_DONE) return;  // the whole block: the solve has finished
   if (src == SRC_DIIS) {  // a failed solve leaves the plain update
     if (t0 < 64) {
@@ -638,12 +655,11 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
     }
     __syncthreads();
   }
-  // Y below is looked up at n_i + n_k - n_s - n_l = n_b - n_l, b the partner
-  // of (i,k,s): a virtual-occupied difference, within the staged table.
-  // Without a partner the index can leave the table; those terms multiply a
-  // zero amplitude, so the table reads as zero there (the last entry is
-  // always past the staged ones).
-  for (int k = slice_n2(S) + 1 + t0; k < kMaxN2; k += BT) L.vt[k] = 0.0;
+  // (Y below is looked up at n_i + n_k - n_s - n_l = n_b - n_l, b the
+  // partner of (i,k,s): a virtual-occupied difference, within the staged
+  // table.  Without a partner the index can leave the table; those terms
+  // multiply a zero amplitude, so the table reads as zero there: the last
+  // entry is always past the staged ones.)
   if (src == SRC_TCUR) {
     for (int idx = t0; idx < o2; idx += BT) {
       const double2 t = ld(S.tcur + xs + idx);
