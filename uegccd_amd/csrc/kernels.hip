@@ -542,6 +542,26 @@ template <int MAXQ>
 __device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int prec64, double* vals, double* red,
                                               State& sst, bool write_trace);
 __device__ __forceinline__ void put_state(const State& sst, GlobalPtr<State> to);
+// The loads of a state update, requested ahead of its reductions: a thread's
+// state words (words t and t + kFinThreads), and its first round of block
+// partials (blocks k and k + kFinThreads of thread k; the 32-wide form one
+// block a round, for registers).
+struct StateLoads {
+  double sw0, sw1;
+};
+template <int NQ>
+struct PartialLoads {
+  double p[NQ], p2[NQ];
+};
+__device__ __forceinline__ void state_load(const SolveG& S, StateLoads& L);
+__device__ __forceinline__ int finalize_dots(const SolveG& S, int kind);
+template <int NQ>
+__device__ __forceinline__ void partials_load(const SolveG& S, int nd, int k, PartialLoads<NQ>& L);
+template <int NQ>
+__device__ __forceinline__ void partials_finish(const SolveG& S, int nd, PartialLoads<NQ>& L, double* vals,
+                                                double* red);
+__device__ __forceinline__ int finalize_apply(const SolveG& S, int kind, int prec64, const double* vals,
+                                              State& sst, bool write_trace, const StateLoads& SL);
 
 template <int BT, bool SPLIT>
 __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
@@ -581,6 +601,14 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
   const int xs = o2 * sv;  // the slab's first element
   const int c0 = S.slab_coff[sv], nvs = S.slab_coff[sv + 1] - c0;
   const int t0 = threadIdx.x;
+  // the previous pass's state words and block partials go out first, so that
+  // its update waits on them alone, not on the history behind them
+  StateLoads fsl{0.0, 0.0};
+  PartialLoads<16> fpl;
+  if (fin_prev) {
+    state_load(S, fsl);
+    partials_load<16>(S, finalize_dots(S, FIN_ITER), t0, fpl);
+  }
   const auto HC = S.histT + c0 + t0;
   const uint2 cm = t0 < nvs ? ld(S.cmap + c0 + t0) : make_uint2(0u, 0u);
   const double vr = t0 < o2 ? voo_r[t0] : 0.0;
@@ -606,7 +634,8 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
   int phase, src, nh, slot_cur;
   double bw;
   if (fin_prev) {
-    finalize_state<16>(S, FIN_ITER, 1, fvals, fred, *fsst, sv == 0);
+    partials_finish<16>(S, finalize_dots(S, FIN_ITER), fpl, fvals, fred);
+    finalize_apply(S, FIN_ITER, 1, fvals, *fsst, sv == 0, fsl);
     if (sv == 0) put_state(*fsst, S.st2);
     phase = fsst->phase;
     src = fsst->next_src;
@@ -1599,35 +1628,46 @@ __global__ __launch_bounds__(kElemThreads, NQ == 16 ? 4 : 1) void k_main(const S
 // are written by threads 0..3+nd: valid after a barrier.  red holds
 // (blockDim/64)*32 doubles.
 template <int NQ>
-__device__ __forceinline__ void reduce_partials_n(const SolveG& S, int nd, double* vals, double* red) {
+__device__ __forceinline__ void partials_load(const SolveG& S, int nd, int k, PartialLoads<NQ>& L) {
+  constexpr int kStep = NQ <= 16 ? 2 : 1;
+  const int nq = 4 + nd, nb = S.nblk_elem, k2 = k + kFinThreads;
+  const bool one = k < nb && threadIdx.x < kFinThreads, two = one && kStep == 2 && k2 < nb;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int src = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q;
+    L.p[q] = src < nq && one ? *partial(S, src, k) : 0.0;
+    L.p2[q] = src < nq && two ? *partial(S, src, k2) : 0.0;
+  }
+}
+// The reduction of the partials from the first round of loads (L, requested
+// by partials_load(S, nd, threadIdx.x, L)) on: blocks k, k + kFinThreads, ...
+// of thread k in that order, then the fixed tree of block_reduce_scatter.
+template <int NQ>
+__device__ __forceinline__ void partials_finish(const SolveG& S, int nd, PartialLoads<NQ>& L, double* vals,
+                                                double* red) {
+  constexpr int kStep = NQ <= 16 ? 2 : 1;
   const int nq = 4 + nd, nb = S.nblk_elem;
   double acc[NQ];  // dT, max|R|, E, non-finite, dots
 #pragma unroll
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  // blocks k, k + kFinThreads, ... in that order; the 16-wide form takes two
-  // at a time, so that both loads of a pair go out in one round (the partials
-  // of up to 2 kFinThreads blocks); the 32-wide form one at a time (registers)
-  constexpr int kStep = NQ <= 16 ? 2 : 1;
   for (int k = threadIdx.x; k < nb && threadIdx.x < kFinThreads; k += kStep * kFinThreads) {
-    const int k2 = k + kFinThreads;
-    const bool two = kStep == 2 && k2 < nb;
-    double p[NQ], p2[NQ];
+    if (k != (int)threadIdx.x) partials_load<NQ>(S, nd, k, L);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int src = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q;
-      p[q] = src < nq ? *partial(S, src, k) : 0.0;
-      p2[q] = src < nq && two ? *partial(S, src, k2) : 0.0;
-    }
+    for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], L.p[q]) : acc[q] + L.p[q];
+    if (kStep == 2 && k + kFinThreads < nb) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], p[q]) : acc[q] + p[q];
-    if (two) {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], p2[q]) : acc[q] + p2[q];
+      for (int q = 0; q < NQ; ++q) acc[q] = q < 2 ? fmax(acc[q], L.p2[q]) : acc[q] + L.p2[q];
     }
   }
   const double r = block_reduce_scatter<NQ, 2>(acc, red);
   const int q = threadIdx.x;
   if (q < nq) vals[q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q] = r;
+}
+template <int NQ>
+__device__ __forceinline__ void reduce_partials_n(const SolveG& S, int nd, double* vals, double* red) {
+  PartialLoads<NQ> L;
+  partials_load<NQ>(S, nd, threadIdx.x, L);
+  partials_finish<NQ>(S, nd, L, vals, red);
 }
 // MAXQ: the widest partials the caller reduces (16: DIIS histories up to 12,
 // the only width the slice blocks' folded update instantiates).
@@ -1711,21 +1751,39 @@ __device__ __forceinline__ bool advance_state(const SolveG& S, State* st, const 
 // kFinThreads threads or more (vals and red are its LDS); it returns the
 // phase the state had (a FIN_ITER on a finished solve changes nothing).
 // write_trace: this block writes the trace line (one block per solve does).
+__device__ __forceinline__ void state_load(const SolveG& S, StateLoads& L) {
+  const auto src = reinterpret_cast<GlobalPtr<const double>>(S.st);
+  const int w0 = threadIdx.x, w1 = threadIdx.x + kFinThreads;
+  const bool two = threadIdx.x < kFinThreads;
+  L.sw0 = w0 < kStateWords ? src[w0] : 0.0;
+  L.sw1 = two && w1 < kStateWords ? src[w1] : 0.0;
+}
+// The DIIS dots a FIN_ITER reduction reads.
+__device__ __forceinline__ int finalize_dots(const SolveG& S, int kind) {
+  return (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS) ? S.ndiis : 0;
+}
+__device__ __forceinline__ int finalize_apply(const SolveG& S, int kind, int prec64, const double* vals,
+                                              State& sst, bool write_trace, const StateLoads& SL);
+
 template <int MAXQ>
 __device__ __forceinline__ int finalize_state(const SolveG& S, int kind, int prec64, double* vals, double* red,
                                               State& sst, bool write_trace) {
   // One round of loads: each thread's state words go out with its partials
   // (the DIIS dots are read whatever the phase; a finishing pass ignores them).
-  const double* src = reinterpret_cast<const double*>(S.st);
+  StateLoads SL;
+  state_load(S, SL);
+  reduce_partials<MAXQ>(S, finalize_dots(S, kind), vals, red);
+  return finalize_apply(S, kind, prec64, vals, sst, write_trace, SL);
+}
+
+// The state update from the reduced partials vals and the state words SL.
+__device__ __forceinline__ int finalize_apply(const SolveG& S, int kind, int prec64, const double* vals,
+                                              State& sst, bool write_trace, const StateLoads& SL) {
   const int w0 = threadIdx.x, w1 = threadIdx.x + kFinThreads;
   const bool two = threadIdx.x < kFinThreads;
-  const double sw0 = w0 < kStateWords ? src[w0] : 0.0;
-  const double sw1 = two && w1 < kStateWords ? src[w1] : 0.0;
-  const int nd = (kind == FIN_ITER && S.mode == UEGCCD_MODE_DIIS) ? S.ndiis : 0;
-  reduce_partials<MAXQ>(S, nd, vals, red);
   double* dst = reinterpret_cast<double*>(&sst);
-  if (w0 < kStateWords) dst[w0] = sw0;
-  if (two && w1 < kStateWords) dst[w1] = sw1;
+  if (w0 < kStateWords) dst[w0] = SL.sw0;
+  if (two && w1 < kStateWords) dst[w1] = SL.sw1;
   __syncthreads();
   const int phase0 = sst.phase;
   if (kind == FIN_ITER && phase0 == PH_DONE) return phase0;  // the whole block
