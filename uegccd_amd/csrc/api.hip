@@ -217,6 +217,8 @@ struct uegccd_ctx {
   std::vector<int> core_elems;     // valid elements with a core index (held at zero)
   double screening = 0.0;          // HEGData%ScreeningDistance
   std::vector<int4> hh_items;   // (0, K, row0, 0) hole-hole units
+  std::vector<int4> hh_items2;  // (0, K, row0, part) the same, split in two over the W columns of
+                                // blocks of more than 16 pairs, parts 1 and 2 (a solve launched alone)
   std::vector<int4> lad_items;  // (0, K, row0, col0) particle-particle tiles of 16 columns
   std::vector<int4> lad_items2; // the same of 32 columns (two tiles per block, in batches)
   std::vector<int> qb_roff, qb_rows, qb_neg, qrow, qb_of;  // hole-particle sets P(q)
@@ -437,8 +439,13 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     });
   // ladder work: per 16 pair rows of a block, one hole-hole unit (its W rows
   // and the hole-hole products of those rows) and the particle-particle
-  // tiles of 16 a columns; the largest blocks' units first
+  // tiles of 16 a columns; the largest blocks' units first.  A solve launched
+  // alone takes the split units (two, over the halves of the W columns, in a
+  // block of more than 16 pairs: the largest units held its contraction
+  // launch twice as long as a particle-particle tile); a batch, which fills
+  // the chip, the whole ones.
   c->hh_items.clear();
+  c->hh_items2.clear();
   c->lad_items.clear();
   c->lad_items2.clear();
   for (int K = 0; K < c->nK; ++K) {
@@ -447,17 +454,25 @@ int build_maps(uegccd_ctx* c, const uegccd_system& s) {
     if (nvv == 0) continue;
     for (int r0 = 0; r0 < noo; r0 += 16) {
       c->hh_items.push_back(make_int4(0, K, r0, 0));
+      if (noo > 16) {
+        c->hh_items2.push_back(make_int4(0, K, r0, 1));
+        c->hh_items2.push_back(make_int4(0, K, r0, 2));
+      } else {
+        c->hh_items2.push_back(make_int4(0, K, r0, 0));
+      }
       for (int c0 = 0; c0 < nvv; c0 += 16) c->lad_items.push_back(make_int4(0, K, r0, c0));
       for (int c0 = 0; c0 < nvv; c0 += 32) c->lad_items2.push_back(make_int4(0, K, r0, c0));
     }
   }
-  std::stable_sort(c->hh_items.begin(), c->hh_items.end(), [c](const int4& a, const int4& b) {
+  auto by_work = [c](const int4& a, const int4& b) {
     auto work = [c](int K) {
       const int64_t noo = c->kb_poff[K + 1] - c->kb_poff[K], nvv = c->kb_voff[K + 1] - c->kb_voff[K];
       return noo * nvv;
     };
     return work(a.y) > work(b.y);
-  });
+  };
+  std::stable_sort(c->hh_items.begin(), c->hh_items.end(), by_work);
+  std::stable_sort(c->hh_items2.begin(), c->hh_items2.end(), by_work);
   // Coulomb table, flag 0 of the ERI variants (source/HEG.f90:729-772), up to
   // the largest |n_p - n_q|^2 of the basis.  |n_p - n_q| <= |n_p| + |n_q|, so
   // with the plane waves in decreasing |n| the pairs of p can raise the
@@ -683,7 +698,7 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(double2) * std::max<size_t>(1, c->kb_toff.back()));  // tk
   bytes += aligned(sizeof(int) * (c->nK + 1));                         // kb_toff
   bytes += aligned(sizeof(double2) * n3);                              // lad
-  bytes += aligned(sizeof(double2) * std::max(1, c->kb_toff.back()));  // hh
+  bytes += 2 * aligned(sizeof(double2) * std::max(1, c->kb_toff.back()));  // hh, hh2
   bytes += 2 * aligned(sizeof(double) * jring_doubles(c));             // jring, jxring
   bytes += aligned(sizeof(double2) * n3);                              // tv
   bytes += 2 * aligned(sizeof(double) * c->o * c->v);                  // rsum, csum
@@ -695,7 +710,8 @@ size_t arena_bytes(const uegccd_ctx* c, int ndiis) {
   bytes += aligned(sizeof(int) * (8 + 2 * kSliceClasses)) +
            aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items.size())) +
            aligned(sizeof(int4) * std::max<size_t>(1, c->lad_items2.size())) + aligned(sizeof(int) * 2) +
-           aligned(sizeof(int4) * std::max<size_t>(1, c->hh_items.size()));
+           aligned(sizeof(int4) * std::max<size_t>(1, c->hh_items.size())) +
+           aligned(sizeof(int4) * std::max<size_t>(1, c->hh_items2.size()));
   bytes += aligned(sizeof(int) * c->qb_roff.size()) + aligned(sizeof(int) * std::max<size_t>(1, c->qb_rows.size()));
   bytes += aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size())) +
            aligned(sizeof(int) * std::max<size_t>(1, c->qb_neg.size() * (size_t)c->o));  // qb_d
@@ -893,7 +909,8 @@ size_t const_section_bound(const uegccd_ctx* c) {
   add(sizeof(int) * (c->slab_coff.size() + c->pair_kb.size() + c->pair_pos.size()));
   add(sizeof(int) * (c->qb_roff.size() + c->qb_rows.size() + c->qb_neg.size() + c->qrow.size() +
                      c->qb_of.size()));
-  add(sizeof(int4) * (c->qb_info.size() + c->hh_items.size() + c->lad_items.size() + c->con_items.size()));
+  add(sizeof(int4) * (c->qb_info.size() + c->hh_items.size() + c->hh_items2.size() + c->lad_items.size() +
+                      c->con_items.size()));
   add(sizeof(int4) * c->lad_items2.size() + sizeof(int) * 2);
   add(sizeof(SolveDev) + sizeof(int) * (8 + 2 * kSliceClasses));
   return b + 24 * 256;  // (one alignment pad per array)
@@ -983,6 +1000,8 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   // the solve's ladder tiles and contraction items (any batch reads them here)
   S.nhh = sys->do_ladder ? (int)c->hh_items.size() : 0;
   S.hh_items = im.put(c->hh_items);
+  S.nhh2 = sys->do_ladder ? (int)c->hh_items2.size() : 0;
+  S.hh_items2 = im.put(c->hh_items2);
   S.nlad = sys->do_ladder ? (int)c->lad_items.size() : 0;
   S.lad_items = im.put(c->lad_items);
   S.nlad2 = sys->do_ladder ? (int)c->lad_items2.size() : 0;
@@ -995,7 +1014,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   // from the history, whichever terms are on)
   const int sb = c->v;
   const int cls = slice_threads(c->o) / 256 - 1;
-  int h_pref[8 + 2 * kSliceClasses] = {0, eb, 0, S.nlad, 0, S.ncon, 0, S.nhh};
+  int h_pref[8 + 2 * kSliceClasses] = {0, eb, 0, S.nlad, 0, S.ncon, 0, S.nhh2};
   h_pref[8 + 2 * cls + 1] = sb;
   int* d_pref = im.reserve<int>(8 + 2 * kSliceClasses);
   im.write(d_pref, h_pref, 8 + 2 * kSliceClasses);
@@ -1014,8 +1033,9 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
     b.slice_cls_blocks[k] = k == cls ? sb : 0;
     b.slice_lds[k] = k == cls ? slice_lds_bytes(c->o, S.slice_split) : 0;
   }
-  b.nhh = S.nhh;
-  b.nhh_pad = pad8(S.nhh);
+  b.nhh = S.nhh2;
+  b.nhh_pad = pad8(S.nhh2);
+  b.hh_split = 1;
   b.nlad = S.nlad;
   b.nlad_pad = pad8(S.nlad);
   {
@@ -1062,6 +1082,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.rout = c->arena.take<double>(2 * n3);
   S.lad = c->arena.take<double2>(n3);
   S.hh = c->arena.take<double2>(std::max(1, c->kb_toff[c->nK]));
+  S.hh2 = c->arena.take<double2>(std::max(1, c->kb_toff[c->nK]));
   S.jring = c->arena.take<double2>(jring_doubles(c) / 2);
   S.jxring = c->arena.take<double2>(jring_doubles(c) / 2);
   S.tv = c->arena.take<double2>(n3);
@@ -1506,6 +1527,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.hh_prefix = const_cast<int*>(im.put(hp));
   b.nhh = hp[n];
   b.nhh_pad = pad8(hp[n]);
+  b.hh_split = 0;
   b.nlad_pad = pad8(lp[n]);
   b.tab_doubles = tab_doubles;
   b.cls_max = cls_max;

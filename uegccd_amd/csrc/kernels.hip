@@ -1024,7 +1024,7 @@ __device__ __forceinline__ void ladder_tile(const SolveG& S, int K, int row0, in
 // read the quadratic tables v(c-k), v(c-l), and v(i-k), v(i-l) the linear
 // ones.  Every thread of the block calls it.
 template <typename Real, int G, int NC>
-__device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cbase, const double* vt,
+__device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cbase, int qend, const double* vt,
                                        const int4* cls, double* red, double2* Wl) {
   typedef typename MfmaTraits<Real>::acc_t acc_t;
   const int o = S.o;
@@ -1041,7 +1041,7 @@ __device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cba
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int q = cbase + 16 * c + r16;
-    colv[c] = q < noo;
+    colv[c] = q < qend;
     const int xq = colv[c] ? S.kb_pairs[poff + q] : 0;
     nx[c] = S.nv4[xq % o];
     ny[c] = S.nv4[xq / o];
@@ -1087,7 +1087,7 @@ __device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cba
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    if (cbase + 16 * c >= noo) break;
+    if (cbase + 16 * c >= qend) break;
     if (wave > 0) {
       double* mine = red + ((wave - 1) * 64 + lane) * 8;
       for (int r = 0; r < 4; ++r) {
@@ -1124,43 +1124,49 @@ __device__ __forceinline__ void w_rows(const SolveG& S, int K, int row0, int cba
   }
 }
 
-// The hole-hole unit of rows row0..row0+15 of block K: their W rows over all
-// noo pair columns into Wl (w_rows, two column tiles per pass over the
-// virtual list), then hh = W . (S, P) as 16x16 FP64 MFMA tiles over the
-// block's a columns, dealt over the waves, A from Wl and B from the
-// K-blocked amplitudes (a column's pairs contiguous), into hh at the
-// amplitudes' K-blocked positions.  HS: k-steps of 4 pairs per round of
-// loads (from the tuned tile table).
+// The hole-hole unit of rows row0..row0+15 of block K over part `part` of its
+// W columns (0: all of them; 1, 2: the first and second half of their 16-pair
+// tiles): the W rows over those pair columns into Wl (w_rows, two column
+// tiles per pass over the virtual list), then their part of hh = W . (S, P)
+// as 16x16 FP64 MFMA tiles over the block's a columns, dealt over the waves,
+// A from Wl and B from the K-blocked amplitudes (a column's pairs
+// contiguous), into hh (parts 0 and 1) or hh2 (part 2) at the amplitudes'
+// K-blocked positions; k_main adds hh2 in launches of halved units.  HS:
+// k-steps of 4 pairs per round of loads (from the tuned tile table).
 constexpr int kWGroup = 8;
 template <typename Real, int G, int HS>
-__device__ __forceinline__ void hh_unit(const SolveG& S, int K, int row0, const double* vt, const int4* cls, double* red,
-                        double2* Wl) {
+__device__ __forceinline__ void hh_unit(const SolveG& S, int K, int row0, int part, const double* vt, const int4* cls,
+                                        double* red, double2* Wl) {
   typedef MfmaTraits<double>::acc_t acc64_t;
   const int poff = S.kb_poff[K], noo = S.kb_poff[K + 1] - poff;
   const int nvv = S.kb_voff[K + 1] - S.kb_voff[K];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  // the unit's W column tiles [tlo, thi) and pair columns [qlo, qhi)
+  const int ntq = (noo + 15) >> 4, hsplit = (ntq + 1) >> 1;
+  const int tlo = part == 2 ? hsplit : 0, thi = part == 0 ? ntq : (part == 1 ? hsplit : ntq);
+  const int qlo = 16 * tlo, qhi = min(noo, 16 * thi);
   // (the W pass is the unit's longest chain of loads: kWGroup steps per round)
-  if (noo <= 16)
-    w_rows<Real, kWGroup, 1>(S, K, row0, 0, vt, cls, red, Wl);
+  if (thi - tlo == 1)
+    w_rows<Real, kWGroup, 1>(S, K, row0, qlo, qhi, vt, cls, red, Wl);
   else
-    for (int cb = 0; cb < noo; cb += 32) w_rows<Real, kWGroup, 2>(S, K, row0, cb, vt, cls, red, Wl);
+    for (int cb = qlo; cb < qhi; cb += 32) w_rows<Real, kWGroup, 2>(S, K, row0, cb, qhi, vt, cls, red, Wl);
   const int rowA = row0 + r16;
   const bool rv = rowA < noo;
   const double2* Wrow = Wl + r16 * noo;
   const auto TKb = S.tk + S.kb_toff[K];
-  const auto HH = S.hh + S.kb_toff[K];
+  const auto HH = (part == 2 ? S.hh2 : S.hh) + S.kb_toff[K];
   for (int c0 = 16 * wave; c0 < nvv; c0 += 64) {
     const int colB = c0 + r16;
     const bool cv = colB < nvv;
     const auto TK = TKb + (size_t)(cv ? colB : 0) * noo;
     acc64_t accs = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-    for (int g0 = 0; g0 < noo; g0 += 4 * HS) {
+    for (int g0 = qlo; g0 < qhi; g0 += 4 * HS) {
       double ws[HS], wp[HS];
 #pragma unroll
       for (int st = 0; st < HS; ++st) {
         const int q = g0 + 4 * st + kq;
-        const double2 w = Wrow[min(q, noo - 1)];
-        const bool ok = q < noo && rv;
+        const double2 w = Wrow[min(q, qhi - 1)];
+        const bool ok = q < qhi && rv;
         ws[st] = ok ? w.x : 0.0;
         wp[st] = ok ? w.y : 0.0;
       }
@@ -1168,8 +1174,8 @@ __device__ __forceinline__ void hh_unit(const SolveG& S, int K, int row0, const 
 #pragma unroll
       for (int st = 0; st < HS; ++st) {
         const int q = g0 + 4 * st + kq;
-        const double2 t = ld(TK + min(q, noo - 1));
-        const bool ok = q < noo && cv;
+        const double2 t = ld(TK + min(q, qhi - 1));
+        const bool ok = q < qhi && cv;
         bs[st] = ok ? t.x + t.y : 0.0;
         bp[st] = ok ? t.x - t.y : 0.0;
       }
@@ -1241,16 +1247,16 @@ __device__ __forceinline__ void ladder_block(const SolveDev* __restrict__ solves
 
 template <int G, int HS>
 __device__ __forceinline__ void hh_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                         int n, int blk, const LadderLds& l, double* red) {
+                         int n, int blk, int split, const LadderLds& l, double* red) {
   const int s = find_solve(prefix, n, blk);
   const SolveG& S = as_global(solves)[s];
-  const int4 it = ld(S.hh_items + (blk - prefix[s]));
+  const int4 it = ld((split ? S.hh_items2 : S.hh_items) + (blk - prefix[s]));
   const int2 st = stage_ladder(S, it.y, l);
   if (pass_idle(S, st.x)) return;
   if (st.y)
-    hh_unit<double, G, HS>(S, it.y, it.z, l.vt, l.cls, red, l.wl);
+    hh_unit<double, G, HS>(S, it.y, it.z, it.w, l.vt, l.cls, red, l.wl);
   else
-    hh_unit<float, G, HS>(S, it.y, it.z, l.vt, l.cls, red, l.wl);
+    hh_unit<float, G, HS>(S, it.y, it.z, it.w, l.vt, l.cls, red, l.wl);
 }
 
 template <int BT, bool SPLIT>
@@ -1413,12 +1419,12 @@ __global__ __launch_bounds__(256, 4) void k_contract(const SolveDev* __restrict_
                                                   const int* __restrict__ hh_prefix, int nhh, int nhh_pad,
                                                   const int* __restrict__ lad_prefix, int nlad, int nlad_pad,
                                                   const int* __restrict__ con_prefix, int ncon, int n,
-                                                  int tab_doubles, int cls_max) {
+                                                  int tab_doubles, int cls_max, int hh_split) {
   extern __shared__ double dyn[];  // ladder tables, a virtual list, W rows
   __shared__ double red[3 * 64 * 8];
   const int bid = blockIdx.x;
   if (bid < nhh_pad) {
-    if (bid < nhh) hh_block<G, HS>(solves, hh_prefix, n, bid, ladder_lds(dyn, tab_doubles, cls_max), red);
+    if (bid < nhh) hh_block<G, HS>(solves, hh_prefix, n, bid, hh_split, ladder_lds(dyn, tab_doubles, cls_max), red);
     return;
   }
   int rb = bid - nhh_pad;
@@ -1554,7 +1560,8 @@ __device__ __forceinline__ void main_block(const SolveDev* __restrict__ solves, 
     // ladders: particle-particle part with the hole-hole sum (k_contract)
     if (S.do_ladder) {
       const double2 lp = S.lad[x], lh = S.hh[pos];
-      const double2 l2 = make_double2(lp.x + lh.x, lp.y + lh.y);
+      const double2 lh2 = (flags & MF_HH2) ? S.hh2[pos] : make_double2(0.0, 0.0);
+      const double2 l2 = make_double2(lp.x + lh.x + lh2.x, lp.y + lh.y + lh2.y);
       gab += 0.5 * (l2.x + l2.y);
       gba += 0.5 * (l2.x - l2.y);
     }
@@ -2311,7 +2318,7 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
     for (int it = blockIdx.x; it < nhh + nlad + ncb; it += gridDim.x) {
       __syncthreads();
       if (it < nhh) {
-        hh_block<2, 8>(solves, hh_prefix, 1, it, ll, P.u.la.red);
+        hh_block<2, 8>(solves, hh_prefix, 1, it, 1, ll, P.u.la.red);
       } else if (it < nhh + nlad) {
         ladder_block<2, 1>(solves, lad_prefix, 1, it - nhh, ll, P.u.la.red);
       } else {
@@ -2322,7 +2329,7 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
     grid.sync();
     for (int it = blockIdx.x; it < nelem; it += gridDim.x) {
       __syncthreads();
-      main_block<0>(solves, elem_prefix, 1, it, MF_ITERATE, P.u.mn.red, P.u.mn.joo);
+      main_block<0>(solves, elem_prefix, 1, it, MF_ITERATE | MF_HH2, P.u.mn.red, P.u.mn.joo);
     }
     grid.sync();
     if (blockIdx.x == 0) finalize_block(solves, 0, FIN_ITER, 1, P.u.fi.vals, P.u.fi.red, P.u.fi.sst);
@@ -2473,11 +2480,11 @@ static void launch_contract_t(const BatchDev& b, hipStream_t s, hipEvent_t stop)
   if (nc == 2)
     launch_k(k_contract<G, QS, HS, 2>, stop, dim3(grid), dim3(256), (uint32_t)b.con_lds, s,
              (const SolveDev*)b.solves, (const int*)b.hh_prefix, b.nhh, b.nhh_pad, lad_prefix,
-             nlad, nlad_pad, (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max);
+             nlad, nlad_pad, (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max, b.hh_split);
   else
     launch_k(k_contract<G, QS, HS, 1>, stop, dim3(grid), dim3(256), (uint32_t)b.con_lds, s,
              (const SolveDev*)b.solves, (const int*)b.hh_prefix, b.nhh, b.nhh_pad, lad_prefix,
-             nlad, nlad_pad, (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max);
+             nlad, nlad_pad, (const int*)b.con_prefix, b.ncon, b.n, b.tab_doubles, b.cls_max, b.hh_split);
 }
 
 template <int G>
@@ -2753,10 +2760,10 @@ void launch_one(const BatchDev& b, int which, hipStream_t s) {
     case 3:
       if (b.main_nq == 16)
         hipLaunchKernelGGL(k_main<16>, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
-                           b.elem_prefix, b.n, (int)MF_ITERATE);
+                           b.elem_prefix, b.n, (int)MF_ITERATE | (b.hh_split ? (int)MF_HH2 : 0));
       else
         hipLaunchKernelGGL(k_main<32>, dim3(b.elem_blocks), dim3(kElemThreads), 0, s, b.solves,
-                           b.elem_prefix, b.n, (int)MF_ITERATE);
+                           b.elem_prefix, b.n, (int)MF_ITERATE | (b.hh_split ? (int)MF_HH2 : 0));
       break;
     default:
       break;
@@ -2781,10 +2788,12 @@ void launch_iteration(const BatchDev& b, int main_flags, hipStream_t s, const Ke
       case 3:
         if (b.main_nq == 16)
           launch_k(k_main<16>, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
-                   (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
+                   (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n,
+                   main_flags | (b.hh_split ? (int)MF_HH2 : 0));
         else
           launch_k(k_main<32>, stop[i], dim3(b.elem_blocks), dim3(kElemThreads), 0, s,
-                   (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n, main_flags);
+                   (const SolveDev*)b.solves, (const int*)b.elem_prefix, b.n,
+                   main_flags | (b.hh_split ? (int)MF_HH2 : 0));
         break;
     }
   }

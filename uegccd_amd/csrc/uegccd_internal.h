@@ -79,7 +79,7 @@ constexpr int kMaxTabSlots = 4;  // flags 0..3
 
 enum Phase : int { PH_ITER = 0, PH_FINAL = 1, PH_DONE = 2 };
 enum FinKind : int { FIN_ITER = 0, FIN_INIT = 1, FIN_SUM = 2 };
-enum MainFlags : int { MF_ITERATE = 1, MF_WRITE_GR = 2 };
+enum MainFlags : int { MF_ITERATE = 1, MF_WRITE_GR = 2, MF_HH2 = 4 };  // MF_HH2: add the hh2 half
 // Where the amplitudes of the next residual pass come from (read by k_slice,
 // which builds each slab of them and writes it back to tcur and tk).
 enum AmpSource : int {
@@ -163,6 +163,8 @@ struct SolveDevT {
   // ring / crossed-ring / Joo items
   Ptr<const int4> hh_items;
   int nhh;
+  Ptr<const int4> hh_items2;  // the units split over the W column halves (a solve alone)
+  int nhh2;
   Ptr<const int4> lad_items;   // one 16-column tile each
   int nlad;
   Ptr<const int4> lad_items2;  // two 16-column tiles each (the batches' blocks)
@@ -180,6 +182,8 @@ struct SolveDevT {
   Ptr<double> rout;         // [2*n3] R of the last pass (when MF_WRITE_GR)
   Ptr<double2> lad;         // [n3] particle-particle ladder, (S, P) parts
   Ptr<double2> hh;          // [nvalid] hole-hole ladder, (S, P) parts, laid out as tk
+  Ptr<double2> hh2;         // the same over the second half of the W columns of blocks of
+                            // more than 16 pairs (zeros at the other blocks' entries)
   Ptr<double2> jring;       // [o*v rows][o] quadratic ring intermediates (J2, J3), rows
                             // in the grouped P(q) order (qrow)
   Ptr<double2> jxring;      // [o*v rows][o] quadratic crossed-ring intermediates, same rows
@@ -240,6 +244,7 @@ struct BatchDev {
   // ranges of the launch are padded to multiples of 8 blocks
   int* hh_prefix;       // [n+1]
   int nhh, nhh_pad;
+  int hh_split;         // the units are each solve's hh_items2 (a batch of one), else hh_items
   int* lad_prefix;      // [n+1]
   int nlad, nlad_pad;
   int* lad_prefix2;     // [n+1] the same over the two-tile items
