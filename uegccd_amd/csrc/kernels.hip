@@ -2472,9 +2472,11 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
 
 template <int G, int QS, int HS>
 static void launch_contract_t(const BatchDev& b, hipStream_t s, hipEvent_t stop) {
-  // (a batch of several solves runs two column tiles per ladder block, a
-  // solve alone one: fewer amplitude loads against more parallelism)
-  const int nc = b.n > 1 ? 2 : 1, nlad = nc == 2 ? b.nlad2 : b.nlad, nlad_pad = (nlad + 7) & ~7;
+  // (two column tiles per ladder block, sharing each k-step's amplitude
+  // loads: with the hole-hole units split, a solve alone gains from the
+  // shorter launch too, C2's contraction 18.8 -> 17.9 us; the one-tile items
+  // serve the persistent kernel)
+  const int nc = 2, nlad = nc == 2 ? b.nlad2 : b.nlad, nlad_pad = (nlad + 7) & ~7;
   const int* lad_prefix = nc == 2 ? b.lad_prefix2 : b.lad_prefix;
   const int grid = b.nhh_pad + nlad_pad + (b.ncon + kItemsPerBlock - 1) / kItemsPerBlock;
   if (nc == 2)
