@@ -130,7 +130,12 @@ void uegccd_default_options(uegccd_options* opt);
 /* Builds the per-basis index maps (momentum partners, pair-momentum blocks,
  * Coulomb table) and device buffers.  Replaces SetUpCCScr/SetUpDIISCC
  * (source/CCScr.f90:9-30, source/DIISCC.f90:10-45) and the FindIndex/ERI
- * machinery (source/HEG.f90:357-404, 689-978). */
+ * machinery (source/HEG.f90:357-404, 689-978).  Any basis with NOcc <= 64
+ * whose element map fits 64 bits (about 10^6 orbitals at N=14) is taken;
+ * the CCD passes (uegccd_solve*, uegccd_residual, uegccd_kernel_time,
+ * served requests) return UEGCCD_ERR_UNSUPPORTED for a basis with a
+ * virtual-occupied |n_a - n_k|^2 above 1022, while uegccd_drpa, uegccd_mp2,
+ * uegccd_energy and the structure factors run on it. */
 int uegccd_ctx_create(const uegccd_system* sys, const uegccd_options* opt, uegccd_ctx** out);
 void uegccd_ctx_destroy(uegccd_ctx* ctx);
 
