@@ -88,6 +88,15 @@ __device__ __forceinline__ int d2(int4 p, int4 q) {
   return a * a + b * b + c * c;
 }
 
+// Lane `lane`'s x in every lane (a scalar read, no trip through the LDS
+// crossbar as a shuffle takes; lane must be uniform).
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)v, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Block-wide reductions (wave64 shuffles, then LDS), fixed order.
 __device__ __forceinline__ double wave_sum(double x) {
   for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
@@ -443,12 +452,14 @@ __device__ bool diis_gauss_jordan8(const double* bmat, int nh, int slot_cur, int
   double b = 1.0;
 #pragma unroll
   for (int k = 0; k < M; ++k) {
-    const double p = __shfl(a, k * M + k, 64);
+    // (the pivot and its right-hand side are read as scalars, so that the
+    // division runs while the row and column shuffles are on their way)
+    const double p = readlane_f64(a, k * M + k);
     if (!(p > 1e-14)) return false;
     const double inv_p = 1.0 / p;
     const double rk = __shfl(a, k * M + c, 64);  // pivot row, this lane's column
     const double ck = __shfl(a, r * M + k, 64);  // this lane's row, pivot column
-    const double bk = __shfl(b, k * M, 64);
+    const double bk = readlane_f64(b, k * M);
     if (r == k) {
       a *= inv_p;
       b *= inv_p;
@@ -462,7 +473,7 @@ __device__ bool diis_gauss_jordan8(const double* bmat, int nh, int slot_cur, int
   double sum = 0.0;
 #pragma unroll
   for (int q = 0; q < M; ++q) {
-    const double yq = __shfl(b, q * M, 64);
+    const double yq = readlane_f64(b, q * M);
     if (q < nh) sum += yq;
   }
   if (!(fabs(sum) > 0.0) || !isfinite(sum)) return false;
