@@ -189,3 +189,21 @@ def test_full_twist_stream_poisson():
         assert s.niter == mx.niter and s.iters_fp32 == mx.iters_fp32, (q, s.niter, mx.niter)
         assert abs(s.ecorr - mx.ecorr) < 1e-12, (q, s.ecorr, mx.ecorr)
         assert abs(s.ecorr - e64.ecorr) < 1e-6, (q, s.ecorr, e64.ecorr)
+
+
+def test_request_past_the_slice_table_is_refused():
+    """A request whose basis the CCD passes do not take (N=2, |n|^2 <= 1050:
+    a virtual-occupied |dn|^2 past the slice blocks' Coulomb table) finishes
+    with UEGCCD_ERR_UNSUPPORTED from planning; a request beside it is served."""
+    from uegccd_amd.driver import Server, options
+    from uegccd_amd.inputfile import parse_input, standard_input
+    print("[server] refused large basis", flush=True)
+    opts = options(mode="diis", ndiis=8)
+    srv = Server(max_active=4, iters_per_step=4)
+    big = srv.submit(parse_input(standard_input(2, 1050, 1.0, fcut=10000.0)), opts)
+    ok = srv.submit(_info("n14_cut5", {}), opts)
+    srv.drain()
+    rb, ro = srv.poll(big), srv.poll(ok)
+    srv.close()
+    assert rb.state == L.REQ_DONE and rb.status == L.ERR_UNSUPPORTED
+    assert ro.state == L.REQ_DONE and ro.status == 0
