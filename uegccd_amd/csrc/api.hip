@@ -923,6 +923,16 @@ size_t const_section_bound(const uegccd_ctx* c) {
 // Only the descriptor depends on `base`: with fill false, `image` already
 // holds the section from an earlier layout (at any base) and only the
 // descriptor and the prefixes are written again.
+// The kernel arguments of a folded update: solve S's partials and state.
+FoldArgs fold_args(const SolveDev& S) {
+  FoldArgs f;
+  f.partials = S.partials;
+  f.nblk = S.nblk_elem;
+  f.nd = S.mode == UEGCCD_MODE_DIIS ? S.ndiis : 0;
+  f.st = S.st;
+  return f;
+}
+
 void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t bytes,
                 std::vector<char>& image, size_t* work_off, bool fill) {
   const uegccd_options& opt = c->opt;
@@ -1098,6 +1108,7 @@ void layout_ctx(uegccd_ctx* c, const uegccd_system* sys, char* base, size_t byte
   S.st = c->d_state;
   S.st2 = c->arena.take<State>(1);
   S.trace = c->arena.take<double>(2 * (size_t)c->maxiter);
+  b.fold = fold_args(S);
   im.write(c->d_sd, &S, 1);
   c->pa.sd = c->d_sd;
   c->pa.w = c->box_w;
@@ -1528,6 +1539,7 @@ int make_batch(uegccd_ctx** cs, int n, BatchScratch* out, hipStream_t stream = n
   b.nhh = hp[n];
   b.nhh_pad = pad8(hp[n]);
   b.hh_split = 0;
+  b.fold = n == 1 ? fold_args(cs[0]->sd) : FoldArgs{};
   b.nlad_pad = pad8(lp[n]);
   b.tab_doubles = tab_doubles;
   b.cls_max = cls_max;

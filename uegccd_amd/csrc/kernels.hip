@@ -576,7 +576,7 @@ __device__ __forceinline__ int finalize_apply(const SolveG& S, int kind, int pre
 
 template <int BT, bool SPLIT>
 __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves, const int* __restrict__ prefix,
-                            int n, int bid, Lds& L, SliceDiis& D, double* dyn, double* vt2, int fin_prev,
+                            int n, int bid, Lds& L, SliceDiis& D, double* dyn, double* vt2, int fin_prev, FoldArgs fa,
                             double* fvals, double* fred, State* fsst) {
   const int s = find_solve(prefix, n, bid);
   const SolveG& S = as_global(solves)[s];
@@ -617,8 +617,21 @@ __device__ __forceinline__ void slice_block(const SolveDev* __restrict__ solves,
   StateLoads fsl{0.0, 0.0};
   PartialLoads<16> fpl;
   if (fin_prev) {
-    state_load(S, fsl);
-    partials_load<16>(S, finalize_dots(S, FIN_ITER), t0, fpl);
+    // (state_load and partials_load<16> from the kernel arguments, so that
+    // the loads wait on no descriptor read)
+    const auto stw = (GlobalPtr<const double>)(uintptr_t)fa.st;
+    const int w1 = t0 + kFinThreads;
+    fsl.sw0 = t0 < kStateWords ? stw[t0] : 0.0;
+    fsl.sw1 = t0 < kFinThreads && w1 < kStateWords ? stw[w1] : 0.0;
+    const auto pp = (GlobalPtr<const double>)(uintptr_t)fa.partials;
+    const int k2 = t0 + kFinThreads;
+    const bool one = t0 < fa.nblk && t0 < kFinThreads, two = one && k2 < fa.nblk;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int src = q == 0 ? 1 : q == 1 ? 2 : q == 2 ? 0 : q;
+      fpl.p[q] = src < 4 + fa.nd && one ? pp[(size_t)src * fa.nblk + t0] : 0.0;
+      fpl.p2[q] = src < 4 + fa.nd && two ? pp[(size_t)src * fa.nblk + k2] : 0.0;
+    }
   }
   const auto HC = S.histT + c0 + t0;
   const uint2 cm = t0 < nvs ? ld(S.cmap + c0 + t0) : make_uint2(0u, 0u);
@@ -1272,7 +1285,7 @@ __device__ __forceinline__ void hh_block(const SolveDev* __restrict__ solves, co
 
 template <int BT, bool SPLIT>
 __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solves,
-                                              const int* __restrict__ prefix, int n, int fin_prev) {
+                                              const int* __restrict__ prefix, int n, int fin_prev, FoldArgs fa) {
   __shared__ Lds L;
   __shared__ SliceDiis D;
   __shared__ double vt2[SPLIT ? kMaxN2 : 1];
@@ -1280,7 +1293,7 @@ __global__ __launch_bounds__(BT) void k_slice(const SolveDev* __restrict__ solve
   __shared__ double fred[(BT / 64) * 32];
   __shared__ State fsst;
   extern __shared__ double dyn[];
-  slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2, fin_prev, fvals, fred, &fsst);
+  slice_block<BT, SPLIT>(solves, prefix, n, blockIdx.x, L, D, dyn, vt2, fin_prev, fa, fvals, fred, &fsst);
 }
 
 // --------------------------------------------------------------------------
@@ -2322,7 +2335,7 @@ __global__ __launch_bounds__(256) void k_persist(const SolveDev* __restrict__ so
   for (int pass = 0; pass < passes; ++pass) {
     for (int it = blockIdx.x; it < nslice; it += gridDim.x) {
       __syncthreads();
-      slice_block<256, false>(solves, slice_prefix, 1, it, P.u.sl.L, P.u.sl.D, pdyn, nullptr, 0, nullptr,
+      slice_block<256, false>(solves, slice_prefix, 1, it, P.u.sl.L, P.u.sl.D, pdyn, nullptr, 0, FoldArgs{}, nullptr,
                               nullptr, nullptr);
     }
     grid.sync();
@@ -2470,13 +2483,13 @@ static void launch_slice(const BatchDev& b, hipStream_t s, hipEvent_t stop = nul
     const auto* pf = (const int*)b.slice_prefix[q];
     const hipEvent_t ev = q == last ? stop : nullptr;
     if (b.slice_split) {
-      if (q == 0) launch_k(k_slice<256, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
-      else if (q == 1) launch_k(k_slice<512, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
-      else launch_k(k_slice<768, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
+      if (q == 0) launch_k(k_slice<256, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev, b.fold);
+      else if (q == 1) launch_k(k_slice<512, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev, b.fold);
+      else launch_k(k_slice<768, true>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev, b.fold);
     } else {
-      if (q == 0) launch_k(k_slice<256, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
-      else if (q == 1) launch_k(k_slice<512, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
-      else launch_k(k_slice<768, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev);
+      if (q == 0) launch_k(k_slice<256, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev, b.fold);
+      else if (q == 1) launch_k(k_slice<512, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev, b.fold);
+      else launch_k(k_slice<768, false>, ev, g, t, b.slice_lds[q], s, sd, pf, b.n, fin_prev, b.fold);
     }
   }
 }

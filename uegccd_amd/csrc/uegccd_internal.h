@@ -231,6 +231,16 @@ struct Tiles {
 // Launch geometry of a batch of solves iterated together.
 struct PlanArgs;
 
+// A batch of one's pass-state arrays, handed to its slice launches as kernel
+// arguments: the folded state update requests them without first reading
+// the solve's descriptor.
+struct FoldArgs {
+  const double* partials;  // the element blocks' partials (slot-major)
+  int nblk;                // element blocks
+  int nd;                  // DIIS dots among the partials of an iterating pass
+  const void* st;          // the state between passes (State)
+};
+
 struct BatchDev {
   int n;
   SolveDev* solves;     // device array [n]
@@ -245,6 +255,7 @@ struct BatchDev {
   int* hh_prefix;       // [n+1]
   int nhh, nhh_pad;
   int hh_split;         // the units are each solve's hh_items2 (a batch of one), else hh_items
+  FoldArgs fold;        // a batch of one's state-update arrays (others: unused)
   int* lad_prefix;      // [n+1]
   int nlad, nlad_pad;
   int* lad_prefix2;     // [n+1] the same over the two-tile items
